@@ -1,0 +1,162 @@
+"""Benchmark: solver time steps/s of the DRD-Micro implicit-solve path on one MI355X.
+
+One "step" is one full time step of the reference's driver sequence
+(TP/mcl6.f90:395-502): Cahn-Hilliard phi/mu solve, BDF2 momentum solve,
+pressure projection, every Krylov solve run to the configured relative
+residual.  The timed region starts with every field resident in HBM.
+
+Modes
+  dropin : the plain FP64 path with the reference's own preconditioners and
+           stopping rule (what a Fortran caller of the drop-in gets).
+
+Prints one JSON line (see the `result` dict below).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_kernel(g, which: int, reps: int):
+    from drd_application_amd import solver
+
+    L = solver.lib()
+    L.drd_bench_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    ms, by = C.c_double(0), C.c_double(0)
+    solver._check(L.drd_bench_kernel(g._h, which, reps, C.byref(ms), C.byref(by)))
+    return ms.value, by.value
+
+
+def cpu_baseline(cfg, velo_iters: float, phmu_iters: float, budget_s: float = 20.0):
+    """Times the CPU oracle (serial FP64 restatement of the reference) on a
+    bounded sample: k BiCGSTAB iterations of the momentum and of the CH solve
+    on the same fields, scaled by the GPU run's iteration counts per step."""
+    import numpy as np
+
+    from drd_application_amd import abi, geometry
+    from oracle.oracle import Oracle
+
+    o = Oracle(cfg.params)
+    geometry.load(o, cfg)
+    p = cfg.params
+    k = 4
+    nv = abi.n_uv(p)
+    bv = np.random.default_rng(1).standard_normal(nv)
+    t0 = time.perf_counter()
+    _, itv, _, _ = o.bicgstab(abi.OP_VELO2, abi.PS_IDENTITY, bv, np.zeros(nv), k, 1e-300)
+    tv = (time.perf_counter() - t0) / max(itv, 1)
+    nc = 2 * p.nx * p.ny
+    bc = o.ph_mu_rhs()
+    t0 = time.perf_counter()
+    _, itc, _, _ = o.bicgstab(abi.OP_PHMU, abi.PS_PHMU_JACOBI, bc, np.zeros(nc), k, 1e-300)
+    tc = (time.perf_counter() - t0) / max(itc, 1)
+    o.close()
+    step_s = velo_iters * tv + phmu_iters * tc
+    return {
+        "value": 1.0 / step_s if step_s > 0 else None,
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle (serial C, FP64) on the {cfg.name} fields: {itv} momentum and {itc} CH "
+                   f"BiCGSTAB iterations timed ({tv*1e3:.1f} ms and {tc*1e3:.1f} ms each), scaled by "
+                   f"the GPU run's {velo_iters:.1f} + {phmu_iters:.1f} iterations per step; RHS "
+                   f"assembly and the pressure solve are not included"),
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--mode", default="dropin", choices=["dropin"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path "
+              f"is single-GPU, see DESIGN.md for how it would shard", file=sys.stderr)
+        return 2
+    from drd_application_amd import abi, geometry
+    from drd_application_amd.solver import GpuSolver
+
+    cfg = geometry.config(args.config)
+    p = cfg.params
+    log(f"config {cfg.name}: {p.nx}x{p.ny}, tol {p.tol:g}, dt {p.dt:g}")
+    g = GpuSolver(p)
+    geometry.load(g, cfg)
+    g.synchronize()
+    t = 0.0
+    warm = max(1, args.warmup)
+    for s in range(warm):
+        info = g.step(1 if s == 0 else 2, t)
+        t += p.dt
+        log(f"warmup step {s}: phmu {info['phmu']['iter']} ({info['phmu']['info']}), "
+            f"velo {info['velo']['iter']} ({info['velo']['info']})")
+    g.synchronize()
+    iters_v, iters_c, infos = [], [], []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        info = g.step(2, t)
+        t += p.dt
+        iters_v.append(info["velo"]["iter"])
+        iters_c.append(info["phmu"]["iter"])
+        infos.append((info["phmu"]["info"], info["velo"]["info"]))
+    g.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; velo iters {iters_v}; phmu iters {iters_c}; "
+        f"info {infos}")
+    ms_k, bytes_k = bench_kernel(g, 0, 50)
+    achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    log(f"velocity matvec: {ms_k*1e3:.1f} us, {bytes_k/1e6:.1f} MB -> {achieved:.0f} GB/s")
+    mean_v = sum(iters_v) / len(iters_v)
+    mean_c = sum(iters_c) / len(iters_c)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(cfg, mean_v, mean_c)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+    value = args.steps / elapsed
+    result = {
+        "metric": "solver steps/s at 1e-10 rel-res, learned+FP32 vs plain FP64 precond; HBM GB/s",
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": warm,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name} {p.nx}x{p.ny} two-phase CH+NS, tol {p.tol:g}",
+                   "mode": args.mode, "velo_iters_per_step": mean_v, "phmu_iters_per_step": mean_c},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
+                     "us_per_launch": ms_k * 1e3},
+        "cpu_baseline": cpu,
+    }
+    g.close()
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,833 @@
+// drd_api.hip -- host runtime and C-ABI (include/drd_gpu.h) of the gfx950
+// implicit-solve path.
+//
+// One drd_ctx per grid instance owns every device buffer: the module-`global`
+// fields, the Krylov workspace, DCT plans and eigenvalue tables.  Solves run
+// stream-ordered on the context's stream; BiCGSTAB iterations are queued in
+// chunks without host round trips, the device-side state machine
+// (kfinalize) deciding when the kernels turn into no-ops.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+using namespace drd;
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return set_err(DRD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+#define RCHK(expr)            \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != 0) return r_;   \
+  } while (0)
+
+struct drd_ctx {
+  drd_params p;
+  Grid g;
+  Scal s;
+  hipStream_t stream = nullptr;
+  double* f[DRD_F_COUNT] = {};
+  // per-step scratch fields
+  double *u_after = nullptr, *v_after = nullptr, *ph_after = nullptr, *rmu_after = nullptr;
+  double* zero_c = nullptr;               // zero cell field (particle forces)
+  double* arg[8] = {};                    // upload slots for host-array arguments
+  // Krylov workspace
+  int64_t nmax = 0;
+  double* kw[7] = {};                     // R RT P V T PH SH  (LIB/BiCGSTABREVCOM.f:159-166)
+  double *rhs = nullptr, *xv = nullptr, *diag = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+  double* part = nullptr;
+  KState* ks = nullptr;
+  KState* ks_host = nullptr;              // pinned
+  // tables
+  double *pre_lcc = nullptr, *ph_lcc = nullptr, *cfB = nullptr, *cfT = nullptr;
+  double* uL = nullptr;
+  double* uL_host = nullptr;              // pinned
+  double* pn = nullptr;
+  DctPlan* dct = nullptr;
+  double* dct_work = nullptr;
+  double cosa = 0, sqrt2 = 0;
+};
+
+// ---------------------------------------------------------------------------
+extern "C" void drd_params_default(drd_params* p, int32_t nx, int32_t ny) {
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx;
+  p->ny = ny;
+  p->variant = DRD_VARIANT_TP;
+  p->phi_mode = 2;
+  p->lr_bd_type = 'n';
+  p->velo_prec = DRD_PREC_REFERENCE;
+  p->maxit_velo1 = 800;
+  p->maxit_velo2 = 2000;
+  p->maxit_phmu = 5000;
+  p->maxit_ph = 500;
+  p->dx = 1.0 / nx;
+  p->dz = 1.0 / ny;
+  p->dt = 1.0e-4 / 2.0;
+  p->re = 3;
+  p->ca = 12;
+  p->xld = 5.0e-2;
+  p->s_implicit = 1.5;
+  p->I_thickness = 0.01;
+  p->tol = 1.0e-8;
+  p->penalty_k = 1.0e8;
+  p->angle = 60;
+  p->lambda_rho = 1;
+  p->bdr_ph_used_adjust = 1.0e-6;
+  p->twall = 0.1;
+  p->bdr_upp = 1;
+}
+
+extern "C" const char* drd_last_error(void) { return g_err.c_str(); }
+extern "C" int drd_abi_version(void) { return DRD_ABI_VERSION; }
+
+extern "C" int64_t drd_field_size(const drd_params* p, int32_t f) {
+  if (!p) return 0;
+  const int64_t nc = (int64_t)(p->nx + 2) * (p->ny + 2);
+  switch (f) {
+    case DRD_F_U:
+    case DRD_F_U_PRE:
+      return (int64_t)(p->nx + 3) * (p->ny + 2);
+    case DRD_F_V:
+    case DRD_F_V_PRE:
+      return (int64_t)(p->nx + 2) * (p->ny + 1);
+    case DRD_F_ANGLEB:
+    case DRD_F_ANGLET:
+      return p->nx + 1;
+    default:
+      return (f >= 0 && f < DRD_F_COUNT) ? nc : 0;
+  }
+}
+
+static Scal make_scal(const drd_params& p) {
+  Scal s{};
+  s.dt = p.dt;
+  s.re = p.re;
+  s.ca = p.ca;
+  s.xld = p.xld;
+  s.s = p.s_implicit;
+  s.I = p.I_thickness;
+  s.dr = p.dx;
+  s.dz = p.dz;
+  s.dr2 = p.dx * p.dx;
+  s.dz2 = p.dz * p.dz;
+  s.penalty_k = p.penalty_k;
+  s.ggx = p.ggx;
+  s.ggz = p.ggz;
+  s.uwT = p.uwT;
+  s.uwB = p.uwB;
+  s.adj = p.bdr_ph_used_adjust;
+  // coef_bot = (I/dz)/(I/dz)  TP/ph_mu_solve.f90:122-123
+  const double tmp = p.I_thickness / p.dz;
+  s.coef_bot = (p.I_thickness / p.dz) / tmp;
+  s.coef_top = (p.I_thickness / p.dz) / tmp;
+  s.lr_d = (p.lr_bd_type == 'd');
+  s.phi_mode = p.phi_mode;
+  return s;
+}
+
+static const double kPiG = 3.14159265358979323846264338327950288419716939937510;  // TP/global.f90:39
+
+template <class T>
+static int dalloc(T** ptr, int64_t n) {
+  HIPCHK(hipMalloc((void**)ptr, (size_t)std::max<int64_t>(n, 1) * sizeof(T)));
+  HIPCHK(hipMemset(*ptr, 0, (size_t)std::max<int64_t>(n, 1) * sizeof(T)));
+  return 0;
+}
+
+static int build_tables(drd_ctx* c) {
+  const drd_params& p = c->p;
+  const int nx = p.nx, ny = p.ny;
+  const double dr2 = p.dx * p.dx, dz2 = p.dz * p.dz;
+  // Pre_Lcc and Ph_Lcc  TP/setup.f90:108-155
+  std::vector<double> fmx(nx + 1), fmy(ny + 1), pre((size_t)(nx + 1) * (ny + 1)), ph((size_t)nx * ny);
+  for (int i = 0; i <= nx; ++i) {
+    const double sn = std::sin(kPiG * i / 2.0 / nx);
+    fmx[i] = 4.0 * (sn * sn) / dr2;
+  }
+  for (int j = 0; j <= ny; ++j) {
+    const double sn = std::sin(kPiG * j / 2.0 / ny);
+    fmy[j] = 4.0 * (sn * sn) / dz2;
+  }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) pre[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
+  pre[0] = -1.0;
+  for (int i = 0; i < nx; ++i) {
+    const double sn = std::sin(kPiG * i / 2.0 / (nx - 1));
+    fmx[i] = 4.0 * (sn * sn) / dr2;
+  }
+  for (int j = 0; j < ny; ++j) {
+    const double sn = std::sin(kPiG * j / 2.0 / (ny - 1));
+    fmy[j] = 4.0 * (sn * sn) / dz2;
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double temp = fmx[i] + fmy[j];
+      ph[i + (size_t)nx * j] =
+          1. + p.xld * p.dt * p.s_implicit * temp / p.I_thickness + p.xld * p.I_thickness * p.dt * (temp * temp);
+    }
+  RCHK(dalloc(&c->pre_lcc, (int64_t)pre.size()));
+  RCHK(dalloc(&c->ph_lcc, (int64_t)ph.size()));
+  HIPCHK(hipMemcpy(c->pre_lcc, pre.data(), pre.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph_lcc, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->cosa = std::cos(p.angle * kPiG / 180);
+  c->sqrt2 = std::sqrt(2.0);
+  return 0;
+}
+
+// cf = cos(angle*pi/180)/sqrt(2) per column  TP/ch_bdr_condition.f90:61
+static int upload_angles(drd_ctx* c) {
+  const int nx = c->p.nx;
+  std::vector<double> aB(nx + 1), aT(nx + 1), cB(nx + 1), cT(nx + 1);
+  HIPCHK(hipMemcpy(aB.data(), c->f[DRD_F_ANGLEB], (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(aT.data(), c->f[DRD_F_ANGLET], (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i <= nx; ++i) {
+    cB[i] = std::cos(aB[i] * kPiG / 180.0) / std::sqrt(2.0);
+    cT[i] = std::cos(aT[i] * kPiG / 180.0) / std::sqrt(2.0);
+  }
+  HIPCHK(hipMemcpy(c->cfB, cB.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->cfT, cT.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int ctx_alloc(drd_ctx* c) {
+  const drd_params& p = c->p;
+  const Grid& g = c->g;
+  for (int f = 0; f < DRD_F_COUNT; ++f) RCHK(dalloc(&c->f[f], drd_field_size(&p, f)));
+  RCHK(dalloc(&c->u_after, g.nuf()));
+  RCHK(dalloc(&c->v_after, g.nvf()));
+  RCHK(dalloc(&c->ph_after, g.nc()));
+  RCHK(dalloc(&c->rmu_after, g.nc()));
+  RCHK(dalloc(&c->zero_c, g.nc()));
+  const int64_t big = std::max<int64_t>(g.nuf(), g.nc());
+  for (auto& a : c->arg) RCHK(dalloc(&a, big));
+  c->nmax = std::max<int64_t>(g.nuv(), 2 * g.ncell());
+  for (auto& w : c->kw) RCHK(dalloc(&w, c->nmax));
+  RCHK(dalloc(&c->rhs, c->nmax));
+  RCHK(dalloc(&c->xv, c->nmax));
+  RCHK(dalloc(&c->diag, c->nmax));
+  RCHK(dalloc(&c->tmp, c->nmax));
+  RCHK(dalloc(&c->tmp2, c->nmax));
+  const int64_t maxblocks = std::max<int64_t>((c->nmax + kTile - 1) / kTile, g.ny + g.nx + 64);
+  RCHK(dalloc(&c->part, 2 * maxblocks + 64));
+  RCHK(dalloc(&c->ks, 1));
+  HIPCHK(hipHostMalloc((void**)&c->ks_host, sizeof(KState), hipHostMallocDefault));
+  RCHK(dalloc(&c->cfB, p.nx + 1));
+  RCHK(dalloc(&c->cfT, p.nx + 1));
+  RCHK(dalloc(&c->uL, p.ny + 1));
+  HIPCHK(hipHostMalloc((void**)&c->uL_host, (p.ny + 1) * sizeof(double), hipHostMallocDefault));
+  RCHK(dalloc(&c->pn, 2 * (int64_t)p.nx));
+  RCHK(dalloc(&c->dct_work, 2 * g.ncell()));
+  c->dct = dct_plan_create(p.nx, p.ny);
+  return 0;
+}
+
+static void ctx_free(drd_ctx* c) {
+  for (auto& a : c->f) hipFree(a);
+  hipFree(c->u_after);
+  hipFree(c->v_after);
+  hipFree(c->ph_after);
+  hipFree(c->rmu_after);
+  hipFree(c->zero_c);
+  for (auto& a : c->arg) hipFree(a);
+  for (auto& w : c->kw) hipFree(w);
+  hipFree(c->rhs);
+  hipFree(c->xv);
+  hipFree(c->diag);
+  hipFree(c->tmp);
+  hipFree(c->tmp2);
+  hipFree(c->part);
+  hipFree(c->ks);
+  if (c->ks_host) hipHostFree(c->ks_host);
+  hipFree(c->pre_lcc);
+  hipFree(c->ph_lcc);
+  hipFree(c->cfB);
+  hipFree(c->cfT);
+  hipFree(c->uL);
+  if (c->uL_host) hipHostFree(c->uL_host);
+  hipFree(c->pn);
+  hipFree(c->dct_work);
+  dct_plan_destroy(c->dct);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
+  if (!prm || !out) return set_err(DRD_E_ARG, "null argument");
+  if (prm->nx < 4 || prm->ny < 4) return set_err(DRD_E_ARG, "nx, ny must be >= 4");
+  if (prm->nx > 4096 || prm->ny > 4096) return set_err(DRD_E_ARG, "nx, ny must be <= 4096");
+  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
+    return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
+  if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
+  if (prm->variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "only the TP variant is built");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(DRD_E_HIP, "no HIP device visible");
+  drd_ctx* c = new drd_ctx();
+  c->p = *prm;
+  c->g = Grid{prm->nx, prm->ny};
+  c->s = make_scal(*prm);
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = set_err(DRD_E_HIP, "hipStreamCreate failed");
+  if (!rc) rc = ctx_alloc(c);
+  if (!rc) rc = build_tables(c);
+  if (rc) {
+    ctx_free(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int drd_finalize(drd_ctx* c) {
+  if (!c) return 0;
+  hipStreamSynchronize(c->stream);
+  ctx_free(c);
+  delete c;
+  return 0;
+}
+
+extern "C" int drd_synchronize(drd_ctx* c) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int drd_set_field(drd_ctx* c, int32_t field, const double* host) {
+  if (!c || !host) return set_err(DRD_E_ARG, "null argument");
+  const int64_t n = drd_field_size(&c->p, field);
+  if (n <= 0) return set_err(DRD_E_ARG, "unknown field id");
+  HIPCHK(hipMemcpyAsync(c->f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (field == DRD_F_ANGLEB || field == DRD_F_ANGLET) RCHK(upload_angles(c));
+  return 0;
+}
+
+extern "C" int drd_get_field(drd_ctx* c, int32_t field, double* host) {
+  if (!c || !host) return set_err(DRD_E_ARG, "null argument");
+  const int64_t n = drd_field_size(&c->p, field);
+  if (n <= 0) return set_err(DRD_E_ARG, "unknown field id");
+  HIPCHK(hipMemcpyAsync(host, c->f[field], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static Coef coef_of(const drd_ctx* c) {
+  Coef k{};
+  k.rho = c->f[DRD_F_RHO];
+  k.eta = c->f[DRD_F_ETA];
+  k.chi = c->f[DRD_F_BDR_PH_PENALTY];
+  k.bdr = c->f[DRD_F_BDR_PH];
+  k.bu = c->f[DRD_F_BDR_PH_USED];
+  k.bg = c->f[DRD_F_BDR_PH_GRAD];
+  k.minner = c->f[DRD_F_PH_M_INNER];
+  k.m0 = c->f[DRD_F_PH_M_0];
+  return k;
+}
+
+static int64_t op_size(const drd_ctx* c, int op) {
+  switch (op) {
+    case DRD_OP_VELO1:
+    case DRD_OP_VELO2:
+      return c->g.nuv();
+    case DRD_OP_PHMU:
+      return 2 * c->g.ncell();
+    case DRD_OP_PH:
+      return c->g.ncell();
+  }
+  return -1;
+}
+
+// y = alpha*A*x + beta*y (device pointers), or a fused Krylov mode
+static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, double alpha, double beta,
+                    double* a0, const double* b, KState* ks) {
+  const Coef k = coef_of(c);
+  switch (op) {
+    case DRD_OP_VELO1:
+    case DRD_OP_VELO2:
+      launch_velo_matvec(c->stream, op == DRD_OP_VELO1 ? 1 : 2, mode, c->g, c->s, k, x, y, alpha, beta, a0, b,
+                         c->part, ks);
+      break;
+    case DRD_OP_PHMU:
+      launch_phmu_matvec(c->stream, mode, c->g, c->s, k, x, y, alpha, beta, a0, b, c->part, ks);
+      break;
+    default:
+      return set_err(DRD_E_UNSUPPORTED, "operator not built");
+  }
+  return 0;
+}
+
+// Prepares a preconditioner once per solve (Jacobi diagonal).
+static int prec_setup(drd_ctx* c, int prec) {
+  if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag);
+  return 0;
+}
+
+// x_out = M^-1 b_in; may return b_in itself (identity).
+static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, const double** res,
+                      const KState* ks) {
+  switch (prec) {
+    case DRD_PS_IDENTITY:
+      *res = b;
+      return 0;
+    case DRD_PS_VELO1_JACOBI:
+      launch_div_diag(c->stream, op_size(c, op), b, c->diag, x, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_PHMU_JACOBI:
+      launch_phmu_jacobi(c->stream, c->g, c->s, b, x, ks);
+      *res = x;
+      return 0;
+  }
+  return set_err(DRD_E_UNSUPPORTED, "preconditioner not built");
+}
+
+static int read_state(drd_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->ks_host, c->ks, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// FP64 ||b - A x|| / ||b|| (true residual; an addition to the reference)
+static int true_resid(drd_ctx* c, int op, int64_t n, const double* b, const double* x, double* out) {
+  launch_copy(c->stream, n, b, c->tmp2);
+  RCHK(apply_op(c, op, MV_PLAIN, x, c->tmp2, 1.0, 0.0, nullptr, nullptr, nullptr));
+  launch_resid_norms(c->stream, n, b, c->tmp2, c->part, c->ks);
+  RCHK(read_state(c));
+  const double bn = std::sqrt(c->ks_host->red[1]);
+  *out = std::sqrt(c->ks_host->red[0]) / (bn == 0.0 ? 1.0 : bn);
+  return 0;
+}
+
+// BICGSTAB LIB/BiCGSTAB.f + LIB/BiCGSTABREVCOM.f on device vectors b, x.
+static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b, double* x, int maxit,
+                        double tol, drd_solve_info* info, bool want_true) {
+  if (n != op_size(c, op)) return set_err(DRD_E_ARG, "vector length does not match the operator");
+  if (maxit <= 0) {  // LIB/BiCGSTAB.f:161-163
+    if (info) *info = drd_solve_info{maxit, -3, tol, 0.0};
+    return 0;
+  }
+  double *R = c->kw[0], *RT = c->kw[1], *P = c->kw[2], *V = c->kw[3], *T = c->kw[4], *PHb = c->kw[5],
+         *SHb = c->kw[6];
+  KState h{};
+  h.tol = tol;
+  h.maxit = maxit;
+  h.iter = maxit;
+  h.resid = tol;
+  h.rho1 = 1.0;
+  h.omega = 1.0;
+  const double eps = std::ldexp(1.0, -53);  // DLAMCH('E')
+  h.breaktol = eps * eps;                   // LIB/getbreak.f:11-12
+  *c->ks_host = h;
+  HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  RCHK(prec_setup(c, prec));
+  launch_init_norms(c->stream, n, x, b, c->part, c->ks);
+  RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
+  int chunk = 1, total = 0;
+  for (;;) {
+    for (int it = 0; it < chunk; ++it) {
+      const double* ph = nullptr;
+      const double* sh = nullptr;
+      launch_bicg_p(c->stream, n, R, V, P, c->ks);
+      RCHK(apply_prec(c, prec, op, P, PHb, &ph, c->ks));
+      RCHK(apply_op(c, op, MV_DOT1, ph, V, 1.0, 0.0, RT, nullptr, c->ks));
+      launch_bicg_s(c->stream, n, R, V, c->part, c->ks);
+      launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
+      RCHK(apply_prec(c, prec, op, R, SHb, &sh, c->ks));
+      RCHK(apply_op(c, op, MV_DOT2, sh, T, 1.0, 0.0, R, nullptr, c->ks));
+      launch_bicg_xr(c->stream, n, x, R, RT, ph, sh, T, c->part, c->ks);
+    }
+    HIPCHK(hipGetLastError());
+    total += chunk;
+    RCHK(read_state(c));
+    if (c->ks_host->done || total > maxit + 2) break;
+    chunk = std::min(chunk * 2, 32);
+  }
+  const KState& s = *c->ks_host;
+  if (!s.done) return set_err(DRD_E_STATE, "BiCGSTAB state machine did not terminate");
+  if (info) {
+    info->iter = s.iter;
+    info->info = s.info;
+    info->resid = s.resid;
+    info->true_resid = 0.0;
+    if (want_true) RCHK(true_resid(c, op, n, b, x, &info->true_resid));
+  }
+  return 0;
+}
+
+// ---- velocity --------------------------------------------------------------
+static int velo_prec(const drd_ctx* c, int bdf) {
+  switch (c->p.velo_prec) {
+    case DRD_PREC_IDENTITY:
+      return DRD_PS_IDENTITY;
+    case DRD_PREC_JACOBI:
+      return DRD_PS_VELO1_JACOBI;
+    default:
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
+  }
+}
+
+// uL(k) TP/VeloSolve_2_combined.f90:272-307, z(0,k) = (k+0.5) dz
+static int inflow(drd_ctx* c, double t) {
+  const drd_params& p = c->p;
+  if (p.lr_bd_type != 'd') return 0;
+  const double cf = 1.0 - std::exp(-t / p.twall);
+  const double th = p.bdr_upp - p.bdr_low;
+  for (int k = 0; k < p.ny; ++k) {
+    const double z0 = (k + 0.5) * p.dz;
+    if (z0 > p.bdr_low && z0 < p.bdr_upp) {
+      const double d = z0 - p.bdr_low - th / 2;
+      c->uL_host[k] = p.UU * (-(d * d) / th / th * 4 + 1) * cf;
+    } else {
+      c->uL_host[k] = 0.0;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->uL, c->uL_host, p.ny * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double* psi2, const double* u_pre,
+                          const double* v_pre, const double* ph_after, const double* rmu_after, double t,
+                          const double* pf1, const double* pf2, double* u_after, double* v_after,
+                          drd_solve_info* info, bool want_true) {
+  if (c->p.xlsB >= 1.0e-8) return set_err(DRD_E_UNSUPPORTED, "slip (xlsB >= 1e-8) branch is empty in the reference");
+  RCHK(inflow(c, t));
+  VeloRhsArgs a{};
+  a.u = c->f[DRD_F_U];
+  a.v = c->f[DRD_F_V];
+  a.u_pre = u_pre;
+  a.v_pre = v_pre;
+  a.p = c->f[DRD_F_P];
+  a.psi1 = psi1;
+  a.psi2 = psi2;
+  a.rho = c->f[DRD_F_RHO];
+  a.eta = c->f[DRD_F_ETA];
+  a.bu = c->f[DRD_F_BDR_PH_USED];
+  a.ph_after = ph_after;
+  a.rmu_after = rmu_after;
+  a.pf1 = pf1 ? pf1 : c->zero_c;
+  a.pf2 = pf2 ? pf2 : c->zero_c;
+  a.uL = c->uL;
+  if (bdf == 1) {  // unused by the BDF1 expressions
+    a.u_pre = c->f[DRD_F_U];
+    a.v_pre = c->f[DRD_F_V];
+    a.psi1 = a.psi2 = c->zero_c;
+  }
+  launch_velo_rhs(c->stream, bdf, c->g, c->s, a, c->rhs);
+  launch_velo_pack(c->stream, c->g, c->f[DRD_F_U], c->f[DRD_F_V], c->xv);
+  const int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
+  RCHK(run_bicgstab(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv,
+                    bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2, c->p.tol, info, want_true));
+  launch_velo_unpack(c->stream, c->g, c->s, c->xv, c->uL, u_after, v_after);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- Cahn-Hilliard ---------------------------------------------------------
+static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_solve_info* info,
+                           bool want_true) {
+  launch_phmu_m0(c->stream, c->g, c->p.phi_mode, c->f[DRD_F_PH], c->f[DRD_F_BDR_PH], c->f[DRD_F_PH_M_0]);
+  PhmuRhsArgs a{};
+  a.ph = c->f[DRD_F_PH];
+  a.rmu = c->f[DRD_F_RMU];
+  a.u = c->f[DRD_F_U];
+  a.v = c->f[DRD_F_V];
+  a.bdr = c->f[DRD_F_BDR_PH];
+  a.bu = c->f[DRD_F_BDR_PH_USED];
+  a.bg = c->f[DRD_F_BDR_PH_GRAD];
+  a.cfB = c->cfB;
+  a.cfT = c->cfT;
+  a.cosa = c->cosa;
+  a.sqrt2 = c->sqrt2;
+  launch_phmu_rhs(c->stream, c->g, c->s, a, c->rhs, c->xv, c->pn);
+  RCHK(run_bicgstab(c, DRD_OP_PHMU, DRD_PS_PHMU_JACOBI, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu,
+                    c->p.tol, info, want_true));
+  launch_phmu_unpack(c->stream, c->g, c->s, c->xv, c->pn, ph_after, rmu_after);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- pressure ---------------------------------------------------------------
+static int presolve_dev(drd_ctx* c, double* psi, double coef, double* p_update, double* mean_div) {
+  launch_pressure(c->stream, c->dct, c->g, c->s, c->f[DRD_F_U], c->f[DRD_F_V], c->pre_lcc, coef, psi, p_update,
+                  c->dct_work, c->part, c->ks);
+  HIPCHK(hipGetLastError());
+  if (mean_div) {
+    RCHK(read_state(c));
+    *mean_div = c->ks_host->red[0] / c->p.nx / c->p.ny;
+  }
+  return 0;
+}
+
+// ---- host-array entry points ---------------------------------------------
+static int up(drd_ctx* c, double* dst, const double* src, int64_t n) {
+  if (!src) return 0;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+static int down(drd_ctx* c, double* dst, const double* src, int64_t n) {
+  if (!dst) return 0;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+extern "C" int drd_velosolve_combined(drd_ctx* c, const double* psi, double* u_after, double* v_after,
+                                      const double* rmu_after, const double* ph_after, double t,
+                                      const double* pf1, const double* pf2, drd_solve_info* info) {
+  if (!c || !u_after || !v_after || !rmu_after || !ph_after) return set_err(DRD_E_ARG, "null argument");
+  (void)psi;  // temp_p = p (TP/VeloSolve_combined.f90:192)
+  const Grid& g = c->g;
+  RCHK(up(c, c->arg[0], ph_after, g.nc()));
+  RCHK(up(c, c->arg[1], rmu_after, g.nc()));
+  RCHK(up(c, c->arg[2], pf1, g.nc()));
+  RCHK(up(c, c->arg[3], pf2, g.nc()));
+  drd_solve_info loc{};
+  RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->arg[0], c->arg[1], t, pf1 ? c->arg[2] : nullptr,
+                      pf2 ? c->arg[3] : nullptr, c->u_after, c->v_after, &loc, true));
+  RCHK(down(c, u_after, c->u_after, g.nuf()));
+  RCHK(down(c, v_after, c->v_after, g.nvf()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
+  return 0;
+}
+
+extern "C" int drd_velosolve_2_combined(drd_ctx* c, const double* psi1, const double* psi2, const double* u_pre,
+                                        double* u_after, const double* v_pre, double* v_after,
+                                        const double* ph_pre, const double* ph_after, const double* rmu_pre,
+                                        const double* rmu_after, double t, const double* pf1, const double* pf2,
+                                        drd_solve_info* info) {
+  if (!c || !psi1 || !psi2 || !u_pre || !u_after || !v_pre || !v_after || !ph_after || !rmu_after)
+    return set_err(DRD_E_ARG, "null argument");
+  (void)ph_pre;
+  (void)rmu_pre;  // not read by the active code
+  const Grid& g = c->g;
+  RCHK(up(c, c->arg[0], ph_after, g.nc()));
+  RCHK(up(c, c->arg[1], rmu_after, g.nc()));
+  RCHK(up(c, c->arg[2], pf1, g.nc()));
+  RCHK(up(c, c->arg[3], pf2, g.nc()));
+  RCHK(up(c, c->arg[4], psi1, g.nc()));
+  RCHK(up(c, c->arg[5], psi2, g.nc()));
+  RCHK(up(c, c->arg[6], u_pre, g.nuf()));
+  RCHK(up(c, c->arg[7], v_pre, g.nvf()));
+  drd_solve_info loc{};
+  RCHK(velo_solve_dev(c, 2, c->arg[4], c->arg[5], c->arg[6], c->arg[7], c->arg[0], c->arg[1], t,
+                      pf1 ? c->arg[2] : nullptr, pf2 ? c->arg[3] : nullptr, c->u_after, c->v_after, &loc, true));
+  RCHK(down(c, u_after, c->u_after, g.nuf()));
+  RCHK(down(c, v_after, c->v_after, g.nvf()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
+  return 0;
+}
+
+extern "C" int drd_presolve_2(drd_ctx* c, double* psi, double coef, double* mean_div) {
+  if (!c || !psi) return set_err(DRD_E_ARG, "null argument");
+  RCHK(presolve_dev(c, c->arg[0], coef, nullptr, mean_div));
+  RCHK(down(c, psi, c->arg[0], c->g.nc()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int drd_ph_mu_solve(drd_ctx* c, double* ph_after, double* rmu_after, double t, drd_solve_info* info) {
+  if (!c || !ph_after || !rmu_after) return set_err(DRD_E_ARG, "null argument");
+  (void)t;
+  drd_solve_info loc{};
+  RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc, true));
+  RCHK(down(c, ph_after, c->ph_after, c->g.nc()));
+  RCHK(down(c, rmu_after, c->rmu_after, c->g.nc()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
+  return 0;
+}
+
+extern "C" int drd_phsolve(drd_ctx* c, double* rmu_after, double* ph_after, double t, drd_solve_info* info) {
+  (void)c;
+  (void)rmu_after;
+  (void)ph_after;
+  (void)t;
+  (void)info;
+  return set_err(DRD_E_UNSUPPORTED, "PhSolve (MF variant) is not built");
+}
+
+extern "C" int drd_matvec(drd_ctx* c, int32_t op, double alpha, const double* x, double beta, double* y) {
+  if (!c || !x || !y) return set_err(DRD_E_ARG, "null argument");
+  const int64_t n = op_size(c, op);
+  if (n < 0) return set_err(DRD_E_ARG, "unknown operator");
+  RCHK(up(c, c->tmp, x, n));
+  RCHK(up(c, c->tmp2, y, n));
+  RCHK(apply_op(c, op, MV_PLAIN, c->tmp, c->tmp2, alpha, beta, nullptr, nullptr, nullptr));
+  RCHK(down(c, y, c->tmp2, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int drd_psolve(drd_ctx* c, int32_t prec, int32_t op, double* x_out, const double* b_in) {
+  if (!c || !x_out || !b_in) return set_err(DRD_E_ARG, "null argument");
+  const int64_t n = op_size(c, op);
+  if (n < 0) return set_err(DRD_E_ARG, "unknown operator");
+  RCHK(up(c, c->tmp, b_in, n));
+  RCHK(prec_setup(c, prec));
+  const double* res = nullptr;
+  RCHK(apply_prec(c, prec, op, c->tmp, c->tmp2, &res, nullptr));
+  RCHK(down(c, x_out, res, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int drd_bicgstab(drd_ctx* c, int32_t op, int32_t prec, int32_t n, const double* b, double* x,
+                            int32_t* iter, double* resid, int32_t* info) {
+  if (!c || !b || !x || !iter || !resid || !info) return set_err(DRD_E_ARG, "null argument");
+  if (n < 0) {  // LIB/BiCGSTAB.f:157-158
+    *info = -1;
+    return 0;
+  }
+  RCHK(up(c, c->rhs, b, n));
+  RCHK(up(c, c->xv, x, n));
+  drd_solve_info loc{};
+  RCHK(run_bicgstab(c, op, prec, n, c->rhs, c->xv, *iter, *resid, &loc, false));
+  RCHK(down(c, x, c->xv, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *iter = loc.iter;
+  *resid = loc.resid;
+  *info = loc.info;
+  return 0;
+}
+
+extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32_t restart, const double* b,
+                          double* x, int32_t* iter, double* resid, int32_t* info) {
+  (void)c;
+  (void)op;
+  (void)prec;
+  (void)n;
+  (void)restart;
+  (void)b;
+  (void)x;
+  (void)iter;
+  (void)resid;
+  (void)info;
+  return set_err(DRD_E_UNSUPPORTED, "FGMRES is not built yet");
+}
+
+// ---- measurement hook ---------------------------------------------------------
+extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double* avg_ms, double* bytes) {
+  if (!c || !avg_ms || !bytes || reps <= 0) return set_err(DRD_E_ARG, "bad argument");
+  const Grid& g = c->g;
+  const double C8 = 8.0 * (double)g.ncell(), N8 = 8.0 * (double)g.nuv();
+  KState h{};
+  h.tol = 0.0;
+  h.maxit = 1 << 30;
+  h.iter = 2;
+  h.rho = h.rho1 = h.alpha = h.omega = 1.0;
+  h.bnrm2 = 1.0;
+  *c->ks_host = h;
+  HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  const Coef k = coef_of(c);
+  auto launch = [&]() {
+    switch (which) {
+      case DRD_K_VELO_MATVEC:
+        launch_velo_matvec(c->stream, 2, MV_DOT1, g, c->s, k, c->kw[2], c->kw[3], 1.0, 0.0, c->kw[1], nullptr,
+                           c->part, c->ks);
+        break;
+      case DRD_K_PHMU_MATVEC:
+        launch_phmu_matvec(c->stream, MV_DOT1, g, c->s, k, c->kw[2], c->kw[3], 1.0, 0.0, c->kw[1], nullptr,
+                           c->part, c->ks);
+        break;
+      case DRD_K_BICG_XR:
+        launch_bicg_xr(c->stream, g.nuv(), c->xv, c->kw[0], c->kw[1], c->kw[5], c->kw[6], c->kw[4], c->part,
+                       c->ks);
+        break;
+      case DRD_K_PRESSURE:
+        launch_pressure(c->stream, c->dct, g, c->s, c->f[DRD_F_U], c->f[DRD_F_V], c->pre_lcc, 1.0, c->arg[0],
+                        nullptr, c->dct_work, c->part, nullptr);
+        break;
+    }
+  };
+  switch (which) {
+    case DRD_K_VELO_MATVEC:  // x, y, rtld (Nuv each) + rho, eta, chi
+      *bytes = 3 * N8 + 3 * C8;
+      break;
+    case DRD_K_PHMU_MATVEC:  // x, y, rtld (2C each) + bdr_used, M_inner, M_0
+      *bytes = 6 * C8 + 3 * C8;
+      break;
+    case DRD_K_BICG_XR:  // x rw, r rw, rtld, phat, shat, t
+      *bytes = 8 * N8;
+      break;
+    case DRD_K_PRESSURE:  // u, v in; psi out (p untouched here)
+      *bytes = N8 + 2 * C8 + C8;
+      break;
+    default:
+      return set_err(DRD_E_ARG, "unknown kernel id");
+  }
+  launch();  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) launch();
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *avg_ms = (double)ms / reps;
+  return 0;
+}
+
+// ---- time step (TP/mcl6.f90:257-303 and :395-502) ---------------------------
+extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  if (bdf != 1 && bdf != 2) return set_err(DRD_E_ARG, "bdf must be 1 or 2");
+  const drd_params& p = c->p;
+  drd_step_info loc{};
+  const double gamma1 = std::min(1.0, p.lambda_rho);
+  double** F = c->f;
+  RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false));
+  if (bdf == 1) {
+    RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt, nullptr,
+                        nullptr, c->u_after, c->v_after, &loc.velo, false));
+  } else {
+    RCHK(velo_solve_dev(c, 2, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], F[DRD_F_V_PRE], c->ph_after,
+                        c->rmu_after, t + p.dt, nullptr, nullptr, c->u_after, c->v_after, &loc.velo, false));
+  }
+  // u_pre = u; u = u_after (pointer rotation: u_after is fully rewritten next step)
+  std::swap(F[DRD_F_U_PRE], F[DRD_F_U]);
+  std::swap(F[DRD_F_U], c->u_after);
+  std::swap(F[DRD_F_V_PRE], F[DRD_F_V]);
+  std::swap(F[DRD_F_V], c->v_after);
+  // ph_pre = ph; ph = ph_after; rmu likewise
+  std::swap(F[DRD_F_PH_PRE], F[DRD_F_PH]);
+  std::swap(F[DRD_F_PH], c->ph_after);
+  std::swap(F[DRD_F_RMU_PRE], F[DRD_F_RMU]);
+  std::swap(F[DRD_F_RMU], c->rmu_after);
+  double coef;
+  if (bdf == 1) {
+    coef = p.re * gamma1 / p.dt;
+  } else {
+    std::swap(F[DRD_F_PSI1], F[DRD_F_PSI2]);  // psi1 = psi2 (psi2 fully rewritten below)
+    coef = 3 * p.re * gamma1 / 2. / p.dt;
+  }
+  RCHK(presolve_dev(c, F[DRD_F_PSI2], coef, F[DRD_F_P], &loc.div_before));
+  if (info) *info = loc;
+  return 0;
+}

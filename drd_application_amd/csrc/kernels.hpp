@@ -1,0 +1,156 @@
+// kernels.hpp -- kernel launchers and the device-side BiCGSTAB state machine.
+#pragma once
+
+#include "drd_common.hpp"
+
+namespace drd {
+
+constexpr int kTile = 1024;  // unknowns per workgroup in the fused sweeps
+
+// matvec modes (see k_velo_matvec)
+enum { MV_PLAIN = 0, MV_DOT1 = 1, MV_DOT2 = 2, MV_INIT = 3 };
+
+// phases whose finished reductions drive the state machine
+enum { PH_INIT_NORMS = 0, PH_INIT_R, PH_MV1, PH_S, PH_MV2, PH_XR, PH_EARLY, PH_NONE };
+
+struct Coef {
+  const double *rho, *eta, *chi, *bdr, *bu, *bg, *minner, *m0;
+};
+
+struct VeloRhsArgs {
+  const double *u, *v, *u_pre, *v_pre, *p, *psi1, *psi2;
+  const double *rho, *eta, *bu, *ph_after, *rmu_after, *pf1, *pf2, *uL;
+};
+
+struct PhmuRhsArgs {
+  const double *ph, *rmu, *u, *v, *bdr, *bu, *bg, *cfB, *cfT;
+  double cosa, sqrt2;
+};
+
+// Decisions of LIB/BiCGSTABREVCOM.f taken where each reduction finishes.
+// Runs on thread 0 of the last workgroup of the producing kernel.
+__device__ inline void kfinalize(KState* st, int phase, const double* t) {
+  switch (phase) {
+    case PH_INIT_NORMS: {  // t = {x.x, b.b}            REVCOM:229-253
+      st->xnz = (sqrt(t[0]) != 0.0);
+      double bn = sqrt(t[1]);
+      st->bnrm2 = (bn == 0.0) ? 1.0 : bn;
+      break;
+    }
+    case PH_INIT_R: {  // t = {r.r} with rtld = r      REVCOM:248-263
+      if (st->xnz && sqrt(t[0]) <= st->tol) {  // ITER and RESID stay as passed
+        st->info = 0;
+        st->done = 1;
+        break;
+      }
+      st->iter = 1;
+      st->rho = t[0];
+      if (fabs(st->rho) < st->breaktol) {
+        st->info = -10;
+        st->done = 1;
+      }
+      break;
+    }
+    case PH_MV1:  // t = {rtld.v}                     REVCOM:307
+      st->alpha = st->rho / t[0];
+      break;
+    case PH_S: {  // t = {s.s}                        REVCOM:311-316
+      const double sn = sqrt(t[0]);
+      st->snrm = sn;
+      if (sn <= st->tol) {
+        st->resid = sn / st->bnrm2;
+        st->info = 0;
+        st->early = 1;
+      }
+      break;
+    }
+    case PH_MV2:  // t = {t.s, t.t}                   REVCOM:349-350
+      st->omega = t[0] / t[1];
+      break;
+    case PH_XR: {  // t = {r.r, rtld.r}  STOPTEST2.f:13-16, REVCOM:374-388, :261-264
+      st->resid = sqrt(t[0]) / st->bnrm2;
+      if (st->resid <= st->tol) {
+        st->info = 0;
+        st->done = 1;
+        break;
+      }
+      if (st->iter == st->maxit) {
+        st->info = 1;
+        st->done = 1;
+        break;
+      }
+      if (fabs(st->omega) < st->breaktol) {
+        st->info = -11;
+        st->done = 1;
+        break;
+      }
+      st->rho1 = st->rho;
+      st->iter = st->iter + 1;
+      st->rho = t[1];
+      if (fabs(st->rho) < st->breaktol) {
+        st->info = -10;
+        st->done = 1;
+      }
+      break;
+    }
+    case PH_EARLY:
+      st->done = 1;
+      break;
+    default:
+      break;
+  }
+  st->red[0] = t[0];
+}
+
+// ---- launchers (kernels_stencil.hip) ----
+void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
+                        const double* x, double* y, double alpha, double beta, double* a0,
+                        const double* b, double* part, KState* ks);
+void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
+                        const double* x, double* y, double alpha, double beta, double* a0,
+                        const double* b, double* part, KState* ks);
+void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag);
+void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,
+                     const KState* ks);
+void launch_phmu_jacobi(hipStream_t st, const Grid& g, const Scal& s, const double* b, double* x,
+                        const KState* ks);
+void launch_velo_pack(hipStream_t st, const Grid& g, const double* u, const double* v, double* x);
+void launch_velo_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* uL,
+                        double* u, double* v);
+void launch_velo_rhs(hipStream_t st, int bdf, const Grid& g, const Scal& s, const VeloRhsArgs& a,
+                     double* rhs);
+void launch_phmu_m0(hipStream_t st, const Grid& g, int mode, const double* ph, const double* bdr,
+                    double* m0);
+void launch_phmu_rhs(hipStream_t st, const Grid& g, const Scal& s, const PhmuRhsArgs& a, double* rhs,
+                     double* x0, double* pn);
+void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* pn,
+                        double* ph, double* mu);
+
+// ---- launchers (kernels_krylov.hip) ----
+void launch_init_norms(hipStream_t st, int64_t n, const double* x, const double* b, double* part,
+                       KState* ks);
+void launch_bicg_p(hipStream_t st, int64_t n, const double* r, const double* v, double* p, KState* ks);
+void launch_bicg_s(hipStream_t st, int64_t n, double* r, const double* v, double* part, KState* ks);
+void launch_bicg_early(hipStream_t st, int64_t n, double* x, const double* ph, double* part, KState* ks);
+void launch_bicg_xr(hipStream_t st, int64_t n, double* x, double* r, const double* rt, const double* ph,
+                    const double* sh, const double* t, double* part, KState* ks);
+void launch_copy(hipStream_t st, int64_t n, const double* a, double* b);
+void launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta, double* y);
+// sum of squares of (b - y) and of b -> out[0], out[1] (FP64, deterministic)
+void launch_resid_norms(hipStream_t st, int64_t n, const double* b, const double* ax, double* part,
+                        KState* scratch_ks);
+void launch_cell_axpy(hipStream_t st, int64_t n, const double* a, double* b);  // b = a + b
+
+// ---- launchers (kernels_dct.hip) ----
+struct DctPlan;
+DctPlan* dct_plan_create(int nx, int ny);
+void dct_plan_destroy(DctPlan* p);
+// pressure projection: psi2 = coef * Poisson^-1 div(u,v); p = psi2 + p
+void launch_pressure(hipStream_t st, DctPlan* pl, const Grid& g, const Scal& s, const double* u,
+                     const double* v, const double* pre_lcc, double coef, double* psi, double* p,
+                     double* work, double* part, KState* ks);
+// 2-D FFTW-convention REDFT10 / REDFT01 of an nx x ny array (tests)
+void launch_dct2_2d(hipStream_t st, DctPlan* pl, double* a, double* work);
+void launch_dct3_2d(hipStream_t st, DctPlan* pl, double* a, double* work);
+
+}  // namespace drd

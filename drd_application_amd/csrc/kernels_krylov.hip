@@ -1,0 +1,198 @@
+// kernels_krylov.hip -- fused BiCGSTAB vector kernels (FP64).
+//
+// Each iteration of LIB/BiCGSTABREVCOM.f:257-388 runs as
+//   p-update | M^-1 | A + (rtld,v) | s-update + ||s|| | [early x] | M^-1 |
+//   A + (t,s),(t,t) | x,r-update + ||r||,(rtld,r)
+// with every reduction finished by the last workgroup of its kernel
+// (grid_sum), which also takes the branch decisions (kfinalize).  The
+// kernels test KState::done/early first, so the host may queue whole
+// iterations ahead and only polls `done` now and then.
+//
+// The vector updates keep DAXPY's y + a*x rounding (no contraction), so
+// they reproduce the CPU oracle bit for bit; only the reduction order differs.
+#include "kernels.hpp"
+
+namespace drd {
+
+__device__ __forceinline__ bool kact(const KState* st) { return !st->done && !st->early; }
+
+// ||x||^2, ||b||^2                                 REVCOM:229-253
+__global__ __launch_bounds__(kBlock) void k_init_norms(int64_t n, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* part,
+                                                       KState* st) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      const double a = x[q], c = b[q];
+      acc[0] += a * a;
+      acc[1] += c * c;
+    }
+  }
+  double tot[2];
+  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_INIT_NORMS, tot);
+}
+
+// p = r (first iteration) or p = beta*(p - omega*v) + r   REVCOM:266-275
+__global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const double* __restrict__ r,
+                                                   const double* __restrict__ v, double* __restrict__ p,
+                                                   const KState* st) {
+  if (!kact(st)) return;
+  const bool first = st->iter <= 1;
+  const double omega = st->omega;
+  const double beta = first ? 0.0 : (st->rho / st->rho1) * (st->alpha / st->omega);
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      if (first) {
+        p[q] = r[q];
+      } else {
+        double t = p[q] + (-omega) * v[q];
+        t = beta * t;
+        p[q] = t + r[q];
+      }
+    }
+  }
+}
+
+// s = r - alpha*v (S aliases R), ||s||^2           REVCOM:311-313
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t n, double* __restrict__ r,
+                                                   const double* __restrict__ v, double* part, KState* st) {
+  if (!kact(st)) return;
+  const double alpha = st->alpha;
+  double acc[1] = {0.0};
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      const double s = r[q] + (-alpha) * v[q];
+      r[q] = s;
+      acc[0] += s * s;
+    }
+  }
+  double tot[1];
+  if (grid_sum<1>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_S, tot);
+}
+
+// early exit: x = x + alpha*phat                   REVCOM:314
+__global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __restrict__ x,
+                                                       const double* __restrict__ ph, double* part,
+                                                       KState* st) {
+  if (!st->early || st->done) return;
+  const double alpha = st->alpha;
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) x[q] = x[q] + alpha * ph[q];
+  }
+  double acc[1] = {0.0}, tot[1];
+  if (grid_sum<1>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_EARLY, tot);
+}
+
+// x = x + alpha*phat + omega*shat; r = s - omega*t; ||r||^2, (rtld,r)
+// REVCOM:354-361 and the next iteration's rho (:263).  With the identity
+// preconditioner shat aliases r (and phat aliases p), so r and sh carry no
+// __restrict__; each element is read before it is written by its own thread.
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t n, double* __restrict__ x, double* r,
+                                                    const double* __restrict__ rt, const double* __restrict__ ph,
+                                                    const double* sh, const double* __restrict__ t,
+                                                    double* part, KState* st) {
+  if (!kact(st)) return;
+  const double alpha = st->alpha, omega = st->omega;
+  double acc[2] = {0.0, 0.0};
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      double xv = x[q] + alpha * ph[q];
+      xv = xv + omega * sh[q];
+      x[q] = xv;
+      const double rv = r[q] + (-omega) * t[q];
+      r[q] = rv;
+      acc[0] += rv * rv;
+      acc[1] += rt[q] * rv;
+    }
+  }
+  double tot[2];
+  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_XR, tot);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < n) b[q] = a[q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double alpha, const double* __restrict__ x,
+                                                  double beta, double* __restrict__ y) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < n) y[q] = alpha * x[q] + beta * y[q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_axpy(int64_t n, const double* __restrict__ a,
+                                                      double* __restrict__ b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < n) b[q] = a[q] + b[q];
+}
+
+// ||b - ax||^2 and ||b||^2 into ks->red[0..1] (true-residual check)
+__global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t n, const double* __restrict__ b,
+                                                        const double* __restrict__ ax, double* part,
+                                                        KState* st) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      const double d = b[q] - ax[q];
+      acc[0] += d * d;
+      acc[1] += b[q] * b[q];
+    }
+  }
+  double tot[2];
+  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) {
+    st->red[0] = tot[0];
+    st->red[1] = tot[1];
+  }
+}
+
+static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void launch_init_norms(hipStream_t s, int64_t n, const double* x, const double* b, double* part, KState* ks) {
+  hipLaunchKernelGGL(k_init_norms, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, b, part, ks);
+}
+void launch_bicg_p(hipStream_t s, int64_t n, const double* r, const double* v, double* p, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_p, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, p, ks);
+}
+void launch_bicg_s(hipStream_t s, int64_t n, double* r, const double* v, double* part, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_s, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, part, ks);
+}
+void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, part, ks);
+}
+void launch_bicg_xr(hipStream_t s, int64_t n, double* x, double* r, const double* rt, const double* ph,
+                    const double* sh, const double* t, double* part, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_xr, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, r, rt, ph, sh, t, part, ks);
+}
+void launch_copy(hipStream_t s, int64_t n, const double* a, double* b) {
+  hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
+}
+void launch_axpby(hipStream_t s, int64_t n, double alpha, const double* x, double beta, double* y) {
+  hipLaunchKernelGGL(k_axpby, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, alpha, x, beta, y);
+}
+void launch_cell_axpy(hipStream_t s, int64_t n, const double* a, double* b) {
+  hipLaunchKernelGGL(k_cell_axpy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
+}
+void launch_resid_norms(hipStream_t s, int64_t n, const double* b, const double* ax, double* part,
+                        KState* ks) {
+  hipLaunchKernelGGL(k_resid_norms, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, b, ax, part, ks);
+}
+
+}  // namespace drd

@@ -1,0 +1,649 @@
+// kernels_stencil.hip -- MAC-grid and Cahn-Hilliard stencil kernels (FP64).
+//
+// Every kernel keeps the reference's floating-point evaluation order and is
+// compiled with -ffp-contract=off, so its results equal the CPU oracle (and
+// the reference Fortran) bit for bit.  One thread owns one unknown of the
+// packed Krylov vector; a workgroup covers kTile consecutive unknowns, so
+// neighbouring rows of a tile are served from L2 and each HBM line of x is
+// fetched once per sweep.
+#include "kernels.hpp"
+
+namespace drd {
+
+// Ghost rules applied on the fly to a packed velocity vector:
+// btoA_uv TP/matrix_tool.f90:32-75 plus the overrides of
+// MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:554-583.
+struct VeloIn {
+  const double* __restrict__ x;
+  int nx, ny, lr_d;
+  int64_t mn;
+  __device__ __forceinline__ double uraw(int i, int k) const { return x[i + (int64_t)(nx + 1) * k]; }
+  // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils)
+  __device__ __forceinline__ double U(int i, int k) const {
+    if (k < 0) return -U(i, 0);
+    if (k >= ny) return -U(i, ny - 1);
+    if (i < 0) return lr_d ? -uraw(1, k) : uraw(1, k);
+    if (i > nx) return lr_d ? -uraw(nx - 1, k) : uraw(nx - 1, k);
+    return uraw(i, k);
+  }
+  // f2(i,k) for -1<=i<=nx, 0<=k<=ny
+  __device__ __forceinline__ double V(int i, int k) const {
+    if (k <= 0 || k >= ny) return 0.0;
+    i = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+    return x[mn + i + (int64_t)nx * (k - 1)];
+  }
+};
+
+// y = A x for one unknown q of the velocity operator.
+// MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
+// MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
+template <int BDF>
+__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const double* __restrict__ rho,
+                                           const double* __restrict__ eta,
+                                           const double* __restrict__ chi, const VeloIn& X,
+                                           int64_t q) {
+  const int nx = g.nx;
+  double r;
+  if (q < X.mn) {
+    const int k = (int)(q / (nx + 1));
+    const int i = (int)(q - (int64_t)k * (nx + 1));
+    const double f = X.U(i, k);
+    if (BDF == 2)
+      r = f * 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
+          f * s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
+    else
+      r = f * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0;
+    const double e00 = eta[g.ci(i, k)], e10 = eta[g.ci(i - 1, k)];
+    double cr = e00, cl = e10;
+    r = r - 2.0 * s.dt * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f) / s.dr2;
+    cr = (eta[g.ci(i, k + 1)] + eta[g.ci(i - 1, k + 1)] + e00 + e10) / 4.0;
+    cl = (e00 + e10 + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    r = r - s.dt * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f) / s.dz2;
+    r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
+  } else {
+    const int64_t t = q - X.mn;
+    const int k = (int)(t / nx) + 1;
+    const int i = (int)(t - (int64_t)(k - 1) * nx);
+    const double f = X.V(i, k);
+    if (BDF == 2)
+      r = f * 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
+          f * s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
+    else
+      r = f * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0;
+    const double e00 = eta[g.ci(i, k)], e01 = eta[g.ci(i, k - 1)];
+    double cr = e00, cl = e01;
+    r = r - 2.0 * s.dt * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f) / s.dz2;
+    cr = (eta[g.ci(i + 1, k)] + e00 + eta[g.ci(i + 1, k - 1)] + e01) / 4.0;
+    cl = (e00 + eta[g.ci(i - 1, k)] + e01 + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    r = r - s.dt * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f) / s.dr2;
+    r = r - s.dt * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1))) / s.dr / s.dz;
+  }
+  return r;
+}
+
+__device__ __forceinline__ bool kactive(const KState* st) {
+  return st == nullptr || (!st->done && !st->early);
+}
+
+// Generic matvec with the reductions BiCGSTAB fuses into it.
+//   MV_PLAIN : y = alpha*A*x + beta*y
+//   MV_DOT1  : y = A x, partial (a0, y)                 -> PH_MV1
+//   MV_DOT2  : y = A x, partials (y, a0), (y, y)        -> PH_MV2
+//   MV_INIT  : y = b - A x (if ||x||!=0, else b); a0 := y; partial (y,y) -> PH_INIT_R
+template <int BDF, int MODE>
+__global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const double* __restrict__ rho,
+                                                        const double* __restrict__ eta,
+                                                        const double* __restrict__ chi,
+                                                        const double* __restrict__ x, double* __restrict__ y,
+                                                        double alpha, double beta, double* __restrict__ a0,
+                                                        const double* __restrict__ b, double* part,
+                                                        KState* st) {
+  if (MODE != MV_PLAIN && !kactive(st)) return;
+  VeloIn X{x, g.nx, g.ny, s.lr_d, g.mn()};
+  const int64_t n = g.nuv();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double acc[2] = {0.0, 0.0};
+  const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q >= n) break;
+    if (MODE == MV_PLAIN) {
+      const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
+      y[q] = alpha * z + beta * y[q];
+    } else if (MODE == MV_INIT) {
+      double r = b[q];
+      if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
+      y[q] = r;
+      a0[q] = r;
+      acc[0] += r * r;
+    } else {
+      const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
+      y[q] = z;
+      if (MODE == MV_DOT1) {
+        acc[0] += a0[q] * z;
+      } else {
+        acc[0] += z * a0[q];
+        acc[1] += z * z;
+      }
+    }
+  }
+  if (MODE == MV_PLAIN) return;
+  double tot[2];
+  if (MODE == MV_DOT2) {
+    if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+  } else {
+    double a1[1] = {acc[0]}, t1[1];
+    if (grid_sum<1>(a1, part, &st->ticket, t1) && threadIdx.x == 0)
+      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, t1);
+  }
+}
+
+// PSolve_uv_CG (Jacobi) TP/VeloSolve_combined.f90:692-741: the diagonal,
+// with the reference's rho(i-1,j) average on the v rows (:720).
+__global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, const double* __restrict__ rho,
+                                                             const double* __restrict__ eta,
+                                                             double* __restrict__ diag) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  if (q >= g.nuv()) return;
+  double tmp;
+  if (q < g.mn()) {
+    const int j = (int)(q / (nx + 1));
+    const int i = (int)(q - (int64_t)j * (nx + 1));
+    tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
+    double cr = eta[g.ci(i, j)], cl = eta[g.ci(i - 1, j)];
+    tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dr2;
+    cr = (eta[g.ci(i, j + 1)] + eta[g.ci(i - 1, j + 1)] + eta[g.ci(i, j)] + eta[g.ci(i - 1, j)]) / 4.0;
+    cl = (eta[g.ci(i, j)] + eta[g.ci(i - 1, j)] + eta[g.ci(i, j - 1)] + eta[g.ci(i - 1, j - 1)]) / 4.0;
+    if (j == 0)
+      tmp = tmp + s.dt * (cl + (cl + cr)) / s.dz2;
+    else if (j == ny - 1)
+      tmp = tmp + s.dt * (cr + (cl + cr)) / s.dz2;
+    else
+      tmp = tmp + s.dt * (cl + cr) / s.dz2;
+  } else {
+    const int64_t t = q - g.mn();
+    const int j = (int)(t / nx) + 1;
+    const int i = (int)(t - (int64_t)(j - 1) * nx);
+    tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
+    double cr = eta[g.ci(i, j)], cl = eta[g.ci(i, j - 1)];
+    tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dz2;
+    cr = (eta[g.ci(i + 1, j)] + eta[g.ci(i, j)] + eta[g.ci(i + 1, j - 1)] + eta[g.ci(i, j - 1)]) / 4.0;
+    cl = (eta[g.ci(i, j)] + eta[g.ci(i - 1, j)] + eta[g.ci(i, j - 1)] + eta[g.ci(i - 1, j - 1)]) / 4.0;
+    if (i == 0)
+      tmp = tmp + s.dt * cr / s.dr2;
+    else if (i == nx - 1)
+      tmp = tmp + s.dt * cl / s.dr2;
+    else
+      tmp = tmp + s.dt * (cl + cr) / s.dr2;
+  }
+  diag[q] = tmp;
+}
+
+// x_out = b_in / diag (any diagonal preconditioner)
+__global__ __launch_bounds__(kBlock) void k_div_diag(int64_t n, const double* __restrict__ b,
+                                                     const double* __restrict__ d, double* __restrict__ x,
+                                                     const KState* st) {
+  if (!kactive(st)) return;
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < n) x[q] = b[q] / d[q];
+}
+
+// Atob_uv(u, v -> x) TP/matrix_tool.f90:5-26
+__global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __restrict__ u,
+                                                      const double* __restrict__ v, double* __restrict__ x) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= g.nuv()) return;
+  const int nx = g.nx;
+  if (q < g.mn()) {
+    const int k = (int)(q / (nx + 1));
+    const int i = (int)(q - (int64_t)k * (nx + 1));
+    x[q] = u[g.ui(i, k)];
+  } else {
+    const int64_t t = q - g.mn();
+    const int k = (int)(t / nx) + 1;
+    const int i = (int)(t - (int64_t)(k - 1) * nx);
+    x[q] = v[g.vi(i, k)];
+  }
+}
+
+// btoA_uv + the ghost rows of VeloSolve_2_combined TP/VeloSolve_2_combined.f90:426-457.
+// One thread per element of the u field and one per element of the v field.
+__global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const double* __restrict__ x,
+                                                        const double* __restrict__ uL,
+                                                        double* __restrict__ u, double* __restrict__ v) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  const int64_t nu = g.nuf();
+  VeloIn X{x, nx, ny, 0, g.mn()};
+  if (q < nu) {
+    const int k = (int)(q / (nx + 3)) - 1;
+    const int i = (int)(q - (int64_t)(k + 1) * (nx + 3)) - 1;
+    // interior / left-right ghosts for rows 0..ny-1, then wall rows
+    auto side = [&](int ii, int kk) -> double {
+      if (ii >= 0 && ii <= nx) return X.uraw(ii, kk);
+      if (s.lr_d) return ii < 0 ? 2.0 * uL[kk] - X.uraw(1, kk) : 2.0 * uL[kk] - X.uraw(nx - 1, kk);
+      return ii < 0 ? X.uraw(1, kk) : X.uraw(nx - 1, kk);
+    };
+    double val;
+    if (k < 0)
+      val = s.uwB * 2.0 - side(i, 0);
+    else if (k >= ny)
+      val = s.uwT * 2.0 - side(i, ny - 1);
+    else
+      val = side(i, k);
+    u[q] = val;
+  } else if (q < nu + g.nvf()) {
+    const int64_t t = q - nu;
+    const int k = (int)(t / (nx + 2));
+    const int i = (int)(t - (int64_t)k * (nx + 2)) - 1;
+    double val;
+    if (k == 0 || k == ny)
+      val = 0.0;
+    else
+      val = X.V(i, k);
+    v[t] = val;
+  }
+}
+
+// Momentum right-hand side, one thread per unknown.
+// VeloSolve_combined TP/VeloSolve_combined.f90:48-396 (BDF = 1)
+// VeloSolve_2_combined TP/VeloSolve_2_combined.f90:62-400 (BDF = 2)
+template <int BDF>
+__global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs a, double* __restrict__ rhs) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= g.nuv()) return;
+  const int nx = g.nx, ny = g.ny;
+  const double *rho = a.rho, *eta = a.eta, *bu = a.bu, *ph = a.ph_after, *mu = a.rmu_after;
+  const double *u = a.u, *v = a.v, *up = a.u_pre, *vp = a.v_pre;
+  // extrapolated velocities of the convection term (:109-110)
+  auto UT = [&](int i, int k) -> double {
+    return BDF == 2 ? 2.0 * u[g.ui(i, k)] - up[g.ui(i, k)] : u[g.ui(i, k)];
+  };
+  auto VT = [&](int i, int k) -> double {
+    return BDF == 2 ? 2.0 * v[g.vi(i, k)] - vp[g.vi(i, k)] : v[g.vi(i, k)];
+  };
+  auto TP = [&](int i, int k) -> double {
+    const int64_t c = g.ci(i, k);
+    return BDF == 2 ? a.p[c] + 4 * a.psi2[c] / 3.0 - a.psi1[c] / 3.0 : a.p[c];
+  };
+  double f;
+  if (q < g.mn()) {
+    const int k = (int)(q / (nx + 1));
+    const int i = (int)(q - (int64_t)k * (nx + 1));
+    const int64_t c0 = g.ci(i, k), c1 = g.ci(i - 1, k);
+    f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dr;
+    f = f - s.re * (rho[c0] + rho[c1]) / 2.0 *
+                (UT(i, k) * (UT(i + 1, k) - UT(i - 1, k)) / 2.0 / s.dr +
+                 (VT(i, k) + VT(i - 1, k) + VT(i, k + 1) + VT(i - 1, k + 1)) / 4.0 * (UT(i, k + 1) - UT(i, k - 1)) / 2.0 / s.dz);
+    const double rb = (rho[c0] + rho[c1]) / 2.0;
+    double in;
+    if (BDF == 2)
+      in = 2.0 * u[g.ui(i, k)] * s.re / s.dt - up[g.ui(i, k)] * s.re / 2.0 / s.dt + (ph[c0] + ph[c1]) / 2.0 * s.ggx;
+    else
+      in = u[g.ui(i, k)] * s.re / s.dt + s.ggx;
+    f = f - (TP(i, k) - TP(i - 1, k)) / s.dr + rb * in;
+    f = f + (a.pf1[c0] + a.pf1[c1]) / 2.0;
+    f = f * s.dt;
+    if (s.lr_d) {
+      if (i == 0) f = f + 2.0 * a.uL[k] * eta[c0] * 2.0 * s.dt / s.dr2;
+      if (i == nx) f = f + 2.0 * a.uL[k] * eta[c0] * 2.0 * s.dt / s.dr2;
+    }
+    if (k == 0) {
+      const double etay = (eta[g.ci(i, 0)] + eta[g.ci(i - 1, 0)] + eta[g.ci(i, -1)] + eta[g.ci(i - 1, -1)]) / 4.0;
+      f = f + s.dt * (2.0 * s.uwB / s.dz2 * etay);
+    }
+    if (k == ny - 1) {
+      const double etay = (eta[g.ci(i, ny)] + eta[g.ci(i - 1, ny)] + eta[g.ci(i, ny - 1)] + eta[g.ci(i - 1, ny - 1)]) / 4.0;
+      f = f + s.dt * (2.0 * s.uwT / s.dz2 * etay);
+    }
+  } else {
+    const int64_t t = q - g.mn();
+    const int k = (int)(t / nx) + 1;
+    const int i = (int)(t - (int64_t)(k - 1) * nx);
+    const int64_t c0 = g.ci(i, k), c1 = g.ci(i, k - 1);
+    f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dz;
+    f = f - s.re * (rho[c0] + rho[c1]) / 2.0 *
+                ((UT(i, k) + UT(i + 1, k) + UT(i, k - 1) + UT(i + 1, k - 1)) / 4.0 * (VT(i + 1, k) - VT(i - 1, k)) / 2.0 / s.dr +
+                 VT(i, k) * (VT(i, k + 1) - VT(i, k - 1)) / 2.0 / s.dz);
+    const double rb = (rho[c0] + rho[c1]) / 2.0;
+    double in;
+    if (BDF == 2)
+      in = 2.0 * v[g.vi(i, k)] * s.re / s.dt - vp[g.vi(i, k)] * s.re / 2.0 / s.dt +
+           (ph[c0] + ph[g.ci(i - 1, k)]) / 2.0 * s.ggz;
+    else
+      in = v[g.vi(i, k)] * s.re / s.dt + s.ggz;
+    f = f - (TP(i, k) - TP(i, k - 1)) / s.dz + rb * in;
+    f = f + (a.pf2[c0] + a.pf2[c1]) / 2.0;
+    f = f * s.dt;
+  }
+  rhs[q] = f;
+}
+
+// ---- Cahn-Hilliard ------------------------------------------------------
+
+// op_double_gradient_Neum TP/matrix_tool.f90:178-219 at cell (i,j), reading
+// the unknown through `get` (interior values only) and mobility `mo`.
+template <class F>
+__device__ __forceinline__ double op_dgn(const Grid g, double dx2, double dy2, const double* __restrict__ mo,
+                                         int i, int j, F get) {
+  const int nx = g.nx, ny = g.ny;
+  const double m0 = mo[g.ci(i, j)];
+  const double r0 = get(i, j);
+  double moL, moR, rhs = 0.0;
+  if (i == 0) {
+    moL = (m0 + m0) / 2.0;
+    moR = (m0 + mo[g.ci(i + 1, j)]) / 2.0;
+    rhs = rhs + 1.0 / dx2 * (moL * r0 + moR * get(i + 1, j) - (moL + moR) * r0);
+  } else if (i == nx - 1) {
+    moL = (mo[g.ci(i - 1, j)] + m0) / 2.0;
+    moR = (m0 + m0) / 2.0;
+    rhs = rhs + 1.0 / dx2 * (moL * get(i - 1, j) + moR * r0 - (moL + moR) * r0);
+  } else {
+    moL = (mo[g.ci(i - 1, j)] + m0) / 2.0;
+    moR = (m0 + mo[g.ci(i + 1, j)]) / 2.0;
+    rhs = rhs + 1.0 / dx2 * (moL * get(i - 1, j) + moR * get(i + 1, j) - (moL + moR) * r0);
+  }
+  if (j == 0) {
+    moL = (m0 + m0) / 2.0;
+    moR = (m0 + mo[g.ci(i, j + 1)]) / 2.0;
+    rhs = rhs + 1.0 / dy2 * (moL * r0 + moR * get(i, j + 1) - (moL + moR) * r0);
+  } else if (j == ny - 1) {
+    moL = (mo[g.ci(i, j - 1)] + m0) / 2.0;
+    moR = (m0 + m0) / 2.0;
+    rhs = rhs + 1.0 / dy2 * (moL * get(i, j - 1) + moR * r0 - (moL + moR) * r0);
+  } else {
+    moL = (mo[g.ci(i, j - 1)] + m0) / 2.0;
+    moR = (m0 + mo[g.ci(i, j + 1)]) / 2.0;
+    rhs = rhs + 1.0 / dy2 * (moL * get(i, j - 1) + moR * get(i, j + 1) - (moL + moR) * r0);
+  }
+  return rhs;
+}
+
+// MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:212-284 for cell q (both rows).
+__device__ __forceinline__ void phmu_rows(const Grid g, const Scal& s, const double* __restrict__ bu,
+                                          const double* __restrict__ minner, const double* __restrict__ m0,
+                                          const double* __restrict__ x, int64_t q, double& o1, double& o2) {
+  const int nx = g.nx, ny = g.ny;
+  const int64_t C = g.ncell();
+  const int j = (int)(q / nx), i = (int)(q - (int64_t)j * nx);
+  auto F1 = [&](int a, int b) -> double { return x[a + (int64_t)nx * b]; };
+  auto F2 = [&](int a, int b) -> double { return x[C + a + (int64_t)nx * b]; };
+  const double f1 = F1(i, j), f2 = F2(i, j);
+  const double b = bu[g.ci(i, j)];
+  double ff1 = op_dgn(g, s.dr2, s.dz2, minner, i, j, F1);
+  ff1 = s.s * f1 / s.I - s.I * ff1 - f2 * b;
+  // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
+  if (j == 0) ff1 = ff1 + s.I / s.dz2 * (f1 - s.coef_bot * f1);
+  if (j == ny - 1) ff1 = ff1 + s.I / s.dz2 * (f1 - s.coef_top * f1);
+  double ff2 = op_dgn(g, s.dr2, s.dz2, m0, i, j, F2);
+  ff2 = f1 * (b + s.adj) - s.dt * s.xld * ff2;
+  o1 = ff1;
+  o2 = ff2;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const double* __restrict__ bu,
+                                                        const double* __restrict__ minner,
+                                                        const double* __restrict__ m0,
+                                                        const double* __restrict__ x, double* __restrict__ y,
+                                                        double alpha, double beta, double* __restrict__ a0,
+                                                        const double* __restrict__ b, double* part,
+                                                        KState* st) {
+  if (MODE != MV_PLAIN && !kactive(st)) return;
+  const int64_t C = g.ncell();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double acc[2] = {0.0, 0.0};
+  const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
+#pragma unroll
+  for (int jj = 0; jj < kTile / kBlock; ++jj) {
+    const int64_t q = base + jj * kBlock + threadIdx.x;
+    if (q >= C) break;
+    double z1, z2;
+    if (MODE != MV_INIT || xnz) phmu_rows(g, s, bu, minner, m0, x, q, z1, z2);
+    if (MODE == MV_PLAIN) {
+      y[q] = alpha * z1 + beta * y[q];
+      y[C + q] = alpha * z2 + beta * y[C + q];
+    } else if (MODE == MV_INIT) {
+      double r1 = b[q], r2 = b[C + q];
+      if (xnz) {
+        r1 = -1.0 * z1 + 1.0 * r1;
+        r2 = -1.0 * z2 + 1.0 * r2;
+      }
+      y[q] = r1;
+      y[C + q] = r2;
+      a0[q] = r1;
+      a0[C + q] = r2;
+      acc[0] += r1 * r1 + r2 * r2;
+    } else {
+      y[q] = z1;
+      y[C + q] = z2;
+      if (MODE == MV_DOT1) {
+        acc[0] += a0[q] * z1 + a0[C + q] * z2;
+      } else {
+        acc[0] += z1 * a0[q] + z2 * a0[C + q];
+        acc[1] += z1 * z1 + z2 * z2;
+      }
+    }
+  }
+  if (MODE == MV_PLAIN) return;
+  double tot[2];
+  if (MODE == MV_DOT2) {
+    if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+  } else {
+    double a1[1] = {acc[0]}, t1[1];
+    if (grid_sum<1>(a1, part, &st->ticket, t1) && threadIdx.x == 0)
+      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, t1);
+  }
+}
+
+// PSolve_PH_MU_CG TP/ph_mu_solve.f90:319-353 (note the reference's
+// `j == nr-1` / `i == 0` tests, kept as written).
+__global__ __launch_bounds__(kBlock) void k_phmu_jacobi(Grid g, Scal s, const double* __restrict__ b,
+                                                        double* __restrict__ x, const KState* st) {
+  if (!kactive(st)) return;
+  const int64_t C = g.ncell();
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= C) return;
+  const int nx = g.nx, ny = g.ny;
+  const int j = (int)(q / nx), i = (int)(q - (int64_t)j * nx);
+  const double tmpx = (i == 0 || j == nx - 1) ? 1.0 / s.dr2 : 2.0 / s.dr2;
+  const double tmpz = (i == 0 || j == ny - 1) ? 1.0 / s.dz2 : 2.0 / s.dz2;
+  x[q] = b[q] / ((tmpx + tmpz) * s.I + s.s / s.I);
+  x[C + q] = b[C + q] / (s.dt * s.xld * (tmpx + tmpz));
+}
+
+// ch_bulk_energy TP/ch_bulk_energy.f90:4-39 and ch_bdr_condition
+// TP/ch_bdr_condition.f90:43-67
+// phi**3 rounded once from a double-double product, which is what a
+// correctly rounded pow(phi, 3.0) (the host libm the reference calls) gives.
+__device__ __forceinline__ double cube_rn(double x) {
+  const double h = x * x;
+  const double l = __builtin_fma(x, x, -h);
+  const double r = h * x;
+  const double e = __builtin_fma(h, x, -r) + l * x;
+  return r + e;
+}
+__device__ __forceinline__ double ch_bulk(double phi, double psi, int mode) {
+  if (mode == 1) return 0.5 * phi * (1.0 - phi) * (1.0 - 2.0 * phi) * (1 - psi);
+  return (cube_rn(phi) - phi) * (1 - psi);
+}
+__device__ __forceinline__ double ch_bdr(double cf, double phv, int mode) {
+  if (mode == 1) return cf * (1.0 - phv) * phv;
+  return cf * (1.0 - phv * phv);
+}
+
+// ph_M_0 = (1-bdr_ph) f(ph) over the whole field TP/ph_mu_solve.f90:31-35
+__global__ __launch_bounds__(kBlock) void k_phmu_m0(int64_t nc, int mode, const double* __restrict__ ph,
+                                                    const double* __restrict__ bdr, double* __restrict__ m0) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= nc) return;
+  const double p = ph[q];
+  if (mode == 1)
+    m0[q] = (1.0 - bdr[q]) * (p * p) * ((p - 1.0) * (p - 1.0));
+  else
+    m0[q] = (1.0 - bdr[q]) * ((p * p - 1.0) * (p * p - 1.0));
+}
+
+// Right-hand side of ph_mu_solve TP/ph_mu_solve.f90:40-153 and the
+// initial guess (Atob_ph_mu of ph, rmu :152-153).  pn[0..nx) = pnb,
+// pn[nx..2nx) = pnt for the ghost rows written after the solve.
+__global__ __launch_bounds__(kBlock) void k_phmu_rhs(Grid g, Scal s, PhmuRhsArgs a, double* __restrict__ rhs,
+                                                     double* __restrict__ x0, double* __restrict__ pn) {
+  const int64_t C = g.ncell();
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= C) return;
+  const int nx = g.nx, ny = g.ny;
+  const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
+  const double *ph = a.ph, *bu = a.bu, *u = a.u, *v = a.v;
+  const int64_t c = g.ci(i, k);
+  double f2 = ph[c] * (bu[c] + s.adj);
+  f2 = f2 - s.dt * (u[g.ui(i + 1, k)] + u[g.ui(i, k)]) / 2.0 *
+                (ph[g.ci(i + 1, k)] * bu[g.ci(i + 1, k)] - ph[g.ci(i - 1, k)] * bu[g.ci(i - 1, k)]) / 2.0 / s.dr -
+       s.dt * (v[g.vi(i, k + 1)] + v[g.vi(i, k)]) / 2.0 *
+           (ph[g.ci(i, k + 1)] * bu[g.ci(i, k + 1)] - ph[g.ci(i, k - 1)] * bu[g.ci(i, k - 1)]) / 2.0 / s.dz;
+  const double phv = ph[c];
+  double f1;
+  if (s.phi_mode == 1)
+    f1 = s.s * phv / s.I - ch_bulk(phv, a.bdr[c], 1) / s.I - phv * (phv - 1.0) * a.bg[c] * a.cosa / a.sqrt2;
+  else
+    f1 = s.s * phv / s.I - ch_bulk(phv, a.bdr[c], 2) / s.I - (phv * phv - 1.0) * a.bg[c] * a.cosa / a.sqrt2;
+  const double tmp = s.I / s.dz;
+  if (k == 0) {
+    const double pv = (ph[g.ci(i, -1)] + ph[g.ci(i, 0)]) / 2.0;
+    double pnb = ch_bdr(a.cfB[i], pv, s.phi_mode);
+    pnb = pnb / tmp;
+    f1 = f1 + pnb * s.I / s.dz2;
+    pn[i] = pnb;
+  }
+  if (k == ny - 1) {
+    const double pv = (ph[g.ci(i, ny - 1)] + ph[g.ci(i, ny)]) / 2.0;
+    double pnt = ch_bdr(a.cfT[i], pv, s.phi_mode);
+    pnt = pnt / tmp;
+    f1 = f1 + pnt * s.I / s.dz2;
+    pn[nx + i] = pnt;
+  }
+  rhs[q] = f1;
+  rhs[C + q] = f2;
+  x0[q] = phv;
+  x0[C + q] = a.rmu[c];
+}
+
+// btoA_ph_mu TP/matrix_tool.f90:104-136 and the ph ghost rows of
+// ph_mu_solve TP/ph_mu_solve.f90:173-177.  The corner ghosts keep the value
+// btoA_ph_mu gave them (copied before the ghost-row override).
+__global__ __launch_bounds__(kBlock) void k_phmu_unpack(Grid g, Scal s, const double* __restrict__ x,
+                                                        const double* __restrict__ pn, double* __restrict__ ph,
+                                                        double* __restrict__ mu) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  if (q >= g.nc()) return;
+  const int k = (int)(q / (nx + 2)) - 1;
+  const int i = (int)(q - (int64_t)(k + 1) * (nx + 2)) - 1;
+  const int64_t C = g.ncell();
+  const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+  const int kc = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
+  const double p = x[ic + (int64_t)nx * kc], m = x[C + ic + (int64_t)nx * kc];
+  double pv = p;
+  if (i >= 0 && i < nx) {
+    if (k < 0) pv = p * s.coef_bot + pn[i];
+    if (k >= ny) pv = p * s.coef_top + pn[nx + i];
+  }
+  ph[q] = pv;
+  mu[q] = m;
+}
+
+// ---- host launchers -----------------------------------------------------
+static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
+                        const double* x, double* y, double alpha, double beta, double* a0,
+                        const double* b, double* part, KState* ks) {
+  dim3 grid(nblk(g.nuv(), kTile));
+#define VM(B, M)                                                                                     \
+  hipLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, y, \
+                     alpha, beta, a0, b, part, ks)
+  if (bdf == 1) {
+    switch (mode) {
+      case MV_PLAIN: VM(1, MV_PLAIN); break;
+      case MV_DOT1: VM(1, MV_DOT1); break;
+      case MV_DOT2: VM(1, MV_DOT2); break;
+      case MV_INIT: VM(1, MV_INIT); break;
+    }
+  } else {
+    switch (mode) {
+      case MV_PLAIN: VM(2, MV_PLAIN); break;
+      case MV_DOT1: VM(2, MV_DOT1); break;
+      case MV_DOT2: VM(2, MV_DOT2); break;
+      case MV_INIT: VM(2, MV_INIT); break;
+    }
+  }
+#undef VM
+}
+
+void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
+                        const double* x, double* y, double alpha, double beta, double* a0,
+                        const double* b, double* part, KState* ks) {
+  dim3 grid(nblk(g.ncell(), kTile));
+#define PM(M)                                                                                          \
+  hipLaunchKernelGGL((k_phmu_matvec<M>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, x, y, alpha, \
+                     beta, a0, b, part, ks)
+  switch (mode) {
+    case MV_PLAIN: PM(MV_PLAIN); break;
+    case MV_DOT1: PM(MV_DOT1); break;
+    case MV_DOT2: PM(MV_DOT2); break;
+    case MV_INIT: PM(MV_INIT); break;
+  }
+#undef PM
+}
+
+void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag) {
+  hipLaunchKernelGGL(k_velo_jacobi_diag, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, c.rho,
+                     c.eta, diag);
+}
+
+void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,
+                     const KState* ks) {
+  hipLaunchKernelGGL(k_div_diag, dim3(nblk(n, kBlock)), dim3(kBlock), 0, st, n, b, d, x, ks);
+}
+
+void launch_phmu_jacobi(hipStream_t st, const Grid& g, const Scal& s, const double* b, double* x,
+                        const KState* ks) {
+  hipLaunchKernelGGL(k_phmu_jacobi, dim3(nblk(g.ncell(), kBlock)), dim3(kBlock), 0, st, g, s, b, x, ks);
+}
+
+void launch_velo_pack(hipStream_t st, const Grid& g, const double* u, const double* v, double* x) {
+  hipLaunchKernelGGL(k_velo_pack, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, u, v, x);
+}
+
+void launch_velo_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* uL,
+                        double* u, double* v) {
+  hipLaunchKernelGGL(k_velo_unpack, dim3(nblk(g.nuf() + g.nvf(), kBlock)), dim3(kBlock), 0, st, g, s, x,
+                     uL, u, v);
+}
+
+void launch_velo_rhs(hipStream_t st, int bdf, const Grid& g, const Scal& s, const VeloRhsArgs& a,
+                     double* rhs) {
+  if (bdf == 1)
+    hipLaunchKernelGGL(k_velo_rhs<1>, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs);
+  else
+    hipLaunchKernelGGL(k_velo_rhs<2>, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs);
+}
+
+void launch_phmu_m0(hipStream_t st, const Grid& g, int mode, const double* ph, const double* bdr,
+                    double* m0) {
+  hipLaunchKernelGGL(k_phmu_m0, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g.nc(), mode, ph, bdr, m0);
+}
+
+void launch_phmu_rhs(hipStream_t st, const Grid& g, const Scal& s, const PhmuRhsArgs& a, double* rhs,
+                     double* x0, double* pn) {
+  hipLaunchKernelGGL(k_phmu_rhs, dim3(nblk(g.ncell(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs, x0, pn);
+}
+
+void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* pn,
+                        double* ph, double* mu) {
+  hipLaunchKernelGGL(k_phmu_unpack, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g, s, x, pn, ph, mu);
+}
+
+}  // namespace drd

@@ -1,0 +1,219 @@
+"""Seeded synthetic inputs for the solve path (droplets, solid masks, coefficients).
+
+The reference builds its geometry in `finit` (TP/finit.f90:4-628) by
+commenting blocks in and out and fixes the grid at compile time.  These
+generators produce the same kinds of fields at run time, deterministically
+from a seed (SplitMix64 -> uniform doubles), and derive the secondary fields
+the way TP/finit.f90:540-588 does:
+
+  bdr_ph_grad = |grad bdr_ph| (central differences)        TP/finit.f90:544-549
+  ph_M_inner  = 1 - bdr_ph, bdr_ph_used = 1 - bdr_ph      TP/finit.f90:569-571
+  ph          = (1 - bdr_ph) * phi                         TP/finit.f90:451
+  rmu         = -eps*lap(ph) - ph/eps + ph^3/eps           TP/finit.f90:474-480
+  angleB = angleT = 90 in a complex domain                 TP/finit.f90:586-588
+
+Viscosity follows the phase field, eta = 1 + (lambda_eta - 1)(1 + phi)/2
+(the two-phase form of TP/interpolate_coef.f90:92-112), rho = 1.
+Arrays use the abi layout: a[k+1, i+1] is the Fortran element (i, k).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014); uniform() in [0, 1) with 53 bits."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+
+@dataclass
+class Config:
+    name: str
+    params: abi.Params
+    fields: dict = field(default_factory=dict)  # field id -> ndarray
+    steps: int = 1
+    notes: str = ""
+
+
+def _cell_centres(p: abi.Params):
+    i = np.arange(-1, p.nx + 1)
+    k = np.arange(-1, p.ny + 1)
+    x = (i + 0.5) * p.dx
+    z = (k + 0.5) * p.dz
+    return np.meshgrid(x, z)  # shapes (ny+2, nx+2)
+
+
+def _neumann_ghosts(a: np.ndarray) -> np.ndarray:
+    a[:, 0] = a[:, 1]
+    a[:, -1] = a[:, -2]
+    a[0, :] = a[1, :]
+    a[-1, :] = a[-2, :]
+    return a
+
+
+def droplet_phase(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: float, eps: float):
+    """phi = max_d tanh((r_d - |x - c_d|)/(sqrt2 eps)), centres uniform in the domain."""
+    X, Z = _cell_centres(p)
+    L, H = p.nx * p.dx, p.ny * p.dz
+    phi = -np.ones_like(X)
+    for _ in range(n):
+        cx, cz = rng.uniform(0, L), rng.uniform(0, H)
+        r = rng.uniform(rmin, rmax)
+        d = r - np.hypot(X - cx, Z - cz)
+        phi = np.maximum(phi, np.tanh(d / (math.sqrt(2.0) * eps)))
+    return phi
+
+
+def disc_mask(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: float, width: float):
+    """bdr_ph = 1/2 (1 + tanh(-d_s/(sqrt2 w))) for the union of n seeded discs."""
+    X, Z = _cell_centres(p)
+    L, H = p.nx * p.dx, p.ny * p.dz
+    ds = np.full_like(X, np.inf)
+    for _ in range(n):
+        cx, cz = rng.uniform(0, L), rng.uniform(0, H)
+        r = rng.uniform(rmin, rmax)
+        ds = np.minimum(ds, np.hypot(X - cx, Z - cz) - r)
+    return 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width)))
+
+
+def serpentine_mask(p: abi.Params, amp: float, waves: float, half_width: float, width: float):
+    """Solid outside the channel |z - (0.5 + amp sin(waves pi x))| < half_width."""
+    X, Z = _cell_centres(p)
+    zc = 0.5 + amp * np.sin(waves * math.pi * X)
+    ds = half_width - np.abs(Z - zc)  # > 0 inside the channel
+    return 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width)))
+
+
+def derived_fields(p: abi.Params, phi: np.ndarray, bdr: np.ndarray, lambda_eta: float,
+                   sharp_penalty: bool = False) -> dict:
+    """Secondary fields exactly as TP/finit.f90 derives them from bdr_ph and phi."""
+    bdr = _neumann_ghosts(bdr.copy())
+    ph = _neumann_ghosts((1.0 - bdr) * phi)
+    eps = p.I_thickness
+    dr2, dz2 = p.dx * p.dx, p.dz * p.dz
+    rmu = np.zeros_like(ph)
+    c = ph[1:-1, 1:-1]
+    lap = (ph[1:-1, 2:] + ph[1:-1, :-2] - 2 * c) / dr2 + (ph[2:, 1:-1] + ph[:-2, 1:-1] - 2 * c) / dz2
+    rmu[1:-1, 1:-1] = -eps * lap - c / eps + c ** 3 / eps
+    rmu = _neumann_ghosts(rmu)
+    grad = np.zeros_like(bdr)
+    gx = (bdr[1:-1, 2:] - bdr[1:-1, :-2]) / 2.0 / p.dx
+    gz = (bdr[2:, 1:-1] - bdr[:-2, 1:-1]) / 2.0 / p.dz
+    grad[1:-1, 1:-1] = np.sqrt(gx ** 2 + gz ** 2)
+    # viscosity of the phase field (phi in [-1,1]); solids keep eta of the fluid
+    eta = 1.0 + (lambda_eta - 1.0) * (1.0 + ph) / 2.0
+    rho = np.ones_like(ph)
+    chi = (bdr > 0.5).astype(np.float64) if sharp_penalty else bdr.copy()
+    ang = np.full(p.nx + 1, 90.0)
+    nu = abi.field_shape(p, abi.F_U)
+    nv = abi.field_shape(p, abi.F_V)
+    return {
+        abi.F_U: np.zeros(nu), abi.F_V: np.zeros(nv), abi.F_P: np.zeros_like(ph),
+        abi.F_PH: ph, abi.F_RMU: rmu, abi.F_RHO: rho, abi.F_ETA: _neumann_ghosts(eta),
+        abi.F_BDR_PH: bdr, abi.F_BDR_PH_USED: 1.0 - bdr, abi.F_BDR_PH_PENALTY: chi,
+        abi.F_BDR_PH_GRAD: grad, abi.F_PH_M_INNER: 1.0 - bdr,
+        abi.F_PH_M_0: 1.0 - bdr, abi.F_ANGLEB: ang, abi.F_ANGLET: ang.copy(),
+        abi.F_U_PRE: np.zeros(nu), abi.F_V_PRE: np.zeros(nv), abi.F_PH_PRE: ph.copy(),
+        abi.F_RMU_PRE: rmu.copy(), abi.F_PSI1: np.zeros_like(ph), abi.F_PSI2: np.zeros_like(ph),
+    }
+
+
+def two_phase_config(nx: int, ny: int, seed: int, *, n_drops: int, r_drop: tuple,
+                     n_discs: int = 0, r_disc: tuple = (10, 16), serpentine: bool = False,
+                     lambda_eta: float = 10.0, penalty: float = 1e4, dt: float = 1e-4,
+                     tol: float = 1e-10, steps: int = 10, name: str = "custom") -> Config:
+    """Two-phase CH + NS on an nx x ny grid of height 1 (dx = dz = 1/ny)."""
+    p = abi.default_params(nx, ny)
+    p.dx = p.dz = 1.0 / ny
+    p.dt = dt
+    p.re, p.ca, p.xld = 1.0, 1.0, 1.0e-3
+    p.s_implicit = 1.5
+    p.I_thickness = 1.5 * p.dx
+    p.ggx, p.ggz = 1.0, 0.0
+    p.tol = tol
+    p.penalty_k = penalty
+    p.angle = 90.0
+    rng = SplitMix64(seed)
+    eps = p.I_thickness
+    phi = droplet_phase(p, rng, n_drops, r_drop[0] * p.dx, r_drop[1] * p.dx, eps)
+    if serpentine:
+        bdr = serpentine_mask(p, 0.3, 8.0, 0.125, 2 * p.dx)
+    else:
+        bdr = disc_mask(p, rng, n_discs, r_disc[0] * p.dx, r_disc[1] * p.dx, 2 * p.dx) if n_discs \
+            else np.zeros_like(phi)
+    cfg = Config(name, p, derived_fields(p, phi, bdr, lambda_eta), steps)
+    return cfg
+
+
+def tp_case(nx: int, ny: int | None = None, *, lambda_eta: float = 1.0, steps: int = 10,
+            tol: float = 1e-8) -> Config:
+    """The reference's shipped TP run (TP/input, TP/finit.f90 geometry type 4)
+    on an nx x ny grid of the unit square: half-plane solid below z = 0.2
+    (smoothed over I_thickness/2, sharp penalty mask), one droplet of radius
+    0.2 resting on it, xld = 0.05, re = 3, ca = 12, eps = 0.01 * 128/nx,
+    s = 1.5, dt = 5e-5, contact angle 60 (90 in the complex domain)."""
+    ny = nx if ny is None else ny
+    p = abi.default_params(nx, ny)
+    p.dx, p.dz = 1.0 / nx, 1.0 / ny
+    p.xld, p.re, p.ca, p.s_implicit, p.tol = 0.05, 3.0, 12.0, 1.5, tol
+    p.I_thickness = 0.01 * 128.0 / max(nx, ny)
+    p.dt = 5e-5 * (128.0 / max(nx, ny)) ** 2
+    p.angle = 60.0
+    p.penalty_k = 1e8
+    X, Z = _cell_centres(p)
+    eps_s = p.I_thickness / 2.0
+    bdr = (np.tanh(-(Z - 0.2) / math.sqrt(2.0) / eps_s) + 1.0) / 2.0
+    phi = np.tanh(-(np.sqrt((X - 0.5) ** 2 + (Z - 0.4) ** 2) - 0.2) / (math.sqrt(2.0) * p.I_thickness))
+    return Config(f"tp{nx}x{ny}", p, derived_fields(p, phi, bdr, lambda_eta, sharp_penalty=True), steps,
+                  "reference TP case")
+
+
+def config(name: str) -> Config:
+    """Named workloads.
+
+    c0     256 x 128, 6 droplets r in [8,20] cells, 3 discs r in [10,16],
+           eta ratio 10, R = 1e4, dt = 1e-4, tol 1e-10 (Neumann-x twin of the
+           periodic case: the reference's TP path has no periodic x).
+    c1     2048 x 1024 serpentine channel (0.5 + 0.3 sin(8 pi x), width 0.25),
+           40 droplets r in [10,40], eta ratio 100, R = 1e6, dt = 5e-5, 20 steps.
+    small  64 x 32 version of c0 for parity tests.
+    tiny   32 x 16.
+    """
+    if name == "c0":
+        return two_phase_config(256, 128, 1000, n_drops=6, r_drop=(8, 20), n_discs=3,
+                                lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=50, name=name)
+    if name == "c1":
+        return two_phase_config(2048, 1024, 1001, n_drops=40, r_drop=(10, 40), serpentine=True,
+                                lambda_eta=100.0, penalty=1e6, dt=5e-5, steps=20, name=name)
+    if name == "small":
+        return two_phase_config(64, 32, 7, n_drops=3, r_drop=(3, 6), n_discs=2, r_disc=(3, 5),
+                                lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=10, name=name)
+    if name == "tiny":
+        return two_phase_config(32, 16, 3, n_drops=2, r_drop=(2, 4), n_discs=1, r_disc=(2, 3),
+                                lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=10, name=name)
+    raise KeyError(name)
+
+
+def load(solver, cfg: Config) -> None:
+    """Pushes every field of cfg into a solver (GpuSolver or the oracle)."""
+    for fid, a in cfg.fields.items():
+        solver.set_field(fid, a)

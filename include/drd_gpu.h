@@ -1,0 +1,229 @@
+/*
+ * drd_gpu.h -- C-ABI of the MI355X (gfx950) implicit-solve path of DRD-Micro.
+ *
+ * One process, one GPU.  Every entry point returns 0 on success and a
+ * negative DRD_E* code on failure (drd_last_error() holds the message).
+ * Arrays cross this boundary as plain pointers to contiguous FP64 data in
+ * Fortran (column-major, x fastest) order with the reference's ghost layers:
+ *
+ *   cell field  f(-1:nx, -1:ny)    (nx+2)*(ny+2) doubles   TP/global.f90:56-59
+ *   u field     u(-1:nx+1, -1:ny)  (nx+3)*(ny+2) doubles   TP/global.f90:54
+ *   v field     v(-1:nx, 0:ny)     (nx+2)*(ny+1) doubles   TP/global.f90:54
+ *   velocity Krylov vector  Nuv = (nx+1)*ny + nx*(ny-1)    TP/matrix_tool.f90:5-26
+ *   phi/mu Krylov vector    2*nx*ny                        TP/matrix_tool.f90:82-98
+ *   mu Krylov vector        nx*ny                          TP/matrix_tool.f90:142-155
+ *
+ * nx, ny are the reference's nr, nz (TP/global.f90:15-16), which the
+ * reference fixes at compile time; here they are run-time fields of
+ * drd_params.
+ */
+#ifndef DRD_GPU_H
+#define DRD_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRD_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------- */
+#define DRD_OK 0
+#define DRD_E_ARG (-1)     /* bad argument or size                     */
+#define DRD_E_HIP (-2)     /* HIP runtime error                        */
+#define DRD_E_STATE (-3)   /* call out of order (e.g. ctx not set up)  */
+#define DRD_E_UNSUPPORTED (-4)
+
+/* ---- model variants -------------------------------------------------- */
+/* TP: two_phase_flow_complex_domain; MF: microfluidics_complex_domain.     */
+#define DRD_VARIANT_TP 0
+#define DRD_VARIANT_MF 1
+
+/* ---- preconditioner choices for the velocity solve -------------------- */
+#define DRD_PREC_REFERENCE 0 /* what the reference uses: identity for BDF2,
+                                Jacobi for BDF1 (TP), Jacobi (MF)         */
+#define DRD_PREC_IDENTITY 1  /* TP/VeloSolve_2_combined.f90:779-780       */
+#define DRD_PREC_JACOBI 2    /* TP/VeloSolve_combined.f90:679-741         */
+#define DRD_PREC_DCT32 3     /* FP32 DCT/DST fast-Poisson preconditioner
+                                with diagonal scaling (no reference
+                                counterpart; used inside FGMRES)          */
+
+/* ---- run-time copy of the reference's module `global` scalars -------- */
+typedef struct drd_params {
+  int32_t nx, ny;          /* nr, nz                        TP/global.f90:15-16 */
+  int32_t variant;         /* DRD_VARIANT_*                                      */
+  int32_t phi_mode;        /* 1: phi in [0,1]; 2: phi in [-1,1] TP/mcl6.f90     */
+  int32_t lr_bd_type;      /* 'n' (Neumann) or 'd' (Dirichlet inflow)            */
+  int32_t velo_prec;       /* DRD_PREC_*                                         */
+  int32_t maxit_velo1;     /* 800   TP/VeloSolve_combined.f90:417                */
+  int32_t maxit_velo2;     /* 2000  TP/VeloSolve_2_combined.f90:413              */
+  int32_t maxit_phmu;      /* 5000  TP/ph_mu_solve.f90:156                       */
+  int32_t maxit_ph;        /* 500   MF/PhSolve.f90:72                            */
+  double dx, dz;           /* dr, dz                                             */
+  double dt, re, ca, xld;  /* time step, Reynolds, capillary, mobility           */
+  double s_implicit;       /* stabilisation s                                   */
+  double I_thickness;      /* interface thickness epsilon                       */
+  double ggx, ggz;         /* body force                                         */
+  double uwT, uwB;         /* wall speeds                                        */
+  double xlsB;             /* slip length; must be < 1e-8 (no-slip branch)       */
+  double tol;              /* Krylov tolerance                                   */
+  double penalty_k;        /* diffuse-resistance coefficient TP/global.f90:108   */
+  double angle;            /* static contact angle (degrees)                     */
+  double lambda_rho;       /* density ratio; gamma1 = min(1, lambda_rho) TP/mcl6.f90 */
+  double bdr_ph_used_adjust; /* 1e-6  TP/finit.f90:572                           */
+  double twall, UU, bdr_upp, bdr_low; /* inflow profile ('d' only)              */
+} drd_params;
+
+/* fills p with the reference's TP defaults for an nx x ny grid            */
+void drd_params_default(drd_params* p, int32_t nx, int32_t ny);
+
+/* ---- per-solve report (INFO/ITER/RESID of LIB/BiCGSTAB.f:15-16) ------ */
+typedef struct drd_solve_info {
+  int32_t iter;      /* ITER out                                         */
+  int32_t info;      /* INFO out: 0 ok, 1 maxit, -10/-11 breakdown        */
+  double resid;      /* RESID out (recursive residual / ||b||)            */
+  double true_resid; /* ||b - A x|| / ||b|| recomputed in FP64 (addition) */
+} drd_solve_info;
+
+typedef struct drd_step_info {
+  drd_solve_info phmu;  /* ph_mu_solve                                   */
+  drd_solve_info velo;  /* VeloSolve_(2_)combined                        */
+  double div_before;    /* mean divergence printed by PreSolve_2:23       */
+  double div_after_max; /* max |div u| after projection check (addition)  */
+} drd_step_info;
+
+typedef struct drd_ctx drd_ctx;
+
+/* ---- field identifiers for module-`global` state ---------------------- */
+enum drd_field {
+  DRD_F_U = 0,          /* u(-1:nx+1,-1:ny)   */
+  DRD_F_V = 1,          /* v(-1:nx,0:ny)      */
+  DRD_F_P = 2,          /* cell fields ...    */
+  DRD_F_PH = 3,
+  DRD_F_RMU = 4,
+  DRD_F_RHO = 5,
+  DRD_F_ETA = 6,
+  DRD_F_BDR_PH = 7,
+  DRD_F_BDR_PH_USED = 8,
+  DRD_F_BDR_PH_PENALTY = 9,
+  DRD_F_BDR_PH_GRAD = 10,
+  DRD_F_PH_M_INNER = 11,
+  DRD_F_PH_M_0 = 12,
+  DRD_F_ANGLEB = 13,    /* angleB(0:nx)       */
+  DRD_F_ANGLET = 14,    /* angleT(0:nx)       */
+  /* driver state kept on the device by drd_step (TP/mcl6.f90:21-30)     */
+  DRD_F_U_PRE = 15,
+  DRD_F_V_PRE = 16,
+  DRD_F_PH_PRE = 17,
+  DRD_F_RMU_PRE = 18,
+  DRD_F_PSI1 = 19,
+  DRD_F_PSI2 = 20,
+  DRD_F_COUNT = 21
+};
+
+/* number of doubles of a field for params p (0 for an unknown id)        */
+int64_t drd_field_size(const drd_params* p, int32_t field);
+
+/* ---- lifecycle --------------------------------------------------------
+ * drd_init replaces PreSolve_init (TP/PreSolve_init.f90:1-16, FFTW plans)
+ * and setup (TP/setup.f90:1-267, eigenvalue tables): it creates the device
+ * context, DCT twiddles and Pre_Lcc / Ph_Lcc tables for prm.           */
+int drd_init(const drd_params* prm, drd_ctx** out);
+int drd_finalize(drd_ctx* ctx);
+const char* drd_last_error(void);
+int drd_abi_version(void);
+
+/* copy one module-`global` field host <-> device                         */
+int drd_set_field(drd_ctx* ctx, int32_t field, const double* host);
+int drd_get_field(drd_ctx* ctx, int32_t field, double* host);
+
+/* ---- the Fortran subroutines, same dummy arguments ---------------------
+ * Hidden inputs come from the context (drd_set_field); outputs are host
+ * arrays.  Each call uploads its array arguments, runs on the GPU and
+ * downloads its outputs before returning (synchronous, like the Fortran). */
+
+/* TP/VeloSolve_combined.f90:5-18 (BDF1 momentum predictor)               */
+int drd_velosolve_combined(drd_ctx* ctx, const double* psi, double* u_after,
+                           double* v_after, const double* rmu_after,
+                           const double* ph_after, double t,
+                           const double* particle_force1,
+                           const double* particle_force2,
+                           drd_solve_info* info);
+
+/* TP/VeloSolve_2_combined.f90:4-20 (BDF2 momentum predictor)             */
+int drd_velosolve_2_combined(drd_ctx* ctx, const double* psi1,
+                             const double* psi2, const double* u_pre,
+                             double* u_after, const double* v_pre,
+                             double* v_after, const double* ph_pre,
+                             const double* ph_after, const double* rmu_pre,
+                             const double* rmu_after, double t,
+                             const double* particle_force1,
+                             const double* particle_force2,
+                             drd_solve_info* info);
+
+/* TP/PreSolve_2.f90:1-130: psi from div(u,v) of the context's u, v.      */
+int drd_presolve_2(drd_ctx* ctx, double* psi, double coef,
+                   double* mean_div);
+
+/* TP/ph_mu_solve.f90:3-197 (coupled phi/mu Cahn-Hilliard)                */
+int drd_ph_mu_solve(drd_ctx* ctx, double* ph_after, double* rmu_after,
+                    double t, drd_solve_info* info);
+
+/* MF/PhSolve.f90:1-126 (mu-only Cahn-Hilliard, DCT-I preconditioner)    */
+int drd_phsolve(drd_ctx* ctx, double* rmu_after, double* ph_after, double t,
+                drd_solve_info* info);
+
+/* ---- the MATVEC / PSOLVE callbacks (y := alpha*A*x + beta*y) ---------- */
+#define DRD_OP_VELO1 0 /* MatrixVector_uv_CG   TP/VeloSolve_combined.f90:537-675   */
+#define DRD_OP_VELO2 1 /* MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:536-676 */
+#define DRD_OP_PHMU 2  /* MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:203-299        */
+#define DRD_OP_PH 3    /* MATRIX_VECTOR_CG     MF/PhSolve.f90:240-292              */
+int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
+               double beta, double* y);
+
+#define DRD_PS_VELO1_JACOBI 0 /* PSolve_uv_CG   TP/VeloSolve_combined.f90:679-741   */
+#define DRD_PS_IDENTITY 1     /* PSolve_uv_2_CG TP/VeloSolve_2_combined.f90:779-780 */
+#define DRD_PS_PHMU_JACOBI 2  /* PSolve_PH_MU_CG TP/ph_mu_solve.f90:304-356         */
+#define DRD_PS_PH_DCT 3       /* PSolve_CG      MF/PhSolve.f90:296-313              */
+#define DRD_PS_VELO_DCT32 4   /* FP32 DCT/DST velocity preconditioner (addition)    */
+int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
+               const double* b_in);
+
+/* BICGSTAB(N,B,X,WORK,LDW,ITER,RESID,MATVEC,PSOLVE,INFO) LIB/BiCGSTAB.f:2-3
+ * with the MATVEC/PSOLVE callbacks chosen by id.  *iter in: max
+ * iterations, out: iterations; *resid in: tol, out: measure.             */
+int drd_bicgstab(drd_ctx* ctx, int32_t op, int32_t prec, int32_t n,
+                 const double* b, double* x, int32_t* iter, double* resid,
+                 int32_t* info);
+
+/* FGMRES(m) with right preconditioning (Saad 1993); no reference code
+ * (LIB/fgmres.f wraps a closed library).  Same argument meaning as
+ * drd_bicgstab; restart = m.                                               */
+int drd_fgmres(drd_ctx* ctx, int32_t op, int32_t prec, int32_t n,
+               int32_t restart, const double* b, double* x, int32_t* iter,
+               double* resid, int32_t* info);
+
+/* ---- device-resident time stepping (TP/mcl6.f90:395-696 sequence) ----
+ * bdf=1: first step (ph_mu_solve, VeloSolve_combined, PreSolve_2 with
+ * coef re*gamma/dt); bdf=2: later steps.  State lives on the device
+ * (drd_set_field/drd_get_field move it).                                   */
+int drd_step(drd_ctx* ctx, int32_t bdf, double t, drd_step_info* info);
+int drd_synchronize(drd_ctx* ctx);
+
+/* ---- measurement hook (no reference counterpart) -----------------------
+ * Launches one kernel of the solve path `reps` times on the context's
+ * stream between HIP events and returns the mean launch duration and the
+ * kernel's algorithmic HBM bytes per launch (every input read once, every
+ * output written once).                                                    */
+#define DRD_K_VELO_MATVEC 0 /* fused v = A p, (rtld, v); BDF2 operator     */
+#define DRD_K_PHMU_MATVEC 1 /* fused CH block matvec + dot                 */
+#define DRD_K_BICG_XR 2     /* x,r update + two dots                       */
+#define DRD_K_PRESSURE 3    /* whole pressure projection                   */
+int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
+                     double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DRD_GPU_H */

@@ -1,0 +1,1326 @@
+/*
+ * drd_oracle.c -- CPU oracle for the DRD-Micro implicit-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker: tests/, the smoke test
+ * and bench.py's cpu_baseline leg load it; the product never does.
+ *
+ * Plain C, FP64, serial.  Each function restates one reference routine and
+ * cites it (paths relative to the reference checkout; TP/ is
+ * two_phase_flow_complex_domain/, MF/ microfluidics_complex_domain/, LIB/
+ * lib/).  Floating-point expressions keep the reference's evaluation order
+ * (Fortran evaluates a chain of equal-precedence operators left to right),
+ * so with -ffp-contract=off the stencil routines reproduce the Fortran
+ * results bit for bit; reductions are plain sequential sums.
+ *
+ * How each part is pinned is listed in tests/ (see the test module headers):
+ * operators and preconditioners against the reference Fortran compiled from
+ * its own sources (oracle/_ref), transforms against their published
+ * definitions and scipy, BiCGSTAB against dense LU solves and the
+ * reference's documented control flow, right-hand sides against closed
+ * forms of isolated terms.
+ */
+#include "drd_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* pi as the module parameter of TP/global.f90:39 (rounded to double)      */
+static const double PI_G = 3.14159265358979323846264338327950288419716939937510;
+
+struct oc_ctx {
+  drd_params p;
+  double* f[DRD_F_COUNT];
+  double* pre_lcc; /* (0:nx,0:ny)       TP/setup.f90:122-131 */
+  double* ph_lcc;  /* (0:nx-1,0:ny-1)   TP/setup.f90:149-155 */
+  double coef_bot, coef_top; /* hidden outputs of ph_mu_solve     */
+  double* uL;      /* (0:ny) hidden inflow profile TP/VeloSolve_2_combined.f90:289 */
+  double* uR;
+  /* scratch */
+  double *w1, *w2, *w3, *w4, *w5, *w6;
+  double* work;    /* Krylov WORK(N,7) */
+  int64_t nmax;
+};
+
+/* ---- index helpers -------------------------------------------------- */
+#define NX (c->p.nx)
+#define NY (c->p.ny)
+/* cell field (-1:nx,-1:ny) */
+#define CI(i, k) ((size_t)((i) + 1) + (size_t)(NX + 2) * (size_t)((k) + 1))
+/* u field (-1:nx+1,-1:ny) */
+#define UI(i, k) ((size_t)((i) + 1) + (size_t)(NX + 3) * (size_t)((k) + 1))
+/* v field (-1:nx,0:ny) */
+#define VI(i, k) ((size_t)((i) + 1) + (size_t)(NX + 2) * (size_t)(k))
+
+static int64_t ncell(const drd_params* p) { return (int64_t)(p->nx + 2) * (p->ny + 2); }
+static int64_t nufield(const drd_params* p) { return (int64_t)(p->nx + 3) * (p->ny + 2); }
+static int64_t nvfield(const drd_params* p) { return (int64_t)(p->nx + 2) * (p->ny + 1); }
+static int64_t nuv(const drd_params* p) {
+  return (int64_t)(p->nx + 1) * p->ny + (int64_t)p->nx * (p->ny - 1);
+}
+
+int64_t oc_field_size(const drd_params* p, int32_t f) {
+  switch (f) {
+    case DRD_F_U:
+    case DRD_F_U_PRE:
+      return nufield(p);
+    case DRD_F_V:
+    case DRD_F_V_PRE:
+      return nvfield(p);
+    case DRD_F_ANGLEB:
+    case DRD_F_ANGLET:
+      return p->nx + 1;
+    default:
+      if (f >= 0 && f < DRD_F_COUNT) return ncell(p);
+      return 0;
+  }
+}
+
+void oc_params_default(drd_params* p, int32_t nx, int32_t ny) {
+  memset(p, 0, sizeof(*p));
+  /* TP/input and TP/mcl6.f90 / TP/finit.f90 defaults */
+  p->nx = nx;
+  p->ny = ny;
+  p->variant = DRD_VARIANT_TP;
+  p->phi_mode = 2;
+  p->lr_bd_type = 'n';
+  p->velo_prec = DRD_PREC_REFERENCE;
+  p->maxit_velo1 = 800;
+  p->maxit_velo2 = 2000;
+  p->maxit_phmu = 5000;
+  p->maxit_ph = 500;
+  p->dx = 1.0 / nx;
+  p->dz = 1.0 / ny;
+  p->dt = 1.0e-4 / 2.0;
+  p->re = 3;
+  p->ca = 12;
+  p->xld = 5.0e-2;
+  p->s_implicit = 1.5;
+  p->I_thickness = 0.01;
+  p->ggx = 0;
+  p->ggz = 0;
+  p->uwT = 0;
+  p->uwB = 0;
+  p->xlsB = 0;
+  p->tol = 1.0e-8;
+  p->penalty_k = 1.0e8;
+  p->angle = 60;
+  p->lambda_rho = 1;
+  p->bdr_ph_used_adjust = 1.0e-6;
+  p->twall = 0.1;
+  p->UU = 0;
+  p->bdr_upp = 1;
+  p->bdr_low = 0;
+}
+
+int oc_init(const drd_params* prm, oc_ctx** out) {
+  if (!prm || !out || prm->nx < 2 || prm->ny < 2) return DRD_E_ARG;
+  oc_ctx* c = (oc_ctx*)calloc(1, sizeof(oc_ctx));
+  c->p = *prm;
+  for (int f = 0; f < DRD_F_COUNT; ++f)
+    c->f[f] = (double*)calloc((size_t)oc_field_size(prm, f), sizeof(double));
+  c->nmax = nuv(prm) > 2 * (int64_t)prm->nx * prm->ny ? nuv(prm) : 2 * (int64_t)prm->nx * prm->ny;
+  /* scratch holds a field with ghosts or a Krylov vector, whichever is larger */
+  int64_t big = nufield(prm) > ncell(prm) ? nufield(prm) : ncell(prm);
+  if (c->nmax > big) big = c->nmax;
+  double** ws[] = {&c->w1, &c->w2, &c->w3, &c->w4, &c->w5, &c->w6};
+  for (int i = 0; i < 6; ++i) *ws[i] = (double*)calloc((size_t)big, sizeof(double));
+  c->work = (double*)calloc((size_t)c->nmax * 7, sizeof(double));
+  c->uL = (double*)calloc((size_t)prm->ny + 1, sizeof(double));
+  c->uR = (double*)calloc((size_t)prm->ny + 1, sizeof(double));
+  c->coef_bot = c->coef_top = 1.0;
+
+  /* setup: Pre_Lcc and Ph_Lcc, TP/setup.f90:108-155 */
+  const drd_params* p = prm;
+  int nx = p->nx, ny = p->ny;
+  double dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  c->pre_lcc = (double*)calloc((size_t)(nx + 1) * (ny + 1), sizeof(double));
+  c->ph_lcc = (double*)calloc((size_t)nx * ny, sizeof(double));
+  double* fmx = (double*)calloc((size_t)nx + 1, sizeof(double));
+  double* fmy = (double*)calloc((size_t)ny + 1, sizeof(double));
+  for (int i = 0; i <= nx; ++i) {
+    double t1 = PI_G * i / 2.0 / nx;
+    double s = sin(t1);
+    fmx[i] = 4.0 * (s * s) / dr2;
+  }
+  for (int j = 0; j <= ny; ++j) {
+    double t1 = PI_G * j / 2.0 / ny;
+    double s = sin(t1);
+    fmy[j] = 4.0 * (s * s) / dz2;
+  }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) c->pre_lcc[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
+  c->pre_lcc[0] = -1.0;
+  for (int i = 0; i < nx; ++i) {
+    double t1 = PI_G * i / 2.0 / (nx - 1);
+    double s = sin(t1);
+    fmx[i] = 4.0 * (s * s) / dr2;
+  }
+  for (int j = 0; j < ny; ++j) {
+    double t1 = PI_G * j / 2.0 / (ny - 1);
+    double s = sin(t1);
+    fmy[j] = 4.0 * (s * s) / dz2;
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double temp = fmx[i] + fmy[j];
+      c->ph_lcc[i + (size_t)nx * j] = 1. + p->xld * p->dt * p->s_implicit * temp / p->I_thickness +
+                                       p->xld * p->I_thickness * p->dt * (temp * temp);
+    }
+  free(fmx);
+  free(fmy);
+  *out = c;
+  return 0;
+}
+
+int oc_finalize(oc_ctx* c) {
+  if (!c) return 0;
+  for (int f = 0; f < DRD_F_COUNT; ++f) free(c->f[f]);
+  free(c->w1); free(c->w2); free(c->w3); free(c->w4); free(c->w5); free(c->w6);
+  free(c->work); free(c->uL); free(c->uR); free(c->pre_lcc); free(c->ph_lcc);
+  free(c);
+  return 0;
+}
+
+double* oc_field(oc_ctx* c, int32_t f) {
+  if (!c || f < 0 || f >= DRD_F_COUNT) return NULL;
+  return c->f[f];
+}
+const double* oc_pre_lcc(oc_ctx* c) { return c->pre_lcc; }
+const double* oc_ph_lcc(oc_ctx* c) { return c->ph_lcc; }
+
+/* ---- BLAS-1 (the reference links MKL; plain sequential sums here) ------ */
+static double dnrm2(int64_t n, const double* x) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
+  return sqrt(s);
+}
+static double ddot(int64_t n, const double* x, const double* y) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += x[i] * y[i];
+  return s;
+}
+static void daxpy(int64_t n, double a, const double* x, double* y) {
+  for (int64_t i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
+}
+
+/* ---- packers: Atob_uv / btoA_uv  TP/matrix_tool.f90:5-75 -------------- */
+static void atob_uv(const oc_ctx* c, const double* f1, const double* f2, double* b) {
+  int nx = NX, ny = NY;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) b[i + (size_t)(nx + 1) * j] = f1[UI(i, j)];
+  size_t mn = (size_t)(nx + 1) * ny;
+  for (int j = 1; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) b[i + (size_t)nx * (j - 1) + mn] = f2[VI(i, j)];
+}
+
+static void btoa_uv(const oc_ctx* c, double* f1, double* f2, const double* b) {
+  int nx = NX, ny = NY;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) f1[UI(i, j)] = b[i + (size_t)(nx + 1) * j];
+  size_t mn = (size_t)(nx + 1) * ny;
+  for (int j = 1; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) f2[VI(i, j)] = b[i + (size_t)nx * (j - 1) + mn];
+  for (int i = 0; i <= nx; ++i) {
+    f1[UI(i, -1)] = -f1[UI(i, 0)];
+    f1[UI(i, ny)] = -f1[UI(i, ny - 1)];
+  }
+  for (int j = -1; j <= ny; ++j) {
+    f1[UI(-1, j)] = f1[UI(1, j)];
+    f1[UI(nx + 1, j)] = f1[UI(nx - 1, j)];
+  }
+  for (int i = 0; i < nx; ++i) {
+    f2[VI(i, 0)] = 0;
+    f2[VI(i, ny)] = 0;
+  }
+  for (int j = 0; j <= ny; ++j) {
+    f2[VI(-1, j)] = f2[VI(0, j)];
+    f2[VI(nx, j)] = f2[VI(nx - 1, j)];
+  }
+}
+
+/* Atob_ph_mu / btoA_ph_mu  TP/matrix_tool.f90:82-136 */
+static void atob_ph_mu(const oc_ctx* c, const double* f1, const double* f2, double* b) {
+  int nx = NX, ny = NY;
+  size_t mn = (size_t)nx * ny;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      b[i + (size_t)nx * j] = f1[CI(i, j)];
+      b[i + (size_t)nx * j + mn] = f2[CI(i, j)];
+    }
+}
+
+static void btoa_ph_mu(const oc_ctx* c, double* f1, double* f2, const double* b) {
+  int nx = NX, ny = NY;
+  size_t mn = (size_t)nx * ny;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      f1[CI(i, j)] = b[i + (size_t)nx * j];
+      f2[CI(i, j)] = b[i + (size_t)nx * j + mn];
+    }
+  for (int i = 0; i < nx; ++i) {
+    f1[CI(i, -1)] = f1[CI(i, 0)];
+    f1[CI(i, ny)] = f1[CI(i, ny - 1)];
+    f2[CI(i, -1)] = f2[CI(i, 0)];
+    f2[CI(i, ny)] = f2[CI(i, ny - 1)];
+  }
+  for (int j = -1; j <= ny; ++j) {
+    f1[CI(-1, j)] = f1[CI(0, j)];
+    f1[CI(nx, j)] = f1[CI(nx - 1, j)];
+    f2[CI(-1, j)] = f2[CI(0, j)];
+    f2[CI(nx, j)] = f2[CI(nx - 1, j)];
+  }
+}
+
+/* Atob / btoA on (0:m,0:n) of a (-1:m+1,-1:n+1) array TP/matrix_tool.f90:142-173 */
+static void atob_c(const oc_ctx* c, const double* f, double* b) {
+  for (int j = 0; j < NY; ++j)
+    for (int i = 0; i < NX; ++i) b[i + (size_t)NX * j] = f[CI(i, j)];
+}
+static void btoa_c(const oc_ctx* c, double* f, const double* b) {
+  for (int j = 0; j < NY; ++j)
+    for (int i = 0; i < NX; ++i) f[CI(i, j)] = b[i + (size_t)NX * j];
+}
+
+/* op_double_gradient_Neum TP/matrix_tool.f90:178-219 */
+void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
+                                const double* rmu, const double* mo, double* rhs) {
+#define GI(i, k) ((size_t)((i) + 1) + (size_t)(nx + 2) * (size_t)((k) + 1))
+  size_t n = (size_t)(nx + 2) * (ny + 2);
+  for (size_t q = 0; q < n; ++q) rhs[q] = 0.0;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double moL, moR;
+      if (i == 0) {
+        moL = (mo[GI(i, j)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i + 1, j)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dx2 * (moL * rmu[GI(i, j)] + moR * rmu[GI(i + 1, j)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      } else if (i == nx - 1) {
+        moL = (mo[GI(i - 1, j)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i, j)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dx2 * (moL * rmu[GI(i - 1, j)] + moR * rmu[GI(i, j)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      } else {
+        moL = (mo[GI(i - 1, j)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i + 1, j)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dx2 * (moL * rmu[GI(i - 1, j)] + moR * rmu[GI(i + 1, j)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      }
+      if (j == 0) {
+        moL = (mo[GI(i, j)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i, j + 1)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dy2 * (moL * rmu[GI(i, j)] + moR * rmu[GI(i, j + 1)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      } else if (j == ny - 1) {
+        moL = (mo[GI(i, j - 1)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i, j)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dy2 * (moL * rmu[GI(i, j - 1)] + moR * rmu[GI(i, j)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      } else {
+        moL = (mo[GI(i, j - 1)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(i, j + 1)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dy2 * (moL * rmu[GI(i, j - 1)] + moR * rmu[GI(i, j + 1)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      }
+    }
+#undef GI
+}
+
+/* ---- velocity operator ------------------------------------------------
+ * MatrixVector_uv_CG  TP/VeloSolve_combined.f90:537-675   (bdf = 1)
+ * MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:536-676 (bdf = 2)   */
+static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, double beta,
+                        double* yy) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  double *f1 = c->w1, *f2 = c->w2, *f11 = c->w3, *f22 = c->w4;
+  const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA], *chi = c->f[DRD_F_BDR_PH_PENALTY];
+  double dt = p->dt, re = p->re, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
+  btoa_uv(c, f1, f2, xx);
+  if (p->lr_bd_type == 'd') {
+    for (int k = 0; k < ny; ++k) {
+      f1[UI(-1, k)] = -f1[UI(1, k)];
+      f1[UI(nx + 1, k)] = -f1[UI(nx - 1, k)];
+    }
+  } else {
+    for (int k = -1; k <= ny; ++k) {
+      f1[UI(-1, k)] = f1[UI(1, k)];
+      f1[UI(nx + 1, k)] = f1[UI(nx - 1, k)];
+    }
+  }
+  for (int k = 0; k <= ny; ++k) {
+    f2[VI(-1, k)] = f2[VI(0, k)];
+    f2[VI(nx, k)] = f2[VI(nx - 1, k)];
+  }
+  for (int i = -1; i <= nx + 1; ++i) {
+    f1[UI(i, -1)] = -f1[UI(i, 0)];
+    f1[UI(i, ny)] = -f1[UI(i, ny - 1)];
+  }
+  for (int i = -1; i <= nx; ++i) {
+    f2[VI(i, 0)] = 0.0;
+    f2[VI(i, ny)] = 0.0;
+  }
+  /* u rows */
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i) {
+      double r;
+      if (bdf == 2)
+        r = f1[UI(i, k)] * 1.5 * re * (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0 +
+            f1[UI(i, k)] * p->penalty_k * (chi[CI(i, k)] + chi[CI(i - 1, k)]) / 2.0 * dt;
+      else
+        r = f1[UI(i, k)] * re * (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0;
+      double cr = eta[CI(i, k)], cl = eta[CI(i - 1, k)];
+      r = r - 2.0 * dt * (cr * f1[UI(i + 1, k)] + cl * f1[UI(i - 1, k)] - (cl + cr) * f1[UI(i, k)]) / dr2;
+      cr = (eta[CI(i, k + 1)] + eta[CI(i - 1, k + 1)] + eta[CI(i, k)] + eta[CI(i - 1, k)]) / 4.0;
+      cl = (eta[CI(i, k)] + eta[CI(i - 1, k)] + eta[CI(i, k - 1)] + eta[CI(i - 1, k - 1)]) / 4.0;
+      r = r - dt * (cr * f1[UI(i, k + 1)] + cl * f1[UI(i, k - 1)] - (cl + cr) * f1[UI(i, k)]) / dz2;
+      r = r - dt * (cr * (f2[VI(i, k + 1)] - f2[VI(i - 1, k + 1)]) - cl * (f2[VI(i, k)] - f2[VI(i - 1, k)])) / dr / dz;
+      f11[UI(i, k)] = r;
+    }
+  /* v rows */
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double r;
+      if (bdf == 2)
+        r = f2[VI(i, k)] * 1.5 * re * (rho[CI(i, k)] + rho[CI(i, k - 1)]) / 2.0 +
+            f2[VI(i, k)] * p->penalty_k * (chi[CI(i, k)] + chi[CI(i, k - 1)]) / 2.0 * dt;
+      else
+        r = f2[VI(i, k)] * re * (rho[CI(i, k)] + rho[CI(i, k - 1)]) / 2.0;
+      double cr = eta[CI(i, k)], cl = eta[CI(i, k - 1)];
+      r = r - 2.0 * dt * (cr * f2[VI(i, k + 1)] + cl * f2[VI(i, k - 1)] - (cl + cr) * f2[VI(i, k)]) / dz2;
+      cr = (eta[CI(i + 1, k)] + eta[CI(i, k)] + eta[CI(i + 1, k - 1)] + eta[CI(i, k - 1)]) / 4.0;
+      cl = (eta[CI(i, k)] + eta[CI(i - 1, k)] + eta[CI(i, k - 1)] + eta[CI(i - 1, k - 1)]) / 4.0;
+      r = r - dt * (cr * f2[VI(i + 1, k)] + cl * f2[VI(i - 1, k)] - (cl + cr) * f2[VI(i, k)]) / dr2;
+      r = r - dt * (cr * (f1[UI(i + 1, k)] - f1[UI(i + 1, k - 1)]) - cl * (f1[UI(i, k)] - f1[UI(i, k - 1)])) / dr / dz;
+      f22[VI(i, k)] = r;
+    }
+  int64_t n = nuv(p);
+  double* zz = c->w5;
+  atob_uv(c, f11, f22, zz);
+  for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
+}
+
+/* PSolve_uv_CG (Jacobi) TP/VeloSolve_combined.f90:679-741.  The v rows
+ * average rho over (i-1,j) as the reference does (:720).               */
+static void velo_jacobi(oc_ctx* c, double* yy, const double* xx) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  double *f1 = c->w1, *f2 = c->w2, *f11 = c->w3, *f22 = c->w4;
+  const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA];
+  double dt = p->dt, re = p->re, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  btoa_uv(c, f1, f2, xx);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      double tmp = re * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
+      double cr = eta[CI(i, j)], cl = eta[CI(i - 1, j)];
+      tmp = tmp + 2.0 * dt * (cl + cr) / dr2;
+      cr = (eta[CI(i, j + 1)] + eta[CI(i - 1, j + 1)] + eta[CI(i, j)] + eta[CI(i - 1, j)]) / 4.0;
+      cl = (eta[CI(i, j)] + eta[CI(i - 1, j)] + eta[CI(i, j - 1)] + eta[CI(i - 1, j - 1)]) / 4.0;
+      if (j == 0)
+        tmp = tmp + dt * (cl + (cl + cr)) / dz2;
+      else if (j == ny - 1)
+        tmp = tmp + dt * (cr + (cl + cr)) / dz2;
+      else
+        tmp = tmp + dt * (cl + cr) / dz2;
+      f11[UI(i, j)] = f1[UI(i, j)] / tmp;
+    }
+  for (int j = 1; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double tmp = re * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
+      double cr = eta[CI(i, j)], cl = eta[CI(i, j - 1)];
+      tmp = tmp + 2.0 * dt * (cl + cr) / dz2;
+      cr = (eta[CI(i + 1, j)] + eta[CI(i, j)] + eta[CI(i + 1, j - 1)] + eta[CI(i, j - 1)]) / 4.0;
+      cl = (eta[CI(i, j)] + eta[CI(i - 1, j)] + eta[CI(i, j - 1)] + eta[CI(i - 1, j - 1)]) / 4.0;
+      if (i == 0)
+        tmp = tmp + dt * cr / dr2;
+      else if (i == nx - 1)
+        tmp = tmp + dt * cl / dr2;
+      else
+        tmp = tmp + dt * (cl + cr) / dr2;
+      f22[VI(i, j)] = f2[VI(i, j)] / tmp;
+    }
+  atob_uv(c, f11, f22, yy);
+}
+
+/* ---- Cahn-Hilliard operators ------------------------------------------ */
+/* MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:203-299 */
+static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, double* yy) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  double *f1 = c->w1, *f2 = c->w2, *ff1 = c->w3, *ff2 = c->w4;
+  const double* bu = c->f[DRD_F_BDR_PH_USED];
+  double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  btoa_ph_mu(c, f1, f2, xx);
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f1, c->f[DRD_F_PH_M_INNER], ff1);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      ff1[CI(i, j)] = s * f1[CI(i, j)] / I - I * ff1[CI(i, j)] - f2[CI(i, j)] * bu[CI(i, j)];
+  for (int i = 0; i < nx; ++i) {
+    int j = 0;
+    ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j - 1)] - c->coef_bot * f1[CI(i, j)]);
+    j = ny - 1;
+    ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j + 1)] - c->coef_top * f1[CI(i, j)]);
+  }
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f2, c->f[DRD_F_PH_M_0], ff2);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      ff2[CI(i, j)] = f1[CI(i, j)] * (bu[CI(i, j)] + p->bdr_ph_used_adjust) - p->dt * p->xld * ff2[CI(i, j)];
+  int64_t n = 2 * (int64_t)nx * ny;
+  double* zz = c->w5;
+  atob_ph_mu(c, ff1, ff2, zz);
+  for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
+}
+
+/* PSolve_PH_MU_CG (Jacobi) TP/ph_mu_solve.f90:304-356.  The x test reads
+ * `j == nr-1` and the z test `i == 0` as in the reference (:323,:328).  */
+static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  double *f1 = c->w1, *f2 = c->w2, *ff1 = c->w3, *ff2 = c->w4;
+  double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  btoa_ph_mu(c, f1, f2, xx);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
+      double tmpz = (i == 0 || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
+      ff1[CI(i, j)] = f1[CI(i, j)] / ((tmpx + tmpz) * I + s / I);
+    }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
+      double tmpz = (i == 0 || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
+      ff2[CI(i, j)] = f2[CI(i, j)] / (p->dt * p->xld * (tmpx + tmpz));
+    }
+  atob_ph_mu(c, ff1, ff2, yy);
+}
+
+/* MATRIX_VECTOR_CG MF/PhSolve.f90:240-292 */
+static void ph_matvec(oc_ctx* c, double alpha, const double* xx, double beta, double* yy) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  double *f1 = c->w1, *ff1 = c->w3, *ff2 = c->w4;
+  double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  int64_t n = ncell(&c->p);
+  for (int64_t q = 0; q < n; ++q) f1[q] = 0.0;
+  btoa_c(c, f1, xx);
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f1, c->f[DRD_F_PH_M_0], ff1);
+  for (int64_t q = 0; q < n; ++q) ff1[q] = -p->dt * p->xld * ff1[q];
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, ff1, c->f[DRD_F_PH_M_INNER], ff2);
+  for (int64_t q = 0; q < n; ++q) ff2[q] = s * ff1[q] / I - I * ff2[q];
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      size_t q = i + (size_t)nx * j;
+      yy[q] = alpha * (f1[CI(i, j)] + ff2[CI(i, j)]) + beta * yy[q];
+    }
+}
+
+/* ---- transforms -------------------------------------------------------- */
+/* FFTW REDFT10 along one dimension: Y_k = 2 sum_j x_j cos(pi (j+1/2) k / n) */
+static void dct2_lines(int n, int m, double* a, int stride, int ld, double* tmp, const double* tab) {
+  for (int l = 0; l < m; ++l) {
+    double* x = a + (size_t)l * ld;
+    for (int k = 0; k < n; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += x[(size_t)j * stride] * tab[(size_t)k * n + j];
+      tmp[k] = 2.0 * s;
+    }
+    for (int k = 0; k < n; ++k) x[(size_t)k * stride] = tmp[k];
+  }
+}
+/* FFTW REDFT01: Y_k = x_0 + 2 sum_{j>=1} x_j cos(pi j (k+1/2) / n)        */
+static void dct3_lines(int n, int m, double* a, int stride, int ld, double* tmp, const double* tab) {
+  for (int l = 0; l < m; ++l) {
+    double* x = a + (size_t)l * ld;
+    for (int k = 0; k < n; ++k) {
+      double s = 0.0;
+      for (int j = 1; j < n; ++j) s += x[(size_t)j * stride] * tab[(size_t)j * n + k];
+      tmp[k] = x[0] + 2.0 * s;
+    }
+    for (int k = 0; k < n; ++k) x[(size_t)k * stride] = tmp[k];
+  }
+}
+static double* dct_table(int n) {
+  /* tab[k*n+j] = cos(pi (2j+1) k / (2n)) with the argument reduced mod 4n */
+  double* t = (double*)malloc((size_t)n * n * sizeof(double));
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j) {
+      long long q = ((long long)(2 * j + 1) * k) % (4LL * n);
+      t[(size_t)k * n + j] = cos(M_PI * (double)q / (2.0 * n));
+    }
+  return t;
+}
+void oc_dct2_2d(int32_t nx, int32_t ny, double* a) {
+  int mx = nx > ny ? nx : ny;
+  double* tmp = (double*)malloc((size_t)mx * sizeof(double));
+  double* tx = dct_table(nx);
+  double* ty = dct_table(ny);
+  dct2_lines(nx, ny, a, 1, nx, tmp, tx);
+  dct2_lines(ny, nx, a, nx, 1, tmp, ty);
+  free(tmp); free(tx); free(ty);
+}
+void oc_dct3_2d(int32_t nx, int32_t ny, double* a) {
+  int mx = nx > ny ? nx : ny;
+  double* tmp = (double*)malloc((size_t)mx * sizeof(double));
+  double* tx = dct_table(nx);
+  double* ty = dct_table(ny);
+  dct3_lines(nx, ny, a, 1, nx, tmp, tx);
+  dct3_lines(ny, nx, a, nx, 1, tmp, ty);
+  free(tmp); free(tx); free(ty);
+}
+
+/* VCOST LIB/vfftpack.f:643-704, by its definition:
+ * X(i) = [X(1) + (-1)^(i-1) X(N) + sum_{k=2}^{N-1} 2 X(k) cos((k-1)(i-1) pi/(N-1))] / sqrt(2(N-1)) */
+void oc_vcost(int32_t m, int32_t n, double* x, int32_t ld) {
+  double* tmp = (double*)malloc((size_t)n * sizeof(double));
+  double sc = 1.0 / sqrt(2.0 * (n - 1));
+  for (int l = 0; l < m; ++l) {
+    double* s = x + (size_t)l * ld;
+    for (int i = 0; i < n; ++i) {
+      double acc = s[0] + ((i & 1) ? -s[n - 1] : s[n - 1]);
+      for (int k = 1; k < n - 1; ++k) {
+        long long q = ((long long)k * i) % (2LL * (n - 1));
+        acc += 2.0 * s[k] * cos(M_PI * (double)q / (double)(n - 1));
+      }
+      tmp[i] = acc * sc;
+    }
+    for (int i = 0; i < n; ++i) s[i] = tmp[i];
+  }
+  free(tmp);
+}
+
+/* DCT TP/DCT.f90:1-57.  fO,fN (-1:nr+1,-1:nz+1); cof (0:nr,0:nz).  The
+ * VCOST passes along z run over the nr+1 columns, those along x over the
+ * nz+1 rows (the reference transposes through `temp` to do the latter). */
+void oc_dct_solve(int32_t nr, int32_t nz, double* fO, double* fN, const double* cof) {
+  int ld = nr + 3;
+#define DI(i, k) ((size_t)((i) + 1) + (size_t)ld * (size_t)((k) + 1))
+  int n1 = nr + 1, n2 = nz + 1;
+  double* col = (double*)malloc((size_t)(n1 > n2 ? n1 : n2) * sizeof(double));
+  for (int pass = 0; pass < 2; ++pass) {
+    /* along z for each x */
+    for (int i = 0; i <= nr; ++i) {
+      for (int k = 0; k <= nz; ++k) col[k] = fO[DI(i, k)];
+      oc_vcost(1, n2, col, n2);
+      for (int k = 0; k <= nz; ++k) fO[DI(i, k)] = col[k];
+    }
+    /* along x for each z */
+    for (int k = 0; k <= nz; ++k) {
+      for (int i = 0; i <= nr; ++i) col[i] = fO[DI(i, k)];
+      oc_vcost(1, n1, col, n1);
+      for (int i = 0; i <= nr; ++i) fO[DI(i, k)] = col[i];
+    }
+    if (pass == 0)
+      for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i <= nr; ++i) fO[DI(i, k)] = fO[DI(i, k)] / cof[i + (size_t)(nr + 1) * k];
+  }
+  for (int k = 0; k <= nz; ++k)
+    for (int i = 0; i <= nr; ++i) fN[DI(i, k)] = fO[DI(i, k)];
+  for (int i = 0; i <= nr; ++i) {
+    fN[DI(i, -1)] = fN[DI(i, 1)];
+    fN[DI(i, nz + 1)] = fN[DI(i, nz - 1)];
+  }
+  for (int k = 0; k <= nz; ++k) {
+    fN[DI(-1, k)] = fN[DI(1, k)];
+    fN[DI(nr + 1, k)] = fN[DI(nr - 1, k)];
+  }
+  free(col);
+#undef DI
+}
+
+/* rfords TP/setup.f90:327-354 (Thomas algorithm, 1-based arrays)       */
+void oc_rfords(int32_t n, double* ff, const double* aa, const double* bb, const double* cc) {
+  double* f = (double*)malloc((size_t)n * sizeof(double));
+  double* b = (double*)malloc((size_t)n * sizeof(double));
+  for (int j = 0; j < n; ++j) {
+    f[j] = ff[j];
+    b[j] = bb[j];
+  }
+  for (int j = 1; j < n; ++j) {
+    double r = -aa[j] / b[j - 1];
+    b[j] = b[j] + r * cc[j - 1];
+    f[j] = f[j] + r * f[j - 1];
+  }
+  if (fabs(b[n - 1]) <= 1.0e-12)
+    ff[n - 1] = 0.0;
+  else
+    ff[n - 1] = f[n - 1] / b[n - 1];
+  for (int j = n - 2; j >= 0; --j) ff[j] = (f[j] - cc[j] * ff[j + 1]) / b[j];
+  free(f);
+  free(b);
+}
+
+/* ESOLP_x TP/ESOLP.f90:1-82: VCOST along x per row, Thomas along z per
+ * x-mode, VCOST along x again. */
+void oc_esolp_x(int32_t nr, int32_t nz, const double* pa, const double* pb, const double* pc,
+                const double* b, double* p) {
+  int ld = nr + 3;
+#define DI(i, k) ((size_t)((i) + 1) + (size_t)ld * (size_t)((k) + 1))
+  int n1 = nr + 1, n2 = nz + 1;
+  /* bb(i,k) = b(k,i): rows of b become rows of work arrays */
+  double* w = (double*)malloc((size_t)n1 * n2 * sizeof(double)); /* w[x + n1*z] */
+  double* line = (double*)malloc((size_t)(n1 > n2 ? n1 : n2) * sizeof(double));
+  for (int z = 0; z <= nz; ++z)
+    for (int x = 0; x <= nr; ++x) w[x + (size_t)n1 * z] = b[DI(x, z)];
+  oc_vcost(n2, n1, w, n1);
+  double *wa = (double*)malloc((size_t)n2 * sizeof(double)), *wb = (double*)malloc((size_t)n2 * sizeof(double)),
+         *wc = (double*)malloc((size_t)n2 * sizeof(double));
+  for (int k = 0; k <= nr; ++k) {
+    for (int i = 0; i < n2; ++i) {
+      line[i] = w[k + (size_t)n1 * i];
+      wa[i] = pa[i + (size_t)n2 * k];
+      wb[i] = pb[i + (size_t)n2 * k];
+      wc[i] = pc[i + (size_t)n2 * k];
+    }
+    oc_rfords(n2, line, wa, wb, wc);
+    for (int i = 0; i < n2; ++i) w[k + (size_t)n1 * i] = line[i];
+  }
+  oc_vcost(n2, n1, w, n1);
+  for (int z = 0; z <= nz; ++z)
+    for (int x = 0; x <= nr; ++x) p[DI(x, z)] = w[x + (size_t)n1 * z];
+  free(w); free(line); free(wa); free(wb); free(wc);
+#undef DI
+}
+
+/* PSolve_CG MF/PhSolve.f90:296-313: DCT(nr-1,nz-1,...,Ph_Lcc) */
+static void ph_psolve(oc_ctx* c, double* yy, const double* xx) {
+  double *f1 = c->w1, *f2 = c->w2;
+  int64_t n = ncell(&c->p);
+  for (int64_t q = 0; q < n; ++q) f1[q] = 0.0;
+  btoa_c(c, f1, xx);
+  oc_dct_solve(NX - 1, NY - 1, f1, f2, c->ph_lcc);
+  atob_c(c, f2, yy);
+}
+
+int oc_matvec(oc_ctx* c, int32_t op, double alpha, const double* x, double beta, double* y) {
+  switch (op) {
+    case DRD_OP_VELO1: velo_matvec(c, 1, alpha, x, beta, y); return 0;
+    case DRD_OP_VELO2: velo_matvec(c, 2, alpha, x, beta, y); return 0;
+    case DRD_OP_PHMU: phmu_matvec(c, alpha, x, beta, y); return 0;
+    case DRD_OP_PH: ph_matvec(c, alpha, x, beta, y); return 0;
+  }
+  return DRD_E_ARG;
+}
+
+static int64_t op_size(const oc_ctx* c, int32_t op) {
+  switch (op) {
+    case DRD_OP_VELO1:
+    case DRD_OP_VELO2: return nuv(&c->p);
+    case DRD_OP_PHMU: return 2 * (int64_t)c->p.nx * c->p.ny;
+    case DRD_OP_PH: return (int64_t)c->p.nx * c->p.ny;
+  }
+  return -1;
+}
+
+int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* b_in) {
+  int64_t n = op_size(c, op);
+  if (n < 0) return DRD_E_ARG;
+  switch (prec) {
+    case DRD_PS_VELO1_JACOBI: velo_jacobi(c, x_out, b_in); return 0;
+    case DRD_PS_IDENTITY: memcpy(x_out, b_in, (size_t)n * sizeof(double)); return 0;
+    case DRD_PS_PHMU_JACOBI: phmu_jacobi(c, x_out, b_in); return 0;
+    case DRD_PS_PH_DCT: ph_psolve(c, x_out, b_in); return 0;
+  }
+  return DRD_E_UNSUPPORTED;
+}
+
+/* ---- BiCGSTAB ----------------------------------------------------------
+ * LIB/BiCGSTAB.f:153-217 driving LIB/BiCGSTABREVCOM.f:150-418 with
+ * LIB/STOPTEST2.f and LIB/getbreak.f, restated as one loop.  The control
+ * flow, tests and reported ITER/RESID/INFO follow the reverse-communication
+ * original exactly:
+ *   - r0 = b - A x0 only if ||x0|| != 0; then exit if ||r0|| <= tol
+ *     (absolute), leaving ITER and RESID as passed in   (REVCOM:229-249)
+ *   - breakdown tolerance eps^2 with eps = DLAMCH('E') = 2^-53 (getbreak.f)
+ *   - early exit when ||s|| <= tol (absolute)            (REVCOM:311-316)
+ *   - convergence when ||r||/||b|| <= tol (STOPTEST2.f:13-16)
+ *   - INFO = 1 at ITER == MAXIT, -10 (rho) / -11 (omega) breakdowns.    */
+typedef void (*mv_fn)(void* u, double alpha, const double* x, double beta, double* y);
+typedef void (*ps_fn)(void* u, double* x_out, const double* b_in);
+
+static int bicgstab_core(int64_t n, const double* b, double* x, double* work, int32_t* iter,
+                         double* resid, mv_fn mv, ps_fn ps, void* u) {
+  if (n < 0) return -1;
+  if (*iter <= 0) return -3;
+  int32_t maxit = *iter;
+  double tol = *resid;
+  double *R = work, *RT = work + n, *P = work + 2 * n, *V = work + 3 * n, *T = work + 4 * n,
+         *PH = work + 5 * n, *SH = work + 6 * n, *S = R;
+  const double eps = ldexp(1.0, -53);
+  const double breaktol = eps * eps;
+  double rho = 0, rho1 = 1, alpha = 0, omega = 1, bnrm2;
+  memcpy(R, b, (size_t)n * sizeof(double));
+  if (dnrm2(n, x) != 0.0) {
+    mv(u, -1.0, x, 1.0, R);
+    if (dnrm2(n, R) <= tol) return 0;
+  }
+  memcpy(RT, R, (size_t)n * sizeof(double));
+  bnrm2 = dnrm2(n, b);
+  if (bnrm2 == 0.0) bnrm2 = 1.0;
+  int32_t it = 0;
+  for (;;) {
+    it = it + 1;
+    *iter = it;
+    rho = ddot(n, RT, R);
+    if (fabs(rho) < breaktol) return -10;
+    if (it > 1) {
+      double beta = (rho / rho1) * (alpha / omega);
+      daxpy(n, -omega, V, P);
+      for (int64_t q = 0; q < n; ++q) P[q] = beta * P[q];
+      daxpy(n, 1.0, R, P);
+    } else {
+      memcpy(P, R, (size_t)n * sizeof(double));
+    }
+    ps(u, PH, P);
+    mv(u, 1.0, PH, 0.0, V);
+    alpha = rho / ddot(n, RT, V);
+    daxpy(n, -alpha, V, R);
+    /* S aliases R (DCOPY of a column onto itself) */
+    double snrm = dnrm2(n, S);
+    if (snrm <= tol) {
+      daxpy(n, alpha, PH, x);
+      *resid = snrm / bnrm2;
+      return 0;
+    }
+    ps(u, SH, S);
+    mv(u, 1.0, SH, 0.0, T);
+    omega = ddot(n, T, S) / ddot(n, T, T);
+    daxpy(n, alpha, PH, x);
+    daxpy(n, omega, SH, x);
+    daxpy(n, -omega, T, R);
+    *resid = dnrm2(n, R) / bnrm2;
+    if (*resid <= tol) return 0;
+    if (it == maxit) return 1;
+    if (fabs(omega) < breaktol) return -11;
+    rho1 = rho;
+  }
+}
+
+typedef struct {
+  oc_ctx* c;
+  int32_t op, prec;
+} op_bind;
+static void mv_bound(void* u, double alpha, const double* x, double beta, double* y) {
+  op_bind* b = (op_bind*)u;
+  oc_matvec(b->c, b->op, alpha, x, beta, y);
+}
+static void ps_bound(void* u, double* xo, const double* bi) {
+  op_bind* b = (op_bind*)u;
+  oc_psolve(b->c, b->prec, b->op, xo, bi);
+}
+
+int oc_bicgstab(oc_ctx* c, int32_t op, int32_t prec, int32_t n, const double* b, double* x,
+                int32_t* iter, double* resid, int32_t* info) {
+  if (n != op_size(c, op)) return DRD_E_ARG;
+  op_bind bd = {c, op, prec};
+  *info = bicgstab_core(n, b, x, c->work, iter, resid, mv_bound, ps_bound, &bd);
+  return 0;
+}
+
+typedef struct {
+  int32_t n;
+  const double* a;
+  int32_t jacobi;
+} dense_bind;
+static void mv_dense(void* u, double alpha, const double* x, double beta, double* y) {
+  dense_bind* d = (dense_bind*)u;
+  for (int i = 0; i < d->n; ++i) {
+    double s = 0;
+    for (int j = 0; j < d->n; ++j) s += d->a[i + (size_t)d->n * j] * x[j];
+    y[i] = alpha * s + beta * y[i];
+  }
+}
+static void ps_dense(void* u, double* xo, const double* bi) {
+  dense_bind* d = (dense_bind*)u;
+  for (int i = 0; i < d->n; ++i) xo[i] = d->jacobi ? bi[i] / d->a[i + (size_t)d->n * i] : bi[i];
+}
+int oc_bicgstab_dense(int32_t n, const double* a, int32_t jacobi, const double* b, double* x,
+                      int32_t* iter, double* resid, int32_t* info) {
+  double* work = (double*)calloc((size_t)n * 7, sizeof(double));
+  dense_bind d = {n, a, jacobi};
+  *info = bicgstab_core(n, b, x, work, iter, resid, mv_dense, ps_dense, &d);
+  free(work);
+  return 0;
+}
+
+/* true residual ||b - A x|| / ||b|| (an addition; FP64) */
+static double true_resid(oc_ctx* c, int32_t op, const double* b, const double* x) {
+  int64_t n = op_size(c, op);
+  double* r = (double*)malloc((size_t)n * sizeof(double));
+  memcpy(r, b, (size_t)n * sizeof(double));
+  oc_matvec(c, op, -1.0, x, 1.0, r);
+  double bn = dnrm2(n, b);
+  double v = dnrm2(n, r) / (bn == 0 ? 1.0 : bn);
+  free(r);
+  return v;
+}
+
+/* ---- velocity solve ----------------------------------------------------
+ * RHS of VeloSolve_combined TP/VeloSolve_combined.f90:48-396 (bdf 1) and
+ * VeloSolve_2_combined TP/VeloSolve_2_combined.f90:62-400 (bdf 2).      */
+int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
+                const double* u_pre, const double* v_pre, const double* ph_after,
+                const double* rmu_after, double t, const double* pf1, const double* pf2,
+                double* rhs) {
+  const drd_params* p = &c->p;
+  if (p->xlsB >= 1.0e-8) return DRD_E_UNSUPPORTED; /* GNBC branch is commented out */
+  int nx = NX, ny = NY;
+  const double *u = c->f[DRD_F_U], *v = c->f[DRD_F_V], *pp = c->f[DRD_F_P];
+  const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA], *bu = c->f[DRD_F_BDR_PH_USED];
+  double dt = p->dt, re = p->re, ca = p->ca, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
+  double *fu = c->w1, *fv = c->w2, *ut = c->w3, *vt = c->w4, *tp = c->w6;
+  for (int64_t q = 0; q < nufield(p); ++q) fu[q] = 0.0;
+  for (int64_t q = 0; q < nvfield(p); ++q) fv[q] = 0.0;
+  /* ca * mu * grad(phi * bdr_ph_used) :62-97 */
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i)
+      fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
+                     (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i - 1, k)] * bu[CI(i - 1, k)]) / dr;
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i)
+      fv[VI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i, k - 1)]) / 2.0 *
+                     (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i, k - 1)] * bu[CI(i, k - 1)]) / dz;
+  /* convection re*rho*(u.grad)u with u (bdf1) or 2u - u_pre (bdf2) :108-124 */
+  if (bdf == 2) {
+    for (int64_t q = 0; q < nufield(p); ++q) ut[q] = 2.0 * u[q] - u_pre[q];
+    for (int64_t q = 0; q < nvfield(p); ++q) vt[q] = 2.0 * v[q] - v_pre[q];
+  } else {
+    memcpy(ut, u, (size_t)nufield(p) * sizeof(double));
+    memcpy(vt, v, (size_t)nvfield(p) * sizeof(double));
+  }
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i)
+      fu[UI(i, k)] = fu[UI(i, k)] - re * (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0 *
+                                         (ut[UI(i, k)] * (ut[UI(i + 1, k)] - ut[UI(i - 1, k)]) / 2.0 / dr +
+                                          (vt[VI(i, k)] + vt[VI(i - 1, k)] + vt[VI(i, k + 1)] + vt[VI(i - 1, k + 1)]) / 4.0 *
+                                              (ut[UI(i, k + 1)] - ut[UI(i, k - 1)]) / 2.0 / dz);
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i)
+      fv[VI(i, k)] = fv[VI(i, k)] - re * (rho[CI(i, k)] + rho[CI(i, k - 1)]) / 2.0 *
+                                         ((ut[UI(i, k)] + ut[UI(i + 1, k)] + ut[UI(i, k - 1)] + ut[UI(i + 1, k - 1)]) / 4.0 *
+                                              (vt[VI(i + 1, k)] - vt[VI(i - 1, k)]) / 2.0 / dr +
+                                          vt[VI(i, k)] * (vt[VI(i, k + 1)] - vt[VI(i, k - 1)]) / 2.0 / dz);
+  /* pressure, inertia and body force :198-221 (bdf2), :192-215 (bdf1) */
+  int64_t nc = ncell(p);
+  if (bdf == 2)
+    for (int64_t q = 0; q < nc; ++q) tp[q] = pp[q] + 4 * psi2[q] / 3.0 - psi1[q] / 3.0;
+  else
+    memcpy(tp, pp, (size_t)nc * sizeof(double));
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i) {
+      double rb = (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0;
+      double in;
+      if (bdf == 2)
+        in = 2.0 * u[UI(i, k)] * re / dt - u_pre[UI(i, k)] * re / 2.0 / dt +
+             (ph_after[CI(i, k)] + ph_after[CI(i - 1, k)]) / 2.0 * p->ggx;
+      else
+        in = u[UI(i, k)] * re / dt + p->ggx;
+      fu[UI(i, k)] = fu[UI(i, k)] - (tp[CI(i, k)] - tp[CI(i - 1, k)]) / dr + rb * in;
+    }
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double rb = (rho[CI(i, k)] + rho[CI(i, k - 1)]) / 2.0;
+      double in;
+      if (bdf == 2)
+        in = 2.0 * v[VI(i, k)] * re / dt - v_pre[VI(i, k)] * re / 2.0 / dt +
+             (ph_after[CI(i, k)] + ph_after[CI(i - 1, k)]) / 2.0 * p->ggz;
+      else
+        in = v[VI(i, k)] * re / dt + p->ggz;
+      fv[VI(i, k)] = fv[VI(i, k)] - (tp[CI(i, k)] - tp[CI(i, k - 1)]) / dz + rb * in;
+    }
+  /* particle force :240-250 */
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i) fu[UI(i, k)] = fu[UI(i, k)] + (pf1[CI(i, k)] + pf1[CI(i - 1, k)]) / 2.0;
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) fv[VI(i, k)] = fv[VI(i, k)] + (pf2[CI(i, k)] + pf2[CI(i, k - 1)]) / 2.0;
+  /* times dt :253-263 */
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i <= nx; ++i) fu[UI(i, k)] = fu[UI(i, k)] * dt;
+  for (int k = 1; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) fv[VI(i, k)] = fv[VI(i, k)] * dt;
+  /* Dirichlet inflow :272-310 */
+  if (p->lr_bd_type == 'd') {
+    double cf = 1.0 - exp(-t / p->twall);
+    double tmp_hig = p->bdr_upp - p->bdr_low;
+    for (int k = 0; k < ny; ++k) {
+      double z0 = (k + 0.5) * dz; /* z(0,k) TP/finit.f90:34-35 */
+      if (z0 > p->bdr_low && z0 < p->bdr_upp) {
+        double d = z0 - p->bdr_low - tmp_hig / 2;
+        c->uL[k] = p->UU * (-(d * d) / tmp_hig / tmp_hig * 4 + 1) * cf;
+      } else {
+        c->uL[k] = 0.0;
+      }
+      fu[UI(0, k)] = fu[UI(0, k)] + 2.0 * c->uL[k] * eta[CI(0, k)] * 2.0 * dt / dr2;
+    }
+    for (int k = 0; k < ny; ++k) {
+      c->uR[k] = c->uL[k];
+      fu[UI(nx, k)] = fu[UI(nx, k)] + 2.0 * c->uR[k] * eta[CI(nx, k)] * 2.0 * dt / dr2;
+    }
+  }
+  /* wall speed terms :392-400 */
+  for (int i = 0; i <= nx; ++i) {
+    int k = 0;
+    double etay = (eta[CI(i, k)] + eta[CI(i - 1, k)] + eta[CI(i, k - 1)] + eta[CI(i - 1, k - 1)]) / 4.0;
+    fu[UI(i, k)] = fu[UI(i, k)] + dt * (2.0 * p->uwB / dz2 * etay);
+    k = ny - 1;
+    etay = (eta[CI(i, k + 1)] + eta[CI(i - 1, k + 1)] + eta[CI(i, k)] + eta[CI(i - 1, k)]) / 4.0;
+    fu[UI(i, k)] = fu[UI(i, k)] + dt * (2.0 * p->uwT / dz2 * etay);
+  }
+  atob_uv(c, fu, fv, rhs);
+  return 0;
+}
+
+/* unpack and ghost rows :426-457 */
+static void velo_unpack(oc_ctx* c, const double* x, double* u_after, double* v_after) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  btoa_uv(c, u_after, v_after, x);
+  if (p->lr_bd_type == 'd') {
+    for (int k = 0; k < ny; ++k) u_after[UI(-1, k)] = 2.0 * c->uL[k] - u_after[UI(1, k)];
+    for (int k = 0; k < ny; ++k) u_after[UI(nx + 1, k)] = 2.0 * c->uR[k] - u_after[UI(nx - 1, k)];
+  } else {
+    for (int k = 0; k <= ny; ++k) {
+      u_after[UI(-1, k)] = u_after[UI(1, k)];
+      u_after[UI(nx + 1, k)] = u_after[UI(nx - 1, k)];
+    }
+  }
+  for (int i = -1; i <= nx + 1; ++i) {
+    u_after[UI(i, -1)] = p->uwB * 2.0 - u_after[UI(i, 0)];
+    u_after[UI(i, ny)] = p->uwT * 2.0 - u_after[UI(i, ny - 1)];
+  }
+  for (int i = -1; i <= nx; ++i) {
+    v_after[VI(i, 0)] = 0.0;
+    v_after[VI(i, ny)] = 0.0;
+  }
+  for (int k = 1; k < ny; ++k) {
+    v_after[VI(-1, k)] = v_after[VI(0, k)];
+    v_after[VI(nx, k)] = v_after[VI(nx - 1, k)];
+  }
+}
+
+static int velo_prec_id(const oc_ctx* c, int bdf) {
+  switch (c->p.velo_prec) {
+    case DRD_PREC_IDENTITY: return DRD_PS_IDENTITY;
+    case DRD_PREC_JACOBI: return DRD_PS_VELO1_JACOBI;
+    default: return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
+  }
+}
+
+static int velo_solve(oc_ctx* c, int bdf, const double* psi1, const double* psi2,
+                      const double* u_pre, const double* v_pre, const double* ph_after,
+                      const double* rmu_after, double t, const double* pf1, const double* pf2,
+                      double* u_after, double* v_after, drd_solve_info* info) {
+  int64_t n = nuv(&c->p);
+  double* rhs = (double*)malloc((size_t)n * sizeof(double));
+  double* x = (double*)malloc((size_t)n * sizeof(double));
+  int rc = oc_velo_rhs(c, bdf, psi1, psi2, u_pre, v_pre, ph_after, rmu_after, t, pf1, pf2, rhs);
+  if (rc) {
+    free(rhs);
+    free(x);
+    return rc;
+  }
+  atob_uv(c, c->f[DRD_F_U], c->f[DRD_F_V], x);
+  int32_t it = bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2, inf = 0;
+  double res = c->p.tol;
+  int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
+  oc_bicgstab(c, op, velo_prec_id(c, bdf), (int32_t)n, rhs, x, &it, &res, &inf);
+  if (info) {
+    info->iter = it;
+    info->info = inf;
+    info->resid = res;
+    info->true_resid = true_resid(c, op, rhs, x);
+  }
+  velo_unpack(c, x, u_after, v_after);
+  free(rhs);
+  free(x);
+  return 0;
+}
+
+int oc_velosolve_combined(oc_ctx* c, const double* psi, double* u_after, double* v_after,
+                          const double* rmu_after, const double* ph_after, double t,
+                          const double* pf1, const double* pf2, drd_solve_info* info) {
+  (void)psi; /* temp_p = p (the psi term is commented out, :192) */
+  return velo_solve(c, 1, NULL, NULL, NULL, NULL, ph_after, rmu_after, t, pf1, pf2, u_after,
+                    v_after, info);
+}
+
+int oc_velosolve_2_combined(oc_ctx* c, const double* psi1, const double* psi2,
+                            const double* u_pre, double* u_after, const double* v_pre,
+                            double* v_after, const double* ph_pre, const double* ph_after,
+                            const double* rmu_pre, const double* rmu_after, double t,
+                            const double* pf1, const double* pf2, drd_solve_info* info) {
+  (void)ph_pre;
+  (void)rmu_pre; /* unused by the active code */
+  return velo_solve(c, 2, psi1, psi2, u_pre, v_pre, ph_after, rmu_after, t, pf1, pf2, u_after,
+                    v_after, info);
+}
+
+/* ---- pressure projection  TP/PreSolve_2.f90:16-125 + TP/dct_fftw.f90 --- */
+int oc_presolve_2(oc_ctx* c, double* psi, double coef, double* mean_div) {
+  int nx = NX, ny = NY;
+  const double *u = c->f[DRD_F_U], *v = c->f[DRD_F_V];
+  double dr = c->p.dx, dz = c->p.dz;
+  double* uin = (double*)malloc((size_t)nx * ny * sizeof(double));
+  double sum = 0.0;
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      uin[i + (size_t)nx * k] = (u[UI(i + 1, k)] - u[UI(i, k)]) / dr + (v[VI(i, k + 1)] - v[VI(i, k)]) / dz;
+    }
+  for (int k = 0; k < ny; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < nx; ++i) s += uin[i + (size_t)nx * k];
+    sum += s;
+  }
+  if (mean_div) *mean_div = sum / nx / ny;
+  /* dct_fftw(nr,nz,plan,plan_back,uin,psi,Pre_Lcc,4.0*nr*nz) */
+  oc_dct2_2d(nx, ny, uin);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) uin[i + (size_t)nx * k] = uin[i + (size_t)nx * k] / c->pre_lcc[i + (size_t)(nx + 1) * k];
+  oc_dct3_2d(nx, ny, uin);
+  double sc = 4.0 * nx * ny;
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) psi[CI(i, k)] = uin[i + (size_t)nx * k] / sc;
+  for (int k = 0; k < ny; ++k) {
+    psi[CI(-1, k)] = psi[CI(0, k)];
+    psi[CI(nx, k)] = psi[CI(nx - 1, k)];
+  }
+  for (int i = -1; i <= nx; ++i) {
+    psi[CI(i, -1)] = psi[CI(i, 0)];
+    psi[CI(i, ny)] = psi[CI(i, ny - 1)];
+  }
+  int64_t nc = ncell(&c->p);
+  for (int64_t q = 0; q < nc; ++q) psi[q] = psi[q] * coef;
+  free(uin);
+  return 0;
+}
+
+/* ---- Cahn-Hilliard solves --------------------------------------------- */
+/* ch_bulk_energy TP/ch_bulk_energy.f90:4-39 */
+static double ch_bulk_energy(double phi, double psi, int mode) {
+  if (mode == 1) return 0.5 * phi * (1.0 - phi) * (1.0 - 2.0 * phi) * (1 - psi);
+  return (pow(phi, 3.0) - phi) * (1 - psi);
+}
+/* ch_bdr_condition TP/ch_bdr_condition.f90:43-67 */
+static double ch_bdr_condition(double angle, double phi_val, int mode) {
+  double cf = cos(angle * PI_G / 180.0) / sqrt(2.0);
+  if (mode == 1) return cf * (1.0 - phi_val) * phi_val;
+  return cf * (1.0 - phi_val * phi_val);
+}
+
+/* RHS of ph_mu_solve TP/ph_mu_solve.f90:31-153; writes ph_M_0 and the
+ * wall values pnb/pnt (returned through c->w5/c->w6 first nx entries). */
+int oc_ph_mu_rhs(oc_ctx* c, double* rhs) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  const double *ph = c->f[DRD_F_PH], *u = c->f[DRD_F_U], *v = c->f[DRD_F_V];
+  const double *bdr = c->f[DRD_F_BDR_PH], *bu = c->f[DRD_F_BDR_PH_USED], *bg = c->f[DRD_F_BDR_PH_GRAD];
+  double* m0 = c->f[DRD_F_PH_M_0];
+  double dt = p->dt, dr = p->dx, dz = p->dz, dz2 = dz * dz;
+  double I = p->I_thickness, s = p->s_implicit;
+  int64_t nc = ncell(p);
+  double *f1 = c->w1, *f2 = c->w2;
+  for (int64_t q = 0; q < nc; ++q) {
+    if (p->phi_mode == 1)
+      m0[q] = (1.0 - bdr[q]) * (ph[q] * ph[q]) * ((ph[q] - 1.0) * (ph[q] - 1.0));
+    else
+      m0[q] = (1.0 - bdr[q]) * ((ph[q] * ph[q] - 1.0) * (ph[q] * ph[q] - 1.0));
+  }
+  for (int64_t q = 0; q < nc; ++q) f2[q] = ph[q] * (bu[q] + p->bdr_ph_used_adjust);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i)
+      f2[CI(i, k)] = f2[CI(i, k)] -
+                     dt * (u[UI(i + 1, k)] + u[UI(i, k)]) / 2.0 *
+                         (ph[CI(i + 1, k)] * bu[CI(i + 1, k)] - ph[CI(i - 1, k)] * bu[CI(i - 1, k)]) / 2.0 / dr -
+                     dt * (v[VI(i, k + 1)] + v[VI(i, k)]) / 2.0 *
+                         (ph[CI(i, k + 1)] * bu[CI(i, k + 1)] - ph[CI(i, k - 1)] * bu[CI(i, k - 1)]) / 2.0 / dz;
+  double cosa = cos(p->angle * PI_G / 180);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double phv = ph[CI(i, k)];
+      if (p->phi_mode == 1)
+        f1[CI(i, k)] = s * phv / I - ch_bulk_energy(phv, bdr[CI(i, k)], 1) / I -
+                       phv * (phv - 1.0) * bg[CI(i, k)] * cosa / sqrt(2.0);
+      else
+        f1[CI(i, k)] = s * phv / I - ch_bulk_energy(phv, bdr[CI(i, k)], 2) / I -
+                       (phv * phv - 1.0) * bg[CI(i, k)] * cosa / sqrt(2.0);
+    }
+  double* pnb = c->w5;
+  double* pnt = c->w6;
+  const double *aB = c->f[DRD_F_ANGLEB], *aT = c->f[DRD_F_ANGLET];
+  for (int i = 0; i < nx; ++i) {
+    double tmp = (ph[CI(i, -1)] + ph[CI(i, 0)]) / 2.0;
+    pnb[i] = ch_bdr_condition(aB[i], tmp, p->phi_mode);
+  }
+  for (int i = 0; i < nx; ++i) {
+    double tmp = I / dz;
+    c->coef_bot = (I / dz) / tmp;
+    pnb[i] = pnb[i] / tmp;
+    f1[CI(i, 0)] = f1[CI(i, 0)] + pnb[i] * I / dz2;
+  }
+  for (int i = 0; i < nx; ++i) {
+    double tmp = (ph[CI(i, ny - 1)] + ph[CI(i, ny)]) / 2.0;
+    pnt[i] = ch_bdr_condition(aT[i], tmp, p->phi_mode);
+  }
+  for (int i = 0; i < nx; ++i) {
+    double tmp = I / dz;
+    c->coef_top = (I / dz) / tmp;
+    pnt[i] = pnt[i] / tmp;
+    f1[CI(i, ny - 1)] = f1[CI(i, ny - 1)] + pnt[i] * I / dz2;
+  }
+  atob_ph_mu(c, f1, f2, rhs);
+  return 0;
+}
+
+int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t, drd_solve_info* info) {
+  (void)t;
+  int nx = NX, ny = NY;
+  int64_t n = 2 * (int64_t)nx * ny;
+  double* rhs = (double*)malloc((size_t)n * sizeof(double));
+  double* x = (double*)malloc((size_t)n * sizeof(double));
+  double* pn = (double*)malloc((size_t)2 * nx * sizeof(double));
+  oc_ph_mu_rhs(c, rhs);
+  memcpy(pn, c->w5, (size_t)nx * sizeof(double));
+  memcpy(pn + nx, c->w6, (size_t)nx * sizeof(double));
+  atob_ph_mu(c, c->f[DRD_F_PH], c->f[DRD_F_RMU], x);
+  int32_t it = c->p.maxit_phmu, inf = 0;
+  double res = c->p.tol;
+  oc_bicgstab(c, DRD_OP_PHMU, DRD_PS_PHMU_JACOBI, (int32_t)n, rhs, x, &it, &res, &inf);
+  if (info) {
+    info->iter = it;
+    info->info = inf;
+    info->resid = res;
+    info->true_resid = true_resid(c, DRD_OP_PHMU, rhs, x);
+  }
+  btoa_ph_mu(c, ph_after, rmu_after, x);
+  for (int i = 0; i < nx; ++i) {
+    ph_after[CI(i, -1)] = ph_after[CI(i, 0)] * c->coef_bot + pn[i];
+    ph_after[CI(i, ny)] = ph_after[CI(i, ny - 1)] * c->coef_top + pn[nx + i];
+  }
+  free(rhs);
+  free(x);
+  free(pn);
+  return 0;
+}
+
+/* ch_bulk_energy MF/ch_bulk_energy.f90:4-28 (localised, phi in [0,1]) */
+static double ch_bulk_energy_mf(double phi, double psi) {
+  double aa = pow(phi, 2.0) * pow(1.0 - phi, 2.0);
+  aa = 1.0 - sqrt(aa / (aa + 1.0e-2)) * psi;
+  return 0.5 * phi * (1.0 - phi) * (1.0 - 2.0 * phi) * aa;
+}
+
+/* PhSolve MF/PhSolve.f90:1-126 */
+int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t, drd_solve_info* info) {
+  (void)t;
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  const double *ph = c->f[DRD_F_PH], *u = c->f[DRD_F_U], *v = c->f[DRD_F_V];
+  const double *bdr = c->f[DRD_F_BDR_PH], *bg = c->f[DRD_F_BDR_PH_GRAD];
+  double dt = p->dt, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
+  double I = p->I_thickness, s = p->s_implicit;
+  int64_t nc = ncell(p), n = (int64_t)nx * ny;
+  double* f1 = (double*)calloc((size_t)nc, sizeof(double));
+  double* f2 = (double*)calloc((size_t)nc, sizeof(double));
+  double* ff = (double*)calloc((size_t)nc, sizeof(double));
+  double* rhs = (double*)malloc((size_t)n * sizeof(double));
+  double* x = (double*)malloc((size_t)n * sizeof(double));
+  memcpy(f2, ph, (size_t)nc * sizeof(double));
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i)
+      f2[CI(i, k)] = f2[CI(i, k)] - dt * (u[UI(i + 1, k)] + u[UI(i, k)]) / 2.0 * (ph[CI(i + 1, k)] - ph[CI(i - 1, k)]) / 2.0 / dr -
+                     dt * (v[VI(i, k + 1)] + v[VI(i, k)]) / 2.0 * (ph[CI(i, k + 1)] - ph[CI(i, k - 1)]) / 2.0 / dz;
+  double cosa = cos(p->angle * PI_G / 180);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      double phv = ph[CI(i, k)];
+      f1[CI(i, k)] = s * phv / I - ch_bulk_energy_mf(phv, bdr[CI(i, k)]) / I -
+                     phv * (phv - 1.0) * bg[CI(i, k)] * cosa / sqrt(2.0);
+    }
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f2, c->f[DRD_F_PH_M_INNER], ff);
+  for (int64_t q = 0; q < nc; ++q) f1[q] = -f1[q] + s * f2[q] / I - I * ff[q];
+  atob_c(c, f1, rhs);
+  atob_c(c, c->f[DRD_F_RMU], x);
+  int32_t it = c->p.maxit_ph, inf = 0;
+  double res = c->p.tol;
+  oc_bicgstab(c, DRD_OP_PH, DRD_PS_PH_DCT, (int32_t)n, rhs, x, &it, &res, &inf);
+  if (info) {
+    info->iter = it;
+    info->info = inf;
+    info->resid = res;
+    info->true_resid = true_resid(c, DRD_OP_PH, rhs, x);
+  }
+  btoa_c(c, rmu_after, x);
+  for (int i = 0; i < nx; ++i) {
+    rmu_after[CI(i, -1)] = rmu_after[CI(i, 0)];
+    rmu_after[CI(i, ny)] = rmu_after[CI(i, ny - 1)];
+  }
+  for (int k = -1; k <= ny; ++k) {
+    rmu_after[CI(-1, k)] = rmu_after[CI(0, k)];
+    rmu_after[CI(nx, k)] = rmu_after[CI(nx - 1, k)];
+  }
+  oc_op_double_gradient_neum(nx, ny, dr2, dz2, rmu_after, c->f[DRD_F_PH_M_0], ff);
+  for (int64_t q = 0; q < nc; ++q) ph_after[q] = f2[q] + dt * p->xld * ff[q];
+  for (int i = 0; i < nx; ++i) ph_after[CI(i, -1)] = ph_after[CI(i, 0)];
+  for (int i = 0; i < nx; ++i) ph_after[CI(i, ny)] = ph_after[CI(i, ny - 1)];
+  for (int k = -1; k <= ny; ++k) {
+    ph_after[CI(-1, k)] = ph_after[CI(0, k)];
+    ph_after[CI(nx, k)] = ph_after[CI(nx - 1, k)];
+  }
+  free(f1); free(f2); free(ff); free(rhs); free(x);
+  return 0;
+}
+
+/* ---- one time step  TP/mcl6.f90:257-303 (bdf 1) and :395-502 (bdf 2) -- */
+int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
+  const drd_params* p = &c->p;
+  int64_t nc = ncell(p), nu = nufield(p), nv = nvfield(p);
+  double* ph_after = (double*)calloc((size_t)nc, sizeof(double));
+  double* rmu_after = (double*)calloc((size_t)nc, sizeof(double));
+  double* u_after = (double*)calloc((size_t)nu, sizeof(double));
+  double* v_after = (double*)calloc((size_t)nv, sizeof(double));
+  double* zero = (double*)calloc((size_t)nc, sizeof(double));
+  drd_step_info loc;
+  memset(&loc, 0, sizeof(loc));
+  double gamma1 = p->lambda_rho < 1.0 ? p->lambda_rho : 1.0;
+  double** F = c->f;
+  int rc;
+  if (bdf == 1) {
+    oc_ph_mu_solve(c, ph_after, rmu_after, t, &loc.phmu);
+    rc = oc_velosolve_combined(c, F[DRD_F_PSI1], u_after, v_after, rmu_after, ph_after, t + p->dt,
+                               zero, zero, &loc.velo);
+    if (rc) goto out;
+    memcpy(F[DRD_F_U_PRE], F[DRD_F_U], (size_t)nu * sizeof(double));
+    memcpy(F[DRD_F_V_PRE], F[DRD_F_V], (size_t)nv * sizeof(double));
+    memcpy(F[DRD_F_U], u_after, (size_t)nu * sizeof(double));
+    memcpy(F[DRD_F_V], v_after, (size_t)nv * sizeof(double));
+    oc_presolve_2(c, F[DRD_F_PSI2], p->re * gamma1 / p->dt, &loc.div_before);
+    for (int64_t q = 0; q < nc; ++q) F[DRD_F_P][q] = F[DRD_F_PSI2][q] + F[DRD_F_P][q];
+    memcpy(F[DRD_F_PH_PRE], F[DRD_F_PH], (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_RMU_PRE], F[DRD_F_RMU], (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_RMU], rmu_after, (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_PH], ph_after, (size_t)nc * sizeof(double));
+  } else {
+    oc_ph_mu_solve(c, ph_after, rmu_after, t + p->dt, &loc.phmu);
+    rc = oc_velosolve_2_combined(c, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], u_after,
+                                 F[DRD_F_V_PRE], v_after, F[DRD_F_PH_PRE], ph_after,
+                                 F[DRD_F_RMU_PRE], rmu_after, t + p->dt, zero, zero, &loc.velo);
+    if (rc) goto out;
+    memcpy(F[DRD_F_U_PRE], F[DRD_F_U], (size_t)nu * sizeof(double));
+    memcpy(F[DRD_F_V_PRE], F[DRD_F_V], (size_t)nv * sizeof(double));
+    memcpy(F[DRD_F_U], u_after, (size_t)nu * sizeof(double));
+    memcpy(F[DRD_F_V], v_after, (size_t)nv * sizeof(double));
+    memcpy(F[DRD_F_PH_PRE], F[DRD_F_PH], (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_RMU_PRE], F[DRD_F_RMU], (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_PH], ph_after, (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_RMU], rmu_after, (size_t)nc * sizeof(double));
+    memcpy(F[DRD_F_PSI1], F[DRD_F_PSI2], (size_t)nc * sizeof(double));
+    oc_presolve_2(c, F[DRD_F_PSI2], 3 * p->re * gamma1 / 2. / p->dt, &loc.div_before);
+    for (int64_t q = 0; q < nc; ++q) F[DRD_F_P][q] = F[DRD_F_PSI2][q] + F[DRD_F_P][q];
+  }
+  rc = 0;
+out:
+  if (info) *info = loc;
+  free(ph_after); free(rmu_after); free(u_after); free(v_after); free(zero);
+  return rc;
+}
