@@ -45,6 +45,19 @@ class SplitMix64:
         return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
 
 
+def splitmix_uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    """n SplitMix64 uniforms in [lo, hi), vectorised: element j is the
+    (j+1)-th output of SplitMix64(seed), identical to SplitMix64.uniform()."""
+    with np.errstate(over="ignore"):
+        j = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & _M64) + j * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+    return lo + (hi - lo) * u
+
+
 @dataclass
 class Config:
     name: str

@@ -1096,10 +1096,12 @@ int oc_presolve_2(oc_ctx* c, double* psi, double coef, double* mean_div) {
 }
 
 /* ---- Cahn-Hilliard solves --------------------------------------------- */
-/* ch_bulk_energy TP/ch_bulk_energy.f90:4-39 */
+/* ch_bulk_energy TP/ch_bulk_energy.f90:4-39.  phi**3.0 is evaluated as
+ * phi*phi*phi, which is how the reference build lowers a real power with an
+ * integral constant exponent (pinned against oracle/_ref). */
 static double ch_bulk_energy(double phi, double psi, int mode) {
   if (mode == 1) return 0.5 * phi * (1.0 - phi) * (1.0 - 2.0 * phi) * (1 - psi);
-  return (pow(phi, 3.0) - phi) * (1 - psi);
+  return (phi * phi * phi - phi) * (1 - psi);
 }
 /* ch_bdr_condition TP/ch_bdr_condition.f90:43-67 */
 static double ch_bdr_condition(double angle, double phi_val, int mode) {
@@ -1323,4 +1325,11 @@ out:
   if (info) *info = loc;
   free(ph_after); free(rmu_after); free(u_after); free(v_after); free(zero);
   return rc;
+}
+
+/* scalar helpers exported for pinning against the reference build */
+double oc_ch_bulk_energy(double phi, double psi, int32_t mode) { return ch_bulk_energy(phi, psi, mode); }
+double oc_ch_bulk_energy_mf(double phi, double psi) { return ch_bulk_energy_mf(phi, psi); }
+double oc_ch_bdr_condition(double angle, double phi, int32_t mode) {
+  return ch_bdr_condition(angle, phi, mode);
 }

@@ -85,6 +85,11 @@ void oc_esolp_x(int32_t nr, int32_t nz, const double* pa, const double* pb,
 /* rfords TP/setup.f90:327-354 */
 void oc_rfords(int32_t n, double* ff, const double* aa, const double* bb,
                const double* cc);
+/* ch_bulk_energy TP/ch_bulk_energy.f90, MF/ch_bulk_energy.f90 and
+ * ch_bdr_condition TP/ch_bdr_condition.f90                              */
+double oc_ch_bulk_energy(double phi, double psi, int32_t mode);
+double oc_ch_bulk_energy_mf(double phi, double psi);
+double oc_ch_bdr_condition(double angle, double phi, int32_t mode);
 /* op_double_gradient_Neum TP/matrix_tool.f90:178-219 on (-1:nx,-1:ny)   */
 void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2,
                                 double dy2, const double* rmu,

@@ -64,6 +64,12 @@ def lib():
         L.oc_rfords.argtypes = [C.c_int32, P, P, P, P]
         L.oc_op_double_gradient_neum.argtypes = [C.c_int32, C.c_int32, C.c_double,
                                                  C.c_double, P, P, P]
+        L.oc_ch_bulk_energy.argtypes = [C.c_double, C.c_double, C.c_int32]
+        L.oc_ch_bulk_energy.restype = C.c_double
+        L.oc_ch_bulk_energy_mf.argtypes = [C.c_double, C.c_double]
+        L.oc_ch_bulk_energy_mf.restype = C.c_double
+        L.oc_ch_bdr_condition.argtypes = [C.c_double, C.c_double, C.c_int32]
+        L.oc_ch_bdr_condition.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -248,3 +254,45 @@ def bicgstab_dense(a, b, x0, maxit, tol, jacobi=False):
     lib().oc_bicgstab_dense(len(b), dp(af), int(jacobi), dp(b), dp(x), C.byref(it),
                             C.byref(res), C.byref(info))
     return x, it.value, res.value, info.value
+
+
+def op_double_gradient_neum(nx, ny, dx2, dy2, rmu, mo):
+    rmu = np.ascontiguousarray(rmu, dtype=np.float64)
+    mo = np.ascontiguousarray(mo, dtype=np.float64)
+    out = np.zeros((ny + 2, nx + 2))
+    lib().oc_op_double_gradient_neum(nx, ny, dx2, dy2, dp(rmu), dp(mo), dp(out))
+    return out
+
+
+def dct_solve(nr1, nz1, f0, cof):
+    fo = np.ascontiguousarray(f0, dtype=np.float64).copy()
+    fn = np.zeros_like(fo)
+    lib().oc_dct_solve(nr1, nz1, dp(fo), dp(fn), dp(np.ascontiguousarray(cof, dtype=np.float64)))
+    return fn
+
+
+def esolp_x(nr1, nz1, pa, pb, pc, b):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    p = np.zeros_like(b)
+    arrs = [np.ascontiguousarray(t, dtype=np.float64) for t in (pa, pb, pc)]
+    lib().oc_esolp_x(nr1, nz1, *[dp(t) for t in arrs], dp(b), dp(p))
+    return p
+
+
+def rfords(ff, aa, bb, cc):
+    ff = np.ascontiguousarray(ff, dtype=np.float64).copy()
+    arrs = [np.ascontiguousarray(t, dtype=np.float64) for t in (aa, bb, cc)]
+    lib().oc_rfords(len(ff), dp(ff), *[dp(t) for t in arrs])
+    return ff
+
+
+def ch_bulk_energy(phi, psi, mode):
+    return lib().oc_ch_bulk_energy(phi, psi, mode)
+
+
+def ch_bulk_energy_mf(phi, psi):
+    return lib().oc_ch_bulk_energy_mf(phi, psi)
+
+
+def ch_bdr_condition(angle, phi, mode):
+    return lib().oc_ch_bdr_condition(angle, phi, mode)

@@ -70,12 +70,23 @@ class Ref:
         self.set_scalar("dz", p.dz)
         self.set_scalar("dr2", p.dx ** 2)
         self.set_scalar("dz2", p.dz ** 2)
+        # the MF module lacks some TP names (e.g. bdr_ph_used_adjust, phi_mode)
         for name in ("re", "ca", "xld", "s_implicit", "I_thickness", "ggx", "ggz", "uwT",
                      "uwB", "tol", "angle", "bdr_ph_used_adjust", "twall", "UU", "bdr_upp",
                      "bdr_low", "xlsB", "lambda_rho"):
-            self.set_scalar(name, getattr(p, name))
-        self.set_int("phi_mode", p.phi_mode)
-        self.set_char("lr_bd_type", chr(p.lr_bd_type))
+            if self.has(name):
+                self.set_scalar(name, getattr(p, name))
+        if self.has("phi_mode"):
+            self.set_int("phi_mode", p.phi_mode)
+        if self.has("lr_bd_type"):
+            self.set_char("lr_bd_type", chr(p.lr_bd_type))
+
+    def has(self, name: str) -> bool:
+        try:
+            C.c_char.in_dll(self.lib, f"_QMglobalE{name.lower()}")
+            return True
+        except ValueError:
+            return False
 
     # ---- routines ------------------------------------------------------------
     def _fn(self, name):

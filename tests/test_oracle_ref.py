@@ -1,0 +1,206 @@
+"""The oracle against the reference's own Fortran, compiled from its sources.
+
+oracle/_ref/libdrdref_tp.so (TP, grid fixed at 128 x 128, penalty_k 1e8) and
+libdrdref_mf.so (MF, 420 x 180, penalty_k 1e5) are built by `make -C oracle
+ref` from the reference checkout (oracle/Makefile).  Where they are absent
+these tests skip; tests/test_golden.py then checks the committed fixtures
+made from the same libraries.
+
+Bar: bit-exact for the stencil operators, preconditioners, tables and scalar
+helpers (same evaluation order, no contraction); 1e-12 relative for the
+FFT-based VCOST transforms (the oracle evaluates their definition).
+"""
+import numpy as np
+import pytest
+
+from drd_application_amd import abi
+from oracle import oracle as oc
+from oracle import refpy
+
+needs_tp = pytest.mark.skipif(not refpy.available("tp"), reason="oracle/_ref not built")
+needs_mf = pytest.mark.skipif(not refpy.available("mf"), reason="oracle/_ref not built")
+
+
+def _random_state(ref, o, p, rng, names):
+    for name, fid, lo, hi in names:
+        a = rng.uniform(lo, hi, ref.cell_shape)
+        o.set_field(fid, a)
+        ref.array(name, ref.cell_shape)[...] = a
+
+
+VELO_FIELDS = [("rho", abi.F_RHO, 0.5, 2.0), ("eta", abi.F_ETA, 1.0, 100.0),
+               ("bdr_ph_penalty", abi.F_BDR_PH_PENALTY, 0.0, 1.0)]
+CH_FIELDS = [("bdr_ph_used", abi.F_BDR_PH_USED, 0.0, 1.0), ("ph_m_inner", abi.F_PH_M_INNER, 0.0, 1.0),
+             ("ph_m_0", abi.F_PH_M_0, 0.0, 1.0)]
+
+
+@pytest.fixture(scope="module")
+def tp():
+    return refpy.Ref("tp")
+
+
+def _params(ref, lr="n"):
+    p = abi.default_params(ref.nr, ref.nz)
+    p.penalty_k = ref.penalty_k
+    p.lr_bd_type = ord(lr)
+    p.dt = 3.7e-5
+    p.re = 2.5
+    return p
+
+
+@needs_tp
+@pytest.mark.parametrize("lr", ["n", "d"])
+@pytest.mark.parametrize("op,name", [(abi.OP_VELO1, "matrixvector_uv_cg_"),
+                                     (abi.OP_VELO2, "matrixvector_uv_2_cg_")])
+def test_velocity_matvec(tp, lr, op, name):
+    p = _params(tp, lr)
+    tp.set_params(p)
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(101)
+    _random_state(tp, o, p, rng, VELO_FIELDS)
+    x, y = rng.standard_normal(tp.nuv), rng.standard_normal(tp.nuv)
+    np.testing.assert_array_equal(o.matvec(op, 0.7, x, -0.3, y), tp.matvec(name, 0.7, x, -0.3, y))
+
+
+@needs_tp
+def test_velocity_preconditioners(tp):
+    p = _params(tp)
+    tp.set_params(p)
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(102)
+    _random_state(tp, o, p, rng, VELO_FIELDS)
+    x = rng.standard_normal(tp.nuv)
+    np.testing.assert_array_equal(o.psolve(abi.PS_VELO1_JACOBI, abi.OP_VELO1, x),
+                                  tp.psolve("psolve_uv_cg_", x))
+    np.testing.assert_array_equal(o.psolve(abi.PS_IDENTITY, abi.OP_VELO2, x),
+                                  tp.psolve("psolve_uv_2_cg_", x))
+
+
+@needs_tp
+def test_ch_block_matvec_and_jacobi(tp):
+    p = _params(tp)
+    p.I_thickness, p.s_implicit, p.xld = 0.013, 1.7, 0.04
+    tp.set_params(p)
+    tp.set_scalar("coef_bot", 1.0)
+    tp.set_scalar("coef_top", 1.0)
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(103)
+    _random_state(tp, o, p, rng, CH_FIELDS)
+    n = 2 * p.nx * p.ny
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    np.testing.assert_array_equal(o.matvec(abi.OP_PHMU, -1.3, x, 0.6, y),
+                                  tp.matvec("matrix_vector_ph_mu_cg_", -1.3, x, 0.6, y))
+    np.testing.assert_array_equal(o.psolve(abi.PS_PHMU_JACOBI, abi.OP_PHMU, x),
+                                  tp.psolve("psolve_ph_mu_cg_", x))
+
+
+@needs_tp
+def test_op_double_gradient_neum(tp):
+    p = _params(tp)
+    tp.set_params(p)
+    rng = np.random.default_rng(104)
+    f = rng.standard_normal(tp.cell_shape)
+    m = rng.uniform(0, 1, tp.cell_shape)
+    a = oc.op_double_gradient_neum(p.nx, p.ny, p.dx ** 2, p.dz ** 2, f, m)
+    np.testing.assert_array_equal(a, tp.op_double_gradient_neum(f, m))
+
+
+@needs_tp
+def test_setup_tables(tp):
+    p = _params(tp)
+    p.xld, p.I_thickness, p.s_implicit = 0.05, 0.01, 1.5
+    tp.set_params(p)
+    tp.set_scalar("lambda_eta", 1.0)
+    tp.set_scalar("xlsT", 0.0)
+    tp.set_scalar("slipratio", 1.0)
+    tp.setup()
+    o = oc.Oracle(p)
+    np.testing.assert_array_equal(o.pre_lcc(), tp.array("pre_lcc", (p.ny + 1, p.nx + 1)))
+    np.testing.assert_array_equal(o.ph_lcc(), tp.array("ph_lcc", (p.ny, p.nx)))
+
+
+@needs_tp
+def test_scalar_helpers(tp):
+    rng = np.random.default_rng(105)
+    for _ in range(200):
+        phi, psi, ang = rng.uniform(-1.5, 1.5), rng.uniform(0, 1), rng.uniform(0, 180)
+        for mode in (1, 2):
+            assert oc.ch_bulk_energy(phi, psi, mode) == tp.ch_bulk_energy(phi, psi, mode)
+            assert oc.ch_bdr_condition(ang, phi, mode) == tp.ch_bdr_condition(ang, phi, mode)
+
+
+@needs_mf
+def test_scalar_helpers_mf():
+    mf = refpy.Ref("mf")
+    rng = np.random.default_rng(106)
+    for _ in range(200):
+        phi, psi = rng.uniform(-0.5, 1.5), rng.uniform(0, 1)
+        assert oc.ch_bulk_energy_mf(phi, psi) == mf.ch_bulk_energy(phi, psi)
+
+
+@needs_tp
+@pytest.mark.parametrize("n", [2, 3, 9, 17, 64, 129])
+def test_vcost(tp, n):
+    rng = np.random.default_rng(107 + n)
+    x = rng.standard_normal((3, n))
+    a, b = oc.vcost(x), tp.vcost(x)
+    assert np.max(np.abs(a - b)) <= 1e-12 * max(1.0, np.max(np.abs(b)))
+
+
+@needs_tp
+def test_dct_solve(tp):
+    rng = np.random.default_rng(108)
+    nr1, nz1 = 24, 15
+    f0 = rng.standard_normal((nz1 + 3, nr1 + 3))
+    cof = rng.uniform(1.0, 3.0, (nz1 + 1, nr1 + 1))
+    a = oc.dct_solve(nr1, nz1, f0, cof)
+    b = tp.dct_solve(nr1, nz1, f0, cof)
+    inner = (slice(0, nz1 + 3), slice(0, nr1 + 3))
+    assert np.max(np.abs(a[inner] - b[inner])) <= 1e-12 * np.max(np.abs(b))
+
+
+@needs_tp
+def test_rfords_and_esolp(tp):
+    rng = np.random.default_rng(109)
+    n = 33
+    aa, cc = rng.uniform(-1, 0, n), rng.uniform(-1, 0, n)
+    bb = rng.uniform(3, 4, n)
+    ff = rng.standard_normal(n)
+    np.testing.assert_array_equal(oc.rfords(ff, aa, bb, cc), tp.rfords(ff, aa, bb, cc))
+    nr1, nz1 = 20, 11
+    pa = rng.uniform(-1, 0, (nr1 + 1, nz1 + 1))
+    pc = rng.uniform(-1, 0, (nr1 + 1, nz1 + 1))
+    pb = rng.uniform(3, 4, (nr1 + 1, nz1 + 1))
+    b = rng.standard_normal((nz1 + 3, nr1 + 3))
+    a = oc.esolp_x(nr1, nz1, pa, pb, pc, b)
+    r = tp.esolp_x(nr1, nz1, pa, pb, pc, b)
+    inner = (slice(1, nz1 + 2), slice(1, nr1 + 2))
+    assert np.max(np.abs(a[inner] - r[inner])) <= 1e-12 * np.max(np.abs(r[inner]))
+
+
+@needs_mf
+def test_mf_phsolve_operator_and_dct_preconditioner():
+    mf = refpy.Ref("mf")
+    p = abi.default_params(mf.nr, mf.nz)
+    p.penalty_k = mf.penalty_k
+    p.xld, p.I_thickness, p.s_implicit, p.dt = 0.05, 0.02, 1.5, 2e-5
+    p.variant = abi.VARIANT_MF
+    mf.set_params(p)
+    mf.set_scalar("lambda_eta", 1.0)
+    mf.set_scalar("xlsT", 0.0)
+    mf.set_scalar("slipratio", 1.0)
+    mf.setup()
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(110)
+    for name, fid in (("ph_m_inner", abi.F_PH_M_INNER), ("ph_m_0", abi.F_PH_M_0)):
+        a = rng.uniform(0, 1, mf.cell_shape)
+        o.set_field(fid, a)
+        mf.array(name, mf.cell_shape)[...] = a
+    n = p.nx * p.ny
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    np.testing.assert_array_equal(o.matvec(abi.OP_PH, 0.8, x, 0.2, y),
+                                  mf.matvec("matrix_vector_cg_", 0.8, x, 0.2, y))
+    np.testing.assert_array_equal(o.ph_lcc(), mf.array("ph_lcc", (p.ny, p.nx)))
+    a = o.psolve(abi.PS_PH_DCT, abi.OP_PH, x)
+    b = mf.psolve("psolve_cg_", x)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
