@@ -78,39 +78,36 @@ def cpu_baseline(cfg, velo_iters: float, phmu_iters: float, budget_s: float = 20
     }
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="dropin", choices=["dropin"])
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args(argv)
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path "
-              f"is single-GPU, see DESIGN.md for how it would shard", file=sys.stderr)
-        return 2
+MODES = {
+    # name: (velocity preconditioner, CH preconditioner)
+    "dropin": ("PREC_REFERENCE", "CH_PREC_REFERENCE"),
+    "fp64dct": ("PREC_DCT64", "CH_PREC_BLOCK"),
+    "fp32": ("PREC_DCT32", "CH_PREC_BLOCK"),
+}
+
+
+def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False):
+    """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1)."""
     from drd_application_amd import abi, geometry
     from drd_application_amd.solver import GpuSolver
 
-    cfg = geometry.config(args.config)
-    p = cfg.params
-    log(f"config {cfg.name}: {p.nx}x{p.ny}, tol {p.tol:g}, dt {p.dt:g}")
+    p = cfg.params.copy()
+    vp, cp = MODES[mode]
+    p.velo_prec = getattr(abi, vp)
+    p.ch_prec = getattr(abi, cp)
     g = GpuSolver(p)
     geometry.load(g, cfg)
     g.synchronize()
     t = 0.0
-    warm = max(1, args.warmup)
-    for s in range(warm):
+    for s in range(max(1, warmup)):
         info = g.step(1 if s == 0 else 2, t)
         t += p.dt
-        log(f"warmup step {s}: phmu {info['phmu']['iter']} ({info['phmu']['info']}), "
+        log(f"[{mode}] warmup step {s}: phmu {info['phmu']['iter']} ({info['phmu']['info']}), "
             f"velo {info['velo']['iter']} ({info['velo']['info']})")
     g.synchronize()
     iters_v, iters_c, infos = [], [], []
     t0 = time.perf_counter()
-    for s in range(args.steps):
+    for _ in range(steps):
         info = g.step(2, t)
         t += p.dt
         iters_v.append(info["velo"]["iter"])
@@ -118,42 +115,84 @@ def main(argv=None) -> int:
         infos.append((info["phmu"]["info"], info["velo"]["info"]))
     g.synchronize()
     elapsed = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {elapsed:.3f} s; velo iters {iters_v}; phmu iters {iters_c}; "
-        f"info {infos}")
+    log(f"[{mode}] {steps} steps in {elapsed:.3f} s ({steps/elapsed:.3f} steps/s); velo iters {iters_v}; "
+        f"phmu iters {iters_c}; info {infos}")
+    out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "velo_iters": sum(iters_v) / steps, "phmu_iters": sum(iters_c) / steps,
+           "converged": all(a == 0 and b == 0 for a, b in infos)}
+    if keep:
+        return out, g
+    g.close()
+    return out, None
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--mode", default="fp32", choices=sorted(MODES))
+    ap.add_argument("--compare", default="fp64dct,dropin",
+                    help="comma-separated modes timed beside the headline mode ('' for none)")
+    ap.add_argument("--compare-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path "
+              f"is single-GPU, see DESIGN.md for how it would shard", file=sys.stderr)
+        return 2
+    from drd_application_amd import geometry
+
+    cfg = geometry.config(args.config)
+    p = cfg.params
+    log(f"config {cfg.name}: {p.nx}x{p.ny}, tol {p.tol:g}, dt {p.dt:g}")
+    head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True)
     ms_k, bytes_k = bench_kernel(g, 0, 50)
     achieved = bytes_k / (ms_k * 1e-3) / 1e9
     log(f"velocity matvec: {ms_k*1e3:.1f} us, {bytes_k/1e6:.1f} MB -> {achieved:.0f} GB/s")
-    mean_v = sum(iters_v) / len(iters_v)
-    mean_c = sum(iters_c) / len(iters_c)
+    kern = {}
+    for which, name in ((1, "phmu_matvec"), (2, "bicg_xr"), (3, "pressure")):
+        ms, by = bench_kernel(g, which, 20)
+        kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
+        log(f"{name}: {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")
+    g.close()
+    modes = {args.mode: head}
+    for m in [s for s in args.compare.split(",") if s and s != args.mode]:
+        modes[m], _ = run_mode(cfg, m, args.compare_steps, 1)
     cpu = None
     if not args.no_cpu:
+        ref = modes.get("dropin", head)
         try:
-            cpu = cpu_baseline(cfg, mean_v, mean_c)
+            cpu = cpu_baseline(cfg, ref["velo_iters"], ref["phmu_iters"])
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e}")
-    value = args.steps / elapsed
     result = {
         "metric": "solver steps/s at 1e-10 rel-res, learned+FP32 vs plain FP64 precond; HBM GB/s",
-        "value": value,
+        "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": warm,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{cfg.name} {p.nx}x{p.ny} two-phase CH+NS, tol {p.tol:g}",
-                   "mode": args.mode, "velo_iters_per_step": mean_v, "phmu_iters_per_step": mean_c},
+                   "mode": args.mode, "modes": modes,
+                   "speedup_vs_fp64dct": (head["steps_per_s"] / modes["fp64dct"]["steps_per_s"]
+                                          if "fp64dct" in modes and args.mode != "fp64dct" else None),
+                   "speedup_vs_dropin": (head["steps_per_s"] / modes["dropin"]["steps_per_s"]
+                                         if "dropin" in modes and args.mode != "dropin" else None),
+                   "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
                      "us_per_launch": ms_k * 1e3},
         "cpu_baseline": cpu,
     }
-    g.close()
     print(json.dumps(result))
     return 0
 

@@ -28,6 +28,13 @@ PREC_REFERENCE = 0
 PREC_IDENTITY = 1
 PREC_JACOBI = 2
 PREC_DCT32 = 3
+PREC_DCT64 = 4
+
+CH_PREC_REFERENCE = 0
+CH_PREC_BLOCK = 1
+
+KRYLOV_BICGSTAB = 0
+KRYLOV_FGMRES = 1
 
 OP_VELO1 = 0
 OP_VELO2 = 1
@@ -39,6 +46,8 @@ PS_IDENTITY = 1
 PS_PHMU_JACOBI = 2
 PS_PH_DCT = 3
 PS_VELO_DCT32 = 4
+PS_PHMU_BLOCK = 5
+PS_VELO_DCT64 = 6
 
 F_U, F_V, F_P, F_PH, F_RMU, F_RHO, F_ETA = 0, 1, 2, 3, 4, 5, 6
 F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10
@@ -65,7 +74,7 @@ class Params(C.Structure):
         ("phi_mode", C.c_int32), ("lr_bd_type", C.c_int32),
         ("velo_prec", C.c_int32), ("maxit_velo1", C.c_int32),
         ("maxit_velo2", C.c_int32), ("maxit_phmu", C.c_int32),
-        ("maxit_ph", C.c_int32),
+        ("maxit_ph", C.c_int32), ("ch_prec", C.c_int32), ("krylov", C.c_int32),
         ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
         ("re", C.c_double), ("ca", C.c_double), ("xld", C.c_double),
         ("s_implicit", C.c_double), ("I_thickness", C.c_double),

@@ -62,6 +62,8 @@ struct drd_ctx {
   double* uL_host = nullptr;              // pinned
   double* pn = nullptr;
   DctPlan* dct = nullptr;
+  VeloPrec* vprec = nullptr;  // fast-path velocity preconditioner (lazy)
+  bool vprec_fp32 = true;
   double* dct_work = nullptr;
   double cosa = 0, sqrt2 = 0;
 };
@@ -266,6 +268,7 @@ static void ctx_free(drd_ctx* c) {
   hipFree(c->pn);
   hipFree(c->dct_work);
   dct_plan_destroy(c->dct);
+  velo_prec_destroy(c->vprec);
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -295,6 +298,26 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
     return rc;
   }
   *out = c;
+  return 0;
+}
+
+extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
+  if (!c || !prm) return set_err(DRD_E_ARG, "null argument");
+  if (prm->nx != c->p.nx || prm->ny != c->p.ny) return set_err(DRD_E_ARG, "grid size differs from the context");
+  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
+    return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
+  if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
+  if (prm->variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "only the TP variant is built");
+  if (std::memcmp(prm, &c->p, sizeof(drd_params)) == 0) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->p = *prm;
+  c->s = make_scal(*prm);
+  hipFree(c->pre_lcc);
+  hipFree(c->ph_lcc);
+  c->pre_lcc = c->ph_lcc = nullptr;
+  velo_prec_destroy(c->vprec);  // its eigenvalues depend on dt, re, dx, dz
+  c->vprec = nullptr;
+  RCHK(build_tables(c));
   return 0;
 }
 
@@ -376,9 +399,25 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
   return 0;
 }
 
-// Prepares a preconditioner once per solve (Jacobi diagonal).
+// Prepares a preconditioner once per solve (Jacobi diagonal, DCT scaling).
 static int prec_setup(drd_ctx* c, int prec) {
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag);
+  if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64) {
+    const bool fp32 = (prec == DRD_PS_VELO_DCT32);
+    if (c->vprec && c->vprec_fp32 != fp32) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      velo_prec_destroy(c->vprec);
+      c->vprec = nullptr;
+    }
+    if (!c->vprec) {
+      int err = 0;
+      c->vprec = velo_prec_create(c->g, c->s, fp32, &err);
+      if (!c->vprec)
+        return set_err(err ? err : DRD_E_HIP, "DCT velocity preconditioner needs power-of-two nx, ny <= 4096");
+      c->vprec_fp32 = fp32;
+    }
+    velo_prec_setup(c->stream, c->vprec, c->g, c->s, coef_of(c));
+  }
   return 0;
 }
 
@@ -397,8 +436,19 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       launch_phmu_jacobi(c->stream, c->g, c->s, b, x, ks);
       *res = x;
       return 0;
+    case DRD_PS_PHMU_BLOCK:
+      if (op != DRD_OP_PHMU) break;
+      launch_phmu_block(c->stream, c->g, c->s, coef_of(c), b, x, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_VELO_DCT32:
+    case DRD_PS_VELO_DCT64:
+      if (op != DRD_OP_VELO2 || !c->vprec) break;
+      velo_prec_apply(c->stream, c->vprec, c->g, b, x, ks);
+      *res = x;
+      return 0;
   }
-  return set_err(DRD_E_UNSUPPORTED, "preconditioner not built");
+  return set_err(DRD_E_UNSUPPORTED, "preconditioner not built for this operator");
 }
 
 static int read_state(drd_ctx* c) {
@@ -481,6 +531,10 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return DRD_PS_IDENTITY;
     case DRD_PREC_JACOBI:
       return DRD_PS_VELO1_JACOBI;
+    case DRD_PREC_DCT32:  // built for the BDF2 operator; the first step keeps Jacobi
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
+    case DRD_PREC_DCT64:
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT64;
     default:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
   }
@@ -560,8 +614,9 @@ static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_
   a.cosa = c->cosa;
   a.sqrt2 = c->sqrt2;
   launch_phmu_rhs(c->stream, c->g, c->s, a, c->rhs, c->xv, c->pn);
-  RCHK(run_bicgstab(c, DRD_OP_PHMU, DRD_PS_PHMU_JACOBI, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu,
-                    c->p.tol, info, want_true));
+  const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK ? DRD_PS_PHMU_BLOCK : DRD_PS_PHMU_JACOBI;
+  RCHK(run_bicgstab(c, DRD_OP_PHMU, prec, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu, c->p.tol, info,
+                    want_true));
   launch_phmu_unpack(c->stream, c->g, c->s, c->xv, c->pn, ph_after, rmu_after);
   HIPCHK(hipGetLastError());
   return 0;

@@ -141,6 +141,16 @@ void launch_resid_norms(hipStream_t st, int64_t n, const double* b, const double
                         KState* scratch_ks);
 void launch_cell_axpy(hipStream_t st, int64_t n, const double* a, double* b);  // b = a + b
 
+// ---- fast-path preconditioners (kernels_prec.hip) ----
+void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
+                       const KState* ks);
+struct VeloPrec;
+VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err);
+void velo_prec_destroy(VeloPrec* pc);
+// diagonal scaling from the current coefficients (once per solve)
+void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s, const Coef& c);
+void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks);
+
 // ---- launchers (kernels_dct.hip) ----
 struct DctPlan;
 DctPlan* dct_plan_create(int nx, int ny);

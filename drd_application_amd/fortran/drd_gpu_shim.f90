@@ -1,0 +1,335 @@
+! drd_gpu_shim.f90 -- Fortran drop-in for the reference's implicit-solve
+! subroutines, calling the C-ABI of include/drd_gpu.h through ISO_C_BINDING.
+!
+! Build it next to the caller's own module `global` (TP/global.f90) with the
+! caller's real kind (ifort -r8, or flang -fdefault-real-8
+! -fdefault-double-8), link libdrd_gpu.so, and leave the reference's
+! VeloSolve_combined.f90, VeloSolve_2_combined.f90, PreSolve_init.f90,
+! PreSolve_2.f90, dct_fftw.f90 and ph_mu_solve.f90 off the link line.
+!
+! The subroutines keep the reference's names and dummy arguments.  Hidden
+! inputs are read from module `global` and copied to the device context on
+! every call; hidden outputs (uL, uR, coef_bot, coef_top, ph_M_0) are
+! written back.  Iteration counts are printed as the reference prints them,
+! and a non-converged Krylov solve is reported and the run continues, as in
+! the reference; a failure of the GPU library itself stops the program.
+
+module drd_gpu_c
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: DRD_F_U = 0, DRD_F_V = 1, DRD_F_P = 2, DRD_F_PH = 3, &
+       DRD_F_RMU = 4, DRD_F_RHO = 5, DRD_F_ETA = 6, DRD_F_BDR_PH = 7, DRD_F_BDR_PH_USED = 8, &
+       DRD_F_BDR_PH_PENALTY = 9, DRD_F_BDR_PH_GRAD = 10, DRD_F_PH_M_INNER = 11, &
+       DRD_F_PH_M_0 = 12, DRD_F_ANGLEB = 13, DRD_F_ANGLET = 14
+
+  ! drd_params (include/drd_gpu.h): 12 int32 then doubles, no padding
+  type, bind(C) :: drd_params_t
+     integer(c_int32_t) :: nx, ny, variant, phi_mode, lr_bd_type, velo_prec
+     integer(c_int32_t) :: maxit_velo1, maxit_velo2, maxit_phmu, maxit_ph, ch_prec, krylov
+     real(c_double) :: dx, dz, dt, re, ca, xld, s_implicit, I_thickness
+     real(c_double) :: ggx, ggz, uwT, uwB, xlsB, tol, penalty_k, angle, lambda_rho
+     real(c_double) :: bdr_ph_used_adjust, twall, UU, bdr_upp, bdr_low
+  end type drd_params_t
+
+  type, bind(C) :: drd_solve_info_t
+     integer(c_int32_t) :: iter, info
+     real(c_double) :: resid, true_resid
+  end type drd_solve_info_t
+
+  interface
+     subroutine drd_params_default(p, nx, ny) bind(C, name='drd_params_default')
+       import :: drd_params_t, c_int32_t
+       type(drd_params_t), intent(out) :: p
+       integer(c_int32_t), value :: nx, ny
+     end subroutine drd_params_default
+
+     integer(c_int) function drd_init(p, ctx) bind(C, name='drd_init')
+       import :: drd_params_t, c_ptr, c_int
+       type(drd_params_t), intent(in) :: p
+       type(c_ptr), intent(out) :: ctx
+     end function drd_init
+
+     integer(c_int) function drd_set_params(ctx, p) bind(C, name='drd_set_params')
+       import :: drd_params_t, c_ptr, c_int
+       type(c_ptr), value :: ctx
+       type(drd_params_t), intent(in) :: p
+     end function drd_set_params
+
+     integer(c_int) function drd_set_field(ctx, field, host) bind(C, name='drd_set_field')
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: field
+       real(c_double), intent(in) :: host(*)
+     end function drd_set_field
+
+     integer(c_int) function drd_get_field(ctx, field, host) bind(C, name='drd_get_field')
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: field
+       real(c_double), intent(out) :: host(*)
+     end function drd_get_field
+
+     integer(c_int) function drd_velosolve_combined(ctx, psi, u_after, v_after, rmu_after, &
+          ph_after, t, pf1, pf2, info) bind(C, name='drd_velosolve_combined')
+       import :: c_ptr, c_int, c_double, drd_solve_info_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: psi(*), rmu_after(*), ph_after(*), pf1(*), pf2(*)
+       real(c_double), intent(out) :: u_after(*), v_after(*)
+       real(c_double), value :: t
+       type(drd_solve_info_t), intent(out) :: info
+     end function drd_velosolve_combined
+
+     integer(c_int) function drd_velosolve_2_combined(ctx, psi1, psi2, u_pre, u_after, v_pre, &
+          v_after, ph_pre, ph_after, rmu_pre, rmu_after, t, pf1, pf2, info) &
+          bind(C, name='drd_velosolve_2_combined')
+       import :: c_ptr, c_int, c_double, drd_solve_info_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: psi1(*), psi2(*), u_pre(*), v_pre(*), ph_pre(*), ph_after(*)
+       real(c_double), intent(in) :: rmu_pre(*), rmu_after(*), pf1(*), pf2(*)
+       real(c_double), intent(out) :: u_after(*), v_after(*)
+       real(c_double), value :: t
+       type(drd_solve_info_t), intent(out) :: info
+     end function drd_velosolve_2_combined
+
+     integer(c_int) function drd_presolve_2(ctx, psi, coef, mean_div) bind(C, name='drd_presolve_2')
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: psi(*)
+       real(c_double), value :: coef
+       real(c_double), intent(out) :: mean_div
+     end function drd_presolve_2
+
+     integer(c_int) function drd_ph_mu_solve(ctx, ph_after, rmu_after, t, info) &
+          bind(C, name='drd_ph_mu_solve')
+       import :: c_ptr, c_int, c_double, drd_solve_info_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ph_after(*), rmu_after(*)
+       real(c_double), value :: t
+       type(drd_solve_info_t), intent(out) :: info
+     end function drd_ph_mu_solve
+
+     integer(c_int) function drd_phsolve(ctx, rmu_after, ph_after, t, info) bind(C, name='drd_phsolve')
+       import :: c_ptr, c_int, c_double, drd_solve_info_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: rmu_after(*), ph_after(*)
+       real(c_double), value :: t
+       type(drd_solve_info_t), intent(out) :: info
+     end function drd_phsolve
+
+     type(c_ptr) function drd_last_error() bind(C, name='drd_last_error')
+       import :: c_ptr
+     end function drd_last_error
+  end interface
+
+  type(c_ptr), save :: drd_shim_ctx = c_null_ptr
+
+contains
+
+  ! stops the program with the library's message when a call failed
+  subroutine drd_shim_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=256) :: text
+    integer :: i
+    if (rc == 0) return
+    call c_f_pointer(drd_last_error(), msg, [256])
+    text = ' '
+    do i = 1, 256
+       if (msg(i) == c_null_char) exit
+       text(i:i) = msg(i)
+    end do
+    write(*, '(a,a,a,i0,a,a)') 'drd_gpu: ', where, ' failed (', rc, '): ', trim(text)
+    error stop 1
+  end subroutine drd_shim_check
+
+end module drd_gpu_c
+
+
+! Creates the device context on first use (after finit/setup have set the
+! module scalars) and refreshes its scalars on every later call.
+subroutine drd_shim_ready()
+  use global
+  use drd_gpu_c
+  implicit none
+  type(drd_params_t) :: prm
+  integer(c_int) :: rc
+  call drd_params_default(prm, int(nr, c_int32_t), int(nz, c_int32_t))
+  prm%phi_mode = phi_mode
+  prm%lr_bd_type = ichar(lr_bd_type)
+  prm%dx = dr
+  prm%dz = dz
+  prm%dt = dt
+  prm%re = re
+  prm%ca = ca
+  prm%xld = xld
+  prm%s_implicit = s_implicit
+  prm%I_thickness = I_thickness
+  prm%ggx = ggx
+  prm%ggz = ggz
+  prm%uwT = uwT
+  prm%uwB = uwB
+  prm%xlsB = xlsB
+  prm%tol = tol
+  prm%penalty_k = penalty_k
+  prm%angle = angle
+  prm%lambda_rho = lambda_rho
+  prm%bdr_ph_used_adjust = bdr_ph_used_adjust
+  prm%twall = twall
+  prm%UU = UU
+  prm%bdr_upp = bdr_upp
+  prm%bdr_low = bdr_low
+  if (.not. c_associated(drd_shim_ctx)) then
+     rc = drd_init(prm, drd_shim_ctx)
+     call drd_shim_check(rc, 'drd_init')
+  else
+     rc = drd_set_params(drd_shim_ctx, prm)
+     call drd_shim_check(rc, 'drd_set_params')
+  end if
+end subroutine drd_shim_ready
+
+
+subroutine drd_shim_push(field, a)
+  use drd_gpu_c
+  implicit none
+  integer(c_int32_t), intent(in) :: field
+  real(c_double), intent(in) :: a(*)
+  call drd_shim_check(drd_set_field(drd_shim_ctx, field, a), 'drd_set_field')
+end subroutine drd_shim_push
+
+
+! TP/PreSolve_init.f90:1-16.  FFTW plans have no meaning here: the context
+! (and its DCT twiddles) is made at the first solve.
+subroutine PreSolve_init(nr_, nz_, plan, plan_back)
+  implicit none
+  integer nr_, nz_
+  integer*8 plan, plan_back
+  plan = int(nr_, 8)
+  plan_back = int(nz_, 8)
+end subroutine PreSolve_init
+
+
+! TP/PreSolve_2.f90:1-130
+subroutine PreSolve_2(plan, plan_back, psi, coef)
+  use global
+  use drd_gpu_c
+  implicit none
+  integer*8 plan, plan_back
+  real psi(-1:nr,-1:nz)
+  real coef
+  real(c_double) :: mean_div
+  call drd_shim_ready()
+  call drd_shim_push(DRD_F_U, u)
+  call drd_shim_push(DRD_F_V, v)
+  call drd_shim_check(drd_presolve_2(drd_shim_ctx, psi, real(coef, c_double), mean_div), 'PreSolve_2')
+  print *, 'output SUM SUM', mean_div
+end subroutine PreSolve_2
+
+
+! hidden inflow outputs uL, uR of TP/VeloSolve_combined.f90:266-306
+subroutine drd_shim_inflow(t)
+  use global
+  implicit none
+  real t, cf, tmp_hig
+  integer k
+  if (lr_bd_type /= 'd') return
+  cf = 1.0 - exp(-t/twall)
+  tmp_hig = bdr_upp - bdr_low
+  do k = 0, nz-1
+     if (z(0,k) > bdr_low .and. z(0,k) < bdr_upp) then
+        uL(k) = UU*( -(z(0,k) - bdr_low - tmp_hig/2)**2/tmp_hig/tmp_hig*4 + 1 )*cf
+     else
+        uL(k) = 0.0
+     end if
+     uR(k) = uL(k)
+  end do
+end subroutine drd_shim_inflow
+
+
+subroutine drd_shim_push_velo()
+  use global
+  use drd_gpu_c
+  implicit none
+  call drd_shim_push(DRD_F_U, u)
+  call drd_shim_push(DRD_F_V, v)
+  call drd_shim_push(DRD_F_P, p)
+  call drd_shim_push(DRD_F_RHO, rho)
+  call drd_shim_push(DRD_F_ETA, eta)
+  call drd_shim_push(DRD_F_BDR_PH_USED, bdr_ph_used)
+  call drd_shim_push(DRD_F_BDR_PH_PENALTY, bdr_ph_penalty)
+end subroutine drd_shim_push_velo
+
+
+! TP/VeloSolve_combined.f90:5-532
+subroutine VeloSolve_combined(psi, u_after, v_after, rmu_after, ph_after, t, &
+     particle_force1, particle_force2)
+  use global
+  use drd_gpu_c
+  implicit none
+  real psi(-1:nr,-1:nz)
+  real u_after(-1:nr+1,-1:nz), v_after(-1:nr,0:nz)
+  real rmu_after(-1:nr,-1:nz), ph_after(-1:nr,-1:nz), t
+  real particle_force1(-1:nr,-1:nz), particle_force2(-1:nr,-1:nz)
+  type(drd_solve_info_t) :: info
+  call drd_shim_ready()
+  call drd_shim_push_velo()
+  call drd_shim_check(drd_velosolve_combined(drd_shim_ctx, psi, u_after, v_after, rmu_after, &
+       ph_after, real(t, c_double), particle_force1, particle_force2, info), 'VeloSolve_combined')
+  call drd_shim_inflow(t)
+  write(*,*) 'uv 1 BICGSTAB no slip', info%iter
+  if (info%info /= 0) print *, 'uv 1 BICGSTAB no slip: error', info%iter, info%info
+end subroutine VeloSolve_combined
+
+
+! TP/VeloSolve_2_combined.f90:4-531
+subroutine VeloSolve_2_combined(psi1, psi2, u_pre, u_after, v_pre, v_after, &
+     ph_pre, ph_after, rmu_pre, rmu_after, t, particle_force1, particle_force2)
+  use global
+  use drd_gpu_c
+  implicit none
+  real psi1(-1:nr,-1:nz), psi2(-1:nr,-1:nz)
+  real u_pre(-1:nr+1,-1:nz), u_after(-1:nr+1,-1:nz)
+  real v_pre(-1:nr,0:nz), v_after(-1:nr,0:nz)
+  real rmu_pre(-1:nr,-1:nz), rmu_after(-1:nr,-1:nz)
+  real ph_pre(-1:nr,-1:nz), ph_after(-1:nr,-1:nz), t
+  real particle_force1(-1:nr,-1:nz), particle_force2(-1:nr,-1:nz)
+  type(drd_solve_info_t) :: info
+  call drd_shim_ready()
+  call drd_shim_push_velo()
+  call drd_shim_check(drd_velosolve_2_combined(drd_shim_ctx, psi1, psi2, u_pre, u_after, v_pre, &
+       v_after, ph_pre, ph_after, rmu_pre, rmu_after, real(t, c_double), particle_force1, &
+       particle_force2, info), 'VeloSolve_2_combined')
+  call drd_shim_inflow(t)
+  write(*,*) 'uv 2 BICGSTAB no slip', info%iter
+  if (info%info /= 0) print *, 'uv 2 BICGSTAB no slip: error', info%iter, info%info
+end subroutine VeloSolve_2_combined
+
+
+! TP/ph_mu_solve.f90:3-197
+subroutine ph_mu_solve(ph_after, rmu_after, t)
+  use global
+  use drd_gpu_c
+  implicit none
+  real ph_after(-1:nr,-1:nz), rmu_after(-1:nr,-1:nz), t
+  type(drd_solve_info_t) :: info
+  call drd_shim_ready()
+  call drd_shim_push(DRD_F_U, u)
+  call drd_shim_push(DRD_F_V, v)
+  call drd_shim_push(DRD_F_PH, ph)
+  call drd_shim_push(DRD_F_RMU, rmu)
+  call drd_shim_push(DRD_F_BDR_PH, bdr_ph)
+  call drd_shim_push(DRD_F_BDR_PH_USED, bdr_ph_used)
+  call drd_shim_push(DRD_F_BDR_PH_GRAD, bdr_ph_grad)
+  call drd_shim_push(DRD_F_PH_M_INNER, ph_M_inner)
+  call drd_shim_push(DRD_F_ANGLEB, angleB)
+  call drd_shim_push(DRD_F_ANGLET, angleT)
+  call drd_shim_check(drd_ph_mu_solve(drd_shim_ctx, ph_after, rmu_after, real(t, c_double), info), &
+       'ph_mu_solve')
+  ! hidden outputs: ph_M_0 (:31-35) and the wall coefficients (:123, :144)
+  call drd_shim_check(drd_get_field(drd_shim_ctx, DRD_F_PH_M_0, ph_M_0), 'drd_get_field')
+  coef_bot = (I_thickness/dz) / (I_thickness/dz)
+  coef_top = (I_thickness/dz) / (I_thickness/dz)
+  write(*,*) 'ph_mu_solve: \phi 1 BICGSTAB', info%iter, info%resid, tol
+  if (info%info /= 0) print *, 'error', info%info, tol
+end subroutine ph_mu_solve

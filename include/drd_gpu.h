@@ -47,7 +47,17 @@ extern "C" {
 #define DRD_PREC_JACOBI 2    /* TP/VeloSolve_combined.f90:679-741         */
 #define DRD_PREC_DCT32 3     /* FP32 DCT/DST fast-Poisson preconditioner
                                 with diagonal scaling (no reference
-                                counterpart; used inside FGMRES)          */
+                                counterpart; power-of-two nx, ny)         */
+#define DRD_PREC_DCT64 4     /* the same in FP64                          */
+
+/* ---- preconditioner choices for the Cahn-Hilliard phi/mu solve -------- */
+#define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
+#define DRD_CH_PREC_BLOCK 1     /* exact 2x2 per-cell block Jacobi (no
+                                   reference counterpart)                   */
+
+/* ---- outer Krylov method ------------------------------------------------ */
+#define DRD_KRYLOV_BICGSTAB 0 /* LIB/BiCGSTAB.f, reference stopping rule    */
+#define DRD_KRYLOV_FGMRES 1   /* restarted flexible GMRES (Saad 1993)       */
 
 /* ---- run-time copy of the reference's module `global` scalars -------- */
 typedef struct drd_params {
@@ -60,6 +70,8 @@ typedef struct drd_params {
   int32_t maxit_velo2;     /* 2000  TP/VeloSolve_2_combined.f90:413              */
   int32_t maxit_phmu;      /* 5000  TP/ph_mu_solve.f90:156                       */
   int32_t maxit_ph;        /* 500   MF/PhSolve.f90:72                            */
+  int32_t ch_prec;         /* DRD_CH_PREC_*                                      */
+  int32_t krylov;          /* DRD_KRYLOV_* for the velocity solve                */
   double dx, dz;           /* dr, dz                                             */
   double dt, re, ca, xld;  /* time step, Reynolds, capillary, mobility           */
   double s_implicit;       /* stabilisation s                                   */
@@ -131,6 +143,9 @@ int64_t drd_field_size(const drd_params* p, int32_t field);
  * context, DCT twiddles and Pre_Lcc / Ph_Lcc tables for prm.           */
 int drd_init(const drd_params* prm, drd_ctx** out);
 int drd_finalize(drd_ctx* ctx);
+/* replaces the scalars of an existing context (same nx, ny); rebuilds the
+ * eigenvalue tables when dt, dx, dz or the CH coefficients change          */
+int drd_set_params(drd_ctx* ctx, const drd_params* prm);
 const char* drd_last_error(void);
 int drd_abi_version(void);
 
@@ -187,6 +202,8 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 #define DRD_PS_PHMU_JACOBI 2  /* PSolve_PH_MU_CG TP/ph_mu_solve.f90:304-356         */
 #define DRD_PS_PH_DCT 3       /* PSolve_CG      MF/PhSolve.f90:296-313              */
 #define DRD_PS_VELO_DCT32 4   /* FP32 DCT/DST velocity preconditioner (addition)    */
+#define DRD_PS_PHMU_BLOCK 5   /* 2x2 block Jacobi for the phi/mu system (addition)   */
+#define DRD_PS_VELO_DCT64 6   /* FP64 DCT/DST velocity preconditioner (addition)    */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 

@@ -1,0 +1,190 @@
+"""The fast path (additions without a reference counterpart) on the GPU.
+
+* CH 2x2 block Jacobi: against a numpy block inverse whose 2x2 blocks are
+  probed from the oracle's operator (so the kernel's closed-form diagonal is
+  checked independently), 1e-12;
+* DCT/DST velocity preconditioner: against a scipy implementation of
+  S^-1 A0^-1 S^-1 with S probed from the oracle's operator; 1e-11 (FP64),
+  2e-5 relative (FP32);
+* solves and time steps with the fast preconditioners converge to the
+  reference-preconditioned oracle solution (unique solution; differences are
+  bounded by the conditioning times the tolerance).
+"""
+import numpy as np
+import pytest
+import scipy.fft as F
+
+from drd_application_amd import abi, geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def _pair(cfg):
+    from drd_application_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    g = GpuSolver(cfg.params)
+    geometry.load(g, cfg)
+    o = Oracle(cfg.params)
+    geometry.load(o, cfg)
+    return g, o
+
+
+def probe_diag(o, op, p):
+    """Diagonal (and for the CH system the 2x2 cell blocks) of the oracle
+    operator by colour probing (5-point stencils: 3 x 3 colouring)."""
+    nx, ny = p.nx, p.ny
+    n = abi.op_size(p, op)
+    if op == abi.OP_PHMU:
+        C = nx * ny
+        ii = np.arange(C) % nx
+        kk = np.arange(C) // nx
+        blocks = np.zeros((4, C))
+        for cx in range(3):
+            for cy in range(3):
+                m = ((ii % 3 == cx) & (kk % 3 == cy)).astype(float)
+                y = o.matvec(op, 1.0, np.concatenate([m, 0 * m]), 0.0, np.zeros(n))
+                blocks[0] += m * y[:C]
+                blocks[2] += m * y[C:]
+                y = o.matvec(op, 1.0, np.concatenate([0 * m, m]), 0.0, np.zeros(n))
+                blocks[1] += m * y[:C]
+                blocks[3] += m * y[C:]
+        return blocks
+    mn = (nx + 1) * ny
+    ic = np.concatenate([np.arange(mn) % (nx + 1), np.arange(n - mn) % nx])
+    kc = np.concatenate([np.arange(mn) // (nx + 1), np.arange(n - mn) // nx + 1])
+    d = np.zeros(n)
+    for cx in range(3):
+        for cy in range(3):
+            m = ((ic % 3 == cx) & (kc % 3 == cy)).astype(float)
+            d += m * o.matvec(op, 1.0, m, 0.0, np.zeros(n))
+    return d
+
+
+def test_ch_block_jacobi_matches_probed_blocks():
+    cfg = geometry.tp_case(64, 32, lambda_eta=10.0)
+    g, o = _pair(cfg)
+    p = cfg.params
+    o.ph_mu_rhs()  # sets ph_M_0 as ph_mu_solve does
+    g.set_field(abi.F_PH_M_0, o.get_field(abi.F_PH_M_0))
+    d11, d12, d21, d22 = probe_diag(o, abi.OP_PHMU, p)
+    floor = 1e-3 * p.dt * p.xld * (2 / p.dx ** 2 + 2 / p.dz ** 2)
+    d22 = np.maximum(d22, floor)
+    C = p.nx * p.ny
+    b = np.random.default_rng(3).standard_normal(2 * C)
+    det = d11 * d22 - d12 * d21
+    want = np.concatenate([(d22 * b[:C] - d12 * b[C:]) / det, (-d21 * b[:C] + d11 * b[C:]) / det])
+    got = g.psolve(abi.PS_PHMU_BLOCK, abi.OP_PHMU, b)
+    assert rel_l2(got, want) < 1e-12
+    g.close()
+
+
+def scipy_dct_prec(o, p, r):
+    nx, ny = p.nx, p.ny
+    mn = (nx + 1) * ny
+    n = abi.n_uv(p)
+    d = probe_diag(o, abi.OP_VELO2, p)
+    a0, dt = 1.5 * p.re, p.dt
+    d0 = np.concatenate([np.full(mn, a0 + dt * (4 / p.dx ** 2 + 2 / p.dz ** 2)),
+                         np.full(n - mn, a0 + dt * (2 / p.dx ** 2 + 4 / p.dz ** 2))])
+    s = np.sqrt(d / d0)
+    lx_u = 4 * np.sin(np.pi * np.arange(nx + 1) / (2 * nx)) ** 2 / p.dx ** 2
+    ly_u = 4 * np.sin(np.pi * (np.arange(ny) + 1) / (2 * ny)) ** 2 / p.dz ** 2
+    lam_u = a0 + dt * (2 * lx_u[None, :] + ly_u[:, None])
+    lx_v = 4 * np.sin(np.pi * np.arange(nx) / (2 * nx)) ** 2 / p.dx ** 2
+    ly_v = 4 * np.sin(np.pi * np.arange(1, ny) / (2 * ny)) ** 2 / p.dz ** 2
+    lam_v = a0 + dt * (lx_v[None, :] + 2 * ly_v[:, None])
+    r = r / s
+    ru, rv = r[:mn].reshape(ny, nx + 1), r[mn:].reshape(ny - 1, nx)
+    zu = F.idct(F.idst(F.dst(F.dct(ru, type=1, axis=1), type=2, axis=0) / lam_u, type=2, axis=0), type=1, axis=1)
+    zv = F.idct(F.idst(F.dst(F.dct(rv, type=2, axis=1), type=1, axis=0) / lam_v, type=1, axis=0), type=2, axis=1)
+    return np.concatenate([zu.ravel(), zv.ravel()]) / s
+
+
+@pytest.mark.parametrize("prec,tol", [(abi.PS_VELO_DCT64, 1e-11), (abi.PS_VELO_DCT32, 2e-5)])
+@pytest.mark.parametrize("nx,ny", [(64, 32), (128, 128)])
+def test_velocity_dct_preconditioner(prec, tol, nx, ny):
+    print(f"dct prec {prec} {nx}x{ny}", flush=True)
+    cfg = geometry.two_phase_config(nx, ny, 5, n_drops=3, r_drop=(3, 8), n_discs=2, r_disc=(3, 6),
+                                    lambda_eta=50.0, penalty=1e5, dt=5e-5, name="t")
+    g, o = _pair(cfg)
+    r = np.random.default_rng(4).standard_normal(abi.n_uv(cfg.params))
+    got = g.psolve(prec, abi.OP_VELO2, r)
+    want = scipy_dct_prec(o, cfg.params, r)
+    e = rel_l2(got, want)
+    print("rel err", e)
+    assert e < tol
+    g.close()
+
+
+@pytest.mark.parametrize("vprec", [abi.PREC_DCT32, abi.PREC_DCT64])
+def test_fast_velocity_solve_converges_to_reference_solution(vprec):
+    cfg = geometry.two_phase_config(128, 64, 9, n_drops=4, r_drop=(4, 10), n_discs=2, r_disc=(4, 8),
+                                    lambda_eta=50.0, penalty=1e5, dt=5e-5, tol=1e-10, name="t")
+    p = cfg.params
+    g, o = _pair(cfg)
+    cs = abi.cell_shape(p)
+    args = (np.zeros(cs), np.zeros(cs), np.zeros(abi.field_shape(p, abi.F_U)),
+            np.zeros(abi.field_shape(p, abi.F_V)), cfg.fields[abi.F_PH], cfg.fields[abi.F_PH],
+            cfg.fields[abi.F_RMU], cfg.fields[abi.F_RMU], p.dt)
+    uo, vo, io = o.VeloSolve_2_combined(*args)
+    ur, vr, ir = g.VeloSolve_2_combined(*args)
+    q = p.copy()
+    q.velo_prec = vprec
+    from drd_application_amd.solver import GpuSolver
+    gf = GpuSolver(q)
+    geometry.load(gf, cfg)
+    uf, vf, inf = gf.VeloSolve_2_combined(*args)
+    print("oracle", io, "gpu ref", ir, "gpu fast", inf)
+    assert inf["info"] == 0 and inf["true_resid"] < 1e-9
+    assert inf["iter"] < ir["iter"]
+    assert rel_l2(uf, uo) < 1e-7 and rel_l2(vf, vo) < 1e-7
+    g.close()
+    gf.close()
+
+
+def test_fast_ch_solve_converges_to_reference_solution():
+    cfg = geometry.tp_case(64, 64, tol=1e-10)
+    p = cfg.params
+    g, o = _pair(cfg)
+    po, mo, io = o.ph_mu_solve(0.0)
+    q = p.copy()
+    q.ch_prec = abi.CH_PREC_BLOCK
+    from drd_application_amd.solver import GpuSolver
+    gf = GpuSolver(q)
+    geometry.load(gf, cfg)
+    pf, mf, inf = gf.ph_mu_solve(0.0)
+    print("oracle", io, "gpu block", inf)
+    assert inf["info"] == 0 and inf["iter"] < io["iter"]
+    assert rel_l2(pf, po) < 1e-8 and rel_l2(mf, mo) < 1e-8
+    g.close()
+    gf.close()
+
+
+def test_fast_steps_track_reference_steps():
+    cfg = geometry.tp_case(64, 64, lambda_eta=10.0, tol=1e-10)
+    p = cfg.params
+    q = p.copy()
+    q.ch_prec = abi.CH_PREC_BLOCK
+    q.velo_prec = abi.PREC_DCT32
+    from drd_application_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    gf = GpuSolver(q)
+    geometry.load(gf, cfg)
+    o = Oracle(p)
+    geometry.load(o, cfg)
+    t = 0.0
+    for s in range(1, 6):
+        bdf = 1 if s == 1 else 2
+        i_f = gf.step(bdf, t)
+        i_o = o.step(bdf, t)
+        print(s, "fast", i_f["phmu"]["iter"], i_f["velo"]["iter"], "ref", i_o["phmu"]["iter"], i_o["velo"]["iter"])
+        t += p.dt
+    for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH):
+        e = rel_l2(gf.get_field(fid), o.get_field(fid))
+        print(abi.FIELD_NAMES[fid], e)
+        assert e < 1e-6
+    gf.close()
