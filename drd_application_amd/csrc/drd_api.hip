@@ -64,6 +64,9 @@ struct drd_ctx {
   DctPlan* dct = nullptr;
   VeloPrec* vprec = nullptr;  // fast-path velocity preconditioner (lazy)
   bool vprec_fp32 = true;
+  // FGMRES workspace (lazy): V (m+1 vectors), Z (m vectors), small arrays
+  int fg_m = 0;
+  double *fg_V = nullptr, *fg_Z = nullptr, *fg_h = nullptr, *fg_hh = nullptr;
   double* dct_work = nullptr;
   double cosa = 0, sqrt2 = 0;
 };
@@ -269,6 +272,10 @@ static void ctx_free(drd_ctx* c) {
   hipFree(c->dct_work);
   dct_plan_destroy(c->dct);
   velo_prec_destroy(c->vprec);
+  hipFree(c->fg_V);
+  hipFree(c->fg_Z);
+  hipFree(c->fg_h);
+  if (c->fg_hh) hipHostFree(c->fg_hh);
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -524,6 +531,142 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   return 0;
 }
 
+// ---- FGMRES(m) ----------------------------------------------------------------
+constexpr int kFgmresRestart = 30;  // restart length used by the velocity solve
+
+// Restarted flexible GMRES with right preconditioning (Saad, SIAM J. Sci.
+// Comput. 14(2), 1993).  Orthogonalisation by classical Gram-Schmidt with a
+// second pass when the first one cancels more than half of the norm
+// ("twice is enough"); the Hessenberg matrix, Givens rotations and the
+// small triangular solve live on the host, one synchronisation per inner
+// iteration.  Stops when the Arnoldi residual estimate reaches tol*||b||,
+// then confirms with the true residual (restarting if needed).
+static int fgmres_alloc(drd_ctx* c, int m) {
+  if (c->fg_m >= m && c->fg_V) return 0;
+  hipFree(c->fg_V);
+  hipFree(c->fg_Z);
+  hipFree(c->fg_h);
+  if (c->fg_hh) hipHostFree(c->fg_hh);
+  c->fg_V = c->fg_Z = c->fg_h = nullptr;
+  c->fg_hh = nullptr;
+  c->fg_m = 0;
+  RCHK(dalloc(&c->fg_V, (int64_t)(m + 1) * c->nmax));
+  RCHK(dalloc(&c->fg_Z, (int64_t)m * c->nmax));
+  RCHK(dalloc(&c->fg_h, 4 * (int64_t)(m + 2)));
+  HIPCHK(hipHostMalloc((void**)&c->fg_hh, 4 * (m + 2) * sizeof(double), hipHostMallocDefault));
+  c->fg_m = m;
+  return 0;
+}
+
+static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, double* x, int m, int maxit,
+                      double tol, drd_solve_info* info) {
+  if (n != op_size(c, op)) return set_err(DRD_E_ARG, "vector length does not match the operator");
+  if (m < 1 || m > 200) return set_err(DRD_E_ARG, "restart must be in [1, 200]");
+  RCHK(fgmres_alloc(c, m));
+  RCHK(prec_setup(c, prec));
+  uint32_t* ticket = &c->ks->ticket;
+  double* V = c->fg_V;
+  double* Z = c->fg_Z;
+  const int64_t ld = n;
+  double* hd = c->fg_h;  // [0, m+2): first pass, [m+2, 2m+4): second pass, then norms
+  double* hh = c->fg_hh;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  auto sync_read = [&](int count) -> int {
+    HIPCHK(hipMemcpyAsync(hh, hd, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  // ||b||
+  launch_multidot(c->stream, n, b, b, ld, 1, c->part, ticket, hd);
+  RCHK(sync_read(1));
+  const double bn = std::sqrt(hh[0]);
+  drd_solve_info loc{0, 1, tol, 0.0};
+  if (bn == 0.0) {
+    HIPCHK(hipMemsetAsync(x, 0, n * sizeof(double), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    loc = drd_solve_info{0, 0, 0.0, 0.0};
+    if (info) *info = loc;
+    return 0;
+  }
+  int it = 0;
+  for (;;) {
+    // r = b - A x into V_0, beta = ||r||
+    launch_copy(c->stream, n, b, V);
+    RCHK(apply_op(c, op, MV_PLAIN, x, V, -1.0, 1.0, nullptr, nullptr, nullptr));
+    launch_multidot(c->stream, n, V, V, ld, 1, c->part, ticket, hd);
+    RCHK(sync_read(1));
+    const double beta = std::sqrt(hh[0]);
+    loc.resid = beta / bn;
+    loc.true_resid = loc.resid;
+    if (loc.resid <= tol) {
+      loc.info = 0;
+      break;
+    }
+    if (it >= maxit) {
+      loc.info = 1;
+      break;
+    }
+    launch_scale_into(c->stream, n, V, beta, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jdone = 0;
+    for (int j = 0; j < m && it < maxit; ++j) {
+      double* vj = V + (size_t)j * ld;
+      double* zj = Z + (size_t)j * ld;
+      double* w = V + (size_t)(j + 1) * ld;
+      const double* zr = nullptr;
+      RCHK(apply_prec(c, prec, op, vj, zj, &zr, nullptr));
+      if (zr != zj) launch_copy(c->stream, n, zr, zj);
+      RCHK(apply_op(c, op, MV_PLAIN, zj, w, 1.0, 0.0, nullptr, nullptr, nullptr));
+      // pass 1: h = V^T w and ||w||^2 (w = V_{j+1} is contiguous with V)
+      launch_multidot(c->stream, n, w, V, ld, j + 2, c->part, ticket, hd);
+      launch_multiaxpy(c->stream, n, w, V, ld, j + 1, hd, -1.0, c->part, ticket, hd + 2 * (m + 2), 1);
+      RCHK(sync_read(2 * (m + 2) + 1));
+      double* hcol = &H[(size_t)(m + 1) * j];
+      for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
+      const double before = hh[j + 1];
+      double after = hh[2 * (m + 2)];
+      if (after < 0.25 * before) {  // pass 2
+        launch_multidot(c->stream, n, w, V, ld, j + 1, c->part, ticket, hd + (m + 2));
+        launch_multiaxpy(c->stream, n, w, V, ld, j + 1, hd + (m + 2), -1.0, c->part, ticket, hd + 2 * (m + 2), 1);
+        RCHK(sync_read(2 * (m + 2) + 1));
+        for (int i = 0; i <= j; ++i) hcol[i] += hh[(m + 2) + i];
+        after = hh[2 * (m + 2)];
+      }
+      const double hn = std::sqrt(after);
+      hcol[j + 1] = hn;
+      if (hn != 0.0) launch_scale_into(c->stream, n, w, hn, w);
+      for (int i = 0; i < j; ++i) {
+        const double t1 = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+        hcol[i] = t1;
+      }
+      const double den = std::hypot(hcol[j], hcol[j + 1]);
+      cs[j] = den == 0.0 ? 1.0 : hcol[j] / den;
+      sn[j] = den == 0.0 ? 0.0 : hcol[j + 1] / den;
+      hcol[j] = den;
+      hcol[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++it;
+      jdone = j + 1;
+      if (std::fabs(g[j + 1]) <= tol * bn || hn == 0.0) break;
+    }
+    for (int i = jdone - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int k = i + 1; k < jdone; ++k) s -= H[i + (size_t)(m + 1) * k] * y[k];
+      y[i] = s / H[i + (size_t)(m + 1) * i];
+    }
+    for (int i = 0; i < jdone; ++i) hh[i] = y[i];
+    HIPCHK(hipMemcpyAsync(hd, hh, jdone * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_multiaxpy(c->stream, n, x, Z, ld, jdone, hd, 1.0, c->part, ticket, nullptr, 0);
+  }
+  HIPCHK(hipGetLastError());
+  loc.iter = it;
+  if (info) *info = loc;
+  return 0;
+}
+
 // ---- velocity --------------------------------------------------------------
 static int velo_prec(const drd_ctx* c, int bdf) {
   switch (c->p.velo_prec) {
@@ -590,8 +733,11 @@ static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double*
   launch_velo_rhs(c->stream, bdf, c->g, c->s, a, c->rhs);
   launch_velo_pack(c->stream, c->g, c->f[DRD_F_U], c->f[DRD_F_V], c->xv);
   const int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
-  RCHK(run_bicgstab(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv,
-                    bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2, c->p.tol, info, want_true));
+  const int maxit = bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2;
+  if (c->p.krylov == DRD_KRYLOV_FGMRES && bdf == 2)
+    RCHK(run_fgmres(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv, kFgmresRestart, maxit, c->p.tol, info));
+  else
+    RCHK(run_bicgstab(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv, maxit, c->p.tol, info, want_true));
   launch_velo_unpack(c->stream, c->g, c->s, c->xv, c->uL, u_after, v_after);
   HIPCHK(hipGetLastError());
   return 0;
@@ -769,17 +915,18 @@ extern "C" int drd_bicgstab(drd_ctx* c, int32_t op, int32_t prec, int32_t n, con
 
 extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32_t restart, const double* b,
                           double* x, int32_t* iter, double* resid, int32_t* info) {
-  (void)c;
-  (void)op;
-  (void)prec;
-  (void)n;
-  (void)restart;
-  (void)b;
-  (void)x;
-  (void)iter;
-  (void)resid;
-  (void)info;
-  return set_err(DRD_E_UNSUPPORTED, "FGMRES is not built yet");
+  if (!c || !b || !x || !iter || !resid || !info) return set_err(DRD_E_ARG, "null argument");
+  if (n < 0) return set_err(DRD_E_ARG, "negative length");
+  RCHK(up(c, c->rhs, b, n));
+  RCHK(up(c, c->xv, x, n));
+  drd_solve_info loc{};
+  RCHK(run_fgmres(c, op, prec, n, c->rhs, c->xv, restart, *iter, *resid, &loc));
+  RCHK(down(c, x, c->xv, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *iter = loc.iter;
+  *resid = loc.resid;
+  *info = loc.info;
+  return 0;
 }
 
 // ---- measurement hook ---------------------------------------------------------

@@ -140,6 +140,12 @@ void launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, doub
 void launch_resid_norms(hipStream_t st, int64_t n, const double* b, const double* ax, double* part,
                         KState* scratch_ks);
 void launch_cell_axpy(hipStream_t st, int64_t n, const double* a, double* b);  // b = a + b
+// FGMRES: res[i] = (w, V_i); w += sign * sum h_i V_i (and ||w||^2); b = a / s
+void launch_multidot(hipStream_t st, int64_t n, const double* w, const double* V, int64_t ldv, int nv,
+                     double* part, uint32_t* ticket, double* res);
+void launch_multiaxpy(hipStream_t st, int64_t n, double* w, const double* V, int64_t ldv, int nv, const double* h,
+                      double sign, double* part, uint32_t* ticket, double* res, int want_norm);
+void launch_scale_into(hipStream_t st, int64_t n, const double* a, double s, double* b);
 
 // ---- fast-path preconditioners (kernels_prec.hip) ----
 void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,

@@ -163,7 +163,88 @@ __global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t n, const double*
   }
 }
 
+// ---- FGMRES orthogonalisation ------------------------------------------------
+// res[i] = (w, V_i) for i < nv, V_i = V + i*ldv.  One sweep reads w once and
+// each V_i once; partials reduced by the last workgroup in block order.
+__global__ __launch_bounds__(kBlock) void k_multidot(int64_t n, const double* __restrict__ w,
+                                                     const double* __restrict__ V, int64_t ldv, int nv,
+                                                     double* part, uint32_t* ticket, double* res) {
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double wl[kTile / kBlock];
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    wl[j] = q < n ? w[q] : 0.0;
+  }
+  for (int i = 0; i < nv; ++i) {
+    double a[1] = {0.0};
+#pragma unroll
+    for (int j = 0; j < kTile / kBlock; ++j) {
+      const int64_t q = base + j * kBlock + threadIdx.x;
+      if (q < n) a[0] += wl[j] * V[i * ldv + q];
+    }
+    block_sum<1>(a);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * nv + i] = a[0];
+  }
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  for (int i = 0; i < nv; ++i) {
+    double a[1] = {0.0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+      a[0] += __hip_atomic_load(part + (size_t)b * nv + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<1>(a);
+    if (threadIdx.x == 0) res[i] = a[0];
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// w = w + sign * sum_i h[i] V_i ; if want_norm, res[0] = ||w||^2
+__global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, double* __restrict__ w,
+                                                      const double* __restrict__ V, int64_t ldv, int nv,
+                                                      const double* __restrict__ h, double sign, double* part,
+                                                      uint32_t* ticket, double* res, int want_norm) {
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double acc[1] = {0.0};
+#pragma unroll
+  for (int j = 0; j < kTile / kBlock; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x;
+    if (q < n) {
+      double s = w[q];
+      for (int i = 0; i < nv; ++i) s = s + (sign * h[i]) * V[i * ldv + q];
+      w[q] = s;
+      acc[0] += s * s;
+    }
+  }
+  if (!want_norm) return;
+  double tot[1];
+  if (grid_sum<1>(acc, part, ticket, tot) && threadIdx.x == 0) res[0] = tot[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* a, double s, double* b) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < n) b[q] = a[q] / s;
+}
+
 static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void launch_multidot(hipStream_t s, int64_t n, const double* w, const double* V, int64_t ldv, int nv, double* part,
+                     uint32_t* ticket, double* res) {
+  hipLaunchKernelGGL(k_multidot, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, w, V, ldv, nv, part, ticket, res);
+}
+void launch_multiaxpy(hipStream_t s, int64_t n, double* w, const double* V, int64_t ldv, int nv, const double* h,
+                      double sign, double* part, uint32_t* ticket, double* res, int want_norm) {
+  hipLaunchKernelGGL(k_multiaxpy, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, w, V, ldv, nv, h, sign, part, ticket,
+                     res, want_norm);
+}
+void launch_scale_into(hipStream_t s, int64_t n, const double* a, double sc, double* b) {
+  hipLaunchKernelGGL(k_scale_into, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, sc, b);
+}
 
 void launch_init_norms(hipStream_t s, int64_t n, const double* x, const double* b, double* part, KState* ks) {
   hipLaunchKernelGGL(k_init_norms, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, b, part, ks);

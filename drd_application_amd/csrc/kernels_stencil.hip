@@ -109,8 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
     const int64_t q = base + j * kBlock + threadIdx.x;
     if (q >= n) break;
     if (MODE == MV_PLAIN) {
+      // y is not read when beta == 0 (as in BLAS), so it may be uninitialised
       const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
-      y[q] = alpha * z + beta * y[q];
+      y[q] = beta == 0.0 ? alpha * z : alpha * z + beta * y[q];
     } else if (MODE == MV_INIT) {
       double r = b[q];
       if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
@@ -403,8 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
     double z1, z2;
     if (MODE != MV_INIT || xnz) phmu_rows(g, s, bu, minner, m0, x, q, z1, z2);
     if (MODE == MV_PLAIN) {
-      y[q] = alpha * z1 + beta * y[q];
-      y[C + q] = alpha * z2 + beta * y[C + q];
+      y[q] = beta == 0.0 ? alpha * z1 : alpha * z1 + beta * y[q];
+      y[C + q] = beta == 0.0 ? alpha * z2 : alpha * z2 + beta * y[C + q];
     } else if (MODE == MV_INIT) {
       double r1 = b[q], r2 = b[C + q];
       if (xnz) {

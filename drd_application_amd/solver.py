@@ -140,6 +140,16 @@ class GpuSolver:
                                   C.byref(res), C.byref(info)))
         return x, it.value, res.value, info.value
 
+    def fgmres(self, op, prec, b, x0, maxit, tol, restart=30):
+        b = _f64(b)
+        x = _f64(x0).copy()
+        it = C.c_int32(maxit)
+        res = C.c_double(tol)
+        info = C.c_int32(0)
+        _check(lib().drd_fgmres(self._h, op, prec, len(b), restart, dp(b), dp(x), C.byref(it),
+                                C.byref(res), C.byref(info)))
+        return x, it.value, res.value, info.value
+
     # ---- the Fortran subroutines ----------------------------------------
     def VeloSolve_combined(self, psi, rmu_after, ph_after, t, pf1=None, pf2=None):
         ua = np.zeros(abi.field_shape(self.p, abi.F_U))

@@ -844,6 +844,111 @@ int oc_bicgstab_dense(int32_t n, const double* a, int32_t jacobi, const double* 
   return 0;
 }
 
+/* ---- FGMRES(m) -----------------------------------------------------------
+ * Restarted flexible GMRES with right preconditioning, Saad, SIAM J. Sci.
+ * Comput. 14(2) 1993, Algorithm 2.2, with modified Gram-Schmidt and Givens
+ * rotations.  LIB/fgmres.f wraps a closed library routine of this form (its
+ * call sites are commented out); this is written from the published
+ * algorithm.  Stops when the Arnoldi residual estimate |g_{j+1}| <=
+ * tol*||b||; the solution is then formed and its true residual checked, and
+ * the method restarts if that check fails and iterations remain.
+ * ITER = inner iterations; RESID = true ||b-Ax||/||b||; INFO 0 converged,
+ * 1 maximum iterations. */
+static int fgmres_core(int64_t n, const double* b, double* x, int32_t m, int32_t* iter, double* resid,
+                       mv_fn mv, ps_fn ps, void* u) {
+  int32_t maxit = *iter;
+  double tol = *resid;
+  double* V = (double*)malloc((size_t)n * (m + 1) * sizeof(double));
+  double* Z = (double*)malloc((size_t)n * m * sizeof(double));
+  double* H = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = (double*)calloc(m, sizeof(double)), *sn = (double*)calloc(m, sizeof(double));
+  double *g = (double*)calloc(m + 1, sizeof(double)), *y = (double*)calloc(m, sizeof(double));
+  double* r = (double*)malloc((size_t)n * sizeof(double));
+  double bn = dnrm2(n, b);
+  int info = 1, it = 0;
+  if (bn == 0.0) {
+    for (int64_t q = 0; q < n; ++q) x[q] = 0.0;
+    *iter = 0;
+    *resid = 0.0;
+    info = 0;
+    goto done;
+  }
+  for (;;) {
+    memcpy(r, b, (size_t)n * sizeof(double));
+    mv(u, -1.0, x, 1.0, r);
+    double beta = dnrm2(n, r);
+    *resid = beta / bn;
+    if (*resid <= tol) {
+      info = 0;
+      break;
+    }
+    if (it >= maxit) break;
+    for (int64_t q = 0; q < n; ++q) V[q] = r[q] / beta;
+    for (int k = 0; k <= m; ++k) g[k] = 0.0;
+    g[0] = beta;
+    int j, jdone = 0;
+    for (j = 0; j < m && it < maxit; ++j) {
+      double* vj = V + (size_t)j * n;
+      double* zj = Z + (size_t)j * n;
+      double* w = V + (size_t)(j + 1) * n;
+      ps(u, zj, vj);
+      mv(u, 1.0, zj, 0.0, w);
+      for (int i = 0; i <= j; ++i) {
+        double h = ddot(n, w, V + (size_t)i * n);
+        H[i + (size_t)(m + 1) * j] = h;
+        daxpy(n, -h, V + (size_t)i * n, w);
+      }
+      double hn = dnrm2(n, w);
+      H[j + 1 + (size_t)(m + 1) * j] = hn;
+      if (hn != 0.0)
+        for (int64_t q = 0; q < n; ++q) w[q] = w[q] / hn;
+      for (int i = 0; i < j; ++i) {  /* previous rotations */
+        double* hc = H + (size_t)(m + 1) * j;
+        double t1 = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+        hc[i] = t1;
+      }
+      double* hc = H + (size_t)(m + 1) * j;
+      double den = hypot(hc[j], hc[j + 1]);
+      cs[j] = den == 0.0 ? 1.0 : hc[j] / den;
+      sn[j] = den == 0.0 ? 0.0 : hc[j + 1] / den;
+      hc[j] = den;
+      hc[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++it;
+      jdone = j + 1;
+      if (fabs(g[j + 1]) <= tol * bn || hn == 0.0) break;
+    }
+    /* y = R^-1 g, x += Z y */
+    for (int i = jdone - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int k = i + 1; k < jdone; ++k) s -= H[i + (size_t)(m + 1) * k] * y[k];
+      y[i] = s / H[i + (size_t)(m + 1) * i];
+    }
+    for (int i = 0; i < jdone; ++i) daxpy(n, y[i], Z + (size_t)i * n, x);
+  }
+  *iter = it;
+done:
+  free(V); free(Z); free(H); free(cs); free(sn); free(g); free(y); free(r);
+  return info;
+}
+
+int oc_fgmres(oc_ctx* c, int32_t op, int32_t prec, int32_t n, int32_t restart, const double* b, double* x,
+              int32_t* iter, double* resid, int32_t* info) {
+  if (n != op_size(c, op) || restart < 1) return DRD_E_ARG;
+  op_bind bd = {c, op, prec};
+  *info = fgmres_core(n, b, x, restart, iter, resid, mv_bound, ps_bound, &bd);
+  return 0;
+}
+
+int oc_fgmres_dense(int32_t n, const double* a, int32_t jacobi, int32_t restart, const double* b, double* x,
+                    int32_t* iter, double* resid, int32_t* info) {
+  dense_bind d = {n, a, jacobi};
+  *info = fgmres_core(n, b, x, restart, iter, resid, mv_dense, ps_dense, &d);
+  return 0;
+}
+
 /* true residual ||b - A x|| / ||b|| (an addition; FP64) */
 static double true_resid(oc_ctx* c, int32_t op, const double* b, const double* x) {
   int64_t n = op_size(c, op);

@@ -67,6 +67,11 @@ int oc_bicgstab(oc_ctx* c, int32_t op, int32_t prec, int32_t n,
 int oc_bicgstab_dense(int32_t n, const double* a, int32_t jacobi,
                       const double* b, double* x, int32_t* iter,
                       double* resid, int32_t* info);
+/* restarted flexible GMRES, right preconditioned (Saad 1993) */
+int oc_fgmres(oc_ctx* c, int32_t op, int32_t prec, int32_t n, int32_t restart, const double* b, double* x,
+              int32_t* iter, double* resid, int32_t* info);
+int oc_fgmres_dense(int32_t n, const double* a, int32_t jacobi, int32_t restart, const double* b, double* x,
+                    int32_t* iter, double* resid, int32_t* info);
 int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info);
 
 /* transforms (definitions, O(n^2)) */

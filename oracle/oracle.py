@@ -55,6 +55,8 @@ def lib():
         L.oc_bicgstab.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P, P, I,
                                   P, I]
         L.oc_bicgstab_dense.argtypes = [C.c_int32, P, C.c_int32, P, P, I, P, I]
+        L.oc_fgmres.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
+        L.oc_fgmres_dense.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, P, P, I, P, I]
         L.oc_step.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.oc_dct2_2d.argtypes = [C.c_int32, C.c_int32, P]
         L.oc_dct3_2d.argtypes = [C.c_int32, C.c_int32, P]
@@ -141,6 +143,17 @@ class Oracle:
         info = C.c_int32(0)
         rc = lib().oc_bicgstab(self._h, op, prec, len(b), dp(b), dp(x), C.byref(it),
                                C.byref(res), C.byref(info))
+        assert rc == 0, rc
+        return x, it.value, res.value, info.value
+
+    def fgmres(self, op, prec, b, x0, maxit, tol, restart=30):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        it = C.c_int32(maxit)
+        res = C.c_double(tol)
+        info = C.c_int32(0)
+        rc = lib().oc_fgmres(self._h, op, prec, len(b), restart, dp(b), dp(x), C.byref(it),
+                             C.byref(res), C.byref(info))
         assert rc == 0, rc
         return x, it.value, res.value, info.value
 
@@ -253,6 +266,18 @@ def bicgstab_dense(a, b, x0, maxit, tol, jacobi=False):
     info = C.c_int32(0)
     lib().oc_bicgstab_dense(len(b), dp(af), int(jacobi), dp(b), dp(x), C.byref(it),
                             C.byref(res), C.byref(info))
+    return x, it.value, res.value, info.value
+
+
+def fgmres_dense(a, b, x0, maxit, tol, restart=30, jacobi=False):
+    af = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    it = C.c_int32(maxit)
+    res = C.c_double(tol)
+    info = C.c_int32(0)
+    lib().oc_fgmres_dense(len(b), dp(af), int(jacobi), restart, dp(b), dp(x), C.byref(it),
+                          C.byref(res), C.byref(info))
     return x, it.value, res.value, info.value
 
 

@@ -146,6 +146,28 @@ def test_fast_velocity_solve_converges_to_reference_solution(vprec):
     gf.close()
 
 
+@pytest.mark.parametrize("prec", [abi.PS_IDENTITY, abi.PS_VELO_DCT32])
+def test_fgmres_matches_oracle(prec):
+    """drd_fgmres (CGS twice-is-enough) against the oracle's FGMRES (MGS) on
+    the velocity operator: same solution, close iteration counts."""
+    cfg = geometry.two_phase_config(64, 32, 13, n_drops=2, r_drop=(3, 6), n_discs=1, r_disc=(3, 5),
+                                    lambda_eta=20.0, penalty=1e5, dt=5e-5, name="t")
+    p = cfg.params
+    g, o = _pair(cfg)
+    n = abi.n_uv(p)
+    b = np.random.default_rng(14).standard_normal(n)
+    xg, itg, rg, ig = g.fgmres(abi.OP_VELO2, prec, b, np.zeros(n), 3000, 1e-10, restart=20)
+    if prec == abi.PS_IDENTITY:
+        xo, ito, ro, io = o.fgmres(abi.OP_VELO2, prec, b, np.zeros(n), 3000, 1e-10, restart=20)
+        assert abs(itg - ito) <= 2
+    else:
+        xo, ito, ro, io = o.bicgstab(abi.OP_VELO2, abi.PS_IDENTITY, b, np.zeros(n), 3000, 1e-12)
+    print("gpu", itg, rg, ig, "oracle", ito, ro, io)
+    assert ig == 0 and rg <= 1e-10
+    assert rel_l2(xg, xo) < 1e-7
+    g.close()
+
+
 def test_fast_ch_solve_converges_to_reference_solution():
     cfg = geometry.tp_case(64, 64, tol=1e-10)
     p = cfg.params

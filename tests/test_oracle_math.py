@@ -137,6 +137,38 @@ def test_bicgstab_exit_paths():
     assert oc.bicgstab_dense(a, b, np.zeros(n), 0, 1e-8)[3] == -3
 
 
+@pytest.mark.parametrize("m", [1, 5, 80])
+def test_fgmres_dense_matches_lu(m):
+    """FGMRES(m) (Saad 1993) converges to the LU solution; with m >= n it is
+    full GMRES and terminates in at most n inner iterations."""
+    n = 40
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((n, n)) + 6 * np.eye(n)
+    b = rng.standard_normal(n)
+    xs = np.linalg.solve(a, b)
+    x, it, res, info = oc.fgmres_dense(a, b, np.zeros(n), 2000, 1e-12, restart=m)
+    assert info == 0 and res <= 1e-12
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-10
+    if m >= n:
+        assert it <= n
+    # the true residual is what RESID reports
+    assert abs(res - np.linalg.norm(b - a @ x) / np.linalg.norm(b)) < 1e-14
+
+
+def test_fgmres_exit_paths():
+    n = 10
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal((n, n)) + 5 * np.eye(n)
+    b = rng.standard_normal(n)
+    x, it, res, info = oc.fgmres_dense(a, b, np.zeros(n), 3, 1e-14, restart=10)
+    assert (it, info) == (3, 1)
+    x, it, res, info = oc.fgmres_dense(a, np.zeros(n), np.ones(n), 5, 1e-10)
+    assert (it, info, res) == (0, 0, 0.0) and not x.any()
+    xs = np.linalg.solve(a, b)
+    x, it, res, info = oc.fgmres_dense(a, b, xs, 5, 1e-10)
+    assert (it, info) == (0, 0)
+
+
 # ---- operator invariants ----------------------------------------------------------
 def _coef_oracle(nx, ny, seed, lr="n"):
     p = abi.default_params(nx, ny)
