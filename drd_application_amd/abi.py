@@ -48,6 +48,7 @@ PS_PH_DCT = 3
 PS_VELO_DCT32 = 4
 PS_PHMU_BLOCK = 5
 PS_VELO_DCT64 = 6
+PS_VELO_ESOLP = 7
 
 F_U, F_V, F_P, F_PH, F_RMU, F_RHO, F_ETA = 0, 1, 2, 3, 4, 5, 6
 F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10

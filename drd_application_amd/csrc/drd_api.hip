@@ -67,6 +67,9 @@ struct drd_ctx {
   // FGMRES workspace (lazy): V (m+1 vectors), Z (m vectors), small arrays
   int fg_m = 0;
   double *fg_V = nullptr, *fg_Z = nullptr, *fg_h = nullptr, *fg_hh = nullptr;
+  // PhSolve / VCOST / ESOLP (lazy)
+  Dct1Tab *d1_nx = nullptr, *d1_ny = nullptr, *d1_nx1 = nullptr;
+  double *ph_f2 = nullptr, *ph_t = nullptr, *aux_work = nullptr, *esolp_tabs = nullptr;
   double* dct_work = nullptr;
   double cosa = 0, sqrt2 = 0;
 };
@@ -193,6 +196,57 @@ static int build_tables(drd_ctx* c) {
   RCHK(dalloc(&c->ph_lcc, (int64_t)ph.size()));
   HIPCHK(hipMemcpy(c->pre_lcc, pre.data(), pre.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->ph_lcc, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+  // ESOLP tables TP/setup.f90:157-235 (BDF1 set).  t1 keeps the value of the
+  // last loop iteration (:167, :203), (pi*nx/nx)/2 and (pi*(nx-1)/(nx-1))/2,
+  // overwriting the mass shift of :159 as the reference does.
+  {
+    const int64_t nu_t = (int64_t)ny * (nx + 1), nv_t = (int64_t)(ny - 1) * nx, nt = nu_t + nv_t;
+    std::vector<double> ta(nt), tb(nt), tc(nt), fx(nx + 1);
+    const double xb = (2.0 * 0.0 - p.dz) / (2.0 * 0.0 + p.dz);  // no-slip: xlsB = xlsT = 0
+    double t1 = 0.0;
+    for (int i = 0; i <= nx; ++i) {
+      t1 = kPiG * i / nx / 2.;
+      const double sn = std::sin(t1);
+      fx[i] = 4.0 * p.dt * (sn * sn) / dr2;
+    }
+    for (int k = 0; k <= nx; ++k)
+      for (int i = 1; i <= ny; ++i) {
+        const int64_t q = (i - 1) + (int64_t)ny * k;
+        if (i == 1) {
+          ta[q] = 0.0;
+          tb[q] = t1 + ((2.0 - xb) * p.dt / dz2 + fx[k]);
+          tc[q] = (-1.0 * p.dt / dz2);
+        } else if (i == ny) {
+          ta[q] = (-1.0 * p.dt / dz2);
+          tb[q] = t1 + ((2.0 - xb) * p.dt / dz2 + fx[k]);
+          tc[q] = 0.0;
+        } else {
+          ta[q] = (-p.dt / dz2);
+          tb[q] = t1 + (2.0 * p.dt / dz2 + fx[k]);
+          tc[q] = (-p.dt / dz2);
+        }
+      }
+    for (int i = 0; i < nx; ++i) {
+      t1 = kPiG * i / (nx - 1) / 2.;
+      const double sn = std::sin(t1);
+      fx[i] = 4.0 * p.dt * (sn * sn) / dr2;
+    }
+    for (int k = 0; k < nx; ++k)
+      for (int i = 1; i <= ny - 1; ++i) {
+        const int64_t q = nu_t + (i - 1) + (int64_t)(ny - 1) * k;
+        ta[q] = (i == 1) ? 0.0 : -p.dt / dz2;
+        tb[q] = t1 + 2.0 * p.dt / dz2 + fx[k];
+        tc[q] = (i == ny - 1) ? 0.0 : -p.dt / dz2;
+        if (i == 1) tc[q] = -1.0 * p.dt / dz2;
+        if (i == ny - 1) ta[q] = -1.0 * p.dt / dz2;
+      }
+    hipFree(c->esolp_tabs);
+    c->esolp_tabs = nullptr;
+    RCHK(dalloc(&c->esolp_tabs, 3 * nt));
+    HIPCHK(hipMemcpy(c->esolp_tabs, ta.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->esolp_tabs + nt, tb.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->esolp_tabs + 2 * nt, tc.data(), nt * sizeof(double), hipMemcpyHostToDevice));
+  }
   c->cosa = std::cos(p.angle * kPiG / 180);
   c->sqrt2 = std::sqrt(2.0);
   return 0;
@@ -276,6 +330,13 @@ static void ctx_free(drd_ctx* c) {
   hipFree(c->fg_Z);
   hipFree(c->fg_h);
   if (c->fg_hh) hipHostFree(c->fg_hh);
+  dct1_table_destroy(c->d1_nx);
+  dct1_table_destroy(c->d1_ny);
+  dct1_table_destroy(c->d1_nx1);
+  hipFree(c->ph_f2);
+  hipFree(c->ph_t);
+  hipFree(c->aux_work);
+  hipFree(c->esolp_tabs);
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
@@ -387,11 +448,30 @@ static int64_t op_size(const drd_ctx* c, int op) {
   return -1;
 }
 
+// resources of PhSolve, its VCOST preconditioner and ESOLP, made on first use
+static int aux_ready(drd_ctx* c) {
+  if (c->aux_work) return 0;
+  const Grid& g = c->g;
+  c->d1_nx = dct1_table_create(g.nx);
+  c->d1_ny = dct1_table_create(g.ny);
+  c->d1_nx1 = dct1_table_create(g.nx + 1);
+  if (!c->d1_nx || !c->d1_ny || !c->d1_nx1) return set_err(DRD_E_HIP, "VCOST tables");
+  RCHK(dalloc(&c->ph_f2, g.nc()));
+  RCHK(dalloc(&c->ph_t, g.ncell()));
+  RCHK(dalloc(&c->aux_work, std::max<int64_t>(2 * g.ncell(), 3 * g.nuv())));
+  return 0;
+}
+
 // y = alpha*A*x + beta*y (device pointers), or a fused Krylov mode
 static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, double alpha, double beta,
                     double* a0, const double* b, KState* ks) {
   const Coef k = coef_of(c);
   switch (op) {
+    case DRD_OP_PH:
+      RCHK(aux_ready(c));
+      launch_ph_matvec(c->stream, mode, c->g, c->s, k.m0, k.minner, x, c->ph_t, y, alpha, beta, a0, b, c->part,
+                       ks);
+      break;
     case DRD_OP_VELO1:
     case DRD_OP_VELO2:
       launch_velo_matvec(c->stream, op == DRD_OP_VELO1 ? 1 : 2, mode, c->g, c->s, k, x, y, alpha, beta, a0, b,
@@ -452,6 +532,18 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
     case DRD_PS_VELO_DCT64:
       if (op != DRD_OP_VELO2 || !c->vprec) break;
       velo_prec_apply(c->stream, c->vprec, c->g, b, x, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_PH_DCT:
+      if (op != DRD_OP_PH) break;
+      RCHK(aux_ready(c));
+      launch_ph_dct_prec(c->stream, c->g, c->d1_nx, c->d1_ny, c->ph_lcc, b, x, c->aux_work, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_VELO_ESOLP:
+      if (op != DRD_OP_VELO1 && op != DRD_OP_VELO2) break;
+      RCHK(aux_ready(c));
+      launch_esolp(c->stream, c->g, c->d1_nx1, c->d1_nx, c->esolp_tabs, b, x, c->aux_work, ks);
       *res = x;
       return 0;
   }
@@ -860,13 +952,30 @@ extern "C" int drd_ph_mu_solve(drd_ctx* c, double* ph_after, double* rmu_after, 
   return 0;
 }
 
+// PhSolve MF/PhSolve.f90:1-126 on the device: RHS, BiCGSTAB(ITER<=500) with
+// the DCT-I preconditioner, rmu_after and the phi recovery.
+static int ph_solve_dev(drd_ctx* c, double* rmu_after, double* ph_after, drd_solve_info* info, bool want_true) {
+  RCHK(aux_ready(c));
+  launch_ph_rhs(c->stream, c->g, c->s, c->f[DRD_F_PH], c->f[DRD_F_U], c->f[DRD_F_V], c->f[DRD_F_BDR_PH],
+                c->f[DRD_F_BDR_PH_GRAD], c->f[DRD_F_PH_M_INNER], c->f[DRD_F_RMU], c->cosa, c->sqrt2, c->ph_f2,
+                c->rhs, c->xv);
+  RCHK(run_bicgstab(c, DRD_OP_PH, DRD_PS_PH_DCT, c->g.ncell(), c->rhs, c->xv, c->p.maxit_ph, c->p.tol, info,
+                    want_true));
+  launch_ph_finish(c->stream, c->g, c->s, c->xv, c->ph_f2, c->f[DRD_F_PH_M_0], rmu_after, ph_after);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int drd_phsolve(drd_ctx* c, double* rmu_after, double* ph_after, double t, drd_solve_info* info) {
-  (void)c;
-  (void)rmu_after;
-  (void)ph_after;
+  if (!c || !rmu_after || !ph_after) return set_err(DRD_E_ARG, "null argument");
   (void)t;
-  (void)info;
-  return set_err(DRD_E_UNSUPPORTED, "PhSolve (MF variant) is not built");
+  drd_solve_info loc{};
+  RCHK(ph_solve_dev(c, c->rmu_after, c->ph_after, &loc, true));
+  RCHK(down(c, rmu_after, c->rmu_after, c->g.nc()));
+  RCHK(down(c, ph_after, c->ph_after, c->g.nc()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
+  return 0;
 }
 
 extern "C" int drd_matvec(drd_ctx* c, int32_t op, double alpha, const double* x, double beta, double* y) {

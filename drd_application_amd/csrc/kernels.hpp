@@ -157,6 +157,24 @@ void velo_prec_destroy(VeloPrec* pc);
 void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s, const Coef& c);
 void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks);
 
+// ---- PhSolve, VCOST and ESOLP (kernels_phsolve.hip) ----
+struct Dct1Tab;
+Dct1Tab* dct1_table_create(int n);  // VCOST of lines of n points
+void dct1_table_destroy(Dct1Tab* t);
+void launch_ph_dct_prec(hipStream_t s, const Grid& g, const Dct1Tab* tx, const Dct1Tab* ty, const double* ph_lcc,
+                        const double* b, double* x, double* work, const KState* ks);
+void launch_ph_rhs(hipStream_t st, const Grid& g, const Scal& s, const double* ph, const double* u, const double* v,
+                   const double* bdr, const double* bg, const double* minner, const double* rmu, double cosa,
+                   double sqrt2, double* f2, double* rhs, double* x0);
+void launch_ph_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const double* m0,
+                      const double* minner, const double* x, double* t, double* y, double alpha, double beta,
+                      double* a0, const double* b, double* part, KState* ks);
+void launch_ph_finish(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* f2,
+                      const double* m0, double* mu, double* ph);
+// tabs: [ua|va], [ub|vb], [uc|vc], each table (1:n, 0:ld-1) column-major
+void launch_esolp(hipStream_t s, const Grid& g, const Dct1Tab* tu, const Dct1Tab* tv, const double* tabs,
+                  const double* b, double* x, double* work, const KState* ks);
+
 // ---- launchers (kernels_dct.hip) ----
 struct DctPlan;
 DctPlan* dct_plan_create(int nx, int ny);

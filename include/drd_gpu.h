@@ -204,6 +204,8 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 #define DRD_PS_VELO_DCT32 4   /* FP32 DCT/DST velocity preconditioner (addition)    */
 #define DRD_PS_PHMU_BLOCK 5   /* 2x2 block Jacobi for the phi/mu system (addition)   */
 #define DRD_PS_VELO_DCT64 6   /* FP64 DCT/DST velocity preconditioner (addition)    */
+#define DRD_PS_VELO_ESOLP 7   /* PSolve_uv_CG PD/VeloSolve_combined.f90:697-807
+                                 (ESOLP_x TP/ESOLP.f90 on both blocks)            */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 

@@ -34,6 +34,8 @@ struct oc_ctx {
   double* f[DRD_F_COUNT];
   double* pre_lcc; /* (0:nx,0:ny)       TP/setup.f90:122-131 */
   double* ph_lcc;  /* (0:nx-1,0:ny-1)   TP/setup.f90:149-155 */
+  /* ESOLP tables TP/setup.f90:157-235: ua..uc (1:ny,0:nx), va..vc (1:ny-1,0:nx-1) */
+  double *ua, *ub, *uc, *va, *vb, *vc;
   double coef_bot, coef_top; /* hidden outputs of ph_mu_solve     */
   double* uL;      /* (0:ny) hidden inflow profile TP/VeloSolve_2_combined.f90:289 */
   double* uR;
@@ -168,10 +170,73 @@ int oc_init(const drd_params* prm, oc_ctx** out) {
       c->ph_lcc[i + (size_t)nx * j] = 1. + p->xld * p->dt * p->s_implicit * temp / p->I_thickness +
                                        p->xld * p->I_thickness * p->dt * (temp * temp);
     }
+  /* ESOLP tables TP/setup.f90:157-235 (BDF1 set).  The mass shift t1 set at
+   * :159 is overwritten by the loops at :167 and :203, so the tables use the
+   * last loop value, (pi*nx/nx)/2 and (pi*(nx-1)/(nx-1))/2. */
+  {
+    double t1 = 0.0;
+    double slip = 1.0; /* slipratio; xlsB = xlsT = 0 on the no-slip path */
+    double xb = 0.0 * (1.0 + slip) / 2.0, xt = 0.0 * (1.0 + slip) / 2.0;
+    xb = (2.0 * xb - p->dz) / (2.0 * xb + p->dz);
+    xt = (2.0 * xt - p->dz) / (2.0 * xt + p->dz);
+    c->ua = (double*)calloc((size_t)ny * (nx + 1), sizeof(double));
+    c->ub = (double*)calloc((size_t)ny * (nx + 1), sizeof(double));
+    c->uc = (double*)calloc((size_t)ny * (nx + 1), sizeof(double));
+    c->va = (double*)calloc((size_t)(ny - 1) * nx, sizeof(double));
+    c->vb = (double*)calloc((size_t)(ny - 1) * nx, sizeof(double));
+    c->vc = (double*)calloc((size_t)(ny - 1) * nx, sizeof(double));
+    double* fx = (double*)calloc((size_t)nx + 1, sizeof(double));
+    for (int i = 0; i <= nx; ++i) {
+      t1 = PI_G * i / nx / 2.;
+      double s = sin(t1);
+      fx[i] = 4.0 * p->dt * (s * s) / dr2;
+    }
+#define TU(a, i, k) a[((i) - 1) + (size_t)ny * (k)]
+    for (int k = 0; k <= nx; ++k) {
+      TU(c->ua, 1, k) = 0.0;
+      TU(c->ub, 1, k) = t1 + ((2.0 - xb) * p->dt / dz2 + fx[k]);
+      TU(c->uc, 1, k) = (-1.0 * p->dt / dz2);
+      for (int i = 2; i <= ny - 1; ++i) {
+        TU(c->ua, i, k) = (-p->dt / dz2);
+        TU(c->ub, i, k) = t1 + (2.0 * p->dt / dz2 + fx[k]);
+        TU(c->uc, i, k) = (-p->dt / dz2);
+      }
+      TU(c->ua, ny, k) = (-1.0 * p->dt / dz2);
+      TU(c->ub, ny, k) = t1 + ((2.0 - xt) * p->dt / dz2 + fx[k]);
+      TU(c->uc, ny, k) = 0.0;
+    }
+#undef TU
+    for (int i = 0; i < nx; ++i) {
+      t1 = PI_G * i / (nx - 1) / 2.;
+      double s = sin(t1);
+      fx[i] = 4.0 * p->dt * (s * s) / dr2;
+    }
+#define TV(a, i, k) a[((i) - 1) + (size_t)(ny - 1) * (k)]
+    for (int k = 0; k < nx; ++k) {
+      TV(c->va, 1, k) = 0.0;
+      TV(c->vb, 1, k) = t1 + 2.0 * p->dt / dz2 + fx[k];
+      TV(c->vc, 1, k) = -1.0 * p->dt / dz2;
+      for (int i = 2; i <= ny - 2; ++i) {
+        TV(c->va, i, k) = -p->dt / dz2;
+        TV(c->vb, i, k) = t1 + 2.0 * p->dt / dz2 + fx[k];
+        TV(c->vc, i, k) = -p->dt / dz2;
+      }
+      TV(c->va, ny - 1, k) = -1.0 * p->dt / dz2;
+      TV(c->vb, ny - 1, k) = t1 + 2.0 * p->dt / dz2 + fx[k];
+      TV(c->vc, ny - 1, k) = 0.0;
+    }
+#undef TV
+    free(fx);
+  }
   free(fmx);
   free(fmy);
   *out = c;
   return 0;
+}
+
+const double* oc_esolp_table(oc_ctx* c, int32_t which) {
+  const double* t[6] = {c->ua, c->ub, c->uc, c->va, c->vb, c->vc};
+  return (which >= 0 && which < 6) ? t[which] : NULL;
 }
 
 int oc_finalize(oc_ctx* c) {
@@ -179,6 +244,7 @@ int oc_finalize(oc_ctx* c) {
   for (int f = 0; f < DRD_F_COUNT; ++f) free(c->f[f]);
   free(c->w1); free(c->w2); free(c->w3); free(c->w4); free(c->w5); free(c->w6);
   free(c->work); free(c->uL); free(c->uR); free(c->pre_lcc); free(c->ph_lcc);
+  free(c->ua); free(c->ub); free(c->uc); free(c->va); free(c->vb); free(c->vc);
   free(c);
   return 0;
 }
@@ -713,10 +779,22 @@ static int64_t op_size(const oc_ctx* c, int32_t op) {
   return -1;
 }
 
+/* PSolve_uv_CG of PD/VeloSolve_combined.f90:697-807: ESOLP_x on the u block
+ * (nr, nz-1; tables ua..uc) and on the v block (nr-1, nz-2; va..vc), the v
+ * field (-1:nr,0:nz) read as (-1:nr,-1:nz-1). */
+static void velo_esolp(oc_ctx* c, double* yy, const double* xx) {
+  double *f1 = c->w1, *f2 = c->w2;
+  btoa_uv(c, f1, f2, xx);
+  oc_esolp_x(NX, NY - 1, c->ua, c->ub, c->uc, f1, f1);
+  oc_esolp_x(NX - 1, NY - 2, c->va, c->vb, c->vc, f2, f2);
+  atob_uv(c, f1, f2, yy);
+}
+
 int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* b_in) {
   int64_t n = op_size(c, op);
   if (n < 0) return DRD_E_ARG;
   switch (prec) {
+    case DRD_PS_VELO_ESOLP: velo_esolp(c, x_out, b_in); return 0;
     case DRD_PS_VELO1_JACOBI: velo_jacobi(c, x_out, b_in); return 0;
     case DRD_PS_IDENTITY: memcpy(x_out, b_in, (size_t)n * sizeof(double)); return 0;
     case DRD_PS_PHMU_JACOBI: phmu_jacobi(c, x_out, b_in); return 0;

@@ -28,6 +28,9 @@ double* oc_field(oc_ctx* c, int32_t field);
 int64_t oc_field_size(const drd_params* p, int32_t field);
 /* eigenvalue tables of setup (TP/setup.f90:108-155)                      */
 const double* oc_pre_lcc(oc_ctx* c); /* (0:nx,0:ny) */
+/* ESOLP tables TP/setup.f90:157-235: 0..2 = ua,ub,uc (1:ny,0:nx),
+ * 3..5 = va,vb,vc (1:ny-1,0:nx-1) */
+const double* oc_esolp_table(oc_ctx* c, int32_t which);
 const double* oc_ph_lcc(oc_ctx* c);  /* (0:nx-1,0:ny-1) */
 
 int oc_velosolve_combined(oc_ctx* c, const double* psi, double* u_after,

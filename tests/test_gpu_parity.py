@@ -173,6 +173,48 @@ def test_bicgstab_control_flow(op, prec):
     g.close()
 
 
+@pytest.mark.parametrize("nx,ny", [(64, 32), (33, 20)])
+def test_phsolve_operator_and_preconditioner(nx, ny):
+    """MATRIX_VECTOR_CG (bit-exact) and the VCOST/DCT-I PSolve_CG (1e-12;
+    FFT when 2(n-1) is a power of two, the definition otherwise)."""
+    print(f"phsolve ops {nx}x{ny}", flush=True)
+    cfg = random_cfg(nx, ny, 15)
+    g, o = pair(cfg)
+    n = nx * ny
+    rng = np.random.default_rng(16)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    np.testing.assert_array_equal(g.matvec(abi.OP_PH, 0.8, x, 0.2, y), o.matvec(abi.OP_PH, 0.8, x, 0.2, y))
+    assert rel_l2(g.psolve(abi.PS_PH_DCT, abi.OP_PH, x), o.psolve(abi.PS_PH_DCT, abi.OP_PH, x)) < 1e-12
+    g.close()
+
+
+def test_phsolve_solve():
+    print("phsolve", flush=True)
+    cfg = geometry.tp_case(64, 32, tol=1e-10)
+    p = cfg.params
+    p.phi_mode = 1
+    cfg.fields[abi.F_PH] = (cfg.fields[abi.F_PH] + 1.0) / 2.0  # MF works with phi in [0, 1]
+    g, o = pair(cfg)
+    mg, pg, ig = g.PhSolve(0.0)
+    mo, po, io = o.PhSolve(0.0)
+    print("gpu", ig, "oracle", io)
+    assert ig["info"] == io["info"]
+    assert abs(ig["iter"] - io["iter"]) <= 2
+    assert rel_l2(mg, mo) < 1e-9 and rel_l2(pg, po) < 1e-9
+    g.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 32), (48, 24)])
+def test_esolp_preconditioner(nx, ny):
+    """PSolve_uv_CG of PD (ESOLP_x on both blocks): 1e-12 against the oracle."""
+    print(f"esolp {nx}x{ny}", flush=True)
+    cfg = random_cfg(nx, ny, 17)
+    g, o = pair(cfg)
+    b = np.random.default_rng(18).standard_normal(abi.n_uv(cfg.params))
+    assert rel_l2(g.psolve(abi.PS_VELO_ESOLP, abi.OP_VELO1, b), o.psolve(abi.PS_VELO_ESOLP, abi.OP_VELO1, b)) < 1e-12
+    g.close()
+
+
 def test_ten_steps():
     """u, v, p, phi after 10 steps within 1e-9 relative L2 of the oracle."""
     cfg = geometry.tp_case(64, 32, lambda_eta=10.0, tol=1e-10)

@@ -120,6 +120,42 @@ def test_setup_tables(tp):
 
 
 @needs_tp
+def test_esolp_tables_and_preconditioner(tp):
+    """ua..vc of TP/setup.f90:157-235 (with its t1 quirk) bit-exact, and the
+    PD ESOLP preconditioner composed from the reference's ESOLP_x."""
+    import ctypes as C
+    p = _params(tp)
+    p.dt, p.re = 5e-5, 3.0
+    tp.set_params(p)
+    tp.set_scalar("lambda_eta", 1.0)
+    tp.set_scalar("xlsT", 0.0)
+    tp.set_scalar("slipratio", 1.0)
+    tp.setup()
+    o = oc.Oracle(p)
+    L = oc.lib()
+    L.oc_esolp_table.argtypes = [C.c_void_p, C.c_int32]
+    L.oc_esolp_table.restype = C.POINTER(C.c_double)
+    shapes = [("ua", (p.nx + 1, p.ny)), ("ub", (p.nx + 1, p.ny)), ("uc", (p.nx + 1, p.ny)),
+              ("va", (p.nx, p.ny - 1)), ("vb", (p.nx, p.ny - 1)), ("vc", (p.nx, p.ny - 1))]
+    for w, (name, shape) in enumerate(shapes):
+        a = np.ctypeslib.as_array(L.oc_esolp_table(o._h, w), shape=shape)
+        np.testing.assert_array_equal(a, tp.array(name, shape), err_msg=name)
+    rng = np.random.default_rng(111)
+    b = rng.standard_normal(tp.nuv)
+    got = o.psolve(abi.PS_VELO_ESOLP, abi.OP_VELO1, b)
+    mn = (p.nx + 1) * p.ny
+    f1 = np.zeros(tp.u_shape)
+    f1[1:-1, 1:-1] = b[:mn].reshape(p.ny, p.nx + 1)
+    f2 = np.zeros(tp.v_shape)
+    f2[1:-1, 1:-1] = b[mn:].reshape(p.ny - 1, p.nx)
+    tabs = {name: tp.array(name, shape).copy() for name, shape in shapes}
+    zu = tp.esolp_x(p.nx, p.ny - 1, tabs["ua"], tabs["ub"], tabs["uc"], f1)
+    zv = tp.esolp_x(p.nx - 1, p.ny - 2, tabs["va"], tabs["vb"], tabs["vc"], f2)
+    want = np.concatenate([zu[1:-1, 1:-1].ravel(), zv[1:-1, 1:-1].ravel()])
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+@needs_tp
 def test_scalar_helpers(tp):
     rng = np.random.default_rng(105)
     for _ in range(200):
