@@ -51,6 +51,7 @@ struct drd_ctx {
   double* arg[8] = {};                    // upload slots for host-array arguments
   // Krylov workspace
   int64_t nmax = 0;
+  int64_t tiles = 0;  // kTile tiles of the longest vector
   double* kw[7] = {};                     // R RT P V T PH SH  (LIB/BiCGSTABREVCOM.f:159-166)
   double *rhs = nullptr, *xv = nullptr, *diag = nullptr, *tmp = nullptr, *tmp2 = nullptr;
   double* part = nullptr;
@@ -67,6 +68,7 @@ struct drd_ctx {
   // FGMRES workspace (lazy): V (m+1 vectors), Z (m vectors), small arrays
   int fg_m = 0;
   double *fg_V = nullptr, *fg_Z = nullptr, *fg_h = nullptr, *fg_hh = nullptr;
+  double* fg_part = nullptr;  // (m + 2) * 2 partial sums per tile
   // PhSolve / VCOST / ESOLP (lazy)
   Dct1Tab *d1_nx = nullptr, *d1_ny = nullptr, *d1_nx1 = nullptr;
   double *ph_f2 = nullptr, *ph_t = nullptr, *aux_work = nullptr, *esolp_tabs = nullptr;
@@ -285,8 +287,11 @@ static int ctx_alloc(drd_ctx* c) {
   RCHK(dalloc(&c->diag, c->nmax));
   RCHK(dalloc(&c->tmp, c->nmax));
   RCHK(dalloc(&c->tmp2, c->nmax));
-  const int64_t maxblocks = std::max<int64_t>((c->nmax + kTile - 1) / kTile, g.ny + g.nx + 64);
-  RCHK(dalloc(&c->part, 2 * maxblocks + 64));
+  // partial sums: up to 2 segments x 2 values per tile, or one per grid row
+  const int64_t tiles = (c->nmax + kTile - 1) / kTile;
+  if (tiles > kMaxTiles) return set_err(DRD_E_UNSUPPORTED, "grid too large for the reduction tree");
+  c->tiles = tiles;
+  RCHK(dalloc(&c->part, std::max<int64_t>(4 * tiles, g.ny + g.nx + 64)));
   RCHK(dalloc(&c->ks, 1));
   HIPCHK(hipHostMalloc((void**)&c->ks_host, sizeof(KState), hipHostMallocDefault));
   RCHK(dalloc(&c->cfB, p.nx + 1));
@@ -329,6 +334,7 @@ static void ctx_free(drd_ctx* c) {
   hipFree(c->fg_V);
   hipFree(c->fg_Z);
   hipFree(c->fg_h);
+  hipFree(c->fg_part);
   if (c->fg_hh) hipHostFree(c->fg_hh);
   dct1_table_destroy(c->d1_nx);
   dct1_table_destroy(c->d1_ny);
@@ -448,6 +454,10 @@ static int64_t op_size(const drd_ctx* c, int op) {
   return -1;
 }
 
+// Segments of the operator's Krylov vectors: the phi|mu vector is summed as
+// two blocks (its kernels produce the phi and mu rows of a cell together).
+static int op_nseg(int op) { return op == DRD_OP_PHMU ? 2 : 1; }
+
 // resources of PhSolve, its VCOST preconditioner and ESOLP, made on first use
 static int aux_ready(drd_ctx* c) {
   if (c->aux_work) return 0;
@@ -560,7 +570,7 @@ static int read_state(drd_ctx* c) {
 static int true_resid(drd_ctx* c, int op, int64_t n, const double* b, const double* x, double* out) {
   launch_copy(c->stream, n, b, c->tmp2);
   RCHK(apply_op(c, op, MV_PLAIN, x, c->tmp2, 1.0, 0.0, nullptr, nullptr, nullptr));
-  launch_resid_norms(c->stream, n, b, c->tmp2, c->part, c->ks);
+  launch_resid_norms(c->stream, n, op_nseg(op), b, c->tmp2, c->part, c->ks);
   RCHK(read_state(c));
   const double bn = std::sqrt(c->ks_host->red[1]);
   *out = std::sqrt(c->ks_host->red[0]) / (bn == 0.0 ? 1.0 : bn);
@@ -589,7 +599,8 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   *c->ks_host = h;
   HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
   RCHK(prec_setup(c, prec));
-  launch_init_norms(c->stream, n, x, b, c->part, c->ks);
+  const int ns = op_nseg(op);
+  launch_init_norms(c->stream, n, ns, x, b, c->part, c->ks);
   RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
   int chunk = 1, total = 0;
   for (;;) {
@@ -599,11 +610,11 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
       launch_bicg_p(c->stream, n, R, V, P, c->ks);
       RCHK(apply_prec(c, prec, op, P, PHb, &ph, c->ks));
       RCHK(apply_op(c, op, MV_DOT1, ph, V, 1.0, 0.0, RT, nullptr, c->ks));
-      launch_bicg_s(c->stream, n, R, V, c->part, c->ks);
+      launch_bicg_s(c->stream, n, ns, R, V, c->part, c->ks);
       launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
       RCHK(apply_prec(c, prec, op, R, SHb, &sh, c->ks));
       RCHK(apply_op(c, op, MV_DOT2, sh, T, 1.0, 0.0, R, nullptr, c->ks));
-      launch_bicg_xr(c->stream, n, x, R, RT, ph, sh, T, c->part, c->ks);
+      launch_bicg_xr(c->stream, n, ns, x, R, RT, ph, sh, T, c->part, c->ks);
     }
     HIPCHK(hipGetLastError());
     total += chunk;
@@ -638,13 +649,15 @@ static int fgmres_alloc(drd_ctx* c, int m) {
   hipFree(c->fg_V);
   hipFree(c->fg_Z);
   hipFree(c->fg_h);
+  hipFree(c->fg_part);
   if (c->fg_hh) hipHostFree(c->fg_hh);
-  c->fg_V = c->fg_Z = c->fg_h = nullptr;
+  c->fg_V = c->fg_Z = c->fg_h = c->fg_part = nullptr;
   c->fg_hh = nullptr;
   c->fg_m = 0;
   RCHK(dalloc(&c->fg_V, (int64_t)(m + 1) * c->nmax));
   RCHK(dalloc(&c->fg_Z, (int64_t)m * c->nmax));
   RCHK(dalloc(&c->fg_h, 4 * (int64_t)(m + 2)));
+  RCHK(dalloc(&c->fg_part, 2 * (int64_t)(m + 2) * c->tiles));
   HIPCHK(hipHostMalloc((void**)&c->fg_hh, 4 * (m + 2) * sizeof(double), hipHostMallocDefault));
   c->fg_m = m;
   return 0;
@@ -669,7 +682,9 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
     return 0;
   };
   // ||b||
-  launch_multidot(c->stream, n, b, b, ld, 1, c->part, ticket, hd);
+  const int ns = op_nseg(op);
+  double* fpart = c->fg_part;
+  launch_multidot(c->stream, n, ns, b, b, ld, 1, fpart, ticket, hd);
   RCHK(sync_read(1));
   const double bn = std::sqrt(hh[0]);
   drd_solve_info loc{0, 1, tol, 0.0};
@@ -685,7 +700,7 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
     // r = b - A x into V_0, beta = ||r||
     launch_copy(c->stream, n, b, V);
     RCHK(apply_op(c, op, MV_PLAIN, x, V, -1.0, 1.0, nullptr, nullptr, nullptr));
-    launch_multidot(c->stream, n, V, V, ld, 1, c->part, ticket, hd);
+    launch_multidot(c->stream, n, ns, V, V, ld, 1, fpart, ticket, hd);
     RCHK(sync_read(1));
     const double beta = std::sqrt(hh[0]);
     loc.resid = beta / bn;
@@ -711,16 +726,17 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
       if (zr != zj) launch_copy(c->stream, n, zr, zj);
       RCHK(apply_op(c, op, MV_PLAIN, zj, w, 1.0, 0.0, nullptr, nullptr, nullptr));
       // pass 1: h = V^T w and ||w||^2 (w = V_{j+1} is contiguous with V)
-      launch_multidot(c->stream, n, w, V, ld, j + 2, c->part, ticket, hd);
-      launch_multiaxpy(c->stream, n, w, V, ld, j + 1, hd, -1.0, c->part, ticket, hd + 2 * (m + 2), 1);
+      launch_multidot(c->stream, n, ns, w, V, ld, j + 2, fpart, ticket, hd);
+      launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd, -1.0, fpart, ticket, hd + 2 * (m + 2), 1);
       RCHK(sync_read(2 * (m + 2) + 1));
       double* hcol = &H[(size_t)(m + 1) * j];
       for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
       const double before = hh[j + 1];
       double after = hh[2 * (m + 2)];
       if (after < 0.25 * before) {  // pass 2
-        launch_multidot(c->stream, n, w, V, ld, j + 1, c->part, ticket, hd + (m + 2));
-        launch_multiaxpy(c->stream, n, w, V, ld, j + 1, hd + (m + 2), -1.0, c->part, ticket, hd + 2 * (m + 2), 1);
+        launch_multidot(c->stream, n, ns, w, V, ld, j + 1, fpart, ticket, hd + (m + 2));
+        launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd + (m + 2), -1.0, fpart, ticket, hd + 2 * (m + 2),
+                         1);
         RCHK(sync_read(2 * (m + 2) + 1));
         for (int i = 0; i <= j; ++i) hcol[i] += hh[(m + 2) + i];
         after = hh[2 * (m + 2)];
@@ -751,7 +767,7 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
     }
     for (int i = 0; i < jdone; ++i) hh[i] = y[i];
     HIPCHK(hipMemcpyAsync(hd, hh, jdone * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    launch_multiaxpy(c->stream, n, x, Z, ld, jdone, hd, 1.0, c->part, ticket, nullptr, 0);
+    launch_multiaxpy(c->stream, n, ns, x, Z, ld, jdone, hd, 1.0, fpart, ticket, nullptr, 0);
   }
   HIPCHK(hipGetLastError());
   loc.iter = it;
@@ -1063,7 +1079,7 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
                            c->part, c->ks);
         break;
       case DRD_K_BICG_XR:
-        launch_bicg_xr(c->stream, g.nuv(), c->xv, c->kw[0], c->kw[1], c->kw[5], c->kw[6], c->kw[4], c->part,
+        launch_bicg_xr(c->stream, g.nuv(), 1, c->xv, c->kw[0], c->kw[1], c->kw[5], c->kw[6], c->kw[4], c->part,
                        c->ks);
         break;
       case DRD_K_PRESSURE:

@@ -54,68 +54,135 @@ struct KState {
   int32_t pad;
 };
 
-// ---- wave64 / workgroup reductions (fixed order -> deterministic) -------
-__device__ __forceinline__ double wave_sum(double v) {
+// ---- canonical reductions ---------------------------------------------------
+// Every sum on the path (dot products, norms, the divergence mean) is the
+// pairwise sum over the element index, padded with zeros to a power of two:
+//   S[lo, lo+2L) = S[lo, lo+L) + S[lo+L, lo+2L).
+// The value depends on the data only, not on how many threads or
+// workgroups compute it, and a serial restatement reproduces it bit for
+// bit.  (The padding width can change only the sign of an all-zero sum.)
+// A Krylov vector is 1 segment, or 2 equal segments (the phi|mu blocks of
+// the Cahn-Hilliard system); its dot product is the sum of the segment
+// sums in segment order.
+//
+// A workgroup of a fused sweep covers kTile consecutive elements of each
+// segment; thread t holds element j*kBlock + t of the tile in slot j, so
+// wave w holds the aligned 64-element chunk j*kWaves + w in every slot.
+constexpr int kTile = 1024;                 // elements per workgroup and segment
+constexpr int kSlots = kTile / kBlock;      // 4 elements per thread
+constexpr int kChunks = kSlots * kWaves;    // 16 chunks of 64 per tile
+constexpr int64_t kMaxTiles = (int64_t)kBlock * kBlock;  // per segment
+
+// Pairwise sum of the wave's 64 values (lane l holds element l).  Lane l
+// adds its xor-partner's subtotal at every level; both partners form the
+// same sum because IEEE addition commutes, so every lane ends with it.
+__device__ __forceinline__ double wave_pairwise(double v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
 
-// Sums NV values over the workgroup; thread 0 gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double lds[NV][kWaves];
+template <int N>
+__device__ __forceinline__ double pairwise_fixed(const double* a) {
+  if constexpr (N == 1)
+    return a[0];
+  else
+    return pairwise_fixed<N / 2>(a) + pairwise_fixed<N / 2>(a + N / 2);
+}
+
+// Pairwise sum of kBlock consecutive values (thread t holds value t);
+// the result is on thread 0.  Every thread of the workgroup must call it.
+__device__ __forceinline__ double block_pairwise(double v) {
+  __shared__ double lw[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_pairwise(v);
+  if (lane == 0) lw[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) r = pairwise_fixed<kWaves>(lw);
+  __syncthreads();
+  return r;
+}
+
+// Pairwise sums of K tiles at once; t[k][j] is slot j of value k.  The
+// results are on thread 0.
+template <int K>
+__device__ __forceinline__ void tile_pairwise(const double (&t)[K][kSlots], double (&out)[K]) {
+  __shared__ double lc[K][kChunks];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double t = wave_sum(v[j]);
-    if (lane == 0) lds[j][w] = t;
-  }
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const double c = wave_pairwise(t[k][j]);
+      if (lane == 0) lc[k][j * kWaves + w] = c;
+    }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      double t = 0.0;
-#pragma unroll
-      for (int q = 0; q < kWaves; ++q) t += lds[j][q];
-      v[j] = t;
-    }
+    for (int k = 0; k < K; ++k) out[k] = pairwise_fixed<kChunks>(lc[k]);
   }
   __syncthreads();
 }
 
-// Writes this block's NV partials, and returns true in the one block that
-// finishes last; that block then holds the grand totals (in fixed block
-// order) in `tot` on thread 0.
-template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* part, uint32_t* ticket,
-                                         double (&tot)[NV]) {
-  block_sum<NV>(v);
-  const uint32_t nb = gridDim.x * gridDim.y;
-  const uint32_t b = blockIdx.x + gridDim.x * blockIdx.y;
+// Pairwise sum over tiles 0..nt-1 of part[tile*stride + k] (nt <= kMaxTiles):
+// chunks of kBlock tiles, then the chunk sums.  Result on thread 0.
+__device__ __forceinline__ double tiles_pairwise(const double* part, uint32_t nt, int stride, int k) {
+  __shared__ double cs[kBlock];
+  uint32_t ntp = 1;
+  while (ntp < nt) ntp <<= 1;
+  const uint32_t nch = ntp > (uint32_t)kBlock ? ntp / kBlock : 1u;
+  for (uint32_t c = 0; c < nch; ++c) {
+    const uint32_t q = c * kBlock + threadIdx.x;
+    const double v =
+        q < nt ? __hip_atomic_load(part + (size_t)q * stride + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    const double r = block_pairwise(v);
+    if (threadIdx.x == 0) cs[c] = r;
+  }
+  __syncthreads();
+  const double v = threadIdx.x < nch ? cs[threadIdx.x] : 0.0;
+  __syncthreads();
+  return block_pairwise(v);
+}
+
+// Index of slot j of segment s in this workgroup's tile, or -1 past the end
+// of the segment (segments of length seg, laid out one after the other).
+__device__ __forceinline__ int64_t tile_elem(int64_t seg, int s, int j) {
+  const int64_t o = (int64_t)blockIdx.x * kTile + j * kBlock + threadIdx.x;
+  return o < seg ? (int64_t)s * seg + o : -1;
+}
+
+// Publishes this workgroup's partial sums (value k of segment s at
+// part[tile*(NS*NV) + s*NV + k]) and returns true in the workgroup that
+// finishes last; its thread 0 then holds in tot[k] the canonical total
+// (segment sums added in segment order).  One workgroup per tile, 1-D grid.
+template <int NS, int NV>
+__device__ __forceinline__ bool grid_pairwise(const double (&t)[NS * NV][kSlots], double* part, uint32_t* ticket,
+                                              double (&tot)[NV]) {
+  constexpr int K = NS * NV;
+  double p[K];
+  tile_pairwise<K>(t, p);
+  const uint32_t nt = gridDim.x;
   __shared__ int is_last;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NV; ++j) part[(size_t)b * NV + j] = v[j];
+    for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * K + k] = p[k];
     __threadfence();
-    uint32_t t = atomicAdd(ticket, 1u);
-    is_last = (t == nb - 1);
+    is_last = (atomicAdd(ticket, 1u) == nt - 1);
   }
   __syncthreads();
   if (!is_last) return false;
   __threadfence();
-  double acc[NV];
 #pragma unroll
-  for (int j = 0; j < NV; ++j) acc[j] = 0.0;
-  for (uint32_t q = threadIdx.x; q < nb; q += blockDim.x) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      acc[j] += __hip_atomic_load(part + (size_t)q * NV + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum<NV>(acc);
+  for (int k = 0; k < K; ++k) p[k] = tiles_pairwise(part, nt, K, k);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int j = 0; j < NV; ++j) tot[j] = acc[j];
+    for (int v = 0; v < NV; ++v) {
+      double s = p[v];
+#pragma unroll
+      for (int sg = 1; sg < NS; ++sg) s = s + p[sg * NV + v];
+      tot[v] = s;
+    }
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return true;

@@ -5,8 +5,6 @@
 
 namespace drd {
 
-constexpr int kTile = 1024;  // unknowns per workgroup in the fused sweeps
-
 // matvec modes (see k_velo_matvec)
 enum { MV_PLAIN = 0, MV_DOT1 = 1, MV_DOT2 = 2, MV_INIT = 3 };
 
@@ -127,24 +125,27 @@ void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const doub
                         double* ph, double* mu);
 
 // ---- launchers (kernels_krylov.hip) ----
-void launch_init_norms(hipStream_t st, int64_t n, const double* x, const double* b, double* part,
+// Reducing kernels take the vector's segment count (1, or 2 for the phi|mu
+// vector of the Cahn-Hilliard system; see grid_pairwise).
+void launch_init_norms(hipStream_t st, int64_t n, int nseg, const double* x, const double* b, double* part,
                        KState* ks);
 void launch_bicg_p(hipStream_t st, int64_t n, const double* r, const double* v, double* p, KState* ks);
-void launch_bicg_s(hipStream_t st, int64_t n, double* r, const double* v, double* part, KState* ks);
+void launch_bicg_s(hipStream_t st, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks);
 void launch_bicg_early(hipStream_t st, int64_t n, double* x, const double* ph, double* part, KState* ks);
-void launch_bicg_xr(hipStream_t st, int64_t n, double* x, double* r, const double* rt, const double* ph,
-                    const double* sh, const double* t, double* part, KState* ks);
+void launch_bicg_xr(hipStream_t st, int64_t n, int nseg, double* x, double* r, const double* rt,
+                    const double* ph, const double* sh, const double* t, double* part, KState* ks);
 void launch_copy(hipStream_t st, int64_t n, const double* a, double* b);
 void launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta, double* y);
-// sum of squares of (b - y) and of b -> out[0], out[1] (FP64, deterministic)
-void launch_resid_norms(hipStream_t st, int64_t n, const double* b, const double* ax, double* part,
+// sum of squares of (b - ax) and of b -> ks->red[0], red[1] (FP64, canonical order)
+void launch_resid_norms(hipStream_t st, int64_t n, int nseg, const double* b, const double* ax, double* part,
                         KState* scratch_ks);
 void launch_cell_axpy(hipStream_t st, int64_t n, const double* a, double* b);  // b = a + b
-// FGMRES: res[i] = (w, V_i); w += sign * sum h_i V_i (and ||w||^2); b = a / s
-void launch_multidot(hipStream_t st, int64_t n, const double* w, const double* V, int64_t ldv, int nv,
+// FGMRES: res[i] = (w, V_i); w += sign * sum h_i V_i (and ||w||^2); b = a / s.
+// part holds nv * nseg partials per tile of kTile elements.
+void launch_multidot(hipStream_t st, int64_t n, int nseg, const double* w, const double* V, int64_t ldv, int nv,
                      double* part, uint32_t* ticket, double* res);
-void launch_multiaxpy(hipStream_t st, int64_t n, double* w, const double* V, int64_t ldv, int nv, const double* h,
-                      double sign, double* part, uint32_t* ticket, double* res, int want_norm);
+void launch_multiaxpy(hipStream_t st, int64_t n, int nseg, double* w, const double* V, int64_t ldv, int nv,
+                      const double* h, double sign, double* part, uint32_t* ticket, double* res, int want_norm);
 void launch_scale_into(hipStream_t st, int64_t n, const double* a, double s, double* b);
 
 // ---- fast-path preconditioners (kernels_prec.hip) ----

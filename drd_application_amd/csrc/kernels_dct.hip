@@ -227,7 +227,6 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
   double2* A = smem;
   double2* B = smem + N;
   double r[kMaxPer];
-  double dsum = 0.0;
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
     const int i = threadIdx.x + m * kBlock;
@@ -240,11 +239,26 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
         const double d =
             (a.u[g.ui(i + 1, k)] - a.u[g.ui(i, k)]) / a.dr + (a.v[g.vi(i, k + 1)] - a.v[g.vi(i, k)]) / a.dz;
         r[m] = d;
-        dsum += d;
       } else {
         r[m] = a.in[(size_t)row * N + i];
       }
     }
+  }
+  // canonical (pairwise) sum of the row's divergence, taken before the
+  // transform overwrites it; element i = m*kBlock + t lies in 64-chunk
+  // m*kWaves + wave.
+  double rowsum = 0.0;
+  if (MODE == R_DIV_DCT2 && a.ks) {
+    __shared__ double lc[kMaxPer * kWaves];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < kMaxPer; ++m) {
+      const double c = wave_pairwise(r[m]);
+      if (lane == 0) lc[m * kWaves + w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) rowsum = pairwise_fixed<kMaxPer * kWaves>(lc);
+    __syncthreads();
   }
   if (MODE == R_DCT2 || MODE == R_DIV_DCT2 || MODE == R_SOLVE) dct2_regs(r, N, a.pow2, A, B, a.tw, a.rot, a.cs);
   if (MODE == R_SOLVE) {
@@ -290,9 +304,22 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
       if (i < N) a.out[(size_t)row * N + i] = r[m];
     }
   }
-  if (MODE == R_DIV_DCT2 && a.ks) {
-    double acc[1] = {dsum}, tot[1];
-    if (grid_sum<1>(acc, a.part, &a.ks->ticket, tot) && threadIdx.x == 0) a.ks->red[0] = tot[0];
+  if (MODE == R_DIV_DCT2 && a.ks) {  // rows summed pairwise by the last workgroup
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+      a.part[row] = rowsum;
+      __threadfence();
+      is_last = (atomicAdd(&a.ks->ticket, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) {
+      __threadfence();
+      const double tot = tiles_pairwise(a.part, gridDim.x, 1, 0);
+      if (threadIdx.x == 0) {
+        a.ks->red[0] = tot;
+        __hip_atomic_store(&a.ks->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 

@@ -17,22 +17,22 @@ namespace drd {
 __device__ __forceinline__ bool kact(const KState* st) { return !st->done && !st->early; }
 
 // ||x||^2, ||b||^2                                 REVCOM:229-253
-__global__ __launch_bounds__(kBlock) void k_init_norms(int64_t n, const double* __restrict__ x,
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_init_norms(int64_t seg, const double* __restrict__ x,
                                                        const double* __restrict__ b, double* part,
                                                        KState* st) {
-  double acc[2] = {0.0, 0.0};
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double t[NS * 2][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) {
-      const double a = x[q], c = b[q];
-      acc[0] += a * a;
-      acc[1] += c * c;
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      const double a = q >= 0 ? x[q] : 0.0, c = q >= 0 ? b[q] : 0.0;
+      t[s * 2][j] = a * a;
+      t[s * 2 + 1][j] = c * c;
     }
-  }
   double tot[2];
-  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_INIT_NORMS, tot);
+  if (grid_pairwise<NS, 2>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_INIT_NORMS, tot);
 }
 
 // p = r (first iteration) or p = beta*(p - omega*v) + r   REVCOM:266-275
@@ -60,23 +60,26 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const double* __re
 }
 
 // s = r - alpha*v (S aliases R), ||s||^2           REVCOM:311-313
-__global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t n, double* __restrict__ r,
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restrict__ r,
                                                    const double* __restrict__ v, double* part, KState* st) {
   if (!kact(st)) return;
   const double alpha = st->alpha;
-  double acc[1] = {0.0};
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double t[NS][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) {
-      const double s = r[q] + (-alpha) * v[q];
-      r[q] = s;
-      acc[0] += s * s;
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      double sv = 0.0;
+      if (q >= 0) {
+        sv = r[q] + (-alpha) * v[q];
+        r[q] = sv;
+      }
+      t[s][j] = sv * sv;
     }
-  }
   double tot[1];
-  if (grid_sum<1>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_S, tot);
+  if (grid_pairwise<NS, 1>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_S, tot);
 }
 
 // early exit: x = x + alpha*phat                   REVCOM:314
@@ -85,43 +88,46 @@ __global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __rest
                                                        KState* st) {
   if (!st->early || st->done) return;
   const double alpha = st->alpha;
-  const int64_t base = (int64_t)blockIdx.x * kTile;
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) x[q] = x[q] + alpha * ph[q];
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t q = tile_elem(n, 0, j);
+    if (q >= 0) x[q] = x[q] + alpha * ph[q];
   }
-  double acc[1] = {0.0}, tot[1];
-  if (grid_sum<1>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_EARLY, tot);
+  double t[1][kSlots] = {}, tot[1];  // only to find the last workgroup
+  if (grid_pairwise<1, 1>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_EARLY, tot);
 }
 
 // x = x + alpha*phat + omega*shat; r = s - omega*t; ||r||^2, (rtld,r)
 // REVCOM:354-361 and the next iteration's rho (:263).  With the identity
 // preconditioner shat aliases r (and phat aliases p), so r and sh carry no
 // __restrict__; each element is read before it is written by its own thread.
-__global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t n, double* __restrict__ x, double* r,
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restrict__ x, double* r,
                                                     const double* __restrict__ rt, const double* __restrict__ ph,
                                                     const double* sh, const double* __restrict__ t,
                                                     double* part, KState* st) {
   if (!kact(st)) return;
   const double alpha = st->alpha, omega = st->omega;
-  double acc[2] = {0.0, 0.0};
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double tm[NS * 2][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) {
-      double xv = x[q] + alpha * ph[q];
-      xv = xv + omega * sh[q];
-      x[q] = xv;
-      const double rv = r[q] + (-omega) * t[q];
-      r[q] = rv;
-      acc[0] += rv * rv;
-      acc[1] += rt[q] * rv;
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      double rv = 0.0, rtv = 0.0;
+      if (q >= 0) {
+        double xv = x[q] + alpha * ph[q];
+        xv = xv + omega * sh[q];
+        x[q] = xv;
+        rv = r[q] + (-omega) * t[q];
+        r[q] = rv;
+        rtv = rt[q];
+      }
+      tm[s * 2][j] = rv * rv;
+      tm[s * 2 + 1][j] = rtv * rv;
     }
-  }
   double tot[2];
-  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_XR, tot);
+  if (grid_pairwise<NS, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_XR, tot);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -142,22 +148,23 @@ __global__ __launch_bounds__(kBlock) void k_cell_axpy(int64_t n, const double* _
 }
 
 // ||b - ax||^2 and ||b||^2 into ks->red[0..1] (true-residual check)
-__global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t n, const double* __restrict__ b,
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t seg, const double* __restrict__ b,
                                                         const double* __restrict__ ax, double* part,
                                                         KState* st) {
-  double acc[2] = {0.0, 0.0};
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  double t[NS * 2][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) {
-      const double d = b[q] - ax[q];
-      acc[0] += d * d;
-      acc[1] += b[q] * b[q];
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      const double bv = q >= 0 ? b[q] : 0.0;
+      const double d = q >= 0 ? bv - ax[q] : 0.0;
+      t[s * 2][j] = d * d;
+      t[s * 2 + 1][j] = bv * bv;
     }
-  }
   double tot[2];
-  if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) {
+  if (grid_pairwise<NS, 2>(t, part, &st->ticket, tot) && threadIdx.x == 0) {
     st->red[0] = tot[0];
     st->red[1] = tot[1];
   }
@@ -165,26 +172,34 @@ __global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t n, const double*
 
 // ---- FGMRES orthogonalisation ------------------------------------------------
 // res[i] = (w, V_i) for i < nv, V_i = V + i*ldv.  One sweep reads w once and
-// each V_i once; partials reduced by the last workgroup in block order.
-__global__ __launch_bounds__(kBlock) void k_multidot(int64_t n, const double* __restrict__ w,
+// each V_i once; partials part[(tile*nv + i)*NS + s], summed canonically by
+// the last workgroup.
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_multidot(int64_t seg, const double* __restrict__ w,
                                                      const double* __restrict__ V, int64_t ldv, int nv,
                                                      double* part, uint32_t* ticket, double* res) {
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  double wl[kTile / kBlock];
+  double wl[NS][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    wl[j] = q < n ? w[q] : 0.0;
-  }
-  for (int i = 0; i < nv; ++i) {
-    double a[1] = {0.0};
+  for (int s = 0; s < NS; ++s)
 #pragma unroll
-    for (int j = 0; j < kTile / kBlock; ++j) {
-      const int64_t q = base + j * kBlock + threadIdx.x;
-      if (q < n) a[0] += wl[j] * V[i * ldv + q];
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      wl[s][j] = q >= 0 ? w[q] : 0.0;
     }
-    block_sum<1>(a);
-    if (threadIdx.x == 0) part[(size_t)blockIdx.x * nv + i] = a[0];
+  for (int i = 0; i < nv; ++i) {
+    double t[NS][kSlots], p[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int64_t q = tile_elem(seg, s, j);
+        t[s][j] = q >= 0 ? wl[s][j] * V[i * ldv + q] : 0.0;
+      }
+    tile_pairwise<NS>(t, p);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) part[((size_t)blockIdx.x * nv + i) * NS + s] = p[s];
+    }
   }
   __shared__ int is_last;
   if (threadIdx.x == 0) {
@@ -195,35 +210,40 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int64_t n, const double* __
   if (!is_last) return;
   __threadfence();
   for (int i = 0; i < nv; ++i) {
-    double a[1] = {0.0};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-      a[0] += __hip_atomic_load(part + (size_t)b * nv + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    block_sum<1>(a);
-    if (threadIdx.x == 0) res[i] = a[0];
+    double tot = tiles_pairwise(part, gridDim.x, nv * NS, i * NS);
+#pragma unroll
+    for (int s = 1; s < NS; ++s) {
+      const double ts = tiles_pairwise(part, gridDim.x, nv * NS, i * NS + s);
+      tot = tot + ts;
+    }
+    if (threadIdx.x == 0) res[i] = tot;
   }
   if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // w = w + sign * sum_i h[i] V_i ; if want_norm, res[0] = ||w||^2
-__global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t n, double* __restrict__ w,
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __restrict__ w,
                                                       const double* __restrict__ V, int64_t ldv, int nv,
                                                       const double* __restrict__ h, double sign, double* part,
                                                       uint32_t* ticket, double* res, int want_norm) {
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  double acc[1] = {0.0};
+  double t[NS][kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q < n) {
-      double s = w[q];
-      for (int i = 0; i < nv; ++i) s = s + (sign * h[i]) * V[i * ldv + q];
-      w[q] = s;
-      acc[0] += s * s;
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j);
+      double sv = 0.0;
+      if (q >= 0) {
+        sv = w[q];
+        for (int i = 0; i < nv; ++i) sv = sv + (sign * h[i]) * V[i * ldv + q];
+        w[q] = sv;
+      }
+      t[s][j] = sv * sv;
     }
-  }
   if (!want_norm) return;
   double tot[1];
-  if (grid_sum<1>(acc, part, ticket, tot) && threadIdx.x == 0) res[0] = tot[0];
+  if (grid_pairwise<NS, 1>(t, part, ticket, tot) && threadIdx.x == 0) res[0] = tot[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* a, double s, double* b) {
@@ -233,34 +253,45 @@ __global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* 
 
 static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-void launch_multidot(hipStream_t s, int64_t n, const double* w, const double* V, int64_t ldv, int nv, double* part,
-                     uint32_t* ticket, double* res) {
-  hipLaunchKernelGGL(k_multidot, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, w, V, ldv, nv, part, ticket, res);
+// Reductions see a vector of n = nseg * seg elements (nseg 1 or 2) and run
+// one workgroup per kTile elements of a segment.
+#define DRD_SEG_LAUNCH(kern, n, nseg, ...)                                                          \
+  do {                                                                                              \
+    const int64_t seg_ = (n) / (nseg);                                                              \
+    if ((nseg) == 2)                                                                                \
+      hipLaunchKernelGGL(kern<2>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);   \
+    else                                                                                            \
+      hipLaunchKernelGGL(kern<1>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);   \
+  } while (0)
+
+void launch_multidot(hipStream_t s, int64_t n, int nseg, const double* w, const double* V, int64_t ldv, int nv,
+                     double* part, uint32_t* ticket, double* res) {
+  DRD_SEG_LAUNCH(k_multidot, n, nseg, w, V, ldv, nv, part, ticket, res);
 }
-void launch_multiaxpy(hipStream_t s, int64_t n, double* w, const double* V, int64_t ldv, int nv, const double* h,
-                      double sign, double* part, uint32_t* ticket, double* res, int want_norm) {
-  hipLaunchKernelGGL(k_multiaxpy, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, w, V, ldv, nv, h, sign, part, ticket,
-                     res, want_norm);
+void launch_multiaxpy(hipStream_t s, int64_t n, int nseg, double* w, const double* V, int64_t ldv, int nv,
+                      const double* h, double sign, double* part, uint32_t* ticket, double* res, int want_norm) {
+  DRD_SEG_LAUNCH(k_multiaxpy, n, nseg, w, V, ldv, nv, h, sign, part, ticket, res, want_norm);
 }
 void launch_scale_into(hipStream_t s, int64_t n, const double* a, double sc, double* b) {
   hipLaunchKernelGGL(k_scale_into, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, sc, b);
 }
 
-void launch_init_norms(hipStream_t s, int64_t n, const double* x, const double* b, double* part, KState* ks) {
-  hipLaunchKernelGGL(k_init_norms, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, b, part, ks);
+void launch_init_norms(hipStream_t s, int64_t n, int nseg, const double* x, const double* b, double* part,
+                       KState* ks) {
+  DRD_SEG_LAUNCH(k_init_norms, n, nseg, x, b, part, ks);
 }
 void launch_bicg_p(hipStream_t s, int64_t n, const double* r, const double* v, double* p, KState* ks) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, p, ks);
 }
-void launch_bicg_s(hipStream_t s, int64_t n, double* r, const double* v, double* part, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_s, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, part, ks);
+void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks) {
+  DRD_SEG_LAUNCH(k_bicg_s, n, nseg, r, v, part, ks);
 }
 void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
   hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, part, ks);
 }
-void launch_bicg_xr(hipStream_t s, int64_t n, double* x, double* r, const double* rt, const double* ph,
+void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, const double* rt, const double* ph,
                     const double* sh, const double* t, double* part, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_xr, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, r, rt, ph, sh, t, part, ks);
+  DRD_SEG_LAUNCH(k_bicg_xr, n, nseg, x, r, rt, ph, sh, t, part, ks);
 }
 void launch_copy(hipStream_t s, int64_t n, const double* a, double* b) {
   hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
@@ -271,9 +302,9 @@ void launch_axpby(hipStream_t s, int64_t n, double alpha, const double* x, doubl
 void launch_cell_axpy(hipStream_t s, int64_t n, const double* a, double* b) {
   hipLaunchKernelGGL(k_cell_axpy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
 }
-void launch_resid_norms(hipStream_t s, int64_t n, const double* b, const double* ax, double* part,
+void launch_resid_norms(hipStream_t s, int64_t n, int nseg, const double* b, const double* ax, double* part,
                         KState* ks) {
-  hipLaunchKernelGGL(k_resid_norms, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, b, ax, part, ks);
+  DRD_SEG_LAUNCH(k_resid_norms, n, nseg, b, ax, part, ks);
 }
 
 }  // namespace drd

@@ -258,13 +258,12 @@ __global__ __launch_bounds__(kBlock) void k_ph_mv2(Grid g, Scal s, const double*
   if (MODE != MV_PLAIN && (st->done || st->early)) return;
   const int64_t n = g.ncell();
   const int nx = g.nx;
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  double acc[2] = {0.0, 0.0};
+  double tm[2][kSlots] = {};
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
 #pragma unroll
-  for (int jj = 0; jj < kTile / kBlock; ++jj) {
-    const int64_t q = base + jj * kBlock + threadIdx.x;
-    if (q >= n) break;
+  for (int jj = 0; jj < kSlots; ++jj) {
+    const int64_t q = tile_elem(n, 0, jj);
+    if (q < 0) break;
     const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
     double z = 0.0;
     if (MODE != MV_INIT || xnz) {
@@ -279,25 +278,27 @@ __global__ __launch_bounds__(kBlock) void k_ph_mv2(Grid g, Scal s, const double*
       if (xnz) r = -1.0 * z + 1.0 * r;
       y[q] = r;
       a0[q] = r;
-      acc[0] += r * r;
+      tm[0][jj] = r * r;
     } else {
       y[q] = z;
       if (MODE == MV_DOT1) {
-        acc[0] += a0[q] * z;
+        tm[0][jj] = a0[q] * z;
       } else {
-        acc[0] += z * a0[q];
-        acc[1] += z * z;
+        tm[0][jj] = z * a0[q];
+        tm[1][jj] = z * z;
       }
     }
   }
   if (MODE == MV_PLAIN) return;
-  double tot[2];
   if (MODE == MV_DOT2) {
-    if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    double tot[2];
+    if (grid_pairwise<1, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
   } else {
-    double a1[1] = {acc[0]}, t1[1];
-    if (grid_sum<1>(a1, part, &st->ticket, t1) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, t1);
+    double t1[1][kSlots], tot[1];
+#pragma unroll
+    for (int jj = 0; jj < kSlots; ++jj) t1[0][jj] = tm[0][jj];
+    if (grid_pairwise<1, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
+      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
   }
 }
 

@@ -101,13 +101,12 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
   if (MODE != MV_PLAIN && !kactive(st)) return;
   VeloIn X{x, g.nx, g.ny, s.lr_d, g.mn()};
   const int64_t n = g.nuv();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  double acc[2] = {0.0, 0.0};
+  double tm[2][kSlots] = {};
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
-    const int64_t q = base + j * kBlock + threadIdx.x;
-    if (q >= n) break;
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t q = tile_elem(n, 0, j);
+    if (q < 0) break;
     if (MODE == MV_PLAIN) {
       // y is not read when beta == 0 (as in BLAS), so it may be uninitialised
       const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
@@ -117,26 +116,28 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
       if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
       y[q] = r;
       a0[q] = r;
-      acc[0] += r * r;
+      tm[0][j] = r * r;
     } else {
       const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
       y[q] = z;
       if (MODE == MV_DOT1) {
-        acc[0] += a0[q] * z;
+        tm[0][j] = a0[q] * z;
       } else {
-        acc[0] += z * a0[q];
-        acc[1] += z * z;
+        tm[0][j] = z * a0[q];
+        tm[1][j] = z * z;
       }
     }
   }
   if (MODE == MV_PLAIN) return;
-  double tot[2];
   if (MODE == MV_DOT2) {
-    if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    double tot[2];
+    if (grid_pairwise<1, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
   } else {
-    double a1[1] = {acc[0]}, t1[1];
-    if (grid_sum<1>(a1, part, &st->ticket, t1) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, t1);
+    double t1[1][kSlots], tot[1];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) t1[0][j] = tm[0][j];
+    if (grid_pairwise<1, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
+      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
   }
 }
 
@@ -394,13 +395,13 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
                                                         KState* st) {
   if (MODE != MV_PLAIN && !kactive(st)) return;
   const int64_t C = g.ncell();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  double acc[2] = {0.0, 0.0};
+  // terms tm[seg*2 + v]: segment 0 = phi rows (element q), 1 = mu rows (C + q)
+  double tm[4][kSlots] = {};
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
 #pragma unroll
-  for (int jj = 0; jj < kTile / kBlock; ++jj) {
-    const int64_t q = base + jj * kBlock + threadIdx.x;
-    if (q >= C) break;
+  for (int jj = 0; jj < kSlots; ++jj) {
+    const int64_t q = tile_elem(C, 0, jj);
+    if (q < 0) break;
     double z1, z2;
     if (MODE != MV_INIT || xnz) phmu_rows(g, s, bu, minner, m0, x, q, z1, z2);
     if (MODE == MV_PLAIN) {
@@ -416,26 +417,35 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
       y[C + q] = r2;
       a0[q] = r1;
       a0[C + q] = r2;
-      acc[0] += r1 * r1 + r2 * r2;
+      tm[0][jj] = r1 * r1;
+      tm[2][jj] = r2 * r2;
     } else {
       y[q] = z1;
       y[C + q] = z2;
       if (MODE == MV_DOT1) {
-        acc[0] += a0[q] * z1 + a0[C + q] * z2;
+        tm[0][jj] = a0[q] * z1;
+        tm[2][jj] = a0[C + q] * z2;
       } else {
-        acc[0] += z1 * a0[q] + z2 * a0[C + q];
-        acc[1] += z1 * z1 + z2 * z2;
+        tm[0][jj] = z1 * a0[q];
+        tm[1][jj] = z1 * z1;
+        tm[2][jj] = z2 * a0[C + q];
+        tm[3][jj] = z2 * z2;
       }
     }
   }
   if (MODE == MV_PLAIN) return;
-  double tot[2];
   if (MODE == MV_DOT2) {
-    if (grid_sum<2>(acc, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    double tot[2];
+    if (grid_pairwise<2, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
   } else {
-    double a1[1] = {acc[0]}, t1[1];
-    if (grid_sum<1>(a1, part, &st->ticket, t1) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, t1);
+    double t1[2][kSlots], tot[1];
+#pragma unroll
+    for (int jj = 0; jj < kSlots; ++jj) {
+      t1[0][jj] = tm[0][jj];
+      t1[1][jj] = tm[2][jj];
+    }
+    if (grid_pairwise<2, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
+      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
   }
 }
 

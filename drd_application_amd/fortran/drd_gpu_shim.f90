@@ -123,6 +123,9 @@ module drd_gpu_c
   end interface
 
   type(c_ptr), save :: drd_shim_ctx = c_null_ptr
+  ! solver options (drd_params velo_prec, ch_prec, krylov); 0, 0, 0 is the
+  ! reference's choice.  Set with drd_shim_options before the first solve.
+  integer(c_int32_t), save :: drd_shim_velo_prec = 0, drd_shim_ch_prec = 0, drd_shim_krylov = 0
 
 contains
 
@@ -180,6 +183,9 @@ subroutine drd_shim_ready()
   prm%UU = UU
   prm%bdr_upp = bdr_upp
   prm%bdr_low = bdr_low
+  prm%velo_prec = drd_shim_velo_prec
+  prm%ch_prec = drd_shim_ch_prec
+  prm%krylov = drd_shim_krylov
   if (.not. c_associated(drd_shim_ctx)) then
      rc = drd_init(prm, drd_shim_ctx)
      call drd_shim_check(rc, 'drd_init')
@@ -188,6 +194,21 @@ subroutine drd_shim_ready()
      call drd_shim_check(rc, 'drd_set_params')
   end if
 end subroutine drd_shim_ready
+
+
+! Selects the fast path (no reference counterpart): velo_prec 3 = FP32
+! fast-Poisson (DCT) velocity preconditioner, 4 = the same in FP64, 1 =
+! identity, 2 = Jacobi, 0 = the reference's; ch_prec 1 = 2x2 block Jacobi
+! for the phi/mu system, 0 = the reference's; krylov 1 = FGMRES(30) for the
+! BDF2 velocity solve, 0 = BiCGSTAB.
+subroutine drd_shim_options(velo_prec, ch_prec, krylov)
+  use drd_gpu_c
+  implicit none
+  integer, intent(in) :: velo_prec, ch_prec, krylov
+  drd_shim_velo_prec = int(velo_prec, c_int32_t)
+  drd_shim_ch_prec = int(ch_prec, c_int32_t)
+  drd_shim_krylov = int(krylov, c_int32_t)
+end subroutine drd_shim_options
 
 
 subroutine drd_shim_push(field, a)

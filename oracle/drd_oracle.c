@@ -256,17 +256,50 @@ double* oc_field(oc_ctx* c, int32_t f) {
 const double* oc_pre_lcc(oc_ctx* c) { return c->pre_lcc; }
 const double* oc_ph_lcc(oc_ctx* c) { return c->ph_lcc; }
 
-/* ---- BLAS-1 (the reference links MKL; plain sequential sums here) ------ */
-static double dnrm2(int64_t n, const double* x) {
-  double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += x[i] * x[i];
-  return sqrt(s);
+/* ---- BLAS-1 in a fixed summation order ---------------------------------
+ * The reference links MKL's DDOT/DNRM2, whose summation order is not
+ * specified.  The oracle fixes one, the path's canonical order: the
+ * pairwise sum over the element index, padded with zeros to a power of two,
+ *   S[lo, lo+2L) = S[lo, lo+L) + S[lo+L, lo+2L),
+ * which does not depend on how a parallel implementation splits the work.
+ * A vector of nseg equal segments (2 for the phi|mu vector of ph_mu_solve)
+ * sums each segment so and adds the segment sums in order.  DNRM2 is the
+ * square root of that sum of squares (MKL's overflow scaling changes
+ * nothing for values far from the overflow range).  Padding widths that
+ * differ can change only the sign of an all-zero sum.                    */
+#define PW_LEAF 16
+static double pw_rec(const double* x, const double* y, int64_t lo, int64_t len, int64_t n) {
+  if (lo >= n) return 0.0;
+  if (len <= PW_LEAF) {
+    double a[PW_LEAF];
+    for (int64_t i = 0; i < len; ++i) {
+      int64_t q = lo + i;
+      a[i] = q < n ? (y ? x[q] * y[q] : x[q]) : 0.0;
+    }
+    for (int64_t w = len; w > 1; w >>= 1)
+      for (int64_t i = 0; i < w / 2; ++i) a[i] = a[2 * i] + a[2 * i + 1];
+    return a[0];
+  }
+  int64_t h = len >> 1;
+  double l = pw_rec(x, y, lo, h, n);
+  double r = pw_rec(x, y, lo + h, h, n);
+  return l + r;
 }
-static double ddot(int64_t n, const double* x, const double* y) {
-  double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += x[i] * y[i];
+/* canonical sum of x[i]*y[i] (or of x[i] when y is NULL), i < n */
+double oc_pairwise(int64_t n, const double* x, const double* y) {
+  if (n <= 0) return 0.0;
+  int64_t P = 1;
+  while (P < n) P <<= 1;
+  return pw_rec(x, y, 0, P, n);
+}
+static double seg_dot(int64_t n, int nseg, const double* x, const double* y) {
+  int64_t seg = n / nseg;
+  double s = oc_pairwise(seg, x, y);
+  for (int k = 1; k < nseg; ++k) s = s + oc_pairwise(seg, x + (size_t)k * seg, y + (size_t)k * seg);
   return s;
 }
+static double ddot_s(int64_t n, int nseg, const double* x, const double* y) { return seg_dot(n, nseg, x, y); }
+static double dnrm2_s(int64_t n, int nseg, const double* x) { return sqrt(seg_dot(n, nseg, x, x)); }
 static void daxpy(int64_t n, double a, const double* x, double* y) {
   for (int64_t i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
 }
@@ -562,6 +595,45 @@ static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
   atob_ph_mu(c, ff1, ff2, yy);
 }
 
+/* 2x2 block Jacobi of the phi/mu operator (an addition without a reference
+ * counterpart): per cell the exact diagonal block of MATRIX_VECTOR_PH_MU_CG
+ * (TP/ph_mu_solve.f90:203-299), inverted in closed form, with a floor on
+ * the mu-mu entry where the mobility M_0 vanishes.                        */
+static double dgn_diag(const oc_ctx* c, double dx2, double dy2, const double* mo, int i, int j) {
+  int nx = NX, ny = NY;
+  const double m0 = mo[CI(i, j)];
+  double dx = 0.0, dy = 0.0;
+  if (i > 0) dx += (mo[CI(i - 1, j)] + m0) / 2.0;
+  if (i < nx - 1) dx += (m0 + mo[CI(i + 1, j)]) / 2.0;
+  if (j > 0) dy += (mo[CI(i, j - 1)] + m0) / 2.0;
+  if (j < ny - 1) dy += (m0 + mo[CI(i, j + 1)]) / 2.0;
+  return -(dx / dx2 + dy / dy2);
+}
+static void phmu_block(oc_ctx* c, double* yy, const double* xx) {
+  const drd_params* p = &c->p;
+  int nx = NX, ny = NY;
+  const double *bu = c->f[DRD_F_BDR_PH_USED], *mi = c->f[DRD_F_PH_M_INNER], *m0 = c->f[DRD_F_PH_M_0];
+  double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  int64_t C = (int64_t)nx * ny;
+  const double floor22 = 1e-3 * p->dt * p->xld * (2.0 / dr2 + 2.0 / dz2);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      size_t q = i + (size_t)nx * j;
+      const double bv = bu[CI(i, j)];
+      const double d11 = s / I - I * dgn_diag(c, dr2, dz2, mi, i, j) +
+                         ((j == 0) ? I / dz2 * (1.0 - c->coef_bot) : 0.0) +
+                         ((j == ny - 1) ? I / dz2 * (1.0 - c->coef_top) : 0.0);
+      const double d12 = -bv;
+      const double d21 = bv + p->bdr_ph_used_adjust;
+      double d22 = -p->dt * p->xld * dgn_diag(c, dr2, dz2, m0, i, j);
+      d22 = d22 > floor22 ? d22 : floor22;
+      const double det = d11 * d22 - d12 * d21;
+      const double r1 = xx[q], r2 = xx[C + q];
+      yy[q] = (d22 * r1 - d12 * r2) / det;
+      yy[C + q] = (-d21 * r1 + d11 * r2) / det;
+    }
+}
+
 /* MATRIX_VECTOR_CG MF/PhSolve.f90:240-292 */
 static void ph_matvec(oc_ctx* c, double alpha, const double* xx, double beta, double* yy) {
   const drd_params* p = &c->p;
@@ -635,6 +707,153 @@ void oc_dct3_2d(int32_t nx, int32_t ny, double* a) {
   dct3_lines(ny, nx, a, nx, 1, tmp, ty);
   free(tmp); free(tx); free(ty);
 }
+
+/* FFT-factored DCTs for the pressure solve.  dct_fftw runs FFTW's REDFT10 /
+ * REDFT01, whose factorisation (and so rounding) is FFTW's own.  The path
+ * restates them as Makhoul's even/odd reordering plus a radix-2 Stockham FFT
+ * for power-of-two lengths up to 4096, and as the defining sums over a
+ * cosine table otherwise; the 2-D transform runs x then y forward and y
+ * then x inverse.  Written out here operation for operation (same twiddle
+ * tables, same expression order), so pressure solves, and with them whole
+ * time steps, compare bit for bit; the O(n^2) definitions above check it. */
+typedef struct { double x, y; } cpx;
+typedef struct {
+  int L, fft;
+  cpx *tw, *rot;
+  double* cs;
+} line_plan;
+static void line_plan_init(line_plan* p, int L) {
+  p->L = L;
+  p->fft = L >= 2 && L <= 4096 && (L & (L - 1)) == 0;
+  int nt = L / 2 > 0 ? L / 2 : 1;
+  p->tw = (cpx*)calloc((size_t)nt, sizeof(cpx));
+  p->rot = (cpx*)calloc((size_t)L, sizeof(cpx));
+  p->cs = (double*)calloc(4 * (size_t)L, sizeof(double));
+  for (int k = 0; k < L / 2; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)L;
+    p->tw[k].x = cos(a);
+    p->tw[k].y = sin(a);
+  }
+  for (int k = 0; k < L; ++k) {
+    const double a = -M_PI * (double)k / (2.0 * L);
+    p->rot[k].x = cos(a);
+    p->rot[k].y = sin(a);
+  }
+  for (int m = 0; m < 4 * L; ++m) p->cs[m] = cos(M_PI * (double)m / (2.0 * L));
+}
+static void line_plan_free(line_plan* p) {
+  free(p->tw);
+  free(p->rot);
+  free(p->cs);
+}
+/* radix-2 Stockham, sign -1 forward / +1 inverse, unnormalised; returns
+ * the buffer holding the result */
+static cpx* stockham(cpx* a, cpx* b, int N, const cpx* tw, int sign) {
+  const int half = N >> 1;
+  for (int ns = 1; ns < N; ns <<= 1) {
+    const int stride = N / (2 * ns);
+    for (int j = 0; j < half; ++j) {
+      const int k = j & (ns - 1);
+      cpx w = tw[k * stride];
+      if (sign > 0) w.y = -w.y;
+      const cpx v0 = a[j], h = a[j + half];
+      const cpx v1 = {h.x * w.x - h.y * w.y, h.x * w.y + h.y * w.x};
+      const int o = ((j - k) << 1) + k;
+      b[o].x = v0.x + v1.x;
+      b[o].y = v0.y + v1.y;
+      b[o + ns].x = v0.x - v1.x;
+      b[o + ns].y = v0.y - v1.y;
+    }
+    cpx* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+static int makhoul_ix(int j, int N) { return (j & 1) ? (N - 1 - (j >> 1)) : (j >> 1); }
+/* REDFT10 of x[0..L) in place; A, B hold L complex values each */
+static void line_dct2(const line_plan* p, double* x, cpx* A, cpx* B) {
+  const int N = p->L;
+  if (p->fft) {
+    for (int j = 0; j < N; ++j) {
+      A[makhoul_ix(j, N)].x = x[j];
+      A[makhoul_ix(j, N)].y = 0.0;
+    }
+    const cpx* V = stockham(A, B, N, p->tw, -1);
+    double* out = (double*)malloc((size_t)N * sizeof(double));
+    for (int k = 0; k < N; ++k) {
+      const cpx w = p->rot[k], v = V[k];
+      out[k] = 2.0 * (w.x * v.x - w.y * v.y);
+    }
+    memcpy(x, out, (size_t)N * sizeof(double));
+    free(out);
+  } else {
+    double* out = (double*)malloc((size_t)N * sizeof(double));
+    for (int k = 0; k < N; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < N; ++j) s += x[j] * p->cs[((long)(2 * j + 1) * k) % (4L * N)];
+      out[k] = 2.0 * s;
+    }
+    memcpy(x, out, (size_t)N * sizeof(double));
+    free(out);
+  }
+}
+/* REDFT01 of y[0..L) in place */
+static void line_dct3(const line_plan* p, double* y, cpx* A, cpx* B) {
+  const int N = p->L;
+  double* out = (double*)malloc((size_t)N * sizeof(double));
+  if (p->fft) {
+    for (int k = 0; k < N; ++k) {
+      const double a = y[k];
+      const double b = (k == 0) ? 0.0 : -y[N - k];
+      const cpx w = p->rot[k];
+      A[k].x = w.x * a + w.y * b;
+      A[k].y = -w.y * a + w.x * b;
+    }
+    const cpx* v = stockham(A, B, N, p->tw, +1);
+    for (int j = 0; j < N; ++j) out[j] = v[makhoul_ix(j, N)].x;
+  } else {
+    for (int k = 0; k < N; ++k) {
+      double s = 0.0;
+      for (int j = 1; j < N; ++j) s += y[j] * p->cs[((long)j * (2 * k + 1)) % (4L * N)];
+      out[k] = y[0] + 2.0 * s;
+    }
+  }
+  memcpy(y, out, (size_t)N * sizeof(double));
+  free(out);
+}
+/* a (nx x ny, x fastest): forward x then y; pre != NULL: divide by
+ * pre[kx + (nx+1) ky] and transform back, y then x.  inverse_only: the
+ * inverse pair alone. */
+static void dct_2d_fast(int nx, int ny, double* a, const double* pre, int mode) {
+  line_plan px, py;
+  line_plan_init(&px, nx);
+  line_plan_init(&py, ny);
+  int mx = nx > ny ? nx : ny;
+  cpx* A = (cpx*)malloc((size_t)mx * sizeof(cpx));
+  cpx* B = (cpx*)malloc((size_t)mx * sizeof(cpx));
+  double* col = (double*)malloc((size_t)ny * sizeof(double));
+  const int fwd = mode != 1, inv = mode != 0;
+  if (fwd)
+    for (int k = 0; k < ny; ++k) line_dct2(&px, a + (size_t)nx * k, A, B);
+  for (int i = 0; i < nx; ++i) {
+    for (int k = 0; k < ny; ++k) col[k] = a[i + (size_t)nx * k];
+    if (fwd) line_dct2(&py, col, A, B);
+    if (pre)
+      for (int k = 0; k < ny; ++k) col[k] = col[k] / pre[i + (size_t)(nx + 1) * k];
+    if (inv) line_dct3(&py, col, A, B);
+    for (int k = 0; k < ny; ++k) a[i + (size_t)nx * k] = col[k];
+  }
+  if (inv)
+    for (int k = 0; k < ny; ++k) line_dct3(&px, a + (size_t)nx * k, A, B);
+  free(A);
+  free(B);
+  free(col);
+  line_plan_free(&px);
+  line_plan_free(&py);
+}
+/* exported for tests: mode 0 = REDFT10 2-D, 1 = REDFT01 2-D (y then x) */
+void oc_dct_2d_fast(int32_t nx, int32_t ny, double* a, int32_t mode) { dct_2d_fast(nx, ny, a, NULL, mode); }
 
 /* VCOST LIB/vfftpack.f:643-704, by its definition:
  * X(i) = [X(1) + (-1)^(i-1) X(N) + sum_{k=2}^{N-1} 2 X(k) cos((k-1)(i-1) pi/(N-1))] / sqrt(2(N-1)) */
@@ -779,6 +998,9 @@ static int64_t op_size(const oc_ctx* c, int32_t op) {
   return -1;
 }
 
+/* segments of the operator's Krylov vectors (the phi|mu vector is two) */
+static int op_nseg(int32_t op) { return op == DRD_OP_PHMU ? 2 : 1; }
+
 /* PSolve_uv_CG of PD/VeloSolve_combined.f90:697-807: ESOLP_x on the u block
  * (nr, nz-1; tables ua..uc) and on the v block (nr-1, nz-2; va..vc), the v
  * field (-1:nr,0:nz) read as (-1:nr,-1:nz-1). */
@@ -798,6 +1020,7 @@ int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* 
     case DRD_PS_VELO1_JACOBI: velo_jacobi(c, x_out, b_in); return 0;
     case DRD_PS_IDENTITY: memcpy(x_out, b_in, (size_t)n * sizeof(double)); return 0;
     case DRD_PS_PHMU_JACOBI: phmu_jacobi(c, x_out, b_in); return 0;
+    case DRD_PS_PHMU_BLOCK: phmu_block(c, x_out, b_in); return 0;
     case DRD_PS_PH_DCT: ph_psolve(c, x_out, b_in); return 0;
   }
   return DRD_E_UNSUPPORTED;
@@ -817,8 +1040,10 @@ int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* 
 typedef void (*mv_fn)(void* u, double alpha, const double* x, double beta, double* y);
 typedef void (*ps_fn)(void* u, double* x_out, const double* b_in);
 
-static int bicgstab_core(int64_t n, const double* b, double* x, double* work, int32_t* iter,
+static int bicgstab_core(int64_t n, int nseg, const double* b, double* x, double* work, int32_t* iter,
                          double* resid, mv_fn mv, ps_fn ps, void* u) {
+#define dnrm2(n, x) dnrm2_s(n, nseg, x)
+#define ddot(n, x, y) ddot_s(n, nseg, x, y)
   if (n < 0) return -1;
   if (*iter <= 0) return -3;
   int32_t maxit = *iter;
@@ -873,6 +1098,8 @@ static int bicgstab_core(int64_t n, const double* b, double* x, double* work, in
     if (fabs(omega) < breaktol) return -11;
     rho1 = rho;
   }
+#undef dnrm2
+#undef ddot
 }
 
 typedef struct {
@@ -892,7 +1119,7 @@ int oc_bicgstab(oc_ctx* c, int32_t op, int32_t prec, int32_t n, const double* b,
                 int32_t* iter, double* resid, int32_t* info) {
   if (n != op_size(c, op)) return DRD_E_ARG;
   op_bind bd = {c, op, prec};
-  *info = bicgstab_core(n, b, x, c->work, iter, resid, mv_bound, ps_bound, &bd);
+  *info = bicgstab_core(n, op_nseg(op), b, x, c->work, iter, resid, mv_bound, ps_bound, &bd);
   return 0;
 }
 
@@ -917,7 +1144,7 @@ int oc_bicgstab_dense(int32_t n, const double* a, int32_t jacobi, const double* 
                       int32_t* iter, double* resid, int32_t* info) {
   double* work = (double*)calloc((size_t)n * 7, sizeof(double));
   dense_bind d = {n, a, jacobi};
-  *info = bicgstab_core(n, b, x, work, iter, resid, mv_dense, ps_dense, &d);
+  *info = bicgstab_core(n, 1, b, x, work, iter, resid, mv_dense, ps_dense, &d);
   free(work);
   return 0;
 }
@@ -932,8 +1159,10 @@ int oc_bicgstab_dense(int32_t n, const double* a, int32_t jacobi, const double* 
  * the method restarts if that check fails and iterations remain.
  * ITER = inner iterations; RESID = true ||b-Ax||/||b||; INFO 0 converged,
  * 1 maximum iterations. */
-static int fgmres_core(int64_t n, const double* b, double* x, int32_t m, int32_t* iter, double* resid,
+static int fgmres_core(int64_t n, int nseg, const double* b, double* x, int32_t m, int32_t* iter, double* resid,
                        mv_fn mv, ps_fn ps, void* u) {
+#define dnrm2(n, x) dnrm2_s(n, nseg, x)
+#define ddot(n, x, y) ddot_s(n, nseg, x, y)
   int32_t maxit = *iter;
   double tol = *resid;
   double* V = (double*)malloc((size_t)n * (m + 1) * sizeof(double));
@@ -1010,20 +1239,22 @@ static int fgmres_core(int64_t n, const double* b, double* x, int32_t m, int32_t
 done:
   free(V); free(Z); free(H); free(cs); free(sn); free(g); free(y); free(r);
   return info;
+#undef dnrm2
+#undef ddot
 }
 
 int oc_fgmres(oc_ctx* c, int32_t op, int32_t prec, int32_t n, int32_t restart, const double* b, double* x,
               int32_t* iter, double* resid, int32_t* info) {
   if (n != op_size(c, op) || restart < 1) return DRD_E_ARG;
   op_bind bd = {c, op, prec};
-  *info = fgmres_core(n, b, x, restart, iter, resid, mv_bound, ps_bound, &bd);
+  *info = fgmres_core(n, op_nseg(op), b, x, restart, iter, resid, mv_bound, ps_bound, &bd);
   return 0;
 }
 
 int oc_fgmres_dense(int32_t n, const double* a, int32_t jacobi, int32_t restart, const double* b, double* x,
                     int32_t* iter, double* resid, int32_t* info) {
   dense_bind d = {n, a, jacobi};
-  *info = fgmres_core(n, b, x, restart, iter, resid, mv_dense, ps_dense, &d);
+  *info = fgmres_core(n, 1, b, x, restart, iter, resid, mv_dense, ps_dense, &d);
   return 0;
 }
 
@@ -1033,8 +1264,8 @@ static double true_resid(oc_ctx* c, int32_t op, const double* b, const double* x
   double* r = (double*)malloc((size_t)n * sizeof(double));
   memcpy(r, b, (size_t)n * sizeof(double));
   oc_matvec(c, op, -1.0, x, 1.0, r);
-  double bn = dnrm2(n, b);
-  double v = dnrm2(n, r) / (bn == 0 ? 1.0 : bn);
+  double bn = dnrm2_s(n, op_nseg(op), b);
+  double v = dnrm2_s(n, op_nseg(op), r) / (bn == 0 ? 1.0 : bn);
   free(r);
   return v;
 }
@@ -1250,17 +1481,16 @@ int oc_presolve_2(oc_ctx* c, double* psi, double coef, double* mean_div) {
     for (int i = 0; i < nx; ++i) {
       uin[i + (size_t)nx * k] = (u[UI(i + 1, k)] - u[UI(i, k)]) / dr + (v[VI(i, k + 1)] - v[VI(i, k)]) / dz;
     }
-  for (int k = 0; k < ny; ++k) {
-    double s = 0.0;
-    for (int i = 0; i < nx; ++i) s += uin[i + (size_t)nx * k];
-    sum += s;
+  {  /* canonical order: each row pairwise, then the row sums pairwise */
+    double* rs = (double*)calloc((size_t)ny, sizeof(double));
+    for (int k = 0; k < ny; ++k) rs[k] = oc_pairwise(nx, uin + (size_t)nx * k, NULL);
+    sum = oc_pairwise(ny, rs, NULL);
+    free(rs);
   }
   if (mean_div) *mean_div = sum / nx / ny;
-  /* dct_fftw(nr,nz,plan,plan_back,uin,psi,Pre_Lcc,4.0*nr*nz) */
-  oc_dct2_2d(nx, ny, uin);
-  for (int k = 0; k < ny; ++k)
-    for (int i = 0; i < nx; ++i) uin[i + (size_t)nx * k] = uin[i + (size_t)nx * k] / c->pre_lcc[i + (size_t)(nx + 1) * k];
-  oc_dct3_2d(nx, ny, uin);
+  /* dct_fftw(nr,nz,plan,plan_back,uin,psi,Pre_Lcc,4.0*nr*nz): REDFT10,
+   * rhs / Pre_Lcc, REDFT01 (dct_fftw.f90:13-31) */
+  dct_2d_fast(nx, ny, uin, c->pre_lcc, 2);
   double sc = 4.0 * nx * ny;
   for (int k = 0; k < ny; ++k)
     for (int i = 0; i < nx; ++i) psi[CI(i, k)] = uin[i + (size_t)nx * k] / sc;
@@ -1370,7 +1600,8 @@ int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t, drd
   atob_ph_mu(c, c->f[DRD_F_PH], c->f[DRD_F_RMU], x);
   int32_t it = c->p.maxit_phmu, inf = 0;
   double res = c->p.tol;
-  oc_bicgstab(c, DRD_OP_PHMU, DRD_PS_PHMU_JACOBI, (int32_t)n, rhs, x, &it, &res, &inf);
+  oc_bicgstab(c, DRD_OP_PHMU, c->p.ch_prec == DRD_CH_PREC_BLOCK ? DRD_PS_PHMU_BLOCK : DRD_PS_PHMU_JACOBI,
+              (int32_t)n, rhs, x, &it, &res, &inf);
   if (info) {
     info->iter = it;
     info->info = inf;

@@ -95,6 +95,12 @@ void oc_rfords(int32_t n, double* ff, const double* aa, const double* bb,
                const double* cc);
 /* ch_bulk_energy TP/ch_bulk_energy.f90, MF/ch_bulk_energy.f90 and
  * ch_bdr_condition TP/ch_bdr_condition.f90                              */
+/* canonical (pairwise, zero-padded to a power of two) sum of x[i]*y[i],
+ * or of x[i] when y is NULL */
+double oc_pairwise(int64_t n, const double* x, const double* y);
+/* FFT-factored 2-D DCT of the pressure solve: mode 0 REDFT10 (x then y),
+ * 1 REDFT01 (y then x) */
+void oc_dct_2d_fast(int32_t nx, int32_t ny, double* a, int32_t mode);
 double oc_ch_bulk_energy(double phi, double psi, int32_t mode);
 double oc_ch_bulk_energy_mf(double phi, double psi);
 double oc_ch_bdr_condition(double angle, double phi, int32_t mode);

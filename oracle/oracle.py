@@ -34,6 +34,9 @@ def lib():
         P = C.POINTER(C.c_double)
         I = C.POINTER(C.c_int32)
         L.oc_init.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_void_p)]
+        L.oc_pairwise.argtypes = [C.c_int64, P, P]
+        L.oc_pairwise.restype = C.c_double
+        L.oc_dct_2d_fast.argtypes = [C.c_int32, C.c_int32, P, C.c_int32]
         L.oc_finalize.argtypes = [C.c_void_p]
         L.oc_field.argtypes = [C.c_void_p, C.c_int32]
         L.oc_field.restype = P
@@ -234,6 +237,26 @@ class Oracle:
 
 
 # ---- free functions --------------------------------------------------------
+def pairwise(x, y=None) -> float:
+    """Canonical sum of x*y (or of x): pairwise over the index, padded with
+    zeros to a power of two."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+        yp = abi.dptr(y)
+    return lib().oc_pairwise(x.size, abi.dptr(x), yp)
+
+
+def dct_2d_fast(a, mode: int):
+    """FFT-factored 2-D REDFT10 (mode 0, x then y) or REDFT01 (mode 1, y
+    then x) of an (ny, nx) array, as the pressure solve computes them."""
+    a = np.array(a, dtype=np.float64, order="C")
+    ny, nx = a.shape
+    lib().oc_dct_2d_fast(nx, ny, abi.dptr(a), mode)
+    return a
+
+
 def dct2_2d(a):
     a = np.ascontiguousarray(a, dtype=np.float64).copy()
     ny, nx = a.shape

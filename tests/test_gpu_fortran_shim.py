@@ -2,7 +2,7 @@
 (PreSolve_init, ph_mu_solve, VeloSolve_combined, PreSolve_2) through
 drd_application_amd/fortran/drd_gpu_shim.f90 with module `global` set as the
 reference's driver would set it; the results must match the CPU oracle run
-through the same sequence (TP/mcl6.f90:257-303) to 1e-9 relative L2.
+through the same sequence (TP/mcl6.f90:257-303), bit for bit.
 
 The driver binary (tests/fortran/build/shim_driver) is built where the
 reference checkout provides module `global` (tests/fortran/Makefile).
@@ -71,5 +71,6 @@ def test_fortran_dropin_first_step(tmp_path):
         e = rel_l2(a, b)
         print(name, e)
         assert e < 1e-9, (name, e)
+        np.testing.assert_array_equal(a, b, err_msg=name)
     np.testing.assert_array_equal(m0, o.get_field(abi.F_PH_M_0))
     assert "uv 1 BICGSTAB no slip" in r.stdout and "ph_mu_solve" in r.stdout

@@ -3,10 +3,15 @@
 Bars:
 * stencil operators, preconditioners, packers and right-hand sides: bit-exact
   (both sides keep the reference's FP64 evaluation order, no contraction);
-* DCT-based pressure solve: 1e-12 relative (FFT vs. definition rounding);
-* Krylov solves: 1e-9 relative L2 on the solution at tol 1e-10 (only the
-  reduction order of the dot products differs);
-* 10 time steps: u, v, p, phi within 1e-9 relative L2 (BASELINE tolerance).
+* pressure solve: bit-exact (both sides use the same FFT factorisation of
+  FFTW's REDFT10/REDFT01; the oracle's O(n^2) definitions check it on CPU);
+* Krylov solves and whole time steps: bit-exact.  Every dot product and
+  norm is the canonical pairwise sum on both sides, so the BiCGSTAB
+  trajectory (ITER, RESID, INFO and x) is the same;
+* 10 time steps: u, v, p, phi within 1e-9 relative L2 (the stated bar),
+  and in fact identical.
+* PhSolve (MF): its VCOST preconditioner differs from the oracle's in
+  rounding, so the solve agrees to 1e-9 with ITER within 2.
 """
 import numpy as np
 import pytest
@@ -101,8 +106,8 @@ def test_presolve(nx, ny):
     g, o = pair(cfg)
     psi_g, md_g = g.PreSolve_2(2.5)
     psi_o, md_o = o.PreSolve_2(2.5)
-    assert rel_l2(psi_g, psi_o) < 1e-12
-    assert abs(md_g - md_o) <= 1e-9 * max(1.0, abs(md_o))
+    np.testing.assert_array_equal(psi_g, psi_o)
+    assert md_g == md_o
     g.close()
 
 
@@ -128,9 +133,10 @@ def test_velosolve_fortran_signature(bdf):
         uo, vo, io = o.VeloSolve_2_combined(*args)
     print("gpu", ig, "oracle", io)
     assert ig["info"] == io["info"] == 0
-    assert abs(ig["iter"] - io["iter"]) <= 2
+    assert ig["iter"] == io["iter"] and ig["resid"] == io["resid"]
     assert ig["resid"] <= p.tol and ig["true_resid"] < 10 * p.tol
-    assert rel_l2(ug, uo) < 1e-9 and rel_l2(vg, vo) < 1e-9
+    np.testing.assert_array_equal(ug, uo)
+    np.testing.assert_array_equal(vg, vo)
     g.close()
 
 
@@ -141,8 +147,10 @@ def test_ph_mu_solve():
     pg, mg, ig = g.ph_mu_solve(0.0)
     po, mo, io = o.ph_mu_solve(0.0)
     print("gpu", ig, "oracle", io)
-    assert ig["info"] == io["info"] == 0
-    assert rel_l2(pg, po) < 1e-9 and rel_l2(mg, mo) < 1e-9
+    assert ig["info"] == io["info"]
+    assert ig["iter"] == io["iter"] and ig["resid"] == io["resid"]
+    np.testing.assert_array_equal(pg, po)
+    np.testing.assert_array_equal(mg, mo)
     g.close()
 
 
@@ -168,8 +176,8 @@ def test_bicgstab_control_flow(op, prec):
     # a converging solve agrees with the oracle
     rg = g.bicgstab(op, prec, b, np.zeros(n), 2000, 1e-10)
     ro = o.bicgstab(op, prec, b, np.zeros(n), 2000, 1e-10)
-    assert rg[3] == ro[3] == 0 and abs(rg[1] - ro[1]) <= 2
-    assert rel_l2(rg[0], ro[0]) < 1e-9
+    assert rg[3] == ro[3] == 0 and rg[1] == ro[1] and rg[2] == ro[2]
+    np.testing.assert_array_equal(rg[0], ro[0])
     g.close()
 
 
@@ -215,8 +223,10 @@ def test_esolp_preconditioner(nx, ny):
     g.close()
 
 
-def test_ten_steps():
-    """u, v, p, phi after 10 steps within 1e-9 relative L2 of the oracle."""
+@pytest.fixture(scope="module")
+def ten_steps():
+    """u, v, p, phi, mu after 10 steps (BDF1, then BDF2) on the GPU and in
+    the oracle, with the reference's preconditioners."""
     cfg = geometry.tp_case(64, 32, lambda_eta=10.0, tol=1e-10)
     g, o = pair(cfg)
     t = 0.0
@@ -226,8 +236,19 @@ def test_ten_steps():
         io = o.step(bdf, t)
         print(f"step {s}", ig["phmu"]["iter"], io["phmu"]["iter"], ig["velo"]["iter"], io["velo"]["iter"], flush=True)
         t += cfg.params.dt
-    for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU):
-        e = rel_l2(g.get_field(fid), o.get_field(fid))
+    out = {fid: (g.get_field(fid), o.get_field(fid)) for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU)}
+    g.close()
+    return out
+
+
+def test_ten_steps(ten_steps):
+    """u, v, p, phi after 10 steps within 1e-9 relative L2 of the oracle."""
+    for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH):
+        e = rel_l2(*ten_steps[fid])
         print(abi.FIELD_NAMES[fid], e)
         assert e < 1e-9, (abi.FIELD_NAMES[fid], e)
-    g.close()
+
+
+def test_ten_steps_bitexact(ten_steps):
+    for fid, (a, b) in ten_steps.items():
+        np.testing.assert_array_equal(a, b, err_msg=abi.FIELD_NAMES[fid])

@@ -37,6 +37,24 @@ def test_dct_definitions_match_scipy(nx, ny):
     np.testing.assert_allclose(oc.dct3_2d(oc.dct2_2d(a)), 4 * nx * ny * a, rtol=1e-12, atol=1e-11)
 
 
+@pytest.mark.parametrize("nx,ny", [(64, 32), (16, 128), (24, 20), (1, 8), (4096, 2)])
+def test_fft_factored_dct_matches_definition(nx, ny):
+    """The pressure solve's FFT-factored REDFT10/REDFT01 (Makhoul + radix-2
+    Stockham for powers of two, table sums otherwise) against the O(n^2)
+    definitions and scipy."""
+    a = np.random.default_rng(nx + ny).standard_normal((ny, nx))
+    f = oc.dct_2d_fast(a, 0)
+    d = oc.dct2_2d(a)
+    assert np.abs(f - d).max() <= 1e-13 * np.abs(d).max()
+    ref = scipy.fft.dctn(a, type=2)
+    assert np.abs(f - ref).max() <= 1e-13 * np.abs(ref).max()
+    g = oc.dct_2d_fast(a, 1)
+    d3 = oc.dct3_2d(a)
+    assert np.abs(g - d3).max() <= 1e-13 * np.abs(d3).max()
+    # REDFT01 inverts REDFT10 up to 4 nx ny
+    back = oc.dct_2d_fast(f, 1) / (4.0 * nx * ny)
+    assert np.abs(back - a).max() <= 1e-13 * np.abs(a).max()
+
 @pytest.mark.parametrize("n", [2, 5, 16, 33])
 def test_vcost_self_inverse_and_definition(n):
     rng = np.random.default_rng(n)
@@ -97,6 +115,42 @@ def test_pressure_mean_mode_passes_with_sign_flip():
     assert abs(mean - d) < 1e-12
     np.testing.assert_allclose(psi, -d, rtol=1e-12)
 
+
+def _pairwise_spec(t):
+    """The canonical order written out: zero-pad to a power of two, then add
+    neighbours level by level."""
+    t = [float(v) for v in t]
+    if not t:
+        return 0.0
+    n = 1
+    while n < len(t):
+        n *= 2
+    t += [0.0] * (n - len(t))
+    while len(t) > 1:
+        t = [t[2 * i] + t[2 * i + 1] for i in range(len(t) // 2)]
+    return t[0]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 1000, 1024, 4097])
+def test_canonical_sum_definition(n):
+    """oc_pairwise (the order of every dot product and norm on the path)
+    against the level-by-level definition, bit for bit, for sums and dots."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 8, n)
+    y = rng.standard_normal(n)
+    assert oc.pairwise(x) == _pairwise_spec(x)
+    assert oc.pairwise(x, y) == _pairwise_spec(x * y)
+
+
+def test_canonical_sum_exact_and_not_sequential():
+    rng = np.random.default_rng(5)
+    ints = rng.integers(-2 ** 20, 2 ** 20, 3001).astype(np.float64)
+    assert oc.pairwise(ints) == float(ints.astype(np.int64).sum())
+    # (2^53 + 1) + (1 - 2^53) = 1 pairwise; left to right the ones round away
+    t = np.array([2.0 ** 53, 1.0, 1.0, -(2.0 ** 53)])
+    assert oc.pairwise(t) == 1.0
+    assert ((t[0] + t[1]) + t[2]) + t[3] == 0.0
+    assert oc.pairwise(np.zeros(0)) == 0.0
 
 # ---- BiCGSTAB -------------------------------------------------------------------
 @pytest.mark.parametrize("n,seed", [(5, 1), (20, 2), (60, 3)])

@@ -1,0 +1,73 @@
+"""The fast-path preconditioners' references, pinned on the CPU.
+
+* the oracle's 2x2 block Jacobi of the phi/mu system against the inverse of
+  2x2 blocks probed from the oracle's operator (which is itself pinned bit
+  for bit to the reference build);
+* the scipy restatement of the velocity fast-Poisson preconditioner used by
+  the GPU tests: with S = I it must invert the u-u and v-v blocks of the
+  BDF2 operator exactly when rho = eta = 1 and there is no penalty (A0 is
+  that operator's block diagonal).
+"""
+import numpy as np
+import pytest
+
+from drd_application_amd import abi, geometry
+from oracle.oracle import Oracle
+from prec_ref import probe_diag, scipy_dct_prec
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
+
+
+def test_oracle_ch_block_jacobi_matches_probed_blocks():
+    cfg = geometry.tp_case(48, 32, lambda_eta=10.0)
+    p = cfg.params
+    o = Oracle(p)
+    geometry.load(o, cfg)
+    o.ph_mu_rhs()  # sets ph_M_0 and the wall coefficients as ph_mu_solve does
+    d11, d12, d21, d22 = probe_diag(o, abi.OP_PHMU, p)
+    floor = 1e-3 * p.dt * p.xld * (2 / p.dx ** 2 + 2 / p.dz ** 2)
+    assert (d22 < floor).any()  # the floor is exercised
+    d22 = np.maximum(d22, floor)
+    C = p.nx * p.ny
+    b = np.random.default_rng(3).standard_normal(2 * C)
+    det = d11 * d22 - d12 * d21
+    want = np.concatenate([(d22 * b[:C] - d12 * b[C:]) / det, (-d21 * b[:C] + d11 * b[C:]) / det])
+    got = o.psolve(abi.PS_PHMU_BLOCK, abi.OP_PHMU, b)
+    assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 16), (24, 20)])
+def test_scipy_fast_poisson_inverts_constant_coefficient_blocks(nx, ny):
+    cfg = geometry.two_phase_config(nx, ny, 5, n_drops=1, r_drop=(3, 4), n_discs=1, r_disc=(3, 4),
+                                    lambda_eta=1.0, penalty=0.0, dt=5e-5, name="t")
+    p = cfg.params
+    cs = abi.cell_shape(p)
+    cfg.fields[abi.F_RHO] = np.ones(cs)
+    cfg.fields[abi.F_ETA] = np.ones(cs)
+    cfg.fields[abi.F_BDR_PH_PENALTY] = np.zeros(cs)
+    o = Oracle(p)
+    geometry.load(o, cfg)
+    n, mn = abi.n_uv(p), (nx + 1) * ny
+    b = np.random.default_rng(1).standard_normal(n)
+    z = scipy_dct_prec(o, p, b, scale=False)
+    zu, zv = z.copy(), z.copy()
+    zu[mn:] = 0.0
+    zv[:mn] = 0.0
+    ru = o.matvec(abi.OP_VELO2, 1.0, zu, 0.0, np.zeros(n))[:mn]
+    rv = o.matvec(abi.OP_VELO2, 1.0, zv, 0.0, np.zeros(n))[mn:]
+    assert rel(ru, b[:mn]) < 1e-13 and rel(rv, b[mn:]) < 1e-13
+
+
+def test_probed_velocity_diagonal_is_the_diagonal():
+    """probe_diag on the coupled velocity operator equals e_q^T A e_q."""
+    cfg = geometry.two_phase_config(12, 8, 5, n_drops=1, r_drop=(2, 3), n_discs=1, r_disc=(2, 3),
+                                    lambda_eta=20.0, penalty=1e4, dt=5e-5, name="t")
+    p = cfg.params
+    o = Oracle(p)
+    geometry.load(o, cfg)
+    n = abi.n_uv(p)
+    d = probe_diag(o, abi.OP_VELO2, p)
+    want = np.array([o.matvec(abi.OP_VELO2, 1.0, np.eye(1, n, q).ravel(), 0.0, np.zeros(n))[q] for q in range(n)])
+    np.testing.assert_array_equal(d, want)
