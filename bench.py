@@ -14,7 +14,6 @@ Prints one JSON line (see the `result` dict below).
 from __future__ import annotations
 
 import argparse
-import ctypes as C
 import json
 import os
 import sys
@@ -28,17 +27,6 @@ PEAK_HBM_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def bench_kernel(g, which: int, reps: int):
-    from drd_application_amd import solver
-
-    L = solver.lib()
-    L.drd_bench_kernel.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double),
-                                   C.POINTER(C.c_double)]
-    ms, by = C.c_double(0), C.c_double(0)
-    solver._check(L.drd_bench_kernel(g._h, which, reps, C.byref(ms), C.byref(by)))
-    return ms.value, by.value
 
 
 def cpu_baseline(cfg, velo_iters: float, phmu_iters: float, budget_s: float = 20.0):
@@ -86,8 +74,10 @@ MODES = {
 }
 
 
-def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False):
-    """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1)."""
+def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer: bool = False):
+    """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1).
+    With timer, every launch of the BDF2 velocity matvec (MV_DOT1) inside
+    the timed steps is bracketed by HIP events on the library's stream."""
     from drd_application_amd import abi, geometry
     from drd_application_amd.solver import GpuSolver
 
@@ -105,6 +95,8 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False):
         log(f"[{mode}] warmup step {s}: phmu {info['phmu']['iter']} ({info['phmu']['info']}), "
             f"velo {info['velo']['iter']} ({info['velo']['info']})")
     g.synchronize()
+    if timer:
+        g.kernel_timer(abi.K_VELO_MATVEC, 1 << 14)
     iters_v, iters_c, infos = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -120,10 +112,38 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False):
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
            "velo_iters": sum(iters_v) / steps, "phmu_iters": sum(iters_c) / steps,
            "converged": all(a == 0 and b == 0 for a, b in infos)}
+    if timer:
+        n, ms, by = g.kernel_timer_read()
+        g.kernel_timer(abi.K_VELO_MATVEC, 0)
+        out["timed_launches"] = {"count": n, "total_ms": ms, "bytes_per_launch": by}
     if keep:
         return out, g
     g.close()
     return out, None
+
+
+def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
+    """The velocity matvec and a plain copy of the same length, each launched
+    `reps` times after one BDF1 step has filled the fields; under
+    rocprofv3 --pmc the copy's FETCH_SIZE/WRITE_SIZE calibrate the counters."""
+    from drd_application_amd import abi
+    from drd_application_amd.solver import GpuSolver
+    from drd_application_amd import geometry
+
+    p = cfg.params.copy()
+    vp, cp = MODES[mode]
+    p.velo_prec, p.ch_prec = getattr(abi, vp), getattr(abi, cp)
+    g = GpuSolver(p)
+    geometry.load(g, cfg)
+    g.step(1, 0.0)
+    out = {}
+    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy")):
+        ms, by = g.bench_kernel(which, reps)
+        out[name] = {"us": ms * 1e3, "bytes": by, "launches": reps + 1}
+        log(f"{name}: {ms*1e3:.1f} us, {by/1e6:.1f} MB algorithmic")
+    g.close()
+    print(json.dumps(out))
+    return 0
 
 
 def main(argv=None) -> int:
@@ -137,25 +157,38 @@ def main(argv=None) -> int:
                     help="comma-separated modes timed beside the headline mode ('' for none)")
     ap.add_argument("--compare-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the stand-alone kernel loops on the config's fields (for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path "
               f"is single-GPU, see DESIGN.md for how it would shard", file=sys.stderr)
         return 2
-    from drd_application_amd import geometry
+    from drd_application_amd import abi, geometry
 
     cfg = geometry.config(args.config)
     p = cfg.params
     log(f"config {cfg.name}: {p.nx}x{p.ny}, tol {p.tol:g}, dt {p.dt:g}")
-    head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True)
-    ms_k, bytes_k = bench_kernel(g, 0, 50)
-    achieved = bytes_k / (ms_k * 1e-3) / 1e9
-    log(f"velocity matvec: {ms_k*1e3:.1f} us, {bytes_k/1e6:.1f} MB -> {achieved:.0f} GB/s")
+    if args.profile_kernels:
+        return profile_kernels(cfg, args.mode)
+    head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=True)
+    tl = head.pop("timed_launches")
+    if tl["count"] == 0:
+        log("the headline mode launched no BDF2 velocity matvec; no roofline")
+        ms_k, bytes_k, achieved = None, tl["bytes_per_launch"], None
+    else:
+        ms_k = tl["total_ms"] / tl["count"]
+        bytes_k = tl["bytes_per_launch"]
+        achieved = bytes_k / (ms_k * 1e-3) / 1e9
+        log(f"velocity matvec in the timed steps: {tl['count']} launches, {ms_k*1e3:.1f} us each, "
+            f"{bytes_k/1e6:.1f} MB -> {achieved:.0f} GB/s")
+    # stand-alone timings of other kernels on the final state (same launch shapes)
     kern = {}
-    for which, name in ((1, "phmu_matvec"), (2, "bicg_xr"), (3, "pressure")):
-        ms, by = bench_kernel(g, which, 20)
+    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
+                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure")):
+        ms, by = g.bench_kernel(which, 20)
         kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
-        log(f"{name}: {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")
+        log(f"{name} (stand-alone): {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")
     g.close()
     modes = {args.mode: head}
     for m in [s for s in args.compare.split(",") if s and s != args.mode]:
@@ -188,9 +221,10 @@ def main(argv=None) -> int:
                                          if "dropin" in modes and args.mode != "dropin" else None),
                    "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
                      "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
-                     "us_per_launch": ms_k * 1e3},
+                     "us_per_launch": ms_k * 1e3 if ms_k else None,
+                     "launches_timed": tl["count"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))

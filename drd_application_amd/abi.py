@@ -50,6 +50,8 @@ PS_PHMU_BLOCK = 5
 PS_VELO_DCT64 = 6
 PS_VELO_ESOLP = 7
 
+K_VELO_MATVEC, K_PHMU_MATVEC, K_BICG_XR, K_PRESSURE, K_COPY = 0, 1, 2, 3, 4
+
 F_U, F_V, F_P, F_PH, F_RMU, F_RHO, F_ETA = 0, 1, 2, 3, 4, 5, 6
 F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10
 F_PH_M_INNER, F_PH_M_0, F_ANGLEB, F_ANGLET = 11, 12, 13, 14

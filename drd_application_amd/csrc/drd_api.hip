@@ -44,6 +44,9 @@ struct drd_ctx {
   Grid g;
   Scal s;
   hipStream_t stream = nullptr;
+  // drd_kernel_timer: event pairs around the instrumented kernel's launches
+  int tm_which = -1, tm_n = 0;
+  std::vector<hipEvent_t> tm_ev;
   double* f[DRD_F_COUNT] = {};
   // per-step scratch fields
   double *u_after = nullptr, *v_after = nullptr, *ph_after = nullptr, *rmu_after = nullptr;
@@ -305,6 +308,8 @@ static int ctx_alloc(drd_ctx* c) {
 }
 
 static void ctx_free(drd_ctx* c) {
+  for (auto e : c->tm_ev) hipEventDestroy(e);
+  c->tm_ev.clear();
   for (auto& a : c->f) hipFree(a);
   hipFree(c->u_after);
   hipFree(c->v_after);
@@ -359,7 +364,9 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
     return set_err(DRD_E_HIP, "no HIP device visible");
   drd_ctx* c = new drd_ctx();
   c->p = *prm;
-  c->g = Grid{prm->nx, prm->ny};
+  if ((int64_t)(prm->nx + 3) * (prm->ny + 2) >= ((int64_t)1 << 31))
+    return set_err(DRD_E_UNSUPPORTED, "grid too large for 32-bit indexing");
+  c->g = Grid::make(prm->nx, prm->ny);
   c->s = make_scal(*prm);
   int rc = 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
@@ -483,10 +490,19 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
                        ks);
       break;
     case DRD_OP_VELO1:
-    case DRD_OP_VELO2:
+    case DRD_OP_VELO2: {
+      const bool timed = op == DRD_OP_VELO2 && mode == MV_DOT1 && c->tm_which == DRD_K_VELO_MATVEC &&
+                         2 * (size_t)c->tm_n < c->tm_ev.size();
+      if (timed) HIPCHK(hipEventRecord(c->tm_ev[2 * c->tm_n], c->stream));
       launch_velo_matvec(c->stream, op == DRD_OP_VELO1 ? 1 : 2, mode, c->g, c->s, k, x, y, alpha, beta, a0, b,
-                         c->part, ks);
+                         c->part, ks, !timed);
+      if (timed) {
+        HIPCHK(hipEventRecord(c->tm_ev[2 * c->tm_n + 1], c->stream));
+        ++c->tm_n;
+        launch_mv_finalize(c->stream, mode, c->part, tiles_of(c->g.nuv()), 1, ks);
+      }
       break;
+    }
     case DRD_OP_PHMU:
       launch_phmu_matvec(c->stream, mode, c->g, c->s, k, x, y, alpha, beta, a0, b, c->part, ks);
       break;
@@ -669,7 +685,6 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
   if (m < 1 || m > 200) return set_err(DRD_E_ARG, "restart must be in [1, 200]");
   RCHK(fgmres_alloc(c, m));
   RCHK(prec_setup(c, prec));
-  uint32_t* ticket = &c->ks->ticket;
   double* V = c->fg_V;
   double* Z = c->fg_Z;
   const int64_t ld = n;
@@ -684,7 +699,7 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
   // ||b||
   const int ns = op_nseg(op);
   double* fpart = c->fg_part;
-  launch_multidot(c->stream, n, ns, b, b, ld, 1, fpart, ticket, hd);
+  launch_multidot(c->stream, n, ns, b, b, ld, 1, fpart, hd);
   RCHK(sync_read(1));
   const double bn = std::sqrt(hh[0]);
   drd_solve_info loc{0, 1, tol, 0.0};
@@ -700,7 +715,7 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
     // r = b - A x into V_0, beta = ||r||
     launch_copy(c->stream, n, b, V);
     RCHK(apply_op(c, op, MV_PLAIN, x, V, -1.0, 1.0, nullptr, nullptr, nullptr));
-    launch_multidot(c->stream, n, ns, V, V, ld, 1, fpart, ticket, hd);
+    launch_multidot(c->stream, n, ns, V, V, ld, 1, fpart, hd);
     RCHK(sync_read(1));
     const double beta = std::sqrt(hh[0]);
     loc.resid = beta / bn;
@@ -726,16 +741,16 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
       if (zr != zj) launch_copy(c->stream, n, zr, zj);
       RCHK(apply_op(c, op, MV_PLAIN, zj, w, 1.0, 0.0, nullptr, nullptr, nullptr));
       // pass 1: h = V^T w and ||w||^2 (w = V_{j+1} is contiguous with V)
-      launch_multidot(c->stream, n, ns, w, V, ld, j + 2, fpart, ticket, hd);
-      launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd, -1.0, fpart, ticket, hd + 2 * (m + 2), 1);
+      launch_multidot(c->stream, n, ns, w, V, ld, j + 2, fpart, hd);
+      launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd, -1.0, fpart, hd + 2 * (m + 2), 1);
       RCHK(sync_read(2 * (m + 2) + 1));
       double* hcol = &H[(size_t)(m + 1) * j];
       for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
       const double before = hh[j + 1];
       double after = hh[2 * (m + 2)];
       if (after < 0.25 * before) {  // pass 2
-        launch_multidot(c->stream, n, ns, w, V, ld, j + 1, fpart, ticket, hd + (m + 2));
-        launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd + (m + 2), -1.0, fpart, ticket, hd + 2 * (m + 2),
+        launch_multidot(c->stream, n, ns, w, V, ld, j + 1, fpart, hd + (m + 2));
+        launch_multiaxpy(c->stream, n, ns, w, V, ld, j + 1, hd + (m + 2), -1.0, fpart, hd + 2 * (m + 2),
                          1);
         RCHK(sync_read(2 * (m + 2) + 1));
         for (int i = 0; i <= j; ++i) hcol[i] += hh[(m + 2) + i];
@@ -767,7 +782,7 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
     }
     for (int i = 0; i < jdone; ++i) hh[i] = y[i];
     HIPCHK(hipMemcpyAsync(hd, hh, jdone * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    launch_multiaxpy(c->stream, n, ns, x, Z, ld, jdone, hd, 1.0, fpart, ticket, nullptr, 0);
+    launch_multiaxpy(c->stream, n, ns, x, Z, ld, jdone, hd, 1.0, fpart, nullptr, 0);
   }
   HIPCHK(hipGetLastError());
   loc.iter = it;
@@ -1055,6 +1070,34 @@ extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32
 }
 
 // ---- measurement hook ---------------------------------------------------------
+extern "C" int drd_kernel_timer(drd_ctx* c, int32_t which, int32_t capacity) {
+  if (!c || capacity < 0) return set_err(DRD_E_ARG, "bad argument");
+  if (capacity > 0 && which != DRD_K_VELO_MATVEC) return set_err(DRD_E_UNSUPPORTED, "kernel not instrumented");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto e : c->tm_ev) HIPCHK(hipEventDestroy(e));
+  c->tm_ev.clear();
+  c->tm_n = 0;
+  c->tm_which = capacity > 0 ? which : -1;
+  c->tm_ev.resize(2 * (size_t)capacity);
+  for (auto& e : c->tm_ev) HIPCHK(hipEventCreate(&e));
+  return 0;
+}
+
+extern "C" int drd_kernel_timer_read(drd_ctx* c, int32_t* count, double* total_ms, double* bytes) {
+  if (!c || !count || !total_ms || !bytes) return set_err(DRD_E_ARG, "bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  for (int i = 0; i < c->tm_n; ++i) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->tm_ev[2 * i], c->tm_ev[2 * i + 1]));
+    tot += ms;
+  }
+  *count = c->tm_n;
+  *total_ms = tot;
+  *bytes = 8.0 * (3.0 * (double)c->g.nuv() + 3.0 * (double)c->g.ncell());  // x, y, rtld + rho, eta, chi
+  return 0;
+}
+
 extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double* avg_ms, double* bytes) {
   if (!c || !avg_ms || !bytes || reps <= 0) return set_err(DRD_E_ARG, "bad argument");
   const Grid& g = c->g;
@@ -1072,19 +1115,22 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
     switch (which) {
       case DRD_K_VELO_MATVEC:
         launch_velo_matvec(c->stream, 2, MV_DOT1, g, c->s, k, c->kw[2], c->kw[3], 1.0, 0.0, c->kw[1], nullptr,
-                           c->part, c->ks);
+                           c->part, c->ks, false);
         break;
       case DRD_K_PHMU_MATVEC:
         launch_phmu_matvec(c->stream, MV_DOT1, g, c->s, k, c->kw[2], c->kw[3], 1.0, 0.0, c->kw[1], nullptr,
-                           c->part, c->ks);
+                           c->part, c->ks, false);
         break;
       case DRD_K_BICG_XR:
         launch_bicg_xr(c->stream, g.nuv(), 1, c->xv, c->kw[0], c->kw[1], c->kw[5], c->kw[6], c->kw[4], c->part,
-                       c->ks);
+                       c->ks, false);
         break;
       case DRD_K_PRESSURE:
         launch_pressure(c->stream, c->dct, g, c->s, c->f[DRD_F_U], c->f[DRD_F_V], c->pre_lcc, 1.0, c->arg[0],
                         nullptr, c->dct_work, c->part, nullptr);
+        break;
+      case DRD_K_COPY:
+        launch_copy(c->stream, g.nuv(), c->kw[2], c->kw[3]);
         break;
     }
   };
@@ -1100,6 +1146,9 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       break;
     case DRD_K_PRESSURE:  // u, v in; psi out (p untouched here)
       *bytes = N8 + 2 * C8 + C8;
+      break;
+    case DRD_K_COPY:
+      *bytes = 2 * N8;
       break;
     default:
       return set_err(DRD_E_ARG, "unknown kernel id");

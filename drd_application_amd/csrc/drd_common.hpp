@@ -18,8 +18,30 @@ namespace drd {
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kWaves = kBlock / 64;
 
+// Exact floor(n / d) for 0 <= n < 2^31 by multiply and shift (Granlund and
+// Montgomery, PLDI 1994): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (mulhi(m, n) + n) >> l.  Replaces the software integer division
+// in the index decode of every stencil thread.
+struct FastDiv {
+  uint32_t m;
+  int l;
+  __host__ static FastDiv make(uint32_t d) {
+    int s = 0;
+    while ((1ull << s) < d) ++s;
+    return FastDiv{(uint32_t)(((1ull << 32) * ((1ull << s) - d)) / d + 1), s};
+  }
+  __device__ __forceinline__ int div(int64_t n) const {
+    const uint32_t u = (uint32_t)n;
+    return (int)((__umulhi(m, u) + u) >> l);
+  }
+};
+
 struct Grid {
   int nx, ny;
+  FastDiv by_nx1, by_nx;  // division by nx + 1 and by nx
+  __host__ static Grid make(int nx, int ny) {
+    return Grid{nx, ny, FastDiv::make((uint32_t)nx + 1), FastDiv::make((uint32_t)nx)};
+  }
   __host__ __device__ int ldc() const { return nx + 2; }
   __host__ __device__ int ldu() const { return nx + 3; }
   __host__ __device__ int64_t nc() const { return (int64_t)(nx + 2) * (ny + 2); }
@@ -50,8 +72,6 @@ struct KState {
   double breaktol;
   double red[8];      // finished reductions of the last phase
   int32_t iter, maxit, info, done, early, xnz;
-  uint32_t ticket;    // last-block counter of the running reduction
-  int32_t pad;
 };
 
 // ---- canonical reductions ---------------------------------------------------
@@ -125,67 +145,78 @@ __device__ __forceinline__ void tile_pairwise(const double (&t)[K][kSlots], doub
   __syncthreads();
 }
 
-// Pairwise sum over tiles 0..nt-1 of part[tile*stride + k] (nt <= kMaxTiles):
-// chunks of kBlock tiles, then the chunk sums.  Result on thread 0.
+// Pairwise sum of 16 values in registers (zeros pad shorter runs: a
+// subtree sum plus zeros keeps its value).
+__device__ __forceinline__ double pairwise16(double (&r)[16]) {
+#pragma unroll
+  for (int w = 16; w > 1; w >>= 1)
+#pragma unroll
+    for (int i = 0; i < w / 2; ++i) r[i] = r[2 * i] + r[2 * i + 1];
+  return r[0];
+}
+
+// Pairwise sum over tiles 0..nt-1 of part[tile*stride + k] (nt <= kMaxTiles).
+// With ntp = nt rounded up to a power of two, thread t sums the aligned run
+// [t P, (t+1) P), P = max(1, ntp / kBlock), in registers (runs of 16, then
+// the run sums), and the workgroup adds the kBlock run sums pairwise.
+// Result on thread 0.  The partials come from an earlier kernel, so plain
+// loads see them.
 __device__ __forceinline__ double tiles_pairwise(const double* part, uint32_t nt, int stride, int k) {
-  __shared__ double cs[kBlock];
   uint32_t ntp = 1;
   while (ntp < nt) ntp <<= 1;
-  const uint32_t nch = ntp > (uint32_t)kBlock ? ntp / kBlock : 1u;
-  for (uint32_t c = 0; c < nch; ++c) {
-    const uint32_t q = c * kBlock + threadIdx.x;
-    const double v =
-        q < nt ? __hip_atomic_load(part + (size_t)q * stride + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-    const double r = block_pairwise(v);
-    if (threadIdx.x == 0) cs[c] = r;
+  const uint32_t P = ntp > (uint32_t)kBlock ? ntp / kBlock : 1u;  // 1 .. 256
+  const uint32_t base = threadIdx.x * P;
+  const uint32_t runs = P > 16u ? P / 16u : 1u;                     // 1 .. 16
+  const uint32_t len = P > 16u ? 16u : P;
+  double g[16];
+#pragma unroll
+  for (int gi = 0; gi < 16; ++gi) {
+    g[gi] = 0.0;
+    if ((uint32_t)gi < runs) {
+      double r[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const uint32_t q = base + (uint32_t)gi * 16u + (uint32_t)i;
+        r[i] = ((uint32_t)i < len && q < nt) ? part[(size_t)q * stride + k] : 0.0;
+      }
+      g[gi] = pairwise16(r);
+    }
   }
-  __syncthreads();
-  const double v = threadIdx.x < nch ? cs[threadIdx.x] : 0.0;
-  __syncthreads();
-  return block_pairwise(v);
+  return block_pairwise(pairwise16(g));
+}
+
+// Tile of this workgroup.  The dispatcher deals workgroup b to XCD b % 8;
+// XCD x is given the contiguous tiles [start_x, start_x + count_x), in
+// order, so the rows a stencil tile shares with its neighbours are already
+// in that XCD's L2 instead of being fetched once per XCD.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t tile_id() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t per = nb / kXcds, rem = nb % kXcds;
+  const uint32_t x = b % kXcds, j = b / kXcds;
+  return x * per + (x < rem ? x : rem) + j;
 }
 
 // Index of slot j of segment s in this workgroup's tile, or -1 past the end
 // of the segment (segments of length seg, laid out one after the other).
 __device__ __forceinline__ int64_t tile_elem(int64_t seg, int s, int j) {
-  const int64_t o = (int64_t)blockIdx.x * kTile + j * kBlock + threadIdx.x;
+  const int64_t o = (int64_t)tile_id() * kTile + j * kBlock + threadIdx.x;
   return o < seg ? (int64_t)s * seg + o : -1;
 }
 
-// Publishes this workgroup's partial sums (value k of segment s at
-// part[tile*(NS*NV) + s*NV + k]) and returns true in the workgroup that
-// finishes last; its thread 0 then holds in tot[k] the canonical total
-// (segment sums added in segment order).  One workgroup per tile, 1-D grid.
+// Publishes this workgroup's partial sums: value v of segment s at
+// part[tile*(NS*NV) + s*NV + v].  A one-workgroup kernel launched after the
+// producer (k_finalize) adds them up; the kernel boundary makes them
+// visible across the XCDs' L2 caches without a per-workgroup fence.
 template <int NS, int NV>
-__device__ __forceinline__ bool grid_pairwise(const double (&t)[NS * NV][kSlots], double* part, uint32_t* ticket,
-                                              double (&tot)[NV]) {
+__device__ __forceinline__ void tile_publish(const double (&t)[NS * NV][kSlots], double* part) {
   constexpr int K = NS * NV;
   double p[K];
   tile_pairwise<K>(t, p);
-  const uint32_t nt = gridDim.x;
-  __shared__ int is_last;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * K + k] = p[k];
-    __threadfence();
-    is_last = (atomicAdd(ticket, 1u) == nt - 1);
+    for (int k = 0; k < K; ++k) part[(size_t)tile_id() * K + k] = p[k];
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < K; ++k) p[k] = tiles_pairwise(part, nt, K, k);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      double s = p[v];
-#pragma unroll
-      for (int sg = 1; sg < NS; ++sg) s = s + p[sg * NV + v];
-      tot[v] = s;
-    }
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return true;
 }
 
 }  // namespace drd

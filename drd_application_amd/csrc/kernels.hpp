@@ -100,13 +100,32 @@ __device__ inline void kfinalize(KState* st, int phase, const double* t) {
   st->red[0] = t[0];
 }
 
+// ---- reduction finish (kernels_krylov.hip) ----
+// k_finalize runs as one workgroup after a reducing kernel: it adds the
+// tile partials (layout [tile][segment][value]) in the canonical order, then
+// either applies the BiCGSTAB decisions of `phase` (kfinalize) or, for
+// PH_NONE, writes the nv totals to out.  `guard` repeats the producer's
+// own skip test so a finished solve stays untouched.
+enum { FIN_ALWAYS = 0, FIN_ACTIVE = 1, FIN_EARLY = 2 };
+void launch_finalize(hipStream_t st, const double* part, int64_t ntiles, int nseg, int nv, double* out,
+                     KState* ks, int phase, int guard);
+inline unsigned tiles_of(int64_t n) { return (unsigned)((n + kTile - 1) / kTile); }
+// the finish of a fused matvec sweep (nothing for MV_PLAIN)
+inline void launch_mv_finalize(hipStream_t st, int mode, const double* part, int64_t ntiles, int nseg,
+                               KState* ks) {
+  if (mode == MV_DOT1) launch_finalize(st, part, ntiles, nseg, 1, nullptr, ks, PH_MV1, FIN_ACTIVE);
+  if (mode == MV_DOT2) launch_finalize(st, part, ntiles, nseg, 2, nullptr, ks, PH_MV2, FIN_ACTIVE);
+  if (mode == MV_INIT) launch_finalize(st, part, ntiles, nseg, 1, nullptr, ks, PH_INIT_R, FIN_ACTIVE);
+}
+
 // ---- launchers (kernels_stencil.hip) ----
+// finish = false leaves the partial sums unreduced (kernel timing only)
 void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks);
+                        const double* b, double* part, KState* ks, bool finish = true);
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks);
+                        const double* b, double* part, KState* ks, bool finish = true);
 void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag);
 void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,
                      const KState* ks);
@@ -126,14 +145,15 @@ void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const doub
 
 // ---- launchers (kernels_krylov.hip) ----
 // Reducing kernels take the vector's segment count (1, or 2 for the phi|mu
-// vector of the Cahn-Hilliard system; see grid_pairwise).
+// vector of the Cahn-Hilliard system; see tile_publish).
 void launch_init_norms(hipStream_t st, int64_t n, int nseg, const double* x, const double* b, double* part,
                        KState* ks);
 void launch_bicg_p(hipStream_t st, int64_t n, const double* r, const double* v, double* p, KState* ks);
 void launch_bicg_s(hipStream_t st, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks);
 void launch_bicg_early(hipStream_t st, int64_t n, double* x, const double* ph, double* part, KState* ks);
 void launch_bicg_xr(hipStream_t st, int64_t n, int nseg, double* x, double* r, const double* rt,
-                    const double* ph, const double* sh, const double* t, double* part, KState* ks);
+                    const double* ph, const double* sh, const double* t, double* part, KState* ks,
+                    bool finish = true);
 void launch_copy(hipStream_t st, int64_t n, const double* a, double* b);
 void launch_axpby(hipStream_t st, int64_t n, double alpha, const double* x, double beta, double* y);
 // sum of squares of (b - ax) and of b -> ks->red[0], red[1] (FP64, canonical order)
@@ -143,9 +163,9 @@ void launch_cell_axpy(hipStream_t st, int64_t n, const double* a, double* b);  /
 // FGMRES: res[i] = (w, V_i); w += sign * sum h_i V_i (and ||w||^2); b = a / s.
 // part holds nv * nseg partials per tile of kTile elements.
 void launch_multidot(hipStream_t st, int64_t n, int nseg, const double* w, const double* V, int64_t ldv, int nv,
-                     double* part, uint32_t* ticket, double* res);
+                     double* part, double* res);
 void launch_multiaxpy(hipStream_t st, int64_t n, int nseg, double* w, const double* V, int64_t ldv, int nv,
-                      const double* h, double sign, double* part, uint32_t* ticket, double* res, int want_norm);
+                      const double* h, double sign, double* part, double* res, int want_norm);
 void launch_scale_into(hipStream_t st, int64_t n, const double* a, double s, double* b);
 
 // ---- fast-path preconditioners (kernels_prec.hip) ----

@@ -304,23 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
       if (i < N) a.out[(size_t)row * N + i] = r[m];
     }
   }
-  if (MODE == R_DIV_DCT2 && a.ks) {  // rows summed pairwise by the last workgroup
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-      a.part[row] = rowsum;
-      __threadfence();
-      is_last = (atomicAdd(&a.ks->ticket, 1u) == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-      __threadfence();
-      const double tot = tiles_pairwise(a.part, gridDim.x, 1, 0);
-      if (threadIdx.x == 0) {
-        a.ks->red[0] = tot;
-        __hip_atomic_store(&a.ks->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-  }
+  if (MODE == R_DIV_DCT2 && a.ks && threadIdx.x == 0) a.part[row] = rowsum;  // rows summed by k_finalize
 }
 
 // out (cols x rows) = transpose of in (rows x cols), 32x32 LDS tiles
@@ -422,6 +406,7 @@ void launch_pressure(hipStream_t st, DctPlan* pl, const Grid& g, const Scal& s, 
   a.part = part;
   a.ks = ks;
   launch_rows<R_DIV_DCT2>(st, a);
+  if (ks) launch_finalize(st, part, a.rows, 1, 1, ks->red, nullptr, PH_NONE, FIN_ALWAYS);
   launch_transpose(st, g.ny, g.nx, w0, w1);
   RowArgs b = rows_y(pl);
   b.g = g;

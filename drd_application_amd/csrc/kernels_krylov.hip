@@ -3,10 +3,11 @@
 // Each iteration of LIB/BiCGSTABREVCOM.f:257-388 runs as
 //   p-update | M^-1 | A + (rtld,v) | s-update + ||s|| | [early x] | M^-1 |
 //   A + (t,s),(t,t) | x,r-update + ||r||,(rtld,r)
-// with every reduction finished by the last workgroup of its kernel
-// (grid_sum), which also takes the branch decisions (kfinalize).  The
-// kernels test KState::done/early first, so the host may queue whole
-// iterations ahead and only polls `done` now and then.
+// where each reducing kernel publishes per-tile partial sums and a
+// one-workgroup k_finalize adds them in the canonical order and takes the
+// branch decisions (kfinalize).  The kernels test KState::done/early first,
+// so the host may queue whole iterations ahead and only polls `done` now
+// and then.
 //
 // The vector updates keep DAXPY's y + a*x rounding (no contraction), so
 // they reproduce the CPU oracle bit for bit; only the reduction order differs.
@@ -31,8 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_init_norms(int64_t seg, const double
       t[s * 2][j] = a * a;
       t[s * 2 + 1][j] = c * c;
     }
-  double tot[2];
-  if (grid_pairwise<NS, 2>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_INIT_NORMS, tot);
+  tile_publish<NS, 2>(t, part);
 }
 
 // p = r (first iteration) or p = beta*(p - omega*v) + r   REVCOM:266-275
@@ -78,14 +78,12 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
       }
       t[s][j] = sv * sv;
     }
-  double tot[1];
-  if (grid_pairwise<NS, 1>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_S, tot);
+  tile_publish<NS, 1>(t, part);
 }
 
 // early exit: x = x + alpha*phat                   REVCOM:314
 __global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __restrict__ x,
-                                                       const double* __restrict__ ph, double* part,
-                                                       KState* st) {
+                                                       const double* __restrict__ ph, const KState* st) {
   if (!st->early || st->done) return;
   const double alpha = st->alpha;
 #pragma unroll
@@ -93,8 +91,6 @@ __global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __rest
     const int64_t q = tile_elem(n, 0, j);
     if (q >= 0) x[q] = x[q] + alpha * ph[q];
   }
-  double t[1][kSlots] = {}, tot[1];  // only to find the last workgroup
-  if (grid_pairwise<1, 1>(t, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_EARLY, tot);
 }
 
 // x = x + alpha*phat + omega*shat; r = s - omega*t; ||r||^2, (rtld,r)
@@ -126,8 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restr
       tm[s * 2][j] = rv * rv;
       tm[s * 2 + 1][j] = rtv * rv;
     }
-  double tot[2];
-  if (grid_pairwise<NS, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_XR, tot);
+  tile_publish<NS, 2>(tm, part);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -163,21 +158,16 @@ __global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t seg, const doubl
       t[s * 2][j] = d * d;
       t[s * 2 + 1][j] = bv * bv;
     }
-  double tot[2];
-  if (grid_pairwise<NS, 2>(t, part, &st->ticket, tot) && threadIdx.x == 0) {
-    st->red[0] = tot[0];
-    st->red[1] = tot[1];
-  }
+  tile_publish<NS, 2>(t, part);
 }
 
 // ---- FGMRES orthogonalisation ------------------------------------------------
 // res[i] = (w, V_i) for i < nv, V_i = V + i*ldv.  One sweep reads w once and
-// each V_i once; partials part[(tile*nv + i)*NS + s], summed canonically by
-// the last workgroup.
+// each V_i once; partials part[(tile*NS + s)*nv + i], finished by k_finalize.
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_multidot(int64_t seg, const double* __restrict__ w,
                                                      const double* __restrict__ V, int64_t ldv, int nv,
-                                                     double* part, uint32_t* ticket, double* res) {
+                                                     double* part) {
   double wl[NS][kSlots];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -198,27 +188,9 @@ __global__ __launch_bounds__(kBlock) void k_multidot(int64_t seg, const double* 
     tile_pairwise<NS>(t, p);
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) part[((size_t)blockIdx.x * nv + i) * NS + s] = p[s];
+      for (int s = 0; s < NS; ++s) part[((size_t)tile_id() * NS + s) * nv + i] = p[s];
     }
   }
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  for (int i = 0; i < nv; ++i) {
-    double tot = tiles_pairwise(part, gridDim.x, nv * NS, i * NS);
-#pragma unroll
-    for (int s = 1; s < NS; ++s) {
-      const double ts = tiles_pairwise(part, gridDim.x, nv * NS, i * NS + s);
-      tot = tot + ts;
-    }
-    if (threadIdx.x == 0) res[i] = tot;
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // w = w + sign * sum_i h[i] V_i ; if want_norm, res[0] = ||w||^2
@@ -226,7 +198,7 @@ template <int NS>
 __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __restrict__ w,
                                                       const double* __restrict__ V, int64_t ldv, int nv,
                                                       const double* __restrict__ h, double sign, double* part,
-                                                      uint32_t* ticket, double* res, int want_norm) {
+                                                      int want_norm) {
   double t[NS][kSlots];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -241,9 +213,30 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __res
       }
       t[s][j] = sv * sv;
     }
-  if (!want_norm) return;
-  double tot[1];
-  if (grid_pairwise<NS, 1>(t, part, ticket, tot) && threadIdx.x == 0) res[0] = tot[0];
+  if (want_norm) tile_publish<NS, 1>(t, part);
+}
+
+// Canonical total of the preceding kernel's tile partials; see kernels.hpp.
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part, uint32_t nt, int ns,
+                                                     int nv, double* out, KState* st, int phase, int guard) {
+  if (guard == FIN_ACTIVE && (st->done || st->early)) return;
+  if (guard == FIN_EARLY && (!st->early || st->done)) return;
+  const int K = ns * nv;
+  double tot[2] = {0.0, 0.0};
+  for (int v = 0; v < nv; ++v) {
+    double s = tiles_pairwise(part, nt, K, v);
+    for (int sg = 1; sg < ns; ++sg) {
+      const double x = tiles_pairwise(part, nt, K, sg * nv + v);
+      s = s + x;
+    }
+    if (threadIdx.x == 0) {
+      if (phase == PH_NONE)
+        out[v] = s;
+      else if (v < 2)
+        tot[v] = s;
+    }
+  }
+  if (threadIdx.x == 0 && phase != PH_NONE) kfinalize(st, phase, tot);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* a, double s, double* b) {
@@ -264,13 +257,21 @@ static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) 
       hipLaunchKernelGGL(kern<1>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);   \
   } while (0)
 
+void launch_finalize(hipStream_t s, const double* part, int64_t ntiles, int nseg, int nv, double* out, KState* ks,
+                     int phase, int guard) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out, ks, phase,
+                     guard);
+}
+
 void launch_multidot(hipStream_t s, int64_t n, int nseg, const double* w, const double* V, int64_t ldv, int nv,
-                     double* part, uint32_t* ticket, double* res) {
-  DRD_SEG_LAUNCH(k_multidot, n, nseg, w, V, ldv, nv, part, ticket, res);
+                     double* part, double* res) {
+  DRD_SEG_LAUNCH(k_multidot, n, nseg, w, V, ldv, nv, part);
+  launch_finalize(s, part, tiles_of(n / nseg), nseg, nv, res, nullptr, PH_NONE, FIN_ALWAYS);
 }
 void launch_multiaxpy(hipStream_t s, int64_t n, int nseg, double* w, const double* V, int64_t ldv, int nv,
-                      const double* h, double sign, double* part, uint32_t* ticket, double* res, int want_norm) {
-  DRD_SEG_LAUNCH(k_multiaxpy, n, nseg, w, V, ldv, nv, h, sign, part, ticket, res, want_norm);
+                      const double* h, double sign, double* part, double* res, int want_norm) {
+  DRD_SEG_LAUNCH(k_multiaxpy, n, nseg, w, V, ldv, nv, h, sign, part, want_norm);
+  if (want_norm) launch_finalize(s, part, tiles_of(n / nseg), nseg, 1, res, nullptr, PH_NONE, FIN_ALWAYS);
 }
 void launch_scale_into(hipStream_t s, int64_t n, const double* a, double sc, double* b) {
   hipLaunchKernelGGL(k_scale_into, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, sc, b);
@@ -279,19 +280,24 @@ void launch_scale_into(hipStream_t s, int64_t n, const double* a, double sc, dou
 void launch_init_norms(hipStream_t s, int64_t n, int nseg, const double* x, const double* b, double* part,
                        KState* ks) {
   DRD_SEG_LAUNCH(k_init_norms, n, nseg, x, b, part, ks);
+  launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, nullptr, ks, PH_INIT_NORMS, FIN_ALWAYS);
 }
 void launch_bicg_p(hipStream_t s, int64_t n, const double* r, const double* v, double* p, KState* ks) {
   hipLaunchKernelGGL(k_bicg_p, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, p, ks);
 }
 void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks) {
   DRD_SEG_LAUNCH(k_bicg_s, n, nseg, r, v, part, ks);
+  launch_finalize(s, part, tiles_of(n / nseg), nseg, 1, nullptr, ks, PH_S, FIN_ACTIVE);
 }
 void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, part, ks);
+  (void)part;
+  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, ks);
+  launch_finalize(s, nullptr, 0, 1, 0, nullptr, ks, PH_EARLY, FIN_EARLY);
 }
 void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, const double* rt, const double* ph,
-                    const double* sh, const double* t, double* part, KState* ks) {
+                    const double* sh, const double* t, double* part, KState* ks, bool finish) {
   DRD_SEG_LAUNCH(k_bicg_xr, n, nseg, x, r, rt, ph, sh, t, part, ks);
+  if (finish) launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, nullptr, ks, PH_XR, FIN_ACTIVE);
 }
 void launch_copy(hipStream_t s, int64_t n, const double* a, double* b) {
   hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
@@ -305,6 +311,7 @@ void launch_cell_axpy(hipStream_t s, int64_t n, const double* a, double* b) {
 void launch_resid_norms(hipStream_t s, int64_t n, int nseg, const double* b, const double* ax, double* part,
                         KState* ks) {
   DRD_SEG_LAUNCH(k_resid_norms, n, nseg, b, ax, part, ks);
+  launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, ks->red, nullptr, PH_NONE, FIN_ALWAYS);
 }
 
 }  // namespace drd

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_ph_rhs(Grid g, Scal s, const double*
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= g.ncell()) return;
   const int nx = g.nx;
-  const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
+  const int k = g.by_nx.div(q), i = (int)(q - (int64_t)k * nx);
   const int64_t c = g.ci(i, k);
   const double phv = ph[c];
   const double f1 = s.s * phv / s.I - chb_mf(phv, bdr[c]) / s.I - phv * (phv - 1.0) * bg[c] * cosa / sqrt2;
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_ph_mv1(Grid g, Scal s, const double*
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= g.ncell()) return;
   const int nx = g.nx;
-  const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
+  const int k = g.by_nx.div(q), i = (int)(q - (int64_t)k * nx);
   const double ff = dgn(g, s.dr2, s.dz2, m0, i, k, [&](int a, int b) { return x[a + (int64_t)nx * b]; });
   t[q] = -s.dt * s.xld * ff;
 }
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_ph_mv2(Grid g, Scal s, const double*
   for (int jj = 0; jj < kSlots; ++jj) {
     const int64_t q = tile_elem(n, 0, jj);
     if (q < 0) break;
-    const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
+    const int k = g.by_nx.div(q), i = (int)(q - (int64_t)k * nx);
     double z = 0.0;
     if (MODE != MV_INIT || xnz) {
       double ff2 = dgn(g, s.dr2, s.dz2, minner, i, k, [&](int a, int bb) { return t[a + (int64_t)nx * bb]; });
@@ -291,14 +291,12 @@ __global__ __launch_bounds__(kBlock) void k_ph_mv2(Grid g, Scal s, const double*
   }
   if (MODE == MV_PLAIN) return;
   if (MODE == MV_DOT2) {
-    double tot[2];
-    if (grid_pairwise<1, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    tile_publish<1, 2>(tm, part);
   } else {
-    double t1[1][kSlots], tot[1];
+    double t1[1][kSlots];
 #pragma unroll
     for (int jj = 0; jj < kSlots; ++jj) t1[0][jj] = tm[0][jj];
-    if (grid_pairwise<1, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
+    tile_publish<1, 1>(t1, part);
   }
 }
 
@@ -343,6 +341,7 @@ void launch_ph_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, co
     case MV_INIT: PM2(MV_INIT); break;
   }
 #undef PM2
+  launch_mv_finalize(st, mode, part, grid.x, 1, ks);
 }
 
 void launch_ph_finish(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* f2,

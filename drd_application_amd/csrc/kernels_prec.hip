@@ -45,9 +45,9 @@ __global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const dou
                                                        const KState* st) {
   if (st && (st->done || st->early)) return;
   const int64_t C = g.ncell();
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous rows
   if (q >= C) return;
-  const int j = (int)(q / g.nx), i = (int)(q - (int64_t)j * g.nx);
+  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
   const double bv = bu[g.ci(i, j)];
   const double d11 = s.s / s.I - s.I * dgn_diag(g, s.dr2, s.dz2, minner, i, j) +
                      ((j == 0) ? s.I / s.dz2 * (1.0 - s.coef_bot) : 0.0) +
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_scale(Grid g, Scal s, const dou
   if (q >= g.nuv()) return;
   double d, d0;
   if (q < g.mn()) {
-    const int k = (int)(q / (nx + 1)), i = (int)(q - (int64_t)k * (nx + 1));
+    const int k = g.by_nx1.div(q), i = (int)(q - (int64_t)k * (nx + 1));
     d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
         s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
     d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / s.dr2;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_scale(Grid g, Scal s, const dou
     d0 = a0 + s.dt * (4.0 / s.dr2 + 2.0 / s.dz2);
   } else {
     const int64_t t = q - g.mn();
-    const int k = (int)(t / nx) + 1, i = (int)(t - (int64_t)(k - 1) * nx);
+    const int k = g.by_nx.div(t) + 1, i = (int)(t - (int64_t)(k - 1) * nx);
     d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
         s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
     d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i, k - 1)]) / s.dz2;

@@ -19,12 +19,29 @@ struct VeloIn {
   int64_t mn;
   __device__ __forceinline__ double uraw(int i, int k) const { return x[i + (int64_t)(nx + 1) * k]; }
   // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils)
+  // (wall ghosts are odd mirrors of the first/last row, side ghosts even or
+  // odd mirrors of column 1 / nx-1; negation is exact, so the order of the
+  // two mirrorings does not matter)
   __device__ __forceinline__ double U(int i, int k) const {
-    if (k < 0) return -U(i, 0);
-    if (k >= ny) return -U(i, ny - 1);
-    if (i < 0) return lr_d ? -uraw(1, k) : uraw(1, k);
-    if (i > nx) return lr_d ? -uraw(nx - 1, k) : uraw(nx - 1, k);
-    return uraw(i, k);
+    bool neg = false;
+    if (k < 0) {
+      k = 0;
+      neg = true;
+    } else if (k >= ny) {
+      k = ny - 1;
+      neg = true;
+    }
+    double v;
+    if (i < 0) {
+      v = uraw(1, k);
+      if (lr_d) v = -v;
+    } else if (i > nx) {
+      v = uraw(nx - 1, k);
+      if (lr_d) v = -v;
+    } else {
+      v = uraw(i, k);
+    }
+    return neg ? -v : v;
   }
   // f2(i,k) for -1<=i<=nx, 0<=k<=ny
   __device__ __forceinline__ double V(int i, int k) const {
@@ -45,7 +62,7 @@ __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const do
   const int nx = g.nx;
   double r;
   if (q < X.mn) {
-    const int k = (int)(q / (nx + 1));
+    const int k = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)k * (nx + 1));
     const double f = X.U(i, k);
     if (BDF == 2)
@@ -62,7 +79,7 @@ __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const do
     r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
   } else {
     const int64_t t = q - X.mn;
-    const int k = (int)(t / nx) + 1;
+    const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(k - 1) * nx);
     const double f = X.V(i, k);
     if (BDF == 2)
@@ -130,14 +147,12 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
   }
   if (MODE == MV_PLAIN) return;
   if (MODE == MV_DOT2) {
-    double tot[2];
-    if (grid_pairwise<1, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    tile_publish<1, 2>(tm, part);
   } else {
-    double t1[1][kSlots], tot[1];
+    double t1[1][kSlots];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) t1[0][j] = tm[0][j];
-    if (grid_pairwise<1, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
+    tile_publish<1, 1>(t1, part);
   }
 }
 
@@ -151,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
   if (q >= g.nuv()) return;
   double tmp;
   if (q < g.mn()) {
-    const int j = (int)(q / (nx + 1));
+    const int j = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)j * (nx + 1));
     tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
     double cr = eta[g.ci(i, j)], cl = eta[g.ci(i - 1, j)];
@@ -166,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
       tmp = tmp + s.dt * (cl + cr) / s.dz2;
   } else {
     const int64_t t = q - g.mn();
-    const int j = (int)(t / nx) + 1;
+    const int j = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(j - 1) * nx);
     tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
     double cr = eta[g.ci(i, j)], cl = eta[g.ci(i, j - 1)];
@@ -199,12 +214,12 @@ __global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __re
   if (q >= g.nuv()) return;
   const int nx = g.nx;
   if (q < g.mn()) {
-    const int k = (int)(q / (nx + 1));
+    const int k = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)k * (nx + 1));
     x[q] = u[g.ui(i, k)];
   } else {
     const int64_t t = q - g.mn();
-    const int k = (int)(t / nx) + 1;
+    const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(k - 1) * nx);
     x[q] = v[g.vi(i, k)];
   }
@@ -272,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
   };
   double f;
   if (q < g.mn()) {
-    const int k = (int)(q / (nx + 1));
+    const int k = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)k * (nx + 1));
     const int64_t c0 = g.ci(i, k), c1 = g.ci(i - 1, k);
     f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dr;
@@ -302,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
     }
   } else {
     const int64_t t = q - g.mn();
-    const int k = (int)(t / nx) + 1;
+    const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(k - 1) * nx);
     const int64_t c0 = g.ci(i, k), c1 = g.ci(i, k - 1);
     f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dz;
@@ -369,7 +384,7 @@ __device__ __forceinline__ void phmu_rows(const Grid g, const Scal& s, const dou
                                           const double* __restrict__ x, int64_t q, double& o1, double& o2) {
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
-  const int j = (int)(q / nx), i = (int)(q - (int64_t)j * nx);
+  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * nx);
   auto F1 = [&](int a, int b) -> double { return x[a + (int64_t)nx * b]; };
   auto F2 = [&](int a, int b) -> double { return x[C + a + (int64_t)nx * b]; };
   const double f1 = F1(i, j), f2 = F2(i, j);
@@ -435,17 +450,15 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
   }
   if (MODE == MV_PLAIN) return;
   if (MODE == MV_DOT2) {
-    double tot[2];
-    if (grid_pairwise<2, 2>(tm, part, &st->ticket, tot) && threadIdx.x == 0) kfinalize(st, PH_MV2, tot);
+    tile_publish<2, 2>(tm, part);
   } else {
-    double t1[2][kSlots], tot[1];
+    double t1[2][kSlots];
 #pragma unroll
     for (int jj = 0; jj < kSlots; ++jj) {
       t1[0][jj] = tm[0][jj];
       t1[1][jj] = tm[2][jj];
     }
-    if (grid_pairwise<2, 1>(t1, part, &st->ticket, tot) && threadIdx.x == 0)
-      kfinalize(st, MODE == MV_INIT ? PH_INIT_R : PH_MV1, tot);
+    tile_publish<2, 1>(t1, part);
   }
 }
 
@@ -458,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_jacobi(Grid g, Scal s, const do
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= C) return;
   const int nx = g.nx, ny = g.ny;
-  const int j = (int)(q / nx), i = (int)(q - (int64_t)j * nx);
+  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * nx);
   const double tmpx = (i == 0 || j == nx - 1) ? 1.0 / s.dr2 : 2.0 / s.dr2;
   const double tmpz = (i == 0 || j == ny - 1) ? 1.0 / s.dz2 : 2.0 / s.dz2;
   x[q] = b[q] / ((tmpx + tmpz) * s.I + s.s / s.I);
@@ -498,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rhs(Grid g, Scal s, PhmuRhsArgs
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= C) return;
   const int nx = g.nx, ny = g.ny;
-  const int k = (int)(q / nx), i = (int)(q - (int64_t)k * nx);
+  const int k = g.by_nx.div(q), i = (int)(q - (int64_t)k * nx);
   const double *ph = a.ph, *bu = a.bu, *u = a.u, *v = a.v;
   const int64_t c = g.ci(i, k);
   double f2 = ph[c] * (bu[c] + s.adj);
@@ -562,7 +575,7 @@ static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1
 
 void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks) {
+                        const double* b, double* part, KState* ks, bool finish) {
   dim3 grid(nblk(g.nuv(), kTile));
 #define VM(B, M)                                                                                     \
   hipLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, y, \
@@ -583,11 +596,12 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
     }
   }
 #undef VM
+  if (finish) launch_mv_finalize(st, mode, part, grid.x, 1, ks);
 }
 
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks) {
+                        const double* b, double* part, KState* ks, bool finish) {
   dim3 grid(nblk(g.ncell(), kTile));
 #define PM(M)                                                                                          \
   hipLaunchKernelGGL((k_phmu_matvec<M>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, x, y, alpha, \
@@ -599,6 +613,7 @@ void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, 
     case MV_INIT: PM(MV_INIT); break;
   }
 #undef PM
+  if (finish) launch_mv_finalize(st, mode, part, grid.x, 2, ks);
 }
 
 void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag) {

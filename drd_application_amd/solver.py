@@ -63,6 +63,9 @@ def lib():
         L.drd_bicgstab.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
         L.drd_fgmres.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
         L.drd_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
+        L.drd_bench_kernel.argtypes = [V, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.drd_kernel_timer.argtypes = [V, C.c_int32, C.c_int32]
+        L.drd_kernel_timer_read.argtypes = [V, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -201,3 +204,22 @@ class GpuSolver:
         _check(lib().drd_step(self._h, bdf, t, C.byref(info)))
         return {"phmu": info.phmu.as_dict(), "velo": info.velo.as_dict(),
                 "div_before": info.div_before}
+
+    # ---- measurement -------------------------------------------------------
+    def bench_kernel(self, which, reps):
+        """(mean ms per launch, algorithmic bytes per launch) of kernel
+        `which` (abi.K_*) launched `reps` times on the context's stream."""
+        ms, by = C.c_double(0), C.c_double(0)
+        _check(lib().drd_bench_kernel(self._h, which, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def kernel_timer(self, which, capacity):
+        """Brackets up to `capacity` launches of kernel `which` inside the
+        following solves with HIP events (0 disables)."""
+        _check(lib().drd_kernel_timer(self._h, which, capacity))
+
+    def kernel_timer_read(self):
+        """(timed launches, summed ms, algorithmic bytes per launch)."""
+        n, ms, by = C.c_int32(0), C.c_double(0), C.c_double(0)
+        _check(lib().drd_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value

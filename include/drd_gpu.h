@@ -239,8 +239,18 @@ int drd_synchronize(drd_ctx* ctx);
 #define DRD_K_PHMU_MATVEC 1 /* fused CH block matvec + dot                 */
 #define DRD_K_BICG_XR 2     /* x,r update + two dots                       */
 #define DRD_K_PRESSURE 3    /* whole pressure projection                   */
+#define DRD_K_COPY 4        /* plain copy of Nuv doubles (counter scale)    */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
+/* Times kernel `which` inside the caller's own solves: while enabled, each
+ * launch of it (up to `capacity` launches; 0 disables and resets) is
+ * bracketed by a pair of HIP events on the context's stream.  Only
+ * DRD_K_VELO_MATVEC (the BDF2 operator's fused matvec with one dot, as
+ * BiCGSTAB launches it) is instrumented.  drd_kernel_timer_read waits for
+ * the stream and returns the number of timed launches, their summed
+ * duration and the kernel's algorithmic bytes per launch.                 */
+int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);
+int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
 
 #ifdef __cplusplus
 }
