@@ -122,6 +122,19 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
     return out, None
 
 
+def measured_traffic(cfg_name: str):
+    """HBM bytes per launch of the velocity matvec from the committed
+    rocprofv3 counter summary for this workload, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"k_velo_matvec_{cfg_name}_counters.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     """The velocity matvec and a plain copy of the same length, each launched
     `reps` times after one BDF1 step has filled the fields; under
@@ -221,7 +234,8 @@ def main(argv=None) -> int:
                                          if "dropin" in modes and args.mode != "dropin" else None),
                    "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                     "traffic": measured_traffic(cfg.name),
                      "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
                      "us_per_launch": ms_k * 1e3 if ms_k else None,
                      "launches_timed": tl["count"]},
