@@ -20,7 +20,7 @@
 #include <cmath>
 #include <vector>
 
-#include "fft_lines.hpp"
+#include "fft_regs.hpp"
 #include "kernels.hpp"
 
 namespace drd {
@@ -118,67 +118,202 @@ struct PrecTables {
   const T* lam_v;      // nx x (ny-1): lam_v[kx*(ny-1) + ky]
 };
 
-// pass 1 (fwd) / 3 (inv) over rows: u rows (blocks [0,ny)), v rows ([ny, 2ny-1)).
+// ---- paired real line transforms -------------------------------------------
+// Two real lines share one complex FFT (line 0 in the real part, line 1 in
+// the imaginary part), FFTW conventions, unnormalised.  load(i) returns the
+// pair's i-th inputs; store(i, y0, y1) takes the outputs.  A is the LDS
+// line (FFT length complex values); every helper ends with a barrier.
+__device__ __forceinline__ int makhoul_ix(int j, int n) { return (j & 1) ? (n - 1 - (j >> 1)) : (j >> 1); }
+
+// REDFT00 of n = N+1 points: FFT of the even extension (length 2N); both
+// spectra are real, so line 0 is Re V and line 1 is Im V.
+template <class T, int LMAX, class Load, class Store>
+__device__ void dct1_pair(Cx<T>* A, int N, const Cx<T>* __restrict__ tw, Load load, Store store) {
+  const int L = 2 * N;
+  for (int m = threadIdx.x; m < L; m += blockDim.x) A[m] = load(m <= N ? m : L - m);
+  __syncthreads();
+  fft_lds<T, LMAX>(A, L, tw, T(-1));
+  for (int k = threadIdx.x; k <= N; k += blockDim.x) store(k, A[k].x, A[k].y);
+  __syncthreads();
+}
+
+// REDFT10 (dst = false) or RODFT10 (dst = true: inputs (-1)^j x_j, outputs
+// reversed) by Makhoul's reordering; the two spectra are separated by
+// V0_k = (V_k + conj V_{-k}) / 2, V1_k = (V_k - conj V_{-k}) / 2i, and
+// y_k = 2 Re(exp(-i pi k / 2n) V_k).
+template <class T, int LMAX, class Load, class Store>
+__device__ void dct2_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const Cx<T>* __restrict__ rot, bool dst,
+                          Load load, Store store) {
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    Cx<T> v = load(j);
+    if (dst && (j & 1)) v = Cx<T>{-v.x, -v.y};
+    A[makhoul_ix(j, n)] = v;
+  }
+  __syncthreads();
+  fft_lds<T, LMAX>(A, n, tw, T(-1));
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const Cx<T> V = A[k], W = A[(n - k) & (n - 1)];
+    const Cx<T> sp{V.x + W.x, V.y - W.y}, sm{V.x - W.x, V.y + W.y};  // V +- conj W
+    const Cx<T> r = rot[k];
+    store(dst ? n - 1 - k : k, r.x * sp.x - r.y * sp.y, r.x * sm.y + r.y * sm.x);
+  }
+  __syncthreads();
+}
+
+// REDFT01 (dst = false) or RODFT01 (dst = true) of the pair held where
+// load(i) reads it (random access): inverse Makhoul; the two inverse FFT
+// outputs are real, so they separate as Re and Im.
+template <class T, int LMAX, class Load, class Store>
+__device__ void dct3_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const Cx<T>* __restrict__ rot, bool dst,
+                          Load load, Store store) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const Cx<T> a = load(dst ? n - 1 - k : k);
+    Cx<T> b{T(0), T(0)};
+    if (k > 0) {
+      const Cx<T> t = load(dst ? k - 1 : n - k);
+      b = Cx<T>{-t.x, -t.y};
+    }
+    const Cx<T> r = rot[k];
+    const Cx<T> a0{r.x * a.x + r.y * b.x, -r.y * a.x + r.x * b.x};  // line 0
+    const Cx<T> a1{r.x * a.y + r.y * b.y, -r.y * a.y + r.x * b.y};  // line 1
+    A[k] = Cx<T>{a0.x - a1.y, a0.y + a1.x};
+  }
+  __syncthreads();
+  fft_lds<T, LMAX>(A, n, tw, T(1));
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    Cx<T> v = A[makhoul_ix(j, n)];
+    if (dst && (j & 1)) v = Cx<T>{-v.x, -v.y};
+    store(j, v.x, v.y);
+  }
+  __syncthreads();
+}
+
+// RODFT00 of n points: FFT of the odd extension (length 2(n+1)); both
+// spectra are imaginary: y0_k = -Im V_{k+1}, y1_k = Re V_{k+1}.
+template <class T, int LMAX, class Load, class Store>
+__device__ void dst1_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, Load load, Store store) {
+  const int L = 2 * (n + 1);
+  for (int m = threadIdx.x; m < L; m += blockDim.x) {
+    Cx<T> v{T(0), T(0)};
+    if (m >= 1 && m <= n) v = load(m - 1);
+    if (m >= n + 2) {
+      const Cx<T> t = load(L - m - 1);
+      v = Cx<T>{-t.x, -t.y};
+    }
+    A[m] = v;
+  }
+  __syncthreads();
+  fft_lds<T, LMAX>(A, L, tw, T(-1));
+  for (int k = threadIdx.x; k < n; k += blockDim.x) store(k, -A[k + 1].y, A[k + 1].x);
+  __syncthreads();
+}
+
+// pass 1 (fwd) / 3 (inv) over rows, two rows per workgroup: u rows (pairs
+// [0, nu)), v rows (pairs [nu, nu + nv)).
 // fwd: w = DCT_x(inv_s * r) in T;  inv: z = inv_s * DCT_x^-1(w) in FP64
-template <class T, bool FWD>
+template <class T, bool FWD, int LMAX>
 __global__ __launch_bounds__(kBlock) void k_prec_rows(Grid g, PrecTables<T> tb, const double* __restrict__ r,
                                                       const T* __restrict__ inv_s, T* __restrict__ wu,
                                                       T* __restrict__ wv, double* __restrict__ z,
                                                       const KState* st) {
   if (st && (st->done || st->early)) return;
   extern __shared__ unsigned char smem_raw[];
-  const int nx = g.nx, ny = g.ny;
   Cx<T>* A = reinterpret_cast<Cx<T>*>(smem_raw);
-  Cx<T>* B = A + 2 * nx;
-  T* line = reinterpret_cast<T*>(B + 2 * nx);
-  const bool is_u = blockIdx.x < (unsigned)ny;
-  const int n = is_u ? nx + 1 : nx;
-  const int64_t base = is_u ? (int64_t)blockIdx.x * (nx + 1) : g.mn() + (int64_t)(blockIdx.x - ny) * nx;
-  T* w = is_u ? wu + (int64_t)blockIdx.x * (nx + 1) : wv + (int64_t)(blockIdx.x - ny) * nx;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) line[i] = FWD ? T(r[base + i]) * inv_s[base + i] : w[i];
-  __syncthreads();
-  if (is_u)
-    line_transform<T>(TR_DCT1, line, n, A, B, tb.tw2nx, nullptr);
-  else
-    line_transform<T>(FWD ? TR_DCT2 : TR_DCT3, line, n, A, B, tb.twnx, tb.rotnx);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    if (FWD)
-      w[i] = line[i];
-    else
-      z[base + i] = (double)(line[i] * inv_s[base + i]);
+  const int nx = g.nx, ny = g.ny;
+  const int nu = (ny + 1) / 2;
+  if ((int)blockIdx.x < nu) {  // u rows 2b, 2b+1: n = nx+1 points, DCT-I both ways
+    const int r0 = 2 * blockIdx.x, n = nx + 1;
+    const bool has1 = r0 + 1 < ny;
+    const int64_t b0 = (int64_t)r0 * n, b1 = b0 + n;
+    auto load = [&](int i) -> Cx<T> {
+      if (FWD) return Cx<T>{T(r[b0 + i]) * inv_s[b0 + i], has1 ? T(r[b1 + i]) * inv_s[b1 + i] : T(0)};
+      return Cx<T>{wu[b0 + i], has1 ? wu[b1 + i] : T(0)};
+    };
+    auto store = [&](int k, T y0, T y1) {
+      if (FWD) {
+        wu[b0 + k] = y0;
+        if (has1) wu[b1 + k] = y1;
+      } else {
+        z[b0 + k] = (double)(y0 * inv_s[b0 + k]);
+        if (has1) z[b1 + k] = (double)(y1 * inv_s[b1 + k]);
+      }
+    };
+    dct1_pair<T, LMAX>(A, nx, tb.tw2nx, load, store);
+  } else {  // v rows 2b, 2b+1 of the ny-1: n = nx points, DCT-II forward, DCT-III back
+    const int r0 = 2 * (blockIdx.x - nu), n = nx;
+    const bool has1 = r0 + 1 < ny - 1;
+    const int64_t w0 = (int64_t)r0 * n, w1 = w0 + n;           // in wv
+    const int64_t b0 = g.mn() + w0, b1 = g.mn() + w1;           // in r, z, inv_s
+    if (FWD) {
+      auto load = [&](int i) -> Cx<T> {
+        return Cx<T>{T(r[b0 + i]) * inv_s[b0 + i], has1 ? T(r[b1 + i]) * inv_s[b1 + i] : T(0)};
+      };
+      auto store = [&](int k, T y0, T y1) {
+        wv[w0 + k] = y0;
+        if (has1) wv[w1 + k] = y1;
+      };
+      dct2_pair<T, LMAX>(A, n, tb.twnx, tb.rotnx, false, load, store);
+    } else {
+      auto load = [&](int i) -> Cx<T> { return Cx<T>{wv[w0 + i], has1 ? wv[w1 + i] : T(0)}; };
+      auto store = [&](int j, T x0, T x1) {
+        z[b0 + j] = (double)(x0 * inv_s[b0 + j]);
+        if (has1) z[b1 + j] = (double)(x1 * inv_s[b1 + j]);
+      };
+      dct3_pair<T, LMAX>(A, n, tb.twnx, tb.rotnx, false, load, store);
+    }
   }
 }
 
-// pass 2 over columns of the transposed blocks: u columns (blocks [0,nx+1)),
-// v columns ([nx+1, 2nx+1)); forward transform, divide by the eigenvalue, inverse.
-template <class T>
+// pass 2 over columns of the transposed blocks, two columns per workgroup:
+// u columns (pairs [0, nu)), v columns ([nu, nu + nv)); forward transform,
+// divide by the eigenvalue, inverse.  The divided spectra wait in LDS
+// behind the FFT line.
+template <class T, int LMAX>
 __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, T* __restrict__ tu,
                                                       T* __restrict__ tv, const KState* st) {
   if (st && (st->done || st->early)) return;
   extern __shared__ unsigned char smem_raw[];
   const int nx = g.nx, ny = g.ny;
   Cx<T>* A = reinterpret_cast<Cx<T>*>(smem_raw);
-  Cx<T>* B = A + 2 * ny;
-  T* line = reinterpret_cast<T*>(B + 2 * ny);
-  const bool is_u = blockIdx.x < (unsigned)(nx + 1);
-  const int kx = is_u ? blockIdx.x : blockIdx.x - (nx + 1);
-  const int n = is_u ? ny : ny - 1;
-  T* col = is_u ? tu + (int64_t)kx * ny : tv + (int64_t)kx * (ny - 1);
-  const T* lam = is_u ? tb.lam_u + (int64_t)kx * ny : tb.lam_v + (int64_t)kx * (ny - 1);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) line[i] = col[i];
-  __syncthreads();
-  if (is_u) {
-    line_transform<T>(TR_DST2, line, n, A, B, tb.twny, tb.rotny);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) line[i] = line[i] / lam[i];
-    __syncthreads();
-    line_transform<T>(TR_DST3, line, n, A, B, tb.twny, tb.rotny);
-  } else {
-    line_transform<T>(TR_DST1, line, n, A, B, tb.tw2ny, nullptr);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) line[i] = line[i] / lam[i];
-    __syncthreads();
-    line_transform<T>(TR_DST1, line, n, A, B, tb.tw2ny, nullptr);
+  T* l0 = reinterpret_cast<T*>(A + 2 * ny);
+  T* l1 = l0 + ny;
+  const int nu = (nx + 2) / 2;
+  if ((int)blockIdx.x < nu) {  // u columns: DST-II, / lambda, DST-III (n = ny)
+    const int c0 = 2 * blockIdx.x, n = ny;
+    const bool has1 = c0 + 1 < nx + 1;
+    T* col0 = tu + (int64_t)c0 * n;
+    T* col1 = col0 + n;
+    const T* lam0 = tb.lam_u + (int64_t)c0 * n;
+    const T* lam1 = lam0 + n;
+    dct2_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true,
+                 [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
+                 [&](int k, T y0, T y1) {
+                   l0[k] = y0 / lam0[k];
+                   l1[k] = has1 ? y1 / lam1[k] : T(0);
+                 });
+    dct3_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
+                 [&](int j, T x0, T x1) {
+                   col0[j] = x0;
+                   if (has1) col1[j] = x1;
+                 });
+  } else {  // v columns: DST-I, / lambda, DST-I (n = ny-1)
+    const int c0 = 2 * (blockIdx.x - nu), n = ny - 1;
+    const bool has1 = c0 + 1 < nx;
+    T* col0 = tv + (int64_t)c0 * n;
+    T* col1 = col0 + n;
+    const T* lam0 = tb.lam_v + (int64_t)c0 * n;
+    const T* lam1 = lam0 + n;
+    dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
+                 [&](int k, T y0, T y1) {
+                   l0[k] = y0 / lam0[k];
+                   l1[k] = has1 ? y1 / lam1[k] : T(0);
+                 });
+    dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
+                 [&](int k, T y0, T y1) {
+                   col0[k] = y0;
+                   if (has1) col1[k] = y1;
+                 });
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) col[i] = line[i];
 }
 
 template <class T>
@@ -214,8 +349,8 @@ struct VeloPrec {
 };
 
 template <class T>
-static void host_tw(std::vector<Cx<T>>& v, int L) {
-  for (int t = 0; t < L / 2; ++t) {
+static void host_tw(std::vector<Cx<T>>& v, int L) {  // exp(-2 pi i t / L), t < L
+  for (int t = 0; t < L; ++t) {
     const double a = -2.0 * M_PI * (double)t / (double)L;
     v.push_back(Cx<T>{(T)std::cos(a), (T)std::sin(a)});
   }
@@ -301,17 +436,6 @@ VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err) {
     *err = rc;
     return nullptr;
   }
-  static bool raised = false;
-  if (!raised) {  // rows of 2nx complex (+ the line) may exceed 64 KiB
-    const int lim = 160 * 1024;
-    (void)hipFuncSetAttribute((const void*)k_prec_rows<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_rows<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_rows<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_rows<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_cols<float>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_cols<double>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    raised = true;
-  }
   return pc;
 }
 
@@ -329,6 +453,36 @@ void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s,
                      c.rho, c.eta, c.chi, pc->a0, pc->inv_s64, pc->inv_s32);
 }
 
+template <class T, int LMAX>
+static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T>& tb, const T* inv_s,
+                               const double* r, double* z, T* wu, T* wv, T* tu, T* tv, unsigned rows_blocks,
+                               unsigned cols_blocks, size_t lds_rows, size_t lds_cols, const KState* ks) {
+  const int nx = g.nx, ny = g.ny;
+  static bool raised = false;  // lines of 2nx or 2ny complex may exceed 64 KiB
+  if (!raised) {
+    const int lim = 160 * 1024;
+    (void)hipFuncSetAttribute((const void*)k_prec_rows<T, true, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_prec_rows<T, false, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
+    (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    raised = true;
+  }
+  hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks), dim3(kBlock), lds_rows, st, g, tb, r, inv_s,
+                     wu, wv, (double*)nullptr, ks);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 1 + 31) / 32, (ny + 31) / 32), dim3(kBlock), 0, st, ny, nx + 1,
+                     (const T*)wu, tu, ks);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 31) / 32, (ny - 1 + 31) / 32), dim3(kBlock), 0, st, ny - 1, nx,
+                     (const T*)wv, tv, ks);
+  hipLaunchKernelGGL((k_prec_cols<T, LMAX>), dim3(cols_blocks), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny + 31) / 32, (nx + 1 + 31) / 32), dim3(kBlock), 0, st, nx + 1, ny,
+                     (const T*)tu, wu, ks);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny - 1 + 31) / 32, (nx + 31) / 32), dim3(kBlock), 0, st, nx, ny - 1,
+                     (const T*)tv, wv, ks);
+  hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks), dim3(kBlock), lds_rows, st, g, tb,
+                     (const double*)nullptr, inv_s, wu, wv, z, ks);
+}
+
 template <class T>
 static void apply_T(hipStream_t st, VeloPrec* pc, const Grid& g, const PrecTables<T>& tb, const T* inv_s,
                     const double* r, double* z, const KState* ks) {
@@ -337,21 +491,19 @@ static void apply_T(hipStream_t st, VeloPrec* pc, const Grid& g, const PrecTable
   T* wv = wu + (size_t)(nx + 1) * ny;
   T* tu = wv + (size_t)nx * (ny - 1);
   T* tv = tu + (size_t)(nx + 1) * ny;
-  const size_t lds_rows = 4 * (size_t)nx * sizeof(Cx<T>) + (size_t)(nx + 1) * sizeof(T);
-  const size_t lds_cols = 4 * (size_t)ny * sizeof(Cx<T>) + (size_t)(ny + 1) * sizeof(T);
-  hipLaunchKernelGGL((k_prec_rows<T, true>), dim3(2 * ny - 1), dim3(kBlock), lds_rows, st, g, tb, r, inv_s, wu, wv,
-                     (double*)nullptr, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 1 + 31) / 32, (ny + 31) / 32), dim3(kBlock), 0, st, ny, nx + 1,
-                     (const T*)wu, tu, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 31) / 32, (ny - 1 + 31) / 32), dim3(kBlock), 0, st, ny - 1, nx,
-                     (const T*)wv, tv, ks);
-  hipLaunchKernelGGL(k_prec_cols<T>, dim3(2 * nx + 1), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny + 31) / 32, (nx + 1 + 31) / 32), dim3(kBlock), 0, st, nx + 1, ny,
-                     (const T*)tu, wu, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny - 1 + 31) / 32, (nx + 31) / 32), dim3(kBlock), 0, st, nx, ny - 1,
-                     (const T*)tv, wv, ks);
-  hipLaunchKernelGGL((k_prec_rows<T, false>), dim3(2 * ny - 1), dim3(kBlock), lds_rows, st, g, tb,
-                     (const double*)nullptr, inv_s, wu, wv, z, ks);
+  // two lines per workgroup; LDS holds the longest FFT line (2nx for the
+  // u rows' DCT-I, 2ny for the v columns' DST-I) and, for columns, the
+  // divided spectra of the pair
+  const unsigned rows_blocks = (unsigned)((ny + 1) / 2 + ny / 2);
+  const unsigned cols_blocks = (unsigned)((nx + 2) / 2 + (nx + 1) / 2);
+  const size_t lds_rows = 2 * (size_t)nx * sizeof(Cx<T>);
+  const size_t lds_cols = 2 * (size_t)ny * sizeof(Cx<T>) + 2 * (size_t)ny * sizeof(T);
+  if (std::max(nx, ny) <= 2048)
+    launch_prec_passes<T, 4096>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
+                                lds_cols, ks);
+  else
+    launch_prec_passes<T, 8192>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
+                                lds_cols, ks);
 }
 
 void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks) {
