@@ -351,6 +351,23 @@ static void ctx_free(drd_ctx* c) {
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
+// The fast-Poisson velocity preconditioner is built with the context (or
+// when drd_set_params first asks for it), so no solve pays its set-up.
+static int eager_vprec(drd_ctx* c) {
+  const int vp = c->p.velo_prec;
+  if (vp != DRD_PREC_DCT32 && vp != DRD_PREC_DCT64) return 0;
+  const bool fp32 = vp == DRD_PREC_DCT32;
+  if (c->vprec && c->vprec_fp32 == fp32) return 0;
+  velo_prec_destroy(c->vprec);
+  c->vprec = nullptr;
+  int err = 0;
+  c->vprec = velo_prec_create(c->g, c->s, fp32, &err);
+  if (!c->vprec)
+    return set_err(err ? err : DRD_E_HIP, "DCT velocity preconditioner needs power-of-two nx, ny <= 4096");
+  c->vprec_fp32 = fp32;
+  return 0;
+}
+
 extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
   if (!prm || !out) return set_err(DRD_E_ARG, "null argument");
   if (prm->nx < 4 || prm->ny < 4) return set_err(DRD_E_ARG, "nx, ny must be >= 4");
@@ -362,10 +379,10 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(DRD_E_HIP, "no HIP device visible");
-  drd_ctx* c = new drd_ctx();
-  c->p = *prm;
   if ((int64_t)(prm->nx + 3) * (prm->ny + 2) >= ((int64_t)1 << 31))
     return set_err(DRD_E_UNSUPPORTED, "grid too large for 32-bit indexing");
+  drd_ctx* c = new drd_ctx();
+  c->p = *prm;
   c->g = Grid::make(prm->nx, prm->ny);
   c->s = make_scal(*prm);
   int rc = 0;
@@ -373,6 +390,7 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
     rc = set_err(DRD_E_HIP, "hipStreamCreate failed");
   if (!rc) rc = ctx_alloc(c);
   if (!rc) rc = build_tables(c);
+  if (!rc) rc = eager_vprec(c);
   if (rc) {
     ctx_free(c);
     delete c;
@@ -396,9 +414,10 @@ extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   hipFree(c->pre_lcc);
   hipFree(c->ph_lcc);
   c->pre_lcc = c->ph_lcc = nullptr;
-  velo_prec_destroy(c->vprec);  // its eigenvalues depend on dt, re, dx, dz
-  c->vprec = nullptr;
+  if (c->vprec && velo_prec_update(c->vprec, c->s))  // its eigenvalues depend on dt, re, dx, dz
+    return set_err(DRD_E_HIP, "preconditioner tables could not be rebuilt");
   RCHK(build_tables(c));
+  RCHK(eager_vprec(c));
   return 0;
 }
 
@@ -616,6 +635,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
   RCHK(prec_setup(c, prec));
   const int ns = op_nseg(op);
+  const bool fused_block = (op == DRD_OP_PHMU && prec == DRD_PS_PHMU_BLOCK);
   launch_init_norms(c->stream, n, ns, x, b, c->part, c->ks);
   RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
   int chunk = 1, total = 0;
@@ -623,12 +643,23 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
     for (int it = 0; it < chunk; ++it) {
       const double* ph = nullptr;
       const double* sh = nullptr;
-      launch_bicg_p(c->stream, n, R, V, P, c->ks);
-      RCHK(apply_prec(c, prec, op, P, PHb, &ph, c->ks));
+      if (fused_block) {  // p, phat = B^-1 p in one sweep
+        launch_bicg_p_block(c->stream, c->g, c->s, coef_of(c), R, V, P, PHb, c->ks);
+        ph = PHb;
+      } else {
+        launch_bicg_p(c->stream, n, R, V, P, c->ks);
+        RCHK(apply_prec(c, prec, op, P, PHb, &ph, c->ks));
+      }
       RCHK(apply_op(c, op, MV_DOT1, ph, V, 1.0, 0.0, RT, nullptr, c->ks));
-      launch_bicg_s(c->stream, n, ns, R, V, c->part, c->ks);
-      launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
-      RCHK(apply_prec(c, prec, op, R, SHb, &sh, c->ks));
+      if (fused_block) {  // s, ||s||^2, shat = B^-1 s in one sweep
+        launch_bicg_s_block(c->stream, c->g, c->s, coef_of(c), R, V, SHb, c->part, c->ks);
+        sh = SHb;
+        launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
+      } else {
+        launch_bicg_s(c->stream, n, ns, R, V, c->part, c->ks);
+        launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
+        RCHK(apply_prec(c, prec, op, R, SHb, &sh, c->ks));
+      }
       RCHK(apply_op(c, op, MV_DOT2, sh, T, 1.0, 0.0, R, nullptr, c->ks));
       launch_bicg_xr(c->stream, n, ns, x, R, RT, ph, sh, T, c->part, c->ks);
     }

@@ -72,6 +72,8 @@ struct KState {
   double breaktol;
   double red[8];      // finished reductions of the last phase
   int32_t iter, maxit, info, done, early, xnz;
+  uint32_t eticket;  // workgroups of the early-exit update that have finished
+  int32_t pad;
 };
 
 // ---- canonical reductions ---------------------------------------------------

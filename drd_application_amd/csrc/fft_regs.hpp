@@ -6,7 +6,8 @@
 // of 4, then radix-16 passes.  In a pass each group j of R inputs
 //   v[t] = A[j + t L/R],  t < R,
 // is multiplied by the twiddles w^(t k), w = exp(+-2 pi i / (Ns R)),
-// k = j mod Ns (Ns = product of the earlier radices), goes through an
+// k = j mod Ns (Ns = product of the earlier radices; w^k is read from the
+// table, its powers formed by multiplication), goes through an
 // R-point DFT in registers and is stored at A[(j - k) R + k + t Ns].  A
 // thread holds all its groups of a pass (one for lines up to 4096), so the
 // whole pass is read before the barrier and written after it: one LDS
@@ -98,13 +99,14 @@ __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<
       const int k = j & (Ns - 1);
 #pragma unroll
       for (int t = 0; t < R; ++t) v[g][t] = A[j + t * groups];
-      if (Ns > 1) {
-        const int step = k * (L / (Ns * R));
+      if (Ns > 1) {  // w^t by repeated multiplication from one table entry
+        Cx<T> w1 = tw[k * (L / (Ns * R))];
+        if (sign > T(0)) w1.y = -w1.y;
+        Cx<T> w = w1;
 #pragma unroll
         for (int t = 1; t < R; ++t) {
-          Cx<T> w = tw[t * step];
-          if (sign > T(0)) w.y = -w.y;
           v[g][t] = cmulT(v[g][t], w);
+          if (t + 1 < R) w = cmulT(w, w1);
         }
       }
       dft_regs<T, R>(v[g], sign);

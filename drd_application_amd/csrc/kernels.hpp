@@ -9,7 +9,7 @@ namespace drd {
 enum { MV_PLAIN = 0, MV_DOT1 = 1, MV_DOT2 = 2, MV_INIT = 3 };
 
 // phases whose finished reductions drive the state machine
-enum { PH_INIT_NORMS = 0, PH_INIT_R, PH_MV1, PH_S, PH_MV2, PH_XR, PH_EARLY, PH_NONE };
+enum { PH_INIT_NORMS = 0, PH_INIT_R, PH_MV1, PH_S, PH_MV2, PH_XR, PH_NONE };
 
 struct Coef {
   const double *rho, *eta, *chi, *bdr, *bu, *bg, *minner, *m0;
@@ -91,9 +91,6 @@ __device__ inline void kfinalize(KState* st, int phase, const double* t) {
       }
       break;
     }
-    case PH_EARLY:
-      st->done = 1;
-      break;
     default:
       break;
   }
@@ -106,7 +103,7 @@ __device__ inline void kfinalize(KState* st, int phase, const double* t) {
 // either applies the BiCGSTAB decisions of `phase` (kfinalize) or, for
 // PH_NONE, writes the nv totals to out.  `guard` repeats the producer's
 // own skip test so a finished solve stays untouched.
-enum { FIN_ALWAYS = 0, FIN_ACTIVE = 1, FIN_EARLY = 2 };
+enum { FIN_ALWAYS = 0, FIN_ACTIVE = 1 };
 void launch_finalize(hipStream_t st, const double* part, int64_t ntiles, int nseg, int nv, double* out,
                      KState* ks, int phase, int guard);
 inline unsigned tiles_of(int64_t n) { return (unsigned)((n + kTile - 1) / kTile); }
@@ -171,8 +168,15 @@ void launch_scale_into(hipStream_t st, int64_t n, const double* a, double s, dou
 // ---- fast-path preconditioners (kernels_prec.hip) ----
 void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
                        const KState* ks);
+// BiCGSTAB p and s updates of the phi/mu solve with the block preconditioner
+// fused in: p and phat = B^-1 p; s, ||s||^2 (then k_finalize) and shat = B^-1 s
+void launch_bicg_p_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* r,
+                         const double* v, double* p, double* ph, KState* ks);
+void launch_bicg_s_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* r, const double* v,
+                         double* sh, double* part, KState* ks);
 struct VeloPrec;
 VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err);
+int velo_prec_update(VeloPrec* pc, const Scal& s);  // new dt, re, dx, dz
 void velo_prec_destroy(VeloPrec* pc);
 // diagonal scaling from the current coefficients (once per solve)
 void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s, const Coef& c);

@@ -82,14 +82,23 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
 }
 
 // early exit: x = x + alpha*phat                   REVCOM:314
+// The last workgroup to finish (relaxed ticket) marks the solve done; every
+// workgroup has read `done` before it takes its ticket, and nothing in this
+// kernel reads x or the flag afterwards, so no fence is needed.  Only the
+// one early-exit iteration of a solve gets past the first test.
 __global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __restrict__ x,
-                                                       const double* __restrict__ ph, const KState* st) {
+                                                       const double* __restrict__ ph, KState* st) {
   if (!st->early || st->done) return;
   const double alpha = st->alpha;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
     const int64_t q = tile_elem(n, 0, j);
     if (q >= 0) x[q] = x[q] + alpha * ph[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&st->eticket, 1u) == gridDim.x - 1) {
+    st->eticket = 0;
+    st->done = 1;
   }
 }
 
@@ -220,7 +229,6 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __res
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part, uint32_t nt, int ns,
                                                      int nv, double* out, KState* st, int phase, int guard) {
   if (guard == FIN_ACTIVE && (st->done || st->early)) return;
-  if (guard == FIN_EARLY && (!st->early || st->done)) return;
   const int K = ns * nv;
   double tot[2] = {0.0, 0.0};
   for (int v = 0; v < nv; ++v) {
@@ -292,7 +300,6 @@ void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* 
 void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
   (void)part;
   hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, ks);
-  launch_finalize(s, nullptr, 0, 1, 0, nullptr, ks, PH_EARLY, FIN_EARLY);
 }
 void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, const double* rt, const double* ph,
                     const double* sh, const double* t, double* part, KState* ks, bool finish) {

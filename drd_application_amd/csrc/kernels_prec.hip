@@ -38,16 +38,11 @@ __device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2
   return -(dx / dx2 + dy / dy2);
 }
 
-__global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const double* __restrict__ bu,
-                                                       const double* __restrict__ minner,
-                                                       const double* __restrict__ m0,
-                                                       const double* __restrict__ b, double* __restrict__ x,
-                                                       const KState* st) {
-  if (st && (st->done || st->early)) return;
-  const int64_t C = g.ncell();
-  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous rows
-  if (q >= C) return;
-  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
+// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell
+__device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
+                                                 const double* __restrict__ minner,
+                                                 const double* __restrict__ m0, int i, int j, double r1,
+                                                 double r2, double& x1, double& x2) {
   const double bv = bu[g.ci(i, j)];
   const double d11 = s.s / s.I - s.I * dgn_diag(g, s.dr2, s.dz2, minner, i, j) +
                      ((j == 0) ? s.I / s.dz2 * (1.0 - s.coef_bot) : 0.0) +
@@ -59,9 +54,100 @@ __global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const dou
   double d22 = -s.dt * s.xld * dgn_diag(g, s.dr2, s.dz2, m0, i, j);
   d22 = d22 > floor22 ? d22 : floor22;
   const double det = d11 * d22 - d12 * d21;
-  const double r1 = b[q], r2 = b[C + q];
-  x[q] = (d22 * r1 - d12 * r2) / det;
-  x[C + q] = (-d21 * r1 + d11 * r2) / det;
+  x1 = (d22 * r1 - d12 * r2) / det;
+  x2 = (-d21 * r1 + d11 * r2) / det;
+}
+
+__global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const double* __restrict__ bu,
+                                                       const double* __restrict__ minner,
+                                                       const double* __restrict__ m0,
+                                                       const double* __restrict__ b, double* __restrict__ x,
+                                                       const KState* st) {
+  if (st && (st->done || st->early)) return;
+  const int64_t C = g.ncell();
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous rows
+  if (q >= C) return;
+  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
+  phmu_block_solve(g, s, bu, minner, m0, i, j, b[q], b[C + q], x[q], x[C + q]);
+}
+
+// BiCGSTAB vector updates with the block preconditioner applied on the fly
+// (one thread per cell holds both the phi and the mu entry):
+//   p = r (first iteration) or beta (p - omega v) + r;  phat = B^-1 p
+__global__ __launch_bounds__(kBlock) void k_bicg_p_block(Grid g, Scal s, const double* __restrict__ bu,
+                                                         const double* __restrict__ minner,
+                                                         const double* __restrict__ m0,
+                                                         const double* __restrict__ r,
+                                                         const double* __restrict__ v, double* __restrict__ p,
+                                                         double* __restrict__ ph, const KState* st) {
+  if (st->done || st->early) return;
+  const bool first = st->iter <= 1;
+  const double omega = st->omega;
+  const double beta = first ? 0.0 : (st->rho / st->rho1) * (st->alpha / st->omega);
+  const int64_t C = g.ncell();
+#pragma unroll
+  for (int jj = 0; jj < kSlots; ++jj) {
+    const int64_t q = tile_elem(C, 0, jj);
+    if (q < 0) break;
+    double pv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t e = q + h * C;
+      if (first) {
+        pv[h] = r[e];
+      } else {
+        double t = p[e] + (-omega) * v[e];
+        t = beta * t;
+        pv[h] = t + r[e];
+      }
+      p[e] = pv[h];
+    }
+    const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
+    phmu_block_solve(g, s, bu, minner, m0, i, j, pv[0], pv[1], ph[q], ph[C + q]);
+  }
+}
+
+//   s = r - alpha v (S aliases R), ||s||^2 (canonical partials), shat = B^-1 s
+__global__ __launch_bounds__(kBlock) void k_bicg_s_block(Grid g, Scal s, const double* __restrict__ bu,
+                                                         const double* __restrict__ minner,
+                                                         const double* __restrict__ m0, double* __restrict__ r,
+                                                         const double* __restrict__ v, double* __restrict__ sh,
+                                                         double* part, const KState* st) {
+  if (st->done || st->early) return;
+  const double alpha = st->alpha;
+  const int64_t C = g.ncell();
+  double t[2][kSlots];
+#pragma unroll
+  for (int jj = 0; jj < kSlots; ++jj) {
+    const int64_t q = tile_elem(C, 0, jj);
+    double sv[2] = {0.0, 0.0};
+    if (q >= 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t e = q + h * C;
+        sv[h] = r[e] + (-alpha) * v[e];
+        r[e] = sv[h];
+      }
+      const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
+      phmu_block_solve(g, s, bu, minner, m0, i, j, sv[0], sv[1], sh[q], sh[C + q]);
+    }
+    t[0][jj] = sv[0] * sv[0];
+    t[1][jj] = sv[1] * sv[1];
+  }
+  tile_publish<2, 1>(t, part);
+}
+
+void launch_bicg_p_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* r,
+                         const double* v, double* p, double* ph, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_p_block, dim3(tiles_of(g.ncell())), dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, r,
+                     v, p, ph, ks);
+}
+
+void launch_bicg_s_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* r, const double* v,
+                         double* sh, double* part, KState* ks) {
+  hipLaunchKernelGGL(k_bicg_s_block, dim3(tiles_of(g.ncell())), dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, r,
+                     v, sh, part, ks);
+  launch_finalize(st, part, tiles_of(g.ncell()), 2, 1, nullptr, ks, PH_S, FIN_ACTIVE);
 }
 
 void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
@@ -114,8 +200,15 @@ struct PrecTables {
   const Cx<T>* twny;   // L = ny  (u columns, DST-II/III)
   const Cx<T>* rotny;
   const Cx<T>* tw2ny;  // L = 2ny (v columns, DST-I)
-  const T* lam_u;      // (nx+1) x ny, column-major by kx: lam_u[kx*ny + ky]
-  const T* lam_v;      // nx x (ny-1): lam_v[kx*(ny-1) + ky]
+  // eigenvalues of A0 times the transform scale (2nx)(2ny):
+  //   lambda = sc (a0 + ex[kx] + ey[ky]),
+  // u: ex_u[kx] = 8 dt sin^2(pi kx / 2nx) / dr2, ey_u[k] = 4 dt sin^2(pi (k+1) / 2ny) / dz2
+  // v: ex_v[kx] = 4 dt sin^2(pi kx / 2nx) / dr2, ey_v[k] = 8 dt sin^2(pi (k+1) / 2ny) / dz2
+  const T* ex_u;  // nx+1
+  const T* ey_u;  // ny
+  const T* ex_v;  // nx
+  const T* ey_v;  // ny-1
+  T sc, a0;
 };
 
 // ---- paired real line transforms -------------------------------------------
@@ -283,13 +376,13 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
     const bool has1 = c0 + 1 < nx + 1;
     T* col0 = tu + (int64_t)c0 * n;
     T* col1 = col0 + n;
-    const T* lam0 = tb.lam_u + (int64_t)c0 * n;
-    const T* lam1 = lam0 + n;
+    const T e0 = tb.a0 + tb.ex_u[c0], e1 = has1 ? tb.a0 + tb.ex_u[c0 + 1] : T(1);
     dct2_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true,
                  [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
                  [&](int k, T y0, T y1) {
-                   l0[k] = y0 / lam0[k];
-                   l1[k] = has1 ? y1 / lam1[k] : T(0);
+                   const T ey = tb.ey_u[k];
+                   l0[k] = y0 / (tb.sc * (e0 + ey));
+                   l1[k] = has1 ? y1 / (tb.sc * (e1 + ey)) : T(0);
                  });
     dct3_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int j, T x0, T x1) {
@@ -301,12 +394,12 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
     const bool has1 = c0 + 1 < nx;
     T* col0 = tv + (int64_t)c0 * n;
     T* col1 = col0 + n;
-    const T* lam0 = tb.lam_v + (int64_t)c0 * n;
-    const T* lam1 = lam0 + n;
+    const T e0 = tb.a0 + tb.ex_v[c0], e1 = has1 ? tb.a0 + tb.ex_v[c0 + 1] : T(1);
     dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
                  [&](int k, T y0, T y1) {
-                   l0[k] = y0 / lam0[k];
-                   l1[k] = has1 ? y1 / lam1[k] : T(0);
+                   const T ey = tb.ey_v[k];
+                   l0[k] = y0 / (tb.sc * (e0 + ey));
+                   l1[k] = has1 ? y1 / (tb.sc * (e1 + ey)) : T(0);
                  });
     dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int k, T y0, T y1) {
@@ -338,14 +431,14 @@ __global__ __launch_bounds__(kBlock) void k_transposeT(int rows, int cols, const
 struct VeloPrec {
   int nx, ny;
   bool fp32;
-  void* tabs_mem = nullptr;  // all tables in one allocation
+  void* tabs_mem = nullptr;  // twiddles and rotations (depend on nx, ny only)
+  void* eig_mem = nullptr;   // ex_u, ey_u, ex_v, ey_v (depend on dt, dx, dz)
   PrecTables<float> tf{};
   PrecTables<double> td{};
   double* inv_s64 = nullptr;
   float* inv_s32 = nullptr;
   void* work = nullptr;  // wu, wv, tu, tv in T
   double a0 = 0;
-  bool scaled = false;
 };
 
 template <class T>
@@ -363,9 +456,8 @@ static void host_rot(std::vector<Cx<T>>& v, int L) {
   }
 }
 
-// eigenvalues of A0 times the unnormalised transform pair scale (2nx)(2ny)
 template <class T>
-static int build_prec_tables(VeloPrec* pc, const Scal& s, PrecTables<T>& tb) {
+static int build_twiddles(VeloPrec* pc, PrecTables<T>& tb) {
   const int nx = pc->nx, ny = pc->ny;
   std::vector<Cx<T>> cx;
   std::vector<size_t> off;
@@ -381,25 +473,9 @@ static int build_prec_tables(VeloPrec* pc, const Scal& s, PrecTables<T>& tb) {
   host_rot<T>(cx, ny);
   off.push_back(cx.size());
   host_tw<T>(cx, 2 * ny);
-  off.push_back(cx.size());
-  std::vector<T> lam;
-  const double sc = (2.0 * nx) * (2.0 * ny);
-  for (int kx = 0; kx <= nx; ++kx)
-    for (int ky = 0; ky < ny; ++ky) {
-      const double sx = std::sin(M_PI * kx / (2.0 * nx)), sy = std::sin(M_PI * (ky + 1) / (2.0 * ny));
-      lam.push_back((T)(sc * (pc->a0 + s.dt * (2.0 * 4.0 * sx * sx / s.dr2 + 4.0 * sy * sy / s.dz2))));
-    }
-  const size_t lam_v_off = lam.size();
-  for (int kx = 0; kx < nx; ++kx)
-    for (int ky = 1; ky < ny; ++ky) {
-      const double sx = std::sin(M_PI * kx / (2.0 * nx)), sy = std::sin(M_PI * ky / (2.0 * ny));
-      lam.push_back((T)(sc * (pc->a0 + s.dt * (4.0 * sx * sx / s.dr2 + 2.0 * 4.0 * sy * sy / s.dz2))));
-    }
-  const size_t bytes_c = cx.size() * sizeof(Cx<T>), bytes_l = lam.size() * sizeof(T);
-  if (hipMalloc(&pc->tabs_mem, bytes_c + bytes_l) != hipSuccess) return DRD_E_HIP;
-  if (hipMemcpy(pc->tabs_mem, cx.data(), bytes_c, hipMemcpyHostToDevice) != hipSuccess) return DRD_E_HIP;
-  if (hipMemcpy((char*)pc->tabs_mem + bytes_c, lam.data(), bytes_l, hipMemcpyHostToDevice) != hipSuccess)
-    return DRD_E_HIP;
+  const size_t bytes = cx.size() * sizeof(Cx<T>);
+  if (hipMalloc(&pc->tabs_mem, bytes) != hipSuccess) return DRD_E_HIP;
+  if (hipMemcpy(pc->tabs_mem, cx.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return DRD_E_HIP;
   const Cx<T>* base = reinterpret_cast<const Cx<T>*>(pc->tabs_mem);
   tb.tw2nx = base + off[0];
   tb.twnx = base + off[1];
@@ -407,10 +483,48 @@ static int build_prec_tables(VeloPrec* pc, const Scal& s, PrecTables<T>& tb) {
   tb.twny = base + off[3];
   tb.rotny = base + off[4];
   tb.tw2ny = base + off[5];
-  const T* lb = reinterpret_cast<const T*>((char*)pc->tabs_mem + bytes_c);
-  tb.lam_u = lb;
-  tb.lam_v = lb + lam_v_off;
   return 0;
+}
+
+// 1-D parts of the eigenvalues of A0 = a0 I - dt (2 dxx + dyy) (u) and
+// a0 I - dt (dxx + 2 dyy) (v) under the reference's ghost rules, and the
+// scale (2nx)(2ny) of the unnormalised transform pair.
+template <class T>
+static int build_eigen(VeloPrec* pc, const Scal& s, PrecTables<T>& tb) {
+  const int nx = pc->nx, ny = pc->ny;
+  std::vector<T> e;
+  for (int kx = 0; kx <= nx; ++kx) {
+    const double sx = std::sin(M_PI * kx / (2.0 * nx));
+    e.push_back((T)(s.dt * 2.0 * 4.0 * sx * sx / s.dr2));
+  }
+  for (int ky = 0; ky < ny; ++ky) {
+    const double sy = std::sin(M_PI * (ky + 1) / (2.0 * ny));
+    e.push_back((T)(s.dt * 4.0 * sy * sy / s.dz2));
+  }
+  for (int kx = 0; kx < nx; ++kx) {
+    const double sx = std::sin(M_PI * kx / (2.0 * nx));
+    e.push_back((T)(s.dt * 4.0 * sx * sx / s.dr2));
+  }
+  for (int ky = 1; ky < ny; ++ky) {
+    const double sy = std::sin(M_PI * ky / (2.0 * ny));
+    e.push_back((T)(s.dt * 2.0 * 4.0 * sy * sy / s.dz2));
+  }
+  const size_t bytes = e.size() * sizeof(T);
+  if (!pc->eig_mem && hipMalloc(&pc->eig_mem, bytes) != hipSuccess) return DRD_E_HIP;
+  if (hipMemcpy(pc->eig_mem, e.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return DRD_E_HIP;
+  const T* b = reinterpret_cast<const T*>(pc->eig_mem);
+  tb.ex_u = b;
+  tb.ey_u = b + (nx + 1);
+  tb.ex_v = b + (nx + 1) + ny;
+  tb.ey_v = b + (nx + 1) + ny + nx;
+  tb.sc = (T)((2.0 * nx) * (2.0 * ny));
+  tb.a0 = (T)pc->a0;
+  return 0;
+}
+
+int velo_prec_update(VeloPrec* pc, const Scal& s) {
+  pc->a0 = 1.5 * s.re;  // unit density, unit viscosity, no penalty
+  return pc->fp32 ? build_eigen<float>(pc, s, pc->tf) : build_eigen<double>(pc, s, pc->td);
 }
 
 static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -425,8 +539,8 @@ VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err) {
   pc->nx = g.nx;
   pc->ny = g.ny;
   pc->fp32 = fp32;
-  pc->a0 = 1.5 * s.re;  // unit density, unit viscosity, no penalty
-  int rc = fp32 ? build_prec_tables<float>(pc, s, pc->tf) : build_prec_tables<double>(pc, s, pc->td);
+  int rc = fp32 ? build_twiddles<float>(pc, pc->tf) : build_twiddles<double>(pc, pc->td);
+  if (!rc) rc = velo_prec_update(pc, s);
   const size_t esz = fp32 ? sizeof(float) : sizeof(double);
   if (!rc && hipMalloc(&pc->inv_s64, g.nuv() * sizeof(double)) != hipSuccess) rc = DRD_E_HIP;
   if (!rc && hipMalloc(&pc->inv_s32, g.nuv() * sizeof(float)) != hipSuccess) rc = DRD_E_HIP;
@@ -442,6 +556,7 @@ VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err) {
 void velo_prec_destroy(VeloPrec* pc) {
   if (!pc) return;
   (void)hipFree(pc->tabs_mem);
+  (void)hipFree(pc->eig_mem);
   (void)hipFree(pc->inv_s64);
   (void)hipFree(pc->inv_s32);
   (void)hipFree(pc->work);
