@@ -53,7 +53,21 @@ struct Grid {
   __host__ __device__ int64_t ci(int i, int k) const { return (int64_t)(i + 1) + (int64_t)(nx + 2) * (k + 1); }
   __host__ __device__ int64_t ui(int i, int k) const { return (int64_t)(i + 1) + (int64_t)(nx + 3) * (k + 1); }
   __host__ __device__ int64_t vi(int i, int k) const { return (int64_t)(i + 1) + (int64_t)(nx + 2) * k; }
+  // 32-bit forms for the hot stencils (every array is < 2^31 elements)
+  __device__ __forceinline__ uint32_t cio(int i, int k) const { return (uint32_t)((i + 1) + (nx + 2) * (k + 1)); }
 };
+
+// Load / store at a 32-bit element offset: the byte offset stays 32 bits, so
+// the access uses the scalar-base + 32-bit vector-offset address form
+// instead of 64-bit address arithmetic per neighbour.
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ p, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* __restrict__ p, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (uint32_t)sizeof(T)) = v;
+}
 
 // Scalars of module `global` the kernels read (TP/global.f90:42-52).
 struct Scal {
@@ -204,6 +218,18 @@ __device__ __forceinline__ uint32_t tile_id() {
 __device__ __forceinline__ int64_t tile_elem(int64_t seg, int s, int j) {
   const int64_t o = (int64_t)tile_id() * kTile + j * kBlock + threadIdx.x;
   return o < seg ? (int64_t)s * seg + o : -1;
+}
+
+// The same for an explicit tile index (workgroups that sweep several tiles).
+template <int NS, int NV>
+__device__ __forceinline__ void tile_publish_at(const double (&t)[NS * NV][kSlots], double* part, uint32_t tile) {
+  constexpr int K = NS * NV;
+  double p[K];
+  tile_pairwise<K>(t, p);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[(size_t)tile * K + k] = p[k];
+  }
 }
 
 // Publishes this workgroup's partial sums: value v of segment s at

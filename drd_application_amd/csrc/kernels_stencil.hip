@@ -16,38 +16,27 @@ namespace drd {
 struct VeloIn {
   const double* __restrict__ x;
   int nx, ny, lr_d;
-  int64_t mn;
-  __device__ __forceinline__ double uraw(int i, int k) const { return x[i + (int64_t)(nx + 1) * k]; }
-  // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils)
-  // (wall ghosts are odd mirrors of the first/last row, side ghosts even or
-  // odd mirrors of column 1 / nx-1; negation is exact, so the order of the
-  // two mirrorings does not matter)
+  uint32_t mn;
+  __device__ __forceinline__ double uraw(int i, int k) const { return ld32(x, (uint32_t)(i + (nx + 1) * k)); }
+  // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
+  // wall ghosts are odd mirrors of the first/last row, side ghosts even or
+  // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
+  // commute; both are applied branch-free.
   __device__ __forceinline__ double U(int i, int k) const {
-    bool neg = false;
-    if (k < 0) {
-      k = 0;
-      neg = true;
-    } else if (k >= ny) {
-      k = ny - 1;
-      neg = true;
-    }
-    double v;
-    if (i < 0) {
-      v = uraw(1, k);
-      if (lr_d) v = -v;
-    } else if (i > nx) {
-      v = uraw(nx - 1, k);
-      if (lr_d) v = -v;
-    } else {
-      v = uraw(i, k);
-    }
-    return neg ? -v : v;
+    const bool kw = (k < 0) | (k >= ny);
+    const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
+    const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
+    const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
+    const double v = ld32(x, (uint32_t)(ii + (nx + 1) * kk));
+    const double w = iw ? -v : v;
+    return kw ? -w : w;
   }
-  // f2(i,k) for -1<=i<=nx, 0<=k<=ny
+  // f2(i,k) for -1<=i<=nx, 0<=k<=ny (0 on the walls k = 0, ny)
   __device__ __forceinline__ double V(int i, int k) const {
-    if (k <= 0 || k >= ny) return 0.0;
-    i = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
-    return x[mn + i + (int64_t)nx * (k - 1)];
+    const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+    const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
+    const double v = ld32(x, mn + (uint32_t)(ic + nx * (kc - 1)));
+    return (k <= 0 || k >= ny) ? 0.0 : v;
   }
 };
 
@@ -58,40 +47,42 @@ template <int BDF>
 __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const double* __restrict__ rho,
                                            const double* __restrict__ eta,
                                            const double* __restrict__ chi, const VeloIn& X,
-                                           int64_t q) {
+                                           uint32_t q) {
   const int nx = g.nx;
   double r;
   if (q < X.mn) {
     const int k = g.by_nx1.div(q);
-    const int i = (int)(q - (int64_t)k * (nx + 1));
+    const int i = (int)(q - (uint32_t)k * (uint32_t)(nx + 1));
+    const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
     const double f = X.U(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
-          f * s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
+      r = f * 1.5 * s.re * (ld32(rho, c) + ld32(rho, cw)) / 2.0 +
+          f * s.penalty_k * (ld32(chi, c) + ld32(chi, cw)) / 2.0 * s.dt;
     else
-      r = f * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0;
-    const double e00 = eta[g.ci(i, k)], e10 = eta[g.ci(i - 1, k)];
+      r = f * s.re * (ld32(rho, c) + ld32(rho, cw)) / 2.0;
+    const double e00 = ld32(eta, c), e10 = ld32(eta, cw);
     double cr = e00, cl = e10;
     r = r - 2.0 * s.dt * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f) / s.dr2;
-    cr = (eta[g.ci(i, k + 1)] + eta[g.ci(i - 1, k + 1)] + e00 + e10) / 4.0;
-    cl = (e00 + e10 + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    cr = (ld32(eta, c + ld) + ld32(eta, cw + ld) + e00 + e10) / 4.0;
+    cl = (e00 + e10 + ld32(eta, c - ld) + ld32(eta, cw - ld)) / 4.0;
     r = r - s.dt * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f) / s.dz2;
     r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
   } else {
-    const int64_t t = q - X.mn;
+    const uint32_t t = q - X.mn;
     const int k = g.by_nx.div(t) + 1;
-    const int i = (int)(t - (int64_t)(k - 1) * nx);
+    const int i = (int)(t - (uint32_t)(k - 1) * (uint32_t)nx);
+    const uint32_t c = g.cio(i, k), ld = (uint32_t)(nx + 2), cs = c - ld;
     const double f = X.V(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
-          f * s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
+      r = f * 1.5 * s.re * (ld32(rho, c) + ld32(rho, cs)) / 2.0 +
+          f * s.penalty_k * (ld32(chi, c) + ld32(chi, cs)) / 2.0 * s.dt;
     else
-      r = f * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0;
-    const double e00 = eta[g.ci(i, k)], e01 = eta[g.ci(i, k - 1)];
+      r = f * s.re * (ld32(rho, c) + ld32(rho, cs)) / 2.0;
+    const double e00 = ld32(eta, c), e01 = ld32(eta, cs);
     double cr = e00, cl = e01;
     r = r - 2.0 * s.dt * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f) / s.dz2;
-    cr = (eta[g.ci(i + 1, k)] + e00 + eta[g.ci(i + 1, k - 1)] + e01) / 4.0;
-    cl = (e00 + eta[g.ci(i - 1, k)] + e01 + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    cr = (ld32(eta, c + 1) + e00 + ld32(eta, cs + 1) + e01) / 4.0;
+    cl = (e00 + ld32(eta, c - 1) + e01 + ld32(eta, cs - 1)) / 4.0;
     r = r - s.dt * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f) / s.dr2;
     r = r - s.dt * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1))) / s.dr / s.dz;
   }
@@ -116,43 +107,52 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
                                                         const double* __restrict__ b, double* part,
                                                         KState* st) {
   if (MODE != MV_PLAIN && !kactive(st)) return;
-  VeloIn X{x, g.nx, g.ny, s.lr_d, g.mn()};
-  const int64_t n = g.nuv();
-  double tm[2][kSlots] = {};
+  VeloIn X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+  const uint32_t n = (uint32_t)g.nuv();
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
+  // A workgroup sweeps tile `pair` (u half of the vector) and tile
+  // pair + half (v half): both cover about the same grid rows, so the u-v
+  // cross terms and the shared viscosity rows are read once.
+  const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile), half = (nt + 1) / 2;
+  const uint32_t pair = tile_id();
 #pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int64_t q = tile_elem(n, 0, j);
-    if (q < 0) break;
-    if (MODE == MV_PLAIN) {
-      // y is not read when beta == 0 (as in BLAS), so it may be uninitialised
-      const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
-      y[q] = beta == 0.0 ? alpha * z : alpha * z + beta * y[q];
-    } else if (MODE == MV_INIT) {
-      double r = b[q];
-      if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
-      y[q] = r;
-      a0[q] = r;
-      tm[0][j] = r * r;
-    } else {
-      const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
-      y[q] = z;
-      if (MODE == MV_DOT1) {
-        tm[0][j] = a0[q] * z;
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t tile = pair + (uint32_t)h * half;
+    if (tile >= nt) break;
+    double tm[2][kSlots] = {};
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const uint32_t q = tile * (uint32_t)kTile + (uint32_t)(j * kBlock) + threadIdx.x;
+      if (q >= n) break;
+      if (MODE == MV_PLAIN) {
+        // y is not read when beta == 0 (as in BLAS), so it may be uninitialised
+        const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
+        st32(y, q, beta == 0.0 ? alpha * z : alpha * z + beta * ld32((const double*)y, q));
+      } else if (MODE == MV_INIT) {
+        double r = ld32(b, q);
+        if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
+        st32(y, q, r);
+        st32(a0, q, r);
+        tm[0][j] = r * r;
       } else {
-        tm[0][j] = z * a0[q];
-        tm[1][j] = z * z;
+        const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
+        st32(y, q, z);
+        if (MODE == MV_DOT1) {
+          tm[0][j] = ld32((const double*)a0, q) * z;
+        } else {
+          tm[0][j] = z * ld32((const double*)a0, q);
+          tm[1][j] = z * z;
+        }
       }
     }
-  }
-  if (MODE == MV_PLAIN) return;
-  if (MODE == MV_DOT2) {
-    tile_publish<1, 2>(tm, part);
-  } else {
-    double t1[1][kSlots];
+    if (MODE == MV_DOT2) {
+      tile_publish_at<1, 2>(tm, part, tile);
+    } else if (MODE != MV_PLAIN) {
+      double t1[1][kSlots];
 #pragma unroll
-    for (int j = 0; j < kSlots; ++j) t1[0][j] = tm[0][j];
-    tile_publish<1, 1>(t1, part);
+      for (int j = 0; j < kSlots; ++j) t1[0][j] = tm[0][j];
+      tile_publish_at<1, 1>(t1, part, tile);
+    }
   }
 }
 
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const do
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   const int64_t nu = g.nuf();
-  VeloIn X{x, nx, ny, 0, g.mn()};
+  VeloIn X{x, nx, ny, 0, (uint32_t)g.mn()};
   if (q < nu) {
     const int k = (int)(q / (nx + 3)) - 1;
     const int i = (int)(q - (int64_t)(k + 1) * (nx + 3)) - 1;
@@ -576,7 +576,8 @@ static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1
 void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
                         const double* b, double* part, KState* ks, bool finish) {
-  dim3 grid(nblk(g.nuv(), kTile));
+  const unsigned ntiles = nblk(g.nuv(), kTile);
+  dim3 grid((ntiles + 1) / 2);  // two tiles per workgroup
 #define VM(B, M)                                                                                     \
   hipLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, y, \
                      alpha, beta, a0, b, part, ks)
@@ -596,7 +597,7 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
     }
   }
 #undef VM
-  if (finish) launch_mv_finalize(st, mode, part, grid.x, 1, ks);
+  if (finish) launch_mv_finalize(st, mode, part, ntiles, 1, ks);
 }
 
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
