@@ -29,6 +29,7 @@ PREC_IDENTITY = 1
 PREC_JACOBI = 2
 PREC_DCT32 = 3
 PREC_DCT64 = 4
+PREC_LEARNED = 5
 
 CH_PREC_REFERENCE = 0
 CH_PREC_BLOCK = 1
@@ -49,6 +50,9 @@ PS_VELO_DCT32 = 4
 PS_PHMU_BLOCK = 5
 PS_VELO_DCT64 = 6
 PS_VELO_ESOLP = 7
+PS_VELO_LEARNED = 8
+
+LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 
 K_VELO_MATVEC, K_PHMU_MATVEC, K_BICG_XR, K_PRESSURE, K_COPY = 0, 1, 2, 3, 4
 
@@ -105,6 +109,20 @@ class SolveInfo(C.Structure):
     def as_dict(self) -> dict:
         return {"iter": self.iter, "info": self.info, "resid": self.resid,
                 "true_resid": self.true_resid}
+
+
+class LearnInfo(C.Structure):
+    """``drd_learn_info`` (include/drd_gpu.h)."""
+
+    _fields_ = [("state", C.c_int32), ("pairs", C.c_int32), ("capacity", C.c_int32),
+                ("train_iters", C.c_int32), ("ref_iters", C.c_int32),
+                ("judged_iters", C.c_int32), ("last_iters", C.c_int32),
+                ("fallbacks", C.c_int32), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("train_ms", C.c_double),
+                ("judged_ms", C.c_double), ("plain_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class StepInfo(C.Structure):

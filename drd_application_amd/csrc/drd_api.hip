@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "unet.hpp"
 
 using namespace drd;
 
@@ -38,6 +40,47 @@ static int set_err(int code, const std::string& msg) {
     int r_ = (expr);          \
     if (r_ != 0) return r_;   \
   } while (0)
+
+// Learned correction of the velocity preconditioner: the pair store, the two
+// networks (ny x nx for the preconditioner, crop size for training) and the
+// judge/fallback state.
+constexpr int kSnapMax = 16;        // recorded iterates per collected solve
+constexpr double kLearnSlack = 1.25;  // allowed iteration growth over the judged plain count
+
+struct Learn {
+  UNet* net = nullptr;
+  UNet* tr = nullptr;
+  int ch = 0, cw = 0;
+  int cap = 0, count = 0, next = 0;
+  int64_t npx = 0;
+  uint16_t* feat = nullptr;  // cap x npx x 8 bf16
+  float* targ = nullptr;     // cap x npx x 2
+  uint16_t* img = nullptr;   // npx x 8 (preconditioner input)
+  float* out = nullptr;      // npx x 2
+  uint16_t* cfeat = nullptr; // crop input
+  float* ctarg = nullptr;
+  double *ss = nullptr, *ss_part = nullptr;
+  double *r = nullptr, *d = nullptr, *z = nullptr, *e = nullptr, *w = nullptr;
+  double* dac = nullptr;  // [d | A c], contiguous for one two-vector reduction
+  double *part = nullptr, *dots = nullptr;
+  double* snaps = nullptr;   // kSnapMax x Nuv
+  uint64_t rng = 0;
+  int train_iters = 0;
+  int state = DRD_LEARN_OFF;
+  int ref_its = 0, judged_its = 0, last_its = 0, fallbacks = 0;
+  double judged_ms[2] = {0, 0};  // learned FGMRES, plain method
+  bool record = false;
+  double loss_first = 0, loss_last = 0, train_ms = 0;
+  std::vector<void*> allocs;
+};
+
+static void learn_destroy(Learn* L) {
+  if (!L) return;
+  unet_destroy(L->net);
+  unet_destroy(L->tr);
+  for (void* p : L->allocs) (void)hipFree(p);
+  delete L;
+}
 
 struct drd_ctx {
   drd_params p;
@@ -77,6 +120,7 @@ struct drd_ctx {
   double *ph_f2 = nullptr, *ph_t = nullptr, *aux_work = nullptr, *esolp_tabs = nullptr;
   double* dct_work = nullptr;
   double cosa = 0, sqrt2 = 0;
+  Learn* learn = nullptr;  // drd_learn_init
 };
 
 // ---------------------------------------------------------------------------
@@ -336,6 +380,8 @@ static void ctx_free(drd_ctx* c) {
   hipFree(c->dct_work);
   dct_plan_destroy(c->dct);
   velo_prec_destroy(c->vprec);
+  learn_destroy(c->learn);
+  c->learn = nullptr;
   hipFree(c->fg_V);
   hipFree(c->fg_Z);
   hipFree(c->fg_h);
@@ -355,8 +401,8 @@ static void ctx_free(drd_ctx* c) {
 // when drd_set_params first asks for it), so no solve pays its set-up.
 static int eager_vprec(drd_ctx* c) {
   const int vp = c->p.velo_prec;
-  if (vp != DRD_PREC_DCT32 && vp != DRD_PREC_DCT64) return 0;
-  const bool fp32 = vp == DRD_PREC_DCT32;
+  if (vp != DRD_PREC_DCT32 && vp != DRD_PREC_DCT64 && vp != DRD_PREC_LEARNED) return 0;
+  const bool fp32 = vp != DRD_PREC_DCT64;
   if (c->vprec && c->vprec_fp32 == fp32) return 0;
   velo_prec_destroy(c->vprec);
   c->vprec = nullptr;
@@ -534,8 +580,8 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
 // Prepares a preconditioner once per solve (Jacobi diagonal, DCT scaling).
 static int prec_setup(drd_ctx* c, int prec) {
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag);
-  if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64) {
-    const bool fp32 = (prec == DRD_PS_VELO_DCT32);
+  if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 || prec == DRD_PS_VELO_LEARNED) {
+    const bool fp32 = (prec != DRD_PS_VELO_DCT64);
     if (c->vprec && c->vprec_fp32 != fp32) {
       HIPCHK(hipStreamSynchronize(c->stream));
       velo_prec_destroy(c->vprec);
@@ -553,10 +599,39 @@ static int prec_setup(drd_ctx* c, int prec) {
   return 0;
 }
 
+// z += alpha s_d N: the learned correction for the vector b (z = M b on
+// entry), alpha minimising |b - A z| (device-side scalars, no host sync), so
+// the corrected residual is never larger than the plain DCT solve's.
+static int learn_correct(drd_ctx* c, const double* b, double* z) {
+  Learn* L = c->learn;
+  hipStream_t st = c->stream;
+  const int64_t n = c->g.nuv();
+  double* d = L->dac;       // d = b - A z
+  double* ac = L->dac + n;  // A (correction)
+  launch_copy(st, n, b, d);
+  RCHK(apply_op(c, DRD_OP_VELO2, MV_PLAIN, z, d, -1.0, 1.0, nullptr, nullptr, nullptr));
+  launch_norms3(st, n, b, d, z, L->ss_part, L->ss);
+  launch_pack_features(st, c->g, b, d, z, c->f[DRD_F_ETA], c->f[DRD_F_BDR_PH_PENALTY], L->ss, L->img);
+  RCHK(unet_forward(st, L->net, L->img, L->out));
+  launch_unpack_correction(st, c->g, L->out, L->ss, L->w);
+  RCHK(apply_op(c, DRD_OP_VELO2, MV_PLAIN, L->w, ac, 1.0, 0.0, nullptr, nullptr, nullptr));
+  launch_multidot(st, n, 1, ac, L->dac, n, 2, L->part, L->dots);  // (Ac, d), (Ac, Ac)
+  launch_axpy_ratio(st, n, L->dots, L->w, z);
+  return 0;
+}
+
 // x_out = M^-1 b_in; may return b_in itself (identity).
 static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, const double** res,
                       const KState* ks) {
   switch (prec) {
+    case DRD_PS_VELO_LEARNED:
+      if (op != DRD_OP_VELO2 || !c->vprec) break;
+      if (!c->learn || c->learn->train_iters == 0)
+        return set_err(DRD_E_STATE, "learned preconditioner: network not trained (drd_learn_train)");
+      velo_prec_apply(c->stream, c->vprec, c->g, b, x, ks);
+      RCHK(learn_correct(c, b, x));
+      *res = x;
+      return 0;
     case DRD_PS_IDENTITY:
       *res = b;
       return 0;
@@ -613,9 +688,14 @@ static int true_resid(drd_ctx* c, int op, int64_t n, const double* b, const doub
 }
 
 // BICGSTAB LIB/BiCGSTAB.f + LIB/BiCGSTABREVCOM.f on device vectors b, x.
+// Optional recording: x after iteration snap_at[j] is copied to snaps + j*n
+// (nsnap entries, increasing).
 static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b, double* x, int maxit,
-                        double tol, drd_solve_info* info, bool want_true) {
+                        double tol, drd_solve_info* info, bool want_true, const int* snap_at = nullptr,
+                        int nsnap = 0, double* snaps = nullptr) {
   if (n != op_size(c, op)) return set_err(DRD_E_ARG, "vector length does not match the operator");
+  if (prec == DRD_PS_VELO_LEARNED)
+    return set_err(DRD_E_UNSUPPORTED, "the learned preconditioner is not linear: use FGMRES");
   if (maxit <= 0) {  // LIB/BiCGSTAB.f:161-163
     if (info) *info = drd_solve_info{maxit, -3, tol, 0.0};
     return 0;
@@ -638,7 +718,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   const bool fused_block = (op == DRD_OP_PHMU && prec == DRD_PS_PHMU_BLOCK);
   launch_init_norms(c->stream, n, ns, x, b, c->part, c->ks);
   RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
-  int chunk = 1, total = 0;
+  int chunk = 1, total = 0, next_snap = 0;
   for (;;) {
     for (int it = 0; it < chunk; ++it) {
       const double* ph = nullptr;
@@ -662,6 +742,10 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
       }
       RCHK(apply_op(c, op, MV_DOT2, sh, T, 1.0, 0.0, R, nullptr, c->ks));
       launch_bicg_xr(c->stream, n, ns, x, R, RT, ph, sh, T, c->part, c->ks);
+      if (next_snap < nsnap && total + it + 1 == snap_at[next_snap]) {
+        launch_copy(c->stream, n, x, snaps + (size_t)next_snap * n);
+        ++next_snap;
+      }
     }
     HIPCHK(hipGetLastError());
     total += chunk;
@@ -768,7 +852,10 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
       double* zj = Z + (size_t)j * ld;
       double* w = V + (size_t)(j + 1) * ld;
       const double* zr = nullptr;
-      RCHK(apply_prec(c, prec, op, vj, zj, &zr, nullptr));
+      // the learned correction opens each cycle; the cycle's other directions
+      // come from the plain FP32 DCT solve
+      const int pj = (prec == DRD_PS_VELO_LEARNED && j > 0) ? DRD_PS_VELO_DCT32 : prec;
+      RCHK(apply_prec(c, pj, op, vj, zj, &zr, nullptr));
       if (zr != zj) launch_copy(c->stream, n, zr, zj);
       RCHK(apply_op(c, op, MV_PLAIN, zj, w, 1.0, 0.0, nullptr, nullptr, nullptr));
       // pass 1: h = V^T w and ||w||^2 (w = V_{j+1} is contiguous with V)
@@ -821,6 +908,370 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
   return 0;
 }
 
+static int up(drd_ctx* c, double* dst, const double* src, int64_t n);
+static int down(drd_ctx* c, double* dst, const double* src, int64_t n);
+
+// ---- learned correction: solver-emitted pairs, training, judged solves -------
+static uint64_t splitmix(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// iterations whose iterate a recorded solve keeps: k = round(1.5^j), distinct
+static int snap_schedule(int maxit, int* at) {
+  int n = 0, last = 0;
+  for (double k = 1.0; n < kSnapMax; k *= 1.5) {
+    const int ki = (int)std::lround(k);
+    if (ki > maxit) break;
+    if (ki > last) at[n++] = last = ki;
+  }
+  return n;
+}
+
+// Pairs from the iterates snaps[j] (kept at at[j]) of a solve of A x = b that
+// converged in `iters` iterations to x: r = b - A x_k, z = M r (FP32 DCT),
+// d = r - A z, e = x - x_k (= A^-1 r); features of (r, d, z) and target
+// (e - z) / s_d, the error the DCT solve leaves.
+static int learn_pairs(drd_ctx* c, const double* b, const double* x, const int* at, int nsnap, int iters,
+                       int* added) {
+  Learn* L = c->learn;
+  hipStream_t st = c->stream;
+  const int64_t n = c->g.nuv();
+  *added = 0;
+  for (int j = 0; j < nsnap && at[j] <= 0.8 * iters; ++j) {
+    const double* xk = L->snaps + (size_t)j * n;
+    launch_copy(st, n, b, L->r);
+    RCHK(apply_op(c, DRD_OP_VELO2, MV_PLAIN, xk, L->r, -1.0, 1.0, nullptr, nullptr, nullptr));
+    velo_prec_apply(st, c->vprec, c->g, L->r, L->z, nullptr);
+    launch_copy(st, n, L->r, L->d);
+    RCHK(apply_op(c, DRD_OP_VELO2, MV_PLAIN, L->z, L->d, -1.0, 1.0, nullptr, nullptr, nullptr));
+    launch_copy(st, n, x, L->e);
+    launch_axpby(st, n, -1.0, xk, 1.0, L->e);
+    launch_axpby(st, n, -1.0, L->z, 1.0, L->e);
+    launch_norms3(st, n, L->r, L->d, L->z, L->ss_part, L->ss);
+    const size_t slot = (size_t)L->next;
+    launch_pack_features(st, c->g, L->r, L->d, L->z, c->f[DRD_F_ETA], c->f[DRD_F_BDR_PH_PENALTY], L->ss,
+                         L->feat + slot * L->npx * kUnetIn);
+    launch_make_target(st, c->g, L->e, L->ss, L->targ + slot * L->npx * kUnetOut);
+    L->next = (L->next + 1) % L->cap;
+    L->count = std::min(L->count + 1, L->cap);
+    ++*added;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// BiCGSTAB with the FP32 DCT preconditioner on the BDF2 velocity operator,
+// keeping iterates; the solution in x is the solve's own result.
+static int learn_recorded_solve(drd_ctx* c, const double* b, double* x, int maxit, double tol,
+                                drd_solve_info* info, int* added) {
+  int at[kSnapMax];
+  const int ns = snap_schedule(maxit, at);
+  drd_solve_info loc{};
+  RCHK(run_bicgstab(c, DRD_OP_VELO2, DRD_PS_VELO_DCT32, c->g.nuv(), b, x, maxit, tol, &loc, false, at, ns,
+                    c->learn->snaps));
+  RCHK(learn_pairs(c, b, x, at, ns, loc.iter, added));
+  if (info) *info = loc;
+  return 0;
+}
+
+// The velocity solve of velo_prec = DRD_PREC_LEARNED.  Plain method: BiCGSTAB
+// with the FP32 DCT preconditioner (the fastest plain solve).  The first solve
+// after drd_learn_enable runs three ways from the same start: FGMRES with the
+// correction, FGMRES with the plain preconditioner (the iteration reference)
+// and the plain method (the time reference); the correction stays in use only
+// if it needs no more iterations than plain FGMRES and no more time than the
+// plain method.  A later learned solve that needs more than kLearnSlack times
+// the reference iterations is finished by the plain method and the context
+// falls back to it.
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int velo_solve_learned(drd_ctx* c, const double* b, double* x, int maxit, double tol,
+                              drd_solve_info* info) {
+  Learn* L = c->learn;
+  const int64_t n = c->g.nuv();
+  const int op = DRD_OP_VELO2, m = kFgmresRestart;
+  hipStream_t st = c->stream;
+  auto plain = [&](double* xx, int its, drd_solve_info* out) {
+    return run_bicgstab(c, op, DRD_PS_VELO_DCT32, n, b, xx, its, tol, out, false);
+  };
+  if (L && L->state == DRD_LEARN_JUDGE) {
+    launch_copy(st, n, x, L->d);  // x0 for plain FGMRES
+    launch_copy(st, n, x, L->z);  // x0 for the plain method
+    drd_solve_info li{}, pi{}, bi{};
+    HIPCHK(hipStreamSynchronize(st));
+    double t0 = now_ms();
+    RCHK(run_fgmres(c, op, DRD_PS_VELO_LEARNED, n, b, x, m, maxit, tol, &li));
+    HIPCHK(hipStreamSynchronize(st));
+    const double tl = now_ms() - t0;
+    RCHK(run_fgmres(c, op, DRD_PS_VELO_DCT32, n, b, L->d, m, maxit, tol, &pi));
+    HIPCHK(hipStreamSynchronize(st));
+    t0 = now_ms();
+    RCHK(plain(L->z, maxit, &bi));
+    const double tb = now_ms() - t0;
+    L->ref_its = pi.iter;
+    L->judged_its = li.iter;
+    L->last_its = li.iter;
+    L->judged_ms[0] = tl;
+    L->judged_ms[1] = tb;
+    if (li.info == 0 && li.iter <= pi.iter && tl <= tb) {
+      L->state = DRD_LEARN_ON;
+      *info = li;
+    } else {
+      L->state = DRD_LEARN_FALLBACK;
+      ++L->fallbacks;
+      launch_copy(st, n, L->z, x);
+      *info = bi;
+    }
+    return 0;
+  }
+  if (L && L->state == DRD_LEARN_ON) {
+    const int allow = std::min(maxit, std::max(1, (int)std::ceil(kLearnSlack * L->ref_its)));
+    launch_copy(st, n, x, L->e);
+    drd_solve_info li{};
+    RCHK(run_fgmres(c, op, DRD_PS_VELO_LEARNED, n, b, x, m, allow, tol, &li));
+    L->last_its = li.iter;
+    if (li.info == 0) {
+      *info = li;
+      return 0;
+    }
+    // regression: the plain method finishes from the current iterate (from
+    // the start if the learned iterate is not finite)
+    L->state = DRD_LEARN_FALLBACK;
+    ++L->fallbacks;
+    if (!std::isfinite(li.resid)) launch_copy(st, n, L->e, x);
+    drd_solve_info pi{};
+    RCHK(plain(x, std::max(1, maxit - li.iter), &pi));
+    pi.iter += li.iter;
+    *info = pi;
+    return 0;
+  }
+  return plain(x, maxit, info);
+}
+
+static int learn_ready(drd_ctx* c) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  if (!c->learn) return set_err(DRD_E_STATE, "drd_learn_init has not been called");
+  return 0;
+}
+
+extern "C" int drd_learn_init(drd_ctx* c, int32_t capacity, int32_t crop_h, int32_t crop_w, uint64_t seed) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  const Grid& g = c->g;
+  if (capacity < 1) return set_err(DRD_E_ARG, "capacity must be positive");
+  if (g.nx % 4 || g.ny % 4) return set_err(DRD_E_UNSUPPORTED, "the network needs nx, ny multiples of 4");
+  if (crop_h < 4 || crop_w < 4 || crop_h % 4 || crop_w % 4 || crop_h > g.ny || crop_w > g.nx)
+    return set_err(DRD_E_ARG, "crop must be multiples of 4 within ny x nx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  learn_destroy(c->learn);
+  c->learn = nullptr;
+  Learn* L = new Learn();
+  int err = 0;
+  L->net = unet_create(g.ny, g.nx, seed, &err);
+  if (L->net) L->tr = unet_create(crop_h, crop_w, seed, &err);
+  if (!L->tr) {
+    learn_destroy(L);
+    return set_err(err ? err : DRD_E_HIP, "network allocation");
+  }
+  L->ch = crop_h;
+  L->cw = crop_w;
+  L->cap = capacity;
+  L->npx = (int64_t)g.nx * g.ny;
+  L->rng = seed ^ 0x6A09E667F3BCC909ull;
+  const int64_t nuv = g.nuv(), cpx = (int64_t)crop_h * crop_w;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (err) return;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      err = DRD_E_HIP;
+      *p = nullptr;
+      return;
+    }
+    L->allocs.push_back(*p);
+  };
+  alloc((void**)&L->feat, (size_t)capacity * L->npx * kUnetIn * 2);
+  alloc((void**)&L->targ, (size_t)capacity * L->npx * kUnetOut * 4);
+  alloc((void**)&L->img, (size_t)L->npx * kUnetIn * 2);
+  alloc((void**)&L->out, (size_t)L->npx * kUnetOut * 4);
+  alloc((void**)&L->cfeat, (size_t)cpx * kUnetIn * 2);
+  alloc((void**)&L->ctarg, (size_t)cpx * kUnetOut * 4);
+  alloc((void**)&L->ss, 4 * sizeof(double));
+  alloc((void**)&L->ss_part, 3 * 256 * sizeof(double));
+  alloc((void**)&L->r, nuv * sizeof(double));
+  alloc((void**)&L->d, nuv * sizeof(double));
+  alloc((void**)&L->z, nuv * sizeof(double));
+  alloc((void**)&L->e, nuv * sizeof(double));
+  alloc((void**)&L->w, nuv * sizeof(double));
+  alloc((void**)&L->dac, 2 * nuv * sizeof(double));
+  alloc((void**)&L->part, (2 * (size_t)tiles_of(nuv) + 64) * sizeof(double));
+  alloc((void**)&L->dots, 4 * sizeof(double));
+  alloc((void**)&L->snaps, (size_t)kSnapMax * nuv * sizeof(double));
+  if (err) {
+    learn_destroy(L);
+    return set_err(DRD_E_HIP, "learned-correction buffers do not fit");
+  }
+  c->learn = L;
+  return 0;
+}
+
+extern "C" int drd_learn_collect(drd_ctx* c, const double* b, double tol, double* x, int32_t* pairs_added,
+                                 int32_t* iters) {
+  RCHK(learn_ready(c));
+  if (!b) return set_err(DRD_E_ARG, "null b");
+  const int64_t n = c->g.nuv();
+  RCHK(up(c, c->rhs, b, n));
+  HIPCHK(hipMemsetAsync(c->xv, 0, n * sizeof(double), c->stream));
+  int added = 0;
+  drd_solve_info info{};
+  RCHK(learn_recorded_solve(c, c->rhs, c->xv, std::max(c->p.maxit_velo2, 1), tol, &info, &added));
+  RCHK(down(c, x, c->xv, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (pairs_added) *pairs_added = added;
+  if (iters) *iters = info.iter;
+  if (info.info != 0) return set_err(DRD_E_STATE, "recorded solve did not converge");
+  return 0;
+}
+
+extern "C" int drd_learn_record(drd_ctx* c, int32_t on) {
+  RCHK(learn_ready(c));
+  c->learn->record = on != 0;
+  return 0;
+}
+
+extern "C" int drd_learn_train(drd_ctx* c, int32_t iters, int32_t batch, double lr) {
+  RCHK(learn_ready(c));
+  Learn* L = c->learn;
+  if (L->count == 0) return set_err(DRD_E_STATE, "no pairs collected");
+  if (iters < 1 || batch < 1 || !(lr > 0.0)) return set_err(DRD_E_ARG, "iters, batch, lr must be positive");
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamSynchronize(st));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int nx = c->g.nx, ny = c->g.ny;
+  double lfirst = 0.0, llast = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    RCHK(unet_zero_grad(st, L->tr));
+    for (int bi = 0; bi < batch; ++bi) {
+      const size_t slot = (size_t)(splitmix(L->rng) % (uint64_t)L->count);
+      const int y0 = (int)(splitmix(L->rng) % (uint64_t)(ny - L->ch + 1));
+      const int x0 = (int)(splitmix(L->rng) % (uint64_t)(nx - L->cw + 1));
+      launch_crop(st, nx, kUnetIn, 2, L->feat + slot * L->npx * kUnetIn, y0, x0, L->ch, L->cw, L->cfeat);
+      launch_crop(st, nx, kUnetOut, 4, L->targ + slot * L->npx * kUnetOut, y0, x0, L->ch, L->cw, L->ctarg);
+      RCHK(unet_forward(st, L->tr, L->cfeat, nullptr));
+      RCHK(unet_backward(st, L->tr, L->cfeat, L->ctarg));
+    }
+    const int k = std::max(1, std::min(10, iters / 4));
+    if (it < k || it >= iters - k) {
+      double loss = 0.0;
+      RCHK(unet_read_loss(st, L->tr, &loss));
+      if (it < k) lfirst += loss / batch / k;
+      if (it >= iters - k) llast += loss / batch / k;
+    }
+    const double cosf = 0.5 * (1.0 + std::cos(M_PI * it / iters));
+    RCHK(unet_adam(st, L->tr, (float)(lr * (0.1 + 0.9 * cosf)), 0.9f, 0.999f, 1e-8f));
+  }
+  RCHK(unet_copy_params(st, L->net, L->tr));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  L->loss_first = lfirst;
+  L->loss_last = llast;
+  L->train_iters += iters;
+  L->train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+extern "C" int drd_learn_enable(drd_ctx* c, int32_t on) {
+  RCHK(learn_ready(c));
+  if (on && c->learn->train_iters == 0) return set_err(DRD_E_STATE, "network not trained");
+  c->learn->state = on ? DRD_LEARN_JUDGE : DRD_LEARN_OFF;
+  return 0;
+}
+
+extern "C" int drd_learn_get_info(drd_ctx* c, drd_learn_info* out) {
+  RCHK(learn_ready(c));
+  if (!out) return set_err(DRD_E_ARG, "null out");
+  const Learn* L = c->learn;
+  drd_learn_info o{};
+  o.state = L->state;
+  o.pairs = L->count;
+  o.capacity = L->cap;
+  o.train_iters = L->train_iters;
+  o.ref_iters = L->ref_its;
+  o.judged_iters = L->judged_its;
+  o.last_iters = L->last_its;
+  o.fallbacks = L->fallbacks;
+  o.loss_first = L->loss_first;
+  o.loss_last = L->loss_last;
+  o.train_ms = L->train_ms;
+  o.judged_ms = L->judged_ms[0];
+  o.plain_ms = L->judged_ms[1];
+  *out = o;
+  return 0;
+}
+
+extern "C" int drd_learn_get_params(drd_ctx* c, float* params, int32_t n) {
+  RCHK(learn_ready(c));
+  if (!params) return set_err(DRD_E_ARG, "null params");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (unet_get_params(c->learn->tr, params, n)) return set_err(DRD_E_ARG, "parameter count mismatch");
+  return 0;
+}
+
+extern "C" int drd_learn_set_params(drd_ctx* c, const float* params, int32_t n) {
+  RCHK(learn_ready(c));
+  if (!params) return set_err(DRD_E_ARG, "null params");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (unet_set_params(c->learn->tr, params, n) || unet_set_params(c->learn->net, params, n))
+    return set_err(DRD_E_ARG, "parameter count mismatch");
+  c->learn->train_iters = std::max(c->learn->train_iters, 1);  // loaded parameters count as trained
+  return 0;
+}
+
+extern "C" int drd_unet_param_count(void) { return unet_param_count(); }
+
+extern "C" int drd_unet_run(int32_t H, int32_t W, const float* params, const uint16_t* in, const float* target,
+                            float* out, float* grads, double* loss, float lr, float* params_after) {
+  if (!params || !in || !out) return set_err(DRD_E_ARG, "null argument");
+  int err = 0;
+  UNet* u = unet_create(H, W, 1, &err);
+  if (!u) return set_err(err ? err : DRD_E_HIP, "network sizes must be multiples of 4");
+  const size_t npx = (size_t)H * W;
+  void *din = nullptr, *dout = nullptr, *dtg = nullptr;
+  int rc = unet_set_params(u, params, unet_param_count());
+  auto ok = [](hipError_t e) { return e == hipSuccess ? 0 : DRD_E_HIP; };
+  if (!rc) rc = ok(hipMalloc(&din, npx * kUnetIn * 2));
+  if (!rc) rc = ok(hipMalloc(&dout, npx * kUnetOut * 4));
+  if (!rc && target) rc = ok(hipMalloc(&dtg, npx * kUnetOut * 4));
+  if (!rc) rc = ok(hipMemcpy(din, in, npx * kUnetIn * 2, hipMemcpyHostToDevice));
+  if (!rc && target) rc = ok(hipMemcpy(dtg, target, npx * kUnetOut * 4, hipMemcpyHostToDevice));
+  if (!rc) rc = unet_forward(0, u, din, (float*)dout);
+  if (!rc) rc = ok(hipMemcpy(out, dout, npx * kUnetOut * 4, hipMemcpyDeviceToHost));
+  if (!rc && target) {
+    // the backward reads the network's own output buffer: run the forward into it
+    rc = unet_forward(0, u, din, nullptr);
+    if (!rc) rc = unet_zero_grad(0, u);
+    if (!rc) rc = unet_backward(0, u, din, (const float*)dtg);
+    double l = 0.0;
+    if (!rc) rc = unet_read_loss(0, u, &l);
+    if (!rc && loss) *loss = l;
+    if (!rc && grads) rc = unet_get_grads(u, grads, unet_param_count());
+    if (!rc && params_after) {
+      rc = unet_adam(0, u, lr, 0.9f, 0.999f, 1e-8f);
+      if (!rc) rc = ok(hipDeviceSynchronize());
+      if (!rc) rc = unet_get_params(u, params_after, unet_param_count());
+    }
+  }
+  if (!rc) rc = ok(hipDeviceSynchronize());
+  hipFree(din);
+  hipFree(dout);
+  hipFree(dtg);
+  unet_destroy(u);
+  return rc ? set_err(rc, "network run failed") : 0;
+}
+
 // ---- velocity --------------------------------------------------------------
 static int velo_prec(const drd_ctx* c, int bdf) {
   switch (c->p.velo_prec) {
@@ -832,6 +1283,8 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
     case DRD_PREC_DCT64:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT64;
+    case DRD_PREC_LEARNED:  // plain FP32 DCT unless the correction is judged in (velo_solve_dev)
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
     default:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
   }
@@ -888,7 +1341,14 @@ static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double*
   launch_velo_pack(c->stream, c->g, c->f[DRD_F_U], c->f[DRD_F_V], c->xv);
   const int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
   const int maxit = bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2;
-  if (c->p.krylov == DRD_KRYLOV_FGMRES && bdf == 2)
+  if (bdf == 2 && c->p.velo_prec == DRD_PREC_LEARNED) {
+    if (c->learn && c->learn->record) {
+      int added = 0;
+      RCHK(learn_recorded_solve(c, c->rhs, c->xv, maxit, c->p.tol, info, &added));
+    } else {
+      RCHK(velo_solve_learned(c, c->rhs, c->xv, maxit, c->p.tol, info));
+    }
+  } else if (c->p.krylov == DRD_KRYLOV_FGMRES && bdf == 2)
     RCHK(run_fgmres(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv, kFgmresRestart, maxit, c->p.tol, info));
   else
     RCHK(run_bicgstab(c, op, velo_prec(c, bdf), c->g.nuv(), c->rhs, c->xv, maxit, c->p.tol, info, want_true));

@@ -517,10 +517,16 @@ int unet_forward(hipStream_t st, UNet* u, const void* in, float* out) {
 }
 
 int unet_zero_grad(hipStream_t st, UNet* u) {
+  if (hipMemsetAsync(u->loss, 0, 8, st) != hipSuccess) return DRD_E_HIP;
   return hipMemsetAsync(u->g, 0, (size_t)u->np * 4, st) == hipSuccess ? 0 : DRD_E_HIP;
 }
 
-int unet_backward(hipStream_t st, UNet* u, const void* in, const float* target, double* loss) {
+int unet_read_loss(hipStream_t st, UNet* u, double* loss) {
+  if (hipMemcpyAsync(loss, u->loss, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return DRD_E_HIP;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : DRD_E_HIP;
+}
+
+int unet_backward(hipStream_t st, UNet* u, const void* in, const float* target) {
   const bf16* x = reinterpret_cast<const bf16*>(in);
   const int npix = u->H * u->W;
   for (int l = 0; l < kUnetConvs; ++l) {
@@ -528,7 +534,6 @@ int unet_backward(hipStream_t st, UNet* u, const void* in, const float* target, 
     const size_t n = (size_t)hl(u->H, s.level) * hl(u->W, s.level) * s.cout;
     if (hipMemsetAsync(u->gact[l], 0, n * 4, st) != hipSuccess) return DRD_E_HIP;
   }
-  if (hipMemsetAsync(u->loss, 0, 8, st) != hipSuccess) return DRD_E_HIP;
   hipLaunchKernelGGL(k_out_bwd, dim3(nblk256(npix)), dim3(256), 0, st, npix, (const bf16*)u->act[9],
                      (const float*)(u->p + u->offWo), (const float*)u->out, target, u->gact[9], u->g + u->offWo,
                      u->g + u->offBo, u->loss);
@@ -579,10 +584,6 @@ int unet_backward(hipStream_t st, UNet* u, const void* in, const float* target, 
       default: route(0, Hl, Wl, cin, u->gact[l - 1], 0); break;
     }
   }
-  if (loss) {
-    if (hipMemcpyAsync(loss, u->loss, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return DRD_E_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) return DRD_E_HIP;
-  }
   return hipGetLastError() == hipSuccess ? 0 : DRD_E_HIP;
 }
 
@@ -597,18 +598,46 @@ int unet_adam(hipStream_t st, UNet* u, float lr, float b1, float b2, float eps) 
 
 // ---- velocity vectors <-> network images ---------------------------------------
 // Pixel (k, i) of the ny x nx image carries the u face i of row k (i < nx)
-// and the v face k of column i (k >= 1; 0 on k = 0).
-__global__ __launch_bounds__(256) void k_sumsq(int64_t n, const double* __restrict__ x, double* __restrict__ out) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += x[i] * x[i];
-  red[threadIdx.x] = s;
+// and the v face k of column i (k >= 1; 0 on k = 0).  The u face i = nx is
+// left to the DCT solve.
+constexpr int kSsBlocks = 256;
+
+// part[v * kSsBlocks + b] = sum over block b's grid-stride share of x_v^2
+__global__ __launch_bounds__(256) void k_sumsq3(int64_t n, const double* __restrict__ x0,
+                                                const double* __restrict__ x1, const double* __restrict__ x2,
+                                                double* __restrict__ part) {
+  __shared__ double red[3][256];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kSsBlocks * 256) {
+    s0 += x0[i] * x0[i];
+    s1 += x1[i] * x1[i];
+    s2 += x2[i] * x2[i];
+  }
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if ((int)threadIdx.x < o)
+      for (int v = 0; v < 3; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+  if (threadIdx.x < 3) part[threadIdx.x * kSsBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// ss[v] = sum of part[v * kSsBlocks ...] (fixed order: deterministic)
+__global__ __launch_bounds__(256) void k_sumsq3_fin(const double* __restrict__ part, double* __restrict__ ss) {
+  __shared__ double red[256];
+  for (int v = 0; v < 3; ++v) {
+    red[threadIdx.x] = part[v * kSsBlocks + threadIdx.x];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) ss[v] = red[0];
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ double rms_scale(double ss, int64_t n) {
@@ -616,9 +645,11 @@ __device__ __forceinline__ double rms_scale(double ss, int64_t n) {
   return s > 0.0 ? s : 1.0;
 }
 
-// img = [r_u/s_r, r_v/s_r, z_u/s_z, z_v/s_z, log eta, chi, 0, 0] (bf16 NHWC)
+// img = [r_u/s_r, r_v/s_r, d_u/s_d, d_v/s_d, z_u/s_z, z_v/s_z, log eta, chi]
+// (bf16 NHWC) with ss = (|r|^2, |d|^2, |z|^2)
 __global__ __launch_bounds__(256) void k_pack_features(Grid g, const double* __restrict__ r,
-                                                       const double* __restrict__ z, const double* __restrict__ eta,
+                                                       const double* __restrict__ d, const double* __restrict__ z,
+                                                       const double* __restrict__ eta,
                                                        const double* __restrict__ chi, const double* __restrict__ ss,
                                                        bf16* __restrict__ img) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -626,64 +657,80 @@ __global__ __launch_bounds__(256) void k_pack_features(Grid g, const double* __r
   if (p >= (int64_t)nx * ny) return;
   const int k = (int)(p / nx), i = (int)(p - (int64_t)k * nx);
   const int64_t n = g.nuv();
-  const double sr = rms_scale(ss[0], n), sz = rms_scale(ss[1], n);
+  const double ir = 1.0 / rms_scale(ss[0], n), id = 1.0 / rms_scale(ss[1], n), iz = 1.0 / rms_scale(ss[2], n);
   const int64_t qu = i + (int64_t)(nx + 1) * k, qv = g.mn() + i + (int64_t)nx * (k - 1);
+  const bool hv = k >= 1;
   float v[8];
-  v[0] = (float)(r[qu] / sr);
-  v[1] = k >= 1 ? (float)(r[qv] / sr) : 0.f;
-  v[2] = (float)(z[qu] / sz);
-  v[3] = k >= 1 ? (float)(z[qv] / sz) : 0.f;
-  v[4] = (float)log(eta[g.ci(i, k)]);
-  v[5] = (float)chi[g.ci(i, k)];
-  v[6] = v[7] = 0.f;
+  v[0] = (float)(r[qu] * ir);
+  v[1] = hv ? (float)(r[qv] * ir) : 0.f;
+  v[2] = (float)(d[qu] * id);
+  v[3] = hv ? (float)(d[qv] * id) : 0.f;
+  v[4] = (float)(z[qu] * iz);
+  v[5] = hv ? (float)(z[qv] * iz) : 0.f;
+  v[6] = (float)log(eta[g.ci(i, k)]);
+  v[7] = (float)chi[g.ci(i, k)];
   bf16x8 o;
 #pragma unroll
   for (int c = 0; c < 8; ++c) o[c] = (bf16)v[c];
   *reinterpret_cast<uint4*>(img + p * 8) = __builtin_bit_cast(uint4, o);
 }
 
-// z += s_z * out (the learned correction)
-__global__ __launch_bounds__(256) void k_apply_correction(Grid g, const float* __restrict__ out,
-                                                          const double* __restrict__ ss, double* __restrict__ z) {
+// w = s_d * out as a velocity Krylov vector (the u faces i = nx get 0)
+__global__ __launch_bounds__(256) void k_unpack_correction(Grid g, const float* __restrict__ out,
+                                                           const double* __restrict__ ss, double* __restrict__ w) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   if (p >= (int64_t)nx * ny) return;
   const int k = (int)(p / nx), i = (int)(p - (int64_t)k * nx);
-  const double sz = rms_scale(ss[1], g.nuv());
-  z[i + (int64_t)(nx + 1) * k] += sz * (double)out[p * 2];
-  if (k >= 1) z[g.mn() + i + (int64_t)nx * (k - 1)] += sz * (double)out[p * 2 + 1];
+  const double sd = rms_scale(ss[1], g.nuv());
+  const int64_t qu = i + (int64_t)(nx + 1) * k;
+  w[qu] = sd * (double)out[p * 2];
+  if (i == nx - 1) w[qu + 1] = 0.0;
+  if (k >= 1) w[g.mn() + i + (int64_t)nx * (k - 1)] = sd * (double)out[p * 2 + 1];
 }
 
-// training target (e - z) / s_z on the image
-__global__ __launch_bounds__(256) void k_make_target(Grid g, const double* __restrict__ e,
-                                                     const double* __restrict__ z, const double* __restrict__ ss,
-                                                     float* __restrict__ t) {
+// training target v / s_d on the image
+__global__ __launch_bounds__(256) void k_make_target(Grid g, const double* __restrict__ v,
+                                                     const double* __restrict__ ss, float* __restrict__ t) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   if (p >= (int64_t)nx * ny) return;
   const int k = (int)(p / nx), i = (int)(p - (int64_t)k * nx);
-  const double sz = rms_scale(ss[1], g.nuv());
+  const double id = 1.0 / rms_scale(ss[1], g.nuv());
   const int64_t qu = i + (int64_t)(nx + 1) * k, qv = g.mn() + i + (int64_t)nx * (k - 1);
-  t[p * 2] = (float)((e[qu] - z[qu]) / sz);
-  t[p * 2 + 1] = k >= 1 ? (float)((e[qv] - z[qv]) / sz) : 0.f;
+  t[p * 2] = (float)(v[qu] * id);
+  t[p * 2 + 1] = k >= 1 ? (float)(v[qv] * id) : 0.f;
 }
 
-void launch_norms2(hipStream_t st, int64_t n, const double* r, const double* z, double* ss) {
-  (void)hipMemsetAsync(ss, 0, 2 * sizeof(double), st);
-  hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, st, n, r, ss);
-  hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, st, n, z, ss + 1);
+void launch_norms3(hipStream_t st, int64_t n, const double* r, const double* d, const double* z, double* part,
+                   double* ss) {
+  hipLaunchKernelGGL(k_sumsq3, dim3(kSsBlocks), dim3(256), 0, st, n, r, d, z, part);
+  hipLaunchKernelGGL(k_sumsq3_fin, dim3(1), dim3(256), 0, st, (const double*)part, ss);
 }
-void launch_pack_features(hipStream_t st, const Grid& g, const double* r, const double* z, const double* eta,
-                          const double* chi, const double* ss, void* img) {
-  hipLaunchKernelGGL(k_pack_features, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, r, z, eta, chi, ss,
+void launch_pack_features(hipStream_t st, const Grid& g, const double* r, const double* d, const double* z,
+                          const double* eta, const double* chi, const double* ss, void* img) {
+  hipLaunchKernelGGL(k_pack_features, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, r, d, z, eta, chi, ss,
                      reinterpret_cast<bf16*>(img));
 }
-void launch_apply_correction(hipStream_t st, const Grid& g, const float* out, const double* ss, double* z) {
-  hipLaunchKernelGGL(k_apply_correction, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, out, ss, z);
+// z += alpha c, alpha = dots[0] / dots[1] (0 when dots[1] is not positive)
+__global__ __launch_bounds__(256) void k_axpy_ratio(int64_t n, const double* __restrict__ dots,
+                                                    const double* __restrict__ c, double* __restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double den = dots[1];
+  const double a = den > 0.0 ? dots[0] / den : 0.0;
+  z[i] += a * c[i];
 }
-void launch_make_target(hipStream_t st, const Grid& g, const double* e, const double* z, const double* ss,
-                        float* t) {
-  hipLaunchKernelGGL(k_make_target, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, e, z, ss, t);
+
+void launch_axpy_ratio(hipStream_t st, int64_t n, const double* dots, const double* c, double* z) {
+  hipLaunchKernelGGL(k_axpy_ratio, dim3(nblk256(n)), dim3(256), 0, st, n, dots, c, z);
+}
+
+void launch_unpack_correction(hipStream_t st, const Grid& g, const float* out, const double* ss, double* w) {
+  hipLaunchKernelGGL(k_unpack_correction, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, out, ss, w);
+}
+void launch_make_target(hipStream_t st, const Grid& g, const double* v, const double* ss, float* t) {
+  hipLaunchKernelGGL(k_make_target, dim3(nblk256(g.ncell())), dim3(256), 0, st, g, v, ss, t);
 }
 
 // crop (y0, x0, h x w) of an H x W image with c channels of esz bytes each

@@ -1,10 +1,15 @@
 // unet.hpp -- the learned correction of the velocity preconditioner (no
 // reference counterpart): a 3-level U-Net with 16/32/32 channels applied
 // after the fast-Poisson (DCT) preconditioner,
-//   z' = z + s_z * N(r / s_r, z / s_z, log eta, chi),
-// on the ny x nx cell image (u faces i < nx and v faces k >= 1 mapped to
-// cell (i, k)).  BF16 weights and activations (NHWC), FP32 accumulation on
-// the MFMA units, FP32 master weights and Adam moments.
+//   z' = z + alpha s_d N(r / s_r, d / s_d, z / s_z, log eta, chi),
+//   z = M r,  d = r - A z,
+// with M the fast-Poisson (DCT) solve; N approximates the error the DCT
+// solve leaves, (A^-1 r - z) / s_d.  The correction c = s_d N enters with
+// the step alpha = (d, A c) / (A c, A c) that minimises |r - A (z + alpha c)|,
+// so it never enlarges the residual of the plain DCT solve.  Images are ny x nx
+// (u faces i < nx and v faces k >= 1 mapped to cell (i, k)).  BF16 weights
+// and activations (NHWC), FP32 accumulation on the MFMA units, FP32 master
+// weights and Adam moments.
 //
 // Layers (3x3 convolutions, zero padding, ReLU; "||" is channel concat,
 // "up" nearest-neighbour upsampling by 2, "pool" 2x2 averaging):
@@ -24,7 +29,7 @@
 
 namespace drd {
 
-constexpr int kUnetIn = 8;   // r_u, r_v, z_u, z_v, log eta, chi, 0, 0
+constexpr int kUnetIn = 8;   // r_u, r_v, d_u, d_v, z_u, z_v, log eta, chi
 constexpr int kUnetOut = 2;  // corrections of u and v
 constexpr int kUnetConvs = 10;
 
@@ -63,11 +68,15 @@ void unet_destroy(UNet* u);
 int unet_set_params(UNet* u, const float* host, int n);
 int unet_get_params(UNet* u, float* host, int n);
 // forward on a device NHWC bf16 input (H x W x 8); out: device FP32 H x W x 2
+// (null: the network's own output buffer, which unet_backward reads)
 int unet_forward(hipStream_t st, UNet* u, const void* in_bf16, float* out);
-// backward of 0.5 * sum((out - target)^2 * mask) / count after unet_forward on the same input;
-// grads accumulate into the FP32 gradient vector (zeroed by unet_zero_grad); returns the loss
+// Backward of loss = 0.5 * sum((out - target)^2) / (2 H W) after unet_forward
+// on the same input.  Gradients and the loss accumulate from unet_zero_grad on
+// (a batch is several forward/backward pairs); unet_read_loss waits for the
+// stream and returns the accumulated loss.
 int unet_zero_grad(hipStream_t st, UNet* u);
-int unet_backward(hipStream_t st, UNet* u, const void* in_bf16, const float* target, double* loss);
+int unet_backward(hipStream_t st, UNet* u, const void* in_bf16, const float* target);
+int unet_read_loss(hipStream_t st, UNet* u, double* loss);
 int unet_get_grads(UNet* u, float* host, int n);
 int unet_adam(hipStream_t st, UNet* u, float lr, float beta1, float beta2, float eps);
 // per-layer MFMA FLOPs of one forward pass (for the roofline)
@@ -79,14 +88,18 @@ void launch_crop(hipStream_t st, int W, int c, int esz, const void* src, int y0,
 
 // velocity vectors <-> ny x nx network images (kernels_unet.hip)
 struct Grid;
-// ss[0] = sum r^2, ss[1] = sum z^2 (device)
-void launch_norms2(hipStream_t st, int64_t n, const double* r, const double* z, double* ss);
-// 8-channel bf16 input image of (r, z) with the coefficient fields
-void launch_pack_features(hipStream_t st, const Grid& g, const double* r, const double* z, const double* eta,
-                          const double* chi, const double* ss, void* img);
-// z += s_z * out
-void launch_apply_correction(hipStream_t st, const Grid& g, const float* out, const double* ss, double* z);
-// t = (e - z) / s_z on the image (training target)
-void launch_make_target(hipStream_t st, const Grid& g, const double* e, const double* z, const double* ss, float* t);
+// ss[0..2] = |r|^2, |d|^2, |z|^2 (device; part: 768 doubles of scratch)
+void launch_norms3(hipStream_t st, int64_t n, const double* r, const double* d, const double* z, double* part,
+                   double* ss);
+// 8-channel bf16 input image of r (vector the preconditioner is applied to),
+// d = r - A z, z = DCT solve, and the coefficient fields log eta and chi
+void launch_pack_features(hipStream_t st, const Grid& g, const double* r, const double* d, const double* z,
+                          const double* eta, const double* chi, const double* ss, void* img);
+// w = s_d * out as a velocity Krylov vector, s_d = rms of d (0 on the u faces i = nx)
+void launch_unpack_correction(hipStream_t st, const Grid& g, const float* out, const double* ss, double* w);
+// z += (dots[0] / dots[1]) c (device scalars; no step when dots[1] <= 0)
+void launch_axpy_ratio(hipStream_t st, int64_t n, const double* dots, const double* c, double* z);
+// t = v / s_d on the image (training target, v = A^-1 r - z)
+void launch_make_target(hipStream_t st, const Grid& g, const double* v, const double* ss, float* t);
 
 }  // namespace drd

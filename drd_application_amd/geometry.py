@@ -200,6 +200,61 @@ def tp_case(nx: int, ny: int | None = None, *, lambda_eta: float = 1.0, steps: i
                   "reference TP case")
 
 
+def smooth_field(shape, x, z, rng: SplitMix64, modes: int = 16, kmax: float = 6.0) -> np.ndarray:
+    """Sum of `modes` seeded cosines a cos(2 pi (kx x + kz z) + theta), wave
+    numbers uniform in [-kmax, kmax] (per unit length), amplitudes ~ 1/|k|."""
+    g = np.zeros(shape)
+    for _ in range(modes):
+        kx, kz = rng.uniform(-kmax, kmax), rng.uniform(-kmax, kmax)
+        th = rng.uniform(0.0, 2.0 * math.pi)
+        a = 1.0 / max(1.0, math.hypot(kx, kz))
+        g += a * np.cos(2.0 * math.pi * (kx * x + kz * z) + th)
+    return g
+
+
+def velocity_system(n: int, seed: int, *, max_contrast: float = 1.0e3, n_discs: tuple = (3, 8),
+                    r_disc: tuple = (0.02, 0.05), penalty: float = 1.0e6, dt: float = 5.0e-5):
+    """One learned-preconditioner test system on an n x n grid of the unit
+    square (the 1024^2 workload at n = 1024):
+
+        (I - dt div(eta grad) + dt chi R) u = f
+
+    in the reference's BDF2 velocity operator (TP/VeloSolve_2_combined.f90:
+    536-676) with rho = 1 and re = 2/3, so the mass term 1.5 re rho is I.
+    eta = exp(log(C) phi~), phi~ in [0, 1] a seeded random phase field
+    (tanh of a smooth random field), contrast C log-uniform in [1, max_contrast];
+    chi = smoothed union of seeded discs (radius r_disc of the height,
+    smoothing 2 cells) with R = penalty; f a smooth random field on the u and v
+    faces.  Returns (Config, f packed as the velocity Krylov vector)."""
+    p = abi.default_params(n, n)
+    p.dx = p.dz = 1.0 / n
+    p.dt = dt
+    p.re = 2.0 / 3.0
+    p.penalty_k = penalty
+    p.tol = 1.0e-10
+    p.maxit_velo2 = 5000
+    rng = SplitMix64(seed)
+    X, Z = _cell_centres(p)
+    contrast = 10.0 ** rng.uniform(0.0, math.log10(max_contrast))
+    g = smooth_field(X.shape, X, Z, rng)
+    phi = 0.5 * (1.0 + np.tanh(g / (0.15 * g.std())))
+    eta = np.exp(math.log(contrast) * phi)
+    nd = int(n_discs[0] + (n_discs[1] - n_discs[0] + 1) * rng.uniform())
+    bdr = disc_mask(p, rng, nd, r_disc[0], r_disc[1], 2 * p.dx)
+    f = derived_fields(p, np.zeros_like(phi), bdr, 1.0)
+    f[abi.F_ETA] = _neumann_ghosts(eta)
+    f[abi.F_RHO] = np.ones_like(eta)
+    f[abi.F_BDR_PH_PENALTY] = bdr.copy()
+    # right-hand side on the faces: u(i, k) at (i dx, (k+0.5) dz), v(i, k) at ((i+0.5) dx, k dz)
+    iu, ku = np.meshgrid(np.arange(n + 1) * p.dx, (np.arange(n) + 0.5) * p.dz)
+    iv, kv = np.meshgrid((np.arange(n) + 0.5) * p.dx, np.arange(1, n) * p.dz)
+    fu = smooth_field(iu.shape, iu, ku, rng)
+    fv = smooth_field(iv.shape, iv, kv, rng)
+    b = np.concatenate([fu.ravel(), fv.ravel()])
+    cfg = Config(f"velo{n}_{seed}", p, f, 1, f"contrast {contrast:.3g}, {nd} discs")
+    return cfg, b
+
+
 def config(name: str) -> Config:
     """Named workloads.
 

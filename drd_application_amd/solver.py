@@ -66,6 +66,18 @@ def lib():
         L.drd_bench_kernel.argtypes = [V, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.drd_kernel_timer.argtypes = [V, C.c_int32, C.c_int32]
         L.drd_kernel_timer_read.argtypes = [V, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        F = C.POINTER(C.c_float)
+        L.drd_learn_init.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+        L.drd_learn_collect.argtypes = [V, P, C.c_double, P, I, I]
+        L.drd_learn_record.argtypes = [V, C.c_int32]
+        L.drd_learn_train.argtypes = [V, C.c_int32, C.c_int32, C.c_double]
+        L.drd_learn_enable.argtypes = [V, C.c_int32]
+        L.drd_learn_get_info.argtypes = [V, C.POINTER(abi.LearnInfo)]
+        L.drd_learn_get_params.argtypes = [V, F, C.c_int32]
+        L.drd_learn_set_params.argtypes = [V, F, C.c_int32]
+        L.drd_unet_param_count.argtypes = []
+        L.drd_unet_run.argtypes = [C.c_int32, C.c_int32, F, C.POINTER(C.c_uint16), F, F, F,
+                                   C.POINTER(C.c_double), C.c_float, F]
         _LIB = L
     return _LIB
 
@@ -78,6 +90,44 @@ def _check(rc: int) -> None:
 
 def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    if a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def unet_param_count() -> int:
+    return int(lib().drd_unet_param_count())
+
+
+def unet_run(params, img_bits, target=None, lr=None):
+    """One forward (and, with a target, backward) pass of the learned
+    correction's network on the GPU (test hook drd_unet_run).
+
+    img_bits: (H, W, 8) uint16 BF16 bit patterns; target: (H, W, 2) float32.
+    Returns (out, grads, loss, params_after) -- grads/loss None without a
+    target, params_after None without lr (one Adam step otherwise)."""
+    H, W, c = img_bits.shape
+    if c != 8:
+        raise ValueError("the network takes 8 input channels")
+    n = unet_param_count()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    if p.size != n:
+        raise ValueError(f"expected {n} parameters")
+    img = np.ascontiguousarray(img_bits, dtype=np.uint16)
+    out = np.zeros((H, W, 2), np.float32)
+    t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
+    grads = None if t is None else np.zeros(n, np.float32)
+    after = None if (t is None or lr is None) else np.zeros(n, np.float32)
+    loss = C.c_double(0.0)
+    _check(lib().drd_unet_run(H, W, _fptr(p), img.ctypes.data_as(C.POINTER(C.c_uint16)), _fptr(t),
+                              _fptr(out), _fptr(grads), C.byref(loss),
+                              0.0 if lr is None else float(lr), _fptr(after)))
+    return out, grads, (loss.value if t is not None else None), after
 
 
 class GpuSolver:
@@ -217,6 +267,43 @@ class GpuSolver:
         """Brackets up to `capacity` launches of kernel `which` inside the
         following solves with HIP events (0 disables)."""
         _check(lib().drd_kernel_timer(self._h, which, capacity))
+
+    # ---- learned correction of the velocity preconditioner ------------------
+    def learn_init(self, capacity, crop_h, crop_w, seed=1):
+        _check(lib().drd_learn_init(self._h, capacity, crop_h, crop_w, seed))
+
+    def learn_collect(self, b, tol=1e-12):
+        """Recorded BiCGSTAB(FP32 DCT) solve of A x = b (BDF2 velocity
+        operator) from 0; returns (x, pairs added, iterations)."""
+        b = _f64(b)
+        x = np.zeros_like(b)
+        added, its = C.c_int32(0), C.c_int32(0)
+        _check(lib().drd_learn_collect(self._h, dp(b), tol, dp(x), C.byref(added), C.byref(its)))
+        return x, added.value, its.value
+
+    def learn_record(self, on):
+        _check(lib().drd_learn_record(self._h, 1 if on else 0))
+
+    def learn_train(self, iters, batch, lr):
+        _check(lib().drd_learn_train(self._h, iters, batch, lr))
+        return self.learn_info()
+
+    def learn_enable(self, on):
+        _check(lib().drd_learn_enable(self._h, 1 if on else 0))
+
+    def learn_info(self):
+        info = abi.LearnInfo()
+        _check(lib().drd_learn_get_info(self._h, C.byref(info)))
+        return info.as_dict()
+
+    def learn_params(self):
+        p = np.zeros(unet_param_count(), np.float32)
+        _check(lib().drd_learn_get_params(self._h, _fptr(p), p.size))
+        return p
+
+    def learn_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().drd_learn_set_params(self._h, _fptr(p), p.size))
 
     def kernel_timer_read(self):
         """(timed launches, summed ms, algorithmic bytes per launch)."""

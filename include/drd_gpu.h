@@ -49,6 +49,7 @@ extern "C" {
                                 with diagonal scaling (no reference
                                 counterpart; power-of-two nx, ny)         */
 #define DRD_PREC_DCT64 4     /* the same in FP64                          */
+/* DRD_PREC_LEARNED (5): FP32 DCT + learned correction, defined below      */
 
 /* ---- preconditioner choices for the Cahn-Hilliard phi/mu solve -------- */
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
@@ -206,6 +207,7 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 #define DRD_PS_VELO_DCT64 6   /* FP64 DCT/DST velocity preconditioner (addition)    */
 #define DRD_PS_VELO_ESOLP 7   /* PSolve_uv_CG PD/VeloSolve_combined.f90:697-807
                                  (ESOLP_x TP/ESOLP.f90 on both blocks)            */
+/* DRD_PS_VELO_LEARNED (8): FP32 DCT + learned correction, defined below   */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 
@@ -251,6 +253,85 @@ int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  * duration and the kernel's algorithmic bytes per launch.                 */
 int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);
 int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
+
+/* ---- learned correction of the velocity preconditioner ------------------
+ * No reference counterpart (an addition beside the drop-in).  DRD_PS_VELO_LEARNED
+ * is the FP32 DCT/DST solve z = M r followed by the error correction of a
+ * 3-level U-Net (16/32/32 channels, BF16 MFMA convolutions) on the ny x nx
+ * face image:
+ *   z' = z + alpha s_d N(r/s_r, d/s_d, z/s_z, log eta, chi),  d = r - A z,
+ * s_* the RMS of each vector, N trained towards (A^-1 r - z)/s_d, and alpha
+ * the step that minimises |r - A z'| (so |r - A z'| <= |d|).  The correction
+ * is not linear in r, so only the flexible Krylov accepts it (drd_bicgstab
+ * rejects it); drd_fgmres applies it at the first direction of each restart
+ * cycle and the plain FP32 DCT solve for the cycle's other directions.
+ * velo_prec = DRD_PREC_LEARNED makes drd_step solve the BDF2 velocity system
+ * with it, judged once: the first solve after drd_learn_enable runs FGMRES
+ * with the correction, FGMRES with the plain preconditioner and the plain
+ * method (BiCGSTAB with the FP32 DCT preconditioner) from the same start, and
+ * keeps the correction only if it needed no more iterations than plain FGMRES
+ * and no more time than the plain method.  Otherwise, and whenever a later
+ * learned solve needs more than 1.25x the judged plain FGMRES iterations, the
+ * context falls back to the plain method (DRD_LEARN_FALLBACK).              */
+#define DRD_PREC_LEARNED 5
+#define DRD_PS_VELO_LEARNED 8
+
+#define DRD_LEARN_OFF 0      /* not enabled: the plain method               */
+#define DRD_LEARN_JUDGE 1    /* the next velocity solve is judged           */
+#define DRD_LEARN_ON 2       /* correction in use                           */
+#define DRD_LEARN_FALLBACK 3 /* regressed: the plain method                 */
+
+typedef struct drd_learn_info {
+  int32_t state;      /* DRD_LEARN_*                                        */
+  int32_t pairs;      /* residual/error pairs held                          */
+  int32_t capacity;   /* pairs the store holds (oldest replaced first)       */
+  int32_t train_iters;/* Adam steps taken                                   */
+  int32_t ref_iters;  /* plain-DCT FGMRES iterations of the judged solve    */
+  int32_t judged_iters; /* learned FGMRES iterations of the judged solve    */
+  int32_t last_iters; /* iterations of the last learned velocity solve      */
+  int32_t fallbacks;  /* regressions seen                                   */
+  double loss_first;  /* mean crop loss of the first training batch         */
+  double loss_last;   /* mean crop loss of the last training batch          */
+  double train_ms;    /* wall time of the last drd_learn_train call         */
+  double judged_ms;   /* wall time of the judged learned FGMRES solve       */
+  double plain_ms;    /* wall time of the judged plain-method solve         */
+} drd_learn_info;
+
+/* Allocates the pair store (capacity pairs of ny x nx images: 24 bytes per
+ * cell each) and two networks sharing one parameter vector: an ny x nx one
+ * for the preconditioner and a crop_h x crop_w one for training (multiples
+ * of 4, at most ny x nx).  He-normal initial weights from seed.            */
+int drd_learn_init(drd_ctx* ctx, int32_t capacity, int32_t crop_h, int32_t crop_w, uint64_t seed);
+/* Solver-emitted pairs: solves A x = b (the BDF2 velocity operator of the
+ * context's fields) from x = 0 by BiCGSTAB with the FP32 DCT preconditioner
+ * to `tol`, keeping iterates x_k at k = 1, 2, 3, 5, 8, 12, ... (up to 0.8x
+ * the final count).  Each gives r = b - A x_k, z = M r, d = r - A z,
+ * e = x - x_k and the pair (features, (e - z)/s_d).  x (host, may be null)
+ * receives the solution.                                                   */
+int drd_learn_collect(drd_ctx* ctx, const double* b, double tol, double* x, int32_t* pairs_added,
+                      int32_t* iters);
+/* While on, drd_step's BDF2 velocity solves run as recorded BiCGSTAB with the
+ * FP32 DCT preconditioner and add their pairs.                              */
+int drd_learn_record(drd_ctx* ctx, int32_t on);
+/* iters Adam steps (beta 0.9/0.999) over batches of `batch` random crops of
+ * the stored pairs, learning rate lr decaying to lr/10 (cosine); the
+ * preconditioner's network then takes the new parameters.                  */
+int drd_learn_train(drd_ctx* ctx, int32_t iters, int32_t batch, double lr);
+/* on: judge the next velocity solve (DRD_LEARN_JUDGE); off: DRD_LEARN_OFF   */
+int drd_learn_enable(drd_ctx* ctx, int32_t on);
+int drd_learn_get_info(drd_ctx* ctx, drd_learn_info* out);
+/* copy the network parameters out (get) or in (set); n = drd_unet_param_count() */
+int drd_learn_get_params(drd_ctx* ctx, float* params, int32_t n);
+int drd_learn_set_params(drd_ctx* ctx, const float* params, int32_t n);
+
+/* Stand-alone forward/backward of the network on host data (test hook):
+ * in: H*W*8 BF16 bit patterns (NHWC), target: H*W*2 FP32 (may be null: no
+ * backward), out: H*W*2, grads: drd_unet_param_count() floats of the loss
+ * 0.5*sum((out-target)^2)/(2HW), *loss.  params_after (may be null) gets the
+ * parameters after one Adam step (lr, beta 0.9/0.999, eps 1e-8).             */
+int drd_unet_param_count(void);
+int drd_unet_run(int32_t H, int32_t W, const float* params, const uint16_t* in, const float* target,
+                 float* out, float* grads, double* loss, float lr, float* params_after);
 
 #ifdef __cplusplus
 }

@@ -39,12 +39,14 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         '#include <stdio.h>\n#include <stddef.h>\n#include "drd_gpu.h"\n'
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(drd_params),"
         " offsetof(drd_params, dx), offsetof(drd_params, bdr_low), sizeof(drd_solve_info),"
-        " sizeof(drd_step_info), offsetof(drd_step_info, div_before)); return 0;}\n")
+        " sizeof(drd_step_info), offsetof(drd_step_info, div_before));"
+        " printf(\"%zu %zu\\n\", sizeof(drd_learn_info), offsetof(drd_learn_info, loss_first)); return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = list(map(int, subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()))
     want = [C.sizeof(abi.Params), abi.Params.dx.offset, abi.Params.bdr_low.offset,
-            C.sizeof(abi.SolveInfo), C.sizeof(abi.StepInfo), abi.StepInfo.div_before.offset]
+            C.sizeof(abi.SolveInfo), C.sizeof(abi.StepInfo), abi.StepInfo.div_before.offset,
+            C.sizeof(abi.LearnInfo), abi.LearnInfo.loss_first.offset]
     assert got == want
 
 
