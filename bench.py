@@ -6,8 +6,19 @@ pressure projection, every Krylov solve run to the configured relative
 residual.  The timed region starts with every field resident in HBM.
 
 Modes
-  dropin : the plain FP64 path with the reference's own preconditioners and
-           stopping rule (what a Fortran caller of the drop-in gets).
+  dropin  : the plain FP64 path with the reference's own preconditioners and
+            stopping rule (what a Fortran caller of the drop-in gets).
+  fp64dct : FP64 DCT/DST velocity preconditioner, block-Jacobi CH, BiCGSTAB.
+  fp32    : the same with the velocity preconditioner in FP32.
+  learned : fp32 plus the learned correction: the first BDF2 warmup steps
+            record solver-emitted pairs, the network trains on them, one more
+            warmup step judges it (learned FGMRES against plain FGMRES and
+            the fp32 method) and the timed steps use what the judge kept.
+
+The learned correction is also evaluated on its own workload (1024^2
+variable-viscosity velocity systems, FGMRES to 1e-10 with and without it on
+held-out systems) and its network's forward pass is timed against the BF16
+MFMA peak.
 
 Prints one JSON line (see the `result` dict below).
 """
@@ -23,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0  # dense BF16 MFMA
 
 
 def log(msg: str) -> None:
@@ -71,13 +83,40 @@ MODES = {
     "dropin": ("PREC_REFERENCE", "CH_PREC_REFERENCE"),
     "fp64dct": ("PREC_DCT64", "CH_PREC_BLOCK"),
     "fp32": ("PREC_DCT32", "CH_PREC_BLOCK"),
+    "learned": ("PREC_LEARNED", "CH_PREC_BLOCK"),
 }
+
+LEARN = {"record_steps": 3, "capacity": 64, "crop": 256, "iters": 300, "batch": 4, "lr": 5e-4}
+
+
+def learned_warmup(g, p, t):
+    """BDF2 steps recording pairs, training, one judged step; returns (t, info)."""
+    n = LEARN["crop"]
+    g.learn_init(LEARN["capacity"], min(n, p.ny), min(n, p.nx), seed=1)
+    g.learn_record(True)
+    for _ in range(LEARN["record_steps"]):
+        g.step(2, t)
+        t += p.dt
+    g.learn_record(False)
+    info = g.learn_train(LEARN["iters"], LEARN["batch"], LEARN["lr"])
+    log(f"[learned] {info['pairs']} pairs, {info['train_iters']} Adam steps in {info['train_ms']:.0f} ms, "
+        f"loss {info['loss_first']:.3g} -> {info['loss_last']:.3g}")
+    g.learn_enable(True)
+    s = g.step(2, t)
+    t += p.dt
+    info = g.learn_info()
+    log(f"[learned] judged step: learned FGMRES {info['judged_iters']} its {info['judged_ms']:.1f} ms, "
+        f"plain FGMRES {info['ref_iters']} its, plain method {info['plain_ms']:.1f} ms -> state {info['state']}; "
+        f"velo {s['velo']['iter']} ({s['velo']['info']})")
+    return t, info
 
 
 def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer: bool = False):
-    """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1).
-    With timer, every launch of the BDF2 velocity matvec (MV_DOT1) inside
-    the timed steps is bracketed by HIP events on the library's stream."""
+    """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1), so
+    every mode is timed on the same steps of the same run.  The learned mode
+    spends the last record_steps + 1 warmup steps recording, training and
+    judging.  With timer, every launch of the BDF2 velocity matvec (MV_DOT1)
+    inside the timed steps is bracketed by HIP events on the library's stream."""
     from drd_application_amd import abi, geometry
     from drd_application_amd.solver import GpuSolver
 
@@ -89,11 +128,15 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
     geometry.load(g, cfg)
     g.synchronize()
     t = 0.0
-    for s in range(max(1, warmup)):
+    plain_warm = max(1, warmup - (LEARN["record_steps"] + 1 if mode == "learned" else 0))
+    for s in range(plain_warm):
         info = g.step(1 if s == 0 else 2, t)
         t += p.dt
         log(f"[{mode}] warmup step {s}: phmu {info['phmu']['iter']} ({info['phmu']['info']}), "
             f"velo {info['velo']['iter']} ({info['velo']['info']})")
+    learn = None
+    if mode == "learned":
+        t, learn = learned_warmup(g, p, t)
     g.synchronize()
     if timer:
         g.kernel_timer(abi.K_VELO_MATVEC, 1 << 14)
@@ -110,16 +153,92 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
     log(f"[{mode}] {steps} steps in {elapsed:.3f} s ({steps/elapsed:.3f} steps/s); velo iters {iters_v}; "
         f"phmu iters {iters_c}; info {infos}")
     out = {"steps_per_s": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "warmup_steps": plain_warm + (LEARN["record_steps"] + 1 if learn is not None else 0),
            "velo_iters": sum(iters_v) / steps, "phmu_iters": sum(iters_c) / steps,
            "converged": all(a == 0 and b == 0 for a, b in infos)}
     if timer:
         n, ms, by = g.kernel_timer_read()
         g.kernel_timer(abi.K_VELO_MATVEC, 0)
         out["timed_launches"] = {"count": n, "total_ms": ms, "bytes_per_launch": by}
+    if learn is not None:
+        names = {abi.LEARN_ON: "learned FGMRES", abi.LEARN_FALLBACK: "plain fallback (fp32 BiCGSTAB)"}
+        fin = g.learn_info()
+        out["learned"] = {"pairs": learn["pairs"], "train_ms": learn["train_ms"],
+                          "loss_first": learn["loss_first"], "loss_last": learn["loss_last"],
+                          "judged": {"learned_fgmres_iters": learn["judged_iters"],
+                                     "plain_fgmres_iters": learn["ref_iters"],
+                                     "learned_fgmres_ms": learn["judged_ms"],
+                                     "plain_method_ms": learn["plain_ms"]},
+                          "timed_steps_used": names.get(learn["state"], str(learn["state"])),
+                          "fallbacks": fin["fallbacks"]}
     if keep:
         return out, g
     g.close()
     return out, None
+
+
+def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: int = 600, batch: int = 8,
+               crop: int = 256, lr: float = 5e-4):
+    """The learned correction on its own workload: n x n variable-viscosity
+    velocity systems (geometry.velocity_system).  Pairs come from recorded
+    BiCGSTAB solves of the training systems; held-out systems are solved to
+    1e-10 by FGMRES with and without the correction (and by the plain
+    BiCGSTAB for reference).  Also times the n x n network's forward pass."""
+    import numpy as np
+
+    from drd_application_amd import abi, geometry
+    from drd_application_amd.solver import GpuSolver
+
+    cfg0, _ = geometry.velocity_system(n, 3000)
+    g = GpuSolver(cfg0.params)
+    g.learn_init(train_systems * 14, crop, crop, seed=11)
+    t0 = time.perf_counter()
+    for s in range(train_systems):
+        cfg, b = geometry.velocity_system(n, 3000 + s)
+        geometry.load(g, cfg)
+        g.learn_collect(b, tol=1e-12)
+    collect_s = time.perf_counter() - t0
+    info = g.learn_train(iters, batch, lr)
+    log(f"[c3] {info['pairs']} pairs from {train_systems} systems ({collect_s:.1f} s incl. generation), "
+        f"train {info['train_ms']:.0f} ms, loss {info['loss_first']:.3g} -> {info['loss_last']:.3g}")
+    rows = []
+    for s in range(heldout):
+        cfg, b = geometry.velocity_system(n, 3900 + s)
+        geometry.load(g, cfg)
+        x0 = np.zeros_like(b)
+        g.synchronize()
+        r = {"system": cfg.notes}
+        for key, fn in (("fgmres_plain", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 5000, 1e-10)),
+                        ("fgmres_learned", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_LEARNED, b, x0, 5000, 1e-10)),
+                        ("bicgstab_plain", lambda: g.bicgstab(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 5000, 1e-10))):
+            t1 = time.perf_counter()
+            _, its, _, inf = fn()
+            r[key] = {"iters": its, "ms": 1e3 * (time.perf_counter() - t1), "info": inf}
+        rows.append(r)
+        log(f"[c3] held-out {cfg.notes}: FGMRES plain {r['fgmres_plain']['iters']} its "
+            f"{r['fgmres_plain']['ms']:.0f} ms, learned {r['fgmres_learned']['iters']} its "
+            f"{r['fgmres_learned']['ms']:.0f} ms; BiCGSTAB {r['bicgstab_plain']['iters']} its "
+            f"{r['bicgstab_plain']['ms']:.0f} ms")
+    ms, flops = g.bench_kernel(abi.K_UNET_FWD, 10)
+    g.close()
+    tot = {k: sum(r[k]["iters"] for r in rows) for k in ("fgmres_plain", "fgmres_learned", "bicgstab_plain")}
+    tms = {k: sum(r[k]["ms"] for r in rows) for k in ("fgmres_plain", "fgmres_learned", "bicgstab_plain")}
+    tflops = flops / (ms * 1e-3) / 1e12
+    log(f"[c3] network forward {n}x{n}: {ms*1e3:.0f} us, {flops/1e9:.1f} GFLOP -> {tflops:.1f} TFLOP/s")
+    return {
+        "workload": f"{n}x{n} velocity systems (I - dt div(eta grad) + dt chi R) u = f, eta contrast "
+                    f"log-uniform to 1e3, R = 1e6 discs, FGMRES(30) to 1e-10",
+        "train_systems": train_systems, "pairs": info["pairs"], "adam_steps": info["train_iters"],
+        "crop": crop, "batch": batch, "train_ms": info["train_ms"],
+        "loss_first": info["loss_first"], "loss_last": info["loss_last"],
+        "heldout_systems": heldout, "iters": tot, "ms": tms,
+        "iteration_gain": (tot["fgmres_plain"] - tot["fgmres_learned"]) / tot["fgmres_plain"],
+        "time_gain": (tms["fgmres_plain"] - tms["fgmres_learned"]) / tms["fgmres_plain"],
+        "per_system": rows,
+        "roofline_network": {"bound": "mfma", "achieved": tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tflops / PEAK_BF16_TFLOPS, "flops_per_launch": flops,
+                             "us_per_launch": ms * 1e3},
+    }
 
 
 def measured_traffic(cfg_name: str):
@@ -163,13 +282,17 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps (BDF1 first); the learned mode records, trains and judges in the last "
+                         "4 of them")
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="fp32", choices=sorted(MODES))
-    ap.add_argument("--compare", default="fp64dct,dropin",
+    ap.add_argument("--mode", default="learned", choices=sorted(MODES))
+    ap.add_argument("--compare", default="fp32,fp64dct,dropin",
                     help="comma-separated modes timed beside the headline mode ('' for none)")
-    ap.add_argument("--compare-steps", type=int, default=2)
+    ap.add_argument("--compare-steps", type=int, default=None,
+                    help="timed steps of the comparison modes (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the learned correction's own workload")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the stand-alone kernel loops on the config's fields (for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -186,9 +309,12 @@ def main(argv=None) -> int:
         return profile_kernels(cfg, args.mode)
     head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=True)
     tl = head.pop("timed_launches")
-    if tl["count"] == 0:
-        log("the headline mode launched no BDF2 velocity matvec; no roofline")
-        ms_k, bytes_k, achieved = None, tl["bytes_per_launch"], None
+    source = "in-solve HIP events"
+    if tl["count"] == 0:  # FGMRES launches the plain matvec: time it stand-alone
+        ms_k, bytes_k = g.bench_kernel(abi.K_VELO_MATVEC, 20)
+        achieved = bytes_k / (ms_k * 1e-3) / 1e9
+        source = "stand-alone loop (the timed steps launched no BiCGSTAB matvec)"
+        log(f"velocity matvec stand-alone: {ms_k*1e3:.1f} us -> {achieved:.0f} GB/s")
     else:
         ms_k = tl["total_ms"] / tl["count"]
         bytes_k = tl["bytes_per_launch"]
@@ -205,7 +331,13 @@ def main(argv=None) -> int:
     g.close()
     modes = {args.mode: head}
     for m in [s for s in args.compare.split(",") if s and s != args.mode]:
-        modes[m], _ = run_mode(cfg, m, args.compare_steps, 1)
+        modes[m], _ = run_mode(cfg, m, args.compare_steps or args.steps, args.warmup)
+    c3 = None
+    if not args.no_c3:
+        try:
+            c3 = learned_c3()
+        except Exception as e:  # reported beside the metric, never required for it
+            log(f"learned-correction workload failed: {e}")
     cpu = None
     if not args.no_cpu:
         ref = modes.get("dropin", head)
@@ -219,7 +351,7 @@ def main(argv=None) -> int:
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": max(1, args.warmup),
+        "warmup": head["warmup_steps"],
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -238,8 +370,9 @@ def main(argv=None) -> int:
                      "traffic": measured_traffic(cfg.name),
                      "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
                      "us_per_launch": ms_k * 1e3 if ms_k else None,
-                     "launches_timed": tl["count"]},
+                     "launches_timed": tl["count"], "source": source},
         "cpu_baseline": cpu,
+        "learned_c3": c3,
     }
     print(json.dumps(result))
     return 0

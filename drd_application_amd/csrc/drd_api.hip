@@ -1114,6 +1114,10 @@ extern "C" int drd_learn_init(drd_ctx* c, int32_t capacity, int32_t crop_h, int3
     learn_destroy(L);
     return set_err(DRD_E_HIP, "learned-correction buffers do not fit");
   }
+  if (hipMemset(L->img, 0, (size_t)L->npx * kUnetIn * 2) != hipSuccess) {
+    learn_destroy(L);
+    return set_err(DRD_E_HIP, "hipMemset");
+  }
   c->learn = L;
   return 0;
 }
@@ -1623,6 +1627,9 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       case DRD_K_COPY:
         launch_copy(c->stream, g.nuv(), c->kw[2], c->kw[3]);
         break;
+      case DRD_K_UNET_FWD:
+        (void)unet_forward(c->stream, c->learn->net, c->learn->img, c->learn->out);
+        break;
     }
   };
   switch (which) {
@@ -1640,6 +1647,10 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       break;
     case DRD_K_COPY:
       *bytes = 2 * N8;
+      break;
+    case DRD_K_UNET_FWD:  // MFMA FLOPs, not bytes
+      if (!c->learn) return set_err(DRD_E_STATE, "drd_learn_init has not been called");
+      *bytes = unet_forward_flops(c->learn->net);
       break;
     default:
       return set_err(DRD_E_ARG, "unknown kernel id");

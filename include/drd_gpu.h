@@ -242,6 +242,9 @@ int drd_synchronize(drd_ctx* ctx);
 #define DRD_K_BICG_XR 2     /* x,r update + two dots                       */
 #define DRD_K_PRESSURE 3    /* whole pressure projection                   */
 #define DRD_K_COPY 4        /* plain copy of Nuv doubles (counter scale)    */
+#define DRD_K_UNET_FWD 5    /* forward pass of the learned correction's
+                               ny x nx network (after drd_learn_init);
+                               *bytes returns its MFMA FLOPs instead      */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
 /* Times kernel `which` inside the caller's own solves: while enabled, each
