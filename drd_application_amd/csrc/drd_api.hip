@@ -87,6 +87,13 @@ struct drd_ctx {
   Grid g;
   Scal s;
   hipStream_t stream = nullptr;
+  // batch group: nb instances; instance i's per-instance buffers sit i * bs
+  // bytes after instance 0's (one slab of bs bytes per instance)
+  int nb = 1;
+  int64_t bs = 0;
+  void* slab = nullptr;
+  std::vector<drd_solve_info> last;  // per instance: the latest BiCGSTAB solve
+  std::vector<double> last_div;      // per instance: the latest mean divergence
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
   std::vector<hipEvent_t> tm_ev;
@@ -119,8 +126,23 @@ struct drd_ctx {
   Dct1Tab *d1_nx = nullptr, *d1_ny = nullptr, *d1_nx1 = nullptr;
   double *ph_f2 = nullptr, *ph_t = nullptr, *aux_work = nullptr, *esolp_tabs = nullptr;
   double* dct_work = nullptr;
+  double* vp_inv64 = nullptr;  // per-instance buffers of the velocity preconditioner
+  float* vp_inv32 = nullptr;
+  void* vp_work = nullptr;
   double cosa = 0, sqrt2 = 0;
   Learn* learn = nullptr;  // drd_learn_init
+};
+
+BatchCfg& drd::batch_cfg() {
+  static thread_local BatchCfg cfg;
+  return cfg;
+}
+
+// Launches inside the scope cover every instance of the context's group.
+struct BatchScope {
+  BatchCfg saved;
+  explicit BatchScope(const drd_ctx* c) : saved(batch_cfg()) { batch_cfg() = BatchCfg{c->nb, c->bs}; }
+  ~BatchScope() { batch_cfg() = saved; }
 };
 
 // ---------------------------------------------------------------------------
@@ -302,51 +324,78 @@ static int build_tables(drd_ctx* c) {
 }
 
 // cf = cos(angle*pi/180)/sqrt(2) per column  TP/ch_bdr_condition.f90:61
-static int upload_angles(drd_ctx* c) {
+// wetting coefficients of instance `inst` from its contact angles
+static int upload_angles(drd_ctx* c, int inst = 0) {
   const int nx = c->p.nx;
+  const int64_t o = (int64_t)inst * c->bs;
+  auto at = [&](double* p) { return reinterpret_cast<double*>(reinterpret_cast<char*>(p) + o); };
   std::vector<double> aB(nx + 1), aT(nx + 1), cB(nx + 1), cT(nx + 1);
-  HIPCHK(hipMemcpy(aB.data(), c->f[DRD_F_ANGLEB], (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(aT.data(), c->f[DRD_F_ANGLET], (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(aB.data(), at(c->f[DRD_F_ANGLEB]), (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(aT.data(), at(c->f[DRD_F_ANGLET]), (nx + 1) * sizeof(double), hipMemcpyDeviceToHost));
   for (int i = 0; i <= nx; ++i) {
     cB[i] = std::cos(aB[i] * kPiG / 180.0) / std::sqrt(2.0);
     cT[i] = std::cos(aT[i] * kPiG / 180.0) / std::sqrt(2.0);
   }
-  HIPCHK(hipMemcpy(c->cfB, cB.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->cfT, cT.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(at(c->cfB), cB.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(at(c->cfT), cT.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
+// Per-instance buffers come from one slab per instance, laid out alike;
+// tables, upload slots and the lazily built FGMRES / PhSolve / ESOLP
+// resources are separate allocations shared by the group.
 static int ctx_alloc(drd_ctx* c) {
   const drd_params& p = c->p;
   const Grid& g = c->g;
-  for (int f = 0; f < DRD_F_COUNT; ++f) RCHK(dalloc(&c->f[f], drd_field_size(&p, f)));
-  RCHK(dalloc(&c->u_after, g.nuf()));
-  RCHK(dalloc(&c->v_after, g.nvf()));
-  RCHK(dalloc(&c->ph_after, g.nc()));
-  RCHK(dalloc(&c->rmu_after, g.nc()));
-  RCHK(dalloc(&c->zero_c, g.nc()));
-  const int64_t big = std::max<int64_t>(g.nuf(), g.nc());
-  for (auto& a : c->arg) RCHK(dalloc(&a, big));
   c->nmax = std::max<int64_t>(g.nuv(), 2 * g.ncell());
-  for (auto& w : c->kw) RCHK(dalloc(&w, c->nmax));
-  RCHK(dalloc(&c->rhs, c->nmax));
-  RCHK(dalloc(&c->xv, c->nmax));
-  RCHK(dalloc(&c->diag, c->nmax));
-  RCHK(dalloc(&c->tmp, c->nmax));
-  RCHK(dalloc(&c->tmp2, c->nmax));
-  // partial sums: up to 2 segments x 2 values per tile, or one per grid row
   const int64_t tiles = (c->nmax + kTile - 1) / kTile;
   if (tiles > kMaxTiles) return set_err(DRD_E_UNSUPPORTED, "grid too large for the reduction tree");
   c->tiles = tiles;
-  RCHK(dalloc(&c->part, std::max<int64_t>(4 * tiles, g.ny + g.nx + 64)));
-  RCHK(dalloc(&c->ks, 1));
-  HIPCHK(hipHostMalloc((void**)&c->ks_host, sizeof(KState), hipHostMallocDefault));
-  RCHK(dalloc(&c->cfB, p.nx + 1));
-  RCHK(dalloc(&c->cfT, p.nx + 1));
+  struct Item {
+    void** ptr;
+    size_t bytes;
+  };
+  std::vector<Item> items;
+  auto add = [&](auto** ptr, int64_t n, size_t esz) { items.push_back({(void**)ptr, (size_t)std::max<int64_t>(n, 1) * esz}); };
+  for (int f = 0; f < DRD_F_COUNT; ++f) add(&c->f[f], drd_field_size(&p, f), 8);
+  add(&c->u_after, g.nuf(), 8);
+  add(&c->v_after, g.nvf(), 8);
+  add(&c->ph_after, g.nc(), 8);
+  add(&c->rmu_after, g.nc(), 8);
+  add(&c->zero_c, g.nc(), 8);
+  for (auto& w : c->kw) add(&w, c->nmax, 8);
+  add(&c->rhs, c->nmax, 8);
+  add(&c->xv, c->nmax, 8);
+  add(&c->diag, c->nmax, 8);
+  add(&c->tmp, c->nmax, 8);
+  add(&c->tmp2, c->nmax, 8);
+  // partial sums: up to 2 segments x 2 values per tile, or one per grid row
+  add(&c->part, std::max<int64_t>(4 * tiles, g.ny + g.nx + 64), 8);
+  add(&c->ks, 1, sizeof(KState));
+  add(&c->pn, 2 * (int64_t)p.nx, 8);
+  add(&c->dct_work, 2 * g.ncell(), 8);
+  add(&c->vp_inv64, g.nuv(), 8);
+  add(&c->vp_inv32, g.nuv(), 4);
+  add(&c->vp_work, 2 * g.nuv(), 8);
+  add(&c->cfB, p.nx + 1, 8);
+  add(&c->cfT, p.nx + 1, 8);
+  size_t S = 0;
+  std::vector<size_t> off;
+  for (const Item& it : items) {
+    off.push_back(S);
+    S += (it.bytes + 255) / 256 * 256;
+  }
+  c->bs = (int64_t)S;
+  HIPCHK(hipMalloc(&c->slab, S * (size_t)c->nb));
+  HIPCHK(hipMemset(c->slab, 0, S * (size_t)c->nb));
+  for (size_t i = 0; i < items.size(); ++i) *items[i].ptr = static_cast<char*>(c->slab) + off[i];
+  c->last.assign(c->nb, drd_solve_info{});
+  c->last_div.assign(c->nb, 0.0);
+  HIPCHK(hipHostMalloc((void**)&c->ks_host, c->nb * sizeof(KState), hipHostMallocDefault));
+  const int64_t big = std::max<int64_t>(g.nuf(), g.nc());
+  for (auto& a : c->arg) RCHK(dalloc(&a, big));
   RCHK(dalloc(&c->uL, p.ny + 1));
   HIPCHK(hipHostMalloc((void**)&c->uL_host, (p.ny + 1) * sizeof(double), hipHostMallocDefault));
-  RCHK(dalloc(&c->pn, 2 * (int64_t)p.nx));
-  RCHK(dalloc(&c->dct_work, 2 * g.ncell()));
   c->dct = dct_plan_create(p.nx, p.ny);
   return 0;
 }
@@ -354,30 +403,13 @@ static int ctx_alloc(drd_ctx* c) {
 static void ctx_free(drd_ctx* c) {
   for (auto e : c->tm_ev) hipEventDestroy(e);
   c->tm_ev.clear();
-  for (auto& a : c->f) hipFree(a);
-  hipFree(c->u_after);
-  hipFree(c->v_after);
-  hipFree(c->ph_after);
-  hipFree(c->rmu_after);
-  hipFree(c->zero_c);
+  hipFree(c->slab);
   for (auto& a : c->arg) hipFree(a);
-  for (auto& w : c->kw) hipFree(w);
-  hipFree(c->rhs);
-  hipFree(c->xv);
-  hipFree(c->diag);
-  hipFree(c->tmp);
-  hipFree(c->tmp2);
-  hipFree(c->part);
-  hipFree(c->ks);
   if (c->ks_host) hipHostFree(c->ks_host);
   hipFree(c->pre_lcc);
   hipFree(c->ph_lcc);
-  hipFree(c->cfB);
-  hipFree(c->cfT);
   hipFree(c->uL);
   if (c->uL_host) hipHostFree(c->uL_host);
-  hipFree(c->pn);
-  hipFree(c->dct_work);
   dct_plan_destroy(c->dct);
   velo_prec_destroy(c->vprec);
   learn_destroy(c->learn);
@@ -397,6 +429,11 @@ static void ctx_free(drd_ctx* c) {
   if (c->stream) hipStreamDestroy(c->stream);
 }
 
+static VeloPrec* make_vprec(drd_ctx* c, bool fp32, int* err) {
+  const VeloPrecBufs bufs{c->vp_inv64, c->vp_inv32, c->vp_work};
+  return velo_prec_create(c->g, c->s, fp32, err, &bufs);
+}
+
 // The fast-Poisson velocity preconditioner is built with the context (or
 // when drd_set_params first asks for it), so no solve pays its set-up.
 static int eager_vprec(drd_ctx* c) {
@@ -407,14 +444,14 @@ static int eager_vprec(drd_ctx* c) {
   velo_prec_destroy(c->vprec);
   c->vprec = nullptr;
   int err = 0;
-  c->vprec = velo_prec_create(c->g, c->s, fp32, &err);
+  c->vprec = make_vprec(c, fp32, &err);
   if (!c->vprec)
     return set_err(err ? err : DRD_E_HIP, "DCT velocity preconditioner needs power-of-two nx, ny <= 4096");
   c->vprec_fp32 = fp32;
   return 0;
 }
 
-extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
+static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
   if (!prm || !out) return set_err(DRD_E_ARG, "null argument");
   if (prm->nx < 4 || prm->ny < 4) return set_err(DRD_E_ARG, "nx, ny must be >= 4");
   if (prm->nx > 4096 || prm->ny > 4096) return set_err(DRD_E_ARG, "nx, ny must be <= 4096");
@@ -428,6 +465,7 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
   if ((int64_t)(prm->nx + 3) * (prm->ny + 2) >= ((int64_t)1 << 31))
     return set_err(DRD_E_UNSUPPORTED, "grid too large for 32-bit indexing");
   drd_ctx* c = new drd_ctx();
+  c->nb = nb;
   c->p = *prm;
   c->g = Grid::make(prm->nx, prm->ny);
   c->s = make_scal(*prm);
@@ -445,6 +483,8 @@ extern "C" int drd_init(const drd_params* prm, drd_ctx** out) {
   *out = c;
   return 0;
 }
+
+extern "C" int drd_init(const drd_params* prm, drd_ctx** out) { return ctx_create(prm, 1, out); }
 
 extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   if (!c || !prm) return set_err(DRD_E_ARG, "null argument");
@@ -589,7 +629,7 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     if (!c->vprec) {
       int err = 0;
-      c->vprec = velo_prec_create(c->g, c->s, fp32, &err);
+      c->vprec = make_vprec(c, fp32, &err);
       if (!c->vprec)
         return set_err(err ? err : DRD_E_HIP, "DCT velocity preconditioner needs power-of-two nx, ny <= 4096");
       c->vprec_fp32 = fp32;
@@ -670,10 +710,29 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
   return set_err(DRD_E_UNSUPPORTED, "preconditioner not built for this operator");
 }
 
+// every instance's KState: device <-> pinned host (one strided copy)
 static int read_state(drd_ctx* c) {
-  HIPCHK(hipMemcpyAsync(c->ks_host, c->ks, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+  if (c->nb == 1)
+    HIPCHK(hipMemcpyAsync(c->ks_host, c->ks, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+  else
+    HIPCHK(hipMemcpy2DAsync(c->ks_host, sizeof(KState), c->ks, (size_t)c->bs, sizeof(KState), (size_t)c->nb,
+                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
+}
+static int put_state(drd_ctx* c, const KState& h) {
+  for (int i = 0; i < c->nb; ++i) c->ks_host[i] = h;
+  if (c->nb == 1)
+    HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  else
+    HIPCHK(hipMemcpy2DAsync(c->ks, (size_t)c->bs, c->ks_host, sizeof(KState), sizeof(KState), (size_t)c->nb,
+                            hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+static bool all_done(const drd_ctx* c) {
+  for (int i = 0; i < c->nb; ++i)
+    if (!c->ks_host[i].done) return false;
+  return true;
 }
 
 // FP64 ||b - A x|| / ||b|| (true residual; an addition to the reference)
@@ -711,8 +770,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   h.omega = 1.0;
   const double eps = std::ldexp(1.0, -53);  // DLAMCH('E')
   h.breaktol = eps * eps;                   // LIB/getbreak.f:11-12
-  *c->ks_host = h;
-  HIPCHK(hipMemcpyAsync(c->ks, c->ks_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  RCHK(put_state(c, h));
   RCHK(prec_setup(c, prec));
   const int ns = op_nseg(op);
   const bool fused_block = (op == DRD_OP_PHMU && prec == DRD_PS_PHMU_BLOCK);
@@ -750,11 +808,13 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
     HIPCHK(hipGetLastError());
     total += chunk;
     RCHK(read_state(c));
-    if (c->ks_host->done || total > maxit + 2) break;
+    if (all_done(c) || total > maxit + 2) break;
     chunk = std::min(chunk * 2, 32);
   }
+  if (!all_done(c)) return set_err(DRD_E_STATE, "BiCGSTAB state machine did not terminate");
+  for (int i = 0; i < c->nb; ++i)
+    c->last[i] = drd_solve_info{c->ks_host[i].iter, c->ks_host[i].info, c->ks_host[i].resid, 0.0};
   const KState& s = *c->ks_host;
-  if (!s.done) return set_err(DRD_E_STATE, "BiCGSTAB state machine did not terminate");
   if (info) {
     info->iter = s.iter;
     info->info = s.info;
@@ -1393,7 +1453,8 @@ static int presolve_dev(drd_ctx* c, double* psi, double coef, double* p_update, 
   HIPCHK(hipGetLastError());
   if (mean_div) {
     RCHK(read_state(c));
-    *mean_div = c->ks_host->red[0] / c->p.nx / c->p.ny;
+    for (int i = 0; i < c->nb; ++i) c->last_div[i] = c->ks_host[i].red[0] / c->p.nx / c->p.ny;
+    *mean_div = c->last_div[0];
   }
   return 0;
 }
@@ -1673,14 +1734,16 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
 }
 
 // ---- time step (TP/mcl6.f90:257-303 and :395-502) ---------------------------
-extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) {
-  if (!c) return set_err(DRD_E_ARG, "null ctx");
+// One step of every instance of the context's group; per-instance reports
+// go to out[0 .. nb).
+static int step_dev(drd_ctx* c, int bdf, double t, drd_step_info* out) {
   if (bdf != 1 && bdf != 2) return set_err(DRD_E_ARG, "bdf must be 1 or 2");
   const drd_params& p = c->p;
   drd_step_info loc{};
   const double gamma1 = std::min(1.0, p.lambda_rho);
   double** F = c->f;
   RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false));
+  for (int i = 0; i < c->nb; ++i) out[i].phmu = c->nb == 1 ? loc.phmu : c->last[i];
   if (bdf == 1) {
     RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt, nullptr,
                         nullptr, c->u_after, c->v_after, &loc.velo, false));
@@ -1688,6 +1751,7 @@ extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) 
     RCHK(velo_solve_dev(c, 2, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], F[DRD_F_V_PRE], c->ph_after,
                         c->rmu_after, t + p.dt, nullptr, nullptr, c->u_after, c->v_after, &loc.velo, false));
   }
+  for (int i = 0; i < c->nb; ++i) out[i].velo = c->nb == 1 ? loc.velo : c->last[i];
   // u_pre = u; u = u_after (pointer rotation: u_after is fully rewritten next step)
   std::swap(F[DRD_F_U_PRE], F[DRD_F_U]);
   std::swap(F[DRD_F_U], c->u_after);
@@ -1706,6 +1770,131 @@ extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) 
     coef = 3 * p.re * gamma1 / 2. / p.dt;
   }
   RCHK(presolve_dev(c, F[DRD_F_PSI2], coef, F[DRD_F_P], &loc.div_before));
+  for (int i = 0; i < c->nb; ++i) out[i].div_before = c->last_div[i];
+  return 0;
+}
+
+extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  drd_step_info loc{};
+  RCHK(step_dev(c, bdf, t, &loc));
   if (info) *info = loc;
+  return 0;
+}
+
+// ---- batches: groups of equal-size instances stepped by batched launches -----
+struct drd_batch {
+  std::vector<drd_ctx*> groups;
+  std::vector<std::pair<int, int>> where;  // instance -> (group, index in the group)
+  std::vector<std::vector<drd_step_info>> infos;
+};
+
+extern "C" int drd_batch_destroy(drd_batch* b) {
+  if (!b) return 0;
+  for (drd_ctx* c : b->groups) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ctx_free(c);
+    delete c;
+  }
+  delete b;
+  return 0;
+}
+
+extern "C" int drd_batch_create(int32_t count, const drd_params* params, drd_batch** out) {
+  if (count < 1 || !params || !out) return set_err(DRD_E_ARG, "bad argument");
+  std::vector<int> gid(count, -1);
+  std::vector<int> rep_of;  // group -> first instance
+  for (int i = 0; i < count; ++i) {
+    const drd_params& q = params[i];
+    if (q.velo_prec != DRD_PREC_DCT32 && q.velo_prec != DRD_PREC_DCT64 && q.velo_prec != DRD_PREC_JACOBI &&
+        q.velo_prec != DRD_PREC_IDENTITY && q.velo_prec != DRD_PREC_REFERENCE)
+      return set_err(DRD_E_UNSUPPORTED, "batched steps take the DCT, Jacobi or identity velocity preconditioner");
+    if (q.krylov != DRD_KRYLOV_BICGSTAB) return set_err(DRD_E_UNSUPPORTED, "batched steps use BiCGSTAB");
+    for (size_t gI = 0; gI < rep_of.size(); ++gI) {
+      const drd_params& r = params[rep_of[gI]];
+      if (r.nx == q.nx && r.ny == q.ny) {
+        if (std::memcmp(&r, &q, sizeof(drd_params)) != 0)
+          return set_err(DRD_E_ARG,
+                         "instances of one grid size must share every parameter (put per-instance penalties "
+                         "into bdr_ph_penalty)");
+        gid[i] = (int)gI;
+        break;
+      }
+    }
+    if (gid[i] < 0) {
+      gid[i] = (int)rep_of.size();
+      rep_of.push_back(i);
+    }
+  }
+  drd_batch* b = new drd_batch();
+  std::vector<int> counts(rep_of.size(), 0);
+  b->where.resize(count);
+  for (int i = 0; i < count; ++i) b->where[i] = {gid[i], counts[gid[i]]++};
+  for (size_t gI = 0; gI < rep_of.size(); ++gI) {
+    drd_ctx* c = nullptr;
+    const int rc = ctx_create(&params[rep_of[gI]], counts[gI], &c);
+    if (rc) {
+      drd_batch_destroy(b);
+      return rc;
+    }
+    b->groups.push_back(c);
+    b->infos.emplace_back(counts[gI]);
+  }
+  *out = b;
+  return 0;
+}
+
+static int batch_locate(drd_batch* b, int32_t inst, int32_t field, drd_ctx** c, char** dev, int64_t* n) {
+  if (!b || inst < 0 || inst >= (int32_t)b->where.size()) return set_err(DRD_E_ARG, "bad instance");
+  *c = b->groups[b->where[inst].first];
+  if (field < 0 || field >= DRD_F_COUNT) return set_err(DRD_E_ARG, "bad field id");
+  *n = drd_field_size(&(*c)->p, field);
+  *dev = reinterpret_cast<char*>((*c)->f[field]) + (int64_t)b->where[inst].second * (*c)->bs;
+  return 0;
+}
+
+extern "C" int drd_batch_set_field(drd_batch* b, int32_t inst, int32_t field, const double* host) {
+  drd_ctx* c = nullptr;
+  char* dev = nullptr;
+  int64_t n = 0;
+  RCHK(batch_locate(b, inst, field, &c, &dev, &n));
+  if (!host) return set_err(DRD_E_ARG, "null host array");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(dev, host, n * sizeof(double), hipMemcpyHostToDevice));
+  if (field == DRD_F_ANGLEB || field == DRD_F_ANGLET) RCHK(upload_angles(c, b->where[inst].second));
+  return 0;
+}
+
+extern "C" int drd_batch_get_field(drd_batch* b, int32_t inst, int32_t field, double* host) {
+  drd_ctx* c = nullptr;
+  char* dev = nullptr;
+  int64_t n = 0;
+  RCHK(batch_locate(b, inst, field, &c, &dev, &n));
+  if (!host) return set_err(DRD_E_ARG, "null host array");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, int32_t cap) {
+  if (!b || !ngroups) return set_err(DRD_E_ARG, "null argument");
+  *ngroups = (int32_t)b->groups.size();
+  for (int gI = 0; gI < (int)b->groups.size() && desc && gI < cap; ++gI) {
+    desc[3 * gI] = b->groups[gI]->p.nx;
+    desc[3 * gI + 1] = b->groups[gI]->p.ny;
+    desc[3 * gI + 2] = b->groups[gI]->nb;
+  }
+  return 0;
+}
+
+extern "C" int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info* infos) {
+  if (!b) return set_err(DRD_E_ARG, "null batch");
+  for (size_t gI = 0; gI < b->groups.size(); ++gI) {
+    drd_ctx* c = b->groups[gI];
+    BatchScope scope(c);
+    RCHK(step_dev(c, bdf, t, b->infos[gI].data()));
+  }
+  if (infos)
+    for (size_t i = 0; i < b->where.size(); ++i) infos[i] = b->infos[b->where[i].first][b->where[i].second];
   return 0;
 }

@@ -69,6 +69,26 @@ __device__ __forceinline__ void st32(T* __restrict__ p, uint32_t i, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (uint32_t)sizeof(T)) = v;
 }
 
+// ---- batched launches ---------------------------------------------------------
+// A batch group is several instances of one grid size whose per-instance
+// buffers (fields, Krylov vectors, state, partial sums, scratch) share one
+// layout: instance i's copy of any of them sits i * bs bytes after instance
+// 0's.  Kernels on the time-step path take the instance from blockIdx.z and
+// offset their per-instance pointers with bofs (tables shared by the group
+// are not offset).  A single instance is the case n = 1, bs = 0.
+struct BatchCfg {
+  int n = 1;
+  int64_t bs = 0;
+};
+BatchCfg& batch_cfg();  // the calling host thread's current group (drd_api.hip)
+inline unsigned bat_n() { return (unsigned)batch_cfg().n; }
+inline int64_t bat_bs() { return batch_cfg().bs; }
+
+template <class T>
+__device__ __forceinline__ T* bofs(T* p, int64_t bs) {
+  return p ? (T*)((uintptr_t)p + (uintptr_t)((int64_t)blockIdx.z * bs)) : p;
+}
+
 // Scalars of module `global` the kernels read (TP/global.f90:42-52).
 struct Scal {
   double dt, re, ca, xld, s, I, dr, dz, dr2, dz2;

@@ -175,7 +175,14 @@ void launch_bicg_p_block(hipStream_t st, const Grid& g, const Scal& s, const Coe
 void launch_bicg_s_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* r, const double* v,
                          double* sh, double* part, KState* ks);
 struct VeloPrec;
-VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err);
+// per-instance buffers of the preconditioner (Nuv doubles, Nuv floats and
+// 2 Nuv doubles of transform work); null: the preconditioner allocates them
+struct VeloPrecBufs {
+  double* inv_s64;
+  float* inv_s32;
+  void* work;
+};
+VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err, const VeloPrecBufs* ext = nullptr);
 int velo_prec_update(VeloPrec* pc, const Scal& s);  // new dt, re, dx, dz
 void velo_prec_destroy(VeloPrec* pc);
 // diagonal scaling from the current coefficients (once per solve)

@@ -220,7 +220,15 @@ struct RowArgs {
 
 // One workgroup per line; dynamic LDS = 2N complex.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
+__global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
+  a.in = bofs(a.in, bs);  // the tables (tw, rot, cs, pre) are shared by the group
+  a.out = bofs(a.out, bs);
+  a.u = bofs(a.u, bs);
+  a.v = bofs(a.v, bs);
+  a.psi = bofs(a.psi, bs);
+  a.p = bofs(a.p, bs);
+  a.part = bofs(a.part, bs);
+  a.ks = bofs(a.ks, bs);
   extern __shared__ double2 smem[];
   const int N = a.N;
   const int row = blockIdx.x;
@@ -309,7 +317,9 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a) {
 
 // out (cols x rows) = transpose of in (rows x cols), 32x32 LDS tiles
 __global__ __launch_bounds__(kBlock) void k_transpose(int rows, int cols, const double* __restrict__ in,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, int64_t bs) {
+  in = bofs(in, bs);
+  out = bofs(out, bs);
   __shared__ double tile[32][33];
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -334,12 +344,12 @@ static void launch_rows(hipStream_t st, const RowArgs& a) {
                               (int)row_lds(4096));
     raised = true;
   }
-  hipLaunchKernelGGL(k_rows<MODE>, dim3(a.rows), dim3(kBlock), row_lds(a.N), st, a);
+  hipLaunchKernelGGL(k_rows<MODE>, dim3(a.rows, 1, bat_n()), dim3(kBlock), row_lds(a.N), st, a, bat_bs());
 }
 
 static void launch_transpose(hipStream_t st, int rows, int cols, const double* in, double* out) {
-  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
-  hipLaunchKernelGGL(k_transpose, grid, dim3(kBlock), 0, st, rows, cols, in, out);
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32, bat_n());
+  hipLaunchKernelGGL(k_transpose, grid, dim3(kBlock), 0, st, rows, cols, in, out, bat_bs());
 }
 
 static RowArgs rows_x(DctPlan* pl) {

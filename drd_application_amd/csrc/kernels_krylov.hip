@@ -21,7 +21,10 @@ __device__ __forceinline__ bool kact(const KState* st) { return !st->done && !st
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_init_norms(int64_t seg, const double* __restrict__ x,
                                                        const double* __restrict__ b, double* part,
-                                                       KState* st) {
+                                                       KState* st, int64_t bs) {
+  x = bofs(x, bs);
+  b = bofs(b, bs);
+  part = bofs(part, bs);
   double t[NS * 2][kSlots];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -38,8 +41,12 @@ __global__ __launch_bounds__(kBlock) void k_init_norms(int64_t seg, const double
 // p = r (first iteration) or p = beta*(p - omega*v) + r   REVCOM:266-275
 __global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const double* __restrict__ r,
                                                    const double* __restrict__ v, double* __restrict__ p,
-                                                   const KState* st) {
+                                                   const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (!kact(st)) return;
+  r = bofs(r, bs);
+  v = bofs(v, bs);
+  p = bofs(p, bs);
   const bool first = st->iter <= 1;
   const double omega = st->omega;
   const double beta = first ? 0.0 : (st->rho / st->rho1) * (st->alpha / st->omega);
@@ -62,8 +69,13 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const double* __re
 // s = r - alpha*v (S aliases R), ||s||^2           REVCOM:311-313
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restrict__ r,
-                                                   const double* __restrict__ v, double* part, KState* st) {
+                                                   const double* __restrict__ v, double* part, KState* st,
+                                                   int64_t bs) {
+  st = bofs(st, bs);
   if (!kact(st)) return;
+  r = bofs(r, bs);
+  v = bofs(v, bs);
+  part = bofs(part, bs);
   const double alpha = st->alpha;
   double t[NS][kSlots];
 #pragma unroll
@@ -87,8 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
 // kernel reads x or the flag afterwards, so no fence is needed.  Only the
 // one early-exit iteration of a solve gets past the first test.
 __global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __restrict__ x,
-                                                       const double* __restrict__ ph, KState* st) {
+                                                       const double* __restrict__ ph, KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (!st->early || st->done) return;
+  x = bofs(x, bs);
+  ph = bofs(ph, bs);
   const double alpha = st->alpha;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
@@ -110,8 +125,16 @@ template <int NS>
 __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restrict__ x, double* r,
                                                     const double* __restrict__ rt, const double* __restrict__ ph,
                                                     const double* sh, const double* __restrict__ t,
-                                                    double* part, KState* st) {
+                                                    double* part, KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (!kact(st)) return;
+  x = bofs(x, bs);
+  r = bofs(r, bs);
+  rt = bofs(rt, bs);
+  ph = bofs(ph, bs);
+  sh = bofs(sh, bs);
+  t = bofs(t, bs);
+  part = bofs(part, bs);
   const double alpha = st->alpha, omega = st->omega;
   double tm[NS * 2][kSlots];
 #pragma unroll
@@ -134,19 +157,26 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restr
   tile_publish<NS, 2>(tm, part);
 }
 
-__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b,
+                                                 int64_t bs) {
+  a = bofs(a, bs);
+  b = bofs(b, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q < n) b[q] = a[q];
 }
 
 __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double alpha, const double* __restrict__ x,
-                                                  double beta, double* __restrict__ y) {
+                                                  double beta, double* __restrict__ y, int64_t bs) {
+  x = bofs(x, bs);
+  y = bofs(y, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q < n) y[q] = alpha * x[q] + beta * y[q];
 }
 
 __global__ __launch_bounds__(kBlock) void k_cell_axpy(int64_t n, const double* __restrict__ a,
-                                                      double* __restrict__ b) {
+                                                      double* __restrict__ b, int64_t bs) {
+  a = bofs(a, bs);
+  b = bofs(b, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q < n) b[q] = a[q] + b[q];
 }
@@ -155,7 +185,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_axpy(int64_t n, const double* _
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_resid_norms(int64_t seg, const double* __restrict__ b,
                                                         const double* __restrict__ ax, double* part,
-                                                        KState* st) {
+                                                        KState* st, int64_t bs) {
+  (void)st;
+  b = bofs(b, bs);
+  ax = bofs(ax, bs);
+  part = bofs(part, bs);
   double t[NS * 2][kSlots];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -227,8 +261,12 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __res
 
 // Canonical total of the preceding kernel's tile partials; see kernels.hpp.
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part, uint32_t nt, int ns,
-                                                     int nv, double* out, KState* st, int phase, int guard) {
+                                                     int nv, double* out, KState* st, int phase, int guard,
+                                                     int64_t bs) {
+  st = bofs(st, bs);
   if (guard == FIN_ACTIVE && (st->done || st->early)) return;
+  part = bofs(part, bs);
+  out = bofs(out, bs);
   const int K = ns * nv;
   double tot[2] = {0.0, 0.0};
   for (int v = 0; v < nv; ++v) {
@@ -255,31 +293,46 @@ __global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* 
 static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 // Reductions see a vector of n = nseg * seg elements (nseg 1 or 2) and run
-// one workgroup per kTile elements of a segment.
-#define DRD_SEG_LAUNCH(kern, n, nseg, ...)                                                          \
-  do {                                                                                              \
-    const int64_t seg_ = (n) / (nseg);                                                              \
-    if ((nseg) == 2)                                                                                \
-      hipLaunchKernelGGL(kern<2>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);   \
-    else                                                                                            \
-      hipLaunchKernelGGL(kern<1>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);   \
+// one workgroup per kTile elements of a segment (times the batch group's
+// instances; the FGMRES kernels run single-instance).
+#define DRD_SEG_LAUNCH(kern, n, nseg, ...)                                                                    \
+  do {                                                                                                        \
+    const int64_t seg_ = (n) / (nseg);                                                                        \
+    if ((nseg) == 2)                                                                                          \
+      hipLaunchKernelGGL(kern<2>, dim3(nb(seg_, kTile), 1, bat_n()), dim3(kBlock), 0, s, seg_, __VA_ARGS__, \
+                         bat_bs());                                                                           \
+    else                                                                                                      \
+      hipLaunchKernelGGL(kern<1>, dim3(nb(seg_, kTile), 1, bat_n()), dim3(kBlock), 0, s, seg_, __VA_ARGS__, \
+                         bat_bs());                                                                           \
+  } while (0)
+#define DRD_SEG_LAUNCH1(kern, n, nseg, ...)                                                          \
+  do {                                                                                               \
+    const int64_t seg_ = (n) / (nseg);                                                               \
+    if ((nseg) == 2)                                                                                 \
+      hipLaunchKernelGGL(kern<2>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);    \
+    else                                                                                             \
+      hipLaunchKernelGGL(kern<1>, dim3(nb(seg_, kTile)), dim3(kBlock), 0, s, seg_, __VA_ARGS__);    \
   } while (0)
 
 void launch_finalize(hipStream_t s, const double* part, int64_t ntiles, int nseg, int nv, double* out, KState* ks,
                      int phase, int guard) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out, ks, phase,
-                     guard);
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bat_n()), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out,
+                     ks, phase, guard, bat_bs());
+}
+static void launch_finalize1(hipStream_t s, const double* part, int64_t ntiles, int nseg, int nv, double* out) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out,
+                     (KState*)nullptr, (int)PH_NONE, (int)FIN_ALWAYS, (int64_t)0);
 }
 
 void launch_multidot(hipStream_t s, int64_t n, int nseg, const double* w, const double* V, int64_t ldv, int nv,
                      double* part, double* res) {
-  DRD_SEG_LAUNCH(k_multidot, n, nseg, w, V, ldv, nv, part);
-  launch_finalize(s, part, tiles_of(n / nseg), nseg, nv, res, nullptr, PH_NONE, FIN_ALWAYS);
+  DRD_SEG_LAUNCH1(k_multidot, n, nseg, w, V, ldv, nv, part);
+  launch_finalize1(s, part, tiles_of(n / nseg), nseg, nv, res);
 }
 void launch_multiaxpy(hipStream_t s, int64_t n, int nseg, double* w, const double* V, int64_t ldv, int nv,
                       const double* h, double sign, double* part, double* res, int want_norm) {
-  DRD_SEG_LAUNCH(k_multiaxpy, n, nseg, w, V, ldv, nv, h, sign, part, want_norm);
-  if (want_norm) launch_finalize(s, part, tiles_of(n / nseg), nseg, 1, res, nullptr, PH_NONE, FIN_ALWAYS);
+  DRD_SEG_LAUNCH1(k_multiaxpy, n, nseg, w, V, ldv, nv, h, sign, part, want_norm);
+  if (want_norm) launch_finalize1(s, part, tiles_of(n / nseg), nseg, 1, res);
 }
 void launch_scale_into(hipStream_t s, int64_t n, const double* a, double sc, double* b) {
   hipLaunchKernelGGL(k_scale_into, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, sc, b);
@@ -291,7 +344,7 @@ void launch_init_norms(hipStream_t s, int64_t n, int nseg, const double* x, cons
   launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, nullptr, ks, PH_INIT_NORMS, FIN_ALWAYS);
 }
 void launch_bicg_p(hipStream_t s, int64_t n, const double* r, const double* v, double* p, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_p, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, r, v, p, ks);
+  hipLaunchKernelGGL(k_bicg_p, dim3(nb(n, kTile), 1, bat_n()), dim3(kBlock), 0, s, n, r, v, p, ks, bat_bs());
 }
 void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks) {
   DRD_SEG_LAUNCH(k_bicg_s, n, nseg, r, v, part, ks);
@@ -299,7 +352,7 @@ void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* 
 }
 void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
   (void)part;
-  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile)), dim3(kBlock), 0, s, n, x, ph, ks);
+  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile), 1, bat_n()), dim3(kBlock), 0, s, n, x, ph, ks, bat_bs());
 }
 void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, const double* rt, const double* ph,
                     const double* sh, const double* t, double* part, KState* ks, bool finish) {
@@ -307,13 +360,13 @@ void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, co
   if (finish) launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, nullptr, ks, PH_XR, FIN_ACTIVE);
 }
 void launch_copy(hipStream_t s, int64_t n, const double* a, double* b) {
-  hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
+  hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock), 1, bat_n()), dim3(kBlock), 0, s, n, a, b, bat_bs());
 }
 void launch_axpby(hipStream_t s, int64_t n, double alpha, const double* x, double beta, double* y) {
-  hipLaunchKernelGGL(k_axpby, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, alpha, x, beta, y);
+  hipLaunchKernelGGL(k_axpby, dim3(nb(n, kBlock), 1, bat_n()), dim3(kBlock), 0, s, n, alpha, x, beta, y, bat_bs());
 }
 void launch_cell_axpy(hipStream_t s, int64_t n, const double* a, double* b) {
-  hipLaunchKernelGGL(k_cell_axpy, dim3(nb(n, kBlock)), dim3(kBlock), 0, s, n, a, b);
+  hipLaunchKernelGGL(k_cell_axpy, dim3(nb(n, kBlock), 1, bat_n()), dim3(kBlock), 0, s, n, a, b, bat_bs());
 }
 void launch_resid_norms(hipStream_t s, int64_t n, int nseg, const double* b, const double* ax, double* part,
                         KState* ks) {

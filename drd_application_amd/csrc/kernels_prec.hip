@@ -62,8 +62,14 @@ __global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const dou
                                                        const double* __restrict__ minner,
                                                        const double* __restrict__ m0,
                                                        const double* __restrict__ b, double* __restrict__ x,
-                                                       const KState* st) {
+                                                       const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
+  bu = bofs(bu, bs);
+  minner = bofs(minner, bs);
+  m0 = bofs(m0, bs);
+  b = bofs(b, bs);
+  x = bofs(x, bs);
   const int64_t C = g.ncell();
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous rows
   if (q >= C) return;
@@ -79,8 +85,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p_block(Grid g, Scal s, const d
                                                          const double* __restrict__ m0,
                                                          const double* __restrict__ r,
                                                          const double* __restrict__ v, double* __restrict__ p,
-                                                         double* __restrict__ ph, const KState* st) {
+                                                         double* __restrict__ ph, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st->done || st->early) return;
+  bu = bofs(bu, bs);
+  minner = bofs(minner, bs);
+  m0 = bofs(m0, bs);
+  r = bofs(r, bs);
+  v = bofs(v, bs);
+  p = bofs(p, bs);
+  ph = bofs(ph, bs);
   const bool first = st->iter <= 1;
   const double omega = st->omega;
   const double beta = first ? 0.0 : (st->rho / st->rho1) * (st->alpha / st->omega);
@@ -112,8 +126,16 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s_block(Grid g, Scal s, const d
                                                          const double* __restrict__ minner,
                                                          const double* __restrict__ m0, double* __restrict__ r,
                                                          const double* __restrict__ v, double* __restrict__ sh,
-                                                         double* part, const KState* st) {
+                                                         double* part, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st->done || st->early) return;
+  bu = bofs(bu, bs);
+  minner = bofs(minner, bs);
+  m0 = bofs(m0, bs);
+  r = bofs(r, bs);
+  v = bofs(v, bs);
+  sh = bofs(sh, bs);
+  part = bofs(part, bs);
   const double alpha = st->alpha;
   const int64_t C = g.ncell();
   double t[2][kSlots];
@@ -139,21 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s_block(Grid g, Scal s, const d
 
 void launch_bicg_p_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* r,
                          const double* v, double* p, double* ph, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_p_block, dim3(tiles_of(g.ncell())), dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, r,
-                     v, p, ph, ks);
+  hipLaunchKernelGGL(k_bicg_p_block, dim3(tiles_of(g.ncell()), 1, bat_n()), dim3(kBlock), 0, st, g, s, c.bu,
+                     c.minner, c.m0, r, v, p, ph, ks, bat_bs());
 }
 
 void launch_bicg_s_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* r, const double* v,
                          double* sh, double* part, KState* ks) {
-  hipLaunchKernelGGL(k_bicg_s_block, dim3(tiles_of(g.ncell())), dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, r,
-                     v, sh, part, ks);
+  hipLaunchKernelGGL(k_bicg_s_block, dim3(tiles_of(g.ncell()), 1, bat_n()), dim3(kBlock), 0, st, g, s, c.bu,
+                     c.minner, c.m0, r, v, sh, part, ks, bat_bs());
   launch_finalize(st, part, tiles_of(g.ncell()), 2, 1, nullptr, ks, PH_S, FIN_ACTIVE);
 }
 
 void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
                        const KState* ks) {
-  hipLaunchKernelGGL(k_phmu_block, dim3((unsigned)((g.ncell() + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, g,
-                     s, c.bu, c.minner, c.m0, b, x, ks);
+  hipLaunchKernelGGL(k_phmu_block, dim3((unsigned)((g.ncell() + kBlock - 1) / kBlock), 1, bat_n()), dim3(kBlock),
+                     0, st, g, s, c.bu, c.minner, c.m0, b, x, ks, bat_bs());
 }
 
 // ---- exact diagonal of the BDF2 velocity operator and the scaling S ---------
@@ -162,7 +184,13 @@ void launch_phmu_block(hipStream_t st, const Grid& g, const Scal& s, const Coef&
 __global__ __launch_bounds__(kBlock) void k_velo_scale(Grid g, Scal s, const double* __restrict__ rho,
                                                        const double* __restrict__ eta,
                                                        const double* __restrict__ chi, double a0,
-                                                       double* __restrict__ inv_s64, float* __restrict__ inv_s32) {
+                                                       double* __restrict__ inv_s64, float* __restrict__ inv_s32,
+                                                       int64_t bs) {
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  chi = bofs(chi, bs);
+  inv_s64 = bofs(inv_s64, bs);
+  inv_s32 = bofs(inv_s32, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   if (q >= g.nuv()) return;
@@ -308,8 +336,14 @@ template <class T, bool FWD, int LMAX>
 __global__ __launch_bounds__(kBlock) void k_prec_rows(Grid g, PrecTables<T> tb, const double* __restrict__ r,
                                                       const T* __restrict__ inv_s, T* __restrict__ wu,
                                                       T* __restrict__ wv, double* __restrict__ z,
-                                                      const KState* st) {
+                                                      const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
+  r = bofs(r, bs);  // the tables tb are shared by the group
+  inv_s = bofs(inv_s, bs);
+  wu = bofs(wu, bs);
+  wv = bofs(wv, bs);
+  z = bofs(z, bs);
   extern __shared__ unsigned char smem_raw[];
   Cx<T>* A = reinterpret_cast<Cx<T>*>(smem_raw);
   const int nx = g.nx, ny = g.ny;
@@ -363,8 +397,11 @@ __global__ __launch_bounds__(kBlock) void k_prec_rows(Grid g, PrecTables<T> tb, 
 // behind the FFT line.
 template <class T, int LMAX>
 __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, T* __restrict__ tu,
-                                                      T* __restrict__ tv, const KState* st) {
+                                                      T* __restrict__ tv, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
+  tu = bofs(tu, bs);
+  tv = bofs(tv, bs);
   extern __shared__ unsigned char smem_raw[];
   const int nx = g.nx, ny = g.ny;
   Cx<T>* A = reinterpret_cast<Cx<T>*>(smem_raw);
@@ -411,8 +448,11 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_transposeT(int rows, int cols, const T* __restrict__ in,
-                                                       T* __restrict__ out, const KState* st) {
+                                                       T* __restrict__ out, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
+  in = bofs(in, bs);
+  out = bofs(out, bs);
   __shared__ T tile[32][33];
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -438,6 +478,7 @@ struct VeloPrec {
   double* inv_s64 = nullptr;
   float* inv_s32 = nullptr;
   void* work = nullptr;  // wu, wv, tu, tv in T
+  bool own_bufs = true;  // false: inv_s64, inv_s32, work belong to the caller
   double a0 = 0;
 };
 
@@ -529,7 +570,7 @@ int velo_prec_update(VeloPrec* pc, const Scal& s) {
 
 static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err) {
+VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err, const VeloPrecBufs* ext) {
   *err = 0;
   if (!pow2(g.nx) || !pow2(g.ny) || g.nx > 4096 || g.ny > 4096) {
     *err = DRD_E_UNSUPPORTED;
@@ -542,9 +583,16 @@ VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err) {
   int rc = fp32 ? build_twiddles<float>(pc, pc->tf) : build_twiddles<double>(pc, pc->td);
   if (!rc) rc = velo_prec_update(pc, s);
   const size_t esz = fp32 ? sizeof(float) : sizeof(double);
-  if (!rc && hipMalloc(&pc->inv_s64, g.nuv() * sizeof(double)) != hipSuccess) rc = DRD_E_HIP;
-  if (!rc && hipMalloc(&pc->inv_s32, g.nuv() * sizeof(float)) != hipSuccess) rc = DRD_E_HIP;
-  if (!rc && hipMalloc(&pc->work, 2 * g.nuv() * esz) != hipSuccess) rc = DRD_E_HIP;
+  if (ext) {
+    pc->inv_s64 = ext->inv_s64;
+    pc->inv_s32 = ext->inv_s32;
+    pc->work = ext->work;
+    pc->own_bufs = false;
+  } else {
+    if (!rc && hipMalloc(&pc->inv_s64, g.nuv() * sizeof(double)) != hipSuccess) rc = DRD_E_HIP;
+    if (!rc && hipMalloc(&pc->inv_s32, g.nuv() * sizeof(float)) != hipSuccess) rc = DRD_E_HIP;
+    if (!rc && hipMalloc(&pc->work, 2 * g.nuv() * esz) != hipSuccess) rc = DRD_E_HIP;
+  }
   if (rc) {
     velo_prec_destroy(pc);
     *err = rc;
@@ -557,15 +605,17 @@ void velo_prec_destroy(VeloPrec* pc) {
   if (!pc) return;
   (void)hipFree(pc->tabs_mem);
   (void)hipFree(pc->eig_mem);
-  (void)hipFree(pc->inv_s64);
-  (void)hipFree(pc->inv_s32);
-  (void)hipFree(pc->work);
+  if (pc->own_bufs) {
+    (void)hipFree(pc->inv_s64);
+    (void)hipFree(pc->inv_s32);
+    (void)hipFree(pc->work);
+  }
   delete pc;
 }
 
 void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s, const Coef& c) {
-  hipLaunchKernelGGL(k_velo_scale, dim3((unsigned)((g.nuv() + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, g, s,
-                     c.rho, c.eta, c.chi, pc->a0, pc->inv_s64, pc->inv_s32);
+  hipLaunchKernelGGL(k_velo_scale, dim3((unsigned)((g.nuv() + kBlock - 1) / kBlock), 1, bat_n()), dim3(kBlock), 0,
+                     st, g, s, c.rho, c.eta, c.chi, pc->a0, pc->inv_s64, pc->inv_s32, bat_bs());
 }
 
 template <class T, int LMAX>
@@ -583,19 +633,22 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
     (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     raised = true;
   }
-  hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks), dim3(kBlock), lds_rows, st, g, tb, r, inv_s,
-                     wu, wv, (double*)nullptr, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 1 + 31) / 32, (ny + 31) / 32), dim3(kBlock), 0, st, ny, nx + 1,
-                     (const T*)wu, tu, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 31) / 32, (ny - 1 + 31) / 32), dim3(kBlock), 0, st, ny - 1, nx,
-                     (const T*)wv, tv, ks);
-  hipLaunchKernelGGL((k_prec_cols<T, LMAX>), dim3(cols_blocks), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny + 31) / 32, (nx + 1 + 31) / 32), dim3(kBlock), 0, st, nx + 1, ny,
-                     (const T*)tu, wu, ks);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny - 1 + 31) / 32, (nx + 31) / 32), dim3(kBlock), 0, st, nx, ny - 1,
-                     (const T*)tv, wv, ks);
-  hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks), dim3(kBlock), lds_rows, st, g, tb,
-                     (const double*)nullptr, inv_s, wu, wv, z, ks);
+  const unsigned B = bat_n();
+  const int64_t bs = bat_bs();
+  hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb, r,
+                     inv_s, wu, wv, (double*)nullptr, ks, bs);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 1 + 31) / 32, (ny + 31) / 32, B), dim3(kBlock), 0, st, ny, nx + 1,
+                     (const T*)wu, tu, ks, bs);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 31) / 32, (ny - 1 + 31) / 32, B), dim3(kBlock), 0, st, ny - 1, nx,
+                     (const T*)wv, tv, ks, bs);
+  hipLaunchKernelGGL((k_prec_cols<T, LMAX>), dim3(cols_blocks, 1, B), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks,
+                     bs);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny + 31) / 32, (nx + 1 + 31) / 32, B), dim3(kBlock), 0, st, nx + 1, ny,
+                     (const T*)tu, wu, ks, bs);
+  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny - 1 + 31) / 32, (nx + 31) / 32, B), dim3(kBlock), 0, st, nx, ny - 1,
+                     (const T*)tv, wv, ks, bs);
+  hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb,
+                     (const double*)nullptr, inv_s, wu, wv, z, ks, bs);
 }
 
 template <class T>

@@ -105,8 +105,17 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
                                                         const double* __restrict__ x, double* __restrict__ y,
                                                         double alpha, double beta, double* __restrict__ a0,
                                                         const double* __restrict__ b, double* part,
-                                                        KState* st) {
+                                                        KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (MODE != MV_PLAIN && !kactive(st)) return;
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  chi = bofs(chi, bs);
+  x = bofs(x, bs);
+  y = bofs(y, bs);
+  a0 = bofs(a0, bs);
+  b = bofs(b, bs);
+  part = bofs(part, bs);
   VeloIn X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
   const uint32_t n = (uint32_t)g.nuv();
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
@@ -160,7 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
 // with the reference's rho(i-1,j) average on the v rows (:720).
 __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, const double* __restrict__ rho,
                                                              const double* __restrict__ eta,
-                                                             double* __restrict__ diag) {
+                                                             double* __restrict__ diag, int64_t bs) {
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  diag = bofs(diag, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   if (q >= g.nuv()) return;
@@ -201,15 +213,23 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
 // x_out = b_in / diag (any diagonal preconditioner)
 __global__ __launch_bounds__(kBlock) void k_div_diag(int64_t n, const double* __restrict__ b,
                                                      const double* __restrict__ d, double* __restrict__ x,
-                                                     const KState* st) {
+                                                     const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (!kactive(st)) return;
+  b = bofs(b, bs);
+  d = bofs(d, bs);
+  x = bofs(x, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q < n) x[q] = b[q] / d[q];
 }
 
 // Atob_uv(u, v -> x) TP/matrix_tool.f90:5-26
 __global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __restrict__ u,
-                                                      const double* __restrict__ v, double* __restrict__ x) {
+                                                      const double* __restrict__ v, double* __restrict__ x,
+                                                      int64_t bs) {
+  u = bofs(u, bs);
+  v = bofs(v, bs);
+  x = bofs(x, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= g.nuv()) return;
   const int nx = g.nx;
@@ -229,7 +249,11 @@ __global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __re
 // One thread per element of the u field and one per element of the v field.
 __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const double* __restrict__ x,
                                                         const double* __restrict__ uL,
-                                                        double* __restrict__ u, double* __restrict__ v) {
+                                                        double* __restrict__ u, double* __restrict__ v,
+                                                        int64_t bs) {
+  x = bofs(x, bs);  // uL (inflow profile) is shared by the group
+  u = bofs(u, bs);
+  v = bofs(v, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   const int64_t nu = g.nuf();
@@ -268,7 +292,23 @@ __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const do
 // VeloSolve_combined TP/VeloSolve_combined.f90:48-396 (BDF = 1)
 // VeloSolve_2_combined TP/VeloSolve_2_combined.f90:62-400 (BDF = 2)
 template <int BDF>
-__global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs a, double* __restrict__ rhs) {
+__global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs a, double* __restrict__ rhs,
+                                                     int64_t bs) {
+  a.u = bofs(a.u, bs);  // every field but the shared inflow profile uL
+  a.v = bofs(a.v, bs);
+  a.u_pre = bofs(a.u_pre, bs);
+  a.v_pre = bofs(a.v_pre, bs);
+  a.p = bofs(a.p, bs);
+  a.psi1 = bofs(a.psi1, bs);
+  a.psi2 = bofs(a.psi2, bs);
+  a.rho = bofs(a.rho, bs);
+  a.eta = bofs(a.eta, bs);
+  a.bu = bofs(a.bu, bs);
+  a.ph_after = bofs(a.ph_after, bs);
+  a.rmu_after = bofs(a.rmu_after, bs);
+  a.pf1 = bofs(a.pf1, bs);
+  a.pf2 = bofs(a.pf2, bs);
+  rhs = bofs(rhs, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= g.nuv()) return;
   const int nx = g.nx, ny = g.ny;
@@ -407,8 +447,17 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
                                                         const double* __restrict__ x, double* __restrict__ y,
                                                         double alpha, double beta, double* __restrict__ a0,
                                                         const double* __restrict__ b, double* part,
-                                                        KState* st) {
+                                                        KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (MODE != MV_PLAIN && !kactive(st)) return;
+  bu = bofs(bu, bs);
+  minner = bofs(minner, bs);
+  m0 = bofs(m0, bs);
+  x = bofs(x, bs);
+  y = bofs(y, bs);
+  a0 = bofs(a0, bs);
+  b = bofs(b, bs);
+  part = bofs(part, bs);
   const int64_t C = g.ncell();
   // terms tm[seg*2 + v]: segment 0 = phi rows (element q), 1 = mu rows (C + q)
   double tm[4][kSlots] = {};
@@ -465,8 +514,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_matvec(Grid g, Scal s, const do
 // PSolve_PH_MU_CG TP/ph_mu_solve.f90:319-353 (note the reference's
 // `j == nr-1` / `i == 0` tests, kept as written).
 __global__ __launch_bounds__(kBlock) void k_phmu_jacobi(Grid g, Scal s, const double* __restrict__ b,
-                                                        double* __restrict__ x, const KState* st) {
+                                                        double* __restrict__ x, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
   if (!kactive(st)) return;
+  b = bofs(b, bs);
+  x = bofs(x, bs);
   const int64_t C = g.ncell();
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= C) return;
@@ -492,7 +544,11 @@ __device__ __forceinline__ double ch_bdr(double cf, double phv, int mode) {
 
 // ph_M_0 = (1-bdr_ph) f(ph) over the whole field TP/ph_mu_solve.f90:31-35
 __global__ __launch_bounds__(kBlock) void k_phmu_m0(int64_t nc, int mode, const double* __restrict__ ph,
-                                                    const double* __restrict__ bdr, double* __restrict__ m0) {
+                                                    const double* __restrict__ bdr, double* __restrict__ m0,
+                                                    int64_t bs) {
+  ph = bofs(ph, bs);
+  bdr = bofs(bdr, bs);
+  m0 = bofs(m0, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= nc) return;
   const double p = ph[q];
@@ -506,7 +562,19 @@ __global__ __launch_bounds__(kBlock) void k_phmu_m0(int64_t nc, int mode, const 
 // initial guess (Atob_ph_mu of ph, rmu :152-153).  pn[0..nx) = pnb,
 // pn[nx..2nx) = pnt for the ghost rows written after the solve.
 __global__ __launch_bounds__(kBlock) void k_phmu_rhs(Grid g, Scal s, PhmuRhsArgs a, double* __restrict__ rhs,
-                                                     double* __restrict__ x0, double* __restrict__ pn) {
+                                                     double* __restrict__ x0, double* __restrict__ pn, int64_t bs) {
+  a.ph = bofs(a.ph, bs);  // every field and the per-instance wetting coefficients
+  a.rmu = bofs(a.rmu, bs);
+  a.cfB = bofs(a.cfB, bs);
+  a.cfT = bofs(a.cfT, bs);
+  a.u = bofs(a.u, bs);
+  a.v = bofs(a.v, bs);
+  a.bdr = bofs(a.bdr, bs);
+  a.bu = bofs(a.bu, bs);
+  a.bg = bofs(a.bg, bs);
+  rhs = bofs(rhs, bs);
+  x0 = bofs(x0, bs);
+  pn = bofs(pn, bs);
   const int64_t C = g.ncell();
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= C) return;
@@ -551,7 +619,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rhs(Grid g, Scal s, PhmuRhsArgs
 // btoA_ph_mu gave them (copied before the ghost-row override).
 __global__ __launch_bounds__(kBlock) void k_phmu_unpack(Grid g, Scal s, const double* __restrict__ x,
                                                         const double* __restrict__ pn, double* __restrict__ ph,
-                                                        double* __restrict__ mu) {
+                                                        double* __restrict__ mu, int64_t bs) {
+  x = bofs(x, bs);
+  pn = bofs(pn, bs);
+  ph = bofs(ph, bs);
+  mu = bofs(mu, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   if (q >= g.nc()) return;
@@ -577,10 +649,10 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
                         const double* x, double* y, double alpha, double beta, double* a0,
                         const double* b, double* part, KState* ks, bool finish) {
   const unsigned ntiles = nblk(g.nuv(), kTile);
-  dim3 grid((ntiles + 1) / 2);  // two tiles per workgroup
+  dim3 grid((ntiles + 1) / 2, 1, bat_n());  // two tiles per workgroup
 #define VM(B, M)                                                                                     \
   hipLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, y, \
-                     alpha, beta, a0, b, part, ks)
+                     alpha, beta, a0, b, part, ks, bat_bs())
   if (bdf == 1) {
     switch (mode) {
       case MV_PLAIN: VM(1, MV_PLAIN); break;
@@ -603,10 +675,10 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
                         const double* b, double* part, KState* ks, bool finish) {
-  dim3 grid(nblk(g.ncell(), kTile));
+  dim3 grid(nblk(g.ncell(), kTile), 1, bat_n());
 #define PM(M)                                                                                          \
   hipLaunchKernelGGL((k_phmu_matvec<M>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, x, y, alpha, \
-                     beta, a0, b, part, ks)
+                     beta, a0, b, part, ks, bat_bs())
   switch (mode) {
     case MV_PLAIN: PM(MV_PLAIN); break;
     case MV_DOT1: PM(MV_DOT1); break;
@@ -618,51 +690,58 @@ void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, 
 }
 
 void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag) {
-  hipLaunchKernelGGL(k_velo_jacobi_diag, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, c.rho,
-                     c.eta, diag);
+  hipLaunchKernelGGL(k_velo_jacobi_diag, dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s,
+                     c.rho, c.eta, diag, bat_bs());
 }
 
 void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,
                      const KState* ks) {
-  hipLaunchKernelGGL(k_div_diag, dim3(nblk(n, kBlock)), dim3(kBlock), 0, st, n, b, d, x, ks);
+  hipLaunchKernelGGL(k_div_diag, dim3(nblk(n, kBlock), 1, bat_n()), dim3(kBlock), 0, st, n, b, d, x, ks, bat_bs());
 }
 
 void launch_phmu_jacobi(hipStream_t st, const Grid& g, const Scal& s, const double* b, double* x,
                         const KState* ks) {
-  hipLaunchKernelGGL(k_phmu_jacobi, dim3(nblk(g.ncell(), kBlock)), dim3(kBlock), 0, st, g, s, b, x, ks);
+  hipLaunchKernelGGL(k_phmu_jacobi, dim3(nblk(g.ncell(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, b, x, ks,
+                     bat_bs());
 }
 
 void launch_velo_pack(hipStream_t st, const Grid& g, const double* u, const double* v, double* x) {
-  hipLaunchKernelGGL(k_velo_pack, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, u, v, x);
+  hipLaunchKernelGGL(k_velo_pack, dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, u, v, x,
+                     bat_bs());
 }
 
 void launch_velo_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* uL,
                         double* u, double* v) {
-  hipLaunchKernelGGL(k_velo_unpack, dim3(nblk(g.nuf() + g.nvf(), kBlock)), dim3(kBlock), 0, st, g, s, x,
-                     uL, u, v);
+  hipLaunchKernelGGL(k_velo_unpack, dim3(nblk(g.nuf() + g.nvf(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s,
+                     x, uL, u, v, bat_bs());
 }
 
 void launch_velo_rhs(hipStream_t st, int bdf, const Grid& g, const Scal& s, const VeloRhsArgs& a,
                      double* rhs) {
   if (bdf == 1)
-    hipLaunchKernelGGL(k_velo_rhs<1>, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs);
+    hipLaunchKernelGGL(k_velo_rhs<1>, dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, a, rhs,
+                       bat_bs());
   else
-    hipLaunchKernelGGL(k_velo_rhs<2>, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs);
+    hipLaunchKernelGGL(k_velo_rhs<2>, dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, a, rhs,
+                       bat_bs());
 }
 
 void launch_phmu_m0(hipStream_t st, const Grid& g, int mode, const double* ph, const double* bdr,
                     double* m0) {
-  hipLaunchKernelGGL(k_phmu_m0, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g.nc(), mode, ph, bdr, m0);
+  hipLaunchKernelGGL(k_phmu_m0, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g.nc(), mode, ph, bdr,
+                     m0, bat_bs());
 }
 
 void launch_phmu_rhs(hipStream_t st, const Grid& g, const Scal& s, const PhmuRhsArgs& a, double* rhs,
                      double* x0, double* pn) {
-  hipLaunchKernelGGL(k_phmu_rhs, dim3(nblk(g.ncell(), kBlock)), dim3(kBlock), 0, st, g, s, a, rhs, x0, pn);
+  hipLaunchKernelGGL(k_phmu_rhs, dim3(nblk(g.ncell(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, a, rhs, x0,
+                     pn, bat_bs());
 }
 
 void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* pn,
                         double* ph, double* mu) {
-  hipLaunchKernelGGL(k_phmu_unpack, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g, s, x, pn, ph, mu);
+  hipLaunchKernelGGL(k_phmu_unpack, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, x, pn, ph,
+                     mu, bat_bs());
 }
 
 }  // namespace drd

@@ -255,6 +255,40 @@ def velocity_system(n: int, seed: int, *, max_contrast: float = 1.0e3, n_discs: 
     return cfg, b
 
 
+def batch_configs(count: int = 256, seed: int = 2000, sizes=(128, 256, 512, 1024), r_ref: float = 1.0e6,
+                  dt: float = 5.0e-5, tol: float = 1.0e-10, steps: int = 20):
+    """The batched workload: `count` independent two-phase instances.
+
+    nx log-uniform over `sizes` (powers of two, because the fast-Poisson
+    preconditioner's FFTs are radix-2^k), ny = nx/2, height 1; eta ratio
+    uniform in [1, 100]; 0-8 seeded discs (radius 6-16 cells) whose
+    resistance R is log-uniform in [1e4, 1e6]; 2-12 droplets (radius 4-16
+    cells).  Every instance of one size shares its drd_params (penalty_k =
+    r_ref); an instance's own R enters through bdr_ph_penalty = (R / r_ref)
+    * bdr_ph.  FP32 velocity preconditioner, CH block Jacobi.  Returns a
+    list of Config (instance i seeded with seed + i)."""
+    out = []
+    lo, hi = math.log(sizes[0]), math.log(sizes[-1])
+    for i in range(count):
+        rng = SplitMix64(seed + i)
+        lx = lo + (hi - lo) * rng.uniform()
+        nx = min(sizes, key=lambda s: abs(math.log(s) - lx))
+        ny = nx // 2
+        lam = rng.uniform(1.0, 100.0)
+        n_discs = int(9 * rng.uniform())
+        n_drops = 2 + int(11 * rng.uniform())
+        r_pen = 10.0 ** rng.uniform(4.0, 6.0)
+        cfg = two_phase_config(nx, ny, seed * 7 + i, n_drops=n_drops, r_drop=(4, 16), n_discs=n_discs,
+                               r_disc=(6, 16), lambda_eta=lam, penalty=r_ref, dt=dt, tol=tol, steps=steps,
+                               name=f"b{i}")
+        cfg.params.velo_prec = abi.PREC_DCT32
+        cfg.params.ch_prec = abi.CH_PREC_BLOCK
+        cfg.fields[abi.F_BDR_PH_PENALTY] = cfg.fields[abi.F_BDR_PH_PENALTY] * (r_pen / r_ref)
+        cfg.notes = f"{nx}x{ny}, eta ratio {lam:.1f}, {n_discs} discs R={r_pen:.3g}, {n_drops} droplets"
+        out.append(cfg)
+    return out
+
+
 def config(name: str) -> Config:
     """Named workloads.
 

@@ -76,6 +76,12 @@ def lib():
         L.drd_learn_get_params.argtypes = [V, F, C.c_int32]
         L.drd_learn_set_params.argtypes = [V, F, C.c_int32]
         L.drd_unet_param_count.argtypes = []
+        L.drd_batch_create.argtypes = [C.c_int32, C.POINTER(abi.Params), C.POINTER(V)]
+        L.drd_batch_destroy.argtypes = [V]
+        L.drd_batch_set_field.argtypes = [V, C.c_int32, C.c_int32, P]
+        L.drd_batch_get_field.argtypes = [V, C.c_int32, C.c_int32, P]
+        L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
+        L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_unet_run.argtypes = [C.c_int32, C.c_int32, F, C.POINTER(C.c_uint16), F, F, F,
                                    C.POINTER(C.c_double), C.c_float, F]
         _LIB = L
@@ -310,3 +316,59 @@ class GpuSolver:
         n, ms, by = C.c_int32(0), C.c_double(0), C.c_double(0)
         _check(lib().drd_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(by)))
         return n.value, ms.value, by.value
+
+
+class GpuBatch:
+    """A batch of independent instances (drd_batch): equal-size instances
+    step together through batched launches."""
+
+    def __init__(self, params_list):
+        self.params = [q.copy() for q in params_list]
+        arr = (abi.Params * len(self.params))(*self.params)
+        self._h = C.c_void_p()
+        _check(lib().drd_batch_create(len(self.params), arr, C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().drd_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.params)
+
+    def set_field(self, inst: int, fid: int, a) -> None:
+        a = _f64(a)
+        if a.shape != abi.field_shape(self.params[inst], fid):
+            raise ValueError(f"field {abi.FIELD_NAMES.get(fid, fid)}: shape {a.shape}")
+        _check(lib().drd_batch_set_field(self._h, inst, fid, dp(a)))
+
+    def get_field(self, inst: int, fid: int) -> np.ndarray:
+        out = np.zeros(abi.field_shape(self.params[inst], fid))
+        _check(lib().drd_batch_get_field(self._h, inst, fid, dp(out)))
+        return out
+
+    def step(self, bdf, t):
+        """One step of every instance; a list of per-instance reports."""
+        infos = (abi.StepInfo * len(self.params))()
+        _check(lib().drd_batch_step(self._h, bdf, t, infos))
+        return [{"phmu": i.phmu.as_dict(), "velo": i.velo.as_dict(), "div_before": i.div_before} for i in infos]
+
+    def groups(self):
+        """[(nx, ny, instances)] of the batch's size groups."""
+        n = C.c_int32(0)
+        _check(lib().drd_batch_groups(self._h, C.byref(n), None, 0))
+        desc = (C.c_int32 * (3 * n.value))()
+        _check(lib().drd_batch_groups(self._h, C.byref(n), desc, n.value))
+        return [(desc[3 * g], desc[3 * g + 1], desc[3 * g + 2]) for g in range(n.value)]

@@ -232,6 +232,27 @@ int drd_fgmres(drd_ctx* ctx, int32_t op, int32_t prec, int32_t n,
 int drd_step(drd_ctx* ctx, int32_t bdf, double t, drd_step_info* info);
 int drd_synchronize(drd_ctx* ctx);
 
+/* ---- batches of independent instances (no reference counterpart) --------
+ * One drd_batch holds `count` independent instances (each its own grid,
+ * fields and solver state).  Instances of equal nx x ny form a group that
+ * steps together: every kernel of the step is one launch over all of the
+ * group's instances (the instance is the launch's z index), each instance
+ * keeping its own BiCGSTAB state machine, so finished instances idle while
+ * the others iterate.  Instances of one size must share every drd_params
+ * field; per-instance resistance goes into each instance's bdr_ph_penalty
+ * field (the operator uses penalty_k * bdr_ph_penalty).  BiCGSTAB only;
+ * velocity preconditioner DCT32, DCT64, Jacobi, identity or reference.     */
+typedef struct drd_batch drd_batch;
+int drd_batch_create(int32_t count, const drd_params* params, drd_batch** out);
+int drd_batch_destroy(drd_batch* b);
+/* one instance's field (layout as drd_set_field)                          */
+int drd_batch_set_field(drd_batch* b, int32_t inst, int32_t field, const double* host);
+int drd_batch_get_field(drd_batch* b, int32_t inst, int32_t field, double* host);
+/* drd_step for every instance; infos[count] receives each one's report     */
+int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info* infos);
+/* number of groups; desc[3g..3g+2] = nx, ny, instances of group g (g < cap) */
+int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, int32_t cap);
+
 /* ---- measurement hook (no reference counterpart) -----------------------
  * Launches one kernel of the solve path `reps` times on the context's
  * stream between HIP events and returns the mean launch duration and the
