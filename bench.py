@@ -241,6 +241,92 @@ def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: i
     }
 
 
+def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4):
+    """The batched workload: `count` independent instances of uneven sizes
+    (geometry.batch_configs) stepped together by drd_batch_step, every size
+    group's kernels launched once over all its instances, BiCGSTAB iteration
+    chunks replayed from captured hipGraphs, transpose tiles autotuned.  The
+    sequential figure steps up to `sample` instances of each size alone (each
+    its own context, no graphs) and scales their mean by the number of
+    instances of that size."""
+    import numpy as np
+
+    from drd_application_amd import geometry
+    from drd_application_amd.solver import GpuBatch, GpuSolver
+
+    t0 = time.perf_counter()
+    cfgs = geometry.batch_configs(count)
+    gen_s = time.perf_counter() - t0
+    cells = sum(c.params.nx * c.params.ny for c in cfgs)
+    dt = cfgs[0].params.dt
+    b = GpuBatch([c.params for c in cfgs])
+    for i, c in enumerate(cfgs):
+        for fid, a in c.fields.items():
+            b.set_field(i, fid, a)
+    tuned = b.tune()
+    log(f"[c2] {count} instances, {cells/1e6:.1f} M cells, groups {b.groups()} (generated in {gen_s:.1f} s); "
+        f"tiles {[(t[0], t[2]) for t in tuned]}")
+    t = 0.0
+    for k in range(warmup):
+        b.step(1 if k == 0 else 2, t)
+        t += dt
+    def timed(n):
+        nonlocal t
+        its_v, its_c, bad = 0, 0, 0
+        t1 = time.perf_counter()
+        for _ in range(n):
+            infos = b.step(2, t)
+            t += dt
+            its_v += sum(i["velo"]["iter"] for i in infos)
+            its_c += sum(i["phmu"]["iter"] for i in infos)
+            bad += sum(1 for i in infos if i["velo"]["info"] != 0 or i["phmu"]["info"] != 0)
+        return time.perf_counter() - t1, its_v / n / count, its_c / n / count, bad
+    el, iv, ic, bad = timed(steps)
+    b.use_graphs(False)
+    el_ng, _, _, _ = timed(max(1, steps - 1))
+    el_ng /= max(1, steps - 1)
+    b.close()
+    log(f"[c2] batched: {steps} steps in {el:.3f} s ({el/steps*1e3:.0f} ms/step, {count*steps/el:.0f} "
+        f"instance-steps/s); no graphs {el_ng*1e3:.0f} ms/step; mean its velo {iv:.1f}, CH {ic:.1f}; "
+        f"non-converged solves {bad}")
+    # sequential estimate: one instance per size, its own context
+    seq = 0.0
+    per_size = {}
+    for nx in sorted({c.params.nx for c in cfgs}):
+        group = [c for c in cfgs if c.params.nx == nx]
+        times = []
+        for c in group[:sample]:
+            g = GpuSolver(c.params)
+            geometry.load(g, c)
+            tt = 0.0
+            for k in range(warmup):
+                g.step(1 if k == 0 else 2, tt)
+                tt += dt
+            g.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                g.step(2, tt)
+                tt += dt
+            g.synchronize()
+            times.append((time.perf_counter() - t1) / steps * 1e3)
+            g.close()
+        ms = float(np.mean(times))
+        per_size[f"{nx}x{nx // 2}"] = {"instances": len(group), "sampled": len(times), "ms_per_step_alone": ms}
+        seq += ms * len(group)
+    log(f"[c2] sequential estimate {seq:.0f} ms/step ({per_size})")
+    return {
+        "workload": f"{count} instances, nx log-uniform over powers of two 128..1024, ny = nx/2, eta ratio "
+                    f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, FP32 preconditioner, "
+                    f"FP64 BiCGSTAB, tol 1e-10",
+        "cells": cells, "groups": [list(gr) for gr in tuned],
+        "ms_per_batch_step": el / steps * 1e3, "instance_steps_per_s": count * steps / el,
+        "cell_steps_per_s": cells * steps / el, "ms_per_batch_step_no_graphs": el_ng * 1e3,
+        "sequential_ms_per_step_estimate": seq, "speedup_vs_sequential": seq / (el / steps * 1e3),
+        "mean_iters": {"velo": iv, "phmu": ic}, "non_converged_solves": bad,
+        "per_size_alone": per_size,
+    }
+
+
 def measured_traffic(cfg_name: str):
     """HBM bytes per launch of the velocity matvec from the committed
     rocprofv3 counter summary for this workload, if there is one."""
@@ -293,6 +379,7 @@ def main(argv=None) -> int:
                     help="timed steps of the comparison modes (default: --steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the learned correction's own workload")
+    ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the stand-alone kernel loops on the config's fields (for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -332,6 +419,12 @@ def main(argv=None) -> int:
     modes = {args.mode: head}
     for m in [s for s in args.compare.split(",") if s and s != args.mode]:
         modes[m], _ = run_mode(cfg, m, args.compare_steps or args.steps, args.warmup)
+    c2 = None
+    if not args.no_c2:
+        try:
+            c2 = batched_c2()
+        except Exception as e:  # reported beside the metric, never required for it
+            log(f"batched workload failed: {e}")
     c3 = None
     if not args.no_c3:
         try:
@@ -373,6 +466,7 @@ def main(argv=None) -> int:
                      "launches_timed": tl["count"], "source": source},
         "cpu_baseline": cpu,
         "learned_c3": c3,
+        "batched_c2": c2,
     }
     print(json.dumps(result))
     return 0

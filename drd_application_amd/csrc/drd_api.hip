@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -82,6 +84,18 @@ static void learn_destroy(Learn* L) {
   delete L;
 }
 
+// A captured chunk of BiCGSTAB iterations is valid for one operator,
+// preconditioner, chunk length and pair of vectors (the kernel arguments);
+// parameter changes and preconditioner rebuilds drop every graph.
+struct GraphKey {
+  int op, prec, chunk;
+  const double* b;
+  const double* x;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(op, prec, chunk, b, x) < std::tie(o.op, o.prec, o.chunk, o.b, o.x);
+  }
+};
+
 struct drd_ctx {
   drd_params p;
   Grid g;
@@ -94,6 +108,9 @@ struct drd_ctx {
   void* slab = nullptr;
   std::vector<drd_solve_info> last;  // per instance: the latest BiCGSTAB solve
   std::vector<double> last_div;      // per instance: the latest mean divergence
+  // captured BiCGSTAB iteration chunks (drd_use_graphs)
+  bool use_graphs = false;
+  std::map<GraphKey, hipGraphExec_t> graphs;
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
   std::vector<hipEvent_t> tm_ev;
@@ -400,7 +417,13 @@ static int ctx_alloc(drd_ctx* c) {
   return 0;
 }
 
+static void drop_graphs(drd_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
 static void ctx_free(drd_ctx* c) {
+  drop_graphs(c);
   for (auto e : c->tm_ev) hipEventDestroy(e);
   c->tm_ev.clear();
   hipFree(c->slab);
@@ -441,6 +464,7 @@ static int eager_vprec(drd_ctx* c) {
   if (vp != DRD_PREC_DCT32 && vp != DRD_PREC_DCT64 && vp != DRD_PREC_LEARNED) return 0;
   const bool fp32 = vp != DRD_PREC_DCT64;
   if (c->vprec && c->vprec_fp32 == fp32) return 0;
+  drop_graphs(c);
   velo_prec_destroy(c->vprec);
   c->vprec = nullptr;
   int err = 0;
@@ -486,6 +510,14 @@ static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
 
 extern "C" int drd_init(const drd_params* prm, drd_ctx** out) { return ctx_create(prm, 1, out); }
 
+extern "C" int drd_use_graphs(drd_ctx* c, int32_t on) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!on) drop_graphs(c);
+  c->use_graphs = on != 0;
+  return 0;
+}
+
 extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   if (!c || !prm) return set_err(DRD_E_ARG, "null argument");
   if (prm->nx != c->p.nx || prm->ny != c->p.ny) return set_err(DRD_E_ARG, "grid size differs from the context");
@@ -495,6 +527,7 @@ extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   if (prm->variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "only the TP variant is built");
   if (std::memcmp(prm, &c->p, sizeof(drd_params)) == 0) return 0;
   HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->p = *prm;
   c->s = make_scal(*prm);
   hipFree(c->pre_lcc);
@@ -624,6 +657,7 @@ static int prec_setup(drd_ctx* c, int prec) {
     const bool fp32 = (prec != DRD_PS_VELO_DCT64);
     if (c->vprec && c->vprec_fp32 != fp32) {
       HIPCHK(hipStreamSynchronize(c->stream));
+      drop_graphs(c);
       velo_prec_destroy(c->vprec);
       c->vprec = nullptr;
     }
@@ -776,9 +810,26 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   const bool fused_block = (op == DRD_OP_PHMU && prec == DRD_PS_PHMU_BLOCK);
   launch_init_norms(c->stream, n, ns, x, b, c->part, c->ks);
   RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
+  // chunks of iterations are queued without host round trips; with graphs
+  // on, a chunk of length >= 2 is one replay of a captured hipGraph
+  const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
+                         (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
+                          prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64);
   int chunk = 1, total = 0, next_snap = 0;
   for (;;) {
-    for (int it = 0; it < chunk; ++it) {
+    hipGraphExec_t gx = nullptr;
+    bool capturing = false;
+    if (graphable && chunk >= 2) {
+      const GraphKey key{op, prec, chunk, b, x};
+      auto hit = c->graphs.find(key);
+      if (hit != c->graphs.end()) {
+        gx = hit->second;
+      } else {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        capturing = true;
+      }
+    }
+    for (int it = 0; it < (gx ? 0 : chunk); ++it) {
       const double* ph = nullptr;
       const double* sh = nullptr;
       if (fused_block) {  // p, phat = B^-1 p in one sweep
@@ -805,6 +856,15 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
         ++next_snap;
       }
     }
+    if (capturing) {
+      hipGraph_t gr = nullptr;
+      HIPCHK(hipStreamEndCapture(c->stream, &gr));
+      const hipError_t e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      HIPCHK(e);
+      c->graphs[GraphKey{op, prec, chunk, b, x}] = gx;
+    }
+    if (gx) HIPCHK(hipGraphLaunch(gx, c->stream));
     HIPCHK(hipGetLastError());
     total += chunk;
     RCHK(read_state(c));
@@ -1837,6 +1897,7 @@ extern "C" int drd_batch_create(int32_t count, const drd_params* params, drd_bat
       drd_batch_destroy(b);
       return rc;
     }
+    c->use_graphs = true;
     b->groups.push_back(c);
     b->infos.emplace_back(counts[gI]);
   }
@@ -1873,6 +1934,43 @@ extern "C" int drd_batch_get_field(drd_batch* b, int32_t inst, int32_t field, do
   if (!host) return set_err(DRD_E_ARG, "null host array");
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// The tile autotuner: times the velocity preconditioner's transpose tile
+// shapes on the context's current fields (all instances of its group) and
+// keeps the fastest; captured graphs are dropped (their kernels change).
+static int tune_ctx(drd_ctx* c, int32_t* shape, double* ms) {
+  if (!c->vprec) {
+    *shape = -1;
+    return 0;
+  }
+  RCHK(prec_setup(c, c->vprec_fp32 ? DRD_PS_VELO_DCT32 : DRD_PS_VELO_DCT64));
+  const int r = velo_prec_tune(c->stream, c->vprec, c->g, c->kw[0], c->kw[1], 5, ms);
+  if (r < 0) return set_err(DRD_E_HIP, "autotune failed");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  *shape = r;
+  return 0;
+}
+
+extern "C" int drd_tune(drd_ctx* c, int32_t* shape, double* ms) {
+  if (!c || !shape || !ms) return set_err(DRD_E_ARG, "null argument");
+  return tune_ctx(c, shape, ms);
+}
+
+extern "C" int drd_batch_tune(drd_batch* b, int32_t* shapes, double* ms, int32_t cap) {
+  if (!b || !shapes || !ms) return set_err(DRD_E_ARG, "null argument");
+  for (int gI = 0; gI < (int)b->groups.size() && gI < cap; ++gI) {
+    BatchScope scope(b->groups[gI]);
+    RCHK(tune_ctx(b->groups[gI], &shapes[gI], &ms[3 * gI]));
+  }
+  return 0;
+}
+
+extern "C" int drd_batch_use_graphs(drd_batch* b, int32_t on) {
+  if (!b) return set_err(DRD_E_ARG, "null batch");
+  for (drd_ctx* c : b->groups) RCHK(drd_use_graphs(c, on));
   return 0;
 }
 

@@ -188,6 +188,9 @@ void velo_prec_destroy(VeloPrec* pc);
 // diagonal scaling from the current coefficients (once per solve)
 void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s, const Coef& c);
 void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks);
+// picks the fastest transpose tile shape (32x32, 16x64, 64x16) by timing;
+// ms[3] receives the mean time per application of each
+int velo_prec_tune(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, int reps, double* ms);
 
 // ---- PhSolve, VCOST and ESOLP (kernels_phsolve.hip) ----
 struct Dct1Tab;

@@ -446,24 +446,48 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
   }
 }
 
-template <class T>
+// out (cols x rows) = transpose of in (rows x cols) through a TR x TC LDS
+// tile (TR rows of in, TC columns); the tile shape is the autotuner's choice.
+template <class T, int TR, int TC>
 __global__ __launch_bounds__(kBlock) void k_transposeT(int rows, int cols, const T* __restrict__ in,
                                                        T* __restrict__ out, const KState* st, int64_t bs) {
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
   in = bofs(in, bs);
   out = bofs(out, bs);
-  __shared__ T tile[32][33];
-  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int j = ty; j < 32; j += 8) {
-    const int rr = by + j, c = bx + tx;
-    if (rr < rows && c < cols) tile[j][tx] = in[(size_t)rr * cols + c];
+  __shared__ T tile[TR][TC + 1];
+  const int bx = blockIdx.x * TC, by = blockIdx.y * TR;
+  {
+    const int tx = threadIdx.x % TC;
+    for (int j = threadIdx.x / TC; j < TR; j += kBlock / TC) {
+      const int rr = by + j, c = bx + tx;
+      if (rr < rows && c < cols) tile[j][tx] = in[(size_t)rr * cols + c];
+    }
   }
   __syncthreads();
-  for (int j = ty; j < 32; j += 8) {
-    const int rr = bx + j, c = by + tx;
-    if (rr < cols && c < rows) out[(size_t)rr * rows + c] = tile[tx][j];
+  {
+    const int tx = threadIdx.x % TR;
+    for (int j = threadIdx.x / TR; j < TC; j += kBlock / TR) {
+      const int rr = bx + j, c = by + tx;  // out row = in column
+      if (rr < cols && c < rows) out[(size_t)rr * rows + c] = tile[tx][j];
+    }
+  }
+}
+
+// tile shapes the autotuner chooses from: TR x TC
+constexpr int kTransposeShapes = 3;
+constexpr int kShapeTR[kTransposeShapes] = {32, 16, 64};
+constexpr int kShapeTC[kTransposeShapes] = {32, 64, 16};
+
+template <class T>
+static void launch_transposeT(hipStream_t st, int shape, int rows, int cols, const T* in, T* out, const KState* ks,
+                              unsigned B, int64_t bs) {
+  const int TR = kShapeTR[shape], TC = kShapeTC[shape];
+  const dim3 grid((cols + TC - 1) / TC, (rows + TR - 1) / TR, B);
+  switch (shape) {
+    case 1: hipLaunchKernelGGL((k_transposeT<T, 16, 64>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
+    case 2: hipLaunchKernelGGL((k_transposeT<T, 64, 16>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
+    default: hipLaunchKernelGGL((k_transposeT<T, 32, 32>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
   }
 }
 
@@ -479,6 +503,7 @@ struct VeloPrec {
   float* inv_s32 = nullptr;
   void* work = nullptr;  // wu, wv, tu, tv in T
   bool own_bufs = true;  // false: inv_s64, inv_s32, work belong to the caller
+  int tshape = 0;        // transpose tile shape (velo_prec_tune)
   double a0 = 0;
 };
 
@@ -621,7 +646,7 @@ void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s,
 template <class T, int LMAX>
 static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T>& tb, const T* inv_s,
                                const double* r, double* z, T* wu, T* wv, T* tu, T* tv, unsigned rows_blocks,
-                               unsigned cols_blocks, size_t lds_rows, size_t lds_cols, const KState* ks) {
+                               unsigned cols_blocks, size_t lds_rows, size_t lds_cols, const KState* ks, int shape) {
   const int nx = g.nx, ny = g.ny;
   static bool raised = false;  // lines of 2nx or 2ny complex may exceed 64 KiB
   if (!raised) {
@@ -637,16 +662,12 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
   const int64_t bs = bat_bs();
   hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb, r,
                      inv_s, wu, wv, (double*)nullptr, ks, bs);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 1 + 31) / 32, (ny + 31) / 32, B), dim3(kBlock), 0, st, ny, nx + 1,
-                     (const T*)wu, tu, ks, bs);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((nx + 31) / 32, (ny - 1 + 31) / 32, B), dim3(kBlock), 0, st, ny - 1, nx,
-                     (const T*)wv, tv, ks, bs);
+  launch_transposeT<T>(st, shape, ny, nx + 1, wu, tu, ks, B, bs);
+  launch_transposeT<T>(st, shape, ny - 1, nx, wv, tv, ks, B, bs);
   hipLaunchKernelGGL((k_prec_cols<T, LMAX>), dim3(cols_blocks, 1, B), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks,
                      bs);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny + 31) / 32, (nx + 1 + 31) / 32, B), dim3(kBlock), 0, st, nx + 1, ny,
-                     (const T*)tu, wu, ks, bs);
-  hipLaunchKernelGGL(k_transposeT<T>, dim3((ny - 1 + 31) / 32, (nx + 31) / 32, B), dim3(kBlock), 0, st, nx, ny - 1,
-                     (const T*)tv, wv, ks, bs);
+  launch_transposeT<T>(st, shape, nx + 1, ny, tu, wu, ks, B, bs);
+  launch_transposeT<T>(st, shape, nx, ny - 1, tv, wv, ks, B, bs);
   hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb,
                      (const double*)nullptr, inv_s, wu, wv, z, ks, bs);
 }
@@ -668,10 +689,10 @@ static void apply_T(hipStream_t st, VeloPrec* pc, const Grid& g, const PrecTable
   const size_t lds_cols = 2 * (size_t)ny * sizeof(Cx<T>) + 2 * (size_t)ny * sizeof(T);
   if (std::max(nx, ny) <= 2048)
     launch_prec_passes<T, 4096>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
-                                lds_cols, ks);
+                                lds_cols, ks, pc->tshape);
   else
     launch_prec_passes<T, 8192>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
-                                lds_cols, ks);
+                                lds_cols, ks, pc->tshape);
 }
 
 void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks) {
@@ -679,6 +700,31 @@ void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* 
     apply_T<float>(st, pc, g, pc->tf, pc->inv_s32, r, z, ks);
   else
     apply_T<double>(st, pc, g, pc->td, pc->inv_s64, r, z, ks);
+}
+
+// Autotuner: times `reps` applications with each transpose tile shape (after
+// one warm-up each) and keeps the fastest.  ms[kTransposeShapes] gets the
+// mean times; returns the chosen shape (negative on a HIP error).
+int velo_prec_tune(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, int reps, double* ms) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  int best = 0;
+  for (int k = 0; k < kTransposeShapes; ++k) {
+    pc->tshape = k;
+    velo_prec_apply(st, pc, g, r, z, nullptr);
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < reps; ++i) velo_prec_apply(st, pc, g, r, z, nullptr);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    ms[k] = (double)t / reps;
+    if (ms[k] < ms[best]) best = k;
+  }
+  pc->tshape = best;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return hipGetLastError() == hipSuccess ? best : -1;
 }
 
 }  // namespace drd

@@ -82,6 +82,10 @@ def lib():
         L.drd_batch_get_field.argtypes = [V, C.c_int32, C.c_int32, P]
         L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
+        L.drd_use_graphs.argtypes = [V, C.c_int32]
+        L.drd_tune.argtypes = [V, I, C.POINTER(C.c_double)]
+        L.drd_batch_tune.argtypes = [V, I, C.POINTER(C.c_double), C.c_int32]
+        L.drd_batch_use_graphs.argtypes = [V, C.c_int32]
         L.drd_unet_run.argtypes = [C.c_int32, C.c_int32, F, C.POINTER(C.c_uint16), F, F, F,
                                    C.POINTER(C.c_double), C.c_float, F]
         _LIB = L
@@ -274,6 +278,17 @@ class GpuSolver:
         following solves with HIP events (0 disables)."""
         _check(lib().drd_kernel_timer(self._h, which, capacity))
 
+    def use_graphs(self, on=True):
+        """Capture BiCGSTAB iteration chunks as hipGraphs and replay them."""
+        _check(lib().drd_use_graphs(self._h, 1 if on else 0))
+
+    def tune(self):
+        """(chosen transpose tile shape, [ms per preconditioner application per shape])."""
+        sh = C.c_int32(0)
+        ms = (C.c_double * 3)()
+        _check(lib().drd_tune(self._h, C.byref(sh), ms))
+        return sh.value, list(ms)
+
     # ---- learned correction of the velocity preconditioner ------------------
     def learn_init(self, capacity, crop_h, crop_w, seed=1):
         _check(lib().drd_learn_init(self._h, capacity, crop_h, crop_w, seed))
@@ -364,6 +379,17 @@ class GpuBatch:
         infos = (abi.StepInfo * len(self.params))()
         _check(lib().drd_batch_step(self._h, bdf, t, infos))
         return [{"phmu": i.phmu.as_dict(), "velo": i.velo.as_dict(), "div_before": i.div_before} for i in infos]
+
+    def use_graphs(self, on=True):
+        _check(lib().drd_batch_use_graphs(self._h, 1 if on else 0))
+
+    def tune(self):
+        """[(nx, ny, chosen tile shape, [ms per shape])] for every size group."""
+        grp = self.groups()
+        sh = (C.c_int32 * len(grp))()
+        ms = (C.c_double * (3 * len(grp)))()
+        _check(lib().drd_batch_tune(self._h, sh, ms, len(grp)))
+        return [(g[0], g[1], sh[i], list(ms[3 * i:3 * i + 3])) for i, g in enumerate(grp)]
 
     def groups(self):
         """[(nx, ny, instances)] of the batch's size groups."""

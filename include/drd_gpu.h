@@ -253,6 +253,26 @@ int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info* infos);
 /* number of groups; desc[3g..3g+2] = nx, ny, instances of group g (g < cap) */
 int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, int32_t cap);
 
+/* ---- captured iteration chunks (no reference counterpart) ----------------
+ * With graphs on, BiCGSTAB queues its iterations in chunks (1, 2, 4, ... 32
+ * between polls of the state machine) and each chunk of two or more is a
+ * hipGraph captured on first use and replayed afterwards (one graph per
+ * operator, preconditioner, chunk length and vector pair; kernels only).
+ * Results are bit for bit those without graphs.  Off by default for
+ * contexts, on by default for batches.                                     */
+int drd_use_graphs(drd_ctx* ctx, int32_t on);
+int drd_batch_use_graphs(drd_batch* b, int32_t on);
+
+/* ---- tile autotuner (no reference counterpart) ----------------------------
+ * Times the DCT/DST velocity preconditioner with each transpose tile shape
+ * (0: 32x32, 1: 16x64, 2: 64x16 elements) on the current fields and keeps
+ * the fastest (results do not depend on the shape).  *shape = -1 when the
+ * context has no DCT preconditioner; ms[3] receives the mean time per
+ * application of each shape.  The batch form tunes every size group
+ * (shapes[g], ms[3g..3g+2] for g < cap).                                     */
+int drd_tune(drd_ctx* ctx, int32_t* shape, double* ms);
+int drd_batch_tune(drd_batch* b, int32_t* shapes, double* ms, int32_t cap);
+
 /* ---- measurement hook (no reference counterpart) -----------------------
  * Launches one kernel of the solve path `reps` times on the context's
  * stream between HIP events and returns the mean launch duration and the

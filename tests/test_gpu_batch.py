@@ -1,9 +1,12 @@
-"""Batched stepping (drd_batch) on the GPU.
+"""Batched stepping (drd_batch) and captured iteration chunks on the GPU.
 
 Every instance of a batch runs the same kernels as a single-instance context,
 with its own state machine and its own canonical reductions, so each
 instance's fields and Krylov reports must be bit for bit those of the same
-instance stepped alone (drd_init/drd_step).
+instance stepped alone (drd_init/drd_step).  Batches replay captured hipGraph
+chunks by default; the single-instance runs they are compared with do not,
+and a single context with graphs on is compared step by step with one
+without (every replay after eager work and synchronisation).
 """
 import numpy as np
 import pytest
@@ -46,6 +49,9 @@ def test_batch_steps_are_bitwise_the_single_instance_steps(steps):
     print("batch of", [c.notes for c in cfgs], flush=True)
     with GpuBatch([c.params for c in cfgs]) as b:
         _load_batch(b, cfgs)
+        tuned = b.tune()
+        print("tuned", tuned, flush=True)
+        assert all(0 <= t[2] < 3 and all(m > 0 for m in t[3]) for t in tuned)
         binfo = []
         for k in range(steps):
             binfo.append(b.step(1 if k == 0 else 2, k * cfgs[0].params.dt))
@@ -89,3 +95,24 @@ def test_batch_rejects_mixed_parameters_and_nonlinear_preconditioners():
         q.velo_prec = abi.PREC_LEARNED
     with pytest.raises(DrdError):
         GpuBatch(ps)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_DCT32, abi.PREC_JACOBI])
+def test_graph_replays_match_eager_steps(prec):
+    from drd_application_amd.solver import GpuSolver
+    cfg = geometry.config("small")
+    p = cfg.params.copy()
+    p.velo_prec = prec
+    p.ch_prec = abi.CH_PREC_BLOCK
+    a, b = GpuSolver(p), GpuSolver(p)
+    geometry.load(a, cfg)
+    geometry.load(b, cfg)
+    b.use_graphs(True)
+    for k in range(6):
+        sa = a.step(1 if k == 0 else 2, k * p.dt)
+        sb = b.step(1 if k == 0 else 2, k * p.dt)
+        assert sa == sb, k
+        for fid in FIELDS:
+            np.testing.assert_array_equal(a.get_field(fid), b.get_field(fid), err_msg=f"step {k} field {fid}")
+    a.close()
+    b.close()
