@@ -355,7 +355,7 @@ def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     geometry.load(g, cfg)
     g.step(1, 0.0)
     out = {}
-    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy")):
+    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), (abi.K_VELO_PREC, "velo_prec")):
         ms, by = g.bench_kernel(which, reps)
         out[name] = {"us": ms * 1e3, "bytes": by, "launches": reps + 1}
         log(f"{name}: {ms*1e3:.1f} us, {by/1e6:.1f} MB algorithmic")
@@ -411,7 +411,7 @@ def main(argv=None) -> int:
     # stand-alone timings of other kernels on the final state (same launch shapes)
     kern = {}
     for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
-                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure")):
+                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), (abi.K_VELO_PREC, "velo_prec")):
         ms, by = g.bench_kernel(which, 20)
         kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
         log(f"{name} (stand-alone): {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")

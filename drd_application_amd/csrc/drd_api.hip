@@ -1751,6 +1751,9 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       case DRD_K_UNET_FWD:
         (void)unet_forward(c->stream, c->learn->net, c->learn->img, c->learn->out);
         break;
+      case DRD_K_VELO_PREC:
+        velo_prec_apply(c->stream, c->vprec, g, c->kw[2], c->kw[3], nullptr);
+        break;
     }
   };
   switch (which) {
@@ -1772,6 +1775,11 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
     case DRD_K_UNET_FWD:  // MFMA FLOPs, not bytes
       if (!c->learn) return set_err(DRD_E_STATE, "drd_learn_init has not been called");
       *bytes = unet_forward_flops(c->learn->net);
+      break;
+    case DRD_K_VELO_PREC:  // r in, z out (FP64), scaling in (FP32 or FP64)
+      if (!c->vprec) return set_err(DRD_E_STATE, "no DCT velocity preconditioner in this context");
+      RCHK(prec_setup(c, c->vprec_fp32 ? DRD_PS_VELO_DCT32 : DRD_PS_VELO_DCT64));
+      *bytes = 2 * N8 + (c->vprec_fp32 ? N8 / 2 : N8);
       break;
     default:
       return set_err(DRD_E_ARG, "unknown kernel id");

@@ -19,6 +19,13 @@
 
 namespace drd {
 
+// LDS position of line element i: one pad element after every 16, so the
+// strided stores of a pass (element stride R or 16 Ns) fall into different
+// banks while unit-stride accesses stay contiguous.  A line of L elements
+// occupies lds_len(L) slots.
+__host__ __device__ __forceinline__ constexpr int lp(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ constexpr int lds_len(int L) { return lp(L); }
+
 // cos(j pi / 8) and sin(j pi / 8), j = 0..7: the twiddles inside the
 // radix-2..16 butterflies (2 pi k / m with m | 16).
 __device__ __forceinline__ constexpr double cos8(int j) {
@@ -98,7 +105,7 @@ __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<
     if (j < groups) {
       const int k = j & (Ns - 1);
 #pragma unroll
-      for (int t = 0; t < R; ++t) v[g][t] = A[j + t * groups];
+      for (int t = 0; t < R; ++t) v[g][t] = A[lp(j + t * groups)];
       if (Ns > 1) {  // w^t by repeated multiplication from one table entry
         Cx<T> w1 = tw[k * (L / (Ns * R))];
         if (sign > T(0)) w1.y = -w1.y;
@@ -120,14 +127,15 @@ __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<
       const int k = j & (Ns - 1);
       const int o = (j - k) * R + k;
 #pragma unroll
-      for (int t = 0; t < R; ++t) A[o + t * Ns] = v[g][t];
+      for (int t = 0; t < R; ++t) A[lp(o + t * Ns)] = v[g][t];
     }
   }
   __syncthreads();
 }
 
 // Complex FFT of the LDS line A (length L, power of two, 1 <= L <= LMAX <=
-// 8192) in place; the caller has synchronised after filling A.  LMAX fixes
+// 8192, element i at A[lp(i)]) in place; the caller has synchronised after
+// filling A.  LMAX fixes
 // how many groups a thread holds per pass (1 for LMAX = 4096).
 template <class T, int LMAX>
 __device__ void fft_lds(Cx<T>* A, int L, const Cx<T>* __restrict__ tw, T sign) {

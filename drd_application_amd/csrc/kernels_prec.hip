@@ -243,7 +243,8 @@ struct PrecTables {
 // Two real lines share one complex FFT (line 0 in the real part, line 1 in
 // the imaginary part), FFTW conventions, unnormalised.  load(i) returns the
 // pair's i-th inputs; store(i, y0, y1) takes the outputs.  A is the LDS
-// line (FFT length complex values); every helper ends with a barrier.
+// line (FFT length complex values, element i at A[lp(i)]); every helper
+// ends with a barrier.
 __device__ __forceinline__ int makhoul_ix(int j, int n) { return (j & 1) ? (n - 1 - (j >> 1)) : (j >> 1); }
 
 // REDFT00 of n = N+1 points: FFT of the even extension (length 2N); both
@@ -251,10 +252,10 @@ __device__ __forceinline__ int makhoul_ix(int j, int n) { return (j & 1) ? (n - 
 template <class T, int LMAX, class Load, class Store>
 __device__ void dct1_pair(Cx<T>* A, int N, const Cx<T>* __restrict__ tw, Load load, Store store) {
   const int L = 2 * N;
-  for (int m = threadIdx.x; m < L; m += blockDim.x) A[m] = load(m <= N ? m : L - m);
+  for (int m = threadIdx.x; m < L; m += blockDim.x) A[lp(m)] = load(m <= N ? m : L - m);
   __syncthreads();
   fft_lds<T, LMAX>(A, L, tw, T(-1));
-  for (int k = threadIdx.x; k <= N; k += blockDim.x) store(k, A[k].x, A[k].y);
+  for (int k = threadIdx.x; k <= N; k += blockDim.x) store(k, A[lp(k)].x, A[lp(k)].y);
   __syncthreads();
 }
 
@@ -268,12 +269,12 @@ __device__ void dct2_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     Cx<T> v = load(j);
     if (dst && (j & 1)) v = Cx<T>{-v.x, -v.y};
-    A[makhoul_ix(j, n)] = v;
+    A[lp(makhoul_ix(j, n))] = v;
   }
   __syncthreads();
   fft_lds<T, LMAX>(A, n, tw, T(-1));
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const Cx<T> V = A[k], W = A[(n - k) & (n - 1)];
+    const Cx<T> V = A[lp(k)], W = A[lp((n - k) & (n - 1))];
     const Cx<T> sp{V.x + W.x, V.y - W.y}, sm{V.x - W.x, V.y + W.y};  // V +- conj W
     const Cx<T> r = rot[k];
     store(dst ? n - 1 - k : k, r.x * sp.x - r.y * sp.y, r.x * sm.y + r.y * sm.x);
@@ -297,12 +298,12 @@ __device__ void dct3_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
     const Cx<T> r = rot[k];
     const Cx<T> a0{r.x * a.x + r.y * b.x, -r.y * a.x + r.x * b.x};  // line 0
     const Cx<T> a1{r.x * a.y + r.y * b.y, -r.y * a.y + r.x * b.y};  // line 1
-    A[k] = Cx<T>{a0.x - a1.y, a0.y + a1.x};
+    A[lp(k)] = Cx<T>{a0.x - a1.y, a0.y + a1.x};
   }
   __syncthreads();
   fft_lds<T, LMAX>(A, n, tw, T(1));
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    Cx<T> v = A[makhoul_ix(j, n)];
+    Cx<T> v = A[lp(makhoul_ix(j, n))];
     if (dst && (j & 1)) v = Cx<T>{-v.x, -v.y};
     store(j, v.x, v.y);
   }
@@ -321,11 +322,11 @@ __device__ void dst1_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, Load lo
       const Cx<T> t = load(L - m - 1);
       v = Cx<T>{-t.x, -t.y};
     }
-    A[m] = v;
+    A[lp(m)] = v;
   }
   __syncthreads();
   fft_lds<T, LMAX>(A, L, tw, T(-1));
-  for (int k = threadIdx.x; k < n; k += blockDim.x) store(k, -A[k + 1].y, A[k + 1].x);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) store(k, -A[lp(k + 1)].y, A[lp(k + 1)].x);
   __syncthreads();
 }
 
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
   extern __shared__ unsigned char smem_raw[];
   const int nx = g.nx, ny = g.ny;
   Cx<T>* A = reinterpret_cast<Cx<T>*>(smem_raw);
-  T* l0 = reinterpret_cast<T*>(A + 2 * ny);
+  T* l0 = reinterpret_cast<T*>(A + lds_len(2 * ny));
   T* l1 = l0 + ny;
   const int nu = (nx + 2) / 2;
   if ((int)blockIdx.x < nu) {  // u columns: DST-II, / lambda, DST-III (n = ny)
@@ -601,6 +602,14 @@ VeloPrec* velo_prec_create(const Grid& g, const Scal& s, bool fp32, int* err, co
     *err = DRD_E_UNSUPPORTED;
     return nullptr;
   }
+  {  // the row and column passes hold a padded FFT line (and the column spectra) in LDS
+    const size_t cx = fp32 ? 2 * sizeof(float) : 2 * sizeof(double), re = cx / 2;
+    const size_t rows = (size_t)lds_len(2 * g.nx) * cx, cols = (size_t)lds_len(2 * g.ny) * cx + 2 * (size_t)g.ny * re;
+    if (std::max(rows, cols) > 160 * 1024) {
+      *err = DRD_E_UNSUPPORTED;
+      return nullptr;
+    }
+  }
   VeloPrec* pc = new VeloPrec();
   pc->nx = g.nx;
   pc->ny = g.ny;
@@ -685,8 +694,8 @@ static void apply_T(hipStream_t st, VeloPrec* pc, const Grid& g, const PrecTable
   // divided spectra of the pair
   const unsigned rows_blocks = (unsigned)((ny + 1) / 2 + ny / 2);
   const unsigned cols_blocks = (unsigned)((nx + 2) / 2 + (nx + 1) / 2);
-  const size_t lds_rows = 2 * (size_t)nx * sizeof(Cx<T>);
-  const size_t lds_cols = 2 * (size_t)ny * sizeof(Cx<T>) + 2 * (size_t)ny * sizeof(T);
+  const size_t lds_rows = (size_t)lds_len(2 * nx) * sizeof(Cx<T>);
+  const size_t lds_cols = (size_t)lds_len(2 * ny) * sizeof(Cx<T>) + 2 * (size_t)ny * sizeof(T);
   if (std::max(nx, ny) <= 2048)
     launch_prec_passes<T, 4096>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
                                 lds_cols, ks, pc->tshape);

@@ -286,6 +286,8 @@ int drd_batch_tune(drd_batch* b, int32_t* shapes, double* ms, int32_t cap);
 #define DRD_K_UNET_FWD 5    /* forward pass of the learned correction's
                                ny x nx network (after drd_learn_init);
                                *bytes returns its MFMA FLOPs instead      */
+#define DRD_K_VELO_PREC 6   /* one application of the DCT/DST velocity
+                               preconditioner (all its passes)             */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
 /* Times kernel `which` inside the caller's own solves: while enabled, each
