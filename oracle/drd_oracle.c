@@ -22,11 +22,20 @@
 #include "drd_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 /* pi as the module parameter of TP/global.f90:39 (rounded to double)      */
+/* OpenMP: loops whose iterations write disjoint elements run in parallel
+ * when the grid is large enough to pay for it; every element's arithmetic
+ * and every sum's order are those of the serial code, so results do not
+ * depend on the thread count. */
+#define PAR_MIN 262144
+#define OMP_STR(x) #x
+#define OMP_FOR(cond) _Pragma(OMP_STR(omp parallel for schedule(static) if(cond)))
+
 static const double PI_G = 3.14159265358979323846264338327950288419716939937510;
 
 struct oc_ctx {
@@ -285,13 +294,32 @@ static double pw_rec(const double* x, const double* y, int64_t lo, int64_t len, 
   double r = pw_rec(x, y, lo + h, h, n);
   return l + r;
 }
+/* the same tree with the subtrees above 2^15 elements as OpenMP tasks */
+static double pw_task(const double* x, const double* y, int64_t lo, int64_t len, int64_t n) {
+  if (lo >= n || len <= ((int64_t)1 << 15)) return pw_rec(x, y, lo, len, n);
+  const int64_t h = len >> 1;
+  double l = 0.0, r = 0.0;
+#pragma omp task shared(l)
+  l = pw_task(x, y, lo, h, n);
+  r = pw_task(x, y, lo + h, h, n);
+#pragma omp taskwait
+  return l + r;
+}
 /* canonical sum of x[i]*y[i] (or of x[i] when y is NULL), i < n */
 double oc_pairwise(int64_t n, const double* x, const double* y) {
   if (n <= 0) return 0.0;
   int64_t P = 1;
   while (P < n) P <<= 1;
-  return pw_rec(x, y, 0, P, n);
+  if (n <= 2 * PAR_MIN || omp_get_max_threads() == 1) return pw_rec(x, y, 0, P, n);
+  double s = 0.0;
+#pragma omp parallel
+#pragma omp single
+  s = pw_task(x, y, 0, P, n);
+  return s;
 }
+
+void oc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t oc_get_threads(void) { return omp_get_max_threads(); }
 static double seg_dot(int64_t n, int nseg, const double* x, const double* y) {
   int64_t seg = n / nseg;
   double s = oc_pairwise(seg, x, y);
@@ -301,24 +329,31 @@ static double seg_dot(int64_t n, int nseg, const double* x, const double* y) {
 static double ddot_s(int64_t n, int nseg, const double* x, const double* y) { return seg_dot(n, nseg, x, y); }
 static double dnrm2_s(int64_t n, int nseg, const double* x) { return sqrt(seg_dot(n, nseg, x, x)); }
 static void daxpy(int64_t n, double a, const double* x, double* y) {
+  OMP_FOR(n > PAR_MIN)
   for (int64_t i = 0; i < n; ++i) y[i] = y[i] + a * x[i];
 }
 
 /* ---- packers: Atob_uv / btoA_uv  TP/matrix_tool.f90:5-75 -------------- */
 static void atob_uv(const oc_ctx* c, const double* f1, const double* f2, double* b) {
   int nx = NX, ny = NY;
+  const int big = (int64_t)nx * ny > PAR_MIN;
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i <= nx; ++i) b[i + (size_t)(nx + 1) * j] = f1[UI(i, j)];
   size_t mn = (size_t)(nx + 1) * ny;
+  OMP_FOR(big)
   for (int j = 1; j < ny; ++j)
     for (int i = 0; i < nx; ++i) b[i + (size_t)nx * (j - 1) + mn] = f2[VI(i, j)];
 }
 
 static void btoa_uv(const oc_ctx* c, double* f1, double* f2, const double* b) {
   int nx = NX, ny = NY;
+  const int big = (int64_t)nx * ny > PAR_MIN;
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i <= nx; ++i) f1[UI(i, j)] = b[i + (size_t)(nx + 1) * j];
   size_t mn = (size_t)(nx + 1) * ny;
+  OMP_FOR(big)
   for (int j = 1; j < ny; ++j)
     for (int i = 0; i < nx; ++i) f2[VI(i, j)] = b[i + (size_t)nx * (j - 1) + mn];
   for (int i = 0; i <= nx; ++i) {
@@ -343,6 +378,7 @@ static void btoa_uv(const oc_ctx* c, double* f1, double* f2, const double* b) {
 static void atob_ph_mu(const oc_ctx* c, const double* f1, const double* f2, double* b) {
   int nx = NX, ny = NY;
   size_t mn = (size_t)nx * ny;
+  OMP_FOR(mn > PAR_MIN)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       b[i + (size_t)nx * j] = f1[CI(i, j)];
@@ -353,6 +389,7 @@ static void atob_ph_mu(const oc_ctx* c, const double* f1, const double* f2, doub
 static void btoa_ph_mu(const oc_ctx* c, double* f1, double* f2, const double* b) {
   int nx = NX, ny = NY;
   size_t mn = (size_t)nx * ny;
+  OMP_FOR(mn > PAR_MIN)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       f1[CI(i, j)] = b[i + (size_t)nx * j];
@@ -388,6 +425,7 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
 #define GI(i, k) ((size_t)((i) + 1) + (size_t)(nx + 2) * (size_t)((k) + 1))
   size_t n = (size_t)(nx + 2) * (ny + 2);
   for (size_t q = 0; q < n; ++q) rhs[q] = 0.0;
+  OMP_FOR(n > PAR_MIN)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       double moL, moR;
@@ -461,7 +499,9 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
     f2[VI(i, 0)] = 0.0;
     f2[VI(i, ny)] = 0.0;
   }
+  const int big = (int64_t)nx * ny > PAR_MIN;
   /* u rows */
+  OMP_FOR(big)
   for (int k = 0; k < ny; ++k)
     for (int i = 0; i <= nx; ++i) {
       double r;
@@ -479,6 +519,7 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
       f11[UI(i, k)] = r;
     }
   /* v rows */
+  OMP_FOR(big)
   for (int k = 1; k < ny; ++k)
     for (int i = 0; i < nx; ++i) {
       double r;
@@ -498,6 +539,7 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
   int64_t n = nuv(p);
   double* zz = c->w5;
   atob_uv(c, f11, f22, zz);
+  OMP_FOR(big)
   for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
 }
 
@@ -553,6 +595,8 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
   double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_ph_mu(c, f1, f2, xx);
   oc_op_double_gradient_neum(nx, ny, dr2, dz2, f1, c->f[DRD_F_PH_M_INNER], ff1);
+  const int big = (int64_t)nx * ny > PAR_MIN;
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i)
       ff1[CI(i, j)] = s * f1[CI(i, j)] / I - I * ff1[CI(i, j)] - f2[CI(i, j)] * bu[CI(i, j)];
@@ -563,12 +607,14 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
     ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j + 1)] - c->coef_top * f1[CI(i, j)]);
   }
   oc_op_double_gradient_neum(nx, ny, dr2, dz2, f2, c->f[DRD_F_PH_M_0], ff2);
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i)
       ff2[CI(i, j)] = f1[CI(i, j)] * (bu[CI(i, j)] + p->bdr_ph_used_adjust) - p->dt * p->xld * ff2[CI(i, j)];
   int64_t n = 2 * (int64_t)nx * ny;
   double* zz = c->w5;
   atob_ph_mu(c, ff1, ff2, zz);
+  OMP_FOR(big)
   for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
 }
 
@@ -580,12 +626,15 @@ static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
   double *f1 = c->w1, *f2 = c->w2, *ff1 = c->w3, *ff2 = c->w4;
   double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_ph_mu(c, f1, f2, xx);
+  const int big = (int64_t)nx * ny > PAR_MIN;
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
       double tmpz = (i == 0 || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
       ff1[CI(i, j)] = f1[CI(i, j)] / ((tmpx + tmpz) * I + s / I);
     }
+  OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
@@ -616,6 +665,7 @@ static void phmu_block(oc_ctx* c, double* yy, const double* xx) {
   double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   int64_t C = (int64_t)nx * ny;
   const double floor22 = 1e-3 * p->dt * p->xld * (2.0 / dr2 + 2.0 / dz2);
+  OMP_FOR(C > PAR_MIN)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       size_t q = i + (size_t)nx * j;
@@ -1070,6 +1120,7 @@ static int bicgstab_core(int64_t n, int nseg, const double* b, double* x, double
     if (it > 1) {
       double beta = (rho / rho1) * (alpha / omega);
       daxpy(n, -omega, V, P);
+      OMP_FOR(n > PAR_MIN)
       for (int64_t q = 0; q < n; ++q) P[q] = beta * P[q];
       daxpy(n, 1.0, R, P);
     } else {

@@ -112,4 +112,8 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2,
 #ifdef __cplusplus
 }
 #endif
+/* OpenMP threads of the oracle's parallel loops (results do not depend on it) */
+void oc_set_threads(int32_t n);
+int32_t oc_get_threads(void);
+
 #endif

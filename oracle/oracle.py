@@ -35,6 +35,8 @@ def lib():
         I = C.POINTER(C.c_int32)
         L.oc_init.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_void_p)]
         L.oc_pairwise.argtypes = [C.c_int64, P, P]
+        L.oc_set_threads.argtypes = [C.c_int32]
+        L.oc_get_threads.restype = C.c_int32
         L.oc_pairwise.restype = C.c_double
         L.oc_dct_2d_fast.argtypes = [C.c_int32, C.c_int32, P, C.c_int32]
         L.oc_finalize.argtypes = [C.c_void_p]
@@ -81,6 +83,15 @@ def lib():
 
 dp = abi.dptr
 
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().oc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().oc_get_threads())
 
 class Oracle:
     """Host-resident state + reference routines (FP64, serial)."""

@@ -376,3 +376,30 @@ def test_ch_rhs_isolated_terms():
     np.testing.assert_allclose(rhs[:C].reshape(ny, nx), f1, rtol=1e-12, atol=1e-12)
     # M_0 = (1 - bdr)(phi^2 - 1)^2 over the whole field                :31-35
     np.testing.assert_allclose(o.get_field(abi.F_PH_M_0), (1 - bdr) * (ph ** 2 - 1) ** 2, rtol=1e-14)
+
+
+def test_oracle_threads_do_not_change_results():
+    """The OpenMP loops and the task-parallel pairwise tree give the serial
+    results bit for bit (1024x512: above the parallel threshold)."""
+    from drd_application_amd import abi, geometry
+    from oracle import oracle as O
+    cfg = geometry.two_phase_config(1024, 512, 9, n_drops=4, r_drop=(6, 20), n_discs=2, r_disc=(8, 12),
+                                    lambda_eta=20.0, penalty=1e5, dt=5e-5, name="omp")
+    out = {}
+    for nt in (1, 8):
+        O.set_threads(nt)
+        o = O.Oracle(cfg.params)
+        geometry.load(o, cfg)
+        nv = abi.n_uv(cfg.params)
+        b = np.random.default_rng(3).standard_normal(nv)
+        x, it, res, info = o.bicgstab(abi.OP_VELO2, abi.PS_IDENTITY, b, np.zeros(nv), 6, 1e-300)
+        nc = 2 * cfg.params.nx * cfg.params.ny
+        bc = o.ph_mu_rhs()
+        y, it2, res2, info2 = o.bicgstab(abi.OP_PHMU, abi.PS_PHMU_BLOCK, bc, np.zeros(nc), 6, 1e-300)
+        out[nt] = (x, res, y, res2, O.pairwise(b, b) if hasattr(O, "pairwise") else 0.0)
+        o.close()
+    O.set_threads(1)
+    a, b8 = out[1], out[8]
+    np.testing.assert_array_equal(a[0], b8[0])
+    np.testing.assert_array_equal(a[2], b8[2])
+    assert a[1] == b8[1] and a[3] == b8[3] and a[4] == b8[4]
