@@ -110,6 +110,7 @@ struct drd_ctx {
   std::vector<double> last_div;      // per instance: the latest mean divergence
   // captured BiCGSTAB iteration chunks (drd_use_graphs)
   bool use_graphs = false;
+  int guess = 0;  // drd_set_initial_guess
   std::map<GraphKey, hipGraphExec_t> graphs;
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
@@ -509,6 +510,13 @@ static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
 }
 
 extern "C" int drd_init(const drd_params* prm, drd_ctx** out) { return ctx_create(prm, 1, out); }
+
+extern "C" int drd_set_initial_guess(drd_ctx* c, int32_t mode) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  if (mode != 0 && mode != 1) return set_err(DRD_E_ARG, "initial-guess mode must be 0 or 1");
+  c->guess = mode;
+  return 0;
+}
 
 extern "C" int drd_use_graphs(drd_ctx* c, int32_t on) {
   if (!c) return set_err(DRD_E_ARG, "null ctx");
@@ -1463,6 +1471,10 @@ static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double*
   }
   launch_velo_rhs(c->stream, bdf, c->g, c->s, a, c->rhs);
   launch_velo_pack(c->stream, c->g, c->f[DRD_F_U], c->f[DRD_F_V], c->xv);
+  if (bdf == 2 && c->guess == 1 && u_pre && v_pre) {  // x0 = 2 x^n - x^{n-1}
+    launch_velo_pack(c->stream, c->g, u_pre, v_pre, c->tmp);
+    launch_axpby(c->stream, c->g.nuv(), -1.0, c->tmp, 2.0, c->xv);
+  }
   const int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
   const int maxit = bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2;
   if (bdf == 2 && c->p.velo_prec == DRD_PREC_LEARNED) {
@@ -1483,7 +1495,7 @@ static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double*
 
 // ---- Cahn-Hilliard ---------------------------------------------------------
 static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_solve_info* info,
-                           bool want_true) {
+                           bool want_true, bool extrapolate = false) {
   launch_phmu_m0(c->stream, c->g, c->p.phi_mode, c->f[DRD_F_PH], c->f[DRD_F_BDR_PH], c->f[DRD_F_PH_M_0]);
   PhmuRhsArgs a{};
   a.ph = c->f[DRD_F_PH];
@@ -1498,6 +1510,10 @@ static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_
   a.cosa = c->cosa;
   a.sqrt2 = c->sqrt2;
   launch_phmu_rhs(c->stream, c->g, c->s, a, c->rhs, c->xv, c->pn);
+  if (extrapolate) {  // x0 = 2 (phi, mu)^n - (phi, mu)^{n-1}
+    launch_phmu_pack(c->stream, c->g, c->f[DRD_F_PH_PRE], c->f[DRD_F_RMU_PRE], c->tmp);
+    launch_axpby(c->stream, 2 * c->g.ncell(), -1.0, c->tmp, 2.0, c->xv);
+  }
   const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK ? DRD_PS_PHMU_BLOCK : DRD_PS_PHMU_JACOBI;
   RCHK(run_bicgstab(c, DRD_OP_PHMU, prec, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu, c->p.tol, info,
                     want_true));
@@ -1810,7 +1826,7 @@ static int step_dev(drd_ctx* c, int bdf, double t, drd_step_info* out) {
   drd_step_info loc{};
   const double gamma1 = std::min(1.0, p.lambda_rho);
   double** F = c->f;
-  RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false));
+  RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false, bdf == 2 && c->guess == 1));
   for (int i = 0; i < c->nb; ++i) out[i].phmu = c->nb == 1 ? loc.phmu : c->last[i];
   if (bdf == 1) {
     RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt, nullptr,
@@ -1973,6 +1989,12 @@ extern "C" int drd_batch_tune(drd_batch* b, int32_t* shapes, double* ms, int32_t
     BatchScope scope(b->groups[gI]);
     RCHK(tune_ctx(b->groups[gI], &shapes[gI], &ms[3 * gI]));
   }
+  return 0;
+}
+
+extern "C" int drd_batch_set_initial_guess(drd_batch* b, int32_t mode) {
+  if (!b) return set_err(DRD_E_ARG, "null batch");
+  for (drd_ctx* c : b->groups) RCHK(drd_set_initial_guess(c, mode));
   return 0;
 }
 

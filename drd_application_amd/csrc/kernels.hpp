@@ -139,6 +139,7 @@ void launch_phmu_rhs(hipStream_t st, const Grid& g, const Scal& s, const PhmuRhs
                      double* x0, double* pn);
 void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* pn,
                         double* ph, double* mu);
+void launch_phmu_pack(hipStream_t st, const Grid& g, const double* ph, const double* mu, double* x);
 
 // ---- launchers (kernels_krylov.hip) ----
 // Reducing kernels take the vector's segment count (1, or 2 for the phi|mu

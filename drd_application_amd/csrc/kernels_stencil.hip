@@ -642,6 +642,21 @@ __global__ __launch_bounds__(kBlock) void k_phmu_unpack(Grid g, Scal s, const do
   mu[q] = m;
 }
 
+// Atob_ph_mu of two cell fields (the packed phi|mu vector) TP/matrix_tool.f90:82-98
+__global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __restrict__ ph,
+                                                      const double* __restrict__ mu, double* __restrict__ x,
+                                                      int64_t bs) {
+  ph = bofs(ph, bs);
+  mu = bofs(mu, bs);
+  x = bofs(x, bs);
+  const int64_t C = g.ncell();
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= C) return;
+  const int k = g.by_nx.div(q), i = (int)(q - (int64_t)k * g.nx);
+  x[q] = ph[g.ci(i, k)];
+  x[C + q] = mu[g.ci(i, k)];
+}
+
 // ---- host launchers -----------------------------------------------------
 static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
@@ -736,6 +751,11 @@ void launch_phmu_rhs(hipStream_t st, const Grid& g, const Scal& s, const PhmuRhs
                      double* x0, double* pn) {
   hipLaunchKernelGGL(k_phmu_rhs, dim3(nblk(g.ncell(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, a, rhs, x0,
                      pn, bat_bs());
+}
+
+void launch_phmu_pack(hipStream_t st, const Grid& g, const double* ph, const double* mu, double* x) {
+  hipLaunchKernelGGL(k_phmu_pack, dim3(nblk(g.ncell(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, ph, mu, x,
+                     bat_bs());
 }
 
 void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* pn,

@@ -83,6 +83,8 @@ def lib():
         L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_use_graphs.argtypes = [V, C.c_int32]
+        L.drd_set_initial_guess.argtypes = [V, C.c_int32]
+        L.drd_batch_set_initial_guess.argtypes = [V, C.c_int32]
         L.drd_tune.argtypes = [V, I, C.POINTER(C.c_double)]
         L.drd_batch_tune.argtypes = [V, I, C.POINTER(C.c_double), C.c_int32]
         L.drd_batch_use_graphs.argtypes = [V, C.c_int32]
@@ -278,6 +280,10 @@ class GpuSolver:
         following solves with HIP events (0 disables)."""
         _check(lib().drd_kernel_timer(self._h, which, capacity))
 
+    def set_initial_guess(self, mode):
+        """0: BDF2 solves start from the previous step (the reference); 1: from 2x^n - x^{n-1}."""
+        _check(lib().drd_set_initial_guess(self._h, mode))
+
     def use_graphs(self, on=True):
         """Capture BiCGSTAB iteration chunks as hipGraphs and replay them."""
         _check(lib().drd_use_graphs(self._h, 1 if on else 0))
@@ -382,6 +388,9 @@ class GpuBatch:
 
     def use_graphs(self, on=True):
         _check(lib().drd_batch_use_graphs(self._h, 1 if on else 0))
+
+    def set_initial_guess(self, mode):
+        _check(lib().drd_batch_set_initial_guess(self._h, mode))
 
     def tune(self):
         """[(nx, ny, chosen tile shape, [ms per shape])] for every size group."""

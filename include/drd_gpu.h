@@ -273,6 +273,15 @@ int drd_batch_use_graphs(drd_batch* b, int32_t on);
 int drd_tune(drd_ctx* ctx, int32_t* shape, double* ms);
 int drd_batch_tune(drd_batch* b, int32_t* shapes, double* ms, int32_t cap);
 
+/* ---- initial guess of the BDF2 solves (no reference counterpart) ----------
+ * mode 0 (default, the reference's): a solve starts from the previous step's
+ * solution.  mode 1: from the extrapolation 2 x^n - x^{n-1} -- the velocity
+ * solves of drd_step and drd_velosolve_2_combined (x^{n-1} = u_pre, v_pre)
+ * and the phi/mu solve of drd_step (x^{n-1} = ph_pre, rmu_pre).  The stopping
+ * test is unchanged; a smoothly varying solution needs fewer iterations.    */
+int drd_set_initial_guess(drd_ctx* ctx, int32_t mode);
+int drd_batch_set_initial_guess(drd_batch* b, int32_t mode);
+
 /* ---- measurement hook (no reference counterpart) -----------------------
  * Launches one kernel of the solve path `reps` times on the context's
  * stream between HIP events and returns the mean launch duration and the

@@ -51,6 +51,7 @@ struct oc_ctx {
   /* scratch */
   double *w1, *w2, *w3, *w4, *w5, *w6;
   double* work;    /* Krylov WORK(N,7) */
+  int guess;       /* initial guess of the BDF2 solves: 0 previous step, 1 2x^n - x^{n-1} */
   int64_t nmax;
 };
 
@@ -319,6 +320,11 @@ double oc_pairwise(int64_t n, const double* x, const double* y) {
 }
 
 void oc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oc_set_initial_guess(oc_ctx* c, int32_t mode) {
+  if (mode != 0 && mode != 1) return -1;
+  c->guess = mode;
+  return 0;
+}
 int32_t oc_get_threads(void) { return omp_get_max_threads(); }
 static double seg_dot(int64_t n, int nseg, const double* x, const double* y) {
   int64_t seg = n / nseg;
@@ -1486,6 +1492,12 @@ static int velo_solve(oc_ctx* c, int bdf, const double* psi1, const double* psi2
     return rc;
   }
   atob_uv(c, c->f[DRD_F_U], c->f[DRD_F_V], x);
+  if (bdf == 2 && c->guess == 1 && u_pre && v_pre) {  /* 2 x^n - x^{n-1} */
+    double* xp = (double*)malloc((size_t)n * sizeof(double));
+    atob_uv(c, u_pre, v_pre, xp);
+    for (int64_t q = 0; q < n; ++q) x[q] = 2.0 * x[q] - xp[q];
+    free(xp);
+  }
   int32_t it = bdf == 1 ? c->p.maxit_velo1 : c->p.maxit_velo2, inf = 0;
   double res = c->p.tol;
   int op = bdf == 1 ? DRD_OP_VELO1 : DRD_OP_VELO2;
@@ -1638,8 +1650,12 @@ int oc_ph_mu_rhs(oc_ctx* c, double* rhs) {
   return 0;
 }
 
+static int ph_mu_solve_x(oc_ctx* c, double* ph_after, double* rmu_after, drd_solve_info* info, int extrapolate);
 int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t, drd_solve_info* info) {
   (void)t;
+  return ph_mu_solve_x(c, ph_after, rmu_after, info, 0);
+}
+static int ph_mu_solve_x(oc_ctx* c, double* ph_after, double* rmu_after, drd_solve_info* info, int extrapolate) {
   int nx = NX, ny = NY;
   int64_t n = 2 * (int64_t)nx * ny;
   double* rhs = (double*)malloc((size_t)n * sizeof(double));
@@ -1649,6 +1665,12 @@ int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t, drd
   memcpy(pn, c->w5, (size_t)nx * sizeof(double));
   memcpy(pn + nx, c->w6, (size_t)nx * sizeof(double));
   atob_ph_mu(c, c->f[DRD_F_PH], c->f[DRD_F_RMU], x);
+  if (extrapolate) { /* 2 (phi, mu)^n - (phi, mu)^{n-1} */
+    double* xp = (double*)malloc((size_t)n * sizeof(double));
+    atob_ph_mu(c, c->f[DRD_F_PH_PRE], c->f[DRD_F_RMU_PRE], xp);
+    for (int64_t q = 0; q < n; ++q) x[q] = 2.0 * x[q] - xp[q];
+    free(xp);
+  }
   int32_t it = c->p.maxit_phmu, inf = 0;
   double res = c->p.tol;
   oc_bicgstab(c, DRD_OP_PHMU, c->p.ch_prec == DRD_CH_PREC_BLOCK ? DRD_PS_PHMU_BLOCK : DRD_PS_PHMU_JACOBI,
@@ -1768,7 +1790,7 @@ int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
     memcpy(F[DRD_F_RMU], rmu_after, (size_t)nc * sizeof(double));
     memcpy(F[DRD_F_PH], ph_after, (size_t)nc * sizeof(double));
   } else {
-    oc_ph_mu_solve(c, ph_after, rmu_after, t + p->dt, &loc.phmu);
+    ph_mu_solve_x(c, ph_after, rmu_after, &loc.phmu, c->guess == 1);
     rc = oc_velosolve_2_combined(c, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], u_after,
                                  F[DRD_F_V_PRE], v_after, F[DRD_F_PH_PRE], ph_after,
                                  F[DRD_F_RMU_PRE], rmu_after, t + p->dt, zero, zero, &loc.velo);

@@ -114,6 +114,8 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2,
 #endif
 /* OpenMP threads of the oracle's parallel loops (results do not depend on it) */
 void oc_set_threads(int32_t n);
+/* drd_set_initial_guess of the oracle: 0 previous step, 1 2x^n - x^{n-1} */
+int oc_set_initial_guess(oc_ctx* c, int32_t mode);
 int32_t oc_get_threads(void);
 
 #endif

@@ -36,6 +36,7 @@ def lib():
         L.oc_init.argtypes = [C.POINTER(abi.Params), C.POINTER(C.c_void_p)]
         L.oc_pairwise.argtypes = [C.c_int64, P, P]
         L.oc_set_threads.argtypes = [C.c_int32]
+        L.oc_set_initial_guess.argtypes = [C.c_void_p, C.c_int32]
         L.oc_get_threads.restype = C.c_int32
         L.oc_pairwise.restype = C.c_double
         L.oc_dct_2d_fast.argtypes = [C.c_int32, C.c_int32, P, C.c_int32]
@@ -238,6 +239,9 @@ class Oracle:
         info = abi.SolveInfo()
         lib().oc_phsolve(self._h, dp(mu_a), dp(ph_a), t, C.byref(info))
         return mu_a, ph_a, info.as_dict()
+
+    def set_initial_guess(self, mode):
+        assert lib().oc_set_initial_guess(self._h, mode) == 0
 
     def step(self, bdf, t):
         info = abi.StepInfo()

@@ -160,3 +160,38 @@ def test_fast_steps_track_oracle_steps():
         print(abi.FIELD_NAMES[fid], e)
         assert e < 1e-6
     gf.close()
+
+
+def test_extrapolated_initial_guess_is_bitexact():
+    """drd_set_initial_guess(1) (x0 = 2x^n - x^{n-1}) against the oracle with
+    the same option, bit for bit (reference velocity preconditioner, block
+    CH); the iterations against the default start are printed (they depend
+    on how smoothly the solution varies: startup steps can need more)."""
+    from drd_application_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    cfg = geometry.tp_case(64, 64, lambda_eta=10.0, tol=1e-10)
+    p = cfg.params
+    p.ch_prec = abi.CH_PREC_BLOCK
+    g, g0 = GpuSolver(p), GpuSolver(p)
+    o = Oracle(p)
+    for s in (g, g0, o):
+        geometry.load(s, cfg)
+    g.set_initial_guess(1)
+    o.set_initial_guess(1)
+    its, its0 = 0, 0
+    t = 0.0
+    for k in range(6):
+        bdf = 1 if k == 0 else 2
+        a, b, c = g.step(bdf, t), o.step(bdf, t), g0.step(bdf, t)
+        for key in ("phmu", "velo"):
+            assert a[key]["iter"] == b[key]["iter"] and a[key]["resid"] == b[key]["resid"], (k, key)
+            assert a[key]["info"] == 0 and c[key]["info"] == 0
+        if k > 0:
+            its += a["phmu"]["iter"] + a["velo"]["iter"]
+            its0 += c["phmu"]["iter"] + c["velo"]["iter"]
+        t += p.dt
+    for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU):
+        np.testing.assert_array_equal(g.get_field(fid), o.get_field(fid))
+    print(f"BDF2 iterations with the extrapolated start {its}, from the previous step {its0}")
+    g.close()
+    g0.close()
