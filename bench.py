@@ -412,6 +412,8 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve path "
               f"is single-GPU, see DESIGN.md for how it would shard", file=sys.stderr)
         return 2
+    import numpy as np
+
     from drd_application_amd import abi, geometry
 
     cfg = geometry.config(args.config)
@@ -440,10 +442,21 @@ def main(argv=None) -> int:
         ms, by = g.bench_kernel(which, 20)
         kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
         log(f"{name} (stand-alone): {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")
-    g.close()
     modes = {args.mode: head}
+    field_err = None
     for m in [s for s in args.compare.split(",") if s and s != args.mode]:
-        modes[m], _ = run_mode(cfg, m, args.compare_steps or args.steps, args.warmup)
+        same_steps = (args.compare_steps or args.steps) == args.steps
+        modes[m], g64 = run_mode(cfg, m, args.compare_steps or args.steps, args.warmup,
+                                 keep=(m == "fp64dct" and same_steps))
+        if g64 is not None:  # the headline run against the FP64-preconditioned run, same steps
+            field_err = {}
+            for fid in (abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU):
+                a, b = g.get_field(fid), g64.get_field(fid)
+                field_err[abi.FIELD_NAMES[fid]] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+            g64.close()
+            log(f"relative L2 difference of the fields, {args.mode} vs fp64dct after {args.warmup} + "
+                f"{args.steps} steps: {field_err}")
+    g.close()
     c2 = None
     if not args.no_c2:
         try:
@@ -482,6 +495,7 @@ def main(argv=None) -> int:
                                           if "fp64dct" in modes and args.mode != "fp64dct" else None),
                    "speedup_vs_dropin": (head["steps_per_s"] / modes["dropin"]["steps_per_s"]
                                          if "dropin" in modes and args.mode != "dropin" else None),
+                   "field_rel_l2_vs_fp64dct": field_err,
                    "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,

@@ -15,6 +15,7 @@
 #include <tuple>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -1863,6 +1864,94 @@ extern "C" int drd_step(drd_ctx* c, int32_t bdf, double t, drd_step_info* info) 
   drd_step_info loc{};
   RCHK(step_dev(c, bdf, t, &loc));
   if (info) *info = loc;
+  return 0;
+}
+
+// ---- snapshots in the reference's format (TP/inout.f90:1-61) -----------------
+// Fortran unformatted sequential records (4-byte length markers around each
+// record), every array written as ((a(i,k), k=0,ny), i=0,nx):
+//   x, y (cell centres (i+0.5) dx, (k+0.5) dz), u, v, p, f, f2, t.
+static void put_record(std::ofstream& os, const double* v, size_t n) {
+  const uint32_t len = (uint32_t)(n * sizeof(double));
+  os.write(reinterpret_cast<const char*>(&len), 4);
+  os.write(reinterpret_cast<const char*>(v), (std::streamsize)len);
+  os.write(reinterpret_cast<const char*>(&len), 4);
+}
+static bool get_record(std::ifstream& is, double* v, size_t n) {
+  uint32_t a = 0, b = 0;
+  is.read(reinterpret_cast<char*>(&a), 4);
+  if (!is || a != n * sizeof(double)) return false;
+  is.read(reinterpret_cast<char*>(v), (std::streamsize)a);
+  is.read(reinterpret_cast<char*>(&b), 4);
+  return (bool)is && a == b;
+}
+
+// element (i, k) of a field stored with its ghost layout
+static int64_t field_index(const drd_params& p, int32_t fid, int i, int k) {
+  if (fid == DRD_F_U || fid == DRD_F_U_PRE) return (int64_t)(i + 1) + (int64_t)(p.nx + 3) * (k + 1);
+  if (fid == DRD_F_V || fid == DRD_F_V_PRE) return (int64_t)(i + 1) + (int64_t)(p.nx + 2) * k;
+  return (int64_t)(i + 1) + (int64_t)(p.nx + 2) * (k + 1);
+}
+
+static bool cell_field(int32_t fid) {
+  return fid >= 0 && fid < DRD_F_COUNT && fid != DRD_F_U && fid != DRD_F_V && fid != DRD_F_U_PRE &&
+         fid != DRD_F_V_PRE && fid != DRD_F_ANGLEB && fid != DRD_F_ANGLET;
+}
+
+extern "C" int drd_write_snapshot(drd_ctx* c, const char* path, int32_t f_field, int32_t f2_field, double t) {
+  if (!c || !path) return set_err(DRD_E_ARG, "null argument");
+  if (!cell_field(f_field) || !cell_field(f2_field)) return set_err(DRD_E_ARG, "f, f2 must be cell fields");
+  const drd_params& p = c->p;
+  const int M = p.nx, N = p.ny;
+  const size_t n = (size_t)(M + 1) * (N + 1);
+  std::vector<double> rec(n), host;
+  std::ofstream os(path, std::ios::binary);
+  if (!os) return set_err(DRD_E_ARG, std::string("cannot open ") + path);
+  auto emit = [&](auto value) {
+    size_t q = 0;
+    for (int i = 0; i <= M; ++i)
+      for (int k = 0; k <= N; ++k) rec[q++] = value(i, k);
+    put_record(os, rec.data(), n);
+  };
+  emit([&](int i, int) { return (i + 0.5) * p.dx; });
+  emit([&](int, int k) { return (k + 0.5) * p.dz; });
+  for (int32_t fid : {(int32_t)DRD_F_U, (int32_t)DRD_F_V, (int32_t)DRD_F_P, f_field, f2_field}) {
+    host.resize((size_t)drd_field_size(&p, fid));
+    RCHK(drd_get_field(c, fid, host.data()));
+    emit([&](int i, int k) { return host[(size_t)field_index(p, fid, i, k)]; });
+  }
+  put_record(os, &t, 1);
+  os.flush();
+  if (!os) return set_err(DRD_E_HIP, std::string("write failed: ") + path);
+  return 0;
+}
+
+// Reads all seven arrays and t (the reference's input reads six: it skips
+// f2, TP/inout.f90:41-61) and sets u, v, p, f, f2 at their (0:nx, 0:ny)
+// elements; the other ghost entries keep their values.
+extern "C" int drd_read_snapshot(drd_ctx* c, const char* path, int32_t f_field, int32_t f2_field, double* t) {
+  if (!c || !path) return set_err(DRD_E_ARG, "null argument");
+  if (!cell_field(f_field) || !cell_field(f2_field)) return set_err(DRD_E_ARG, "f, f2 must be cell fields");
+  const drd_params& p = c->p;
+  const int M = p.nx, N = p.ny;
+  const size_t n = (size_t)(M + 1) * (N + 1);
+  std::vector<double> rec(n), host;
+  std::ifstream is(path, std::ios::binary);
+  if (!is) return set_err(DRD_E_ARG, std::string("cannot open ") + path);
+  for (int r = 0; r < 2; ++r)
+    if (!get_record(is, rec.data(), n)) return set_err(DRD_E_ARG, "not a snapshot of this grid size");
+  for (int32_t fid : {(int32_t)DRD_F_U, (int32_t)DRD_F_V, (int32_t)DRD_F_P, f_field, f2_field}) {
+    if (!get_record(is, rec.data(), n)) return set_err(DRD_E_ARG, "truncated snapshot");
+    host.resize((size_t)drd_field_size(&p, fid));
+    RCHK(drd_get_field(c, fid, host.data()));
+    size_t q = 0;
+    for (int i = 0; i <= M; ++i)
+      for (int k = 0; k <= N; ++k) host[(size_t)field_index(p, fid, i, k)] = rec[q++];
+    RCHK(drd_set_field(c, fid, host.data()));
+  }
+  double tv = 0.0;
+  if (!get_record(is, &tv, 1)) return set_err(DRD_E_ARG, "truncated snapshot");
+  if (t) *t = tv;
   return 0;
 }
 

@@ -83,6 +83,8 @@ def lib():
         L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_use_graphs.argtypes = [V, C.c_int32]
+        L.drd_write_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.c_double]
+        L.drd_read_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
         L.drd_set_initial_guess.argtypes = [V, C.c_int32]
         L.drd_batch_set_initial_guess.argtypes = [V, C.c_int32]
         L.drd_tune.argtypes = [V, I, C.POINTER(C.c_double)]
@@ -279,6 +281,16 @@ class GpuSolver:
         """Brackets up to `capacity` launches of kernel `which` inside the
         following solves with HIP events (0 disables)."""
         _check(lib().drd_kernel_timer(self._h, which, capacity))
+
+    def write_snapshot(self, path, f_field=abi.F_BDR_PH, f2_field=abi.F_PH, t=0.0):
+        """The reference's data file (TP/inout.f90 output): x, y, u, v, p, f, f2, t."""
+        _check(lib().drd_write_snapshot(self._h, os.fsencode(path), f_field, f2_field, t))
+
+    def read_snapshot(self, path, f_field=abi.F_BDR_PH, f2_field=abi.F_PH):
+        """Sets u, v, p, f, f2 (0:nx, 0:ny) from a data file; returns its t."""
+        t = C.c_double(0.0)
+        _check(lib().drd_read_snapshot(self._h, os.fsencode(path), f_field, f2_field, C.byref(t)))
+        return t.value
 
     def set_initial_guess(self, mode):
         """0: BDF2 solves start from the previous step (the reference); 1: from 2x^n - x^{n-1}."""

@@ -232,6 +232,18 @@ int drd_fgmres(drd_ctx* ctx, int32_t op, int32_t prec, int32_t n,
 int drd_step(drd_ctx* ctx, int32_t bdf, double t, drd_step_info* info);
 int drd_synchronize(drd_ctx* ctx);
 
+/* ---- snapshots in the reference's data-file format ----------------------
+ * drd_write_snapshot replaces output (TP/inout.f90:1-20): Fortran
+ * unformatted sequential records (4-byte markers) x, y, u, v, p, f, f2, t,
+ * each array written as ((a(i,k), k=0,ny), i=0,nx), with x, y the cell
+ * centres; f and f2 are cell fields chosen by id (the reference writes
+ * fpd_ph and ph, or bdr_ph and ph).  TP/post_process/da1.py reads it.
+ * drd_read_snapshot replaces input (TP/inout.f90:41-61) but reads all seven
+ * arrays (the reference's input skips f2) and sets u, v, p, f, f2 at their
+ * (0:nx, 0:ny) elements; *t receives the time.                              */
+int drd_write_snapshot(drd_ctx* ctx, const char* path, int32_t f_field, int32_t f2_field, double t);
+int drd_read_snapshot(drd_ctx* ctx, const char* path, int32_t f_field, int32_t f2_field, double* t);
+
 /* ---- batches of independent instances (no reference counterpart) --------
  * One drd_batch holds `count` independent instances (each its own grid,
  * fields and solver state).  Instances of equal nx x ny form a group that
