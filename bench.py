@@ -389,6 +389,22 @@ def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     return 0
 
 
+def profile_network(n: int = 1024, reps: int = 10) -> int:
+    """The learned correction's n x n network forward pass launched `reps`
+    times (for rocprofv3 --kernel-trace; the parameters are the initial ones)."""
+    from drd_application_amd import abi, geometry
+    from drd_application_amd.solver import GpuSolver
+
+    cfg, _ = geometry.velocity_system(n, 3000)
+    g = GpuSolver(cfg.params)
+    g.learn_init(4, 256, 256, seed=1)
+    ms, flops = g.bench_kernel(abi.K_UNET_FWD, reps)
+    log(f"network forward {n}x{n}: {ms*1e3:.0f} us, {flops/1e9:.1f} GFLOP -> {flops/(ms*1e-3)/1e12:.1f} TFLOP/s")
+    g.close()
+    print(json.dumps({"us": ms * 1e3, "flops": flops, "launches": reps + 1}))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +421,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the learned correction's own workload")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
+    ap.add_argument("--profile-network", action="store_true",
+                    help="only launch the learned correction's 1024^2 network forward pass")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the stand-alone kernel loops on the config's fields (for rocprofv3 --pmc)")
     args = ap.parse_args(argv)
@@ -421,6 +439,8 @@ def main(argv=None) -> int:
     log(f"config {cfg.name}: {p.nx}x{p.ny}, tol {p.tol:g}, dt {p.dt:g}")
     if args.profile_kernels:
         return profile_kernels(cfg, args.mode)
+    if args.profile_network:
+        return profile_network()
     head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=True)
     tl = head.pop("timed_launches")
     source = "in-solve HIP events"
