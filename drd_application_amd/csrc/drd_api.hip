@@ -640,14 +640,10 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
     case DRD_OP_VELO2: {
       const bool timed = op == DRD_OP_VELO2 && mode == MV_DOT1 && c->tm_which == DRD_K_VELO_MATVEC &&
                          2 * (size_t)c->tm_n < c->tm_ev.size();
-      if (timed) HIPCHK(hipEventRecord(c->tm_ev[2 * c->tm_n], c->stream));
       launch_velo_matvec(c->stream, op == DRD_OP_VELO1 ? 1 : 2, mode, c->g, c->s, k, x, y, alpha, beta, a0, b,
-                         c->part, ks, !timed);
-      if (timed) {
-        HIPCHK(hipEventRecord(c->tm_ev[2 * c->tm_n + 1], c->stream));
-        ++c->tm_n;
-        launch_mv_finalize(c->stream, mode, c->part, tiles_of(c->g.nuv()), 1, ks);
-      }
+                         c->part, ks, true, timed ? c->tm_ev[2 * c->tm_n] : nullptr,
+                         timed ? c->tm_ev[2 * c->tm_n + 1] : nullptr);
+      if (timed) ++c->tm_n;
       break;
     }
     case DRD_OP_PHMU:
@@ -1728,6 +1724,15 @@ extern "C" int drd_kernel_timer_read(drd_ctx* c, int32_t* count, double* total_m
   *count = c->tm_n;
   *total_ms = tot;
   *bytes = 8.0 * (3.0 * (double)c->g.nuv() + 3.0 * (double)c->g.ncell());  // x, y, rtld + rho, eta, chi
+  return 0;
+}
+
+extern "C" int drd_kernel_timer_launches(drd_ctx* c, int32_t capacity, float* ms, int32_t* count) {
+  if (!c || !count || capacity < 0 || (capacity > 0 && !ms)) return set_err(DRD_E_ARG, "bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->tm_n && i < capacity; ++i)
+    HIPCHK(hipEventElapsedTime(&ms[i], c->tm_ev[2 * i], c->tm_ev[2 * i + 1]));
+  *count = c->tm_n;
   return 0;
 }
 

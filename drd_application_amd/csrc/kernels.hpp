@@ -116,10 +116,13 @@ inline void launch_mv_finalize(hipStream_t st, int mode, const double* part, int
 }
 
 // ---- launchers (kernels_stencil.hip) ----
-// finish = false leaves the partial sums unreduced (kernel timing only)
+// finish = false leaves the partial sums unreduced (kernel timing only);
+// with events ev0, ev1 the sweep's own start and end are recorded in them
+// (the dispatch's timestamps, not markers queued around it)
 void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks, bool finish = true);
+                        const double* b, double* part, KState* ks, bool finish = true,
+                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
                         const double* b, double* part, KState* ks, bool finish = true);

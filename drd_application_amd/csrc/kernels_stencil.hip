@@ -6,6 +6,8 @@
 // packed Krylov vector; a workgroup covers kTile consecutive unknowns, so
 // neighbouring rows of a tile are served from L2 and each HBM line of x is
 // fetched once per sweep.
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 
 namespace drd {
@@ -662,12 +664,13 @@ static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1
 
 void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
-                        const double* b, double* part, KState* ks, bool finish) {
+                        const double* b, double* part, KState* ks, bool finish, hipEvent_t ev0,
+                        hipEvent_t ev1) {
   const unsigned ntiles = nblk(g.nuv(), kTile);
   dim3 grid((ntiles + 1) / 2, 1, bat_n());  // two tiles per workgroup
-#define VM(B, M)                                                                                     \
-  hipLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, y, \
-                     alpha, beta, a0, b, part, ks, bat_bs())
+#define VM(B, M)                                                                                                \
+  hipExtLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, ev0, ev1, 0u, g, s, c.rho, c.eta,     \
+                        c.chi, x, y, alpha, beta, a0, b, part, ks, bat_bs())
   if (bdf == 1) {
     switch (mode) {
       case MV_PLAIN: VM(1, MV_PLAIN); break;

@@ -66,6 +66,7 @@ def lib():
         L.drd_bench_kernel.argtypes = [V, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.drd_kernel_timer.argtypes = [V, C.c_int32, C.c_int32]
         L.drd_kernel_timer_read.argtypes = [V, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.drd_kernel_timer_launches.argtypes = [V, C.c_int32, C.POINTER(C.c_float), I]
         F = C.POINTER(C.c_float)
         L.drd_learn_init.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
         L.drd_learn_collect.argtypes = [V, P, C.c_double, P, I, I]
@@ -349,6 +350,15 @@ class GpuSolver:
         n, ms, by = C.c_int32(0), C.c_double(0), C.c_double(0)
         _check(lib().drd_kernel_timer_read(self._h, C.byref(n), C.byref(ms), C.byref(by)))
         return n.value, ms.value, by.value
+
+    def kernel_timer_launches(self):
+        """Durations (ms, float32 array) of the timed launches in launch order."""
+        n = C.c_int32(0)
+        _check(lib().drd_kernel_timer_launches(self._h, 0, None, C.byref(n)))
+        out = np.zeros(n.value, dtype=np.float32)
+        _check(lib().drd_kernel_timer_launches(self._h, n.value, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                               C.byref(n)))
+        return out
 
 
 class GpuBatch:
