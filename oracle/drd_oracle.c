@@ -546,7 +546,7 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
   double* zz = c->w5;
   atob_uv(c, f11, f22, zz);
   OMP_FOR(big)
-  for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
+  for (int64_t q = 0; q < n; ++q) yy[q] = beta == 0.0 ? alpha * zz[q] : alpha * zz[q] + beta * yy[q];
 }
 
 /* PSolve_uv_CG (Jacobi) TP/VeloSolve_combined.f90:679-741.  The v rows
@@ -621,7 +621,7 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
   double* zz = c->w5;
   atob_ph_mu(c, ff1, ff2, zz);
   OMP_FOR(big)
-  for (int64_t q = 0; q < n; ++q) yy[q] = alpha * zz[q] + beta * yy[q];
+  for (int64_t q = 0; q < n; ++q) yy[q] = beta == 0.0 ? alpha * zz[q] : alpha * zz[q] + beta * yy[q];
 }
 
 /* PSolve_PH_MU_CG (Jacobi) TP/ph_mu_solve.f90:304-356.  The x test reads
@@ -706,7 +706,8 @@ static void ph_matvec(oc_ctx* c, double alpha, const double* xx, double beta, do
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       size_t q = i + (size_t)nx * j;
-      yy[q] = alpha * (f1[CI(i, j)] + ff2[CI(i, j)]) + beta * yy[q];
+      const double z = f1[CI(i, j)] + ff2[CI(i, j)];
+      yy[q] = beta == 0.0 ? alpha * z : alpha * z + beta * yy[q];
     }
 }
 
@@ -1190,7 +1191,7 @@ static void mv_dense(void* u, double alpha, const double* x, double beta, double
   for (int i = 0; i < d->n; ++i) {
     double s = 0;
     for (int j = 0; j < d->n; ++j) s += d->a[i + (size_t)d->n * j] * x[j];
-    y[i] = alpha * s + beta * y[i];
+    y[i] = beta == 0.0 ? alpha * s : alpha * s + beta * y[i];
   }
 }
 static void ps_dense(void* u, double* xo, const double* bi) {

@@ -58,6 +58,8 @@ int oc_ph_mu_rhs(oc_ctx* c, double* rhs);
 int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t,
                drd_solve_info* info);
 
+/* y = alpha A x + beta y; with beta == 0, y is not read (as in BLAS), so it
+ * may hold anything, NaN included                                          */
 int oc_matvec(oc_ctx* c, int32_t op, double alpha, const double* x,
               double beta, double* y);
 int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out,

@@ -238,6 +238,24 @@ def _coef_oracle(nx, ny, seed, lr="n"):
     return p, o
 
 
+@pytest.mark.parametrize("op", [abi.OP_VELO1, abi.OP_VELO2, abi.OP_PHMU])
+def test_matvec_with_beta_zero_does_not_read_y(op):
+    """y = alpha A x + beta y with beta = 0 leaves y unread (BLAS), as the
+    GPU sweeps do: Krylov bases hand the matvec fresh, uninitialised memory,
+    and 0 * NaN would otherwise poison the result."""
+    cfg = geometry.config("tiny")
+    o = oc.Oracle(cfg.params)
+    geometry.load(o, cfg)
+    n = abi.op_size(cfg.params, op)
+    x = np.random.default_rng(4).standard_normal(n)
+    want = o.matvec(op, 1.5, x, 0.0, np.zeros(n))
+    got = o.matvec(op, 1.5, x, 0.0, np.full(n, np.nan))
+    assert np.all(np.isfinite(want))
+    np.testing.assert_array_equal(got, want)
+    y0 = np.arange(n, dtype=np.float64)
+    np.testing.assert_array_equal(o.matvec(op, 1.5, x, 2.0, y0.copy()), want + 2.0 * y0)
+
+
 @pytest.mark.parametrize("op", [abi.OP_VELO1, abi.OP_VELO2])
 @pytest.mark.parametrize("lr", ["n", "d"])
 def test_velocity_operator_symmetric_positive(op, lr):
