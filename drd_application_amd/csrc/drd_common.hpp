@@ -129,13 +129,38 @@ constexpr int kSlots = kTile / kBlock;      // 4 elements per thread
 constexpr int kChunks = kSlots * kWaves;    // 16 chunks of 64 per tile
 constexpr int64_t kMaxTiles = (int64_t)kBlock * kBlock;  // per segment
 
-// Pairwise sum of the wave's 64 values (lane l holds element l).  Lane l
-// adds its xor-partner's subtotal at every level; both partners form the
-// same sum because IEEE addition commutes, so every lane ends with it.
+// v as seen from another lane of the same row of 16 (DPP lane permutation
+// CTRL; the whole wave is active wherever this is used).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Pairwise sum of the wave's 64 values (lane l holds element l), on lane 0.
+// Levels 1-4 add aligned groups of 2, 4, 8 and 16 lanes: every lane adds the
+// subtotal of a lane in the sibling group, taken by a DPP permutation (quad
+// swaps 1 and 2, half-row and row mirrors), so every lane of a group ends
+// with the group's sum (x + y and y + x are the same IEEE sum) and lane 0
+// forms S[0, 2m) = S[0, m) + S[m, 2m) at each level.  The row sums R0..R3 of
+// lanes 0, 16, 32 and 48 are then added as (R0 + R1) + (R2 + R3).  The
+// whole wave must be active; only lane 0's result is meaningful.
 __device__ __forceinline__ double wave_pairwise(double v) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp_f64<0x141>(v);  // row_half_mirror
+  v = v + dpp_f64<0x140>(v);  // row_mirror
+  const double r1 = readlane_f64(v, 16), r2 = readlane_f64(v, 32), r3 = readlane_f64(v, 48);
+  const double lo = v + r1, hi = r2 + r3;
+  return lo + hi;
 }
 
 template <int N>
