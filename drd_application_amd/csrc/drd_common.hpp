@@ -84,9 +84,12 @@ BatchCfg& batch_cfg();  // the calling host thread's current group (drd_api.hip)
 inline unsigned bat_n() { return (unsigned)batch_cfg().n; }
 inline int64_t bat_bs() { return batch_cfg().bs; }
 
+// The offset is pointer arithmetic on the argument itself (not a round trip
+// through an integer), so the compiler still knows the result points to
+// global memory and emits global loads and stores rather than flat ones.
 template <class T>
 __device__ __forceinline__ T* bofs(T* p, int64_t bs) {
-  return p ? (T*)((uintptr_t)p + (uintptr_t)((int64_t)blockIdx.z * bs)) : p;
+  return p ? (T*)((const char*)p + (int64_t)blockIdx.z * bs) : p;
 }
 
 // Scalars of module `global` the kernels read (TP/global.f90:42-52).

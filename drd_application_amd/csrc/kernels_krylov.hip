@@ -51,16 +51,26 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(int64_t n, const double* __re
   const double omega = st->omega;
   const double beta = first ? 0.0 : (st->rho / st->rho1) * (st->alpha / st->omega);
   const int64_t base = (int64_t)blockIdx.x * kTile;
+  // every slot's loads first (indices past n clamped to n-1, their values
+  // unused), so they are all in flight together; then the updates
+  double rr[kSlots], pp[kSlots], vv[kSlots];
 #pragma unroll
-  for (int j = 0; j < kTile / kBlock; ++j) {
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t q = base + j * kBlock + threadIdx.x, qc = q < n ? q : n - 1;
+    rr[j] = r[qc];
+    pp[j] = p[qc];
+    vv[j] = v[qc];
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
     const int64_t q = base + j * kBlock + threadIdx.x;
     if (q < n) {
       if (first) {
-        p[q] = r[q];
+        p[q] = rr[j];
       } else {
-        double t = p[q] + (-omega) * v[q];
+        double t = pp[j] + (-omega) * vv[j];
         t = beta * t;
-        p[q] = t + r[q];
+        p[q] = t + rr[j];
       }
     }
   }
@@ -77,7 +87,15 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
   v = bofs(v, bs);
   part = bofs(part, bs);
   const double alpha = st->alpha;
-  double t[NS][kSlots];
+  double t[NS][kSlots], rr[NS][kSlots], vv[NS][kSlots];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {  // loads first (clamped past the segment's end)
+      const int64_t q = tile_elem(seg, s, j), qc = q >= 0 ? q : (int64_t)s * seg + seg - 1;
+      rr[s][j] = r[qc];
+      vv[s][j] = v[qc];
+    }
 #pragma unroll
   for (int s = 0; s < NS; ++s)
 #pragma unroll
@@ -85,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
       const int64_t q = tile_elem(seg, s, j);
       double sv = 0.0;
       if (q >= 0) {
-        sv = r[q] + (-alpha) * v[q];
+        sv = rr[s][j] + (-alpha) * vv[s][j];
         r[q] = sv;
       }
       t[s][j] = sv * sv;
@@ -136,6 +154,22 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restr
   t = bofs(t, bs);
   part = bofs(part, bs);
   const double alpha = st->alpha, omega = st->omega;
+  // every slot's six loads first (clamped past the segment's end, values
+  // unused there), all in flight together; each element is still read
+  // before its own thread writes it, which the aliasing above needs
+  double lx[NS][kSlots], lph[NS][kSlots], lsh[NS][kSlots], lr[NS][kSlots], lt[NS][kSlots], lrt[NS][kSlots];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const int64_t q = tile_elem(seg, s, j), qc = q >= 0 ? q : (int64_t)s * seg + seg - 1;
+      lx[s][j] = x[qc];
+      lph[s][j] = ph[qc];
+      lsh[s][j] = sh[qc];
+      lr[s][j] = r[qc];
+      lt[s][j] = t[qc];
+      lrt[s][j] = rt[qc];
+    }
   double tm[NS * 2][kSlots];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -144,12 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restr
       const int64_t q = tile_elem(seg, s, j);
       double rv = 0.0, rtv = 0.0;
       if (q >= 0) {
-        double xv = x[q] + alpha * ph[q];
-        xv = xv + omega * sh[q];
+        double xv = lx[s][j] + alpha * lph[s][j];
+        xv = xv + omega * lsh[s][j];
         x[q] = xv;
-        rv = r[q] + (-omega) * t[q];
+        rv = lr[s][j] + (-omega) * lt[s][j];
         r[q] = rv;
-        rtv = rt[q];
+        rtv = lrt[s][j];
       }
       tm[s * 2][j] = rv * rv;
       tm[s * 2 + 1][j] = rtv * rv;

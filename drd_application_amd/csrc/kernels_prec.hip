@@ -26,15 +26,21 @@
 namespace drd {
 
 // ---- CH block Jacobi --------------------------------------------------------
-// diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219)
+// diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219);
+// the neighbour mobilities are loaded unconditionally (clamped indices) and
+// only the sums are selected, so a thread issues all its loads at once
 __device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2, const double* __restrict__ mo,
                                            int i, int j) {
+  const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
+  const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
   const double m0 = mo[g.ci(i, j)];
+  const double tl = (mo[g.ci(il, j)] + m0) / 2.0, tr = (m0 + mo[g.ci(ir, j)]) / 2.0;
+  const double bl = (mo[g.ci(i, jl)] + m0) / 2.0, br = (m0 + mo[g.ci(i, jr)]) / 2.0;
   double dx = 0.0, dy = 0.0;
-  if (i > 0) dx += (mo[g.ci(i - 1, j)] + m0) / 2.0;
-  if (i < g.nx - 1) dx += (m0 + mo[g.ci(i + 1, j)]) / 2.0;
-  if (j > 0) dy += (mo[g.ci(i, j - 1)] + m0) / 2.0;
-  if (j < g.ny - 1) dy += (m0 + mo[g.ci(i, j + 1)]) / 2.0;
+  dx = i > 0 ? dx + tl : dx;
+  dx = i < g.nx - 1 ? dx + tr : dx;
+  dy = j > 0 ? dy + bl : dy;
+  dy = j < g.ny - 1 ? dy + br : dy;
   return -(dx / dx2 + dy / dy2);
 }
 
@@ -103,19 +109,19 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p_block(Grid g, Scal s, const d
   for (int jj = 0; jj < kSlots; ++jj) {
     const int64_t q = tile_elem(C, 0, jj);
     if (q < 0) break;
+    // p and v are read in the first iteration too (unused there), so all
+    // loads of the slot issue together
     double pv[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int64_t e = q + h * C;
-      if (first) {
-        pv[h] = r[e];
-      } else {
-        double t = p[e] + (-omega) * v[e];
-        t = beta * t;
-        pv[h] = t + r[e];
-      }
-      p[e] = pv[h];
+      const double re = r[e], pe = p[e], ve = v[e];
+      double t = pe + (-omega) * ve;
+      t = beta * t;
+      pv[h] = first ? re : t + re;
     }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) p[q + h * C] = pv[h];
     const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
     phmu_block_solve(g, s, bu, minner, m0, i, j, pv[0], pv[1], ph[q], ph[C + q]);
   }

@@ -384,39 +384,29 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
 
 // op_double_gradient_Neum TP/matrix_tool.f90:178-219 at cell (i,j), reading
 // the unknown through `get` (interior values only) and mobility `mo`.
+// The reference's boundary branches replace the missing neighbour's value
+// and mobility by the cell's own (i = 0: moL = (m0 + m0)/2 and r0 in place of
+// the left value; likewise on the other walls).  Clamping the neighbour
+// index to the cell does exactly that with the interior expression, term
+// for term, so every load is unconditional and a thread issues them all at
+// once (nx, ny >= 2, which drd_init requires).
 template <class F>
 __device__ __forceinline__ double op_dgn(const Grid g, double dx2, double dy2, const double* __restrict__ mo,
                                          int i, int j, F get) {
   const int nx = g.nx, ny = g.ny;
+  const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
+  const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
   const double m0 = mo[g.ci(i, j)];
   const double r0 = get(i, j);
+  const double mxl = mo[g.ci(il, j)], mxr = mo[g.ci(ir, j)], myl = mo[g.ci(i, jl)], myr = mo[g.ci(i, jr)];
+  const double xl = get(il, j), xr = get(ir, j), yl = get(i, jl), yr = get(i, jr);
   double moL, moR, rhs = 0.0;
-  if (i == 0) {
-    moL = (m0 + m0) / 2.0;
-    moR = (m0 + mo[g.ci(i + 1, j)]) / 2.0;
-    rhs = rhs + 1.0 / dx2 * (moL * r0 + moR * get(i + 1, j) - (moL + moR) * r0);
-  } else if (i == nx - 1) {
-    moL = (mo[g.ci(i - 1, j)] + m0) / 2.0;
-    moR = (m0 + m0) / 2.0;
-    rhs = rhs + 1.0 / dx2 * (moL * get(i - 1, j) + moR * r0 - (moL + moR) * r0);
-  } else {
-    moL = (mo[g.ci(i - 1, j)] + m0) / 2.0;
-    moR = (m0 + mo[g.ci(i + 1, j)]) / 2.0;
-    rhs = rhs + 1.0 / dx2 * (moL * get(i - 1, j) + moR * get(i + 1, j) - (moL + moR) * r0);
-  }
-  if (j == 0) {
-    moL = (m0 + m0) / 2.0;
-    moR = (m0 + mo[g.ci(i, j + 1)]) / 2.0;
-    rhs = rhs + 1.0 / dy2 * (moL * r0 + moR * get(i, j + 1) - (moL + moR) * r0);
-  } else if (j == ny - 1) {
-    moL = (mo[g.ci(i, j - 1)] + m0) / 2.0;
-    moR = (m0 + m0) / 2.0;
-    rhs = rhs + 1.0 / dy2 * (moL * get(i, j - 1) + moR * r0 - (moL + moR) * r0);
-  } else {
-    moL = (mo[g.ci(i, j - 1)] + m0) / 2.0;
-    moR = (m0 + mo[g.ci(i, j + 1)]) / 2.0;
-    rhs = rhs + 1.0 / dy2 * (moL * get(i, j - 1) + moR * get(i, j + 1) - (moL + moR) * r0);
-  }
+  moL = (mxl + m0) / 2.0;
+  moR = (m0 + mxr) / 2.0;
+  rhs = rhs + 1.0 / dx2 * (moL * xl + moR * xr - (moL + moR) * r0);
+  moL = (myl + m0) / 2.0;
+  moR = (m0 + myr) / 2.0;
+  rhs = rhs + 1.0 / dy2 * (moL * yl + moR * yr - (moL + moR) * r0);
   return rhs;
 }
 
