@@ -1,7 +1,8 @@
 // fft_regs.hpp -- complex line FFTs with the butterflies in registers.
 //
-// One workgroup of kBlock threads transforms a complex line of power-of-two
-// length L <= 8192 held in LDS, in place.  The transform is a mixed-radix
+// One workgroup of NT threads (kBlock unless a kernel says otherwise)
+// transforms a complex line of power-of-two length L <= 8192 held in LDS, in
+// place.  The transform is a mixed-radix
 // Stockham FFT: one pass of radix 2, 4 or 8 when log2 L is not a multiple
 // of 4, then radix-16 passes.  In a pass each group j of R inputs
 //   v[t] = A[j + t L/R],  t < R,
@@ -78,9 +79,9 @@ __device__ __forceinline__ void dft_regs(Cx<T> (&v)[R], T sign) {
   }
 }
 
-// Groups of a radix-R pass one thread holds for lines up to LMAX: the
-// radix-R pass (R < 16) occurs only when log2 L = log2 R (mod 4).
-__host__ __device__ constexpr int pass_groups(int R, int LMAX) {
+// Groups of a radix-R pass one of NT threads holds for lines up to LMAX:
+// the radix-R pass (R < 16) occurs only when log2 L = log2 R (mod 4).
+__host__ __device__ constexpr int pass_groups(int R, int LMAX, int NT) {
   int L = 0;
   if (R == 16) {
     L = LMAX;
@@ -89,19 +90,19 @@ __host__ __device__ constexpr int pass_groups(int R, int LMAX) {
     for (int p = 0; (1 << p) <= LMAX; ++p)
       if ((p & 3) == r) L = 1 << p;
   }
-  const int g = L / R / kBlock;
+  const int g = L / R / NT;
   return g > 1 ? g : 1;
 }
 
 // One Stockham pass of radix R over the LDS line A of length L.
 // tw[m] = exp(-2 pi i m / L), m < L.
-template <class T, int R, int G>
+template <class T, int R, int G, int NT>
 __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<T>* __restrict__ tw, T sign) {
   const int groups = L / R;
   Cx<T> v[G][R];
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int j = (int)threadIdx.x + g * kBlock;
+    const int j = (int)threadIdx.x + g * NT;
     if (j < groups) {
       const int k = j & (Ns - 1);
 #pragma unroll
@@ -122,7 +123,7 @@ __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<
   __syncthreads();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    const int j = (int)threadIdx.x + g * kBlock;
+    const int j = (int)threadIdx.x + g * NT;
     if (j < groups) {
       const int k = j & (Ns - 1);
       const int o = (j - k) * R + k;
@@ -136,30 +137,30 @@ __device__ __forceinline__ void stockham_pass(Cx<T>* A, int L, int Ns, const Cx<
 // Complex FFT of the LDS line A (length L, power of two, 1 <= L <= LMAX <=
 // 8192, element i at A[lp(i)]) in place; the caller has synchronised after
 // filling A.  LMAX fixes
-// how many groups a thread holds per pass (1 for LMAX = 4096).
-template <class T, int LMAX>
+// how many groups a thread holds per pass (1 for LMAX = 4096 and NT = 256).
+template <class T, int LMAX, int NT = kBlock>
 __device__ void fft_lds(Cx<T>* A, int L, const Cx<T>* __restrict__ tw, T sign) {
   int p = 0;
   while ((1 << p) < L) ++p;
   int Ns = 1;
   switch (p & 3) {
     case 1:
-      stockham_pass<T, 2, pass_groups(2, LMAX)>(A, L, Ns, tw, sign);
+      stockham_pass<T, 2, pass_groups(2, LMAX, NT), NT>(A, L, Ns, tw, sign);
       Ns = 2;
       break;
     case 2:
-      stockham_pass<T, 4, pass_groups(4, LMAX)>(A, L, Ns, tw, sign);
+      stockham_pass<T, 4, pass_groups(4, LMAX, NT), NT>(A, L, Ns, tw, sign);
       Ns = 4;
       break;
     case 3:
-      stockham_pass<T, 8, pass_groups(8, LMAX)>(A, L, Ns, tw, sign);
+      stockham_pass<T, 8, pass_groups(8, LMAX, NT), NT>(A, L, Ns, tw, sign);
       Ns = 8;
       break;
     default:
       break;
   }
   for (int q = 0; q < (p >> 2); ++q) {
-    stockham_pass<T, 16, pass_groups(16, LMAX)>(A, L, Ns, tw, sign);
+    stockham_pass<T, 16, pass_groups(16, LMAX, NT), NT>(A, L, Ns, tw, sign);
     Ns *= 16;
   }
 }

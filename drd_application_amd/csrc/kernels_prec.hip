@@ -255,12 +255,12 @@ __device__ __forceinline__ int makhoul_ix(int j, int n) { return (j & 1) ? (n - 
 
 // REDFT00 of n = N+1 points: FFT of the even extension (length 2N); both
 // spectra are real, so line 0 is Re V and line 1 is Im V.
-template <class T, int LMAX, class Load, class Store>
+template <class T, int LMAX, int NT = kBlock, class Load, class Store>
 __device__ void dct1_pair(Cx<T>* A, int N, const Cx<T>* __restrict__ tw, Load load, Store store) {
   const int L = 2 * N;
   for (int m = threadIdx.x; m < L; m += blockDim.x) A[lp(m)] = load(m <= N ? m : L - m);
   __syncthreads();
-  fft_lds<T, LMAX>(A, L, tw, T(-1));
+  fft_lds<T, LMAX, NT>(A, L, tw, T(-1));
   for (int k = threadIdx.x; k <= N; k += blockDim.x) store(k, A[lp(k)].x, A[lp(k)].y);
   __syncthreads();
 }
@@ -269,7 +269,7 @@ __device__ void dct1_pair(Cx<T>* A, int N, const Cx<T>* __restrict__ tw, Load lo
 // reversed) by Makhoul's reordering; the two spectra are separated by
 // V0_k = (V_k + conj V_{-k}) / 2, V1_k = (V_k - conj V_{-k}) / 2i, and
 // y_k = 2 Re(exp(-i pi k / 2n) V_k).
-template <class T, int LMAX, class Load, class Store>
+template <class T, int LMAX, int NT = kBlock, class Load, class Store>
 __device__ void dct2_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const Cx<T>* __restrict__ rot, bool dst,
                           Load load, Store store) {
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -278,7 +278,7 @@ __device__ void dct2_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
     A[lp(makhoul_ix(j, n))] = v;
   }
   __syncthreads();
-  fft_lds<T, LMAX>(A, n, tw, T(-1));
+  fft_lds<T, LMAX, NT>(A, n, tw, T(-1));
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const Cx<T> V = A[lp(k)], W = A[lp((n - k) & (n - 1))];
     const Cx<T> sp{V.x + W.x, V.y - W.y}, sm{V.x - W.x, V.y + W.y};  // V +- conj W
@@ -291,7 +291,7 @@ __device__ void dct2_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
 // REDFT01 (dst = false) or RODFT01 (dst = true) of the pair held where
 // load(i) reads it (random access): inverse Makhoul; the two inverse FFT
 // outputs are real, so they separate as Re and Im.
-template <class T, int LMAX, class Load, class Store>
+template <class T, int LMAX, int NT = kBlock, class Load, class Store>
 __device__ void dct3_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const Cx<T>* __restrict__ rot, bool dst,
                           Load load, Store store) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -307,7 +307,7 @@ __device__ void dct3_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
     A[lp(k)] = Cx<T>{a0.x - a1.y, a0.y + a1.x};
   }
   __syncthreads();
-  fft_lds<T, LMAX>(A, n, tw, T(1));
+  fft_lds<T, LMAX, NT>(A, n, tw, T(1));
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     Cx<T> v = A[lp(makhoul_ix(j, n))];
     if (dst && (j & 1)) v = Cx<T>{-v.x, -v.y};
@@ -318,7 +318,7 @@ __device__ void dct3_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, const C
 
 // RODFT00 of n points: FFT of the odd extension (length 2(n+1)); both
 // spectra are imaginary: y0_k = -Im V_{k+1}, y1_k = Re V_{k+1}.
-template <class T, int LMAX, class Load, class Store>
+template <class T, int LMAX, int NT = kBlock, class Load, class Store>
 __device__ void dst1_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, Load load, Store store) {
   const int L = 2 * (n + 1);
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
@@ -331,7 +331,7 @@ __device__ void dst1_pair(Cx<T>* A, int n, const Cx<T>* __restrict__ tw, Load lo
     A[lp(m)] = v;
   }
   __syncthreads();
-  fft_lds<T, LMAX>(A, L, tw, T(-1));
+  fft_lds<T, LMAX, NT>(A, L, tw, T(-1));
   for (int k = threadIdx.x; k < n; k += blockDim.x) store(k, -A[lp(k + 1)].y, A[lp(k + 1)].x);
   __syncthreads();
 }
@@ -401,10 +401,19 @@ __global__ __launch_bounds__(kBlock) void k_prec_rows(Grid g, PrecTables<T> tb, 
 // pass 2 over columns of the transposed blocks, two columns per workgroup:
 // u columns (pairs [0, nu)), v columns ([nu, nu + nv)); forward transform,
 // divide by the eigenvalue, inverse.  The divided spectra wait in LDS
-// behind the FFT line.
+// behind the FFT line.  A 256-thread workgroup per pair leaves threads idle
+// in the radix-16 passes of a 1024-point line, but one wave per pair was
+// slower still (measured: 128 against 106 us per application at 2048 x
+// 1024): the LDS a pair needs then allows only six waves per CU.
 template <class T, int LMAX>
-__global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, T* __restrict__ tu,
-                                                      T* __restrict__ tv, const KState* st, int64_t bs) {
+constexpr int cols_threads() {
+  return kBlock;
+}
+template <class T, int LMAX>
+__global__ __launch_bounds__((cols_threads<T, LMAX>())) void k_prec_cols(Grid g, PrecTables<T> tb, T* __restrict__ tu,
+                                                                       T* __restrict__ tv, const KState* st,
+                                                                       int64_t bs) {
+  constexpr int NT = cols_threads<T, LMAX>();
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
   tu = bofs(tu, bs);
@@ -421,14 +430,14 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
     T* col0 = tu + (int64_t)c0 * n;
     T* col1 = col0 + n;
     const T e0 = tb.a0 + tb.ex_u[c0], e1 = has1 ? tb.a0 + tb.ex_u[c0 + 1] : T(1);
-    dct2_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true,
+    dct2_pair<T, LMAX, NT>(A, n, tb.twny, tb.rotny, true,
                  [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
                  [&](int k, T y0, T y1) {
                    const T ey = tb.ey_u[k];
                    l0[k] = y0 / (tb.sc * (e0 + ey));
                    l1[k] = has1 ? y1 / (tb.sc * (e1 + ey)) : T(0);
                  });
-    dct3_pair<T, LMAX>(A, n, tb.twny, tb.rotny, true, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
+    dct3_pair<T, LMAX, NT>(A, n, tb.twny, tb.rotny, true, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int j, T x0, T x1) {
                    col0[j] = x0;
                    if (has1) col1[j] = x1;
@@ -439,13 +448,13 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
     T* col0 = tv + (int64_t)c0 * n;
     T* col1 = col0 + n;
     const T e0 = tb.a0 + tb.ex_v[c0], e1 = has1 ? tb.a0 + tb.ex_v[c0 + 1] : T(1);
-    dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
+    dst1_pair<T, LMAX, NT>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
                  [&](int k, T y0, T y1) {
                    const T ey = tb.ey_v[k];
                    l0[k] = y0 / (tb.sc * (e0 + ey));
                    l1[k] = has1 ? y1 / (tb.sc * (e1 + ey)) : T(0);
                  });
-    dst1_pair<T, LMAX>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
+    dst1_pair<T, LMAX, NT>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int k, T y0, T y1) {
                    col0[k] = y0;
                    if (has1) col1[k] = y1;
@@ -453,17 +462,30 @@ __global__ __launch_bounds__(kBlock) void k_prec_cols(Grid g, PrecTables<T> tb, 
   }
 }
 
-// out (cols x rows) = transpose of in (rows x cols) through a TR x TC LDS
-// tile (TR rows of in, TC columns); the tile shape is the autotuner's choice.
+// One block of a transpose: out (cols x rows) = in^T, in row-major rows x cols.
+template <class T>
+struct TBlock {
+  int rows, cols;
+  const T* in;
+  T* out;
+};
+
+// The u and the v block of a pass transposed by one launch: grid rows
+// [0, ya) tile block a, the others block b, and the grid is as wide as the
+// wider block.  Each tile goes through a TR x TC LDS tile (TR rows of in,
+// TC columns); the tile shape is the autotuner's choice.
 template <class T, int TR, int TC>
-__global__ __launch_bounds__(kBlock) void k_transposeT(int rows, int cols, const T* __restrict__ in,
-                                                       T* __restrict__ out, const KState* st, int64_t bs) {
+__global__ __launch_bounds__(kBlock) void k_transpose2T(TBlock<T> a, TBlock<T> b, int ya, const KState* st,
+                                                        int64_t bs) {
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
-  in = bofs(in, bs);
-  out = bofs(out, bs);
+  const bool in_a = (int)blockIdx.y < ya;
+  const int rows = in_a ? a.rows : b.rows, cols = in_a ? a.cols : b.cols;
+  const T* __restrict__ in = bofs(in_a ? a.in : b.in, bs);
+  T* __restrict__ out = bofs(in_a ? a.out : b.out, bs);
+  const int bx = blockIdx.x * TC, by = (in_a ? (int)blockIdx.y : (int)blockIdx.y - ya) * TR;
+  if (bx >= cols) return;  // the narrower block's surplus tiles (whole workgroup)
   __shared__ T tile[TR][TC + 1];
-  const int bx = blockIdx.x * TC, by = blockIdx.y * TR;
   {
     const int tx = threadIdx.x % TC;
     for (int j = threadIdx.x / TC; j < TR; j += kBlock / TC) {
@@ -487,15 +509,18 @@ constexpr int kShapeTR[kTransposeShapes] = {32, 16, 64};
 constexpr int kShapeTC[kTransposeShapes] = {32, 64, 16};
 
 template <class T>
-static void launch_transposeT(hipStream_t st, int shape, int rows, int cols, const T* in, T* out, const KState* ks,
-                              unsigned B, int64_t bs) {
+static void launch_transpose2T(hipStream_t st, int shape, TBlock<T> a, TBlock<T> b, const KState* ks, unsigned B,
+                               int64_t bs) {
   const int TR = kShapeTR[shape], TC = kShapeTC[shape];
-  const dim3 grid((cols + TC - 1) / TC, (rows + TR - 1) / TR, B);
+  const int ya = (a.rows + TR - 1) / TR, yb = (b.rows + TR - 1) / TR;
+  const dim3 grid((unsigned)((std::max(a.cols, b.cols) + TC - 1) / TC), (unsigned)(ya + yb), B);
+#define DRD_T2(R, C) hipLaunchKernelGGL((k_transpose2T<T, R, C>), grid, dim3(kBlock), 0, st, a, b, ya, ks, bs)
   switch (shape) {
-    case 1: hipLaunchKernelGGL((k_transposeT<T, 16, 64>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
-    case 2: hipLaunchKernelGGL((k_transposeT<T, 64, 16>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
-    default: hipLaunchKernelGGL((k_transposeT<T, 32, 32>), grid, dim3(kBlock), 0, st, rows, cols, in, out, ks, bs); break;
+    case 1: DRD_T2(16, 64); break;
+    case 2: DRD_T2(64, 16); break;
+    default: DRD_T2(32, 32); break;
   }
+#undef DRD_T2
 }
 
 // ---- host side -------------------------------------------------------------------
@@ -658,7 +683,8 @@ void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s,
                      st, g, s, c.rho, c.eta, c.chi, pc->a0, pc->inv_s64, pc->inv_s32, bat_bs());
 }
 
-template <class T, int LMAX>
+// LMAX: longest row line (2nx); LMAXC: longest column line (2ny)
+template <class T, int LMAX, int LMAXC>
 static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T>& tb, const T* inv_s,
                                const double* r, double* z, T* wu, T* wv, T* tu, T* tv, unsigned rows_blocks,
                                unsigned cols_blocks, size_t lds_rows, size_t lds_cols, const KState* ks, int shape) {
@@ -670,19 +696,17 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
                               lim);
     (void)hipFuncSetAttribute((const void*)k_prec_rows<T, false, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
-    (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAXC>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     raised = true;
   }
   const unsigned B = bat_n();
   const int64_t bs = bat_bs();
   hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb, r,
                      inv_s, wu, wv, (double*)nullptr, ks, bs);
-  launch_transposeT<T>(st, shape, ny, nx + 1, wu, tu, ks, B, bs);
-  launch_transposeT<T>(st, shape, ny - 1, nx, wv, tv, ks, B, bs);
-  hipLaunchKernelGGL((k_prec_cols<T, LMAX>), dim3(cols_blocks, 1, B), dim3(kBlock), lds_cols, st, g, tb, tu, tv, ks,
-                     bs);
-  launch_transposeT<T>(st, shape, nx + 1, ny, tu, wu, ks, B, bs);
-  launch_transposeT<T>(st, shape, nx, ny - 1, tv, wv, ks, B, bs);
+  launch_transpose2T<T>(st, shape, TBlock<T>{ny, nx + 1, wu, tu}, TBlock<T>{ny - 1, nx, wv, tv}, ks, B, bs);
+  hipLaunchKernelGGL((k_prec_cols<T, LMAXC>), dim3(cols_blocks, 1, B), dim3(cols_threads<T, LMAXC>()), lds_cols, st,
+                     g, tb, tu, tv, ks, bs);
+  launch_transpose2T<T>(st, shape, TBlock<T>{nx + 1, ny, tu, wu}, TBlock<T>{nx, ny - 1, tv, wv}, ks, B, bs);
   hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb,
                      (const double*)nullptr, inv_s, wu, wv, z, ks, bs);
 }
@@ -702,12 +726,21 @@ static void apply_T(hipStream_t st, VeloPrec* pc, const Grid& g, const PrecTable
   const unsigned cols_blocks = (unsigned)((nx + 2) / 2 + (nx + 1) / 2);
   const size_t lds_rows = (size_t)lds_len(2 * nx) * sizeof(Cx<T>);
   const size_t lds_cols = (size_t)lds_len(2 * ny) * sizeof(Cx<T>) + 2 * (size_t)ny * sizeof(T);
-  if (std::max(nx, ny) <= 2048)
-    launch_prec_passes<T, 4096>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
-                                lds_cols, ks, pc->tshape);
-  else
-    launch_prec_passes<T, 8192>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows,
-                                lds_cols, ks, pc->tshape);
+#define DRD_PREC_PASSES(LR, LC)                                                                          \
+  launch_prec_passes<T, LR, LC>(st, g, tb, inv_s, r, z, wu, wv, tu, tv, rows_blocks, cols_blocks, lds_rows, \
+                                lds_cols, ks, pc->tshape)
+  if (nx <= 2048) {
+    if (ny <= 1024)
+      DRD_PREC_PASSES(4096, 2048);
+    else
+      DRD_PREC_PASSES(4096, 8192);
+  } else {
+    if (ny <= 1024)
+      DRD_PREC_PASSES(8192, 2048);
+    else
+      DRD_PREC_PASSES(8192, 8192);
+  }
+#undef DRD_PREC_PASSES
 }
 
 void velo_prec_apply(hipStream_t st, VeloPrec* pc, const Grid& g, const double* r, double* z, const KState* ks) {

@@ -46,13 +46,12 @@ struct VeloIn {
 // MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
 template <int BDF>
-__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const double* __restrict__ rho,
-                                           const double* __restrict__ eta,
-                                           const double* __restrict__ chi, const VeloIn& X,
-                                           uint32_t q) {
+__device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const double* __restrict__ rho,
+                                             const double* __restrict__ eta, const double* __restrict__ chi,
+                                             const VeloIn& X, uint32_t q) {
   const int nx = g.nx;
   double r;
-  if (q < X.mn) {
+  {
     const int k = g.by_nx1.div(q);
     const int i = (int)(q - (uint32_t)k * (uint32_t)(nx + 1));
     const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
@@ -69,7 +68,17 @@ __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const do
     cl = (e00 + e10 + ld32(eta, c - ld) + ld32(eta, cw - ld)) / 4.0;
     r = r - s.dt * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f) / s.dz2;
     r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
-  } else {
+  }
+  return r;
+}
+
+template <int BDF>
+__device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const double* __restrict__ rho,
+                                             const double* __restrict__ eta, const double* __restrict__ chi,
+                                             const VeloIn& X, uint32_t q) {
+  const int nx = g.nx;
+  double r;
+  {
     const uint32_t t = q - X.mn;
     const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (uint32_t)(k - 1) * (uint32_t)nx);
@@ -89,6 +98,13 @@ __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const do
     r = r - s.dt * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1))) / s.dr / s.dz;
   }
   return r;
+}
+
+template <int BDF>
+__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const double* __restrict__ rho,
+                                           const double* __restrict__ eta, const double* __restrict__ chi,
+                                           const VeloIn& X, uint32_t q) {
+  return q < X.mn ? velo_u_row<BDF>(g, s, rho, eta, chi, X, q) : velo_v_row<BDF>(g, s, rho, eta, chi, X, q);
 }
 
 __device__ __forceinline__ bool kactive(const KState* st) {
@@ -131,28 +147,50 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
     const uint32_t tile = pair + (uint32_t)h * half;
     if (tile >= nt) break;
     double tm[2][kSlots] = {};
+    // A x for the tile's four slots first, then the stores: a tile lies in
+    // the u half or the v half of the vector except the one tile across the
+    // boundary, so in the others the four rows are straight-line code whose
+    // loads the compiler can issue together.
+    const uint32_t t0 = tile * (uint32_t)kTile;
+    double z[kSlots];
+    const bool need = MODE != MV_INIT || xnz;
+    if (need && t0 + (uint32_t)kTile <= X.mn) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j)
+        z[j] = velo_u_row<BDF>(g, s, rho, eta, chi, X, t0 + (uint32_t)(j * kBlock) + threadIdx.x);
+    } else if (need && t0 >= X.mn) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const uint32_t q = t0 + (uint32_t)(j * kBlock) + threadIdx.x;
+        z[j] = velo_v_row<BDF>(g, s, rho, eta, chi, X, q < n ? q : n - 1);
+      }
+    } else if (need) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const uint32_t q = t0 + (uint32_t)(j * kBlock) + threadIdx.x;
+        z[j] = velo_row<BDF>(g, s, rho, eta, chi, X, q < n ? q : n - 1);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) {
-      const uint32_t q = tile * (uint32_t)kTile + (uint32_t)(j * kBlock) + threadIdx.x;
+      const uint32_t q = t0 + (uint32_t)(j * kBlock) + threadIdx.x;
       if (q >= n) break;
       if (MODE == MV_PLAIN) {
         // y is not read when beta == 0 (as in BLAS), so it may be uninitialised
-        const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
-        st32(y, q, beta == 0.0 ? alpha * z : alpha * z + beta * ld32((const double*)y, q));
+        st32(y, q, beta == 0.0 ? alpha * z[j] : alpha * z[j] + beta * ld32((const double*)y, q));
       } else if (MODE == MV_INIT) {
         double r = ld32(b, q);
-        if (xnz) r = -1.0 * velo_row<BDF>(g, s, rho, eta, chi, X, q) + 1.0 * r;
+        if (xnz) r = -1.0 * z[j] + 1.0 * r;
         st32(y, q, r);
         st32(a0, q, r);
         tm[0][j] = r * r;
       } else {
-        const double z = velo_row<BDF>(g, s, rho, eta, chi, X, q);
-        st32(y, q, z);
+        st32(y, q, z[j]);
         if (MODE == MV_DOT1) {
-          tm[0][j] = ld32((const double*)a0, q) * z;
+          tm[0][j] = ld32((const double*)a0, q) * z[j];
         } else {
-          tm[0][j] = z * ld32((const double*)a0, q);
-          tm[1][j] = z * z;
+          tm[0][j] = z[j] * ld32((const double*)a0, q);
+          tm[1][j] = z[j] * z[j];
         }
       }
     }
