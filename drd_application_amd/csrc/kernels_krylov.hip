@@ -294,29 +294,102 @@ __global__ __launch_bounds__(kBlock) void k_multiaxpy(int64_t seg, double* __res
 }
 
 // Canonical total of the preceding kernel's tile partials; see kernels.hpp.
-__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ part, uint32_t nt, int ns,
-                                                     int nv, double* out, KState* st, int phase, int guard,
-                                                     int64_t bs) {
+// kFinBlock threads: thread t adds the aligned run [t P, (t+1) P) of the
+// tiles padded to a power of two ntp, P = max(1, ntp / kFinBlock), in runs of
+// 8 (then the run sums), the waves add their threads' sums (wave_pairwise)
+// and thread 0 the 16 wave sums, all pairwise: the same tree as any other
+// aligned split of the padded range.  Two values are summed per pass (their
+// runs and run sums fill the 128 registers a thread of 1024 may hold), every
+// load of a pass issued before any is used (indices clamped, values masked).
+// kFinMaxK bounds ns * nv: FGMRES orthogonalises against at most restart + 2
+// <= 202 vectors (drd_fgmres accepts restart <= 200) of at most 2 segments.
+constexpr int kFinBlock = 1024, kFinWaves = kFinBlock / 64, kFinMaxK = 512, kFinPer = 2;
+__device__ __forceinline__ double pairwise8(const double (&r)[8]) {
+  return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+}
+__global__ __launch_bounds__(kFinBlock) void k_finalize(const double* __restrict__ part, uint32_t nt, int ns,
+                                                        int nv, double* out, KState* st, int phase, int guard,
+                                                        int64_t bs) {
   st = bofs(st, bs);
   if (guard == FIN_ACTIVE && (st->done || st->early)) return;
   part = bofs(part, bs);
   out = bofs(out, bs);
+  __shared__ double lw[kFinPer][kFinWaves];
+  __shared__ double res[kFinMaxK];
   const int K = ns * nv;
-  double tot[2] = {0.0, 0.0};
-  for (int v = 0; v < nv; ++v) {
-    double s = tiles_pairwise(part, nt, K, v);
-    for (int sg = 1; sg < ns; ++sg) {
-      const double x = tiles_pairwise(part, nt, K, sg * nv + v);
-      s = s + x;
+  uint32_t ntp = 1;
+  while (ntp < nt) ntp <<= 1;
+  const uint32_t P = ntp > (uint32_t)kFinBlock ? ntp / kFinBlock : 1u;  // 1 .. 64
+  const uint32_t runs = P / 8u, len = P;  // runs: P > 8; len: P <= 8
+  const uint32_t base = threadIdx.x * P;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k0 = 0; k0 < K; k0 += kFinPer) {
+    double tv[kFinPer];
+    if (P <= 8u) {  // one run per thread (up to 8192 tiles): one round of loads for both values
+      double r[kFinPer][8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t q = base + (uint32_t)i;
+        const bool ok = (uint32_t)i < len && q < nt;
+        const uint32_t qc = ok ? q : 0u;
+#pragma unroll
+        for (int v = 0; v < kFinPer; ++v) {
+          const int k = k0 + v < K ? k0 + v : k0;
+          const double x = part[(size_t)qc * K + k];
+          r[v][i] = ok ? x : 0.0;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < kFinPer; ++v) tv[v] = pairwise8(r[v]);
+    } else {  // larger grids: runs of 8, one value at a time
+#pragma unroll
+      for (int v = 0; v < kFinPer; ++v) {
+        const int k = k0 + v < K ? k0 + v : k0;
+        double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (uint32_t gi = 0; gi < runs; ++gi) {
+          double r[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const uint32_t q = base + gi * 8u + (uint32_t)i;
+            const double x = part[(size_t)(q < nt ? q : 0u) * K + k];
+            r[i] = q < nt ? x : 0.0;
+          }
+          const double sv = pairwise8(r);
+#pragma unroll
+          for (int gj = 0; gj < 8; ++gj)
+            if ((uint32_t)gj == gi) g[gj] = sv;
+        }
+        tv[v] = pairwise8(g);
+      }
     }
+#pragma unroll
+    for (int v = 0; v < kFinPer; ++v) {
+      const double sv = wave_pairwise(tv[v]);
+      if (lane == 0) lw[v][w] = sv;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+#pragma unroll
+      for (int v = 0; v < kFinPer; ++v)
+        if (k0 + v < K) res[k0 + v] = pairwise_fixed<kFinWaves>(lw[v]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double tot[2] = {0.0, 0.0};
+    for (int v = 0; v < nv; ++v) {
+      double s = res[v];
+      for (int sg = 1; sg < ns; ++sg) {
+        const double x = res[sg * nv + v];
+        s = s + x;
+      }
       if (phase == PH_NONE)
         out[v] = s;
       else if (v < 2)
         tot[v] = s;
     }
+    if (phase != PH_NONE) kfinalize(st, phase, tot);
   }
-  if (threadIdx.x == 0 && phase != PH_NONE) kfinalize(st, phase, tot);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_into(int64_t n, const double* a, double s, double* b) {
@@ -350,11 +423,11 @@ static inline unsigned nb(int64_t n, int per) { return (unsigned)((n + per - 1) 
 
 void launch_finalize(hipStream_t s, const double* part, int64_t ntiles, int nseg, int nv, double* out, KState* ks,
                      int phase, int guard) {
-  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bat_n()), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out,
-                     ks, phase, guard, bat_bs());
+  hipLaunchKernelGGL(k_finalize, dim3(1, 1, bat_n()), dim3(kFinBlock), 0, s, part, (uint32_t)ntiles, nseg, nv,
+                     out, ks, phase, guard, bat_bs());
 }
 static void launch_finalize1(hipStream_t s, const double* part, int64_t ntiles, int nseg, int nv, double* out) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, s, part, (uint32_t)ntiles, nseg, nv, out,
                      (KState*)nullptr, (int)PH_NONE, (int)FIN_ALWAYS, (int64_t)0);
 }
 
