@@ -845,13 +845,13 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
         RCHK(apply_prec(c, prec, op, P, PHb, &ph, c->ks));
       }
       RCHK(apply_op(c, op, MV_DOT1, ph, V, 1.0, 0.0, RT, nullptr, c->ks));
+      // after an early exit (||s|| <= tol) the kernels up to the x, r sweep
+      // skip, and that sweep makes the exit's update of x
       if (fused_block) {  // s, ||s||^2, shat = B^-1 s in one sweep
         launch_bicg_s_block(c->stream, c->g, c->s, coef_of(c), R, V, SHb, c->part, c->ks);
         sh = SHb;
-        launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
       } else {
         launch_bicg_s(c->stream, n, ns, R, V, c->part, c->ks);
-        launch_bicg_early(c->stream, n, x, ph, c->part, c->ks);
         RCHK(apply_prec(c, prec, op, R, SHb, &sh, c->ks));
       }
       RCHK(apply_op(c, op, MV_DOT2, sh, T, 1.0, 0.0, R, nullptr, c->ks));

@@ -151,7 +151,6 @@ void launch_init_norms(hipStream_t st, int64_t n, int nseg, const double* x, con
                        KState* ks);
 void launch_bicg_p(hipStream_t st, int64_t n, const double* r, const double* v, double* p, KState* ks);
 void launch_bicg_s(hipStream_t st, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks);
-void launch_bicg_early(hipStream_t st, int64_t n, double* x, const double* ph, double* part, KState* ks);
 void launch_bicg_xr(hipStream_t st, int64_t n, int nseg, double* x, double* r, const double* rt,
                     const double* ph, const double* sh, const double* t, double* part, KState* ks,
                     bool finish = true);

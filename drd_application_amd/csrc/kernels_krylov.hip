@@ -111,41 +111,22 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(int64_t seg, double* __restri
   tile_publish<NS, 1>(t, part);
 }
 
-// early exit: x = x + alpha*phat                   REVCOM:314
-// The last workgroup to finish (relaxed ticket) marks the solve done; every
-// workgroup has read `done` before it takes its ticket, and nothing in this
-// kernel reads x or the flag afterwards, so no fence is needed.  Only the
-// one early-exit iteration of a solve gets past the first test.
-__global__ __launch_bounds__(kBlock) void k_bicg_early(int64_t n, double* __restrict__ x,
-                                                       const double* __restrict__ ph, KState* st, int64_t bs) {
-  st = bofs(st, bs);
-  if (!st->early || st->done) return;
-  x = bofs(x, bs);
-  ph = bofs(ph, bs);
-  const double alpha = st->alpha;
-#pragma unroll
-  for (int j = 0; j < kSlots; ++j) {
-    const int64_t q = tile_elem(n, 0, j);
-    if (q >= 0) x[q] = x[q] + alpha * ph[q];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&st->eticket, 1u) == gridDim.x - 1) {
-    st->eticket = 0;
-    st->done = 1;
-  }
-}
-
 // x = x + alpha*phat + omega*shat; r = s - omega*t; ||r||^2, (rtld,r)
 // REVCOM:354-361 and the next iteration's rho (:263).  With the identity
 // preconditioner shat aliases r (and phat aliases p), so r and sh carry no
 // __restrict__; each element is read before it is written by its own thread.
+// After an early exit (||s|| <= tol, REVCOM:311-316; every kernel since the
+// s-update has skipped) the sweep makes the exit's update x = x + alpha*phat
+// (:314) instead, and its last workgroup to finish (relaxed ticket) marks
+// the solve done: every workgroup read the flags before taking its ticket,
+// and nothing here reads x or the flags afterwards, so no fence is needed.
 template <int NS>
 __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restrict__ x, double* r,
                                                     const double* __restrict__ rt, const double* __restrict__ ph,
                                                     const double* sh, const double* __restrict__ t,
                                                     double* part, KState* st, int64_t bs) {
   st = bofs(st, bs);
-  if (!kact(st)) return;
+  if (st->done) return;
   x = bofs(x, bs);
   r = bofs(r, bs);
   rt = bofs(rt, bs);
@@ -154,6 +135,21 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(int64_t seg, double* __restr
   t = bofs(t, bs);
   part = bofs(part, bs);
   const double alpha = st->alpha, omega = st->omega;
+  if (st->early) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        const int64_t q = tile_elem(seg, s, j);
+        if (q >= 0) x[q] = x[q] + alpha * ph[q];
+      }
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(&st->eticket, 1u) == gridDim.x - 1) {
+      st->eticket = 0;
+      st->done = 1;
+    }
+    return;
+  }
   // every slot's six loads first (clamped past the segment's end, values
   // unused there), all in flight together; each element is still read
   // before its own thread writes it, which the aliasing above needs
@@ -456,10 +452,6 @@ void launch_bicg_p(hipStream_t s, int64_t n, const double* r, const double* v, d
 void launch_bicg_s(hipStream_t s, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks) {
   DRD_SEG_LAUNCH(k_bicg_s, n, nseg, r, v, part, ks);
   launch_finalize(s, part, tiles_of(n / nseg), nseg, 1, nullptr, ks, PH_S, FIN_ACTIVE);
-}
-void launch_bicg_early(hipStream_t s, int64_t n, double* x, const double* ph, double* part, KState* ks) {
-  (void)part;
-  hipLaunchKernelGGL(k_bicg_early, dim3(nb(n, kTile), 1, bat_n()), dim3(kBlock), 0, s, n, x, ph, ks, bat_bs());
 }
 void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, const double* rt, const double* ph,
                     const double* sh, const double* t, double* part, KState* ks, bool finish) {

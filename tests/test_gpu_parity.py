@@ -181,6 +181,30 @@ def test_bicgstab_control_flow(op, prec):
     g.close()
 
 
+@pytest.mark.parametrize("op,prec", [(abi.OP_VELO2, abi.PS_VELO1_JACOBI), (abi.OP_VELO2, abi.PS_IDENTITY),
+                                     (abi.OP_PHMU, abi.PS_PHMU_BLOCK)])
+def test_bicgstab_early_exit(op, prec):
+    """The early exit of LIB/BiCGSTABREVCOM.f:311-316 tests the absolute
+    ||s|| against TOL, so a right-hand side of norm ~4e-7 leaves at
+    ||s||/||b|| ~ 1e-4, far above TOL: RESID > TOL with INFO = 0 shows that
+    the exit (x = x + alpha phat, RESID = ||s||/||b||) was taken.  Its ITER,
+    RESID, INFO and x equal the oracle's bit for bit (the identity
+    preconditioner aliases phat with p; the CH block path fuses B^-1 into
+    the vector sweeps)."""
+    print(f"bicgstab early exit op={op} prec={prec}", flush=True)
+    cfg = geometry.tp_case(32, 32, lambda_eta=10.0)
+    g, o = pair(cfg)
+    n = abi.op_size(cfg.params, op)
+    b = 1e-8 * np.random.default_rng(21).standard_normal(n)
+    tol = 1e-10
+    rg = g.bicgstab(op, prec, b, np.zeros(n), 2000, tol)
+    ro = o.bicgstab(op, prec, b, np.zeros(n), 2000, tol)
+    assert ro[3] == 0 and ro[2] > tol, "the oracle did not take the early exit"
+    assert (rg[1], rg[2], rg[3]) == (ro[1], ro[2], ro[3])
+    np.testing.assert_array_equal(rg[0], ro[0])
+    g.close()
+
+
 @pytest.mark.parametrize("nx,ny", [(64, 32), (33, 20)])
 def test_phsolve_operator_and_preconditioner(nx, ny):
     """MATRIX_VECTOR_CG (bit-exact) and the VCOST/DCT-I PSolve_CG (1e-12;
