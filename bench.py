@@ -151,6 +151,9 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
     geometry.load(g, cfg)
     if mode != "dropin" and GUESS:  # the fast modes start each BDF2 solve from 2x^n - x^{n-1}
         g.set_initial_guess(1)
+    if vp in ("PREC_DCT32", "PREC_DCT64", "PREC_LEARNED"):  # transpose tiles of the DCT preconditioner
+        shape, ms = g.tune()
+        log(f"[{mode}] transpose tiles: shape {shape} ({', '.join(f'{m*1e3:.1f}' for m in ms)} us per application)")
     g.synchronize()
     t = 0.0
     plain_warm = max(1, warmup - (LEARN["record_steps"] + 1 if mode == "learned" else 0))
@@ -294,17 +297,28 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     cells = sum(c.params.nx * c.params.ny for c in cfgs)
     dt = cfgs[0].params.dt
     b = GpuBatch([c.params for c in cfgs])
-    for i, c in enumerate(cfgs):
-        for fid, a in c.fields.items():
-            b.set_field(i, fid, a)
+
+    def load():
+        for i, c in enumerate(cfgs):
+            for fid, a in c.fields.items():
+                b.set_field(i, fid, a)
+
+    load()
     b.set_initial_guess(1)
     tuned = b.tune()
     log(f"[c2] {count} instances, {cells/1e6:.1f} M cells, groups {b.groups()} (generated in {gen_s:.1f} s); "
         f"tiles {[(t[0], t[2]) for t in tuned]}")
     t = 0.0
-    for k in range(warmup):
-        b.step(1 if k == 0 else 2, t)
-        t += dt
+
+    def warm():
+        nonlocal t
+        t = 0.0
+        for k in range(warmup):
+            b.step(1 if k == 0 else 2, t)
+            t += dt
+
+    warm()
+
     def timed(n):
         nonlocal t
         its_v, its_c, bad = 0, 0, 0
@@ -317,9 +331,16 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
             bad += sum(1 for i in infos if i["velo"]["info"] != 0 or i["phmu"]["info"] != 0)
         return time.perf_counter() - t1, its_v / n / count, its_c / n / count, bad
     el, iv, ic, bad = timed(steps)
+    # the same warm-up and timed steps again from the same fields, launched
+    # without graphs (later steps need fewer iterations, so only the same
+    # steps compare)
+    load()
     b.use_graphs(False)
-    el_ng, _, _, _ = timed(max(1, steps - 1))
-    el_ng /= max(1, steps - 1)
+    warm()
+    el_ng, iv_ng, ic_ng, _ = timed(steps)
+    el_ng /= steps
+    if (iv_ng, ic_ng) != (iv, ic):
+        log(f"[c2] the runs with and without graphs iterated differently: {iv}/{ic} and {iv_ng}/{ic_ng}")
     b.close()
     log(f"[c2] batched: {steps} steps in {el:.3f} s ({el/steps*1e3:.0f} ms/step, {count*steps/el:.0f} "
         f"instance-steps/s); no graphs {el_ng*1e3:.0f} ms/step; mean its velo {iv:.1f}, CH {ic:.1f}; "
