@@ -16,6 +16,8 @@ Modes
             record solver-emitted pairs, the network trains on them, one more
             warmup step judges it (learned FGMRES against plain FGMRES and
             the fp32 method) and the timed steps use what the judge kept.
+  mg      : (the headline) the BDF2 velocity solve preconditioned by one
+            geometric multigrid V-cycle, block-Jacobi CH, extrapolated start.
 
 The learned correction is also evaluated on its own workload (1024^2
 variable-viscosity velocity systems, FGMRES to 1e-10 with and without it on
@@ -104,6 +106,7 @@ MODES = {
     "fp64dct": ("PREC_DCT64", "CH_PREC_BLOCK"),
     "fp32": ("PREC_DCT32", "CH_PREC_BLOCK"),
     "learned": ("PREC_LEARNED", "CH_PREC_BLOCK"),
+    "mg": ("PREC_MG", "CH_PREC_BLOCK"),
 }
 
 GUESS = os.environ.get("DRD_BENCH_GUESS", "1") != "0"
@@ -445,8 +448,8 @@ def main(argv=None) -> int:
                     help="untimed steps (BDF1 first); the learned mode records, trains and judges in the last "
                          "4 of them")
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="learned", choices=sorted(MODES))
-    ap.add_argument("--compare", default="fp32,fp64dct,dropin",
+    ap.add_argument("--mode", default="mg", choices=sorted(MODES))
+    ap.add_argument("--compare", default="learned,fp32,fp64dct,dropin",
                     help="comma-separated modes timed beside the headline mode ('' for none)")
     ap.add_argument("--compare-steps", type=int, default=None,
                     help="timed steps of the comparison modes (default: --steps)")
@@ -490,8 +493,9 @@ def main(argv=None) -> int:
             f"convergence, {tl['idle_ms']*1e3:.0f} us in all, not counted)")
     # stand-alone timings of other kernels on the final state (same launch shapes)
     kern = {}
+    prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if args.mode == "mg" else (abi.K_VELO_PREC, "velo_prec")
     for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
-                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), (abi.K_VELO_PREC, "velo_prec")):
+                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), prec_k):
         ms, by = g.bench_kernel(which, 20)
         kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
         log(f"{name} (stand-alone): {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")

@@ -30,6 +30,7 @@ PREC_JACOBI = 2
 PREC_DCT32 = 3
 PREC_DCT64 = 4
 PREC_LEARNED = 5
+PREC_MG = 6
 
 CH_PREC_REFERENCE = 0
 CH_PREC_BLOCK = 1
@@ -51,10 +52,11 @@ PS_PHMU_BLOCK = 5
 PS_VELO_DCT64 = 6
 PS_VELO_ESOLP = 7
 PS_VELO_LEARNED = 8
+PS_VELO_MG = 9
 
 LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 
-K_VELO_MATVEC, K_PHMU_MATVEC, K_BICG_XR, K_PRESSURE, K_COPY, K_UNET_FWD, K_VELO_PREC = 0, 1, 2, 3, 4, 5, 6
+K_VELO_MATVEC, K_PHMU_MATVEC, K_BICG_XR, K_PRESSURE, K_COPY, K_UNET_FWD, K_VELO_PREC, K_VELO_MG = 0, 1, 2, 3, 4, 5, 6, 7
 
 F_U, F_V, F_P, F_PH, F_RMU, F_RHO, F_ETA = 0, 1, 2, 3, 4, 5, 6
 F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10

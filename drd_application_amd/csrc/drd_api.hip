@@ -136,6 +136,7 @@ struct drd_ctx {
   double* pn = nullptr;
   DctPlan* dct = nullptr;
   VeloPrec* vprec = nullptr;  // fast-path velocity preconditioner (lazy)
+  VeloMG* vmg = nullptr;      // multigrid velocity preconditioner (lazy)
   bool vprec_fp32 = true;
   // FGMRES workspace (lazy): V (m+1 vectors), Z (m vectors), small arrays
   int fg_m = 0;
@@ -437,6 +438,7 @@ static void ctx_free(drd_ctx* c) {
   if (c->uL_host) hipHostFree(c->uL_host);
   dct_plan_destroy(c->dct);
   velo_prec_destroy(c->vprec);
+  velo_mg_destroy(c->vmg);
   learn_destroy(c->learn);
   c->learn = nullptr;
   hipFree(c->fg_V);
@@ -544,6 +546,8 @@ extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   c->pre_lcc = c->ph_lcc = nullptr;
   if (c->vprec && velo_prec_update(c->vprec, c->s))  // its eigenvalues depend on dt, re, dx, dz
     return set_err(DRD_E_HIP, "preconditioner tables could not be rebuilt");
+  velo_mg_destroy(c->vmg);  // its levels' scalars follow the parameters; rebuilt when next used
+  c->vmg = nullptr;
   RCHK(build_tables(c));
   RCHK(eager_vprec(c));
   return 0;
@@ -675,6 +679,16 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     velo_prec_setup(c->stream, c->vprec, c->g, c->s, coef_of(c));
   }
+  if (prec == DRD_PS_VELO_MG) {
+    if (!c->vmg) {
+      int err = 0;
+      c->vmg = velo_mg_create(c->g, c->s, &err);
+      if (!c->vmg)
+        return set_err(err ? err : DRD_E_HIP,
+                       "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
+    }
+    velo_mg_setup(c->stream, c->vmg, coef_of(c));
+  }
   return 0;
 }
 
@@ -731,6 +745,11 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
     case DRD_PS_VELO_DCT64:
       if (op != DRD_OP_VELO2 || !c->vprec) break;
       velo_prec_apply(c->stream, c->vprec, c->g, b, x, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_VELO_MG:
+      if (op != DRD_OP_VELO2 || !c->vmg) break;
+      velo_mg_apply(c->stream, c->vmg, b, x, ks);
       *res = x;
       return 0;
     case DRD_PS_PH_DCT:
@@ -819,7 +838,8 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   // on, a chunk of length >= 2 is one replay of a captured hipGraph
   const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
                          (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
-                          prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64);
+                          prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
+                          prec == DRD_PS_VELO_MG);
   int chunk = 1, total = 0, next_snap = 0;
   for (;;) {
     hipGraphExec_t gx = nullptr;
@@ -1414,6 +1434,8 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT64;
     case DRD_PREC_LEARNED:  // plain FP32 DCT unless the correction is judged in (velo_solve_dev)
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
+    case DRD_PREC_MG:
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_MG;
     default:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
   }
@@ -1776,6 +1798,9 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       case DRD_K_VELO_PREC:
         velo_prec_apply(c->stream, c->vprec, g, c->kw[2], c->kw[3], nullptr);
         break;
+      case DRD_K_VELO_MG:
+        velo_mg_apply(c->stream, c->vmg, c->kw[2], c->kw[3], nullptr);
+        break;
     }
   };
   switch (which) {
@@ -1802,6 +1827,10 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       if (!c->vprec) return set_err(DRD_E_STATE, "no DCT velocity preconditioner in this context");
       RCHK(prec_setup(c, c->vprec_fp32 ? DRD_PS_VELO_DCT32 : DRD_PS_VELO_DCT64));
       *bytes = 2 * N8 + (c->vprec_fp32 ? N8 / 2 : N8);
+      break;
+    case DRD_K_VELO_MG:  // r in, z out (FP64)
+      RCHK(prec_setup(c, DRD_PS_VELO_MG));
+      *bytes = 2 * N8;
       break;
     default:
       return set_err(DRD_E_ARG, "unknown kernel id");

@@ -97,6 +97,53 @@ __device__ inline void kfinalize(KState* st, int phase, const double* t) {
   st->red[0] = t[0];
 }
 
+// Exact diagonal of the BDF2 velocity operator at packed unknown q
+// (MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 restricted to the
+// coefficient of the row's own unknown, wall and side rules included).
+__device__ inline double velo_diag2(const Grid& g, const Scal& s, const double* __restrict__ rho,
+                                    const double* __restrict__ eta, const double* __restrict__ chi, int64_t q) {
+  const int nx = g.nx, ny = g.ny;
+  double d;
+  if (q < g.mn()) {
+    const int k = g.by_nx1.div(q), i = (int)(q - (int64_t)k * (nx + 1));
+    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
+        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
+    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / s.dr2;
+    const double cr = (eta[g.ci(i, k + 1)] + eta[g.ci(i - 1, k + 1)] + eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / 4.0;
+    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    d += s.dt * ((k == 0 ? 2.0 * cl : cl) + (k == ny - 1 ? 2.0 * cr : cr)) / s.dz2;
+  } else {
+    const int64_t t = q - g.mn();
+    const int k = g.by_nx.div(t) + 1, i = (int)(t - (int64_t)(k - 1) * nx);
+    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
+        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
+    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i, k - 1)]) / s.dz2;
+    const double cr = (eta[g.ci(i + 1, k)] + eta[g.ci(i, k)] + eta[g.ci(i + 1, k - 1)] + eta[g.ci(i, k - 1)]) / 4.0;
+    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    d += s.dt * ((i == 0 ? 0.0 : cl) + (i == nx - 1 ? 0.0 : cr)) / s.dr2;
+  }
+  return d;
+}
+
+// ---- multigrid V-cycle for the BDF2 velocity operator (kernels_mg.hip) ----
+enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2 };
+struct VeloMG;
+VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err);
+void velo_mg_destroy(VeloMG* mg);
+void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0);
+void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks);
+int velo_mg_levels(const VeloMG* mg);
+// its kernels (kernels_stencil.hip)
+void launch_mg_coarsen(hipStream_t st, const Grid& gf, const Grid& gc, const Coef& f, double* rc, double* ec,
+                       double* cc);
+void launch_mg_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* d);
+void launch_mg_smooth(hipStream_t st, int kind, const Grid& g, const Scal& s, const Coef& c, const double* x,
+                      const double* b, const double* d, double* out, double omega, const KState* ks);
+void launch_mg_restrict(hipStream_t st, const Grid& gf, const Grid& gc, const double* r, double* bc,
+                        const KState* ks);
+void launch_mg_prolong_add(hipStream_t st, const Grid& gc, const Grid& gf, const double* xc, double* x,
+                           const KState* ks);
+
 // ---- reduction finish (kernels_krylov.hip) ----
 // k_finalize runs as one workgroup after a reducing kernel: it adds the
 // tile partials (layout [tile][segment][value]) in the canonical order, then

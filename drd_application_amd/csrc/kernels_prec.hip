@@ -198,29 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_velo_scale(Grid g, Scal s, const dou
   inv_s64 = bofs(inv_s64, bs);
   inv_s32 = bofs(inv_s32, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int nx = g.nx, ny = g.ny;
   if (q >= g.nuv()) return;
-  double d, d0;
-  if (q < g.mn()) {
-    const int k = g.by_nx1.div(q), i = (int)(q - (int64_t)k * (nx + 1));
-    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
-        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
-    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / s.dr2;
-    const double cr = (eta[g.ci(i, k + 1)] + eta[g.ci(i - 1, k + 1)] + eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / 4.0;
-    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
-    d += s.dt * ((k == 0 ? 2.0 * cl : cl) + (k == ny - 1 ? 2.0 * cr : cr)) / s.dz2;
-    d0 = a0 + s.dt * (4.0 / s.dr2 + 2.0 / s.dz2);
-  } else {
-    const int64_t t = q - g.mn();
-    const int k = g.by_nx.div(t) + 1, i = (int)(t - (int64_t)(k - 1) * nx);
-    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
-        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
-    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i, k - 1)]) / s.dz2;
-    const double cr = (eta[g.ci(i + 1, k)] + eta[g.ci(i, k)] + eta[g.ci(i + 1, k - 1)] + eta[g.ci(i, k - 1)]) / 4.0;
-    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
-    d += s.dt * ((i == 0 ? 0.0 : cl) + (i == nx - 1 ? 0.0 : cr)) / s.dr2;
-    d0 = a0 + s.dt * (2.0 / s.dr2 + 4.0 / s.dz2);
-  }
+  const double d = velo_diag2(g, s, rho, eta, chi, q);
+  const double d0 = q < g.mn() ? a0 + s.dt * (4.0 / s.dr2 + 2.0 / s.dz2) : a0 + s.dt * (2.0 / s.dr2 + 4.0 / s.dz2);
   const double is = sqrt(d0 / d);
   inv_s64[q] = is;
   inv_s32[q] = (float)is;

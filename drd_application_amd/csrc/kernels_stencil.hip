@@ -687,6 +687,161 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
   x[C + q] = mu[g.ci(i, k)];
 }
 
+// ---- multigrid pieces for the BDF2 velocity operator ----------------------
+// Level l+1 halves level l in both directions.  The transfers between the
+// packed (u, v) vectors of two levels are staggered bilinear (prolongation
+// P) and R = P^T / 4 (restriction); both evaluate one set of 1-D weights, so
+// R is exactly the transpose of P up to the factor.  P[f, C] per direction:
+//   u along x (faces 0..2n <- 0..n): even f: C = f/2 (1); odd f: f/2, f/2+1 (1/2 each)
+//   u along y (cells, odd walls):    C = f/2 (3/4), its other neighbour (1/4);
+//                                    past a wall the neighbour is -C (3/4 - 1/4)
+//   v along x (cells, even walls):   C = f/2 (3/4), other neighbour (1/4), clamped
+//   v along y (inner faces 1..2n-1 <- 1..n-1): as u along x, wall faces dropped
+__device__ __forceinline__ double mg_w_face(int f, int C) {
+  return (f & 1) ? ((C == (f >> 1) || C == (f >> 1) + 1) ? 0.5 : 0.0) : (C == (f >> 1) ? 1.0 : 0.0);
+}
+__device__ __forceinline__ double mg_w_cell_odd(int f, int C, int nc) {
+  const int c = f >> 1, o = (f & 1) ? c + 1 : c - 1;
+  const bool oin = o >= 0 && o < nc;
+  double w = 0.0;
+  if (C == c) w = oin ? 0.75 : 0.5;
+  if (oin && C == o) w = 0.25;
+  return w;
+}
+__device__ __forceinline__ double mg_w_cell_even(int f, int C, int nc) {
+  const int c = f >> 1;
+  int o = (f & 1) ? c + 1 : c - 1;
+  o = o < 0 ? 0 : (o > nc - 1 ? nc - 1 : o);
+  return (C == c ? 0.75 : 0.0) + (C == o ? 0.25 : 0.0);
+}
+
+// Coarse cell field (ghosts included, one thread per coarse cell): the 2x2
+// average of the fine interior cells under it; a ghost takes the value of
+// the nearest interior cell.
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const double* __restrict__ rf,
+                                                       const double* __restrict__ ef,
+                                                       const double* __restrict__ cf, double* __restrict__ rc,
+                                                       double* __restrict__ ec, double* __restrict__ cc) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= gc.nc()) return;
+  const int ld = gc.nx + 2;
+  const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
+  const int ic = i < 0 ? 0 : (i > gc.nx - 1 ? gc.nx - 1 : i);
+  const int kc = k < 0 ? 0 : (k > gc.ny - 1 ? gc.ny - 1 : k);
+  const int64_t a = gf.ci(2 * ic, 2 * kc), b = gf.ci(2 * ic, 2 * kc + 1);
+  const int64_t c = gf.ci(2 * ic + 1, 2 * kc), d = gf.ci(2 * ic + 1, 2 * kc + 1);
+  double v;
+  v = rf[a] + rf[b];
+  v = v + rf[c];
+  v = v + rf[d];
+  rc[q] = 0.25 * v;
+  v = ef[a] + ef[b];
+  v = v + ef[c];
+  v = v + ef[d];
+  ec[q] = 0.25 * v;
+  v = cf[a] + cf[b];
+  v = v + cf[c];
+  v = v + cf[d];
+  cc[q] = 0.25 * v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const double* __restrict__ rho,
+                                                    const double* __restrict__ eta,
+                                                    const double* __restrict__ chi, double* __restrict__ d) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q < g.nuv()) d[q] = velo_diag2(g, s, rho, eta, chi, q);
+}
+
+// Damped Jacobi sweep xo = x + omega (b - A x) / d; FIRST: from x = 0,
+// xo = omega b / d.  RES: r = b - A x instead.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const double* __restrict__ rho,
+                                                      const double* __restrict__ eta,
+                                                      const double* __restrict__ chi, const double* __restrict__ x,
+                                                      const double* __restrict__ b, const double* __restrict__ d,
+                                                      double* __restrict__ out, double omega, const KState* st) {
+  if (st && (st->done || st->early)) return;
+  const uint32_t q = (uint32_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t n = (uint32_t)g.nuv();
+  if (q >= n) return;
+  if (KIND == MG_FIRST) {
+    out[q] = omega * b[q] / d[q];
+    return;
+  }
+  const VeloIn X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+  const double z = velo_row<2>(g, s, rho, eta, chi, X, q);
+  if (KIND == MG_RES)
+    out[q] = b[q] - z;
+  else
+    out[q] = x[q] + omega * (b[q] - z) / d[q];
+}
+
+// bc = R r = P^T r / 4 (one thread per coarse unknown)
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const double* __restrict__ r,
+                                                        double* __restrict__ bc, const KState* st) {
+  if (st && (st->done || st->early)) return;
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= gc.nuv()) return;
+  double acc = 0.0;
+  if (q < gc.mn()) {  // u (I, K): fine faces 2I-1..2I+1, rows 2K-1..2K+2
+    const int K = (int)(q / (gc.nx + 1)), I = (int)(q - (int64_t)K * (gc.nx + 1));
+    for (int k = 2 * K - 1; k <= 2 * K + 2; ++k) {
+      if (k < 0 || k > gf.ny - 1) continue;
+      const double wy = mg_w_cell_odd(k, K, gc.ny);
+      if (wy == 0.0) continue;
+      for (int i = 2 * I - 1; i <= 2 * I + 1; ++i) {
+        if (i < 0 || i > gf.nx) continue;
+        acc += mg_w_face(i, I) * wy * r[i + (int64_t)(gf.nx + 1) * k];
+      }
+    }
+  } else {  // v (I, K), K = 1..ny-1: fine inner faces 2K-1..2K+1, cells 2I-1..2I+2
+    const int64_t t = q - gc.mn();
+    const int K = (int)(t / gc.nx) + 1, I = (int)(t - (int64_t)(K - 1) * gc.nx);
+    for (int k = 2 * K - 1; k <= 2 * K + 1; ++k) {
+      const double wy = mg_w_face(k, K);
+      for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
+        if (i < 0 || i > gf.nx - 1) continue;
+        acc += mg_w_cell_even(i, I, gc.nx) * wy * r[gf.mn() + i + (int64_t)gf.nx * (k - 1)];
+      }
+    }
+  }
+  bc[q] = 0.25 * acc;
+}
+
+// x += P xc (one thread per fine unknown)
+__global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const double* __restrict__ xc,
+                                                           double* __restrict__ x, const KState* st) {
+  if (st && (st->done || st->early)) return;
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= gf.nuv()) return;
+  double acc = 0.0;
+  if (q < gf.mn()) {  // u (i, k): coarse faces i/2 (and i/2 + 1), rows k/2 and its neighbour
+    const int k = (int)(q / (gf.nx + 1)), i = (int)(q - (int64_t)k * (gf.nx + 1));
+    const int I0 = i >> 1, I1 = (i + 1) >> 1;
+    const int K0 = k >> 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
+    for (int K = K0 - 1; K <= K0 + 1; ++K) {
+      if (K < 0 || K > gc.ny - 1 || (K != K0 && K != K1)) continue;
+      const double wy = mg_w_cell_odd(k, K, gc.ny);
+      for (int I = I0; I <= I1; ++I) acc += mg_w_face(i, I) * wy * xc[I + (int64_t)(gc.nx + 1) * K];
+    }
+  } else {  // v (i, k), k = 1..ny-1: coarse inner faces k/2 (and k/2 + 1), cells i/2 and its neighbour
+    const int64_t t = q - gf.mn();
+    const int k = (int)(t / gf.nx) + 1, i = (int)(t - (int64_t)(k - 1) * gf.nx);
+    const int K0 = k >> 1, K1 = (k + 1) >> 1;
+    const int Ic = i >> 1;
+    for (int K = K0; K <= K1; ++K) {
+      if (K < 1 || K > gc.ny - 1) continue;
+      const double wy = mg_w_face(k, K);
+      for (int I = Ic - 1; I <= Ic + 1; ++I) {
+        if (I < 0 || I > gc.nx - 1) continue;
+        const double wx = mg_w_cell_even(i, I, gc.nx);
+        if (wx != 0.0) acc += wx * wy * xc[gc.mn() + I + (int64_t)gc.nx * (K - 1)];
+      }
+    }
+  }
+  x[q] = x[q] + acc;
+}
+
 // ---- host launchers -----------------------------------------------------
 static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
@@ -793,6 +948,37 @@ void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const doub
                         double* ph, double* mu) {
   hipLaunchKernelGGL(k_phmu_unpack, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, x, pn, ph,
                      mu, bat_bs());
+}
+
+
+// ---- multigrid launchers ------------------------------------------------------
+void launch_mg_coarsen(hipStream_t st, const Grid& gf, const Grid& gc, const Coef& f, double* rc, double* ec,
+                       double* cc) {
+  hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(gc.nc(), kBlock)), dim3(kBlock), 0, st, gf, gc, f.rho, f.eta, f.chi,
+                     rc, ec, cc);
+}
+void launch_mg_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* d) {
+  hipLaunchKernelGGL(k_mg_diag, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, d);
+}
+void launch_mg_smooth(hipStream_t st, int kind, const Grid& g, const Scal& s, const Coef& c, const double* x,
+                      const double* b, const double* d, double* out, double omega, const KState* ks) {
+  const dim3 grid(nblk(g.nuv(), kBlock));
+#define MGS(K) \
+  hipLaunchKernelGGL((k_mg_smooth<K>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, b, d, out, omega, ks)
+  switch (kind) {
+    case MG_FIRST: MGS(MG_FIRST); break;
+    case MG_RES: MGS(MG_RES); break;
+    default: MGS(MG_SWEEP); break;
+  }
+#undef MGS
+}
+void launch_mg_restrict(hipStream_t st, const Grid& gf, const Grid& gc, const double* r, double* bc,
+                        const KState* ks) {
+  hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(gc.nuv(), kBlock)), dim3(kBlock), 0, st, gf, gc, r, bc, ks);
+}
+void launch_mg_prolong_add(hipStream_t st, const Grid& gc, const Grid& gf, const double* xc, double* x,
+                           const KState* ks) {
+  hipLaunchKernelGGL(k_mg_prolong_add, dim3(nblk(gf.nuv(), kBlock)), dim3(kBlock), 0, st, gc, gf, xc, x, ks);
 }
 
 }  // namespace drd

@@ -50,6 +50,7 @@ extern "C" {
                                 counterpart; power-of-two nx, ny)         */
 #define DRD_PREC_DCT64 4     /* the same in FP64                          */
 /* DRD_PREC_LEARNED (5): FP32 DCT + learned correction, defined below      */
+/* DRD_PREC_MG (6): geometric multigrid V-cycle, defined below              */
 
 /* ---- preconditioner choices for the Cahn-Hilliard phi/mu solve -------- */
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
@@ -208,6 +209,7 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 #define DRD_PS_VELO_ESOLP 7   /* PSolve_uv_CG PD/VeloSolve_combined.f90:697-807
                                  (ESOLP_x TP/ESOLP.f90 on both blocks)            */
 /* DRD_PS_VELO_LEARNED (8): FP32 DCT + learned correction, defined below   */
+/* DRD_PS_VELO_MG (9): geometric multigrid V-cycle, defined below           */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 
@@ -309,6 +311,8 @@ int drd_batch_set_initial_guess(drd_batch* b, int32_t mode);
                                *bytes returns its MFMA FLOPs instead      */
 #define DRD_K_VELO_PREC 6   /* one application of the DCT/DST velocity
                                preconditioner (all its passes)             */
+#define DRD_K_VELO_MG 7     /* one multigrid V-cycle (all its launches);
+                               *bytes: FP64 in and out of the fine level   */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
 /* Times kernel `which` inside the caller's own solves: while enabled, each
@@ -348,6 +352,25 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  * context falls back to the plain method (DRD_LEARN_FALLBACK).              */
 #define DRD_PREC_LEARNED 5
 #define DRD_PS_VELO_LEARNED 8
+
+/* ---- multigrid velocity preconditioner -----------------------------------
+ * No reference counterpart (an addition beside the drop-in).
+ * DRD_PS_VELO_MG is one V-cycle of geometric multigrid on the BDF2 velocity
+ * operator (power-of-two nx >= 128, ny >= 64):
+ *   - the levels halve the grid while nx >= 64 and ny >= 32 remain;
+ *   - each coarse operator is the same MAC stencil with doubled spacings, on
+ *     rho, eta, chi averaged over 2x2 cells;
+ *   - the transfers are staggered bilinear prolongation P and R = P^T / 4;
+ *   - damped Jacobi smoothing (omega = 0.7, exact diagonal): two sweeps
+ *     before the coarse correction and two after;
+ *   - the FP64 DCT/DST fast-Poisson solve on the coarsest level.
+ * It is a fixed linear map, so BiCGSTAB accepts it.  Its BiCGSTAB iteration
+ * counts stay nearly constant as the grid is refined, unlike the fast-Poisson
+ * preconditioner's.  velo_prec = DRD_PREC_MG makes drd_step solve the BDF2
+ * velocity system with it (the first, BDF1 step keeps Jacobi); batches do not
+ * take it.                                                                   */
+#define DRD_PREC_MG 6
+#define DRD_PS_VELO_MG 9
 
 #define DRD_LEARN_OFF 0      /* not enabled: the plain method               */
 #define DRD_LEARN_JUDGE 1    /* the next velocity solve is judged           */

@@ -1,0 +1,138 @@
+"""Reference construction of the multigrid velocity preconditioner, for tests.
+
+vcycle() is the V-cycle of DRD_PS_VELO_MG written with numpy and the oracle:
+each level's operator is the oracle's BDF2 velocity operator on that level's
+grid and averaged coefficients, its diagonal is probed from that operator
+(prec_ref.probe_diag), and the coarsest level's solve is
+prec_ref.scipy_dct_prec (FP64).  The transfers are sparse matrices built
+from the 1-D prolongation weights one unknown at a time (prolongation);
+prolongation_kron builds the same matrix as Kronecker products of 1-D
+operators, an independent construction the tests compare it with.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from drd_application_amd import abi
+import prec_ref
+
+OMEGA = 0.7
+
+
+# ---- 1-D prolongation weights P[f, C] -------------------------------------------
+def w_face(f, C):  # faces 0..2n <- 0..n (u along x; v along y on inner faces)
+    if f % 2:
+        return 0.5 if C in (f // 2, f // 2 + 1) else 0.0
+    return 1.0 if C == f // 2 else 0.0
+
+
+def w_cell_odd(f, C, nc):  # cells, odd (no-slip) walls: u along y
+    c = f // 2
+    o = c + 1 if f % 2 else c - 1
+    oin = 0 <= o < nc
+    if C == c:
+        return 0.75 if oin else 0.5
+    return 0.25 if (oin and C == o) else 0.0
+
+
+def w_cell_even(f, C, nc):  # cells, even (mirror) walls: v along x
+    c = f // 2
+    o = min(max(c + 1 if f % 2 else c - 1, 0), nc - 1)
+    return (0.75 if C == c else 0.0) + (0.25 if C == o else 0.0)
+
+
+def prolongation(nxc, nyc):
+    """P for the packed (u, v) vector, coarse nxc x nyc -> fine 2nxc x 2nyc."""
+    nxf, nyf = 2 * nxc, 2 * nyc
+    mnf, mnc = (nxf + 1) * nyf, (nxc + 1) * nyc
+    rows, cols, vals = [], [], []
+    for k in range(nyf):
+        for i in range(nxf + 1):
+            for K in range(nyc):
+                wy = w_cell_odd(k, K, nyc)
+                if wy == 0.0:
+                    continue
+                for I in range(max(0, i // 2 - 1), min(nxc, i // 2 + 1) + 1):
+                    w = w_face(i, I) * wy
+                    if w:
+                        rows.append(i + (nxf + 1) * k)
+                        cols.append(I + (nxc + 1) * K)
+                        vals.append(w)
+    for k in range(1, nyf):
+        for i in range(nxf):
+            for K in range(1, nyc):
+                wy = w_face(k, K)
+                if wy == 0.0:
+                    continue
+                for I in range(max(0, i // 2 - 1), min(nxc - 1, i // 2 + 1) + 1):
+                    w = w_cell_even(i, I, nxc) * wy
+                    if w:
+                        rows.append(mnf + i + nxf * (k - 1))
+                        cols.append(mnc + I + nxc * (K - 1))
+                        vals.append(w)
+    n_f = mnf + nxf * (nyf - 1)
+    n_c = mnc + nxc * (nyc - 1)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n_f, n_c))
+
+
+def prolongation_kron(nxc, nyc):
+    """The same P as Kronecker products of the 1-D operators."""
+    def m(nf, nc, w):
+        a = np.zeros((nf, nc))
+        for f in range(nf):
+            for C in range(nc):
+                a[f, C] = w(f, C)
+        return sp.csr_matrix(a)
+    px_u = m(2 * nxc + 1, nxc + 1, w_face)
+    py_u = m(2 * nyc, nyc, lambda f, C: w_cell_odd(f, C, nyc))
+    px_v = m(2 * nxc, nxc, lambda f, C: w_cell_even(f, C, nxc))
+    py_v = m(2 * nyc - 1, nyc - 1, lambda f, C: w_face(f + 1, C + 1))
+    return sp.block_diag([sp.kron(py_u, px_u), sp.kron(py_v, px_v)]).tocsr()
+
+
+# ---- levels ---------------------------------------------------------------------
+def coarse_cell(a, nx, ny):
+    """2x2 averages of the interior of a ghosted cell field a[ny+2, nx+2];
+    a ghost takes the value of the nearest interior cell."""
+    inn = a[1:ny + 1, 1:nx + 1]
+    c = 0.25 * (((inn[0::2, 0::2] + inn[1::2, 0::2]) + inn[0::2, 1::2]) + inn[1::2, 1::2])
+    return np.pad(c, 1, mode="edge")
+
+
+def levels(o, p, Oracle):
+    """[(params, oracle, diagonal)] from the fine level down (nx >= 64, ny >= 32)."""
+    out = [(p, o, prec_ref.probe_diag(o, abi.OP_VELO2, p))]
+    while out[-1][0].nx // 2 >= 64 and out[-1][0].ny // 2 >= 32:
+        pf, of, _ = out[-1]
+        q = pf.copy()
+        q.nx //= 2
+        q.ny //= 2
+        q.dx *= 2.0
+        q.dz *= 2.0
+        oc = Oracle(q)
+        for fid in (abi.F_RHO, abi.F_ETA, abi.F_BDR_PH_PENALTY):
+            oc.set_field(fid, coarse_cell(of.get_field(fid), pf.nx, pf.ny))
+        out.append((q, oc, prec_ref.probe_diag(oc, abi.OP_VELO2, q)))
+    return out
+
+
+def vcycle(lv, b):
+    """z = one V-cycle of b (lv from levels())."""
+    P = [prolongation(lv[l + 1][0].nx, lv[l + 1][0].ny) for l in range(len(lv) - 1)]
+
+    def A(l, x):
+        q, o, _ = lv[l]
+        return o.matvec(abi.OP_VELO2, 1.0, x, 0.0, np.zeros(x.size))
+
+    def down(l, bl):
+        q, o, d = lv[l]
+        if l == len(lv) - 1:
+            return prec_ref.scipy_dct_prec(o, q, bl)
+        xa = OMEGA * bl / d
+        xb = xa + OMEGA * (bl - A(l, xa)) / d
+        r = bl - A(l, xb)
+        xc = down(l + 1, 0.25 * (P[l].T @ r))
+        xb = xb + P[l] @ xc
+        xa = xb + OMEGA * (bl - A(l, xb)) / d
+        return xa + OMEGA * (bl - A(l, xa)) / d
+
+    return down(0, b)
