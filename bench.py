@@ -17,7 +17,8 @@ Modes
             warmup step judges it (learned FGMRES against plain FGMRES and
             the fp32 method) and the timed steps use what the judge kept.
   mg      : (the headline) the BDF2 velocity solve preconditioned by one
-            geometric multigrid V-cycle, block-Jacobi CH, extrapolated start.
+            geometric multigrid V-cycle, the CH solve by four red-black
+            block Gauss-Seidel sweeps, extrapolated start.
 
 The learned correction is also evaluated on its own workload (1024^2
 variable-viscosity velocity systems, FGMRES to 1e-10 with and without it on
@@ -106,7 +107,7 @@ MODES = {
     "fp64dct": ("PREC_DCT64", "CH_PREC_BLOCK"),
     "fp32": ("PREC_DCT32", "CH_PREC_BLOCK"),
     "learned": ("PREC_LEARNED", "CH_PREC_BLOCK"),
-    "mg": ("PREC_MG", "CH_PREC_BLOCK"),
+    "mg": ("PREC_MG", "CH_PREC_RBGS"),
 }
 
 GUESS = os.environ.get("DRD_BENCH_GUESS", "1") != "0"

@@ -34,6 +34,7 @@ PREC_MG = 6
 
 CH_PREC_REFERENCE = 0
 CH_PREC_BLOCK = 1
+CH_PREC_RBGS = 2
 
 KRYLOV_BICGSTAB = 0
 KRYLOV_FGMRES = 1
@@ -53,6 +54,7 @@ PS_VELO_DCT64 = 6
 PS_VELO_ESOLP = 7
 PS_VELO_LEARNED = 8
 PS_VELO_MG = 9
+PS_PHMU_RBGS = 10
 
 LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 

@@ -736,6 +736,11 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       launch_phmu_jacobi(c->stream, c->g, c->s, b, x, ks);
       *res = x;
       return 0;
+    case DRD_PS_PHMU_RBGS:
+      if (op != DRD_OP_PHMU || (c->g.nx & 1)) break;
+      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), b, x, ks);
+      *res = x;
+      return 0;
     case DRD_PS_PHMU_BLOCK:
       if (op != DRD_OP_PHMU) break;
       launch_phmu_block(c->stream, c->g, c->s, coef_of(c), b, x, ks);
@@ -839,7 +844,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
                          (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
                           prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
-                          prec == DRD_PS_VELO_MG);
+                          prec == DRD_PS_VELO_MG || prec == DRD_PS_PHMU_RBGS);
   int chunk = 1, total = 0, next_snap = 0;
   for (;;) {
     hipGraphExec_t gx = nullptr;
@@ -1533,7 +1538,9 @@ static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_
     launch_phmu_pack(c->stream, c->g, c->f[DRD_F_PH_PRE], c->f[DRD_F_RMU_PRE], c->tmp);
     launch_axpby(c->stream, 2 * c->g.ncell(), -1.0, c->tmp, 2.0, c->xv);
   }
-  const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK ? DRD_PS_PHMU_BLOCK : DRD_PS_PHMU_JACOBI;
+  const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK  ? DRD_PS_PHMU_BLOCK
+                   : c->p.ch_prec == DRD_CH_PREC_RBGS ? DRD_PS_PHMU_RBGS
+                                                      : DRD_PS_PHMU_JACOBI;
   RCHK(run_bicgstab(c, DRD_OP_PHMU, prec, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu, c->p.tol, info,
                     want_true));
   launch_phmu_unpack(c->stream, c->g, c->s, c->xv, c->pn, ph_after, rmu_after);

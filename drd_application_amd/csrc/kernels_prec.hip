@@ -26,44 +26,6 @@
 namespace drd {
 
 // ---- CH block Jacobi --------------------------------------------------------
-// diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219);
-// the neighbour mobilities are loaded unconditionally (clamped indices) and
-// only the sums are selected, so a thread issues all its loads at once
-__device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2, const double* __restrict__ mo,
-                                           int i, int j) {
-  const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
-  const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
-  const double m0 = mo[g.ci(i, j)];
-  const double tl = (mo[g.ci(il, j)] + m0) / 2.0, tr = (m0 + mo[g.ci(ir, j)]) / 2.0;
-  const double bl = (mo[g.ci(i, jl)] + m0) / 2.0, br = (m0 + mo[g.ci(i, jr)]) / 2.0;
-  double dx = 0.0, dy = 0.0;
-  dx = i > 0 ? dx + tl : dx;
-  dx = i < g.nx - 1 ? dx + tr : dx;
-  dy = j > 0 ? dy + bl : dy;
-  dy = j < g.ny - 1 ? dy + br : dy;
-  return -(dx / dx2 + dy / dy2);
-}
-
-// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell
-__device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
-                                                 const double* __restrict__ minner,
-                                                 const double* __restrict__ m0, int i, int j, double r1,
-                                                 double r2, double& x1, double& x2) {
-  const double bv = bu[g.ci(i, j)];
-  const double d11 = s.s / s.I - s.I * dgn_diag(g, s.dr2, s.dz2, minner, i, j) +
-                     ((j == 0) ? s.I / s.dz2 * (1.0 - s.coef_bot) : 0.0) +
-                     ((j == g.ny - 1) ? s.I / s.dz2 * (1.0 - s.coef_top) : 0.0);
-  const double d12 = -bv;
-  const double d21 = bv + s.adj;
-  // mobility-free floor keeps the block invertible where M_0 vanishes
-  const double floor22 = 1e-3 * s.dt * s.xld * (2.0 / s.dr2 + 2.0 / s.dz2);
-  double d22 = -s.dt * s.xld * dgn_diag(g, s.dr2, s.dz2, m0, i, j);
-  d22 = d22 > floor22 ? d22 : floor22;
-  const double det = d11 * d22 - d12 * d21;
-  x1 = (d22 * r1 - d12 * r2) / det;
-  x2 = (-d21 * r1 + d11 * r2) / det;
-}
-
 __global__ __launch_bounds__(kBlock) void k_phmu_block(Grid g, Scal s, const double* __restrict__ bu,
                                                        const double* __restrict__ minner,
                                                        const double* __restrict__ m0,

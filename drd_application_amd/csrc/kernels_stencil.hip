@@ -687,6 +687,47 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
   x[C + q] = mu[g.ci(i, k)];
 }
 
+// ---- red-black block Gauss-Seidel for the phi/mu system -------------------
+// One half-sweep over the cells of one colour ((i + j) % 2 == color), one
+// thread per such cell (nx even: nx/2 of them per row): the residual of the
+// cell's two rows on the current x (phmu_rows, as the matvec computes
+// them), the exact 2x2 block solve, x += the correction.  A cell's rows read
+// x at the cell itself and at its four neighbours, which have the other
+// colour, so the cells of one colour update in place without a race.
+// FIRST starts from x = 0: the colour's cells get B^-1 b, the others 0.
+template <bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_phmu_rbgs(Grid g, Scal s, const double* __restrict__ bu,
+                                                      const double* __restrict__ minner,
+                                                      const double* __restrict__ m0, const double* __restrict__ b,
+                                                      double* x, int color, const KState* st, int64_t bs) {
+  st = bofs(st, bs);
+  if (st && (st->done || st->early)) return;
+  bu = bofs(bu, bs);
+  minner = bofs(minner, bs);
+  m0 = bofs(m0, bs);
+  b = bofs(b, bs);
+  x = bofs(x, bs);
+  const int64_t C = g.ncell();
+  const int half = g.nx >> 1;
+  const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (u >= (int64_t)half * g.ny) return;
+  const int j = (int)(u / half), t = (int)(u - (int64_t)j * half);
+  const int i = 2 * t + ((color + j) & 1);
+  const int64_t q = i + (int64_t)g.nx * j;
+  if (FIRST) {
+    phmu_block_solve(g, s, bu, minner, m0, i, j, b[q], b[C + q], x[q], x[C + q]);
+    const int64_t o = (2 * t + ((color + j + 1) & 1)) + (int64_t)g.nx * j;
+    x[o] = 0.0;
+    x[C + o] = 0.0;
+    return;
+  }
+  double z1, z2, y1, y2;
+  phmu_rows(g, s, bu, minner, m0, x, q, z1, z2);
+  phmu_block_solve(g, s, bu, minner, m0, i, j, b[q] - z1, b[C + q] - z2, y1, y2);
+  x[q] = x[q] + y1;
+  x[C + q] = x[C + q] + y2;
+}
+
 // ---- multigrid pieces for the BDF2 velocity operator ----------------------
 // Level l+1 halves level l in both directions.  The transfers between the
 // packed (u, v) vectors of two levels are staggered bilinear (prolongation
@@ -979,6 +1020,20 @@ void launch_mg_restrict(hipStream_t st, const Grid& gf, const Grid& gc, const do
 void launch_mg_prolong_add(hipStream_t st, const Grid& gc, const Grid& gf, const double* xc, double* x,
                            const KState* ks) {
   hipLaunchKernelGGL(k_mg_prolong_add, dim3(nblk(gf.nuv(), kBlock)), dim3(kBlock), 0, st, gc, gf, xc, x, ks);
+}
+
+
+// x = the red-black block Gauss-Seidel preconditioner applied to b:
+// kRbgsSweeps full sweeps (red cells, then black), from x = 0
+constexpr int kRbgsSweeps = 4;
+void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
+                      const KState* ks) {
+  const dim3 grid(nblk((int64_t)(g.nx / 2) * g.ny, kBlock), 1, bat_n());
+  hipLaunchKernelGGL((k_phmu_rbgs<true>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, b, x, 0, ks,
+                     bat_bs());
+  for (int h = 1; h < 2 * kRbgsSweeps; ++h)
+    hipLaunchKernelGGL((k_phmu_rbgs<false>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, b, x, h & 1,
+                       ks, bat_bs());
 }
 
 }  // namespace drd

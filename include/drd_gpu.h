@@ -56,6 +56,9 @@ extern "C" {
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
 #define DRD_CH_PREC_BLOCK 1     /* exact 2x2 per-cell block Jacobi (no
                                    reference counterpart)                   */
+#define DRD_CH_PREC_RBGS 2      /* four red-black Gauss-Seidel sweeps of the
+                                   same 2x2 blocks (no reference
+                                   counterpart; even nx)                    */
 
 /* ---- outer Krylov method ------------------------------------------------ */
 #define DRD_KRYLOV_BICGSTAB 0 /* LIB/BiCGSTAB.f, reference stopping rule    */
@@ -210,6 +213,10 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
                                  (ESOLP_x TP/ESOLP.f90 on both blocks)            */
 /* DRD_PS_VELO_LEARNED (8): FP32 DCT + learned correction, defined below   */
 /* DRD_PS_VELO_MG (9): geometric multigrid V-cycle, defined below           */
+#define DRD_PS_PHMU_RBGS 10   /* red-black block Gauss-Seidel for the phi/mu system
+                                 (addition): x = 0; then 4 times: the red cells,
+                                 then the black cells, x_c += B_c^-1 (b - A x)_c
+                                 with B_c the exact 2x2 block of cell c         */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 
