@@ -129,13 +129,31 @@ __device__ inline double velo_diag2(const Grid& g, const Scal& s, const double* 
 // diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219);
 // the neighbour mobilities are loaded unconditionally (clamped indices) and
 // only the sums are selected, so a thread issues all its loads at once
-__device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2, const double* __restrict__ mo,
-                                           int i, int j) {
+// A cell field at cell (i, j) and its four neighbours, the neighbour index
+// clamped to the cell at a wall (the value the reference's one-sided rules
+// use there).  Loading the five first lets a thread issue all its loads
+// before any arithmetic; the *_v forms below take the values.
+struct Nb5 {
+  double c, xl, xr, yl, yr;
+};
+__device__ __forceinline__ Nb5 nb5_cell(const Grid& g, const double* __restrict__ a, int i, int j) {
   const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
   const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
-  const double m0 = mo[g.ci(i, j)];
-  const double tl = (mo[g.ci(il, j)] + m0) / 2.0, tr = (m0 + mo[g.ci(ir, j)]) / 2.0;
-  const double bl = (mo[g.ci(i, jl)] + m0) / 2.0, br = (m0 + mo[g.ci(i, jr)]) / 2.0;
+  return Nb5{a[g.ci(i, j)], a[g.ci(il, j)], a[g.ci(ir, j)], a[g.ci(i, jl)], a[g.ci(i, jr)]};
+}
+// the same for a packed interior field x[i + nx j]
+__device__ __forceinline__ Nb5 nb5_packed(const Grid& g, const double* x, int i, int j) {
+  const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
+  const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
+  const int64_t nx = g.nx;
+  return Nb5{x[i + nx * j], x[il + nx * j], x[ir + nx * j], x[i + nx * jl], x[i + nx * jr]};
+}
+
+// diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219)
+// from the mobility at the cell and its neighbours; only the sums are selected
+__device__ __forceinline__ double dgn_diag_v(const Grid& g, double dx2, double dy2, const Nb5& mo, int i, int j) {
+  const double tl = (mo.xl + mo.c) / 2.0, tr = (mo.c + mo.xr) / 2.0;
+  const double bl = (mo.yl + mo.c) / 2.0, br = (mo.c + mo.yr) / 2.0;
   double dx = 0.0, dy = 0.0;
   dx = i > 0 ? dx + tl : dx;
   dx = i < g.nx - 1 ? dx + tr : dx;
@@ -143,25 +161,33 @@ __device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2
   dy = j < g.ny - 1 ? dy + br : dy;
   return -(dx / dx2 + dy / dy2);
 }
+__device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2, const double* __restrict__ mo,
+                                           int i, int j) {
+  return dgn_diag_v(g, dx2, dy2, nb5_cell(g, mo, i, j), i, j);
+}
 
-// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell
-__device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
-                                                 const double* __restrict__ minner,
-                                                 const double* __restrict__ m0, int i, int j, double r1,
-                                                 double r2, double& x1, double& x2) {
-  const double bv = bu[g.ci(i, j)];
-  const double d11 = s.s / s.I - s.I * dgn_diag(g, s.dr2, s.dz2, minner, i, j) +
+// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell, from
+// bu at the cell and the two mobilities around it
+__device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, double bv, const Nb5& mi, const Nb5& m0,
+                                             int i, int j, double r1, double r2, double& x1, double& x2) {
+  const double d11 = s.s / s.I - s.I * dgn_diag_v(g, s.dr2, s.dz2, mi, i, j) +
                      ((j == 0) ? s.I / s.dz2 * (1.0 - s.coef_bot) : 0.0) +
                      ((j == g.ny - 1) ? s.I / s.dz2 * (1.0 - s.coef_top) : 0.0);
   const double d12 = -bv;
   const double d21 = bv + s.adj;
   // mobility-free floor keeps the block invertible where M_0 vanishes
   const double floor22 = 1e-3 * s.dt * s.xld * (2.0 / s.dr2 + 2.0 / s.dz2);
-  double d22 = -s.dt * s.xld * dgn_diag(g, s.dr2, s.dz2, m0, i, j);
+  double d22 = -s.dt * s.xld * dgn_diag_v(g, s.dr2, s.dz2, m0, i, j);
   d22 = d22 > floor22 ? d22 : floor22;
   const double det = d11 * d22 - d12 * d21;
   x1 = (d22 * r1 - d12 * r2) / det;
   x2 = (-d21 * r1 + d11 * r2) / det;
+}
+__device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
+                                                 const double* __restrict__ minner,
+                                                 const double* __restrict__ m0, int i, int j, double r1,
+                                                 double r2, double& x1, double& x2) {
+  phmu_block_v(g, s, bu[g.ci(i, j)], nb5_cell(g, minner, i, j), nb5_cell(g, m0, i, j), i, j, r1, r2, x1, x2);
 }
 
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,

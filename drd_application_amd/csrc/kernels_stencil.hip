@@ -428,46 +428,48 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
 // index to the cell does exactly that with the interior expression, term
 // for term, so every load is unconditional and a thread issues them all at
 // once (nx, ny >= 2, which drd_init requires).
+__device__ __forceinline__ double op_dgn_v(double dx2, double dy2, const Nb5& mo, const Nb5& x) {
+  double moL, moR, rhs = 0.0;
+  moL = (mo.xl + mo.c) / 2.0;
+  moR = (mo.c + mo.xr) / 2.0;
+  rhs = rhs + 1.0 / dx2 * (moL * x.xl + moR * x.xr - (moL + moR) * x.c);
+  moL = (mo.yl + mo.c) / 2.0;
+  moR = (mo.c + mo.yr) / 2.0;
+  rhs = rhs + 1.0 / dy2 * (moL * x.yl + moR * x.yr - (moL + moR) * x.c);
+  return rhs;
+}
 template <class F>
 __device__ __forceinline__ double op_dgn(const Grid g, double dx2, double dy2, const double* __restrict__ mo,
                                          int i, int j, F get) {
   const int nx = g.nx, ny = g.ny;
   const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
   const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
-  const double m0 = mo[g.ci(i, j)];
-  const double r0 = get(i, j);
-  const double mxl = mo[g.ci(il, j)], mxr = mo[g.ci(ir, j)], myl = mo[g.ci(i, jl)], myr = mo[g.ci(i, jr)];
-  const double xl = get(il, j), xr = get(ir, j), yl = get(i, jl), yr = get(i, jr);
-  double moL, moR, rhs = 0.0;
-  moL = (mxl + m0) / 2.0;
-  moR = (m0 + mxr) / 2.0;
-  rhs = rhs + 1.0 / dx2 * (moL * xl + moR * xr - (moL + moR) * r0);
-  moL = (myl + m0) / 2.0;
-  moR = (m0 + myr) / 2.0;
-  rhs = rhs + 1.0 / dy2 * (moL * yl + moR * yr - (moL + moR) * r0);
-  return rhs;
+  return op_dgn_v(dx2, dy2, nb5_cell(g, mo, i, j), Nb5{get(i, j), get(il, j), get(ir, j), get(i, jl), get(i, jr)});
+}
+
+// MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:212-284 for cell (i, j) (both
+// rows) from the values around it: phi and mu (f1, f2) and the two mobilities
+__device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, double b, const Nb5& mi,
+                                            const Nb5& m0, const Nb5& f1, const Nb5& f2, double& o1, double& o2) {
+  double ff1 = op_dgn_v(s.dr2, s.dz2, mi, f1);
+  ff1 = s.s * f1.c / s.I - s.I * ff1 - f2.c * b;
+  // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
+  if (j == 0) ff1 = ff1 + s.I / s.dz2 * (f1.c - s.coef_bot * f1.c);
+  if (j == g.ny - 1) ff1 = ff1 + s.I / s.dz2 * (f1.c - s.coef_top * f1.c);
+  double ff2 = op_dgn_v(s.dr2, s.dz2, m0, f2);
+  ff2 = f1.c * (b + s.adj) - s.dt * s.xld * ff2;
+  o1 = ff1;
+  o2 = ff2;
 }
 
 // MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:212-284 for cell q (both rows).
 __device__ __forceinline__ void phmu_rows(const Grid g, const Scal& s, const double* __restrict__ bu,
                                           const double* __restrict__ minner, const double* __restrict__ m0,
                                           const double* __restrict__ x, int64_t q, double& o1, double& o2) {
-  const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
-  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * nx);
-  auto F1 = [&](int a, int b) -> double { return x[a + (int64_t)nx * b]; };
-  auto F2 = [&](int a, int b) -> double { return x[C + a + (int64_t)nx * b]; };
-  const double f1 = F1(i, j), f2 = F2(i, j);
-  const double b = bu[g.ci(i, j)];
-  double ff1 = op_dgn(g, s.dr2, s.dz2, minner, i, j, F1);
-  ff1 = s.s * f1 / s.I - s.I * ff1 - f2 * b;
-  // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
-  if (j == 0) ff1 = ff1 + s.I / s.dz2 * (f1 - s.coef_bot * f1);
-  if (j == ny - 1) ff1 = ff1 + s.I / s.dz2 * (f1 - s.coef_top * f1);
-  double ff2 = op_dgn(g, s.dr2, s.dz2, m0, i, j, F2);
-  ff2 = f1 * (b + s.adj) - s.dt * s.xld * ff2;
-  o1 = ff1;
-  o2 = ff2;
+  const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * g.nx);
+  phmu_rows_v(g, s, j, bu[g.ci(i, j)], nb5_cell(g, minner, i, j), nb5_cell(g, m0, i, j), nb5_packed(g, x, i, j),
+              nb5_packed(g, x + C, i, j), o1, o2);
 }
 
 template <int MODE>
@@ -721,11 +723,16 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rbgs(Grid g, Scal s, const doub
     x[C + o] = 0.0;
     return;
   }
+  // every value first (23 loads in flight), then the arithmetic of the
+  // matvec's rows and of the block solve on them
+  const Nb5 mi = nb5_cell(g, minner, i, j), mz = nb5_cell(g, m0, i, j);
+  const Nb5 f1 = nb5_packed(g, x, i, j), f2 = nb5_packed(g, x + C, i, j);
+  const double bv = bu[g.ci(i, j)], b1 = b[q], b2 = b[C + q];
   double z1, z2, y1, y2;
-  phmu_rows(g, s, bu, minner, m0, x, q, z1, z2);
-  phmu_block_solve(g, s, bu, minner, m0, i, j, b[q] - z1, b[C + q] - z2, y1, y2);
-  x[q] = x[q] + y1;
-  x[C + q] = x[C + q] + y2;
+  phmu_rows_v(g, s, j, bv, mi, mz, f1, f2, z1, z2);
+  phmu_block_v(g, s, bv, mi, mz, i, j, b1 - z1, b2 - z2, y1, y2);
+  x[q] = f1.c + y1;
+  x[C + q] = f2.c + y2;
 }
 
 // ---- multigrid pieces for the BDF2 velocity operator ----------------------
