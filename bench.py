@@ -108,6 +108,7 @@ MODES = {
     "fp32": ("PREC_DCT32", "CH_PREC_BLOCK"),
     "learned": ("PREC_LEARNED", "CH_PREC_BLOCK"),
     "mg": ("PREC_MG", "CH_PREC_RBGS"),
+    "mg32": ("PREC_MG32", "CH_PREC_RBGS32"),
 }
 
 GUESS = os.environ.get("DRD_BENCH_GUESS", "1") != "0"
@@ -416,7 +417,8 @@ def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     geometry.load(g, cfg)
     g.step(1, 0.0)
     out = {}
-    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), (abi.K_VELO_PREC, "velo_prec")):
+    prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if mode.startswith("mg") else (abi.K_VELO_PREC, "velo_prec")
+    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), prec_k):
         ms, by = g.bench_kernel(which, reps)
         out[name] = {"us": ms * 1e3, "bytes": by, "launches": reps + 1}
         log(f"{name}: {ms*1e3:.1f} us, {by/1e6:.1f} MB algorithmic")
@@ -494,7 +496,7 @@ def main(argv=None) -> int:
             f"convergence, {tl['idle_ms']*1e3:.0f} us in all, not counted)")
     # stand-alone timings of other kernels on the final state (same launch shapes)
     kern = {}
-    prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if args.mode == "mg" else (abi.K_VELO_PREC, "velo_prec")
+    prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if args.mode.startswith("mg") else (abi.K_VELO_PREC, "velo_prec")
     for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
                         (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), prec_k):
         ms, by = g.bench_kernel(which, 20)

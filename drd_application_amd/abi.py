@@ -31,10 +31,12 @@ PREC_DCT32 = 3
 PREC_DCT64 = 4
 PREC_LEARNED = 5
 PREC_MG = 6
+PREC_MG32 = 7
 
 CH_PREC_REFERENCE = 0
 CH_PREC_BLOCK = 1
 CH_PREC_RBGS = 2
+CH_PREC_RBGS32 = 3
 
 KRYLOV_BICGSTAB = 0
 KRYLOV_FGMRES = 1
@@ -54,7 +56,9 @@ PS_VELO_DCT64 = 6
 PS_VELO_ESOLP = 7
 PS_VELO_LEARNED = 8
 PS_VELO_MG = 9
+PS_VELO_MG32 = 11
 PS_PHMU_RBGS = 10
+PS_PHMU_RBGS32 = 12
 
 LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 

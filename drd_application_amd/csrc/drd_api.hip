@@ -149,6 +149,8 @@ struct drd_ctx {
   double* vp_inv64 = nullptr;  // per-instance buffers of the velocity preconditioner
   float* vp_inv32 = nullptr;
   void* vp_work = nullptr;
+  void* rb_work = nullptr;    // red-black sweeps: the iterate between sweeps
+  float* rb_c32 = nullptr;    // FP32 bu, M_inner, M_0 (nc each)
   double cosa = 0, sqrt2 = 0;
   Learn* learn = nullptr;  // drd_learn_init
 };
@@ -240,6 +242,7 @@ static Scal make_scal(const drd_params& p) {
   s.coef_top = (p.I_thickness / p.dz) / tmp;
   s.lr_d = (p.lr_bd_type == 'd');
   s.phi_mode = p.phi_mode;
+  scal_derive(s);
   return s;
 }
 
@@ -397,6 +400,8 @@ static int ctx_alloc(drd_ctx* c) {
   add(&c->vp_inv64, g.nuv(), 8);
   add(&c->vp_inv32, g.nuv(), 4);
   add(&c->vp_work, 2 * g.nuv(), 8);
+  add(&c->rb_work, 2 * g.ncell(), 8);
+  add(&c->rb_c32, 3 * g.nc(), 4);
   add(&c->cfB, p.nx + 1, 8);
   add(&c->cfT, p.nx + 1, 8);
   size_t S = 0;
@@ -660,8 +665,13 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
 }
 
 // Prepares a preconditioner once per solve (Jacobi diagonal, DCT scaling).
+static PhmuCoef32 phmu_c32(const drd_ctx* c) {
+  return PhmuCoef32{c->rb_c32, c->rb_c32 + c->g.nc(), c->rb_c32 + 2 * c->g.nc()};
+}
+
 static int prec_setup(drd_ctx* c, int prec) {
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag);
+  if (prec == DRD_PS_PHMU_RBGS32) launch_phmu_coef32(c->stream, c->g, coef_of(c), phmu_c32(c));
   if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 || prec == DRD_PS_VELO_LEARNED) {
     const bool fp32 = (prec != DRD_PS_VELO_DCT64);
     if (c->vprec && c->vprec_fp32 != fp32) {
@@ -679,10 +689,15 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     velo_prec_setup(c->stream, c->vprec, c->g, c->s, coef_of(c));
   }
-  if (prec == DRD_PS_VELO_MG) {
+  if (prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32) {
+    const bool f32 = prec == DRD_PS_VELO_MG32;
+    if (c->vmg && velo_mg_f32(c->vmg) != f32) {
+      velo_mg_destroy(c->vmg);
+      c->vmg = nullptr;
+    }
     if (!c->vmg) {
       int err = 0;
-      c->vmg = velo_mg_create(c->g, c->s, &err);
+      c->vmg = velo_mg_create(c->g, c->s, f32, &err);
       if (!c->vmg)
         return set_err(err ? err : DRD_E_HIP,
                        "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
@@ -737,10 +752,17 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       *res = x;
       return 0;
     case DRD_PS_PHMU_RBGS:
-      if (op != DRD_OP_PHMU || (c->g.nx & 1)) break;
-      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), b, x, ks);
+      if (op != DRD_OP_PHMU) break;
+      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), nullptr, b, x, c->rb_work, ks);
       *res = x;
       return 0;
+    case DRD_PS_PHMU_RBGS32: {  // the FP32 coefficients are those of its last prec_setup
+      if (op != DRD_OP_PHMU) break;
+      const PhmuCoef32 c32 = phmu_c32(c);
+      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), &c32, b, x, c->rb_work, ks);
+      *res = x;
+      return 0;
+    }
     case DRD_PS_PHMU_BLOCK:
       if (op != DRD_OP_PHMU) break;
       launch_phmu_block(c->stream, c->g, c->s, coef_of(c), b, x, ks);
@@ -753,7 +775,8 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       *res = x;
       return 0;
     case DRD_PS_VELO_MG:
-      if (op != DRD_OP_VELO2 || !c->vmg) break;
+    case DRD_PS_VELO_MG32:
+      if (op != DRD_OP_VELO2 || !c->vmg || velo_mg_f32(c->vmg) != (prec == DRD_PS_VELO_MG32)) break;
       velo_mg_apply(c->stream, c->vmg, b, x, ks);
       *res = x;
       return 0;
@@ -844,7 +867,8 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
                          (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
                           prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
-                          prec == DRD_PS_VELO_MG || prec == DRD_PS_PHMU_RBGS);
+                          prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32 || prec == DRD_PS_PHMU_RBGS ||
+                          prec == DRD_PS_PHMU_RBGS32);
   int chunk = 1, total = 0, next_snap = 0;
   for (;;) {
     hipGraphExec_t gx = nullptr;
@@ -1441,6 +1465,8 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
     case DRD_PREC_MG:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_MG;
+    case DRD_PREC_MG32:
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_MG32;
     default:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
   }
@@ -1538,9 +1564,10 @@ static int ph_mu_solve_dev(drd_ctx* c, double* ph_after, double* rmu_after, drd_
     launch_phmu_pack(c->stream, c->g, c->f[DRD_F_PH_PRE], c->f[DRD_F_RMU_PRE], c->tmp);
     launch_axpby(c->stream, 2 * c->g.ncell(), -1.0, c->tmp, 2.0, c->xv);
   }
-  const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK  ? DRD_PS_PHMU_BLOCK
-                   : c->p.ch_prec == DRD_CH_PREC_RBGS ? DRD_PS_PHMU_RBGS
-                                                      : DRD_PS_PHMU_JACOBI;
+  const int prec = c->p.ch_prec == DRD_CH_PREC_BLOCK    ? DRD_PS_PHMU_BLOCK
+                   : c->p.ch_prec == DRD_CH_PREC_RBGS   ? DRD_PS_PHMU_RBGS
+                   : c->p.ch_prec == DRD_CH_PREC_RBGS32 ? DRD_PS_PHMU_RBGS32
+                                                        : DRD_PS_PHMU_JACOBI;
   RCHK(run_bicgstab(c, DRD_OP_PHMU, prec, 2 * c->g.ncell(), c->rhs, c->xv, c->p.maxit_phmu, c->p.tol, info,
                     want_true));
   launch_phmu_unpack(c->stream, c->g, c->s, c->xv, c->pn, ph_after, rmu_after);
@@ -1835,8 +1862,8 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       RCHK(prec_setup(c, c->vprec_fp32 ? DRD_PS_VELO_DCT32 : DRD_PS_VELO_DCT64));
       *bytes = 2 * N8 + (c->vprec_fp32 ? N8 / 2 : N8);
       break;
-    case DRD_K_VELO_MG:  // r in, z out (FP64)
-      RCHK(prec_setup(c, DRD_PS_VELO_MG));
+    case DRD_K_VELO_MG:  // r in, z out (FP64); the context's hierarchy, FP64 if it has none
+      RCHK(prec_setup(c, velo_mg_f32(c->vmg) ? DRD_PS_VELO_MG32 : DRD_PS_VELO_MG));
       *bytes = 2 * N8;
       break;
     default:

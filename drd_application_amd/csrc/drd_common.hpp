@@ -64,6 +64,11 @@ template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ p, uint32_t i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (uint32_t)sizeof(T));
 }
+// a load of a stored FP32 or FP64 value, widened to double for the arithmetic
+template <class T>
+__device__ __forceinline__ double ldd(const T* __restrict__ p, uint32_t i) {
+  return (double)ld32(p, i);
+}
 template <class T>
 __device__ __forceinline__ void st32(T* __restrict__ p, uint32_t i, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (uint32_t)sizeof(T)) = v;
@@ -99,7 +104,20 @@ struct Scal {
   double coef_bot, coef_top;
   int lr_d;  // lr_bd_type == 'd'
   int phi_mode;
+  // quotients of the above that the CH stencils use (scal_derive)
+  double idr2, idz2, s_I, I_dz2, floor22;
 };
+
+// The stencils' uniform quotients, each the correctly rounded value of the
+// expression the stencil would evaluate (1.0 / dr2, s / I, I / dz2, the
+// block floor), so that using them changes no result.
+__host__ __device__ inline void scal_derive(Scal& s) {
+  s.idr2 = 1.0 / s.dr2;
+  s.idz2 = 1.0 / s.dz2;
+  s.s_I = s.s / s.I;
+  s.I_dz2 = s.I / s.dz2;
+  s.floor22 = 1e-3 * s.dt * s.xld * (2.0 / s.dr2 + 2.0 / s.dz2);
+}
 
 // Device-side Krylov state: the reverse-communication scalars of
 // LIB/BiCGSTABREVCOM.f:114-119 plus flags the kernels test so that the

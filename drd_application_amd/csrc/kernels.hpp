@@ -100,26 +100,28 @@ __device__ inline void kfinalize(KState* st, int phase, const double* t) {
 // Exact diagonal of the BDF2 velocity operator at packed unknown q
 // (MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 restricted to the
 // coefficient of the row's own unknown, wall and side rules included).
-__device__ inline double velo_diag2(const Grid& g, const Scal& s, const double* __restrict__ rho,
-                                    const double* __restrict__ eta, const double* __restrict__ chi, int64_t q) {
+template <class T>
+__device__ inline double velo_diag2(const Grid& g, const Scal& s, const T* __restrict__ rho,
+                                    const T* __restrict__ eta, const T* __restrict__ chi, int64_t q) {
   const int nx = g.nx, ny = g.ny;
+  auto R = [&](int i, int k) { return (double)rho[g.ci(i, k)]; };
+  auto E = [&](int i, int k) { return (double)eta[g.ci(i, k)]; };
+  auto X = [&](int i, int k) { return (double)chi[g.ci(i, k)]; };
   double d;
   if (q < g.mn()) {
     const int k = g.by_nx1.div(q), i = (int)(q - (int64_t)k * (nx + 1));
-    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i - 1, k)]) / 2.0 +
-        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i - 1, k)]) / 2.0 * s.dt;
-    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / s.dr2;
-    const double cr = (eta[g.ci(i, k + 1)] + eta[g.ci(i - 1, k + 1)] + eta[g.ci(i, k)] + eta[g.ci(i - 1, k)]) / 4.0;
-    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    d = 1.5 * s.re * (R(i, k) + R(i - 1, k)) / 2.0 + s.penalty_k * (X(i, k) + X(i - 1, k)) / 2.0 * s.dt;
+    d += 2.0 * s.dt * (E(i, k) + E(i - 1, k)) / s.dr2;
+    const double cr = (E(i, k + 1) + E(i - 1, k + 1) + E(i, k) + E(i - 1, k)) / 4.0;
+    const double cl = (E(i, k) + E(i - 1, k) + E(i, k - 1) + E(i - 1, k - 1)) / 4.0;
     d += s.dt * ((k == 0 ? 2.0 * cl : cl) + (k == ny - 1 ? 2.0 * cr : cr)) / s.dz2;
   } else {
     const int64_t t = q - g.mn();
     const int k = g.by_nx.div(t) + 1, i = (int)(t - (int64_t)(k - 1) * nx);
-    d = 1.5 * s.re * (rho[g.ci(i, k)] + rho[g.ci(i, k - 1)]) / 2.0 +
-        s.penalty_k * (chi[g.ci(i, k)] + chi[g.ci(i, k - 1)]) / 2.0 * s.dt;
-    d += 2.0 * s.dt * (eta[g.ci(i, k)] + eta[g.ci(i, k - 1)]) / s.dz2;
-    const double cr = (eta[g.ci(i + 1, k)] + eta[g.ci(i, k)] + eta[g.ci(i + 1, k - 1)] + eta[g.ci(i, k - 1)]) / 4.0;
-    const double cl = (eta[g.ci(i, k)] + eta[g.ci(i - 1, k)] + eta[g.ci(i, k - 1)] + eta[g.ci(i - 1, k - 1)]) / 4.0;
+    d = 1.5 * s.re * (R(i, k) + R(i, k - 1)) / 2.0 + s.penalty_k * (X(i, k) + X(i, k - 1)) / 2.0 * s.dt;
+    d += 2.0 * s.dt * (E(i, k) + E(i, k - 1)) / s.dz2;
+    const double cr = (E(i + 1, k) + E(i, k) + E(i + 1, k - 1) + E(i, k - 1)) / 4.0;
+    const double cl = (E(i, k) + E(i - 1, k) + E(i, k - 1) + E(i - 1, k - 1)) / 4.0;
     d += s.dt * ((i == 0 ? 0.0 : cl) + (i == nx - 1 ? 0.0 : cr)) / s.dr2;
   }
   return d;
@@ -150,8 +152,11 @@ __device__ __forceinline__ Nb5 nb5_packed(const Grid& g, const double* x, int i,
 }
 
 // diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219)
-// from the mobility at the cell and its neighbours; only the sums are selected
-__device__ __forceinline__ double dgn_diag_v(const Grid& g, double dx2, double dy2, const Nb5& mo, int i, int j) {
+// from the mobility at the cell and its neighbours; only the sums are selected.
+// FAST multiplies by the reciprocals instead of dividing (not the
+// reference's rounding; for the FP32-storage preconditioner only).
+template <bool FAST = false>
+__device__ __forceinline__ double dgn_diag_v(const Grid& g, const Scal& s, const Nb5& mo, int i, int j) {
   const double tl = (mo.xl + mo.c) / 2.0, tr = (mo.c + mo.xr) / 2.0;
   const double bl = (mo.yl + mo.c) / 2.0, br = (mo.c + mo.yr) / 2.0;
   double dx = 0.0, dy = 0.0;
@@ -159,29 +164,32 @@ __device__ __forceinline__ double dgn_diag_v(const Grid& g, double dx2, double d
   dx = i < g.nx - 1 ? dx + tr : dx;
   dy = j > 0 ? dy + bl : dy;
   dy = j < g.ny - 1 ? dy + br : dy;
-  return -(dx / dx2 + dy / dy2);
-}
-__device__ __forceinline__ double dgn_diag(const Grid& g, double dx2, double dy2, const double* __restrict__ mo,
-                                           int i, int j) {
-  return dgn_diag_v(g, dx2, dy2, nb5_cell(g, mo, i, j), i, j);
+  if (FAST) return -(dx * s.idr2 + dy * s.idz2);
+  return -(dx / s.dr2 + dy / s.dz2);
 }
 
 // x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell, from
 // bu at the cell and the two mobilities around it
+template <bool FAST = false>
 __device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, double bv, const Nb5& mi, const Nb5& m0,
                                              int i, int j, double r1, double r2, double& x1, double& x2) {
-  const double d11 = s.s / s.I - s.I * dgn_diag_v(g, s.dr2, s.dz2, mi, i, j) +
-                     ((j == 0) ? s.I / s.dz2 * (1.0 - s.coef_bot) : 0.0) +
-                     ((j == g.ny - 1) ? s.I / s.dz2 * (1.0 - s.coef_top) : 0.0);
+  const double d11 = s.s_I - s.I * dgn_diag_v<FAST>(g, s, mi, i, j) +
+                     ((j == 0) ? s.I_dz2 * (1.0 - s.coef_bot) : 0.0) +
+                     ((j == g.ny - 1) ? s.I_dz2 * (1.0 - s.coef_top) : 0.0);
   const double d12 = -bv;
   const double d21 = bv + s.adj;
-  // mobility-free floor keeps the block invertible where M_0 vanishes
-  const double floor22 = 1e-3 * s.dt * s.xld * (2.0 / s.dr2 + 2.0 / s.dz2);
-  double d22 = -s.dt * s.xld * dgn_diag_v(g, s.dr2, s.dz2, m0, i, j);
-  d22 = d22 > floor22 ? d22 : floor22;
+  // mobility-free floor (s.floor22) keeps the block invertible where M_0 vanishes
+  double d22 = -s.dt * s.xld * dgn_diag_v<FAST>(g, s, m0, i, j);
+  d22 = d22 > s.floor22 ? d22 : s.floor22;
   const double det = d11 * d22 - d12 * d21;
-  x1 = (d22 * r1 - d12 * r2) / det;
-  x2 = (-d21 * r1 + d11 * r2) / det;
+  if (FAST) {
+    const double idet = 1.0 / det;
+    x1 = (d22 * r1 - d12 * r2) * idet;
+    x2 = (-d21 * r1 + d11 * r2) * idet;
+  } else {
+    x1 = (d22 * r1 - d12 * r2) / det;
+    x2 = (-d21 * r1 + d11 * r2) / det;
+  }
 }
 __device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
                                                  const double* __restrict__ minner,
@@ -190,27 +198,45 @@ __device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, c
   phmu_block_v(g, s, bu[g.ci(i, j)], nb5_cell(g, minner, i, j), nb5_cell(g, m0, i, j), i, j, r1, r2, x1, x2);
 }
 
-void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
-                      const KState* ks);
+// FP32 copies of the CH block coefficients (cell fields, ghosts included)
+struct PhmuCoef32 {
+  const float *bu, *minner, *m0;
+};
+void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32);
+// c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
+// work = 4 ncell floats
+void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
+                      const double* b, double* x, void* work, const KState* ks);
 
 // ---- multigrid V-cycle for the BDF2 velocity operator (kernels_mg.hip) ----
 enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2 };
 struct VeloMG;
-VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err);
+// f32: the levels' coefficients, diagonals and vectors are stored in FP32
+// (arithmetic stays FP64); the input b, the output z and the coarsest level
+// stay FP64
+VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err);
 void velo_mg_destroy(VeloMG* mg);
+bool velo_mg_f32(const VeloMG* mg);
 void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0);
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks);
 int velo_mg_levels(const VeloMG* mg);
-// its kernels (kernels_stencil.hip)
-void launch_mg_coarsen(hipStream_t st, const Grid& gf, const Grid& gc, const Coef& f, double* rc, double* ec,
-                       double* cc);
-void launch_mg_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* d);
-void launch_mg_smooth(hipStream_t st, int kind, const Grid& g, const Scal& s, const Coef& c, const double* x,
-                      const double* b, const double* d, double* out, double omega, const KState* ks);
-void launch_mg_restrict(hipStream_t st, const Grid& gf, const Grid& gc, const double* r, double* bc,
-                        const KState* ks);
-void launch_mg_prolong_add(hipStream_t st, const Grid& gc, const Grid& gf, const double* xc, double* x,
-                           const KState* ks);
+// its kernels (kernels_stencil.hip); a buffer's element type is FP32 where
+// its flag says so, else FP64
+struct MgCoef {
+  const void *rho, *eta, *chi;
+};
+void launch_mg_convert(hipStream_t st, const Grid& g, const Coef& c, float* rho, float* eta, float* chi);
+void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const Grid& gc, const MgCoef& f,
+                       void* rc, void* ec, void* cc);
+void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* d);
+// coefficients, x and d of the level's type (f32); b and out of their own
+void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
+                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
+                      const KState* ks);
+void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
+                        void* bc, const KState* ks);
+void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,
+                           void* x, const KState* ks);
 
 // ---- reduction finish (kernels_krylov.hip) ----
 // k_finalize runs as one workgroup after a reducing kernel: it adds the

@@ -10,8 +10,13 @@
 //   coarsest: x = the FP64 DCT/DST fast-Poisson solve of its b
 //   up:    x += P x_{l+1}, two damped Jacobi sweeps
 // with omega = 0.7 and d the exact diagonal of each level's operator.  The
-// cycle is a fixed linear map of b, so BiCGSTAB accepts it.  Every kernel
-// skips once the solve it serves is done or has taken its early exit.
+// cycle is a fixed map of b, so BiCGSTAB accepts it.  Every kernel skips
+// once the solve it serves is done or has taken its early exit.
+//
+// FP32 storage: every level but the coarsest keeps its coefficients,
+// diagonal and vectors in FP32 (level 0 an FP32 copy of the context's
+// fields), which halves the bytes of each sweep; the arithmetic, the input b,
+// the output z and the coarsest level stay FP64.
 #include <vector>
 
 #include "kernels.hpp"
@@ -21,20 +26,23 @@ namespace drd {
 struct MGLevel {
   Grid g;
   Scal s;
-  Coef c{};                   // rho, eta, chi of the level (level 0: the context's fields)
-  double* fields = nullptr;   // own coarse rho, eta, chi (levels >= 1)
-  double *d = nullptr, *x = nullptr, *t = nullptr, *b = nullptr, *r = nullptr;
+  bool f32 = false;            // storage type of the level's buffers
+  MgCoef c{};                  // rho, eta, chi of the level (FP64 level 0: the context's fields)
+  void* fields = nullptr;      // own rho, eta, chi
+  void *d = nullptr, *x = nullptr, *t = nullptr, *b = nullptr, *r = nullptr;
 };
 
 struct VeloMG {
   std::vector<MGLevel> lv;
   VeloPrec* coarse = nullptr;
+  bool f32 = false;
   double omega = 0.7;
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 int velo_mg_levels(const VeloMG* mg) { return mg ? (int)mg->lv.size() : 0; }
+bool velo_mg_f32(const VeloMG* mg) { return mg && mg->f32; }
 
 void velo_mg_destroy(VeloMG* mg) {
   if (!mg) return;
@@ -50,13 +58,14 @@ void velo_mg_destroy(VeloMG* mg) {
   delete mg;
 }
 
-VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err) {
+VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err) {
   *err = 0;
   if (!is_pow2(g.nx) || !is_pow2(g.ny) || g.nx < 128 || g.ny < 64) {
     *err = DRD_E_UNSUPPORTED;
     return nullptr;
   }
   VeloMG* mg = new VeloMG();
+  mg->f32 = f32;
   int nx = g.nx, ny = g.ny;
   Scal sl = s;
   for (;;) {
@@ -71,21 +80,25 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err) {
     sl.dz *= 2.0;
     sl.dr2 = sl.dr * sl.dr;
     sl.dz2 = sl.dz * sl.dz;
+    scal_derive(sl);
   }
   bool ok = true;
-  for (size_t l = 0; l < mg->lv.size() && ok; ++l) {
+  const size_t nl = mg->lv.size();
+  for (size_t l = 0; l < nl && ok; ++l) {
     MGLevel& L = mg->lv[l];
-    const size_t n = (size_t)L.g.nuv() * sizeof(double);
+    L.f32 = f32 && l + 1 < nl;
+    const size_t es = L.f32 ? sizeof(float) : sizeof(double);
+    const size_t n = (size_t)L.g.nuv() * es;
     ok = hipMalloc(&L.d, n) == hipSuccess && hipMalloc(&L.t, n) == hipSuccess;
-    if (ok && l + 1 < mg->lv.size()) ok = hipMalloc(&L.r, n) == hipSuccess;
-    if (ok && l > 0) {
-      const size_t nc = (size_t)L.g.nc() * sizeof(double);
-      ok = hipMalloc(&L.fields, 3 * nc) == hipSuccess && hipMalloc(&L.x, n) == hipSuccess &&
-           hipMalloc(&L.b, n) == hipSuccess;
+    if (ok && l + 1 < nl) ok = hipMalloc(&L.r, n) == hipSuccess;
+    if (ok && (l > 0 || L.f32)) ok = hipMalloc(&L.x, n) == hipSuccess;  // FP64 level 0: x is z
+    if (ok && l > 0) ok = hipMalloc(&L.b, n) == hipSuccess;              // level 0: b is the input
+    if (ok && (l > 0 || L.f32)) {
+      const size_t nc = (size_t)L.g.nc() * es;
+      ok = hipMalloc(&L.fields, 3 * nc) == hipSuccess;
       if (ok) {
-        L.c.rho = L.fields;
-        L.c.eta = L.fields + L.g.nc();
-        L.c.chi = L.fields + 2 * L.g.nc();
+        char* f = (char*)L.fields;
+        L.c = MgCoef{f, f + nc, f + 2 * nc};
       }
     }
   }
@@ -94,7 +107,7 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err) {
     mg->coarse = velo_prec_create(C.g, C.s, false, err, nullptr);
     ok = mg->coarse != nullptr;
   }
-  if (!ok || mg->lv.size() < 2) {
+  if (!ok || nl < 2) {
     velo_mg_destroy(mg);
     if (!*err) *err = DRD_E_HIP;
     return nullptr;
@@ -105,42 +118,53 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, int* err) {
 // Coarse coefficients, every level's diagonal and the coarsest level's
 // fast-Poisson scaling from the current fields (once per solve).
 void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
-  mg->lv[0].c = c0;
+  MGLevel& L0 = mg->lv[0];
+  if (L0.f32)
+    launch_mg_convert(st, L0.g, c0, (float*)L0.c.rho, (float*)L0.c.eta, (float*)L0.c.chi);
+  else
+    L0.c = MgCoef{c0.rho, c0.eta, c0.chi};
   for (size_t l = 1; l < mg->lv.size(); ++l) {
     MGLevel& L = mg->lv[l];
-    launch_mg_coarsen(st, mg->lv[l - 1].g, L.g, mg->lv[l - 1].c, const_cast<double*>(L.c.rho),
-                      const_cast<double*>(L.c.eta), const_cast<double*>(L.c.chi));
+    const MGLevel& F = mg->lv[l - 1];
+    launch_mg_coarsen(st, F.f32, L.f32, F.g, L.g, F.c, const_cast<void*>(L.c.rho), const_cast<void*>(L.c.eta),
+                      const_cast<void*>(L.c.chi));
   }
-  for (auto& L : mg->lv) launch_mg_diag(st, L.g, L.s, L.c, L.d);
-  const MGLevel& C = mg->lv.back();
-  velo_prec_setup(st, mg->coarse, C.g, C.s, C.c);
+  for (auto& L : mg->lv) launch_mg_diag(st, L.f32, L.g, L.s, L.c, L.d);
+  const MGLevel& C = mg->lv.back();  // FP64
+  const Coef cc{(const double*)C.c.rho, (const double*)C.c.eta, (const double*)C.c.chi};
+  velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
 }
 
 // z = V-cycle(b).  Level l smooths in two buffers: the first sweep writes
 // A, the next B, the residual and the coarse correction act on B, and the
-// two sweeps after it return to B (level 0: A = its temporary, B = z).
+// two sweeps after it return to B (level 0: A = its temporary, B = z, or its
+// FP32 x with the last sweep writing z).
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks) {
   const int nl = (int)mg->lv.size();
   const double w = mg->omega;
   auto bufA = [&](int l) { return mg->lv[l].t; };
-  auto bufB = [&](int l) { return l == 0 ? z : mg->lv[l].x; };
-  auto rhs = [&](int l) { return l == 0 ? b : (const double*)mg->lv[l].b; };
+  auto bufB = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (void*)z : mg->lv[l].x; };
+  auto rhs = [&](int l) { return l == 0 ? (const void*)b : (const void*)mg->lv[l].b; };
+  auto b32 = [&](int l) { return l > 0 && mg->lv[l].f32; };  // the input b is FP64
   for (int l = 0; l + 1 < nl; ++l) {
     MGLevel& L = mg->lv[l];
-    launch_mg_smooth(st, MG_FIRST, L.g, L.s, L.c, nullptr, rhs(l), L.d, bufA(l), w, ks);
-    launch_mg_smooth(st, MG_SWEEP, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), w, ks);
-    launch_mg_smooth(st, MG_RES, L.g, L.s, L.c, bufB(l), rhs(l), L.d, L.r, w, ks);
-    launch_mg_restrict(st, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
+    const bool f = L.f32;
+    launch_mg_smooth(st, MG_FIRST, f, b32(l), f, L.g, L.s, L.c, nullptr, rhs(l), L.d, bufA(l), w, ks);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), w, ks);
+    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, L.r, w, ks);
+    launch_mg_restrict(st, f, mg->lv[l + 1].f32, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
   }
   {
-    const MGLevel& C = mg->lv[nl - 1];
-    velo_prec_apply(st, mg->coarse, C.g, C.b, C.x, ks);
+    const MGLevel& C = mg->lv[nl - 1];  // FP64
+    velo_prec_apply(st, mg->coarse, C.g, (const double*)C.b, (double*)C.x, ks);
   }
   for (int l = nl - 2; l >= 0; --l) {
     MGLevel& L = mg->lv[l];
-    launch_mg_prolong_add(st, mg->lv[l + 1].g, L.g, mg->lv[l + 1].x, bufB(l), ks);
-    launch_mg_smooth(st, MG_SWEEP, L.g, L.s, L.c, bufB(l), rhs(l), L.d, bufA(l), w, ks);
-    launch_mg_smooth(st, MG_SWEEP, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), w, ks);
+    const bool f = L.f32;
+    launch_mg_prolong_add(st, mg->lv[l + 1].f32, f, mg->lv[l + 1].g, L.g, mg->lv[l + 1].x, bufB(l), ks);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, bufA(l), w, ks);
+    void* out = l == 0 ? (void*)z : bufB(l);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && l > 0, L.g, L.s, L.c, bufA(l), rhs(l), L.d, out, w, ks);
   }
 }
 

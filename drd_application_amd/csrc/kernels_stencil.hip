@@ -15,11 +15,14 @@ namespace drd {
 // Ghost rules applied on the fly to a packed velocity vector:
 // btoA_uv TP/matrix_tool.f90:32-75 plus the overrides of
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:554-583.
-struct VeloIn {
-  const double* __restrict__ x;
+// The vector's element type T is FP64, or FP32 in the multigrid levels that
+// store FP32; values are read as double.
+template <class T>
+struct VeloInT {
+  const T* __restrict__ x;
   int nx, ny, lr_d;
   uint32_t mn;
-  __device__ __forceinline__ double uraw(int i, int k) const { return ld32(x, (uint32_t)(i + (nx + 1) * k)); }
+  __device__ __forceinline__ double uraw(int i, int k) const { return ldd(x, (uint32_t)(i + (nx + 1) * k)); }
   // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
   // wall ghosts are odd mirrors of the first/last row, side ghosts even or
   // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
@@ -29,7 +32,7 @@ struct VeloIn {
     const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
     const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
     const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
-    const double v = ld32(x, (uint32_t)(ii + (nx + 1) * kk));
+    const double v = ldd(x, (uint32_t)(ii + (nx + 1) * kk));
     const double w = iw ? -v : v;
     return kw ? -w : w;
   }
@@ -37,18 +40,19 @@ struct VeloIn {
   __device__ __forceinline__ double V(int i, int k) const {
     const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
     const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
-    const double v = ld32(x, mn + (uint32_t)(ic + nx * (kc - 1)));
+    const double v = ldd(x, mn + (uint32_t)(ic + nx * (kc - 1)));
     return (k <= 0 || k >= ny) ? 0.0 : v;
   }
 };
+using VeloIn = VeloInT<double>;
 
 // y = A x for one unknown q of the velocity operator.
 // MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
-template <int BDF>
-__device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const double* __restrict__ rho,
-                                             const double* __restrict__ eta, const double* __restrict__ chi,
-                                             const VeloIn& X, uint32_t q) {
+template <int BDF, class T, class XT>
+__device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                             const T* __restrict__ eta, const T* __restrict__ chi,
+                                             const VeloInT<XT>& X, uint32_t q) {
   const int nx = g.nx;
   double r;
   {
@@ -57,25 +61,25 @@ __device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const 
     const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
     const double f = X.U(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (ld32(rho, c) + ld32(rho, cw)) / 2.0 +
-          f * s.penalty_k * (ld32(chi, c) + ld32(chi, cw)) / 2.0 * s.dt;
+      r = f * 1.5 * s.re * (ldd(rho, c) + ldd(rho, cw)) / 2.0 +
+          f * s.penalty_k * (ldd(chi, c) + ldd(chi, cw)) / 2.0 * s.dt;
     else
-      r = f * s.re * (ld32(rho, c) + ld32(rho, cw)) / 2.0;
-    const double e00 = ld32(eta, c), e10 = ld32(eta, cw);
+      r = f * s.re * (ldd(rho, c) + ldd(rho, cw)) / 2.0;
+    const double e00 = ldd(eta, c), e10 = ldd(eta, cw);
     double cr = e00, cl = e10;
     r = r - 2.0 * s.dt * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f) / s.dr2;
-    cr = (ld32(eta, c + ld) + ld32(eta, cw + ld) + e00 + e10) / 4.0;
-    cl = (e00 + e10 + ld32(eta, c - ld) + ld32(eta, cw - ld)) / 4.0;
+    cr = (ldd(eta, c + ld) + ldd(eta, cw + ld) + e00 + e10) / 4.0;
+    cl = (e00 + e10 + ldd(eta, c - ld) + ldd(eta, cw - ld)) / 4.0;
     r = r - s.dt * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f) / s.dz2;
     r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
   }
   return r;
 }
 
-template <int BDF>
-__device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const double* __restrict__ rho,
-                                             const double* __restrict__ eta, const double* __restrict__ chi,
-                                             const VeloIn& X, uint32_t q) {
+template <int BDF, class T, class XT>
+__device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                             const T* __restrict__ eta, const T* __restrict__ chi,
+                                             const VeloInT<XT>& X, uint32_t q) {
   const int nx = g.nx;
   double r;
   {
@@ -85,26 +89,27 @@ __device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const 
     const uint32_t c = g.cio(i, k), ld = (uint32_t)(nx + 2), cs = c - ld;
     const double f = X.V(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (ld32(rho, c) + ld32(rho, cs)) / 2.0 +
-          f * s.penalty_k * (ld32(chi, c) + ld32(chi, cs)) / 2.0 * s.dt;
+      r = f * 1.5 * s.re * (ldd(rho, c) + ldd(rho, cs)) / 2.0 +
+          f * s.penalty_k * (ldd(chi, c) + ldd(chi, cs)) / 2.0 * s.dt;
     else
-      r = f * s.re * (ld32(rho, c) + ld32(rho, cs)) / 2.0;
-    const double e00 = ld32(eta, c), e01 = ld32(eta, cs);
+      r = f * s.re * (ldd(rho, c) + ldd(rho, cs)) / 2.0;
+    const double e00 = ldd(eta, c), e01 = ldd(eta, cs);
     double cr = e00, cl = e01;
     r = r - 2.0 * s.dt * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f) / s.dz2;
-    cr = (ld32(eta, c + 1) + e00 + ld32(eta, cs + 1) + e01) / 4.0;
-    cl = (e00 + ld32(eta, c - 1) + e01 + ld32(eta, cs - 1)) / 4.0;
+    cr = (ldd(eta, c + 1) + e00 + ldd(eta, cs + 1) + e01) / 4.0;
+    cl = (e00 + ldd(eta, c - 1) + e01 + ldd(eta, cs - 1)) / 4.0;
     r = r - s.dt * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f) / s.dr2;
     r = r - s.dt * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1))) / s.dr / s.dz;
   }
   return r;
 }
 
-template <int BDF>
-__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const double* __restrict__ rho,
-                                           const double* __restrict__ eta, const double* __restrict__ chi,
-                                           const VeloIn& X, uint32_t q) {
-  return q < X.mn ? velo_u_row<BDF>(g, s, rho, eta, chi, X, q) : velo_v_row<BDF>(g, s, rho, eta, chi, X, q);
+template <int BDF, class T, class XT>
+__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                           const T* __restrict__ eta, const T* __restrict__ chi,
+                                           const VeloInT<XT>& X, uint32_t q) {
+  return q < X.mn ? velo_u_row<BDF, T, XT>(g, s, rho, eta, chi, X, q)
+                   : velo_v_row<BDF, T, XT>(g, s, rho, eta, chi, X, q);
 }
 
 __device__ __forceinline__ bool kactive(const KState* st) {
@@ -428,35 +433,31 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
 // index to the cell does exactly that with the interior expression, term
 // for term, so every load is unconditional and a thread issues them all at
 // once (nx, ny >= 2, which drd_init requires).
-__device__ __forceinline__ double op_dgn_v(double dx2, double dy2, const Nb5& mo, const Nb5& x) {
+// idx2, idy2 = 1.0 / dx2, 1.0 / dy2, the factors the reference evaluates
+__device__ __forceinline__ double op_dgn_v(double idx2, double idy2, const Nb5& mo, const Nb5& x) {
   double moL, moR, rhs = 0.0;
   moL = (mo.xl + mo.c) / 2.0;
   moR = (mo.c + mo.xr) / 2.0;
-  rhs = rhs + 1.0 / dx2 * (moL * x.xl + moR * x.xr - (moL + moR) * x.c);
+  rhs = rhs + idx2 * (moL * x.xl + moR * x.xr - (moL + moR) * x.c);
   moL = (mo.yl + mo.c) / 2.0;
   moR = (mo.c + mo.yr) / 2.0;
-  rhs = rhs + 1.0 / dy2 * (moL * x.yl + moR * x.yr - (moL + moR) * x.c);
+  rhs = rhs + idy2 * (moL * x.yl + moR * x.yr - (moL + moR) * x.c);
   return rhs;
-}
-template <class F>
-__device__ __forceinline__ double op_dgn(const Grid g, double dx2, double dy2, const double* __restrict__ mo,
-                                         int i, int j, F get) {
-  const int nx = g.nx, ny = g.ny;
-  const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
-  const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
-  return op_dgn_v(dx2, dy2, nb5_cell(g, mo, i, j), Nb5{get(i, j), get(il, j), get(ir, j), get(i, jl), get(i, jr)});
 }
 
 // MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:212-284 for cell (i, j) (both
 // rows) from the values around it: phi and mu (f1, f2) and the two mobilities
+// (FAST: s.s / s.I * f1 for s.s * f1 / s.I, not the reference's rounding;
+// for the FP32-storage preconditioner only)
+template <bool FAST = false>
 __device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, double b, const Nb5& mi,
                                             const Nb5& m0, const Nb5& f1, const Nb5& f2, double& o1, double& o2) {
-  double ff1 = op_dgn_v(s.dr2, s.dz2, mi, f1);
-  ff1 = s.s * f1.c / s.I - s.I * ff1 - f2.c * b;
+  double ff1 = op_dgn_v(s.idr2, s.idz2, mi, f1);
+  ff1 = (FAST ? s.s_I * f1.c : s.s * f1.c / s.I) - s.I * ff1 - f2.c * b;
   // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
-  if (j == 0) ff1 = ff1 + s.I / s.dz2 * (f1.c - s.coef_bot * f1.c);
-  if (j == g.ny - 1) ff1 = ff1 + s.I / s.dz2 * (f1.c - s.coef_top * f1.c);
-  double ff2 = op_dgn_v(s.dr2, s.dz2, m0, f2);
+  if (j == 0) ff1 = ff1 + s.I_dz2 * (f1.c - s.coef_bot * f1.c);
+  if (j == g.ny - 1) ff1 = ff1 + s.I_dz2 * (f1.c - s.coef_top * f1.c);
+  double ff2 = op_dgn_v(s.idr2, s.idz2, m0, f2);
   ff2 = f1.c * (b + s.adj) - s.dt * s.xld * ff2;
   o1 = ff1;
   o2 = ff2;
@@ -690,49 +691,135 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 }
 
 // ---- red-black block Gauss-Seidel for the phi/mu system -------------------
-// One half-sweep over the cells of one colour ((i + j) % 2 == color), one
-// thread per such cell (nx even: nx/2 of them per row): the residual of the
-// cell's two rows on the current x (phmu_rows, as the matvec computes
-// them), the exact 2x2 block solve, x += the correction.  A cell's rows read
-// x at the cell itself and at its four neighbours, which have the other
-// colour, so the cells of one colour update in place without a race.
-// FIRST starts from x = 0: the colour's cells get B^-1 b, the others 0.
-template <bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_phmu_rbgs(Grid g, Scal s, const double* __restrict__ bu,
-                                                      const double* __restrict__ minner,
-                                                      const double* __restrict__ m0, const double* __restrict__ b,
-                                                      double* x, int color, const KState* st, int64_t bs) {
+// One full sweep per launch: the red cells ((i + j) even), then the black
+// ones, each updated as x_c += B_c^-1 (b - A x)_c with the rows of the
+// matvec (phmu_rows_v) and the exact 2x2 block solve (phmu_block_v).
+// A workgroup stages its tile of kRbTX x kRbTY cells and a halo of two cells
+// (phi, mu, b, both mobilities, bu) in LDS, updates the red cells of the
+// tile and of a one-cell ring around it, then the black cells of the tile,
+// and writes the tile to xo.  A red cell's rows read only black neighbours
+// (and a black cell's only red ones), so each colour updates in place in
+// LDS; the ring's red cells belong to the neighbouring tiles, and every
+// workgroup that recomputes one does so from the same values of xi, so a
+// launch gives bit for bit the two colour half-sweeps.  xo must not alias
+// xi: a workgroup that starts after its neighbour has written would read
+// that neighbour's new values in its halo.  FIRST: from x = 0 (xi unread).
+// The coefficients are stored as TC, xi as TI and xo as TO (FP64, or FP32
+// storage with the arithmetic in FP64); b is FP64.
+constexpr int kRbTX = 64, kRbTY = 8, kRbH = 2;
+constexpr int kRbW = kRbTX + 2 * kRbH, kRbHt = kRbTY + 2 * kRbH;
+template <bool FIRST, class TC, class TI, class TO>
+__global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
+                                                          const TC* __restrict__ minner,
+                                                          const TC* __restrict__ m0,
+                                                          const double* __restrict__ b, const TI* __restrict__ xi,
+                                                          TO* __restrict__ xo, const KState* st, int64_t bs) {
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
   bu = bofs(bu, bs);
   minner = bofs(minner, bs);
   m0 = bofs(m0, bs);
   b = bofs(b, bs);
-  x = bofs(x, bs);
+  xi = bofs(xi, bs);
+  xo = bofs(xo, bs);
+  __shared__ double sx1[kRbHt][kRbW], sx2[kRbHt][kRbW], smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW];
+  __shared__ double sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
+  const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
-  const int half = g.nx >> 1;
-  const int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (u >= (int64_t)half * g.ny) return;
-  const int j = (int)(u / half), t = (int)(u - (int64_t)j * half);
-  const int i = 2 * t + ((color + j) & 1);
-  const int64_t q = i + (int64_t)g.nx * j;
-  if (FIRST) {
-    phmu_block_solve(g, s, bu, minner, m0, i, j, b[q], b[C + q], x[q], x[C + q]);
-    const int64_t o = (2 * t + ((color + j + 1) & 1)) + (int64_t)g.nx * j;
-    x[o] = 0.0;
-    x[C + o] = 0.0;
-    return;
+  const int i0 = (int)blockIdx.x * kRbTX, j0 = (int)blockIdx.y * kRbTY;
+  // staging: every load of the thread is issued before the first LDS store
+  // (indices clamped into the grid; the stores keep the cells inside it)
+  constexpr int kStage = kRbW * kRbHt, kStageIt = (kStage + kBlock - 1) / kBlock;
+  double v[kStageIt][7];
+#pragma unroll
+  for (int it = 0; it < kStageIt; ++it) {
+    const int e = min((int)threadIdx.x + it * kBlock, kStage - 1);
+    const int i = min(max(i0 - kRbH + e % kRbW, 0), nx - 1), j = min(max(j0 - kRbH + e / kRbW, 0), ny - 1);
+    const int64_t q = i + (int64_t)nx * j, c = g.ci(i, j);
+    v[it][0] = FIRST ? 0.0 : (double)xi[q];
+    v[it][1] = FIRST ? 0.0 : (double)xi[C + q];
+    v[it][2] = (double)minner[c];
+    v[it][3] = (double)m0[c];
+    v[it][4] = (double)bu[c];
+    v[it][5] = b[q];
+    v[it][6] = b[C + q];
   }
-  // every value first (23 loads in flight), then the arithmetic of the
-  // matvec's rows and of the block solve on them
-  const Nb5 mi = nb5_cell(g, minner, i, j), mz = nb5_cell(g, m0, i, j);
-  const Nb5 f1 = nb5_packed(g, x, i, j), f2 = nb5_packed(g, x + C, i, j);
-  const double bv = bu[g.ci(i, j)], b1 = b[q], b2 = b[C + q];
-  double z1, z2, y1, y2;
-  phmu_rows_v(g, s, j, bv, mi, mz, f1, f2, z1, z2);
-  phmu_block_v(g, s, bv, mi, mz, i, j, b1 - z1, b2 - z2, y1, y2);
-  x[q] = f1.c + y1;
-  x[C + q] = f2.c + y2;
+#pragma unroll
+  for (int it = 0; it < kStageIt; ++it) {
+    const int e = (int)threadIdx.x + it * kBlock;
+    const int li = e % kRbW, lj = e / kRbW;
+    const int i = i0 - kRbH + li, j = j0 - kRbH + lj;
+    if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
+    sx1[lj][li] = v[it][0];
+    sx2[lj][li] = v[it][1];
+    smi[lj][li] = v[it][2];
+    smz[lj][li] = v[it][3];
+    sbu[lj][li] = v[it][4];
+    sb1[lj][li] = v[it][5];
+    sb2[lj][li] = v[it][6];
+  }
+  __syncthreads();
+  auto update = [&](int i, int j) {
+    const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
+    const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
+    const int a = i - i0 + kRbH, al = il - i0 + kRbH, ar = ir - i0 + kRbH;
+    const int r = j - j0 + kRbH, rl = jl - j0 + kRbH, rr = jr - j0 + kRbH;
+    const Nb5 mi{smi[r][a], smi[r][al], smi[r][ar], smi[rl][a], smi[rr][a]};
+    const Nb5 mz{smz[r][a], smz[r][al], smz[r][ar], smz[rl][a], smz[rr][a]};
+    const Nb5 f1{sx1[r][a], sx1[r][al], sx1[r][ar], sx1[rl][a], sx1[rr][a]};
+    const Nb5 f2{sx2[r][a], sx2[r][al], sx2[r][ar], sx2[rl][a], sx2[rr][a]};
+    const double bv = sbu[r][a];
+    double z1, z2, y1, y2;
+    constexpr bool kFast = sizeof(TC) == sizeof(float);  // FP32 storage: reciprocals, not divisions
+    phmu_rows_v<kFast>(g, s, j, bv, mi, mz, f1, f2, z1, z2);
+    phmu_block_v<kFast>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
+    sx1[r][a] = f1.c + y1;
+    sx2[r][a] = f2.c + y2;
+  };
+  // threads take the cells of one colour in row order: a row of the red
+  // region (tile and ring, kRbTX + 2 wide) holds (kRbTX + 2) / 2 red cells, a
+  // tile row kRbTX / 2 black ones
+  constexpr int kRedRow = (kRbTX + 2) / 2, kBlackRow = kRbTX / 2;
+  for (int e = threadIdx.x; e < kRedRow * (kRbTY + 2); e += kBlock) {
+    const int j = j0 - 1 + e / kRedRow;
+    const int i = i0 - 1 + 2 * (e % kRedRow) + ((i0 - 1 + j) & 1);
+    if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
+    update(i, j);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kBlackRow * kRbTY; e += kBlock) {
+    const int j = j0 + e / kBlackRow;
+    const int i = i0 + 2 * (e % kBlackRow) + ((i0 + j + 1) & 1);
+    if (i >= nx || j >= ny) continue;
+    update(i, j);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kRbTX * kRbTY; e += kBlock) {
+    const int i = i0 + e % kRbTX, j = j0 + e / kRbTX;
+    if (i >= nx || j >= ny) continue;
+    const int64_t q = i + (int64_t)nx * j;
+    xo[q] = (TO)sx1[j - j0 + kRbH][i - i0 + kRbH];
+    xo[C + q] = (TO)sx2[j - j0 + kRbH][i - i0 + kRbH];
+  }
+}
+
+// FP32 copies of the CH block coefficients bu, M_inner, M_0 (ghosts included)
+__global__ __launch_bounds__(kBlock) void k_phmu_coef32(int64_t n, const double* __restrict__ bu,
+                                                        const double* __restrict__ mi,
+                                                        const double* __restrict__ m0, float* __restrict__ bu32,
+                                                        float* __restrict__ mi32, float* __restrict__ m032,
+                                                        int64_t bs) {
+  bu = bofs(bu, bs);
+  mi = bofs(mi, bs);
+  m0 = bofs(m0, bs);
+  bu32 = bofs(bu32, bs);
+  mi32 = bofs(mi32, bs);
+  m032 = bofs(m032, bs);
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  bu32[q] = (float)bu[q];
+  mi32[q] = (float)mi[q];
+  m032[q] = (float)m0[q];
 }
 
 // ---- multigrid pieces for the BDF2 velocity operator ----------------------
@@ -763,13 +850,26 @@ __device__ __forceinline__ double mg_w_cell_even(int f, int C, int nc) {
   return (C == c ? 0.75 : 0.0) + (C == o ? 0.25 : 0.0);
 }
 
+// Level 0's rho, eta, chi (ghosts included) stored as FP32.
+__global__ __launch_bounds__(kBlock) void k_mg_convert(int64_t n, const double* __restrict__ r,
+                                                       const double* __restrict__ e,
+                                                       const double* __restrict__ c, float* __restrict__ rf,
+                                                       float* __restrict__ ef, float* __restrict__ cf) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  rf[q] = (float)r[q];
+  ef[q] = (float)e[q];
+  cf[q] = (float)c[q];
+}
+
 // Coarse cell field (ghosts included, one thread per coarse cell): the 2x2
 // average of the fine interior cells under it; a ghost takes the value of
 // the nearest interior cell.
-__global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const double* __restrict__ rf,
-                                                       const double* __restrict__ ef,
-                                                       const double* __restrict__ cf, double* __restrict__ rc,
-                                                       double* __restrict__ ec, double* __restrict__ cc) {
+template <class T, class TO>
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const T* __restrict__ rf,
+                                                       const T* __restrict__ ef, const T* __restrict__ cf,
+                                                       TO* __restrict__ rc, TO* __restrict__ ec,
+                                                       TO* __restrict__ cc) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= gc.nc()) return;
   const int ld = gc.nx + 2;
@@ -778,55 +878,54 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const d
   const int kc = k < 0 ? 0 : (k > gc.ny - 1 ? gc.ny - 1 : k);
   const int64_t a = gf.ci(2 * ic, 2 * kc), b = gf.ci(2 * ic, 2 * kc + 1);
   const int64_t c = gf.ci(2 * ic + 1, 2 * kc), d = gf.ci(2 * ic + 1, 2 * kc + 1);
-  double v;
-  v = rf[a] + rf[b];
-  v = v + rf[c];
-  v = v + rf[d];
-  rc[q] = 0.25 * v;
-  v = ef[a] + ef[b];
-  v = v + ef[c];
-  v = v + ef[d];
-  ec[q] = 0.25 * v;
-  v = cf[a] + cf[b];
-  v = v + cf[c];
-  v = v + cf[d];
-  cc[q] = 0.25 * v;
+  auto avg = [&](const T* f) {
+    double v = (double)f[a] + (double)f[b];
+    v = v + (double)f[c];
+    v = v + (double)f[d];
+    return (TO)(0.25 * v);
+  };
+  rc[q] = avg(rf);
+  ec[q] = avg(ef);
+  cc[q] = avg(cf);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const double* __restrict__ rho,
-                                                    const double* __restrict__ eta,
-                                                    const double* __restrict__ chi, double* __restrict__ d) {
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __restrict__ rho,
+                                                    const T* __restrict__ eta, const T* __restrict__ chi,
+                                                    T* __restrict__ d) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (q < g.nuv()) d[q] = velo_diag2(g, s, rho, eta, chi, q);
+  if (q < g.nuv()) d[q] = (T)velo_diag2(g, s, rho, eta, chi, q);
 }
 
 // Damped Jacobi sweep xo = x + omega (b - A x) / d; FIRST: from x = 0,
-// xo = omega b / d.  RES: r = b - A x instead.
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const double* __restrict__ rho,
-                                                      const double* __restrict__ eta,
-                                                      const double* __restrict__ chi, const double* __restrict__ x,
-                                                      const double* __restrict__ b, const double* __restrict__ d,
-                                                      double* __restrict__ out, double omega, const KState* st) {
+// xo = omega b / d.  RES: r = b - A x instead.  The level's coefficients, x
+// and d are stored as T; b as TB, the output as TO.
+template <int KIND, class T, class TB, class TO>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* __restrict__ rho,
+                                                      const T* __restrict__ eta, const T* __restrict__ chi,
+                                                      const T* __restrict__ x, const TB* __restrict__ b,
+                                                      const T* __restrict__ d, TO* __restrict__ out,
+                                                      double omega, const KState* st) {
   if (st && (st->done || st->early)) return;
   const uint32_t q = (uint32_t)blockIdx.x * kBlock + threadIdx.x;
   const uint32_t n = (uint32_t)g.nuv();
   if (q >= n) return;
   if (KIND == MG_FIRST) {
-    out[q] = omega * b[q] / d[q];
+    out[q] = (TO)(omega * (double)b[q] / (double)d[q]);
     return;
   }
-  const VeloIn X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+  const VeloInT<T> X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
   const double z = velo_row<2>(g, s, rho, eta, chi, X, q);
   if (KIND == MG_RES)
-    out[q] = b[q] - z;
+    out[q] = (TO)((double)b[q] - z);
   else
-    out[q] = x[q] + omega * (b[q] - z) / d[q];
+    out[q] = (TO)((double)x[q] + omega * ((double)b[q] - z) / (double)d[q]);
 }
 
 // bc = R r = P^T r / 4 (one thread per coarse unknown)
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const double* __restrict__ r,
-                                                        double* __restrict__ bc, const KState* st) {
+template <class TR, class TB>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const TR* __restrict__ r,
+                                                        TB* __restrict__ bc, const KState* st) {
   if (st && (st->done || st->early)) return;
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= gc.nuv()) return;
@@ -839,7 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
       if (wy == 0.0) continue;
       for (int i = 2 * I - 1; i <= 2 * I + 1; ++i) {
         if (i < 0 || i > gf.nx) continue;
-        acc += mg_w_face(i, I) * wy * r[i + (int64_t)(gf.nx + 1) * k];
+        acc += mg_w_face(i, I) * wy * (double)r[i + (int64_t)(gf.nx + 1) * k];
       }
     }
   } else {  // v (I, K), K = 1..ny-1: fine inner faces 2K-1..2K+1, cells 2I-1..2I+2
@@ -849,16 +948,17 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
       const double wy = mg_w_face(k, K);
       for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
         if (i < 0 || i > gf.nx - 1) continue;
-        acc += mg_w_cell_even(i, I, gc.nx) * wy * r[gf.mn() + i + (int64_t)gf.nx * (k - 1)];
+        acc += mg_w_cell_even(i, I, gc.nx) * wy * (double)r[gf.mn() + i + (int64_t)gf.nx * (k - 1)];
       }
     }
   }
-  bc[q] = 0.25 * acc;
+  bc[q] = (TB)(0.25 * acc);
 }
 
 // x += P xc (one thread per fine unknown)
-__global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const double* __restrict__ xc,
-                                                           double* __restrict__ x, const KState* st) {
+template <class TC, class TF>
+__global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const TC* __restrict__ xc,
+                                                           TF* __restrict__ x, const KState* st) {
   if (st && (st->done || st->early)) return;
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= gf.nuv()) return;
@@ -870,7 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, con
     for (int K = K0 - 1; K <= K0 + 1; ++K) {
       if (K < 0 || K > gc.ny - 1 || (K != K0 && K != K1)) continue;
       const double wy = mg_w_cell_odd(k, K, gc.ny);
-      for (int I = I0; I <= I1; ++I) acc += mg_w_face(i, I) * wy * xc[I + (int64_t)(gc.nx + 1) * K];
+      for (int I = I0; I <= I1; ++I) acc += mg_w_face(i, I) * wy * (double)xc[I + (int64_t)(gc.nx + 1) * K];
     }
   } else {  // v (i, k), k = 1..ny-1: coarse inner faces k/2 (and k/2 + 1), cells i/2 and its neighbour
     const int64_t t = q - gf.mn();
@@ -883,11 +983,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, con
       for (int I = Ic - 1; I <= Ic + 1; ++I) {
         if (I < 0 || I > gc.nx - 1) continue;
         const double wx = mg_w_cell_even(i, I, gc.nx);
-        if (wx != 0.0) acc += wx * wy * xc[gc.mn() + I + (int64_t)gc.nx * (K - 1)];
+        if (wx != 0.0) acc += wx * wy * (double)xc[gc.mn() + I + (int64_t)gc.nx * (K - 1)];
       }
     }
   }
-  x[q] = x[q] + acc;
+  x[q] = (TF)((double)x[q] + acc);
 }
 
 // ---- host launchers -----------------------------------------------------
@@ -1000,47 +1100,131 @@ void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const doub
 
 
 // ---- multigrid launchers ------------------------------------------------------
-void launch_mg_coarsen(hipStream_t st, const Grid& gf, const Grid& gc, const Coef& f, double* rc, double* ec,
-                       double* cc) {
-  hipLaunchKernelGGL(k_mg_coarsen, dim3(nblk(gc.nc(), kBlock)), dim3(kBlock), 0, st, gf, gc, f.rho, f.eta, f.chi,
-                     rc, ec, cc);
+void launch_mg_convert(hipStream_t st, const Grid& g, const Coef& c, float* rho, float* eta, float* chi) {
+  hipLaunchKernelGGL(k_mg_convert, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g.nc(), c.rho, c.eta, c.chi,
+                     rho, eta, chi);
 }
-void launch_mg_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* d) {
-  hipLaunchKernelGGL(k_mg_diag, dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, d);
+void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const Grid& gc, const MgCoef& f,
+                       void* rc, void* ec, void* cc) {
+  const dim3 grid(nblk(gc.nc(), kBlock));
+#define MGC(T, TO)                                                                                     \
+  hipLaunchKernelGGL((k_mg_coarsen<T, TO>), grid, dim3(kBlock), 0, st, gf, gc, (const T*)f.rho,           \
+                     (const T*)f.eta, (const T*)f.chi, (TO*)rc, (TO*)ec, (TO*)cc)
+  if (f32 && o32)
+    MGC(float, float);
+  else if (f32)  // into an FP64 coarsest level
+    MGC(float, double);
+  else
+    MGC(double, double);
+#undef MGC
 }
-void launch_mg_smooth(hipStream_t st, int kind, const Grid& g, const Scal& s, const Coef& c, const double* x,
-                      const double* b, const double* d, double* out, double omega, const KState* ks) {
+void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* d) {
   const dim3 grid(nblk(g.nuv(), kBlock));
-#define MGS(K) \
-  hipLaunchKernelGGL((k_mg_smooth<K>), grid, dim3(kBlock), 0, st, g, s, c.rho, c.eta, c.chi, x, b, d, out, omega, ks)
-  switch (kind) {
-    case MG_FIRST: MGS(MG_FIRST); break;
-    case MG_RES: MGS(MG_RES); break;
-    default: MGS(MG_SWEEP); break;
-  }
-#undef MGS
+  if (f32)
+    hipLaunchKernelGGL(k_mg_diag<float>, grid, dim3(kBlock), 0, st, g, s, (const float*)c.rho, (const float*)c.eta,
+                       (const float*)c.chi, (float*)d);
+  else
+    hipLaunchKernelGGL(k_mg_diag<double>, grid, dim3(kBlock), 0, st, g, s, (const double*)c.rho,
+                       (const double*)c.eta, (const double*)c.chi, (double*)d);
 }
-void launch_mg_restrict(hipStream_t st, const Grid& gf, const Grid& gc, const double* r, double* bc,
+template <int K, class T, class TB, class TO>
+static void mg_smooth_t(hipStream_t st, const Grid& g, const Scal& s, const MgCoef& c, const void* x,
+                        const void* b, const void* d, void* out, double omega, const KState* ks) {
+  hipLaunchKernelGGL((k_mg_smooth<K, T, TB, TO>), dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s,
+                     (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)d,
+                     (TO*)out, omega, ks);
+}
+template <int K>
+static void mg_smooth_k(hipStream_t st, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
+                        const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
                         const KState* ks) {
-  hipLaunchKernelGGL(k_mg_restrict, dim3(nblk(gc.nuv(), kBlock)), dim3(kBlock), 0, st, gf, gc, r, bc, ks);
+  if (!f32)  // an FP64 hierarchy is FP64 throughout
+    mg_smooth_t<K, double, double, double>(st, g, s, c, x, b, d, out, omega, ks);
+  else if (b32)
+    mg_smooth_t<K, float, float, float>(st, g, s, c, x, b, d, out, omega, ks);
+  else if (o32)  // level 0 of an FP32 hierarchy: FP64 b
+    mg_smooth_t<K, float, double, float>(st, g, s, c, x, b, d, out, omega, ks);
+  else  // its last sweep: FP64 output
+    mg_smooth_t<K, float, double, double>(st, g, s, c, x, b, d, out, omega, ks);
 }
-void launch_mg_prolong_add(hipStream_t st, const Grid& gc, const Grid& gf, const double* xc, double* x,
-                           const KState* ks) {
-  hipLaunchKernelGGL(k_mg_prolong_add, dim3(nblk(gf.nuv(), kBlock)), dim3(kBlock), 0, st, gc, gf, xc, x, ks);
+void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
+                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
+                      const KState* ks) {
+  switch (kind) {
+    case MG_FIRST: mg_smooth_k<MG_FIRST>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
+    case MG_RES: mg_smooth_k<MG_RES>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
+    default: mg_smooth_k<MG_SWEEP>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
+  }
 }
-
+void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
+                        void* bc, const KState* ks) {
+  const dim3 grid(nblk(gc.nuv(), kBlock));
+  if (r32 && b32)
+    hipLaunchKernelGGL((k_mg_restrict<float, float>), grid, dim3(kBlock), 0, st, gf, gc, (const float*)r,
+                       (float*)bc, ks);
+  else if (r32)  // into an FP64 coarsest level
+    hipLaunchKernelGGL((k_mg_restrict<float, double>), grid, dim3(kBlock), 0, st, gf, gc, (const float*)r,
+                       (double*)bc, ks);
+  else
+    hipLaunchKernelGGL((k_mg_restrict<double, double>), grid, dim3(kBlock), 0, st, gf, gc, (const double*)r,
+                       (double*)bc, ks);
+}
+void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,
+                           void* x, const KState* ks) {
+  const dim3 grid(nblk(gf.nuv(), kBlock));
+  if (c32 && f32)
+    hipLaunchKernelGGL((k_mg_prolong_add<float, float>), grid, dim3(kBlock), 0, st, gc, gf, (const float*)xc,
+                       (float*)x, ks);
+  else if (f32)  // from an FP64 coarsest level
+    hipLaunchKernelGGL((k_mg_prolong_add<double, float>), grid, dim3(kBlock), 0, st, gc, gf, (const double*)xc,
+                       (float*)x, ks);
+  else
+    hipLaunchKernelGGL((k_mg_prolong_add<double, double>), grid, dim3(kBlock), 0, st, gc, gf, (const double*)xc,
+                       (double*)x, ks);
+}
 
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
-// kRbgsSweeps full sweeps (red cells, then black), from x = 0
+// kRbgsSweeps full sweeps (red cells, then black), from x = 0, alternating
+// between work and x so that the last writes x.  FP64: work holds 2 ncell
+// doubles.  FP32 storage (c32 = FP32 bu, M_inner, M_0): work holds 4 ncell
+// floats, two FP32 vectors.
 constexpr int kRbgsSweeps = 4;
-void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const double* b, double* x,
-                      const KState* ks) {
-  const dim3 grid(nblk((int64_t)(g.nx / 2) * g.ny, kBlock), 1, bat_n());
-  hipLaunchKernelGGL((k_phmu_rbgs<true>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, b, x, 0, ks,
-                     bat_bs());
-  for (int h = 1; h < 2 * kRbgsSweeps; ++h)
-    hipLaunchKernelGGL((k_phmu_rbgs<false>), grid, dim3(kBlock), 0, st, g, s, c.bu, c.minner, c.m0, b, x, h & 1,
-                       ks, bat_bs());
+static_assert(kRbgsSweeps % 2 == 0, "the last sweep must write x");
+void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
+                      const double* b, double* x, void* work, const KState* ks) {
+  const dim3 grid((unsigned)((g.nx + kRbTX - 1) / kRbTX), (unsigned)((g.ny + kRbTY - 1) / kRbTY), bat_n());
+#define RB(F, TC, TI, TO, CO, XI, XO)                                                                        \
+  hipLaunchKernelGGL((k_phmu_rb_sweep<F, TC, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu, CO->minner, \
+                     CO->m0, b, (const TI*)(XI), (TO*)(XO), ks, bat_bs())
+  if (!c32) {
+    double* w = (double*)work;
+    const Coef* co = &c;
+    RB(true, double, double, double, co, nullptr, w);
+    for (int k = 1; k < kRbgsSweeps; ++k) {
+      if (k & 1)
+        RB(false, double, double, double, co, w, x);
+      else
+        RB(false, double, double, double, co, x, w);
+    }
+  } else {
+    float* wa = (float*)work;
+    float* wb = wa + 2 * g.ncell();
+    RB(true, float, float, float, c32, nullptr, wa);
+    for (int k = 1; k + 1 < kRbgsSweeps; ++k) {
+      if (k & 1)
+        RB(false, float, float, float, c32, wa, wb);
+      else
+        RB(false, float, float, float, c32, wb, wa);
+    }
+    RB(false, float, float, double, c32, wa, x);  // kRbgsSweeps - 2 is even: its sweep wrote wa
+  }
+#undef RB
+}
+
+void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32) {
+  hipLaunchKernelGGL(k_phmu_coef32, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g.nc(), c.bu,
+                     c.minner, c.m0, const_cast<float*>(c32.bu), const_cast<float*>(c32.minner),
+                     const_cast<float*>(c32.m0), bat_bs());
 }
 
 }  // namespace drd

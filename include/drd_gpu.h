@@ -50,7 +50,7 @@ extern "C" {
                                 counterpart; power-of-two nx, ny)         */
 #define DRD_PREC_DCT64 4     /* the same in FP64                          */
 /* DRD_PREC_LEARNED (5): FP32 DCT + learned correction, defined below      */
-/* DRD_PREC_MG (6): geometric multigrid V-cycle, defined below              */
+/* DRD_PREC_MG (6), DRD_PREC_MG32 (7): geometric multigrid V-cycle, below */
 
 /* ---- preconditioner choices for the Cahn-Hilliard phi/mu solve -------- */
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
@@ -58,7 +58,10 @@ extern "C" {
                                    reference counterpart)                   */
 #define DRD_CH_PREC_RBGS 2      /* four red-black Gauss-Seidel sweeps of the
                                    same 2x2 blocks (no reference
-                                   counterpart; even nx)                    */
+                                   counterpart)                             */
+#define DRD_CH_PREC_RBGS32 3    /* the same sweeps with the mobilities and the
+                                   iterates between sweeps stored in FP32
+                                   (arithmetic, b and the result FP64)      */
 
 /* ---- outer Krylov method ------------------------------------------------ */
 #define DRD_KRYLOV_BICGSTAB 0 /* LIB/BiCGSTAB.f, reference stopping rule    */
@@ -212,11 +215,14 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 #define DRD_PS_VELO_ESOLP 7   /* PSolve_uv_CG PD/VeloSolve_combined.f90:697-807
                                  (ESOLP_x TP/ESOLP.f90 on both blocks)            */
 /* DRD_PS_VELO_LEARNED (8): FP32 DCT + learned correction, defined below   */
-/* DRD_PS_VELO_MG (9): geometric multigrid V-cycle, defined below           */
+/* DRD_PS_VELO_MG (9), DRD_PS_VELO_MG32 (11): multigrid V-cycle, below     */
 #define DRD_PS_PHMU_RBGS 10   /* red-black block Gauss-Seidel for the phi/mu system
                                  (addition): x = 0; then 4 times: the red cells,
                                  then the black cells, x_c += B_c^-1 (b - A x)_c
                                  with B_c the exact 2x2 block of cell c         */
+#define DRD_PS_PHMU_RBGS32 12 /* DRD_PS_PHMU_RBGS with FP32 storage of bu,
+                                 M_inner, M_0 and of the iterates between
+                                 sweeps (addition)                             */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 
@@ -375,9 +381,15 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  * counts stay nearly constant as the grid is refined, unlike the fast-Poisson
  * preconditioner's.  velo_prec = DRD_PREC_MG makes drd_step solve the BDF2
  * velocity system with it (the first, BDF1 step keeps Jacobi); batches do not
- * take it.                                                                   */
+ * take it.
+ * DRD_PS_VELO_MG32 / DRD_PREC_MG32: the same cycle with every level but the
+ * coarsest stored in FP32 (coefficients, diagonals, vectors; level 0 holds an
+ * FP32 copy of rho, eta, chi), which halves the bytes of each sweep.  The
+ * arithmetic, b, z and the coarsest level stay FP64.                         */
 #define DRD_PREC_MG 6
 #define DRD_PS_VELO_MG 9
+#define DRD_PREC_MG32 7
+#define DRD_PS_VELO_MG32 11
 
 #define DRD_LEARN_OFF 0      /* not enabled: the plain method               */
 #define DRD_LEARN_JUDGE 1    /* the next velocity solve is judged           */

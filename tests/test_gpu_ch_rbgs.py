@@ -9,6 +9,12 @@
   half the iterations of the block-Jacobi preconditioner.
 * Time steps with it and the multigrid velocity preconditioner track the
   oracle's steps (block Jacobi CH, reference velocity preconditioners).
+* 1024x512 holds more tiles than the GPU runs at once, so some sweep
+  workgroups start after their neighbours have finished: the sweeps must
+  read one buffer and write another for the result to stay exact.
+* DRD_PS_PHMU_RBGS32 (FP32 storage of the mobilities and of the iterates
+  between sweeps, FP64 arithmetic) equals the FP64 form to a relative 1e-5,
+  converges as fast inside BiCGSTAB, and its steps track the oracle's.
 """
 import numpy as np
 import pytest
@@ -46,7 +52,7 @@ def _case(nx, ny):
     return cfg
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 32), (130, 70)])
+@pytest.mark.parametrize("nx,ny", [(64, 32), (130, 70), (67, 35), (1024, 512)])
 def test_rbgs_matches_reference(nx, ny):
     from drd_application_amd.solver import GpuSolver
     from oracle.oracle import Oracle
@@ -62,6 +68,10 @@ def test_rbgs_matches_reference(nx, ny):
     want = rbgs_reference(o, cfg.params, b)
     got = g.psolve(abi.PS_PHMU_RBGS, abi.OP_PHMU, b)
     np.testing.assert_array_equal(got, want)
+    got32 = g.psolve(abi.PS_PHMU_RBGS32, abi.OP_PHMU, b)
+    e32 = rel_l2(got32, want)
+    print(f"FP32 storage: rel {e32:.2e}", flush=True)
+    assert 1e-12 < e32 < 1e-5  # rounded, and only by storage
     g.close()
 
 
@@ -84,19 +94,25 @@ def test_bicgstab_with_rbgs_converges():
     res = rhs - o.matvec(abi.OP_PHMU, 1.0, xr, 0.0, np.zeros(n))
     assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(rhs)
     assert 3 * itr < itb
+    xs, its, rs, is_ = g.bicgstab(abi.OP_PHMU, abi.PS_PHMU_RBGS32, rhs, np.zeros(n), 3000, 1e-10)
+    print(f"red-black Gauss-Seidel with FP32 storage {its} its", flush=True)
+    assert is_ == 0 and its <= itr + 2
+    res = rhs - o.matvec(abi.OP_PHMU, 1.0, xs, 0.0, np.zeros(n))
+    assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(rhs)
     g.close()
 
 
-def test_rbgs_and_multigrid_steps_track_oracle_steps():
+@pytest.mark.parametrize("velo,ch", [("PREC_MG", "CH_PREC_RBGS"), ("PREC_MG32", "CH_PREC_RBGS32")])
+def test_rbgs_and_multigrid_steps_track_oracle_steps(velo, ch):
     from drd_application_amd.solver import GpuSolver
     from oracle.oracle import Oracle
-    print("rbgs + mg steps 128x128", flush=True)
+    print(f"rbgs + mg steps 128x128 {velo} {ch}", flush=True)
     cfg = geometry.tp_case(128, 128, lambda_eta=10.0, tol=1e-10)
     p = cfg.params
     p.ch_prec = abi.CH_PREC_BLOCK
     q = p.copy()
-    q.velo_prec = abi.PREC_MG
-    q.ch_prec = abi.CH_PREC_RBGS
+    q.velo_prec = getattr(abi, velo)
+    q.ch_prec = getattr(abi, ch)
     g = GpuSolver(q)
     geometry.load(g, cfg)
     o = Oracle(p)

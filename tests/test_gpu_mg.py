@@ -10,6 +10,12 @@
   solution as that solve.
 * Time steps with velo_prec = DRD_PREC_MG track the oracle's steps (reference
   preconditioners) as the other fast paths do.
+* The FP32-storage cycle (DRD_PS_VELO_MG32): one V-cycle equals the FP64
+  construction to a relative 2e-5 (each stored value rounds to 6e-8
+  relative; a cycle reads and writes each level's vectors a few times and
+  damps, not amplifies, what it reads), BiCGSTAB with it converges to the
+  same 1e-10 in at most two more iterations than with the FP64 cycle, and
+  its steps track the oracle's as closely.
 """
 import numpy as np
 import pytest
@@ -45,6 +51,10 @@ def test_vcycle_matches_reference(nx, ny):
     want = mg_ref.vcycle(lv, b)
     got = g.psolve(abi.PS_VELO_MG, abi.OP_VELO2, b)
     assert rel_l2(got, want) < 1e-10
+    got32 = g.psolve(abi.PS_VELO_MG32, abi.OP_VELO2, b)
+    e32 = rel_l2(got32, want)
+    print(f"FP32 storage: rel {e32:.2e}", flush=True)
+    assert 1e-12 < e32 < 2e-5  # rounded, and only by storage
     g.close()
 
 
@@ -71,21 +81,28 @@ def test_bicgstab_with_multigrid_converges_in_few_iterations():
     em, ed = rel_l2(xm, x_true), rel_l2(xd, x_true)
     print(f"errors against the exact solution: multigrid {em:.2e}, fast-Poisson {ed:.2e}", flush=True)
     assert em < 1e-7 and ed < 1e-7
+    xs, its, rs, is_ = g.bicgstab(abi.OP_VELO2, abi.PS_VELO_MG32, b, np.zeros(n), 2000, 1e-10)
+    print(f"multigrid with FP32 storage {its} its", flush=True)
+    assert is_ == 0 and its <= itm + 2
+    r = b - o.matvec(abi.OP_VELO2, 1.0, xs, 0.0, np.zeros(n))
+    assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b)
+    assert rel_l2(xs, x_true) < 1e-7
     g.close()
 
 
-def test_multigrid_steps_track_oracle_steps():
+@pytest.mark.parametrize("prec", ["PREC_MG", "PREC_MG32"])
+def test_multigrid_steps_track_oracle_steps(prec):
     """Four steps with the multigrid velocity preconditioner and the block
     CH preconditioner against four oracle steps (reference velocity
     preconditioners, block CH), every solve converged to 1e-10."""
     from drd_application_amd.solver import GpuSolver
     from oracle.oracle import Oracle
-    print("mg steps 128x128", flush=True)
+    print(f"mg steps 128x128 {prec}", flush=True)
     cfg = geometry.tp_case(128, 128, lambda_eta=10.0, tol=1e-10)
     p = cfg.params
     p.ch_prec = abi.CH_PREC_BLOCK
     q = p.copy()
-    q.velo_prec = abi.PREC_MG
+    q.velo_prec = getattr(abi, prec)
     gm = GpuSolver(q)
     geometry.load(gm, cfg)
     o = Oracle(p)
