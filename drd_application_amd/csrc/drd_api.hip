@@ -400,7 +400,7 @@ static int ctx_alloc(drd_ctx* c) {
   add(&c->vp_inv64, g.nuv(), 8);
   add(&c->vp_inv32, g.nuv(), 4);
   add(&c->vp_work, 2 * g.nuv(), 8);
-  add(&c->rb_work, 2 * g.ncell(), 8);
+  add(&c->rb_work, 3 * g.ncell(), 8);  // 2 ncell doubles, or 6 ncell floats
   add(&c->rb_c32, 3 * g.nc(), 4);
   add(&c->cfB, p.nx + 1, 8);
   add(&c->cfT, p.nx + 1, 8);

@@ -204,7 +204,7 @@ struct PhmuCoef32 {
 };
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32);
 // c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
-// work = 4 ncell floats
+// work = 6 ncell floats
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks);
 

@@ -704,15 +704,17 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 // launch gives bit for bit the two colour half-sweeps.  xo must not alias
 // xi: a workgroup that starts after its neighbour has written would read
 // that neighbour's new values in its halo.  FIRST: from x = 0 (xi unread).
-// The coefficients are stored as TC, xi as TI and xo as TO (FP64, or FP32
-// storage with the arithmetic in FP64); b is FP64.
+// The coefficients are stored as TC, b as TB, xi as TI and xo as TO: FP64,
+// or FP32 storage with the arithmetic in FP64, where the coefficients and b
+// are also held in LDS as FP32 (more workgroups per CU) and bc, if given,
+// receives the FP32 b of the tile for the later sweeps.
 constexpr int kRbTX = 64, kRbTY = 8, kRbH = 2;
 constexpr int kRbW = kRbTX + 2 * kRbH, kRbHt = kRbTY + 2 * kRbH;
-template <bool FIRST, class TC, class TI, class TO>
+template <bool FIRST, class TC, class TB, class TI, class TO>
 __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
                                                           const TC* __restrict__ minner,
-                                                          const TC* __restrict__ m0,
-                                                          const double* __restrict__ b, const TI* __restrict__ xi,
+                                                          const TC* __restrict__ m0, const TB* __restrict__ b,
+                                                          float* __restrict__ bc, const TI* __restrict__ xi,
                                                           TO* __restrict__ xo, const KState* st, int64_t bs) {
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
@@ -720,10 +722,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   minner = bofs(minner, bs);
   m0 = bofs(m0, bs);
   b = bofs(b, bs);
+  bc = bofs(bc, bs);
   xi = bofs(xi, bs);
   xo = bofs(xo, bs);
-  __shared__ double sx1[kRbHt][kRbW], sx2[kRbHt][kRbW], smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW];
-  __shared__ double sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
+  __shared__ double sx1[kRbHt][kRbW], sx2[kRbHt][kRbW];
+  __shared__ TC smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW], sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
   const int i0 = (int)blockIdx.x * kRbTX, j0 = (int)blockIdx.y * kRbTY;
@@ -741,8 +744,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     v[it][2] = (double)minner[c];
     v[it][3] = (double)m0[c];
     v[it][4] = (double)bu[c];
-    v[it][5] = b[q];
-    v[it][6] = b[C + q];
+    v[it][5] = (double)b[q];
+    v[it][6] = (double)b[C + q];
   }
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
@@ -752,11 +755,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
     sx1[lj][li] = v[it][0];
     sx2[lj][li] = v[it][1];
-    smi[lj][li] = v[it][2];
-    smz[lj][li] = v[it][3];
-    sbu[lj][li] = v[it][4];
-    sb1[lj][li] = v[it][5];
-    sb2[lj][li] = v[it][6];
+    smi[lj][li] = (TC)v[it][2];
+    smz[lj][li] = (TC)v[it][3];
+    sbu[lj][li] = (TC)v[it][4];
+    sb1[lj][li] = (TC)v[it][5];
+    sb2[lj][li] = (TC)v[it][6];
   }
   __syncthreads();
   auto update = [&](int i, int j) {
@@ -800,6 +803,10 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     const int64_t q = i + (int64_t)nx * j;
     xo[q] = (TO)sx1[j - j0 + kRbH][i - i0 + kRbH];
     xo[C + q] = (TO)sx2[j - j0 + kRbH][i - i0 + kRbH];
+    if (bc) {
+      bc[q] = (float)sb1[j - j0 + kRbH][i - i0 + kRbH];
+      bc[C + q] = (float)sb2[j - j0 + kRbH][i - i0 + kRbH];
+    }
   }
 }
 
@@ -1186,37 +1193,39 @@ void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, c
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
 // kRbgsSweeps full sweeps (red cells, then black), from x = 0, alternating
 // between work and x so that the last writes x.  FP64: work holds 2 ncell
-// doubles.  FP32 storage (c32 = FP32 bu, M_inner, M_0): work holds 4 ncell
-// floats, two FP32 vectors.
-constexpr int kRbgsSweeps = 4;
+// doubles.  FP32 storage (c32 = FP32 bu, M_inner, M_0): work holds 6 ncell
+// floats, two FP32 iterates and the FP32 b the first sweep writes.
+constexpr int kRbgsSweeps = 8;
 static_assert(kRbgsSweeps % 2 == 0, "the last sweep must write x");
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks) {
   const dim3 grid((unsigned)((g.nx + kRbTX - 1) / kRbTX), (unsigned)((g.ny + kRbTY - 1) / kRbTY), bat_n());
-#define RB(F, TC, TI, TO, CO, XI, XO)                                                                        \
-  hipLaunchKernelGGL((k_phmu_rb_sweep<F, TC, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu, CO->minner, \
-                     CO->m0, b, (const TI*)(XI), (TO*)(XO), ks, bat_bs())
+#define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                              \
+  hipLaunchKernelGGL((k_phmu_rb_sweep<F, TC, TB, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu, CO->minner, \
+                     CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
   if (!c32) {
     double* w = (double*)work;
     const Coef* co = &c;
-    RB(true, double, double, double, co, nullptr, w);
+    RB(true, double, double, double, double, co, b, nullptr, nullptr, w);
     for (int k = 1; k < kRbgsSweeps; ++k) {
       if (k & 1)
-        RB(false, double, double, double, co, w, x);
+        RB(false, double, double, double, double, co, b, nullptr, w, x);
       else
-        RB(false, double, double, double, co, x, w);
+        RB(false, double, double, double, double, co, b, nullptr, x, w);
     }
   } else {
+    const int64_t n2 = 2 * g.ncell();
     float* wa = (float*)work;
-    float* wb = wa + 2 * g.ncell();
-    RB(true, float, float, float, c32, nullptr, wa);
+    float* wb = wa + n2;
+    float* bf = wb + n2;
+    RB(true, float, double, float, float, c32, b, bf, nullptr, wa);
     for (int k = 1; k + 1 < kRbgsSweeps; ++k) {
       if (k & 1)
-        RB(false, float, float, float, c32, wa, wb);
+        RB(false, float, float, float, float, c32, bf, nullptr, wa, wb);
       else
-        RB(false, float, float, float, c32, wb, wa);
+        RB(false, float, float, float, float, c32, bf, nullptr, wb, wa);
     }
-    RB(false, float, float, double, c32, wa, x);  // kRbgsSweeps - 2 is even: its sweep wrote wa
+    RB(false, float, float, float, double, c32, bf, nullptr, wa, x);  // sweep kRbgsSweeps - 2 (even) wrote wa
   }
 #undef RB
 }

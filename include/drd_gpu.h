@@ -56,12 +56,14 @@ extern "C" {
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
 #define DRD_CH_PREC_BLOCK 1     /* exact 2x2 per-cell block Jacobi (no
                                    reference counterpart)                   */
-#define DRD_CH_PREC_RBGS 2      /* four red-black Gauss-Seidel sweeps of the
+#define DRD_CH_PREC_RBGS 2      /* eight red-black Gauss-Seidel sweeps of the
                                    same 2x2 blocks (no reference
                                    counterpart)                             */
-#define DRD_CH_PREC_RBGS32 3    /* the same sweeps with the mobilities and the
-                                   iterates between sweeps stored in FP32
-                                   (arithmetic, b and the result FP64)      */
+#define DRD_CH_PREC_RBGS32 3    /* the same sweeps with the mobilities, b and
+                                   the iterates between sweeps stored in
+                                   FP32; FP64 arithmetic, multiplying by
+                                   reciprocals where the FP64 form divides;
+                                   FP64 result                              */
 
 /* ---- outer Krylov method ------------------------------------------------ */
 #define DRD_KRYLOV_BICGSTAB 0 /* LIB/BiCGSTAB.f, reference stopping rule    */
@@ -217,12 +219,12 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
 /* DRD_PS_VELO_LEARNED (8): FP32 DCT + learned correction, defined below   */
 /* DRD_PS_VELO_MG (9), DRD_PS_VELO_MG32 (11): multigrid V-cycle, below     */
 #define DRD_PS_PHMU_RBGS 10   /* red-black block Gauss-Seidel for the phi/mu system
-                                 (addition): x = 0; then 4 times: the red cells,
+                                 (addition): x = 0; then 8 times: the red cells,
                                  then the black cells, x_c += B_c^-1 (b - A x)_c
                                  with B_c the exact 2x2 block of cell c         */
 #define DRD_PS_PHMU_RBGS32 12 /* DRD_PS_PHMU_RBGS with FP32 storage of bu,
-                                 M_inner, M_0 and of the iterates between
-                                 sweeps (addition)                             */
+                                 M_inner, M_0, b and the iterates between
+                                 sweeps, reciprocals for divisions (addition) */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 

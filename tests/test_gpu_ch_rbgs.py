@@ -12,8 +12,9 @@
 * 1024x512 holds more tiles than the GPU runs at once, so some sweep
   workgroups start after their neighbours have finished: the sweeps must
   read one buffer and write another for the result to stay exact.
-* DRD_PS_PHMU_RBGS32 (FP32 storage of the mobilities and of the iterates
-  between sweeps, FP64 arithmetic) equals the FP64 form to a relative 1e-5,
+* DRD_PS_PHMU_RBGS32 (FP32 storage of the mobilities, b and the iterates
+  between sweeps; FP64 arithmetic with reciprocals for the divisions)
+  equals the FP64 form to a relative 1e-6,
   converges as fast inside BiCGSTAB, and its steps track the oracle's.
 """
 import numpy as np
@@ -23,7 +24,7 @@ from drd_application_amd import abi, geometry
 
 pytestmark = pytest.mark.gpu
 
-SWEEPS = 4
+SWEEPS = 8
 
 
 def rel_l2(a, b):
@@ -71,7 +72,7 @@ def test_rbgs_matches_reference(nx, ny):
     got32 = g.psolve(abi.PS_PHMU_RBGS32, abi.OP_PHMU, b)
     e32 = rel_l2(got32, want)
     print(f"FP32 storage: rel {e32:.2e}", flush=True)
-    assert 1e-12 < e32 < 1e-5  # rounded, and only by storage
+    assert 1e-12 < e32 < 1e-6  # rounded, and only by storage
     g.close()
 
 
