@@ -112,6 +112,10 @@ MODES = {
 }
 
 GUESS = os.environ.get("DRD_BENCH_GUESS", "1") != "0"
+# launches of the roofline kernel timed with HIP events inside the timed
+# steps (the first ones; an event pair per launch costs a few microseconds,
+# so timing every launch would slow the steps it measures)
+TIMED_LAUNCHES = 160
 LEARN = {"record_steps": 3, "capacity": 64, "crop": 256, "iters": 300, "batch": 4, "lr": 5e-4}
 
 
@@ -137,12 +141,13 @@ def learned_warmup(g, p, t):
     return t, info
 
 
-def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer: bool = False):
+def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer=None):
     """Times `steps` BDF2 steps after `warmup` steps (the first is BDF1), so
     every mode is timed on the same steps of the same run.  The learned mode
     spends the last record_steps + 1 warmup steps recording, training and
-    judging.  With timer, every launch of the BDF2 velocity matvec (MV_DOT1)
-    inside the timed steps is bracketed by HIP events on the library's stream."""
+    judging.  With timer (an instrumented kernel id), every launch of that
+    kernel inside the timed steps is bracketed by HIP events on the library's
+    stream."""
     import numpy as np
 
     from drd_application_amd import abi, geometry
@@ -171,8 +176,8 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
     if mode == "learned":
         t, learn = learned_warmup(g, p, t)
     g.synchronize()
-    if timer:
-        g.kernel_timer(abi.K_VELO_MATVEC, 1 << 14)
+    if timer is not None:
+        g.kernel_timer(timer, TIMED_LAUNCHES)
     iters_v, iters_c, infos = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -189,16 +194,14 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer:
            "warmup_steps": plain_warm + (LEARN["record_steps"] + 1 if learn is not None else 0),
            "velo_iters": sum(iters_v) / steps, "phmu_iters": sum(iters_c) / steps,
            "converged": all(a == 0 and b == 0 for a, b in infos)}
-    if timer:
+    if timer is not None:
         n, ms, by = g.kernel_timer_read()
         each = g.kernel_timer_launches()
-        g.kernel_timer(abi.K_VELO_MATVEC, 0)
-        # every BiCGSTAB iteration makes one working MV_DOT1 launch; launches
-        # queued past the end of a solve return at once and move no data, so
-        # the longest sum(iterations) launches are the working ones and the
-        # rest are counted apart, not averaged in
-        nwork = min(int(sum(iters_v)), int(each.size))
-        work = np.sort(each)[each.size - nwork:]
+        g.kernel_timer(timer, 0)
+        # launches queued past the end of a solve return at once and move no
+        # data: they take a small fraction of a working launch's time and are
+        # counted apart, not averaged in
+        work = each[each > 0.25 * np.median(each)] if each.size else each
         out["timed_launches"] = {"count": int(work.size), "total_ms": float(work.astype(np.float64).sum()),
                                  "bytes_per_launch": by, "idle_launches": int(n - work.size),
                                  "idle_ms": float(ms - work.astype(np.float64).sum())}
@@ -389,10 +392,20 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     }
 
 
-def measured_traffic(cfg_name: str):
-    """HBM bytes per launch of the velocity matvec from the committed
-    rocprofv3 counter summary for this workload, if there is one."""
-    path = os.path.join(ROOT, "profiles", f"k_velo_matvec_{cfg_name}_counters.md")
+# the kernel the roofline reports per mode: the one that takes the most time
+# in its steps (kernel id, name, prefix of its counter summary in profiles/)
+ROOF_SWEEP = ("K_PHMU_SWEEP32", "k_phmu_rb_sweep<FP32 middle sweep>", "k_phmu_rb_sweep32")
+ROOF_MATVEC = ("K_VELO_MATVEC", "k_velo_matvec<2,MV_DOT1>", "k_velo_matvec")
+
+
+def roof_kernel(mode: str):
+    return ROOF_SWEEP if MODES[mode][1] == "CH_PREC_RBGS32" else ROOF_MATVEC
+
+
+def measured_traffic(prefix: str, cfg_name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary for this workload, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"{prefix}_{cfg_name}_counters.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -403,9 +416,11 @@ def measured_traffic(cfg_name: str):
 
 
 def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
-    """The velocity matvec and a plain copy of the same length, each launched
-    `reps` times after one BDF1 step has filled the fields; under
-    rocprofv3 --pmc the copy's FETCH_SIZE/WRITE_SIZE calibrate the counters."""
+    """The velocity matvec, a plain copy of the same length, the mode's
+    velocity preconditioner and (FP32 red-black modes) one middle CH sweep,
+    each launched `reps` times after one BDF1 step has filled the fields;
+    under rocprofv3 --pmc the copy's FETCH_SIZE/WRITE_SIZE calibrate the
+    counters."""
     from drd_application_amd import abi
     from drd_application_amd.solver import GpuSolver
     from drd_application_amd import geometry
@@ -418,7 +433,8 @@ def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     g.step(1, 0.0)
     out = {}
     prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if mode.startswith("mg") else (abi.K_VELO_PREC, "velo_prec")
-    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), prec_k):
+    sweep = [(abi.K_PHMU_SWEEP32, "phmu_rb_sweep32")] if MODES[mode][1] == "CH_PREC_RBGS32" else []
+    for which, name in [(abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), prec_k] + sweep:
         ms, by = g.bench_kernel(which, reps)
         out[name] = {"us": ms * 1e3, "bytes": by, "launches": reps + 1}
         log(f"{name}: {ms*1e3:.1f} us, {by/1e6:.1f} MB algorithmic")
@@ -451,8 +467,8 @@ def main(argv=None) -> int:
                     help="untimed steps (BDF1 first); the learned mode records, trains and judges in the last "
                          "4 of them")
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--mode", default="mg", choices=sorted(MODES))
-    ap.add_argument("--compare", default="learned,fp32,fp64dct,dropin",
+    ap.add_argument("--mode", default="mg32", choices=sorted(MODES))
+    ap.add_argument("--compare", default="mg,learned,fp32,fp64dct,dropin",
                     help="comma-separated modes timed beside the headline mode ('' for none)")
     ap.add_argument("--compare-steps", type=int, default=None,
                     help="timed steps of the comparison modes (default: --steps)")
@@ -479,26 +495,29 @@ def main(argv=None) -> int:
         return profile_kernels(cfg, args.mode)
     if args.profile_network:
         return profile_network()
-    head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=True)
+    rk_id, rk_name, rk_prefix = roof_kernel(args.mode)
+    rk = getattr(abi, rk_id)
+    head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=rk)
     tl = head.pop("timed_launches")
     source = "in-solve HIP events"
     if tl["count"] == 0:  # FGMRES launches the plain matvec: time it stand-alone
-        ms_k, bytes_k = g.bench_kernel(abi.K_VELO_MATVEC, 20)
+        ms_k, bytes_k = g.bench_kernel(rk, 20)
         achieved = bytes_k / (ms_k * 1e-3) / 1e9
-        source = "stand-alone loop (the timed steps launched no BiCGSTAB matvec)"
-        log(f"velocity matvec stand-alone: {ms_k*1e3:.1f} us -> {achieved:.0f} GB/s")
+        source = "stand-alone loop (the timed steps launched no instrumented kernel)"
+        log(f"{rk_name} stand-alone: {ms_k*1e3:.1f} us -> {achieved:.0f} GB/s")
     else:
         ms_k = tl["total_ms"] / tl["count"]
         bytes_k = tl["bytes_per_launch"]
         achieved = bytes_k / (ms_k * 1e-3) / 1e9
-        log(f"velocity matvec in the timed steps: {tl['count']} launches, {ms_k*1e3:.1f} us each, "
+        log(f"{rk_name} in the timed steps: {tl['count']} launches, {ms_k*1e3:.1f} us each, "
             f"{bytes_k/1e6:.1f} MB -> {achieved:.0f} GB/s ({tl['idle_launches']} launches after "
             f"convergence, {tl['idle_ms']*1e3:.0f} us in all, not counted)")
     # stand-alone timings of other kernels on the final state (same launch shapes)
     kern = {}
     prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if args.mode.startswith("mg") else (abi.K_VELO_PREC, "velo_prec")
-    for which, name in ((abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
-                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), prec_k):
+    sweep = [(abi.K_PHMU_SWEEP32, "phmu_rb_sweep32")] if rk_prefix == "k_phmu_rb_sweep32" else []
+    for which, name in [(abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
+                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_PRESSURE, "pressure"), prec_k] + sweep:
         ms, by = g.bench_kernel(which, 20)
         kern[name] = {"us": ms * 1e3, "GB/s": by / (ms * 1e-3) / 1e9}
         log(f"{name} (stand-alone): {ms*1e3:.1f} us -> {kern[name]['GB/s']:.0f} GB/s")
@@ -559,8 +578,8 @@ def main(argv=None) -> int:
                    "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
-                     "traffic": measured_traffic(cfg.name),
-                     "kernel": "k_velo_matvec<2,MV_DOT1>", "bytes_per_launch": bytes_k,
+                     "traffic": measured_traffic(rk_prefix, cfg.name),
+                     "kernel": rk_name, "bytes_per_launch": bytes_k,
                      "us_per_launch": ms_k * 1e3 if ms_k else None,
                      "launches_timed": tl["count"], "idle_launches": tl.get("idle_launches", 0),
                      "source": source},

@@ -63,6 +63,7 @@ PS_PHMU_RBGS32 = 12
 LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 
 K_VELO_MATVEC, K_PHMU_MATVEC, K_BICG_XR, K_PRESSURE, K_COPY, K_UNET_FWD, K_VELO_PREC, K_VELO_MG = 0, 1, 2, 3, 4, 5, 6, 7
+K_PHMU_SWEEP32 = 8
 
 F_U, F_V, F_P, F_PH, F_RMU, F_RHO, F_ETA = 0, 1, 2, 3, 4, 5, 6
 F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10

@@ -759,7 +759,9 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
     case DRD_PS_PHMU_RBGS32: {  // the FP32 coefficients are those of its last prec_setup
       if (op != DRD_OP_PHMU) break;
       const PhmuCoef32 c32 = phmu_c32(c);
-      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), &c32, b, x, c->rb_work, ks);
+      const KTimer tm{c->tm_ev.data(), (int)(c->tm_ev.size() / 2), &c->tm_n};
+      launch_phmu_rbgs(c->stream, c->g, c->s, coef_of(c), &c32, b, x, c->rb_work, ks,
+                       c->tm_which == DRD_K_PHMU_SWEEP32 ? &tm : nullptr);
       *res = x;
       return 0;
     }
@@ -1757,7 +1759,8 @@ extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32
 // ---- measurement hook ---------------------------------------------------------
 extern "C" int drd_kernel_timer(drd_ctx* c, int32_t which, int32_t capacity) {
   if (!c || capacity < 0) return set_err(DRD_E_ARG, "bad argument");
-  if (capacity > 0 && which != DRD_K_VELO_MATVEC) return set_err(DRD_E_UNSUPPORTED, "kernel not instrumented");
+  if (capacity > 0 && which != DRD_K_VELO_MATVEC && which != DRD_K_PHMU_SWEEP32)
+    return set_err(DRD_E_UNSUPPORTED, "kernel not instrumented");
   HIPCHK(hipStreamSynchronize(c->stream));
   for (auto e : c->tm_ev) HIPCHK(hipEventDestroy(e));
   c->tm_ev.clear();
@@ -1779,7 +1782,10 @@ extern "C" int drd_kernel_timer_read(drd_ctx* c, int32_t* count, double* total_m
   }
   *count = c->tm_n;
   *total_ms = tot;
-  *bytes = 8.0 * (3.0 * (double)c->g.nuv() + 3.0 * (double)c->g.ncell());  // x, y, rtld + rho, eta, chi
+  if (c->tm_which == DRD_K_PHMU_SWEEP32)  // x in and out, b: 2 floats per cell each; bu, M_inner, M_0: 1 each
+    *bytes = 4.0 * 9.0 * (double)c->g.ncell();
+  else  // x, y, rtld: Nuv doubles each; rho, eta, chi: C doubles each
+    *bytes = 8.0 * (3.0 * (double)c->g.nuv() + 3.0 * (double)c->g.ncell());
   return 0;
 }
 
@@ -1835,6 +1841,12 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       case DRD_K_VELO_MG:
         velo_mg_apply(c->stream, c->vmg, c->kw[2], c->kw[3], nullptr);
         break;
+      case DRD_K_PHMU_SWEEP32: {  // the first iterate -> the second, with the FP32 b
+        const float* wa = (const float*)c->rb_work;
+        launch_phmu_rb_sweep32(c->stream, g, c->s, phmu_c32(c), wa + 4 * g.ncell(), wa,
+                               (float*)c->rb_work + 2 * g.ncell(), c->ks);
+        break;
+      }
     }
   };
   switch (which) {
@@ -1866,6 +1878,13 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       RCHK(prec_setup(c, velo_mg_f32(c->vmg) ? DRD_PS_VELO_MG32 : DRD_PS_VELO_MG));
       *bytes = 2 * N8;
       break;
+    case DRD_K_PHMU_SWEEP32: {  // x in and out, b: 2 floats per cell each; bu, M_inner, M_0: 1 each
+      RCHK(prec_setup(c, DRD_PS_PHMU_RBGS32));
+      const double* res = nullptr;  // one application fills the iterates and the FP32 b
+      RCHK(apply_prec(c, DRD_PS_PHMU_RBGS32, DRD_OP_PHMU, c->kw[2], c->kw[3], &res, nullptr));
+      *bytes = 4.0 * 9.0 * (double)g.ncell();
+      break;
+    }
     default:
       return set_err(DRD_E_ARG, "unknown kernel id");
   }

@@ -135,9 +135,11 @@ __device__ inline double velo_diag2(const Grid& g, const Scal& s, const T* __res
 // clamped to the cell at a wall (the value the reference's one-sided rules
 // use there).  Loading the five first lets a thread issue all its loads
 // before any arithmetic; the *_v forms below take the values.
-struct Nb5 {
-  double c, xl, xr, yl, yr;
+template <class R>
+struct Nb5T {
+  R c, xl, xr, yl, yr;
 };
+using Nb5 = Nb5T<double>;
 __device__ __forceinline__ Nb5 nb5_cell(const Grid& g, const double* __restrict__ a, int i, int j) {
   const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
   const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
@@ -153,37 +155,38 @@ __device__ __forceinline__ Nb5 nb5_packed(const Grid& g, const double* x, int i,
 
 // diagonal coefficient of op_double_gradient_Neum at (i,j) (TP/matrix_tool.f90:178-219)
 // from the mobility at the cell and its neighbours; only the sums are selected.
-// FAST multiplies by the reciprocals instead of dividing (not the
-// reference's rounding; for the FP32-storage preconditioner only).
-template <bool FAST = false>
-__device__ __forceinline__ double dgn_diag_v(const Grid& g, const Scal& s, const Nb5& mo, int i, int j) {
-  const double tl = (mo.xl + mo.c) / 2.0, tr = (mo.c + mo.xr) / 2.0;
-  const double bl = (mo.yl + mo.c) / 2.0, br = (mo.c + mo.yr) / 2.0;
-  double dx = 0.0, dy = 0.0;
+// R = double is the reference's arithmetic; R = float (the FP32
+// preconditioner only) computes in FP32 and multiplies by reciprocals where
+// the reference divides.
+template <class R = double>
+__device__ __forceinline__ R dgn_diag_v(const Grid& g, const Scal& s, const Nb5T<R>& mo, int i, int j) {
+  const R tl = (mo.xl + mo.c) / R(2.0), tr = (mo.c + mo.xr) / R(2.0);
+  const R bl = (mo.yl + mo.c) / R(2.0), br = (mo.c + mo.yr) / R(2.0);
+  R dx = R(0.0), dy = R(0.0);
   dx = i > 0 ? dx + tl : dx;
   dx = i < g.nx - 1 ? dx + tr : dx;
   dy = j > 0 ? dy + bl : dy;
   dy = j < g.ny - 1 ? dy + br : dy;
-  if (FAST) return -(dx * s.idr2 + dy * s.idz2);
-  return -(dx / s.dr2 + dy / s.dz2);
+  if (sizeof(R) == sizeof(float)) return -(dx * R(s.idr2) + dy * R(s.idz2));
+  return -(dx / R(s.dr2) + dy / R(s.dz2));
 }
 
 // x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell, from
-// bu at the cell and the two mobilities around it
-template <bool FAST = false>
-__device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, double bv, const Nb5& mi, const Nb5& m0,
-                                             int i, int j, double r1, double r2, double& x1, double& x2) {
-  const double d11 = s.s_I - s.I * dgn_diag_v<FAST>(g, s, mi, i, j) +
-                     ((j == 0) ? s.I_dz2 * (1.0 - s.coef_bot) : 0.0) +
-                     ((j == g.ny - 1) ? s.I_dz2 * (1.0 - s.coef_top) : 0.0);
-  const double d12 = -bv;
-  const double d21 = bv + s.adj;
+// bu at the cell and the two mobilities around it (R as for dgn_diag_v)
+template <class R = double>
+__device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, R bv, const Nb5T<R>& mi,
+                                             const Nb5T<R>& m0, int i, int j, R r1, R r2, R& x1, R& x2) {
+  const R d11 = R(s.s_I) - R(s.I) * dgn_diag_v<R>(g, s, mi, i, j) +
+                ((j == 0) ? R(s.I_dz2) * (R(1.0) - R(s.coef_bot)) : R(0.0)) +
+                ((j == g.ny - 1) ? R(s.I_dz2) * (R(1.0) - R(s.coef_top)) : R(0.0));
+  const R d12 = -bv;
+  const R d21 = bv + R(s.adj);
   // mobility-free floor (s.floor22) keeps the block invertible where M_0 vanishes
-  double d22 = -s.dt * s.xld * dgn_diag_v<FAST>(g, s, m0, i, j);
-  d22 = d22 > s.floor22 ? d22 : s.floor22;
-  const double det = d11 * d22 - d12 * d21;
-  if (FAST) {
-    const double idet = 1.0 / det;
+  R d22 = -R(s.dt) * R(s.xld) * dgn_diag_v<R>(g, s, m0, i, j);
+  d22 = d22 > R(s.floor22) ? d22 : R(s.floor22);
+  const R det = d11 * d22 - d12 * d21;
+  if (sizeof(R) == sizeof(float)) {
+    const R idet = R(1.0) / det;
     x1 = (d22 * r1 - d12 * r2) * idet;
     x2 = (-d21 * r1 + d11 * r2) * idet;
   } else {
@@ -203,10 +206,20 @@ struct PhmuCoef32 {
   const float *bu, *minner, *m0;
 };
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32);
+// Event pairs for timing launches: launch k (k < cap) of the instrumented
+// kernel records ev[2k] and ev[2k+1] around its dispatch; *n counts them.
+struct KTimer {
+  hipEvent_t* ev;
+  int cap;
+  int* n;
+};
 // c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
-// work = 6 ncell floats
+// work = 6 ncell floats, and tm (optional) times the middle sweeps
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
-                      const double* b, double* x, void* work, const KState* ks);
+                      const double* b, double* x, void* work, const KState* ks, const KTimer* tm = nullptr);
+// one middle FP32-storage sweep, xi -> xo with the FP32 b (stand-alone timing)
+void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
+                            const float* xi, float* xo, const KState* ks);
 
 // ---- multigrid V-cycle for the BDF2 velocity operator (kernels_mg.hip) ----
 enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2 };

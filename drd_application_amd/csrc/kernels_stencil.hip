@@ -434,31 +434,34 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
 // for term, so every load is unconditional and a thread issues them all at
 // once (nx, ny >= 2, which drd_init requires).
 // idx2, idy2 = 1.0 / dx2, 1.0 / dy2, the factors the reference evaluates
-__device__ __forceinline__ double op_dgn_v(double idx2, double idy2, const Nb5& mo, const Nb5& x) {
-  double moL, moR, rhs = 0.0;
-  moL = (mo.xl + mo.c) / 2.0;
-  moR = (mo.c + mo.xr) / 2.0;
+template <class R = double>
+__device__ __forceinline__ R op_dgn_v(R idx2, R idy2, const Nb5T<R>& mo, const Nb5T<R>& x) {
+  R moL, moR, rhs = R(0.0);
+  moL = (mo.xl + mo.c) / R(2.0);
+  moR = (mo.c + mo.xr) / R(2.0);
   rhs = rhs + idx2 * (moL * x.xl + moR * x.xr - (moL + moR) * x.c);
-  moL = (mo.yl + mo.c) / 2.0;
-  moR = (mo.c + mo.yr) / 2.0;
+  moL = (mo.yl + mo.c) / R(2.0);
+  moR = (mo.c + mo.yr) / R(2.0);
   rhs = rhs + idy2 * (moL * x.yl + moR * x.yr - (moL + moR) * x.c);
   return rhs;
 }
 
 // MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:212-284 for cell (i, j) (both
 // rows) from the values around it: phi and mu (f1, f2) and the two mobilities
-// (FAST: s.s / s.I * f1 for s.s * f1 / s.I, not the reference's rounding;
-// for the FP32-storage preconditioner only)
-template <bool FAST = false>
-__device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, double b, const Nb5& mi,
-                                            const Nb5& m0, const Nb5& f1, const Nb5& f2, double& o1, double& o2) {
-  double ff1 = op_dgn_v(s.idr2, s.idz2, mi, f1);
-  ff1 = (FAST ? s.s_I * f1.c : s.s * f1.c / s.I) - s.I * ff1 - f2.c * b;
+// (R = float: FP32 arithmetic and s.s / s.I * f1 for s.s * f1 / s.I, not the
+// reference's rounding; for the FP32 preconditioner only)
+template <class R = double>
+__device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, R b, const Nb5T<R>& mi,
+                                            const Nb5T<R>& m0, const Nb5T<R>& f1, const Nb5T<R>& f2, R& o1,
+                                            R& o2) {
+  constexpr bool kF32 = sizeof(R) == sizeof(float);
+  R ff1 = op_dgn_v<R>(R(s.idr2), R(s.idz2), mi, f1);
+  ff1 = (kF32 ? R(s.s_I) * f1.c : R(s.s) * f1.c / R(s.I)) - R(s.I) * ff1 - f2.c * b;
   // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
-  if (j == 0) ff1 = ff1 + s.I_dz2 * (f1.c - s.coef_bot * f1.c);
-  if (j == g.ny - 1) ff1 = ff1 + s.I_dz2 * (f1.c - s.coef_top * f1.c);
-  double ff2 = op_dgn_v(s.idr2, s.idz2, m0, f2);
-  ff2 = f1.c * (b + s.adj) - s.dt * s.xld * ff2;
+  if (j == 0) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_bot) * f1.c);
+  if (j == g.ny - 1) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_top) * f1.c);
+  R ff2 = op_dgn_v<R>(R(s.idr2), R(s.idz2), m0, f2);
+  ff2 = f1.c * (b + R(s.adj)) - R(s.dt) * R(s.xld) * ff2;
   o1 = ff1;
   o2 = ff2;
 }
@@ -704,10 +707,10 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 // launch gives bit for bit the two colour half-sweeps.  xo must not alias
 // xi: a workgroup that starts after its neighbour has written would read
 // that neighbour's new values in its halo.  FIRST: from x = 0 (xi unread).
-// The coefficients are stored as TC, b as TB, xi as TI and xo as TO: FP64,
-// or FP32 storage with the arithmetic in FP64, where the coefficients and b
-// are also held in LDS as FP32 (more workgroups per CU) and bc, if given,
-// receives the FP32 b of the tile for the later sweeps.
+// The coefficients are stored as TC, b as TB, xi as TI and xo as TO; the
+// tile is held in LDS and computed in TC: FP64 (the reference's arithmetic),
+// or FP32 (the FP32 preconditioner: FP32 arithmetic with reciprocals), where
+// bc, if given, receives the FP32 b of the tile for the later sweeps.
 constexpr int kRbTX = 64, kRbTY = 8, kRbH = 2;
 constexpr int kRbW = kRbTX + 2 * kRbH, kRbHt = kRbTY + 2 * kRbH;
 template <bool FIRST, class TC, class TB, class TI, class TO>
@@ -725,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   bc = bofs(bc, bs);
   xi = bofs(xi, bs);
   xo = bofs(xo, bs);
-  __shared__ double sx1[kRbHt][kRbW], sx2[kRbHt][kRbW];
+  __shared__ TC sx1[kRbHt][kRbW], sx2[kRbHt][kRbW];
   __shared__ TC smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW], sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
@@ -753,8 +756,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     const int li = e % kRbW, lj = e / kRbW;
     const int i = i0 - kRbH + li, j = j0 - kRbH + lj;
     if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
-    sx1[lj][li] = v[it][0];
-    sx2[lj][li] = v[it][1];
+    sx1[lj][li] = (TC)v[it][0];
+    sx2[lj][li] = (TC)v[it][1];
     smi[lj][li] = (TC)v[it][2];
     smz[lj][li] = (TC)v[it][3];
     sbu[lj][li] = (TC)v[it][4];
@@ -767,15 +770,14 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
     const int a = i - i0 + kRbH, al = il - i0 + kRbH, ar = ir - i0 + kRbH;
     const int r = j - j0 + kRbH, rl = jl - j0 + kRbH, rr = jr - j0 + kRbH;
-    const Nb5 mi{smi[r][a], smi[r][al], smi[r][ar], smi[rl][a], smi[rr][a]};
-    const Nb5 mz{smz[r][a], smz[r][al], smz[r][ar], smz[rl][a], smz[rr][a]};
-    const Nb5 f1{sx1[r][a], sx1[r][al], sx1[r][ar], sx1[rl][a], sx1[rr][a]};
-    const Nb5 f2{sx2[r][a], sx2[r][al], sx2[r][ar], sx2[rl][a], sx2[rr][a]};
-    const double bv = sbu[r][a];
-    double z1, z2, y1, y2;
-    constexpr bool kFast = sizeof(TC) == sizeof(float);  // FP32 storage: reciprocals, not divisions
-    phmu_rows_v<kFast>(g, s, j, bv, mi, mz, f1, f2, z1, z2);
-    phmu_block_v<kFast>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
+    const Nb5T<TC> mi{smi[r][a], smi[r][al], smi[r][ar], smi[rl][a], smi[rr][a]};
+    const Nb5T<TC> mz{smz[r][a], smz[r][al], smz[r][ar], smz[rl][a], smz[rr][a]};
+    const Nb5T<TC> f1{sx1[r][a], sx1[r][al], sx1[r][ar], sx1[rl][a], sx1[rr][a]};
+    const Nb5T<TC> f2{sx2[r][a], sx2[r][al], sx2[r][ar], sx2[rl][a], sx2[rr][a]};
+    const TC bv = sbu[r][a];
+    TC z1, z2, y1, y2;
+    phmu_rows_v<TC>(g, s, j, bv, mi, mz, f1, f2, z1, z2);
+    phmu_block_v<TC>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
     sx1[r][a] = f1.c + y1;
     sx2[r][a] = f2.c + y2;
   };
@@ -1197,12 +1199,36 @@ void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, c
 // floats, two FP32 iterates and the FP32 b the first sweep writes.
 constexpr int kRbgsSweeps = 8;
 static_assert(kRbgsSweeps % 2 == 0, "the last sweep must write x");
+static dim3 rb_grid(const Grid& g) {
+  return dim3((unsigned)((g.nx + kRbTX - 1) / kRbTX), (unsigned)((g.ny + kRbTY - 1) / kRbTY), bat_n());
+}
+void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
+                            const float* xi, float* xo, const KState* ks) {
+  hipLaunchKernelGGL((k_phmu_rb_sweep<false, float, float, float, float>), rb_grid(g), dim3(kBlock), 0, st, g, s,
+                     c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs());
+}
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
-                      const double* b, double* x, void* work, const KState* ks) {
-  const dim3 grid((unsigned)((g.nx + kRbTX - 1) / kRbTX), (unsigned)((g.ny + kRbTY - 1) / kRbTY), bat_n());
+                      const double* b, double* x, void* work, const KState* ks, const KTimer* tm) {
+  const dim3 grid = rb_grid(g);
 #define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                              \
   hipLaunchKernelGGL((k_phmu_rb_sweep<F, TC, TB, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu, CO->minner, \
                      CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
+  // FP32 storage: two iterates and the FP32 b
+  const int64_t n2 = 2 * g.ncell();
+  float* wa = (float*)work;
+  float* wb = wa + n2;
+  float* bf = wb + n2;
+  // a middle FP32 sweep, timed while tm has event pairs left
+  auto mid = [&](const float* xi, float* xo) {
+    if (tm && *tm->n < tm->cap) {
+      const int k = (*tm->n)++;
+      hipExtLaunchKernelGGL((k_phmu_rb_sweep<false, float, float, float, float>), grid, dim3(kBlock), 0, st,
+                            tm->ev[2 * k], tm->ev[2 * k + 1], 0u, g, s, c32->bu, c32->minner, c32->m0,
+                            (const float*)bf, nullptr, xi, xo, ks, bat_bs());
+    } else {
+      launch_phmu_rb_sweep32(st, g, s, *c32, bf, xi, xo, ks);
+    }
+  };
   if (!c32) {
     double* w = (double*)work;
     const Coef* co = &c;
@@ -1214,16 +1240,12 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
         RB(false, double, double, double, double, co, b, nullptr, x, w);
     }
   } else {
-    const int64_t n2 = 2 * g.ncell();
-    float* wa = (float*)work;
-    float* wb = wa + n2;
-    float* bf = wb + n2;
     RB(true, float, double, float, float, c32, b, bf, nullptr, wa);
     for (int k = 1; k + 1 < kRbgsSweeps; ++k) {
       if (k & 1)
-        RB(false, float, float, float, float, c32, bf, nullptr, wa, wb);
+        mid(wa, wb);
       else
-        RB(false, float, float, float, float, c32, bf, nullptr, wb, wa);
+        mid(wb, wa);
     }
     RB(false, float, float, float, double, c32, bf, nullptr, wa, x);  // sweep kRbgsSweeps - 2 (even) wrote wa
   }

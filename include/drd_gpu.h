@@ -59,11 +59,10 @@ extern "C" {
 #define DRD_CH_PREC_RBGS 2      /* eight red-black Gauss-Seidel sweeps of the
                                    same 2x2 blocks (no reference
                                    counterpart)                             */
-#define DRD_CH_PREC_RBGS32 3    /* the same sweeps with the mobilities, b and
-                                   the iterates between sweeps stored in
-                                   FP32; FP64 arithmetic, multiplying by
-                                   reciprocals where the FP64 form divides;
-                                   FP64 result                              */
+#define DRD_CH_PREC_RBGS32 3    /* the same sweeps in FP32: mobilities, b,
+                                   iterates and arithmetic (reciprocals
+                                   where the FP64 form divides); FP64 b in,
+                                   FP64 result out                          */
 
 /* ---- outer Krylov method ------------------------------------------------ */
 #define DRD_KRYLOV_BICGSTAB 0 /* LIB/BiCGSTAB.f, reference stopping rule    */
@@ -222,9 +221,9 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
                                  (addition): x = 0; then 8 times: the red cells,
                                  then the black cells, x_c += B_c^-1 (b - A x)_c
                                  with B_c the exact 2x2 block of cell c         */
-#define DRD_PS_PHMU_RBGS32 12 /* DRD_PS_PHMU_RBGS with FP32 storage of bu,
-                                 M_inner, M_0, b and the iterates between
-                                 sweeps, reciprocals for divisions (addition) */
+#define DRD_PS_PHMU_RBGS32 12 /* DRD_PS_PHMU_RBGS in FP32 (bu, M_inner, M_0,
+                                 b, iterates, arithmetic; reciprocals for
+                                 divisions); FP64 in and out (addition)      */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
 
@@ -328,6 +327,9 @@ int drd_batch_set_initial_guess(drd_batch* b, int32_t mode);
                                preconditioner (all its passes)             */
 #define DRD_K_VELO_MG 7     /* one multigrid V-cycle (all its launches);
                                *bytes: FP64 in and out of the fine level   */
+#define DRD_K_PHMU_SWEEP32 8 /* one middle sweep of DRD_PS_PHMU_RBGS32 (FP32
+                                iterate in and out, FP32 b, bu, M_inner,
+                                M_0), after one application of it         */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
 /* Times kernel `which` inside the caller's own solves: while enabled, each
@@ -335,7 +337,8 @@ int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  * its start and end in a pair of HIP events (the dispatch's own
  * timestamps, hipExtLaunchKernelGGL) on the context's stream.  Only
  * DRD_K_VELO_MATVEC (the BDF2 operator's fused matvec with one dot, as
- * BiCGSTAB launches it) is instrumented.  drd_kernel_timer_read waits for
+ * BiCGSTAB launches it) and DRD_K_PHMU_SWEEP32 (the middle sweeps of
+ * DRD_PS_PHMU_RBGS32) are instrumented.  drd_kernel_timer_read waits for
  * the stream and returns the number of timed launches, their summed
  * duration and the kernel's algorithmic bytes per launch.                 */
 int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);

@@ -12,9 +12,10 @@
 * 1024x512 holds more tiles than the GPU runs at once, so some sweep
   workgroups start after their neighbours have finished: the sweeps must
   read one buffer and write another for the result to stay exact.
-* DRD_PS_PHMU_RBGS32 (FP32 storage of the mobilities, b and the iterates
-  between sweeps; FP64 arithmetic with reciprocals for the divisions)
-  equals the FP64 form to a relative 1e-6,
+* DRD_PS_PHMU_RBGS32 (FP32 mobilities, b, iterates and arithmetic, with
+  reciprocals for the divisions; the 2x2 block determinant is a sum of
+  positive terms, so FP32 loses nothing there) equals the FP64 form to a
+  relative 1e-6,
   converges as fast inside BiCGSTAB, and its steps track the oracle's.
 """
 import numpy as np
