@@ -732,7 +732,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   __shared__ TC smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW], sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
-  const int i0 = (int)blockIdx.x * kRbTX, j0 = (int)blockIdx.y * kRbTY;
+  // tiles in row-major order, a contiguous band of them per XCD (tile_id),
+  // so neighbouring tiles share their halo lines in one L2
+  const int ntx = (nx + kRbTX - 1) / kRbTX;
+  const int t = (int)tile_id();
+  const int i0 = (t % ntx) * kRbTX, j0 = (t / ntx) * kRbTY;
   // staging: every load of the thread is issued before the first LDS store
   // (indices clamped into the grid; the stores keep the cells inside it)
   constexpr int kStage = kRbW * kRbHt, kStageIt = (kStage + kBlock - 1) / kBlock;
@@ -1199,8 +1203,8 @@ void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, c
 // floats, two FP32 iterates and the FP32 b the first sweep writes.
 constexpr int kRbgsSweeps = 8;
 static_assert(kRbgsSweeps % 2 == 0, "the last sweep must write x");
-static dim3 rb_grid(const Grid& g) {
-  return dim3((unsigned)((g.nx + kRbTX - 1) / kRbTX), (unsigned)((g.ny + kRbTY - 1) / kRbTY), bat_n());
+static dim3 rb_grid(const Grid& g) {  // one workgroup per tile (tile_id)
+  return dim3((unsigned)(((g.nx + kRbTX - 1) / kRbTX) * ((g.ny + kRbTY - 1) / kRbTY)), 1, bat_n());
 }
 void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
                             const float* xi, float* xo, const KState* ks) {
