@@ -1874,8 +1874,9 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
       RCHK(prec_setup(c, c->vprec_fp32 ? DRD_PS_VELO_DCT32 : DRD_PS_VELO_DCT64));
       *bytes = 2 * N8 + (c->vprec_fp32 ? N8 / 2 : N8);
       break;
-    case DRD_K_VELO_MG:  // r in, z out (FP64); the context's hierarchy, FP64 if it has none
-      RCHK(prec_setup(c, velo_mg_f32(c->vmg) ? DRD_PS_VELO_MG32 : DRD_PS_VELO_MG));
+    case DRD_K_VELO_MG:  // r in, z out (FP64); the context's hierarchy, else the one its velo_prec names
+      RCHK(prec_setup(c, (c->vmg ? velo_mg_f32(c->vmg) : c->p.velo_prec == DRD_PREC_MG32) ? DRD_PS_VELO_MG32
+                                                                                        : DRD_PS_VELO_MG));
       *bytes = 2 * N8;
       break;
     case DRD_K_PHMU_SWEEP32: {  // x in and out, b: 2 floats per cell each; bu, M_inner, M_0: 1 each

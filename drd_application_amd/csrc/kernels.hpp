@@ -242,10 +242,11 @@ void launch_mg_convert(hipStream_t st, const Grid& g, const Coef& c, float* rho,
 void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const Grid& gc, const MgCoef& f,
                        void* rc, void* ec, void* cc);
 void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* d);
-// coefficients, x and d of the level's type (f32); b and out of their own
+// coefficients, x and d of the level's type (f32); b and out of their own;
+// MG_FIRST also writes b to bc as FP32 if bc is given
 void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
-                      const KState* ks);
+                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
+                      double omega, const KState* ks);
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
                         void* bc, const KState* ks);
 void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,

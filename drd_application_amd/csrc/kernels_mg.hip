@@ -92,7 +92,7 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err) {
     ok = hipMalloc(&L.d, n) == hipSuccess && hipMalloc(&L.t, n) == hipSuccess;
     if (ok && l + 1 < nl) ok = hipMalloc(&L.r, n) == hipSuccess;
     if (ok && (l > 0 || L.f32)) ok = hipMalloc(&L.x, n) == hipSuccess;  // FP64 level 0: x is z
-    if (ok && l > 0) ok = hipMalloc(&L.b, n) == hipSuccess;              // level 0: b is the input
+    if (ok && (l > 0 || L.f32)) ok = hipMalloc(&L.b, n) == hipSuccess;  // FP64 level 0: b is the input
     if (ok && (l > 0 || L.f32)) {
       const size_t nc = (size_t)L.g.nc() * es;
       ok = hipMalloc(&L.fields, 3 * nc) == hipSuccess;
@@ -144,14 +144,18 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
   const double w = mg->omega;
   auto bufA = [&](int l) { return mg->lv[l].t; };
   auto bufB = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (void*)z : mg->lv[l].x; };
-  auto rhs = [&](int l) { return l == 0 ? (const void*)b : (const void*)mg->lv[l].b; };
-  auto b32 = [&](int l) { return l > 0 && mg->lv[l].f32; };  // the input b is FP64
+  // level 0 of an FP32 hierarchy: its first sweep reads the FP64 b and keeps
+  // an FP32 copy in lv[0].b, which the later sweeps read
+  auto rhs = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (const void*)b : (const void*)mg->lv[l].b; };
+  auto b32 = [&](int l) { return mg->lv[l].f32; };
   for (int l = 0; l + 1 < nl; ++l) {
     MGLevel& L = mg->lv[l];
     const bool f = L.f32;
-    launch_mg_smooth(st, MG_FIRST, f, b32(l), f, L.g, L.s, L.c, nullptr, rhs(l), L.d, bufA(l), w, ks);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), w, ks);
-    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, L.r, w, ks);
+    const bool copy = l == 0 && f;
+    launch_mg_smooth(st, MG_FIRST, f, f && !copy, f, L.g, L.s, L.c, nullptr, copy ? (const void*)b : rhs(l), L.d,
+                     bufA(l), copy ? (float*)L.b : nullptr, w, ks);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), nullptr, w, ks);
+    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, L.r, nullptr, w, ks);
     launch_mg_restrict(st, f, mg->lv[l + 1].f32, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
   }
   {
@@ -162,9 +166,10 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
     MGLevel& L = mg->lv[l];
     const bool f = L.f32;
     launch_mg_prolong_add(st, mg->lv[l + 1].f32, f, mg->lv[l + 1].g, L.g, mg->lv[l + 1].x, bufB(l), ks);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, bufA(l), w, ks);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, bufA(l), nullptr, w, ks);
     void* out = l == 0 ? (void*)z : bufB(l);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && l > 0, L.g, L.s, L.c, bufA(l), rhs(l), L.d, out, w, ks);
+    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && l > 0, L.g, L.s, L.c, bufA(l), rhs(l), L.d, out, nullptr, w,
+                     ks);
   }
 }
 

@@ -883,7 +883,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const T
                                                        const T* __restrict__ ef, const T* __restrict__ cf,
                                                        TO* __restrict__ rc, TO* __restrict__ ec,
                                                        TO* __restrict__ cc) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nc()) return;
   const int ld = gc.nx + 2;
   const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
@@ -906,25 +906,28 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __restrict__ rho,
                                                     const T* __restrict__ eta, const T* __restrict__ chi,
                                                     T* __restrict__ d) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q < g.nuv()) d[q] = (T)velo_diag2(g, s, rho, eta, chi, q);
 }
 
 // Damped Jacobi sweep xo = x + omega (b - A x) / d; FIRST: from x = 0,
-// xo = omega b / d.  RES: r = b - A x instead.  The level's coefficients, x
-// and d are stored as T; b as TB, the output as TO.
+// xo = omega b / d, and bc (if given) receives b as FP32.  RES: r = b - A x
+// instead.  The level's coefficients, x and d are stored as T; b as TB, the
+// output as TO.
 template <int KIND, class T, class TB, class TO>
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* __restrict__ rho,
                                                       const T* __restrict__ eta, const T* __restrict__ chi,
                                                       const T* __restrict__ x, const TB* __restrict__ b,
                                                       const T* __restrict__ d, TO* __restrict__ out,
-                                                      double omega, const KState* st) {
+                                                      float* __restrict__ bc, double omega, const KState* st) {
   if (st && (st->done || st->early)) return;
-  const uint32_t q = (uint32_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t q = tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   const uint32_t n = (uint32_t)g.nuv();
   if (q >= n) return;
   if (KIND == MG_FIRST) {
-    out[q] = (TO)(omega * (double)b[q] / (double)d[q]);
+    const TB bq = b[q];
+    out[q] = (TO)(omega * (double)bq / (double)d[q]);
+    if (bc) bc[q] = (float)bq;
     return;
   }
   const VeloInT<T> X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
@@ -940,7 +943,7 @@ template <class TR, class TB>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const TR* __restrict__ r,
                                                         TB* __restrict__ bc, const KState* st) {
   if (st && (st->done || st->early)) return;
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nuv()) return;
   double acc = 0.0;
   if (q < gc.mn()) {  // u (I, K): fine faces 2I-1..2I+1, rows 2K-1..2K+2
@@ -973,7 +976,7 @@ template <class TC, class TF>
 __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const TC* __restrict__ xc,
                                                            TF* __restrict__ x, const KState* st) {
   if (st && (st->done || st->early)) return;
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gf.nuv()) return;
   double acc = 0.0;
   if (q < gf.mn()) {  // u (i, k): coarse faces i/2 (and i/2 + 1), rows k/2 and its neighbour
@@ -1142,31 +1145,31 @@ void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, cons
 }
 template <int K, class T, class TB, class TO>
 static void mg_smooth_t(hipStream_t st, const Grid& g, const Scal& s, const MgCoef& c, const void* x,
-                        const void* b, const void* d, void* out, double omega, const KState* ks) {
+                        const void* b, const void* d, void* out, float* bc, double omega, const KState* ks) {
   hipLaunchKernelGGL((k_mg_smooth<K, T, TB, TO>), dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s,
                      (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)d,
-                     (TO*)out, omega, ks);
+                     (TO*)out, bc, omega, ks);
 }
 template <int K>
 static void mg_smooth_k(hipStream_t st, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                        const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
-                        const KState* ks) {
+                        const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
+                        double omega, const KState* ks) {
   if (!f32)  // an FP64 hierarchy is FP64 throughout
-    mg_smooth_t<K, double, double, double>(st, g, s, c, x, b, d, out, omega, ks);
-  else if (b32)
-    mg_smooth_t<K, float, float, float>(st, g, s, c, x, b, d, out, omega, ks);
-  else if (o32)  // level 0 of an FP32 hierarchy: FP64 b
-    mg_smooth_t<K, float, double, float>(st, g, s, c, x, b, d, out, omega, ks);
-  else  // its last sweep: FP64 output
-    mg_smooth_t<K, float, double, double>(st, g, s, c, x, b, d, out, omega, ks);
+    mg_smooth_t<K, double, double, double>(st, g, s, c, x, b, d, out, bc, omega, ks);
+  else if (b32 && o32)
+    mg_smooth_t<K, float, float, float>(st, g, s, c, x, b, d, out, bc, omega, ks);
+  else if (b32)  // the last sweep of level 0: FP64 output
+    mg_smooth_t<K, float, float, double>(st, g, s, c, x, b, d, out, bc, omega, ks);
+  else  // the first sweep of level 0: FP64 b
+    mg_smooth_t<K, float, double, float>(st, g, s, c, x, b, d, out, bc, omega, ks);
 }
 void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, double omega,
-                      const KState* ks) {
+                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
+                      double omega, const KState* ks) {
   switch (kind) {
-    case MG_FIRST: mg_smooth_k<MG_FIRST>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
-    case MG_RES: mg_smooth_k<MG_RES>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
-    default: mg_smooth_k<MG_SWEEP>(st, f32, b32, o32, g, s, c, x, b, d, out, omega, ks); break;
+    case MG_FIRST: mg_smooth_k<MG_FIRST>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
+    case MG_RES: mg_smooth_k<MG_RES>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
+    default: mg_smooth_k<MG_SWEEP>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
   }
 }
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
