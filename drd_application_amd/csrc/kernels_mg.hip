@@ -37,6 +37,7 @@ struct VeloMG {
   VeloPrec* coarse = nullptr;
   bool f32 = false;
   double omega = 0.7;
+  int nu_pre = 2, nu_post = 2;  // damped Jacobi sweeps before / after the coarse correction (>= 1 each)
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
@@ -135,15 +136,21 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
   velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
 }
 
-// z = V-cycle(b).  Level l smooths in two buffers: the first sweep writes
-// A, the next B, the residual and the coarse correction act on B, and the
-// two sweeps after it return to B (level 0: A = its temporary, B = z, or its
-// FP32 x with the last sweep writing z).
+// z = V-cycle(b).  Level l smooths between two buffers, P (its temporary)
+// and Q (its x, which the level above prolongs from; FP64 level 0: z).  The
+// nu_post sweeps after the coarse correction alternate from D and must end
+// in Q, so D = P for odd nu_post and Q for even; the nu_pre sweeps before it
+// alternate so as to end in D, where the residual is taken and the coarse
+// correction added.  Level 0 of an FP32 hierarchy ends in z, a third
+// buffer, so its D is free (Q).  No sweep reads the buffer it writes.
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks) {
   const int nl = (int)mg->lv.size();
   const double w = mg->omega;
-  auto bufA = [&](int l) { return mg->lv[l].t; };
-  auto bufB = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (void*)z : mg->lv[l].x; };
+  const int n1 = mg->nu_pre, n2 = mg->nu_post;
+  auto P = [&](int l) { return mg->lv[l].t; };
+  auto Q = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (void*)z : mg->lv[l].x; };
+  auto other = [&](int l, void* u) { return u == P(l) ? Q(l) : P(l); };
+  auto D = [&](int l) { return (l == 0 && mg->lv[0].f32) || n2 % 2 == 0 ? Q(l) : P(l); };
   // level 0 of an FP32 hierarchy: its first sweep reads the FP64 b and keeps
   // an FP32 copy in lv[0].b, which the later sweeps read
   auto rhs = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (const void*)b : (const void*)mg->lv[l].b; };
@@ -152,10 +159,15 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
     MGLevel& L = mg->lv[l];
     const bool f = L.f32;
     const bool copy = l == 0 && f;
+    void* cur = (n1 - 1) % 2 == 0 ? D(l) : other(l, D(l));  // the first sweep's output
     launch_mg_smooth(st, MG_FIRST, f, f && !copy, f, L.g, L.s, L.c, nullptr, copy ? (const void*)b : rhs(l), L.d,
-                     bufA(l), copy ? (float*)L.b : nullptr, w, ks);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufA(l), rhs(l), L.d, bufB(l), nullptr, w, ks);
-    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, L.r, nullptr, w, ks);
+                     cur, copy ? (float*)L.b : nullptr, w, ks);
+    for (int k = 1; k < n1; ++k) {
+      void* nxt = other(l, cur);
+      launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, nxt, nullptr, w, ks);
+      cur = nxt;
+    }
+    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, L.r, nullptr, w, ks);
     launch_mg_restrict(st, f, mg->lv[l + 1].f32, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
   }
   {
@@ -165,11 +177,15 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
   for (int l = nl - 2; l >= 0; --l) {
     MGLevel& L = mg->lv[l];
     const bool f = L.f32;
-    launch_mg_prolong_add(st, mg->lv[l + 1].f32, f, mg->lv[l + 1].g, L.g, mg->lv[l + 1].x, bufB(l), ks);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, bufB(l), rhs(l), L.d, bufA(l), nullptr, w, ks);
-    void* out = l == 0 ? (void*)z : bufB(l);
-    launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && l > 0, L.g, L.s, L.c, bufA(l), rhs(l), L.d, out, nullptr, w,
-                     ks);
+    void* cur = D(l);
+    launch_mg_prolong_add(st, mg->lv[l + 1].f32, f, mg->lv[l + 1].g, L.g, mg->lv[l + 1].x, cur, ks);
+    for (int k = 1; k <= n2; ++k) {
+      const bool last = k == n2;
+      void* out = last ? (l == 0 ? (void*)z : Q(l)) : other(l, cur);
+      launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && !(last && l == 0), L.g, L.s, L.c, cur, rhs(l), L.d, out,
+                       nullptr, w, ks);
+      cur = out;
+    }
   }
 }
 
