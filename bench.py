@@ -112,9 +112,9 @@ MODES = {
 }
 
 GUESS = os.environ.get("DRD_BENCH_GUESS", "1") != "0"
-# launches of the roofline kernel timed with HIP events inside the timed
-# steps (the first ones; an event pair per launch costs a few microseconds,
-# so timing every launch would slow the steps it measures)
+# groups of launches of the roofline kernel timed with HIP events inside the
+# timed steps (the first ones; each event pair adds a little to the steps
+# it measures, so not every launch is timed)
 TIMED_LAUNCHES = 160
 LEARN = {"record_steps": 3, "capacity": 64, "crop": 256, "iters": 300, "batch": 4, "lr": 5e-4}
 
@@ -202,9 +202,10 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer=
         # data: they take a small fraction of a working launch's time and are
         # counted apart, not averaged in
         work = each[each > 0.25 * np.median(each)] if each.size else each
-        out["timed_launches"] = {"count": int(work.size), "total_ms": float(work.astype(np.float64).sum()),
-                                 "bytes_per_launch": by, "idle_launches": int(n - work.size),
-                                 "idle_ms": float(ms - work.astype(np.float64).sum())}
+        per = n // max(int(each.size), 1)  # launches per timed group (the mean of each is per launch)
+        wsum = float(work.astype(np.float64).sum()) * per
+        out["timed_launches"] = {"count": int(work.size) * per, "total_ms": wsum, "bytes_per_launch": by,
+                                 "idle_launches": int(n - work.size * per), "idle_ms": float(ms - wsum)}
     if learn is not None:
         names = {abi.LEARN_ON: "learned FGMRES", abi.LEARN_FALLBACK: "plain fallback (fp32 BiCGSTAB)"}
         fin = g.learn_info()

@@ -1757,6 +1757,9 @@ extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32
 }
 
 // ---- measurement hook ---------------------------------------------------------
+// launches per timed group of the instrumented kernel
+static int tm_group(const drd_ctx* c) { return c->tm_which == DRD_K_PHMU_SWEEP32 ? kRbgsMiddle : 1; }
+
 extern "C" int drd_kernel_timer(drd_ctx* c, int32_t which, int32_t capacity) {
   if (!c || capacity < 0) return set_err(DRD_E_ARG, "bad argument");
   if (capacity > 0 && which != DRD_K_VELO_MATVEC && which != DRD_K_PHMU_SWEEP32)
@@ -1780,7 +1783,7 @@ extern "C" int drd_kernel_timer_read(drd_ctx* c, int32_t* count, double* total_m
     HIPCHK(hipEventElapsedTime(&ms, c->tm_ev[2 * i], c->tm_ev[2 * i + 1]));
     tot += ms;
   }
-  *count = c->tm_n;
+  *count = c->tm_n * tm_group(c);
   *total_ms = tot;
   if (c->tm_which == DRD_K_PHMU_SWEEP32)  // x in and out, b: 2 floats per cell each; bu, M_inner, M_0: 1 each
     *bytes = 4.0 * 9.0 * (double)c->g.ncell();
@@ -1792,8 +1795,10 @@ extern "C" int drd_kernel_timer_read(drd_ctx* c, int32_t* count, double* total_m
 extern "C" int drd_kernel_timer_launches(drd_ctx* c, int32_t capacity, float* ms, int32_t* count) {
   if (!c || !count || capacity < 0 || (capacity > 0 && !ms)) return set_err(DRD_E_ARG, "bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < c->tm_n && i < capacity; ++i)
+  for (int i = 0; i < c->tm_n && i < capacity; ++i) {
     HIPCHK(hipEventElapsedTime(&ms[i], c->tm_ev[2 * i], c->tm_ev[2 * i + 1]));
+    ms[i] /= (float)tm_group(c);
+  }
   *count = c->tm_n;
   return 0;
 }

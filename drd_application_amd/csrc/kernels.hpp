@@ -206,15 +206,18 @@ struct PhmuCoef32 {
   const float *bu, *minner, *m0;
 };
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32);
-// Event pairs for timing launches: launch k (k < cap) of the instrumented
-// kernel records ev[2k] and ev[2k+1] around its dispatch; *n counts them.
+// Event pairs for timing: group k (k < cap) of consecutive launches of the
+// instrumented kernel is bracketed by ev[2k] and ev[2k+1]; *n counts the
+// groups.
 struct KTimer {
   hipEvent_t* ev;
   int cap;
   int* n;
 };
+constexpr int kRbgsMiddle = 6;  // middle sweeps per red-black application (all but the first and last)
 // c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
-// work = 6 ncell floats, and tm (optional) times the middle sweeps
+// work = 6 ncell floats, and tm (optional) times the run of middle sweeps of
+// each application as one group of kRbgsMiddle launches
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm = nullptr);
 // one middle FP32-storage sweep, xi -> xo with the FP32 b (stand-alone timing)

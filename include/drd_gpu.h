@@ -332,21 +332,23 @@ int drd_batch_set_initial_guess(drd_batch* b, int32_t mode);
                                 M_0), after one application of it         */
 int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
                      double* bytes);
-/* Times kernel `which` inside the caller's own solves: while enabled, each
- * launch of it (up to `capacity` launches; 0 disables and resets) records
- * its start and end in a pair of HIP events (the dispatch's own
- * timestamps, hipExtLaunchKernelGGL) on the context's stream.  Only
- * DRD_K_VELO_MATVEC (the BDF2 operator's fused matvec with one dot, as
- * BiCGSTAB launches it) and DRD_K_PHMU_SWEEP32 (the middle sweeps of
- * DRD_PS_PHMU_RBGS32) are instrumented.  drd_kernel_timer_read waits for
- * the stream and returns the number of timed launches, their summed
- * duration and the kernel's algorithmic bytes per launch.                 */
+/* Times kernel `which` inside the caller's own solves, in groups of
+ * consecutive launches, each group bracketed by a pair of HIP events on the
+ * context's stream (up to `capacity` groups; 0 disables and resets):
+ *   DRD_K_VELO_MATVEC: the BDF2 operator's fused matvec with one dot, as
+ *     BiCGSTAB launches it; a group is one launch, timed by its own
+ *     dispatch timestamps (hipExtLaunchKernelGGL);
+ *   DRD_K_PHMU_SWEEP32: the middle sweeps of DRD_PS_PHMU_RBGS32; a group is
+ *     the 6 of one application, between two event markers on the stream.
+ * drd_kernel_timer_read waits for the stream and returns the number of timed
+ * launches, their summed duration and the kernel's algorithmic bytes per
+ * launch.                                                                 */
 int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);
 int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
-/* The same launches one by one: ms[i] is the duration of timed launch i
- * (i < min(*count, capacity)), in launch order.  A launch issued after its
- * solve finished (the tail of a queued chunk) returns at once and moves no
- * data; its duration tells it apart.                                       */
+/* The same groups one by one: ms[i] is the mean launch duration of timed
+ * group i (i < min(*count, capacity)), in launch order; *count is the number
+ * of groups.  A launch issued after its solve finished (the tail of a queued
+ * chunk) returns at once and moves no data; its duration tells it apart.  */
 int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t* count);
 
 /* ---- learned correction of the velocity preconditioner ------------------
