@@ -225,7 +225,7 @@ void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const 
                             const float* xi, float* xo, const KState* ks);
 
 // ---- multigrid V-cycle for the BDF2 velocity operator (kernels_mg.hip) ----
-enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2 };
+enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2, MG_SWEEP0 = 3 };
 struct VeloMG;
 // f32: the levels' coefficients, diagonals and vectors are stored in FP32
 // (arithmetic stays FP64); the input b, the output z and the coarsest level
@@ -244,12 +244,14 @@ struct MgCoef {
 void launch_mg_convert(hipStream_t st, const Grid& g, const Coef& c, float* rho, float* eta, float* chi);
 void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const Grid& gc, const MgCoef& f,
                        void* rc, void* ec, void* cc);
-void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* d);
-// coefficients, x and d of the level's type (f32); b and out of their own;
-// MG_FIRST also writes b to bc as FP32 if bc is given
+// dw = omega / the exact diagonal
+void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* dw,
+                    double omega);
+// coefficients, x and dw of the level's type (f32); b and out of their own;
+// MG_FIRST and MG_SWEEP0 also write b to bc as FP32 if bc is given
 void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
-                      double omega, const KState* ks);
+                      const MgCoef& c, const void* x, const void* b, const void* dw, void* out, float* bc,
+                      const KState* ks);
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
                         void* bc, const KState* ks);
 void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,

@@ -5,8 +5,8 @@
 // the result keeps nx >= 64 and ny >= 32.  The coarse operators are
 // rediscretised: the same MAC stencil (velo_row) with doubled spacings, on
 // coefficients rho, eta, chi averaged over 2x2 cells.  One V-cycle:
-//   down:  x = omega b / d, one more damped Jacobi sweep, r = b - A x,
-//          b_{l+1} = R r
+//   down:  x = omega b / d, one more damped Jacobi sweep (the two in one
+//          pass), r = b - A x, b_{l+1} = R r
 //   coarsest: x = the FP64 DCT/DST fast-Poisson solve of its b
 //   up:    x += P x_{l+1}, two damped Jacobi sweeps
 // with omega = 0.7 and d the exact diagonal of each level's operator.  The
@@ -29,7 +29,7 @@ struct MGLevel {
   bool f32 = false;            // storage type of the level's buffers
   MgCoef c{};                  // rho, eta, chi of the level (FP64 level 0: the context's fields)
   void* fields = nullptr;      // own rho, eta, chi
-  void *d = nullptr, *x = nullptr, *t = nullptr, *b = nullptr, *r = nullptr;
+  void *d = nullptr, *x = nullptr, *t = nullptr, *b = nullptr, *r = nullptr;  // d: omega / diagonal
 };
 
 struct VeloMG {
@@ -130,7 +130,7 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
     launch_mg_coarsen(st, F.f32, L.f32, F.g, L.g, F.c, const_cast<void*>(L.c.rho), const_cast<void*>(L.c.eta),
                       const_cast<void*>(L.c.chi));
   }
-  for (auto& L : mg->lv) launch_mg_diag(st, L.f32, L.g, L.s, L.c, L.d);
+  for (auto& L : mg->lv) launch_mg_diag(st, L.f32, L.g, L.s, L.c, L.d, mg->omega);
   const MGLevel& C = mg->lv.back();  // FP64
   const Coef cc{(const double*)C.c.rho, (const double*)C.c.eta, (const double*)C.c.chi};
   velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
@@ -145,7 +145,6 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
 // buffer, so its D is free (Q).  No sweep reads the buffer it writes.
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks) {
   const int nl = (int)mg->lv.size();
-  const double w = mg->omega;
   const int n1 = mg->nu_pre, n2 = mg->nu_post;
   auto P = [&](int l) { return mg->lv[l].t; };
   auto Q = [&](int l) { return (l == 0 && !mg->lv[0].f32) ? (void*)z : mg->lv[l].x; };
@@ -160,14 +159,22 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
     const bool f = L.f32;
     const bool copy = l == 0 && f;
     void* cur = (n1 - 1) % 2 == 0 ? D(l) : other(l, D(l));  // the first sweep's output
-    launch_mg_smooth(st, MG_FIRST, f, f && !copy, f, L.g, L.s, L.c, nullptr, copy ? (const void*)b : rhs(l), L.d,
-                     cur, copy ? (float*)L.b : nullptr, w, ks);
-    for (int k = 1; k < n1; ++k) {
+    const void* b0 = copy ? (const void*)b : rhs(l);
+    float* bc = copy ? (float*)L.b : nullptr;
+    int k = 1;
+    if (n1 >= 2) {  // the first two sweeps in one pass
+      cur = other(l, cur);
+      launch_mg_smooth(st, MG_SWEEP0, f, f && !copy, f, L.g, L.s, L.c, nullptr, b0, L.d, cur, bc, ks);
+      k = 2;
+    } else {
+      launch_mg_smooth(st, MG_FIRST, f, f && !copy, f, L.g, L.s, L.c, nullptr, b0, L.d, cur, bc, ks);
+    }
+    for (; k < n1; ++k) {
       void* nxt = other(l, cur);
-      launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, nxt, nullptr, w, ks);
+      launch_mg_smooth(st, MG_SWEEP, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, nxt, nullptr, ks);
       cur = nxt;
     }
-    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, L.r, nullptr, w, ks);
+    launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, L.r, nullptr, ks);
     launch_mg_restrict(st, f, mg->lv[l + 1].f32, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
   }
   {
@@ -183,7 +190,7 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
       const bool last = k == n2;
       void* out = last ? (l == 0 ? (void*)z : Q(l)) : other(l, cur);
       launch_mg_smooth(st, MG_SWEEP, f, b32(l), f && !(last && l == 0), L.g, L.s, L.c, cur, rhs(l), L.d, out,
-                       nullptr, w, ks);
+                       nullptr, ks);
       cur = out;
     }
   }

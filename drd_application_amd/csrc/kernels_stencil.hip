@@ -15,14 +15,19 @@ namespace drd {
 // Ghost rules applied on the fly to a packed velocity vector:
 // btoA_uv TP/matrix_tool.f90:32-75 plus the overrides of
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:554-583.
-// The vector's element type T is FP64, or FP32 in the multigrid levels that
-// store FP32; values are read as double.
+// The vector's values come from a loader L (value of packed element i, as
+// double): a stored FP64 or FP32 vector (LdVec), or values formed on the fly.
 template <class T>
-struct VeloInT {
+struct LdVec {
   const T* __restrict__ x;
+  __device__ __forceinline__ double operator()(uint32_t i) const { return ldd(x, i); }
+};
+template <class L>
+struct VeloAcc {
+  L ld;
   int nx, ny, lr_d;
   uint32_t mn;
-  __device__ __forceinline__ double uraw(int i, int k) const { return ldd(x, (uint32_t)(i + (nx + 1) * k)); }
+  __device__ __forceinline__ double uraw(int i, int k) const { return ld((uint32_t)(i + (nx + 1) * k)); }
   // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
   // wall ghosts are odd mirrors of the first/last row, side ghosts even or
   // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
@@ -32,7 +37,7 @@ struct VeloInT {
     const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
     const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
     const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
-    const double v = ldd(x, (uint32_t)(ii + (nx + 1) * kk));
+    const double v = ld((uint32_t)(ii + (nx + 1) * kk));
     const double w = iw ? -v : v;
     return kw ? -w : w;
   }
@@ -40,19 +45,21 @@ struct VeloInT {
   __device__ __forceinline__ double V(int i, int k) const {
     const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
     const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
-    const double v = ldd(x, mn + (uint32_t)(ic + nx * (kc - 1)));
+    const double v = ld(mn + (uint32_t)(ic + nx * (kc - 1)));
     return (k <= 0 || k >= ny) ? 0.0 : v;
   }
 };
+template <class T>
+using VeloInT = VeloAcc<LdVec<T>>;
 using VeloIn = VeloInT<double>;
 
 // y = A x for one unknown q of the velocity operator.
 // MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
-template <int BDF, class T, class XT>
+template <int BDF, class T, class XA>
 __device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const T* __restrict__ rho,
                                              const T* __restrict__ eta, const T* __restrict__ chi,
-                                             const VeloInT<XT>& X, uint32_t q) {
+                                             const XA& X, uint32_t q) {
   const int nx = g.nx;
   double r;
   {
@@ -76,10 +83,10 @@ __device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const 
   return r;
 }
 
-template <int BDF, class T, class XT>
+template <int BDF, class T, class XA>
 __device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const T* __restrict__ rho,
                                              const T* __restrict__ eta, const T* __restrict__ chi,
-                                             const VeloInT<XT>& X, uint32_t q) {
+                                             const XA& X, uint32_t q) {
   const int nx = g.nx;
   double r;
   {
@@ -104,12 +111,12 @@ __device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const 
   return r;
 }
 
-template <int BDF, class T, class XT>
+template <int BDF, class T, class XA>
 __device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const T* __restrict__ rho,
                                            const T* __restrict__ eta, const T* __restrict__ chi,
-                                           const VeloInT<XT>& X, uint32_t q) {
-  return q < X.mn ? velo_u_row<BDF, T, XT>(g, s, rho, eta, chi, X, q)
-                   : velo_v_row<BDF, T, XT>(g, s, rho, eta, chi, X, q);
+                                           const XA& X, uint32_t q) {
+  return q < X.mn ? velo_u_row<BDF, T, XA>(g, s, rho, eta, chi, X, q)
+                   : velo_v_row<BDF, T, XA>(g, s, rho, eta, chi, X, q);
 }
 
 __device__ __forceinline__ bool kactive(const KState* st) {
@@ -139,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
   a0 = bofs(a0, bs);
   b = bofs(b, bs);
   part = bofs(part, bs);
-  VeloIn X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+  VeloIn X{{x}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
   const uint32_t n = (uint32_t)g.nuv();
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
   // A workgroup sweeps tile `pair` (u half of the vector) and tile
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const do
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   const int64_t nu = g.nuf();
-  VeloIn X{x, nx, ny, 0, (uint32_t)g.mn()};
+  VeloIn X{{x}, nx, ny, 0, (uint32_t)g.mn()};
   if (q < nu) {
     const int k = (int)(q / (nx + 3)) - 1;
     const int i = (int)(q - (int64_t)(k + 1) * (nx + 3)) - 1;
@@ -902,47 +909,81 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const T
   cc[q] = avg(cf);
 }
 
+// dw = omega / (the exact diagonal): the smoothers multiply by it
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __restrict__ rho,
                                                     const T* __restrict__ eta, const T* __restrict__ chi,
-                                                    T* __restrict__ d) {
+                                                    T* __restrict__ dw, double omega) {
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
-  if (q < g.nuv()) d[q] = (T)velo_diag2(g, s, rho, eta, chi, q);
+  if (q < g.nuv()) dw[q] = (T)(omega / velo_diag2(g, s, rho, eta, chi, q));
 }
 
-// Damped Jacobi sweep xo = x + omega (b - A x) / d; FIRST: from x = 0,
-// xo = omega b / d, and bc (if given) receives b as FP32.  RES: r = b - A x
-// instead.  The level's coefficients, x and d are stored as T; b as TB, the
-// output as TO.
+// x of the first sweep (x = omega b / d from 0) at packed element i, rounded
+// to the level's type as that sweep stores it: the second sweep forms its
+// neighbours' values on the fly
+template <class T, class TB>
+struct LdFirst {
+  const TB* __restrict__ b;
+  const T* __restrict__ dw;
+  __device__ __forceinline__ double operator()(uint32_t i) const {
+    return (double)(T)((double)ld32(b, i) * (double)ld32(dw, i));
+  }
+};
+// levels below this many unknowns are latency-bound: their kernels issue the
+// data loads before testing the solve's state
+constexpr uint32_t kMgLate = 1u << 18;
+
+// Damped Jacobi sweep xo = x + (b - A x) dw, dw = omega / d; FIRST: from
+// x = 0, xo = b dw, and bc (if given) receives b as FP32.  SWEEP0: FIRST and
+// one sweep in one pass (the neighbours' first-sweep values formed on the
+// fly, rounded as FIRST stores them); bc as for FIRST.  RES: r = b - A x.
+// The level's coefficients, x and dw are stored as T; b as TB, the output as
+// TO.  A sweep's b is the level's own (FP32 levels: rounded to FP32, as the
+// copy the later sweeps read).
 template <int KIND, class T, class TB, class TO>
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* __restrict__ rho,
                                                       const T* __restrict__ eta, const T* __restrict__ chi,
                                                       const T* __restrict__ x, const TB* __restrict__ b,
-                                                      const T* __restrict__ d, TO* __restrict__ out,
-                                                      float* __restrict__ bc, double omega, const KState* st) {
-  if (st && (st->done || st->early)) return;
-  const uint32_t q = tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
+                                                      const T* __restrict__ dw, TO* __restrict__ out,
+                                                      float* __restrict__ bc, const KState* st) {
   const uint32_t n = (uint32_t)g.nuv();
+  const bool late = n < kMgLate;
+  if (!late && st && (st->done || st->early)) return;
+  const uint32_t q = tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= n) return;
+  const TB bq = b[q];
+  const double bs = (double)(T)bq, w = (double)dw[q];
   if (KIND == MG_FIRST) {
-    const TB bq = b[q];
-    out[q] = (TO)(omega * (double)bq / (double)d[q]);
+    if (late && st && (st->done || st->early)) return;
+    out[q] = (TO)((double)bq * w);
     if (bc) bc[q] = (float)bq;
     return;
   }
-  const VeloInT<T> X{x, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
-  const double z = velo_row<2>(g, s, rho, eta, chi, X, q);
-  if (KIND == MG_RES)
-    out[q] = (TO)((double)b[q] - z);
-  else
-    out[q] = (TO)((double)x[q] + omega * ((double)b[q] - z) / (double)d[q]);
+  double z, xq;
+  if (KIND == MG_SWEEP0) {
+    const VeloAcc<LdFirst<T, TB>> X{{b, dw}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+    z = velo_row<2>(g, s, rho, eta, chi, X, q);
+    xq = X.ld(q);
+  } else {
+    const VeloInT<T> X{{x}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+    z = velo_row<2>(g, s, rho, eta, chi, X, q);
+    xq = (KIND == MG_RES) ? 0.0 : (double)x[q];
+  }
+  if (late && st && (st->done || st->early)) return;
+  if (KIND == MG_RES) {
+    out[q] = (TO)(bs - z);
+  } else {
+    out[q] = (TO)(xq + (bs - z) * w);
+    if (KIND == MG_SWEEP0 && bc) bc[q] = (float)bq;
+  }
 }
 
 // bc = R r = P^T r / 4 (one thread per coarse unknown)
 template <class TR, class TB>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const TR* __restrict__ r,
                                                         TB* __restrict__ bc, const KState* st) {
-  if (st && (st->done || st->early)) return;
+  const bool late = gf.nuv() < kMgLate;
+  if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nuv()) return;
   double acc = 0.0;
@@ -968,6 +1009,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
       }
     }
   }
+  if (late && st && (st->done || st->early)) return;
   bc[q] = (TB)(0.25 * acc);
 }
 
@@ -975,7 +1017,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
 template <class TC, class TF>
 __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const TC* __restrict__ xc,
                                                            TF* __restrict__ x, const KState* st) {
-  if (st && (st->done || st->early)) return;
+  const bool late = gf.nuv() < kMgLate;
+  if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gf.nuv()) return;
   double acc = 0.0;
@@ -1003,7 +1046,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, con
       }
     }
   }
-  x[q] = (TF)((double)x[q] + acc);
+  const double xq = (double)x[q];
+  if (late && st && (st->done || st->early)) return;
+  x[q] = (TF)(xq + acc);
 }
 
 // ---- host launchers -----------------------------------------------------
@@ -1134,42 +1179,44 @@ void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const
     MGC(double, double);
 #undef MGC
 }
-void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* d) {
+void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* dw,
+                    double omega) {
   const dim3 grid(nblk(g.nuv(), kBlock));
   if (f32)
     hipLaunchKernelGGL(k_mg_diag<float>, grid, dim3(kBlock), 0, st, g, s, (const float*)c.rho, (const float*)c.eta,
-                       (const float*)c.chi, (float*)d);
+                       (const float*)c.chi, (float*)dw, omega);
   else
     hipLaunchKernelGGL(k_mg_diag<double>, grid, dim3(kBlock), 0, st, g, s, (const double*)c.rho,
-                       (const double*)c.eta, (const double*)c.chi, (double*)d);
+                       (const double*)c.eta, (const double*)c.chi, (double*)dw, omega);
 }
 template <int K, class T, class TB, class TO>
 static void mg_smooth_t(hipStream_t st, const Grid& g, const Scal& s, const MgCoef& c, const void* x,
-                        const void* b, const void* d, void* out, float* bc, double omega, const KState* ks) {
+                        const void* b, const void* dw, void* out, float* bc, const KState* ks) {
   hipLaunchKernelGGL((k_mg_smooth<K, T, TB, TO>), dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s,
-                     (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)d,
-                     (TO*)out, bc, omega, ks);
+                     (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)dw,
+                     (TO*)out, bc, ks);
 }
 template <int K>
 static void mg_smooth_k(hipStream_t st, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                        const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
-                        double omega, const KState* ks) {
+                        const MgCoef& c, const void* x, const void* b, const void* dw, void* out, float* bc,
+                        const KState* ks) {
   if (!f32)  // an FP64 hierarchy is FP64 throughout
-    mg_smooth_t<K, double, double, double>(st, g, s, c, x, b, d, out, bc, omega, ks);
+    mg_smooth_t<K, double, double, double>(st, g, s, c, x, b, dw, out, bc, ks);
   else if (b32 && o32)
-    mg_smooth_t<K, float, float, float>(st, g, s, c, x, b, d, out, bc, omega, ks);
+    mg_smooth_t<K, float, float, float>(st, g, s, c, x, b, dw, out, bc, ks);
   else if (b32)  // the last sweep of level 0: FP64 output
-    mg_smooth_t<K, float, float, double>(st, g, s, c, x, b, d, out, bc, omega, ks);
+    mg_smooth_t<K, float, float, double>(st, g, s, c, x, b, dw, out, bc, ks);
   else  // the first sweep of level 0: FP64 b
-    mg_smooth_t<K, float, double, float>(st, g, s, c, x, b, d, out, bc, omega, ks);
+    mg_smooth_t<K, float, double, float>(st, g, s, c, x, b, dw, out, bc, ks);
 }
 void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
-                      const MgCoef& c, const void* x, const void* b, const void* d, void* out, float* bc,
-                      double omega, const KState* ks) {
+                      const MgCoef& c, const void* x, const void* b, const void* dw, void* out, float* bc,
+                      const KState* ks) {
   switch (kind) {
-    case MG_FIRST: mg_smooth_k<MG_FIRST>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
-    case MG_RES: mg_smooth_k<MG_RES>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
-    default: mg_smooth_k<MG_SWEEP>(st, f32, b32, o32, g, s, c, x, b, d, out, bc, omega, ks); break;
+    case MG_FIRST: mg_smooth_k<MG_FIRST>(st, f32, b32, o32, g, s, c, x, b, dw, out, bc, ks); break;
+    case MG_SWEEP0: mg_smooth_k<MG_SWEEP0>(st, f32, b32, o32, g, s, c, x, b, dw, out, bc, ks); break;
+    case MG_RES: mg_smooth_k<MG_RES>(st, f32, b32, o32, g, s, c, x, b, dw, out, bc, ks); break;
+    default: mg_smooth_k<MG_SWEEP>(st, f32, b32, o32, g, s, c, x, b, dw, out, bc, ks); break;
   }
 }
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
