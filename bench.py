@@ -395,7 +395,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
 
 # the kernel the roofline reports per mode: the one that takes the most time
 # in its steps (kernel id, name, prefix of its counter summary in profiles/)
-ROOF_SWEEP = ("K_PHMU_SWEEP32", "k_phmu_rb_sweep<FP32 middle sweep>", "k_phmu_rb_sweep32")
+ROOF_SWEEP = ("K_PHMU_SWEEP32", "k_phmu_rb_sweep<FP32 middle launch, 2 sweeps>", "k_phmu_rb_sweep32")
 ROOF_MATVEC = ("K_VELO_MATVEC", "k_velo_matvec<2,MV_DOT1>", "k_velo_matvec")
 
 

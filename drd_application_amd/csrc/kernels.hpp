@@ -214,13 +214,14 @@ struct KTimer {
   int cap;
   int* n;
 };
-constexpr int kRbgsMiddle = 6;  // middle sweeps per red-black application (all but the first and last)
+constexpr int kRbNS = 2;        // red-black sweeps per launch
+constexpr int kRbgsMiddle = 2;  // launches per red-black application besides the first and the last
 // c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
-// work = 6 ncell floats, and tm (optional) times the run of middle sweeps of
-// each application as one group of kRbgsMiddle launches
+// work = 6 ncell floats, and tm (optional) times the middle launches of each
+// application as one group of kRbgsMiddle launches
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm = nullptr);
-// one middle FP32-storage sweep, xi -> xo with the FP32 b (stand-alone timing)
+// one middle FP32 launch (kRbNS sweeps), xi -> xo with the FP32 b (stand-alone timing)
 void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
                             const float* xi, float* xo, const KState* ks);
 

@@ -701,31 +701,33 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 }
 
 // ---- red-black block Gauss-Seidel for the phi/mu system -------------------
-// One full sweep per launch: the red cells ((i + j) even), then the black
-// ones, each updated as x_c += B_c^-1 (b - A x)_c with the rows of the
+// NS full sweeps per launch: each the red cells ((i + j) even), then the
+// black ones, updated as x_c += B_c^-1 (b - A x)_c with the rows of the
 // matvec (phmu_rows_v) and the exact 2x2 block solve (phmu_block_v).
-// A workgroup stages its tile of kRbTX x kRbTY cells and a halo of two cells
-// (phi, mu, b, both mobilities, bu) in LDS, updates the red cells of the
-// tile and of a one-cell ring around it, then the black cells of the tile,
-// and writes the tile to xo.  A red cell's rows read only black neighbours
-// (and a black cell's only red ones), so each colour updates in place in
-// LDS; the ring's red cells belong to the neighbouring tiles, and every
-// workgroup that recomputes one does so from the same values of xi, so a
-// launch gives bit for bit the two colour half-sweeps.  xo must not alias
-// xi: a workgroup that starts after its neighbour has written would read
-// that neighbour's new values in its halo.  FIRST: from x = 0 (xi unread).
+// A workgroup stages its tile of kRbTX x kRbTY cells and a halo of 2 NS
+// cells (phi, mu, b, both mobilities, bu) in LDS.  Sweep k of NS updates the
+// red cells of the tile and a ring of 2 (NS - k) - 1 cells around it, then
+// the black cells of the tile and a ring one narrower; the last sweep's
+// black phase covers the tile alone, which is written out.  A red cell's
+// rows read only black neighbours (and a black cell's only red ones), so
+// each colour updates in place in LDS, and a ring cell's value after each
+// phase equals the one the tile that owns it computes, because every
+// workgroup recomputes it from the same values of xi.  A launch therefore
+// gives bit for bit 2 NS colour half-sweeps.  xo must not alias xi: a
+// workgroup that starts after its neighbour has written would read that
+// neighbour's new values in its halo.  FIRST: from x = 0 (xi unread).
 // The coefficients are stored as TC, b as TB, xi as TI and xo as TO; the
 // tile is held in LDS and computed in TC: FP64 (the reference's arithmetic),
 // or FP32 (the FP32 preconditioner: FP32 arithmetic with reciprocals), where
-// bc, if given, receives the FP32 b of the tile for the later sweeps.
-constexpr int kRbTX = 64, kRbTY = 8, kRbH = 2;
-constexpr int kRbW = kRbTX + 2 * kRbH, kRbHt = kRbTY + 2 * kRbH;
-template <bool FIRST, class TC, class TB, class TI, class TO>
+// bc, if given, receives the FP32 b of the tile for the later launches.
+constexpr int kRbTX = 64, kRbTY = 8;
+template <bool FIRST, int NS, class TC, class TB, class TI, class TO>
 __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
                                                           const TC* __restrict__ minner,
                                                           const TC* __restrict__ m0, const TB* __restrict__ b,
                                                           float* __restrict__ bc, const TI* __restrict__ xi,
                                                           TO* __restrict__ xo, const KState* st, int64_t bs) {
+  constexpr int kH = 2 * NS, kW = kRbTX + 2 * kH, kHt = kRbTY + 2 * kH;
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
   bu = bofs(bu, bs);
@@ -735,8 +737,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   bc = bofs(bc, bs);
   xi = bofs(xi, bs);
   xo = bofs(xo, bs);
-  __shared__ TC sx1[kRbHt][kRbW], sx2[kRbHt][kRbW];
-  __shared__ TC smi[kRbHt][kRbW], smz[kRbHt][kRbW], sbu[kRbHt][kRbW], sb1[kRbHt][kRbW], sb2[kRbHt][kRbW];
+  __shared__ TC sx1[kHt][kW], sx2[kHt][kW];
+  __shared__ TC smi[kHt][kW], smz[kHt][kW], sbu[kHt][kW], sb1[kHt][kW], sb2[kHt][kW];
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
   // tiles in row-major order, a contiguous band of them per XCD (tile_id),
@@ -746,12 +748,12 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   const int i0 = (t % ntx) * kRbTX, j0 = (t / ntx) * kRbTY;
   // staging: every load of the thread is issued before the first LDS store
   // (indices clamped into the grid; the stores keep the cells inside it)
-  constexpr int kStage = kRbW * kRbHt, kStageIt = (kStage + kBlock - 1) / kBlock;
+  constexpr int kStage = kW * kHt, kStageIt = (kStage + kBlock - 1) / kBlock;
   double v[kStageIt][7];
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
     const int e = min((int)threadIdx.x + it * kBlock, kStage - 1);
-    const int i = min(max(i0 - kRbH + e % kRbW, 0), nx - 1), j = min(max(j0 - kRbH + e / kRbW, 0), ny - 1);
+    const int i = min(max(i0 - kH + e % kW, 0), nx - 1), j = min(max(j0 - kH + e / kW, 0), ny - 1);
     const int64_t q = i + (int64_t)nx * j, c = g.ci(i, j);
     v[it][0] = FIRST ? 0.0 : (double)xi[q];
     v[it][1] = FIRST ? 0.0 : (double)xi[C + q];
@@ -764,8 +766,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
     const int e = (int)threadIdx.x + it * kBlock;
-    const int li = e % kRbW, lj = e / kRbW;
-    const int i = i0 - kRbH + li, j = j0 - kRbH + lj;
+    const int li = e % kW, lj = e / kW;
+    const int i = i0 - kH + li, j = j0 - kH + lj;
     if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
     sx1[lj][li] = (TC)v[it][0];
     sx2[lj][li] = (TC)v[it][1];
@@ -779,8 +781,8 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   auto update = [&](int i, int j) {
     const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
     const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
-    const int a = i - i0 + kRbH, al = il - i0 + kRbH, ar = ir - i0 + kRbH;
-    const int r = j - j0 + kRbH, rl = jl - j0 + kRbH, rr = jr - j0 + kRbH;
+    const int a = i - i0 + kH, al = il - i0 + kH, ar = ir - i0 + kH;
+    const int r = j - j0 + kH, rl = jl - j0 + kH, rr = jr - j0 + kH;
     const Nb5T<TC> mi{smi[r][a], smi[r][al], smi[r][ar], smi[rl][a], smi[rr][a]};
     const Nb5T<TC> mz{smz[r][a], smz[r][al], smz[r][ar], smz[rl][a], smz[rr][a]};
     const Nb5T<TC> f1{sx1[r][a], sx1[r][al], sx1[r][ar], sx1[rl][a], sx1[rr][a]};
@@ -792,33 +794,32 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     sx1[r][a] = f1.c + y1;
     sx2[r][a] = f2.c + y2;
   };
-  // threads take the cells of one colour in row order: a row of the red
-  // region (tile and ring, kRbTX + 2 wide) holds (kRbTX + 2) / 2 red cells, a
-  // tile row kRbTX / 2 black ones
-  constexpr int kRedRow = (kRbTX + 2) / 2, kBlackRow = kRbTX / 2;
-  for (int e = threadIdx.x; e < kRedRow * (kRbTY + 2); e += kBlock) {
-    const int j = j0 - 1 + e / kRedRow;
-    const int i = i0 - 1 + 2 * (e % kRedRow) + ((i0 - 1 + j) & 1);
-    if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
-    update(i, j);
+  // threads take the cells of one colour in row order: a row of a region
+  // kRbTX + 2 w wide holds (kRbTX + 2 w) / 2 cells of each colour
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+#pragma unroll
+    for (int colour = 0; colour < 2; ++colour) {
+      const int w = 2 * (NS - k) - 1 - colour;  // ring width of this phase
+      const int rowc = (kRbTX + 2 * w) / 2, rows = kRbTY + 2 * w;
+      for (int e = threadIdx.x; e < rowc * rows; e += kBlock) {
+        const int j = j0 - w + e / rowc;
+        const int i = i0 - w + 2 * (e % rowc) + ((i0 - w + j + colour) & 1);
+        if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
+        update(i, j);
+      }
+      __syncthreads();
+    }
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kBlackRow * kRbTY; e += kBlock) {
-    const int j = j0 + e / kBlackRow;
-    const int i = i0 + 2 * (e % kBlackRow) + ((i0 + j + 1) & 1);
-    if (i >= nx || j >= ny) continue;
-    update(i, j);
-  }
-  __syncthreads();
   for (int e = threadIdx.x; e < kRbTX * kRbTY; e += kBlock) {
     const int i = i0 + e % kRbTX, j = j0 + e / kRbTX;
     if (i >= nx || j >= ny) continue;
     const int64_t q = i + (int64_t)nx * j;
-    xo[q] = (TO)sx1[j - j0 + kRbH][i - i0 + kRbH];
-    xo[C + q] = (TO)sx2[j - j0 + kRbH][i - i0 + kRbH];
+    xo[q] = (TO)sx1[j - j0 + kH][i - i0 + kH];
+    xo[C + q] = (TO)sx2[j - j0 + kH][i - i0 + kH];
     if (bc) {
-      bc[q] = (float)sb1[j - j0 + kRbH][i - i0 + kRbH];
-      bc[C + q] = (float)sb2[j - j0 + kRbH][i - i0 + kRbH];
+      bc[q] = (float)sb1[j - j0 + kH][i - i0 + kH];
+      bc[C + q] = (float)sb2[j - j0 + kH][i - i0 + kH];
     }
   }
 }
@@ -1246,27 +1247,29 @@ void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, c
                        (double*)x, ks);
 }
 
-// x = the red-black block Gauss-Seidel preconditioner applied to b:
-// kRbgsSweeps full sweeps (red cells, then black), from x = 0, alternating
-// between work and x so that the last writes x.  FP64: work holds 2 ncell
-// doubles.  FP32 storage (c32 = FP32 bu, M_inner, M_0): work holds 6 ncell
-// floats, two FP32 iterates and the FP32 b the first sweep writes.
-constexpr int kRbgsSweeps = kRbgsMiddle + 2;
-static_assert(kRbgsSweeps % 2 == 0, "the last sweep must write x");
 static dim3 rb_grid(const Grid& g) {  // one workgroup per tile (tile_id)
   return dim3((unsigned)(((g.nx + kRbTX - 1) / kRbTX) * ((g.ny + kRbTY - 1) / kRbTY)), 1, bat_n());
 }
 void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
                             const float* xi, float* xo, const KState* ks) {
-  hipLaunchKernelGGL((k_phmu_rb_sweep<false, float, float, float, float>), rb_grid(g), dim3(kBlock), 0, st, g, s,
-                     c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs());
+  hipLaunchKernelGGL((k_phmu_rb_sweep<false, kRbNS, float, float, float, float>), rb_grid(g), dim3(kBlock), 0, st,
+                     g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs());
 }
+// x = the red-black block Gauss-Seidel preconditioner applied to b:
+// eight full sweeps (red cells, then black), from x = 0, kRbNS per launch,
+// alternating between work and x so that the last launch writes x.
+// FP64: work holds 2 ncell doubles.  FP32 storage (c32 = FP32 bu, M_inner,
+// M_0): work holds 6 ncell floats, two FP32 iterates and the FP32 b the
+// first launch writes.
+constexpr int kRbgsLaunches = kRbgsMiddle + 2;
+static_assert(kRbgsLaunches % 2 == 0, "the last launch must write x");
+static_assert(kRbgsLaunches * kRbNS == 8, "eight sweeps per application");
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm) {
   const dim3 grid = rb_grid(g);
-#define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                              \
-  hipLaunchKernelGGL((k_phmu_rb_sweep<F, TC, TB, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu, CO->minner, \
-                     CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
+#define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                                 \
+  hipLaunchKernelGGL((k_phmu_rb_sweep<F, kRbNS, TC, TB, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu,         \
+                     CO->minner, CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
   // FP32 storage: two iterates and the FP32 b
   const int64_t n2 = 2 * g.ncell();
   float* wa = (float*)work;
@@ -1277,7 +1280,7 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
     double* w = (double*)work;
     const Coef* co = &c;
     RB(true, double, double, double, double, co, b, nullptr, nullptr, w);
-    for (int k = 1; k < kRbgsSweeps; ++k) {
+    for (int k = 1; k < kRbgsLaunches; ++k) {
       if (k & 1)
         RB(false, double, double, double, double, co, b, nullptr, w, x);
       else
@@ -1285,18 +1288,18 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
     }
   } else {
     RB(true, float, double, float, float, c32, b, bf, nullptr, wa);
-    // the middle sweeps, timed as one group while tm has event pairs left:
+    // the middle launches, timed as one group while tm has event pairs left:
     // markers on the stream, so each launch runs as it does untimed
     const int tk = (tm && *tm->n < tm->cap) ? (*tm->n)++ : -1;
     if (tk >= 0) (void)hipEventRecord(tm->ev[2 * tk], st);
-    for (int k = 1; k + 1 < kRbgsSweeps; ++k) {
+    for (int k = 1; k + 1 < kRbgsLaunches; ++k) {
       if (k & 1)
         mid(wa, wb);
       else
         mid(wb, wa);
     }
     if (tk >= 0) (void)hipEventRecord(tm->ev[2 * tk + 1], st);
-    RB(false, float, float, float, double, c32, bf, nullptr, wa, x);  // sweep kRbgsSweeps - 2 (even) wrote wa
+    RB(false, float, float, float, double, c32, bf, nullptr, wa, x);  // launch kRbgsLaunches - 2 (even) wrote wa
   }
 #undef RB
 }
