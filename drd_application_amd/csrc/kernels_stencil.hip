@@ -720,9 +720,9 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 // tile is held in LDS and computed in TC: FP64 (the reference's arithmetic),
 // or FP32 (the FP32 preconditioner: FP32 arithmetic with reciprocals), where
 // bc, if given, receives the FP32 b of the tile for the later launches.
-constexpr int kRbTX = 64, kRbTY = 8;
+constexpr int kRbTX = 64, kRbTY = 16, kRbThreads = 1024;  // a thread per cell of the widest phase
 template <bool FIRST, int NS, class TC, class TB, class TI, class TO>
-__global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
+__global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
                                                           const TC* __restrict__ minner,
                                                           const TC* __restrict__ m0, const TB* __restrict__ b,
                                                           float* __restrict__ bc, const TI* __restrict__ xi,
@@ -748,11 +748,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   const int i0 = (t % ntx) * kRbTX, j0 = (t / ntx) * kRbTY;
   // staging: every load of the thread is issued before the first LDS store
   // (indices clamped into the grid; the stores keep the cells inside it)
-  constexpr int kStage = kW * kHt, kStageIt = (kStage + kBlock - 1) / kBlock;
+  constexpr int kStage = kW * kHt, kStageIt = (kStage + kRbThreads - 1) / kRbThreads;
   double v[kStageIt][7];
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
-    const int e = min((int)threadIdx.x + it * kBlock, kStage - 1);
+    const int e = min((int)threadIdx.x + it * kRbThreads, kStage - 1);
     const int i = min(max(i0 - kH + e % kW, 0), nx - 1), j = min(max(j0 - kH + e / kW, 0), ny - 1);
     const int64_t q = i + (int64_t)nx * j, c = g.ci(i, j);
     v[it][0] = FIRST ? 0.0 : (double)xi[q];
@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
   }
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
-    const int e = (int)threadIdx.x + it * kBlock;
+    const int e = (int)threadIdx.x + it * kRbThreads;
     const int li = e % kW, lj = e / kW;
     const int i = i0 - kH + li, j = j0 - kH + lj;
     if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
     for (int colour = 0; colour < 2; ++colour) {
       const int w = 2 * (NS - k) - 1 - colour;  // ring width of this phase
       const int rowc = (kRbTX + 2 * w) / 2, rows = kRbTY + 2 * w;
-      for (int e = threadIdx.x; e < rowc * rows; e += kBlock) {
+      for (int e = threadIdx.x; e < rowc * rows; e += kRbThreads) {
         const int j = j0 - w + e / rowc;
         const int i = i0 - w + 2 * (e % rowc) + ((i0 - w + j + colour) & 1);
         if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_phmu_rb_sweep(Grid g, Scal s, const 
       __syncthreads();
     }
   }
-  for (int e = threadIdx.x; e < kRbTX * kRbTY; e += kBlock) {
+  for (int e = threadIdx.x; e < kRbTX * kRbTY; e += kRbThreads) {
     const int i = i0 + e % kRbTX, j = j0 + e / kRbTX;
     if (i >= nx || j >= ny) continue;
     const int64_t q = i + (int64_t)nx * j;
@@ -1252,7 +1252,7 @@ static dim3 rb_grid(const Grid& g) {  // one workgroup per tile (tile_id)
 }
 void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
                             const float* xi, float* xo, const KState* ks) {
-  hipLaunchKernelGGL((k_phmu_rb_sweep<false, kRbNS, float, float, float, float>), rb_grid(g), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL((k_phmu_rb_sweep<false, kRbNS, float, float, float, float>), rb_grid(g), dim3(kRbThreads), 0, st,
                      g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs());
 }
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
@@ -1268,7 +1268,7 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm) {
   const dim3 grid = rb_grid(g);
 #define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                                 \
-  hipLaunchKernelGGL((k_phmu_rb_sweep<F, kRbNS, TC, TB, TI, TO>), grid, dim3(kBlock), 0, st, g, s, CO->bu,         \
+  hipLaunchKernelGGL((k_phmu_rb_sweep<F, kRbNS, TC, TB, TI, TO>), grid, dim3(kRbThreads), 0, st, g, s, CO->bu,         \
                      CO->minner, CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
   // FP32 storage: two iterates and the FP32 b
   const int64_t n2 = 2 * g.ncell();
