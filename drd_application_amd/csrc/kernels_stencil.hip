@@ -987,27 +987,39 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nuv()) return;
-  double acc = 0.0;
-  if (q < gc.mn()) {  // u (I, K): fine faces 2I-1..2I+1, rows 2K-1..2K+2
+  // the weights of mg_w_face / mg_w_cell_odd / mg_w_cell_even in closed
+  // form; loads are clamped into the grid and their weights zeroed instead
+  double acc;
+  if (q < gc.mn()) {  // u (I, K): fine faces 2I-1..2I+1 (1/2, 1, 1/2), rows 2K-1..2K+2
+    const int ldf = gf.nx + 1;
     const int K = (int)(q / (gc.nx + 1)), I = (int)(q - (int64_t)K * (gc.nx + 1));
-    for (int k = 2 * K - 1; k <= 2 * K + 2; ++k) {
-      if (k < 0 || k > gf.ny - 1) continue;
-      const double wy = mg_w_cell_odd(k, K, gc.ny);
-      if (wy == 0.0) continue;
-      for (int i = 2 * I - 1; i <= 2 * I + 1; ++i) {
-        if (i < 0 || i > gf.nx) continue;
-        acc += mg_w_face(i, I) * wy * (double)r[i + (int64_t)(gf.nx + 1) * k];
-      }
+    const bool lo = K >= 1, hi = K + 1 <= gc.ny - 1;  // the neighbouring coarse rows exist
+    const double wy[4] = {lo ? 0.25 : 0.0, lo ? 0.75 : 0.5, hi ? 0.75 : 0.5, hi ? 0.25 : 0.0};
+    const int il = I > 0 ? 2 * I - 1 : 0, ir = 2 * I + 1 <= gf.nx ? 2 * I + 1 : gf.nx;
+    const double wl = I > 0 ? 0.5 : 0.0, wr = 2 * I + 1 <= gf.nx ? 0.5 : 0.0;
+    acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      int k = 2 * K - 1 + m;
+      k = k < 0 ? 0 : (k > gf.ny - 1 ? gf.ny - 1 : k);
+      const int64_t row = (int64_t)ldf * k;
+      const double sx = wl * (double)r[il + row] + (double)r[2 * I + row] + wr * (double)r[ir + row];
+      acc += wy[m] * sx;
     }
-  } else {  // v (I, K), K = 1..ny-1: fine inner faces 2K-1..2K+1, cells 2I-1..2I+2
+  } else {  // v (I, K), K = 1..ny-1: fine inner faces 2K-1..2K+1 (1/2, 1, 1/2), cells 2I-1..2I+2
     const int64_t t = q - gc.mn();
     const int K = (int)(t / gc.nx) + 1, I = (int)(t - (int64_t)(K - 1) * gc.nx);
-    for (int k = 2 * K - 1; k <= 2 * K + 1; ++k) {
-      const double wy = mg_w_face(k, K);
-      for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
-        if (i < 0 || i > gf.nx - 1) continue;
-        acc += mg_w_cell_even(i, I, gc.nx) * wy * (double)r[gf.mn() + i + (int64_t)gf.nx * (k - 1)];
-      }
+    const bool left = I >= 1, right = I + 1 <= gc.nx - 1;  // the neighbouring coarse cells exist
+    const double wx[4] = {left ? 0.25 : 0.0, left ? 0.75 : 1.0, right ? 0.75 : 1.0, right ? 0.25 : 0.0};
+    const int i0 = left ? 2 * I - 1 : 0, i3 = right ? 2 * I + 2 : gf.nx - 1;
+    const double wy[3] = {0.5, 1.0, 0.5};
+    acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t row = gf.mn() + (int64_t)gf.nx * (2 * K - 2 + m);  // fine face k = 2K-1+m
+      const double sx = wx[0] * (double)r[i0 + row] + wx[1] * (double)r[2 * I + row] +
+                        wx[2] * (double)r[2 * I + 1 + row] + wx[3] * (double)r[i3 + row];
+      acc += wy[m] * sx;
     }
   }
   if (late && st && (st->done || st->early)) return;
@@ -1022,30 +1034,37 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, con
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gf.nuv()) return;
-  double acc = 0.0;
+  // closed-form weights as in k_mg_restrict
+  double acc;
   if (q < gf.mn()) {  // u (i, k): coarse faces i/2 (and i/2 + 1), rows k/2 and its neighbour
+    const int ldc = gc.nx + 1;
     const int k = (int)(q / (gf.nx + 1)), i = (int)(q - (int64_t)k * (gf.nx + 1));
-    const int I0 = i >> 1, I1 = (i + 1) >> 1;
+    const int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0;  // odd i: 1/2 each; even i: I0 alone
+    const double wx = (i & 1) ? 0.5 : 1.0;
     const int K0 = k >> 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
-    for (int K = K0 - 1; K <= K0 + 1; ++K) {
-      if (K < 0 || K > gc.ny - 1 || (K != K0 && K != K1)) continue;
-      const double wy = mg_w_cell_odd(k, K, gc.ny);
-      for (int I = I0; I <= I1; ++I) acc += mg_w_face(i, I) * wy * (double)xc[I + (int64_t)(gc.nx + 1) * K];
-    }
+    const bool in1 = K1 >= 0 && K1 <= gc.ny - 1;
+    const int K1c = in1 ? K1 : K0;
+    const double w0 = in1 ? 0.75 : 0.5, w1 = in1 ? 0.25 : 0.0;
+    const double v0 = (double)xc[I0 + (int64_t)ldc * K0], u0 = (double)xc[I1 + (int64_t)ldc * K0];
+    const double v1 = (double)xc[I0 + (int64_t)ldc * K1c], u1 = (double)xc[I1 + (int64_t)ldc * K1c];
+    const double s0 = (i & 1) ? wx * v0 + wx * u0 : v0, s1 = (i & 1) ? wx * v1 + wx * u1 : v1;
+    acc = w0 * s0 + w1 * s1;
   } else {  // v (i, k), k = 1..ny-1: coarse inner faces k/2 (and k/2 + 1), cells i/2 and its neighbour
     const int64_t t = q - gf.mn();
     const int k = (int)(t / gf.nx) + 1, i = (int)(t - (int64_t)(k - 1) * gf.nx);
-    const int K0 = k >> 1, K1 = (k + 1) >> 1;
-    const int Ic = i >> 1;
-    for (int K = K0; K <= K1; ++K) {
-      if (K < 1 || K > gc.ny - 1) continue;
-      const double wy = mg_w_face(k, K);
-      for (int I = Ic - 1; I <= Ic + 1; ++I) {
-        if (I < 0 || I > gc.nx - 1) continue;
-        const double wx = mg_w_cell_even(i, I, gc.nx);
-        if (wx != 0.0) acc += wx * wy * (double)xc[gc.mn() + I + (int64_t)gc.nx * (K - 1)];
-      }
-    }
+    // faces: even k -> K = k/2 (1); odd k -> k>>1 and k>>1 + 1 (1/2 each), wall faces (0, ny) dropped
+    const int Ka = k >> 1, Kb = (k & 1) ? Ka + 1 : Ka;
+    const bool ina = Ka >= 1 && Ka <= gc.ny - 1, inb = Kb >= 1 && Kb <= gc.ny - 1;
+    const double wa = ina ? ((k & 1) ? 0.5 : 1.0) : 0.0, wb = ((k & 1) && inb) ? 0.5 : 0.0;
+    const int Kac = ina ? Ka : 1, Kbc = inb ? Kb : 1;
+    // cells: c = i/2 (3/4), its neighbour (1/4), clamped (the weights add where they meet)
+    const int c = i >> 1;
+    int o = (i & 1) ? c + 1 : c - 1;
+    o = o < 0 ? 0 : (o > gc.nx - 1 ? gc.nx - 1 : o);
+    const int64_t ra = gc.mn() + (int64_t)gc.nx * (Kac - 1), rb = gc.mn() + (int64_t)gc.nx * (Kbc - 1);
+    const double sa = 0.75 * (double)xc[c + ra] + 0.25 * (double)xc[o + ra];
+    const double sb = 0.75 * (double)xc[c + rb] + 0.25 * (double)xc[o + rb];
+    acc = wa * sa + wb * sb;
   }
   const double xq = (double)x[q];
   if (late && st && (st->done || st->early)) return;
