@@ -106,6 +106,7 @@ struct Scal {
   int phi_mode;
   // quotients of the above that the CH stencils use (scal_derive)
   double idr2, idz2, s_I, I_dz2, floor22;
+  double idrdz;  // 1 / (dr dz): the FP32 velocity stencil multiplies by it
 };
 
 // The stencils' uniform quotients, each the correctly rounded value of the
@@ -114,6 +115,7 @@ struct Scal {
 __host__ __device__ inline void scal_derive(Scal& s) {
   s.idr2 = 1.0 / s.dr2;
   s.idz2 = 1.0 / s.dz2;
+  s.idrdz = 1.0 / (s.dr * s.dz);
   s.s_I = s.s / s.I;
   s.I_dz2 = s.I / s.dz2;
   s.floor22 = 1e-3 * s.dt * s.xld * (2.0 / s.dr2 + 2.0 / s.dz2);

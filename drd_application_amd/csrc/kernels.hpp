@@ -229,8 +229,8 @@ void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const 
 enum { MG_FIRST = 0, MG_SWEEP = 1, MG_RES = 2, MG_SWEEP0 = 3 };
 struct VeloMG;
 // f32: the levels' coefficients, diagonals and vectors are stored in FP32
-// (arithmetic stays FP64); the input b, the output z and the coarsest level
-// stay FP64
+// and their stencils evaluated in FP32; the input b, the output z and the
+// coarsest level stay FP64
 VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err);
 void velo_mg_destroy(VeloMG* mg);
 bool velo_mg_f32(const VeloMG* mg);

@@ -15,8 +15,9 @@
 //
 // FP32 storage: every level but the coarsest keeps its coefficients,
 // diagonal and vectors in FP32 (level 0 an FP32 copy of the context's
-// fields), which halves the bytes of each sweep; the arithmetic, the input b,
-// the output z and the coarsest level stay FP64.
+// fields), which halves the bytes of each sweep, and evaluates its stencil
+// in FP32; the input b, the output z, each sweep's update and the coarsest
+// level stay FP64.
 #include <vector>
 
 #include "kernels.hpp"

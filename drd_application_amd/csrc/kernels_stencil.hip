@@ -19,34 +19,36 @@ namespace drd {
 // double): a stored FP64 or FP32 vector (LdVec), or values formed on the fly.
 template <class T>
 struct LdVec {
+  using R = T;  // the arithmetic type of values read through it
   const T* __restrict__ x;
-  __device__ __forceinline__ double operator()(uint32_t i) const { return ldd(x, i); }
+  __device__ __forceinline__ T operator()(uint32_t i) const { return ld32(x, i); }
 };
 template <class L>
 struct VeloAcc {
+  using R = typename L::R;
   L ld;
   int nx, ny, lr_d;
   uint32_t mn;
-  __device__ __forceinline__ double uraw(int i, int k) const { return ld((uint32_t)(i + (nx + 1) * k)); }
+  __device__ __forceinline__ R uraw(int i, int k) const { return ld((uint32_t)(i + (nx + 1) * k)); }
   // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
   // wall ghosts are odd mirrors of the first/last row, side ghosts even or
   // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
   // commute; both are applied branch-free.
-  __device__ __forceinline__ double U(int i, int k) const {
+  __device__ __forceinline__ R U(int i, int k) const {
     const bool kw = (k < 0) | (k >= ny);
     const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
     const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
     const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
-    const double v = ld((uint32_t)(ii + (nx + 1) * kk));
-    const double w = iw ? -v : v;
+    const R v = ld((uint32_t)(ii + (nx + 1) * kk));
+    const R w = iw ? -v : v;
     return kw ? -w : w;
   }
   // f2(i,k) for -1<=i<=nx, 0<=k<=ny (0 on the walls k = 0, ny)
-  __device__ __forceinline__ double V(int i, int k) const {
+  __device__ __forceinline__ R V(int i, int k) const {
     const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
     const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
-    const double v = ld(mn + (uint32_t)(ic + nx * (kc - 1)));
-    return (k <= 0 || k >= ny) ? 0.0 : v;
+    const R v = ld(mn + (uint32_t)(ic + nx * (kc - 1)));
+    return (k <= 0 || k >= ny) ? R(0.0) : v;
   }
 };
 template <class T>
@@ -56,65 +58,74 @@ using VeloIn = VeloInT<double>;
 // y = A x for one unknown q of the velocity operator.
 // MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
 // MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
+// Computed in the coefficients' type T: FP64 is the reference's arithmetic;
+// FP32 (the FP32 multigrid levels) multiplies by the reciprocals of the
+// spacings where the reference divides.  The vector's loader reads T.
+template <class T>
+__device__ __forceinline__ T vdiv(T a, const Scal& s, double d, double inv) {
+  return sizeof(T) == sizeof(float) ? a * T(inv) : a / T(d);
+}
 template <int BDF, class T, class XA>
-__device__ __forceinline__ double velo_u_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                             const T* __restrict__ eta, const T* __restrict__ chi,
-                                             const XA& X, uint32_t q) {
+__device__ __forceinline__ T velo_u_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                        const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
+                                        uint32_t q) {
   const int nx = g.nx;
-  double r;
+  T r;
   {
     const int k = g.by_nx1.div(q);
     const int i = (int)(q - (uint32_t)k * (uint32_t)(nx + 1));
     const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
-    const double f = X.U(i, k);
+    const T f = X.U(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (ldd(rho, c) + ldd(rho, cw)) / 2.0 +
-          f * s.penalty_k * (ldd(chi, c) + ldd(chi, cw)) / 2.0 * s.dt;
+      r = f * T(1.5) * T(s.re) * (ld32(rho, c) + ld32(rho, cw)) / T(2.0) +
+          f * T(s.penalty_k) * (ld32(chi, c) + ld32(chi, cw)) / T(2.0) * T(s.dt);
     else
-      r = f * s.re * (ldd(rho, c) + ldd(rho, cw)) / 2.0;
-    const double e00 = ldd(eta, c), e10 = ldd(eta, cw);
-    double cr = e00, cl = e10;
-    r = r - 2.0 * s.dt * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f) / s.dr2;
-    cr = (ldd(eta, c + ld) + ldd(eta, cw + ld) + e00 + e10) / 4.0;
-    cl = (e00 + e10 + ldd(eta, c - ld) + ldd(eta, cw - ld)) / 4.0;
-    r = r - s.dt * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f) / s.dz2;
-    r = r - s.dt * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k))) / s.dr / s.dz;
+      r = f * T(s.re) * (ld32(rho, c) + ld32(rho, cw)) / T(2.0);
+    const T e00 = ld32(eta, c), e10 = ld32(eta, cw);
+    T cr = e00, cl = e10;
+    r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
+    cr = (ld32(eta, c + ld) + ld32(eta, cw + ld) + e00 + e10) / T(4.0);
+    cl = (e00 + e10 + ld32(eta, c - ld) + ld32(eta, cw - ld)) / T(4.0);
+    r = r - vdiv(T(s.dt) * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
+    const T cross = T(s.dt) * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k)));
+    r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
   }
   return r;
 }
 
 template <int BDF, class T, class XA>
-__device__ __forceinline__ double velo_v_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                             const T* __restrict__ eta, const T* __restrict__ chi,
-                                             const XA& X, uint32_t q) {
+__device__ __forceinline__ T velo_v_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                        const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
+                                        uint32_t q) {
   const int nx = g.nx;
-  double r;
+  T r;
   {
     const uint32_t t = q - X.mn;
     const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (uint32_t)(k - 1) * (uint32_t)nx);
     const uint32_t c = g.cio(i, k), ld = (uint32_t)(nx + 2), cs = c - ld;
-    const double f = X.V(i, k);
+    const T f = X.V(i, k);
     if (BDF == 2)
-      r = f * 1.5 * s.re * (ldd(rho, c) + ldd(rho, cs)) / 2.0 +
-          f * s.penalty_k * (ldd(chi, c) + ldd(chi, cs)) / 2.0 * s.dt;
+      r = f * T(1.5) * T(s.re) * (ld32(rho, c) + ld32(rho, cs)) / T(2.0) +
+          f * T(s.penalty_k) * (ld32(chi, c) + ld32(chi, cs)) / T(2.0) * T(s.dt);
     else
-      r = f * s.re * (ldd(rho, c) + ldd(rho, cs)) / 2.0;
-    const double e00 = ldd(eta, c), e01 = ldd(eta, cs);
-    double cr = e00, cl = e01;
-    r = r - 2.0 * s.dt * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f) / s.dz2;
-    cr = (ldd(eta, c + 1) + e00 + ldd(eta, cs + 1) + e01) / 4.0;
-    cl = (e00 + ldd(eta, c - 1) + e01 + ldd(eta, cs - 1)) / 4.0;
-    r = r - s.dt * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f) / s.dr2;
-    r = r - s.dt * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1))) / s.dr / s.dz;
+      r = f * T(s.re) * (ld32(rho, c) + ld32(rho, cs)) / T(2.0);
+    const T e00 = ld32(eta, c), e01 = ld32(eta, cs);
+    T cr = e00, cl = e01;
+    r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
+    cr = (ld32(eta, c + 1) + e00 + ld32(eta, cs + 1) + e01) / T(4.0);
+    cl = (e00 + ld32(eta, c - 1) + e01 + ld32(eta, cs - 1)) / T(4.0);
+    r = r - vdiv(T(s.dt) * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
+    const T cross = T(s.dt) * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1)));
+    r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
   }
   return r;
 }
 
 template <int BDF, class T, class XA>
-__device__ __forceinline__ double velo_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                           const T* __restrict__ eta, const T* __restrict__ chi,
-                                           const XA& X, uint32_t q) {
+__device__ __forceinline__ T velo_row(const Grid g, const Scal& s, const T* __restrict__ rho,
+                                      const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
+                                      uint32_t q) {
   return q < X.mn ? velo_u_row<BDF, T, XA>(g, s, rho, eta, chi, X, q)
                    : velo_v_row<BDF, T, XA>(g, s, rho, eta, chi, X, q);
 }
@@ -924,10 +935,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __r
 // neighbours' values on the fly
 template <class T, class TB>
 struct LdFirst {
+  using R = T;
   const TB* __restrict__ b;
   const T* __restrict__ dw;
-  __device__ __forceinline__ double operator()(uint32_t i) const {
-    return (double)(T)((double)ld32(b, i) * (double)ld32(dw, i));
+  __device__ __forceinline__ T operator()(uint32_t i) const {
+    return (T)((double)ld32(b, i) * (double)ld32(dw, i));
   }
 };
 // levels below this many unknowns are latency-bound: their kernels issue the

@@ -391,8 +391,9 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  * take it.
  * DRD_PS_VELO_MG32 / DRD_PREC_MG32: the same cycle with every level but the
  * coarsest stored in FP32 (coefficients, diagonals, vectors; level 0 holds an
- * FP32 copy of rho, eta, chi), which halves the bytes of each sweep.  The
- * arithmetic, b, z and the coarsest level stay FP64.                         */
+ * FP32 copy of rho, eta, chi), which halves the bytes of each sweep, and its
+ * stencils computed in FP32 (reciprocals where the FP64 form divides).  b,
+ * z, each sweep's update and the coarsest level stay FP64.                   */
 #define DRD_PREC_MG 6
 #define DRD_PS_VELO_MG 9
 #define DRD_PREC_MG32 7

@@ -10,7 +10,7 @@
   solution as that solve.
 * Time steps with velo_prec = DRD_PREC_MG track the oracle's steps (reference
   preconditioners) as the other fast paths do.
-* The FP32-storage cycle (DRD_PS_VELO_MG32): one V-cycle equals the FP64
+* The FP32 cycle (DRD_PS_VELO_MG32, FP32 storage and stencils): one V-cycle equals the FP64
   construction to a relative 1e-6 (each stored value rounds to 6e-8
   relative; a cycle reads and writes each level's vectors a few times and
   damps, not amplifies, what it reads), BiCGSTAB with it converges to the
