@@ -163,7 +163,9 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
     const void* b0 = copy ? (const void*)b : rhs(l);
     float* bc = copy ? (float*)L.b : nullptr;
     int k = 1;
-    if (n1 >= 2) {  // the first two sweeps in one pass
+    // the first two sweeps in one pass, except on the FP32 hierarchy's
+    // level 0, where the pass would read the FP64 b at every stencil point
+    if (n1 >= 2 && !copy) {
       cur = other(l, cur);
       launch_mg_smooth(st, MG_SWEEP0, f, f && !copy, f, L.g, L.s, L.c, nullptr, b0, L.d, cur, bc, ks);
       k = 2;

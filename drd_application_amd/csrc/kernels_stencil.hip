@@ -930,17 +930,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __r
   if (q < g.nuv()) dw[q] = (T)(omega / velo_diag2(g, s, rho, eta, chi, q));
 }
 
-// x of the first sweep (x = omega b / d from 0) at packed element i, rounded
-// to the level's type as that sweep stores it: the second sweep forms its
-// neighbours' values on the fly
+// x of the first sweep (x = omega b / d from 0) at packed element i, in the
+// level's type: the second sweep forms its neighbours' values on the fly
 template <class T, class TB>
 struct LdFirst {
   using R = T;
   const TB* __restrict__ b;
   const T* __restrict__ dw;
-  __device__ __forceinline__ T operator()(uint32_t i) const {
-    return (T)((double)ld32(b, i) * (double)ld32(dw, i));
-  }
+  __device__ __forceinline__ T operator()(uint32_t i) const { return (T)ld32(b, i) * ld32(dw, i); }
 };
 // levels below this many unknowns are latency-bound: their kernels issue the
 // data loads before testing the solve's state
