@@ -42,11 +42,16 @@ def test_batch_groups_and_field_round_trip():
                 np.testing.assert_array_equal(b.get_field(i, fid), cfg.fields[fid])
 
 
-@pytest.mark.parametrize("steps", [4])
-def test_batch_steps_are_bitwise_the_single_instance_steps(steps):
+@pytest.mark.parametrize("steps,ch", [(4, None), (3, "CH_PREC_RBGS32"), (3, "CH_PREC_RBGS")])
+def test_batch_steps_are_bitwise_the_single_instance_steps(steps, ch):
+    """ch: the CH preconditioner for every instance (None: the configs' own);
+    the red-black forms keep their iterates in a per-instance scratch."""
     from drd_application_amd.solver import GpuBatch, GpuSolver
     cfgs = _configs()
-    print("batch of", [c.notes for c in cfgs], flush=True)
+    if ch is not None:
+        for c in cfgs:
+            c.params.ch_prec = getattr(abi, ch)
+    print("batch of", [c.notes for c in cfgs], "ch", ch, flush=True)
     with GpuBatch([c.params for c in cfgs]) as b:
         _load_batch(b, cfgs)
         tuned = b.tune()
