@@ -294,14 +294,17 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     chunks replayed from captured hipGraphs, transpose tiles autotuned.  The
     sequential figure steps up to `sample` instances of each size alone (each
     its own context, no graphs) and scales their mean by the number of
-    instances of that size."""
+    instances of that size.  The CH preconditioner is the FP32 red-black one
+    (the configs' own choice is block Jacobi)."""
     import numpy as np
 
-    from drd_application_amd import geometry
+    from drd_application_amd import abi, geometry
     from drd_application_amd.solver import GpuBatch, GpuSolver
 
     t0 = time.perf_counter()
     cfgs = geometry.batch_configs(count)
+    for c in cfgs:
+        c.params.ch_prec = abi.CH_PREC_RBGS32
     gen_s = time.perf_counter() - t0
     cells = sum(c.params.nx * c.params.ny for c in cfgs)
     dt = cfgs[0].params.dt
@@ -382,7 +385,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     log(f"[c2] sequential estimate {seq:.0f} ms/step ({per_size})")
     return {
         "workload": f"{count} instances, nx log-uniform over powers of two 128..1024, ny = nx/2, eta ratio "
-                    f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, FP32 preconditioner, "
+                    f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, FP32 fast-Poisson velocity and FP32 red-black CH preconditioners, "
                     f"FP64 BiCGSTAB, tol 1e-10",
         "cells": cells, "groups": [list(gr) for gr in tuned],
         "ms_per_batch_step": el / steps * 1e3, "instance_steps_per_s": count * steps / el,
