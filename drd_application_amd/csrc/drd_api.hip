@@ -1756,6 +1756,37 @@ extern "C" int drd_fgmres(drd_ctx* c, int32_t op, int32_t prec, int32_t n, int32
   return 0;
 }
 
+// ---- coefficient update ----------------------------------------------------
+extern "C" int drd_interpolate_coef(drd_ctx* c, double lambda_eta, const double* px, const double* py,
+                                    int32_t np, double radius, const double* sx, const double* sy, int32_t nswm,
+                                    double swm_radius) {
+  if (!c || np < 0 || nswm < 0 || (np > 0 && (!px || !py)) || (nswm > 0 && (!sx || !sy)))
+    return set_err(DRD_E_ARG, "bad argument");
+  if (c->nb != 1) return set_err(DRD_E_UNSUPPORTED, "drd_interpolate_coef takes a single-instance context");
+  const int64_t npts = (int64_t)np + nswm;
+  const int64_t slot = std::max<int64_t>(c->g.nuf(), c->g.nc());  // doubles in an upload slot
+  if (2 * npts > slot) return set_err(DRD_E_ARG, "too many particles and swimmers for the grid");
+  std::vector<double> pts(2 * (size_t)std::max<int64_t>(npts, 1));
+  for (int32_t i = 0; i < np; ++i) {
+    pts[2 * (size_t)i] = px[i];
+    pts[2 * (size_t)i + 1] = py[i];
+  }
+  for (int32_t j = 0; j < nswm; ++j) {
+    pts[2 * (size_t)(np + j)] = sx[j];
+    pts[2 * (size_t)(np + j) + 1] = sy[j];
+  }
+  double* dev = c->arg[0];  // an upload slot
+  if (npts > 0)
+    HIPCHK(hipMemcpyAsync(dev, pts.data(), 2 * (size_t)npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const drd_params& p = c->p;
+  launch_interpolate_coef(c->stream, c->g, p.dx, p.dz, p.I_thickness, p.lambda_rho, lambda_eta, dev, np, radius,
+                          nswm, swm_radius, c->f[DRD_F_BDR_PH], c->f[DRD_F_FPD_PH], c->f[DRD_F_RHO],
+                          c->f[DRD_F_ETA]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));  // the host array may be reused on return
+  return 0;
+}
+
 // ---- measurement hook ---------------------------------------------------------
 // launches per timed group of the instrumented kernel
 static int tm_group(const drd_ctx* c) { return c->tm_which == DRD_K_PHMU_SWEEP32 ? kRbgsMiddle : 1; }

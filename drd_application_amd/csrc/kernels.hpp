@@ -201,6 +201,11 @@ __device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, c
   phmu_block_v(g, s, bu[g.ci(i, j)], nb5_cell(g, minner, i, j), nb5_cell(g, m0, i, j), i, j, r1, r2, x1, x2);
 }
 
+// interpolate_coef TP/interpolate_coef.f90:4-115: fpd_ph, rho, eta from the
+// particle and swimmer centres (pts: (x, y) pairs, np then nswm) and bdr_ph
+void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz, double I, double lrho,
+                             double leta, const double* pts, int np, double radius, int nswm,
+                             double swm_radius, const double* bdr, double* fpd, double* rho, double* eta);
 // FP32 copies of the CH block coefficients (cell fields, ghosts included)
 struct PhmuCoef32 {
   const float *bu, *minner, *m0;

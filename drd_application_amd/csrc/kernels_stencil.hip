@@ -835,6 +835,45 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
   }
 }
 
+// interpolate_coef TP/interpolate_coef.f90:4-115 for one cell of the
+// ghosted field (one thread per cell, ghosts included): a ghost takes the
+// value of the nearest interior cell, as the reference's copies leave it.
+// d = fpd_dist TP/particles_move.f90:361-378 at the cell centre (finit
+// TP/finit.f90:44-45): 1e10, then the minimum over the particles and the
+// swimmers, in that order, of sqrt(dx^2 + dy^2) - radius.  pts holds
+// (x, y) pairs, np particles then nswm swimmers.
+__global__ __launch_bounds__(kBlock) void k_interpolate_coef(Grid g, double dr, double dz, double I,
+                                                             double lrho, double leta,
+                                                             const double* __restrict__ pts, int np,
+                                                             double radius, int nswm, double swm_radius,
+                                                             const double* __restrict__ bdr,
+                                                             double* __restrict__ fpd, double* __restrict__ rho,
+                                                             double* __restrict__ eta, int64_t bs) {
+  bdr = bofs(bdr, bs);
+  fpd = bofs(fpd, bs);
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= g.nc()) return;
+  const int ld = g.nx + 2;
+  const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
+  const int ic = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
+  const int kc = k < 0 ? 0 : (k > g.ny - 1 ? g.ny - 1 : k);
+  const double x = (ic + 0.5) * dr, y = (kc + 0.5) * dz;
+  double d = 1.0E10;
+  for (int p = 0; p < np + nswm; ++p) {
+    const double ddx = x - pts[2 * p], ddy = y - pts[2 * p + 1];
+    const double e = sqrt(ddx * ddx + ddy * ddy) - (p < np ? radius : swm_radius);
+    d = fmin(d, e);
+  }
+  double f = tanh(-1.0 * d / sqrt(2.0) / I);
+  f = (f + 1.0) / 2.0;
+  f = f + bdr[g.ci(ic, kc)];
+  fpd[q] = f;
+  rho[q] = 1.0 + (lrho - 1.0) * f;
+  eta[q] = 1.0 + (leta - 1.0) * f;
+}
+
 // FP32 copies of the CH block coefficients bu, M_inner, M_0 (ghosts included)
 __global__ __launch_bounds__(kBlock) void k_phmu_coef32(int64_t n, const double* __restrict__ bu,
                                                         const double* __restrict__ mi,
@@ -1330,6 +1369,13 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
     RB(false, float, float, float, double, c32, bf, nullptr, wa, x);  // launch kRbgsLaunches - 2 (even) wrote wa
   }
 #undef RB
+}
+
+void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz, double I, double lrho,
+                             double leta, const double* pts, int np, double radius, int nswm,
+                             double swm_radius, const double* bdr, double* fpd, double* rho, double* eta) {
+  hipLaunchKernelGGL(k_interpolate_coef, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, dr, dz,
+                     I, lrho, leta, pts, np, radius, nswm, swm_radius, bdr, fpd, rho, eta, bat_bs());
 }
 
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32) {

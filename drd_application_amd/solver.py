@@ -85,6 +85,8 @@ def lib():
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_use_graphs.argtypes = [V, C.c_int32]
         L.drd_write_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.c_double]
+        L.drd_interpolate_coef.argtypes = [V, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
+                                           C.c_double]
         L.drd_read_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
         L.drd_set_initial_guess.argtypes = [V, C.c_int32]
         L.drd_batch_set_initial_guess.argtypes = [V, C.c_int32]
@@ -282,6 +284,19 @@ class GpuSolver:
         """Brackets up to `capacity` launches of kernel `which` inside the
         following solves with HIP events (0 disables)."""
         _check(lib().drd_kernel_timer(self._h, which, capacity))
+
+    def interpolate_coef(self, lambda_eta, particles, radius, swimmers=None, swm_radius=0.0):
+        """interpolate_coef TP/interpolate_coef.f90:4-115 on the device: fpd_ph,
+        rho, eta from the particle and swimmer centres ((n, 2) arrays)."""
+        pa = np.ascontiguousarray(np.asarray(particles, dtype=np.float64).reshape(-1, 2))
+        sa = np.ascontiguousarray(np.asarray(swimmers if swimmers is not None else np.zeros((0, 2)),
+                                             dtype=np.float64).reshape(-1, 2))
+        px, py = np.ascontiguousarray(pa[:, 0]), np.ascontiguousarray(pa[:, 1])
+        sx, sy = np.ascontiguousarray(sa[:, 0]), np.ascontiguousarray(sa[:, 1])
+        P = C.POINTER(C.c_double)
+        cast = lambda a: a.ctypes.data_as(P)
+        _check(lib().drd_interpolate_coef(self._h, lambda_eta, cast(px), cast(py), len(px), radius, cast(sx),
+                                          cast(sy), len(sx), swm_radius))
 
     def write_snapshot(self, path, f_field=abi.F_BDR_PH, f2_field=abi.F_PH, t=0.0):
         """The reference's data file (TP/inout.f90 output): x, y, u, v, p, f, f2, t."""

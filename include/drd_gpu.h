@@ -140,7 +140,9 @@ enum drd_field {
   DRD_F_RMU_PRE = 18,
   DRD_F_PSI1 = 19,
   DRD_F_PSI2 = 20,
-  DRD_F_COUNT = 21
+  DRD_F_FPD_PH = 21,    /* fpd_ph(-1:nx,-1:ny): solid and particle indicator
+                           (drd_interpolate_coef)                        */
+  DRD_F_COUNT = 22
 };
 
 /* number of doubles of a field for params p (0 for an unknown id)        */
@@ -344,6 +346,22 @@ int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  * launches, their summed duration and the kernel's algorithmic bytes per
  * launch.                                                                 */
 int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);
+
+/* ---- coefficient update ----------------------------------------------------
+ * interpolate_coef TP/interpolate_coef.f90:4-115 with dist TP/dist.f90:3-76
+ * and fpd_dist TP/particles_move.f90:361-378, on the device: for every cell,
+ *   fpd_ph = (tanh(-d / sqrt(2) / I_thickness) + 1) / 2 + bdr_ph,
+ *   d = min(1e10, min_i |x - p_i| - radius, min_j |x - s_j| - swm_radius)
+ * at the cell centres ((i + 0.5) dx, (k + 0.5) dz), with ghosts copied from
+ * the nearest interior cell, then rho = 1 + (lambda_rho - 1) fpd_ph and
+ * eta = 1 + (lambda_eta - 1) fpd_ph.  lambda_rho and I_thickness are the
+ * context's; lambda_eta is not a drd_params field and comes as an argument.
+ * px, py: the np particle centres (points(:,0), points(:,1)); sx, sy: the
+ * nswm swimmer centres (pos_swm); host arrays, copied per call (2 (np +
+ * nswm) at most (nx + 2)(ny + 2)).  Writes DRD_F_FPD_PH, DRD_F_RHO and
+ * DRD_F_ETA; single-instance contexts only.                                 */
+int drd_interpolate_coef(drd_ctx* ctx, double lambda_eta, const double* px, const double* py, int32_t np,
+                         double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius);
 int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
 /* The same groups one by one: ms[i] is the mean launch duration of timed
  * group i (i < min(*count, capacity)), in launch order; *count is the number

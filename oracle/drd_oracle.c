@@ -1589,6 +1589,49 @@ static double ch_bdr_condition(double angle, double phi_val, int mode) {
 
 /* RHS of ph_mu_solve TP/ph_mu_solve.f90:31-153; writes ph_M_0 and the
  * wall values pnb/pnt (returned through c->w5/c->w6 first nx entries). */
+/* interpolate_coef TP/interpolate_coef.f90:54-99: fpd_ph at the interior
+ * cells from dist (TP/dist.f90:63 -> fpd_dist TP/particles_move.f90:361-378)
+ * at the cell centres (TP/finit.f90:44-45), rescaled, plus bdr_ph; the
+ * ghost copies (columns, then rows); rho and eta from fpd_ph.  In the
+ * reference's loop order. */
+int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const double* py, int32_t np,
+                        double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius) {
+  const drd_params* p = &c->p;
+  const int nx = NX, ny = NY;
+  double *fpd = c->f[DRD_F_FPD_PH], *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA];
+  const double* bdr = c->f[DRD_F_BDR_PH];
+  const int64_t nc = (int64_t)(nx + 2) * (ny + 2);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      const double x = (i + 0.5) * p->dx, y = (k + 0.5) * p->dz;
+      double d = 1.0E10;
+      for (int32_t j = 0; j < np; ++j) {
+        const double e = sqrt((x - px[j]) * (x - px[j]) + (y - py[j]) * (y - py[j])) - radius;
+        d = e < d ? e : d;
+      }
+      for (int32_t j = 0; j < nswm; ++j) {
+        const double e = sqrt((x - sx[j]) * (x - sx[j]) + (y - sy[j]) * (y - sy[j])) - swm_radius;
+        d = e < d ? e : d;
+      }
+      fpd[CI(i, k)] = tanh(-1.0 * d / sqrt(2.0) / p->I_thickness);
+    }
+  for (int64_t q = 0; q < nc; ++q) fpd[q] = (fpd[q] + 1.0) / 2.0;
+  for (int64_t q = 0; q < nc; ++q) fpd[q] = fpd[q] + bdr[q];
+  for (int k = 0; k < ny; ++k) {
+    fpd[CI(-1, k)] = fpd[CI(0, k)];
+    fpd[CI(nx, k)] = fpd[CI(nx - 1, k)];
+  }
+  for (int i = -1; i <= nx; ++i) {
+    fpd[CI(i, -1)] = fpd[CI(i, 0)];
+    fpd[CI(i, ny)] = fpd[CI(i, ny - 1)];
+  }
+  for (int64_t q = 0; q < nc; ++q) {
+    rho[q] = 1.0 + (p->lambda_rho - 1.0) * fpd[q];
+    eta[q] = 1.0 + (lambda_eta - 1.0) * fpd[q];
+  }
+  return 0;
+}
+
 int oc_ph_mu_rhs(oc_ctx* c, double* rhs) {
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;

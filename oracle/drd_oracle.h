@@ -55,6 +55,8 @@ int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t,
                    drd_solve_info* info);
 /* ph_mu_solve right-hand side only (rhs: 2*nx*ny), also sets ph_M_0     */
 int oc_ph_mu_rhs(oc_ctx* c, double* rhs);
+int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const double* py, int32_t np,
+                        double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius);
 int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t,
                drd_solve_info* info);
 

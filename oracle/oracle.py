@@ -55,6 +55,8 @@ def lib():
         L.oc_presolve_2.argtypes = [C.c_void_p, P, C.c_double, P]
         L.oc_ph_mu_solve.argtypes = [C.c_void_p, P, P, C.c_double, C.POINTER(abi.SolveInfo)]
         L.oc_ph_mu_rhs.argtypes = [C.c_void_p, P]
+        L.oc_interpolate_coef.argtypes = [C.c_void_p, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
+                                          C.c_double]
         L.oc_phsolve.argtypes = [C.c_void_p, P, P, C.c_double, C.POINTER(abi.SolveInfo)]
         L.oc_matvec.argtypes = [C.c_void_p, C.c_int32, C.c_double, P, C.c_double, P]
         L.oc_psolve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P, P]
@@ -185,6 +187,17 @@ class Oracle:
             dp(pf2 if pf2 is not None else z), dp(rhs))
         assert rc == 0, rc
         return rhs
+
+    def interpolate_coef(self, lambda_eta, particles, radius, swimmers=None, swm_radius=0.0):
+        """interpolate_coef TP/interpolate_coef.f90:4-115: fpd_ph, rho, eta.
+        particles, swimmers: (n, 2) arrays of centres."""
+        pa = np.ascontiguousarray(np.asarray(particles, dtype=np.float64).reshape(-1, 2))
+        sa = np.ascontiguousarray(np.asarray(swimmers if swimmers is not None else np.zeros((0, 2)),
+                                             dtype=np.float64).reshape(-1, 2))
+        px, py = np.ascontiguousarray(pa[:, 0]), np.ascontiguousarray(pa[:, 1])
+        sx, sy = np.ascontiguousarray(sa[:, 0]), np.ascontiguousarray(sa[:, 1])
+        lib().oc_interpolate_coef(self._h, lambda_eta, dp(px), dp(py), len(px), radius, dp(sx), dp(sy), len(sx),
+                                  swm_radius)
 
     def ph_mu_rhs(self):
         rhs = np.zeros(2 * self.p.nx * self.p.ny)
