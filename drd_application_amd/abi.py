@@ -70,7 +70,8 @@ F_BDR_PH, F_BDR_PH_USED, F_BDR_PH_PENALTY, F_BDR_PH_GRAD = 7, 8, 9, 10
 F_PH_M_INNER, F_PH_M_0, F_ANGLEB, F_ANGLET = 11, 12, 13, 14
 F_U_PRE, F_V_PRE, F_PH_PRE, F_RMU_PRE, F_PSI1, F_PSI2 = 15, 16, 17, 18, 19, 20
 F_FPD_PH = 21
-F_COUNT = 22
+F_PFORCE1, F_PFORCE2 = 22, 23
+F_COUNT = 24
 
 FIELD_NAMES = {
     F_U: "u", F_V: "v", F_P: "p", F_PH: "ph", F_RMU: "rmu", F_RHO: "rho",
@@ -79,7 +80,7 @@ FIELD_NAMES = {
     F_PH_M_INNER: "ph_M_inner", F_PH_M_0: "ph_M_0", F_ANGLEB: "angleB",
     F_ANGLET: "angleT", F_U_PRE: "u_pre", F_V_PRE: "v_pre",
     F_PH_PRE: "ph_pre", F_RMU_PRE: "rmu_pre", F_PSI1: "psi1", F_PSI2: "psi2",
-    F_FPD_PH: "fpd_ph",
+    F_FPD_PH: "fpd_ph", F_PFORCE1: "particle_force1", F_PFORCE2: "particle_force2",
 }
 FIELD_IDS = {v: k for k, v in FIELD_NAMES.items()}
 

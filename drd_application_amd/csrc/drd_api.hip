@@ -1787,6 +1787,68 @@ extern "C" int drd_interpolate_coef(drd_ctx* c, double lambda_eta, const double*
   return 0;
 }
 
+// ---- particle coupling reductions ----------------------------------------------
+// Bodies (x, y, radius, fx, fy) go to upload slot 0; the per-chunk sums to slot
+// 1 and the per-body results to slot 2.
+static int upload_bodies(drd_ctx* c, const double* bx, const double* by, const double* br, const double* fx,
+                         const double* fy, int32_t nbody, double lx, double ly, int* chunks) {
+  if (!c || nbody < 0 || (nbody > 0 && (!bx || !by || !br)) || !(lx > 0.0) || !(ly > 0.0))
+    return set_err(DRD_E_ARG, "bad argument");
+  if (c->nb != 1) return set_err(DRD_E_UNSUPPORTED, "particle coupling takes a single-instance context");
+  const int64_t slot = std::max<int64_t>(c->g.nuf(), c->g.nc());
+  if (5 * (int64_t)nbody > slot || nbody > 65535) return set_err(DRD_E_ARG, "too many bodies for the grid");
+  std::vector<double> h(5 * (size_t)std::max(nbody, 1));
+  for (int32_t b = 0; b < nbody; ++b) {
+    double* e = &h[5 * (size_t)b];
+    e[0] = bx[b];
+    e[1] = by[b];
+    e[2] = br[b];
+    e[3] = fx ? fx[b] : 0.0;
+    e[4] = fy ? fy[b] : 0.0;
+  }
+  if (nbody > 0)
+    HIPCHK(hipMemcpyAsync(c->arg[0], h.data(), 5 * (size_t)nbody * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  *chunks = body_chunks(c->g, nbody, slot);
+  HIPCHK(hipStreamSynchronize(c->stream));  // h is released on return
+  return 0;
+}
+
+extern "C" int drd_particle_force(drd_ctx* c, const double* bx, const double* by, const double* br,
+                                  const double* fx, const double* fy, int32_t nbody, double lx, double ly) {
+  if (nbody > 0 && (!fx || !fy)) return set_err(DRD_E_ARG, "bad argument");
+  int chunks = 1;
+  RCHK(upload_bodies(c, bx, by, br, fx, fy, nbody, lx, ly, &chunks));
+  const drd_params& p = c->p;
+  launch_body_sums(c->stream, c->g, p.dx, p.dz, p.I_thickness, lx, ly, c->arg[0], nbody, chunks, nullptr, nullptr,
+                   c->arg[1], c->arg[2]);
+  launch_body_spread(c->stream, c->g, p.dx, p.dz, p.I_thickness, lx, ly, c->arg[0], nbody, c->arg[2],
+                     c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int drd_particle_velocity(drd_ctx* c, const double* bx, const double* by, const double* br,
+                                     int32_t nbody, double lx, double ly, double* u_aver, double* v_aver) {
+  if (nbody > 0 && (!u_aver || !v_aver)) return set_err(DRD_E_ARG, "bad argument");
+  int chunks = 1;
+  RCHK(upload_bodies(c, bx, by, br, nullptr, nullptr, nbody, lx, ly, &chunks));
+  if (nbody == 0) return 0;
+  const drd_params& p = c->p;
+  launch_body_sums(c->stream, c->g, p.dx, p.dz, p.I_thickness, lx, ly, c->arg[0], nbody, chunks, c->f[DRD_F_U],
+                   c->f[DRD_F_V], c->arg[1], c->arg[2]);
+  HIPCHK(hipGetLastError());
+  std::vector<double> fin(3 * (size_t)nbody);
+  HIPCHK(hipMemcpyAsync(fin.data(), c->arg[2], fin.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int32_t b = 0; b < nbody; ++b) {
+    u_aver[b] = fin[3 * (size_t)b + 1];
+    v_aver[b] = fin[3 * (size_t)b + 2];
+  }
+  return 0;
+}
+
 // ---- measurement hook ---------------------------------------------------------
 // launches per timed group of the instrumented kernel
 static int tm_group(const drd_ctx* c) { return c->tm_which == DRD_K_PHMU_SWEEP32 ? kRbgsMiddle : 1; }
@@ -1954,11 +2016,12 @@ static int step_dev(drd_ctx* c, int bdf, double t, drd_step_info* out) {
   RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false, bdf == 2 && c->guess == 1));
   for (int i = 0; i < c->nb; ++i) out[i].phmu = c->nb == 1 ? loc.phmu : c->last[i];
   if (bdf == 1) {
-    RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt, nullptr,
-                        nullptr, c->u_after, c->v_after, &loc.velo, false));
+    RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt,
+                        F[DRD_F_PFORCE1], F[DRD_F_PFORCE2], c->u_after, c->v_after, &loc.velo, false));
   } else {
     RCHK(velo_solve_dev(c, 2, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], F[DRD_F_V_PRE], c->ph_after,
-                        c->rmu_after, t + p.dt, nullptr, nullptr, c->u_after, c->v_after, &loc.velo, false));
+                        c->rmu_after, t + p.dt, F[DRD_F_PFORCE1], F[DRD_F_PFORCE2], c->u_after, c->v_after,
+                        &loc.velo, false));
   }
   for (int i = 0; i < c->nb; ++i) out[i].velo = c->nb == 1 ? loc.velo : c->last[i];
   // u_pre = u; u = u_after (pointer rotation: u_after is fully rewritten next step)

@@ -203,6 +203,15 @@ __device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, c
 
 // interpolate_coef TP/interpolate_coef.f90:4-115: fpd_ph, rho, eta from the
 // particle and swimmer centres (pts: (x, y) pairs, np then nswm) and bdr_ph
+// particle coupling: bodies are (x, y, radius, fx, fy); row chunks per body
+// such that 3 * chunks * nbody <= cap; sums (u, v null: weights only) then
+// fin[3b] = weight sum * dr * dz, fin[3b+1..2] = weighted velocity averages
+int body_chunks(const Grid& g, int nbody, int64_t cap);
+void launch_body_sums(hipStream_t st, const Grid& g, double dr, double dz, double I, double lx, double ly,
+                      const double* bodies, int nbody, int chunks, const double* u, const double* v, double* part,
+                      double* fin);
+void launch_body_spread(hipStream_t st, const Grid& g, double dr, double dz, double I, double lx, double ly,
+                        const double* bodies, int nbody, const double* fin, double* pf1, double* pf2);
 void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz, double I, double lrho,
                              double leta, const double* pts, int np, double radius, int nswm,
                              double swm_radius, const double* bdr, double* fpd, double* rho, double* eta);

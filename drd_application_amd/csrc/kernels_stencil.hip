@@ -8,6 +8,8 @@
 // fetched once per sweep.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace drd {
@@ -874,6 +876,103 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_coef(Grid g, double dr, 
   eta[q] = 1.0 + (leta - 1.0) * f;
 }
 
+// ---- particle coupling (TP/particles_move.f90:326-357, 758-1008) ----------
+// A body is 5 doubles: centre x, y, radius, force x, y.  Its weight at a cell
+// centre is (tanh(-d / sqrt(2) / I) + 1) / 2 with d the periodic distance of
+// fpd_dist_periodic: the nearer image across each side within one radius.
+__device__ __forceinline__ double body_weight(double x, double y, double lx, double ly, const double* b,
+                                              double I) {
+  const double bx = b[0], by = b[1], r = b[2];
+  double xs = 0.0, ys = 0.0;
+  if (bx < r) xs = -lx;
+  else if (lx - bx < r) xs = lx;
+  if (by < r) ys = -ly;
+  else if (ly - by < r) ys = ly;
+  const double dx0 = x - bx, dx1 = x + xs - bx, dy0 = y - by, dy1 = y + ys - by;
+  double d = sqrt(dx0 * dx0 + dy0 * dy0) - r;
+  d = fmin(d, sqrt(dx1 * dx1 + dy0 * dy0) - r);
+  d = fmin(d, sqrt(dx0 * dx0 + dy1 * dy1) - r);
+  d = fmin(d, sqrt(dx1 * dx1 + dy1 * dy1) - r);
+  return (tanh(-1.0 * d / sqrt(2.0) / I) + 1.0) / 2.0;
+}
+
+// Per body (blockIdx.y) and row chunk (blockIdx.x): the interior sums of the
+// weight and, with VEL, of the weighted row-averaged velocities
+// (u(i,k) + u(i,k+1)) / 2 and (v(i,k) + v(i,k+1)) / 2.  part[(b * chunks + c) * 3 + j].
+template <bool VEL>
+__global__ __launch_bounds__(kBlock) void k_body_sums(Grid g, double dr, double dz, double I, double lx, double ly,
+                                                      const double* __restrict__ bodies,
+                                                      const double* __restrict__ u, const double* __restrict__ v,
+                                                      double* __restrict__ part) {
+  const int chunks = gridDim.x, c = blockIdx.x, b = blockIdx.y;
+  const int k0 = (int)((int64_t)g.ny * c / chunks), k1 = (int)((int64_t)g.ny * (c + 1) / chunks);
+  const double* body = bodies + 5 * (int64_t)b;
+  double s = 0.0, su = 0.0, sv = 0.0;
+  const int64_t n = (int64_t)(k1 - k0) * g.nx;
+  for (int64_t q = threadIdx.x; q < n; q += kBlock) {
+    const int k = k0 + (int)(q / g.nx), i = (int)(q % g.nx);
+    const double w = body_weight((i + 0.5) * dr, (k + 0.5) * dz, lx, ly, body, I);
+    s += w;
+    if (VEL) {
+      su += (u[g.ui(i, k)] + u[g.ui(i, k + 1)]) / 2.0 * w;
+      sv += (v[g.vi(i, k)] + v[g.vi(i, k + 1)]) / 2.0 * w;
+    }
+  }
+  __shared__ double red[3][kBlock];
+  red[0][threadIdx.x] = s;
+  red[1][threadIdx.x] = su;
+  red[2][threadIdx.x] = sv;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[((int64_t)b * chunks + c) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Per body: the chunk sums in chunk order; fin[3b] = sum * dr * dz (the force
+// normaliser), fin[3b+1], fin[3b+2] = the weighted velocity averages.
+__global__ __launch_bounds__(kBlock) void k_body_finish(int nbody, int chunks, double dr, double dz,
+                                                        const double* __restrict__ part, double* __restrict__ fin) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= nbody) return;
+  double s = 0.0, su = 0.0, sv = 0.0;
+  for (int c = 0; c < chunks; ++c) {
+    const double* pc = part + ((int64_t)b * chunks + c) * 3;
+    s += pc[0];
+    su += pc[1];
+    sv += pc[2];
+  }
+  fin[3 * b] = s * dr * dz;
+  fin[3 * b + 1] = su / s;
+  fin[3 * b + 2] = sv / s;
+}
+
+// particle_force1/2 per ghosted cell: the bodies' (F * weight) / normaliser
+// added in body order; ghosts 0.
+__global__ __launch_bounds__(kBlock) void k_body_spread(Grid g, double dr, double dz, double I, double lx, double ly,
+                                                        const double* __restrict__ bodies, int nbody,
+                                                        const double* __restrict__ fin, double* __restrict__ pf1,
+                                                        double* __restrict__ pf2) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= g.nc()) return;
+  const int ld = g.nx + 2;
+  const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
+  double a1 = 0.0, a2 = 0.0;
+  if (i >= 0 && i < g.nx && k >= 0 && k < g.ny) {
+    const double x = (i + 0.5) * dr, y = (k + 0.5) * dz;
+    for (int b = 0; b < nbody; ++b) {
+      const double* body = bodies + 5 * (int64_t)b;
+      const double w = body_weight(x, y, lx, ly, body, I), S = fin[3 * b];
+      a1 = a1 + body[3] * w / S;
+      a2 = a2 + body[4] * w / S;
+    }
+  }
+  pf1[q] = a1;
+  pf2[q] = a2;
+}
+
 // FP32 copies of the CH block coefficients bu, M_inner, M_0 (ghosts included)
 __global__ __launch_bounds__(kBlock) void k_phmu_coef32(int64_t n, const double* __restrict__ bu,
                                                         const double* __restrict__ mi,
@@ -1376,6 +1475,31 @@ void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz
                              double swm_radius, const double* bdr, double* fpd, double* rho, double* eta) {
   hipLaunchKernelGGL(k_interpolate_coef, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, dr, dz,
                      I, lrho, leta, pts, np, radius, nswm, swm_radius, bdr, fpd, rho, eta, bat_bs());
+}
+
+int body_chunks(const Grid& g, int nbody, int64_t cap) {
+  int64_t c = std::min<int64_t>({(int64_t)g.ny, 64, cap / (3 * (int64_t)std::max(nbody, 1))});
+  return (int)std::max<int64_t>(c, 1);
+}
+
+void launch_body_sums(hipStream_t st, const Grid& g, double dr, double dz, double I, double lx, double ly,
+                      const double* bodies, int nbody, int chunks, const double* u, const double* v, double* part,
+                      double* fin) {
+  if (nbody <= 0) return;
+  if (u)
+    hipLaunchKernelGGL(k_body_sums<true>, dim3(chunks, nbody), dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies, u,
+                       v, part);
+  else
+    hipLaunchKernelGGL(k_body_sums<false>, dim3(chunks, nbody), dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies,
+                       u, v, part);
+  hipLaunchKernelGGL(k_body_finish, dim3(nblk(nbody, kBlock)), dim3(kBlock), 0, st, nbody, chunks, dr, dz, part,
+                     fin);
+}
+
+void launch_body_spread(hipStream_t st, const Grid& g, double dr, double dz, double I, double lx, double ly,
+                        const double* bodies, int nbody, const double* fin, double* pf1, double* pf2) {
+  hipLaunchKernelGGL(k_body_spread, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies,
+                     nbody, fin, pf1, pf2);
 }
 
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32) {

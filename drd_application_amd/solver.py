@@ -87,6 +87,8 @@ def lib():
         L.drd_write_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.c_double]
         L.drd_interpolate_coef.argtypes = [V, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
                                            C.c_double]
+        L.drd_particle_force.argtypes = [V, P, P, P, P, P, C.c_int32, C.c_double, C.c_double]
+        L.drd_particle_velocity.argtypes = [V, P, P, P, C.c_int32, C.c_double, C.c_double, P, P]
         L.drd_read_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
         L.drd_set_initial_guess.argtypes = [V, C.c_int32]
         L.drd_batch_set_initial_guess.argtypes = [V, C.c_int32]
@@ -297,6 +299,26 @@ class GpuSolver:
         cast = lambda a: a.ctypes.data_as(P)
         _check(lib().drd_interpolate_coef(self._h, lambda_eta, cast(px), cast(py), len(px), radius, cast(sx),
                                           cast(sy), len(sx), swm_radius))
+
+    def particle_force(self, bodies, lx, ly):
+        """force_caculation_vel TP/particles_move.f90:758-839 on the device:
+        particle_force1/2 (F_PFORCE1/2, used by step) from bodies, an (n, 5)
+        array of (x, y, radius, fx, fy) in the reference's loop order."""
+        b = np.asarray(bodies, dtype=np.float64).reshape(-1, 5)
+        col = [np.ascontiguousarray(b[:, j]) for j in range(5)]
+        P = C.POINTER(C.c_double)
+        _check(lib().drd_particle_force(self._h, *[a.ctypes.data_as(P) for a in col], len(b), lx, ly))
+
+    def particle_velocity(self, bodies, lx, ly):
+        """update_position_vel TP/particles_move.f90:842-920 on the device: the
+        weighted velocity averages (u_aver, v_aver) of bodies (n, >= 3)."""
+        b = np.asarray(bodies, dtype=np.float64).reshape(len(bodies), -1)
+        col = [np.ascontiguousarray(b[:, j]) for j in range(3)]
+        ua, va = np.zeros(len(b)), np.zeros(len(b))
+        P = C.POINTER(C.c_double)
+        _check(lib().drd_particle_velocity(self._h, *[a.ctypes.data_as(P) for a in col], len(b), lx, ly,
+                                           ua.ctypes.data_as(P), va.ctypes.data_as(P)))
+        return ua, va
 
     def write_snapshot(self, path, f_field=abi.F_BDR_PH, f2_field=abi.F_PH, t=0.0):
         """The reference's data file (TP/inout.f90 output): x, y, u, v, p, f, f2, t."""

@@ -142,7 +142,10 @@ enum drd_field {
   DRD_F_PSI2 = 20,
   DRD_F_FPD_PH = 21,    /* fpd_ph(-1:nx,-1:ny): solid and particle indicator
                            (drd_interpolate_coef)                        */
-  DRD_F_COUNT = 22
+  DRD_F_PFORCE1 = 22,   /* particle_force1, particle_force2 (-1:nx,-1:ny): the  */
+  DRD_F_PFORCE2 = 23,   /* body force drd_step adds to the momentum equation
+                           (TP/VeloSolve_2_combined.f90:242); drd_particle_force */
+  DRD_F_COUNT = 24
 };
 
 /* number of doubles of a field for params p (0 for an unknown id)        */
@@ -341,7 +344,7 @@ int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  *     BiCGSTAB launches it; a group is one launch, timed by its own
  *     dispatch timestamps (hipExtLaunchKernelGGL);
  *   DRD_K_PHMU_SWEEP32: the middle sweeps of DRD_PS_PHMU_RBGS32; a group is
- *     the 6 of one application, between two event markers on the stream.
+ *     the 2 of one application, between two event markers on the stream.
  * drd_kernel_timer_read waits for the stream and returns the number of timed
  * launches, their summed duration and the kernel's algorithmic bytes per
  * launch.                                                                 */
@@ -362,6 +365,32 @@ int drd_kernel_timer(drd_ctx* ctx, int32_t which, int32_t capacity);
  * DRD_F_ETA; single-instance contexts only.                                 */
 int drd_interpolate_coef(drd_ctx* ctx, double lambda_eta, const double* px, const double* py, int32_t np,
                          double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius);
+
+/* ---- particle coupling reductions -------------------------------------------
+ * A body is a particle, a swimmer or a swimmer's phantom: centre (bx, by),
+ * radius br, force (fx, fy).  Its weight at the cell centre ((i + 0.5) dx,
+ * (k + 0.5) dz) is w = (tanh(-d / sqrt(2) / I_thickness) + 1) / 2, d the
+ * distance in the periodic lx x ly box of fpd_dist_periodic
+ * (TP/particles_move.f90:326-357).  Host arrays of nbody entries, copied per
+ * call; 5 nbody at most max(nu + nv, (nx + 2)(ny + 2)); single-instance
+ * contexts only.
+ *
+ * force_caculation_vel TP/particles_move.f90:758-839 after its call of
+ * force_caculation (the per-body forces come in fx, fy): sets DRD_F_PFORCE1/2
+ * to the sum over bodies, in order, of F w / (sum of w over the cells * dx dz),
+ * zero in the ghosts.  The reference's three loops are one list: particles
+ * (points, tot_F, fpd_radius), then swimmers (pos_swm, tot_F_swm, swm_radius),
+ * then, with phantoms on, the phantoms (pos_swm - swm_bp_dist (cos, sin)
+ * (active_F_swm_ang), -active_F_swm, swm_radius).                           */
+int drd_particle_force(drd_ctx* ctx, const double* bx, const double* by, const double* br, const double* fx,
+                       const double* fy, int32_t nbody, double lx, double ly);
+/* update_position_vel TP/particles_move.f90:842-920, the grid part: the
+ * w-weighted averages over the cells of (u(i,k) + u(i,k+1)) / 2 and
+ * (v(i,k) + v(i,k+1)) / 2 from the context's u, v, into u_aver[nbody],
+ * v_aver[nbody].  The position update and the periodic wrap stay with the
+ * caller, as the particle ODE does.                                          */
+int drd_particle_velocity(drd_ctx* ctx, const double* bx, const double* by, const double* br, int32_t nbody,
+                          double lx, double ly, double* u_aver, double* v_aver);
 int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
 /* The same groups one by one: ms[i] is the mean launch duration of timed
  * group i (i < min(*count, capacity)), in launch order; *count is the number

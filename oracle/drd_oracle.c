@@ -1632,6 +1632,78 @@ int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const do
   return 0;
 }
 
+/* fpd_dist_periodic TP/particles_move.f90:326-357, then the weight of
+ * force_caculation_vel / update_position_vel (:786-787, :863-865). */
+static double oc_body_weight(double x, double y, double lx, double ly, double rx, double ry, double r, double I) {
+  double xd = 0.0, yd = 0.0;
+  if (rx < r) xd = -lx;
+  else if (lx - rx < r) xd = lx;
+  if (ry < r) yd = -ly;
+  else if (ly - ry < r) yd = ly;
+  double d = sqrt((x - rx) * (x - rx) + (y - ry) * (y - ry)) - r;
+  double e = sqrt((x + xd - rx) * (x + xd - rx) + (y - ry) * (y - ry)) - r;
+  d = e < d ? e : d;
+  e = sqrt((x - rx) * (x - rx) + (y + yd - ry) * (y + yd - ry)) - r;
+  d = e < d ? e : d;
+  e = sqrt((x + xd - rx) * (x + xd - rx) + (y + yd - ry) * (y + yd - ry)) - r;
+  d = e < d ? e : d;
+  return (tanh(-1.0 * d / sqrt(2.0) / I) + 1.0) / 2.0;
+}
+
+/* force_caculation_vel TP/particles_move.f90:758-839: the bodies' loops in
+ * order (particles, swimmers, phantoms as one list), each adding
+ * force / sum_val with force = F * w and sum_val = (sum of w) * dr * dz. */
+int oc_particle_force(oc_ctx* c, const double* bx, const double* by, const double* br, const double* fx,
+                      const double* fy, int32_t nbody, double lx, double ly) {
+  const drd_params* p = &c->p;
+  const int nx = NX, ny = NY;
+  const double dr = p->dx, dz = p->dz, I = p->I_thickness;
+  double *pf1 = c->f[DRD_F_PFORCE1], *pf2 = c->f[DRD_F_PFORCE2];
+  const size_t nc = (size_t)(nx + 2) * (ny + 2);
+  double* w = (double*)malloc(nc * sizeof(double));
+  if (!w) return -1;
+  for (size_t q = 0; q < nc; ++q) pf1[q] = pf2[q] = 0.0;
+  for (int32_t b = 0; b < nbody; ++b) {
+    double sum_val = 0.0;
+    for (int k = 0; k < ny; ++k)
+      for (int i = 0; i < nx; ++i) {
+        const double dd = oc_body_weight((i + 0.5) * dr, (k + 0.5) * dz, lx, ly, bx[b], by[b], br[b], I);
+        w[CI(i, k)] = dd;
+        sum_val = sum_val + dd;
+      }
+    sum_val = sum_val * dr * dz;
+    for (int k = 0; k < ny; ++k)
+      for (int i = 0; i < nx; ++i) {
+        pf1[CI(i, k)] = pf1[CI(i, k)] + (0.0 + fx[b] * w[CI(i, k)]) / sum_val;
+        pf2[CI(i, k)] = pf2[CI(i, k)] + (0.0 + fy[b] * w[CI(i, k)]) / sum_val;
+      }
+  }
+  free(w);
+  return 0;
+}
+
+/* update_position_vel TP/particles_move.f90:842-920, the averages only. */
+int oc_particle_velocity(oc_ctx* c, const double* bx, const double* by, const double* br, int32_t nbody, double lx,
+                         double ly, double* u_aver, double* v_aver) {
+  const drd_params* p = &c->p;
+  const int nx = NX, ny = NY;
+  const double dr = p->dx, dz = p->dz, I = p->I_thickness;
+  const double *u = c->f[DRD_F_U], *v = c->f[DRD_F_V];
+  for (int32_t b = 0; b < nbody; ++b) {
+    double ua = 0.0, va = 0.0, sum_val = 0.0;
+    for (int k = 0; k < ny; ++k)
+      for (int i = 0; i < nx; ++i) {
+        const double dd = oc_body_weight((i + 0.5) * dr, (k + 0.5) * dz, lx, ly, bx[b], by[b], br[b], I);
+        ua = ua + (u[UI(i, k)] + u[UI(i, k + 1)]) / 2.0 * dd;
+        va = va + (v[VI(i, k)] + v[VI(i, k + 1)]) / 2.0 * dd;
+        sum_val = sum_val + dd;
+      }
+    u_aver[b] = ua / sum_val;
+    v_aver[b] = va / sum_val;
+  }
+  return 0;
+}
+
 int oc_ph_mu_rhs(oc_ctx* c, double* rhs) {
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
@@ -1821,7 +1893,7 @@ int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
   if (bdf == 1) {
     oc_ph_mu_solve(c, ph_after, rmu_after, t, &loc.phmu);
     rc = oc_velosolve_combined(c, F[DRD_F_PSI1], u_after, v_after, rmu_after, ph_after, t + p->dt,
-                               zero, zero, &loc.velo);
+                               c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2], &loc.velo);
     if (rc) goto out;
     memcpy(F[DRD_F_U_PRE], F[DRD_F_U], (size_t)nu * sizeof(double));
     memcpy(F[DRD_F_V_PRE], F[DRD_F_V], (size_t)nv * sizeof(double));
@@ -1837,7 +1909,7 @@ int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
     ph_mu_solve_x(c, ph_after, rmu_after, &loc.phmu, c->guess == 1);
     rc = oc_velosolve_2_combined(c, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], u_after,
                                  F[DRD_F_V_PRE], v_after, F[DRD_F_PH_PRE], ph_after,
-                                 F[DRD_F_RMU_PRE], rmu_after, t + p->dt, zero, zero, &loc.velo);
+                                 F[DRD_F_RMU_PRE], rmu_after, t + p->dt, c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2], &loc.velo);
     if (rc) goto out;
     memcpy(F[DRD_F_U_PRE], F[DRD_F_U], (size_t)nu * sizeof(double));
     memcpy(F[DRD_F_V_PRE], F[DRD_F_V], (size_t)nv * sizeof(double));

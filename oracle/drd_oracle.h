@@ -55,6 +55,10 @@ int oc_ph_mu_solve(oc_ctx* c, double* ph_after, double* rmu_after, double t,
                    drd_solve_info* info);
 /* ph_mu_solve right-hand side only (rhs: 2*nx*ny), also sets ph_M_0     */
 int oc_ph_mu_rhs(oc_ctx* c, double* rhs);
+int oc_particle_force(oc_ctx* c, const double* bx, const double* by, const double* br, const double* fx,
+                      const double* fy, int32_t nbody, double lx, double ly);
+int oc_particle_velocity(oc_ctx* c, const double* bx, const double* by, const double* br, int32_t nbody, double lx,
+                         double ly, double* u_aver, double* v_aver);
 int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const double* py, int32_t np,
                         double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius);
 int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t,

@@ -57,6 +57,8 @@ def lib():
         L.oc_ph_mu_rhs.argtypes = [C.c_void_p, P]
         L.oc_interpolate_coef.argtypes = [C.c_void_p, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
                                           C.c_double]
+        L.oc_particle_force.argtypes = [C.c_void_p, P, P, P, P, P, C.c_int32, C.c_double, C.c_double]
+        L.oc_particle_velocity.argtypes = [C.c_void_p, P, P, P, C.c_int32, C.c_double, C.c_double, P, P]
         L.oc_phsolve.argtypes = [C.c_void_p, P, P, C.c_double, C.POINTER(abi.SolveInfo)]
         L.oc_matvec.argtypes = [C.c_void_p, C.c_int32, C.c_double, P, C.c_double, P]
         L.oc_psolve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P, P]
@@ -198,6 +200,22 @@ class Oracle:
         sx, sy = np.ascontiguousarray(sa[:, 0]), np.ascontiguousarray(sa[:, 1])
         lib().oc_interpolate_coef(self._h, lambda_eta, dp(px), dp(py), len(px), radius, dp(sx), dp(sy), len(sx),
                                   swm_radius)
+
+    def particle_force(self, bodies, lx, ly):
+        """force_caculation_vel TP/particles_move.f90:758-839: particle_force1/2
+        from bodies, an (n, 5) array of (x, y, radius, fx, fy)."""
+        b = np.asarray(bodies, dtype=np.float64).reshape(-1, 5)
+        col = [np.ascontiguousarray(b[:, j]) for j in range(5)]
+        lib().oc_particle_force(self._h, *[dp(a) for a in col], len(b), lx, ly)
+
+    def particle_velocity(self, bodies, lx, ly):
+        """update_position_vel TP/particles_move.f90:842-920, the weighted
+        velocity averages; bodies (n, >= 3): (x, y, radius, ...)."""
+        b = np.asarray(bodies, dtype=np.float64).reshape(len(bodies), -1)
+        col = [np.ascontiguousarray(b[:, j]) for j in range(3)]
+        ua, va = np.zeros(len(b)), np.zeros(len(b))
+        lib().oc_particle_velocity(self._h, *[dp(a) for a in col], len(b), lx, ly, dp(ua), dp(va))
+        return ua, va
 
     def ph_mu_rhs(self):
         rhs = np.zeros(2 * self.p.nx * self.p.ny)
