@@ -287,6 +287,53 @@ def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: i
     }
 
 
+def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cpu: bool = True):
+    """The particle coupling reductions of one step (force_caculation_vel and
+    update_position_vel, TP/particles_move.f90:758-920): `nbody` bodies spread
+    over and averaged against every cell of the grid, O(nbody x cells) weight
+    evaluations each.  Each call includes its upload of the bodies and, for the
+    velocity averages, the read-back.  The CPU figure is the oracle on two
+    bodies, one thread, scaled to `nbody`."""
+    import numpy as np
+    from drd_application_amd import geometry
+    from drd_application_amd.solver import GpuSolver
+    cfg = geometry.config(cfg_name)
+    p = cfg.params
+    lx, ly = p.nx * p.dx, p.ny * p.dz
+    rng = np.random.default_rng(400)
+    bodies = np.stack([rng.uniform(0, lx, nbody), rng.uniform(0, ly, nbody), np.full(nbody, 0.02 * ly),
+                       rng.normal(size=nbody), rng.normal(size=nbody)], axis=1)
+    g = GpuSolver(p)
+    geometry.load(g, cfg)
+    out = {"workload": f"particle force spread + velocity averages, {nbody} bodies on {p.nx}x{p.ny}",
+           "bodies": nbody}
+    try:
+        for name, call in (("force_ms", lambda: g.particle_force(bodies, lx, ly)),
+                           ("velocity_ms", lambda: g.particle_velocity(bodies, lx, ly))):
+            call()
+            call()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            out[name] = (time.perf_counter() - t0) / reps * 1e3
+    finally:
+        g.close()
+    evals = float(nbody) * p.nx * p.ny
+    # the force spread evaluates every weight twice (sums, then spread)
+    out["weight_evals_per_s"] = 2 * evals / (out["force_ms"] * 1e-3)
+    if cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(p)
+        geometry.load(o, cfg)
+        t0 = time.perf_counter()
+        o.particle_force(bodies[:2], lx, ly)
+        dt = time.perf_counter() - t0
+        o.close()
+        out["cpu_force_ms"] = dt / 2 * nbody * 1e3
+        out["cpu_sample"] = "oracle, 2 bodies, 1 thread, scaled to the body count"
+    return out
+
+
 def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4):
     """The batched workload: `count` independent instances of uneven sizes
     (geometry.batch_configs) stepped together by drd_batch_step, every size
@@ -479,6 +526,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the learned correction's own workload")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle coupling reductions")
     ap.add_argument("--profile-network", action="store_true",
                     help="only launch the learned correction's 1024^2 network forward pass")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -546,6 +594,12 @@ def main(argv=None) -> int:
             c2 = batched_c2()
         except Exception as e:  # reported beside the metric, never required for it
             log(f"batched workload failed: {e}")
+    pc = None
+    if not args.no_particles:
+        try:
+            pc = particle_coupling(cfg.name, cpu=not args.no_cpu)
+        except Exception as e:  # reported beside the metric, never required for it
+            log(f"particle coupling failed: {e}")
     c3 = None
     if not args.no_c3:
         try:
@@ -590,6 +644,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "learned_c3": c3,
         "batched_c2": c2,
+        "particle_coupling": pc,
     }
     print(json.dumps(result))
     return 0

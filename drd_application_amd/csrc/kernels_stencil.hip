@@ -880,6 +880,11 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_coef(Grid g, double dr, 
 // A body is 5 doubles: centre x, y, radius, force x, y.  Its weight at a cell
 // centre is (tanh(-d / sqrt(2) / I) + 1) / 2 with d the periodic distance of
 // fpd_dist_periodic: the nearer image across each side within one radius.
+// Beyond kBodyCut * sqrt(2) * I from a body its tanh argument is below -20,
+// where tanh is -1 to double precision (1 - |tanh| < 1e-17), so the weight is
+// 0 and the sums skip it.
+constexpr double kBodyCut = 20.0;
+
 __device__ __forceinline__ double body_weight(double x, double y, double lx, double ly, const double* b,
                                               double I) {
   const double bx = b[0], by = b[1], r = b[2];
@@ -893,7 +898,21 @@ __device__ __forceinline__ double body_weight(double x, double y, double lx, dou
   d = fmin(d, sqrt(dx1 * dx1 + dy0 * dy0) - r);
   d = fmin(d, sqrt(dx0 * dx0 + dy1 * dy1) - r);
   d = fmin(d, sqrt(dx1 * dx1 + dy1 * dy1) - r);
+  if (d >= kBodyCut * sqrt(2.0) * I) return 0.0;
   return (tanh(-1.0 * d / sqrt(2.0) / I) + 1.0) / 2.0;
+}
+
+// True when every cell centre of rows [k0, k1) lies beyond the cut of body b
+// (its centre and the y image fpd_dist_periodic may use).
+__device__ __forceinline__ bool band_beyond(int k0, int k1, double dz, double ly, const double* b, double I) {
+  const double by = b[1], r = b[2], y0 = (k0 + 0.5) * dz, y1 = (k1 - 0.5) * dz;
+  double ys = 0.0;
+  if (by < r) ys = -ly;
+  else if (ly - by < r) ys = ly;
+  const double lim = r + kBodyCut * sqrt(2.0) * I;
+  const double c0 = by, c1 = by - ys;
+  const double g0 = fmax(0.0, fmax(y0 - c0, c0 - y1)), g1 = fmax(0.0, fmax(y0 - c1, c1 - y1));
+  return fmin(g0, g1) > lim * (1.0 + 1e-12);
 }
 
 // Per body (blockIdx.y) and row chunk (blockIdx.x): the interior sums of the
@@ -908,7 +927,7 @@ __global__ __launch_bounds__(kBlock) void k_body_sums(Grid g, double dr, double 
   const int k0 = (int)((int64_t)g.ny * c / chunks), k1 = (int)((int64_t)g.ny * (c + 1) / chunks);
   const double* body = bodies + 5 * (int64_t)b;
   double s = 0.0, su = 0.0, sv = 0.0;
-  const int64_t n = (int64_t)(k1 - k0) * g.nx;
+  const int64_t n = band_beyond(k0, k1, dz, ly, body, I) ? 0 : (int64_t)(k1 - k0) * g.nx;
   for (int64_t q = threadIdx.x; q < n; q += kBlock) {
     const int k = k0 + (int)(q / g.nx), i = (int)(q % g.nx);
     const double w = body_weight((i + 0.5) * dr, (k + 0.5) * dz, lx, ly, body, I);
@@ -965,6 +984,7 @@ __global__ __launch_bounds__(kBlock) void k_body_spread(Grid g, double dr, doubl
     for (int b = 0; b < nbody; ++b) {
       const double* body = bodies + 5 * (int64_t)b;
       const double w = body_weight(x, y, lx, ly, body, I), S = fin[3 * b];
+      if (w == 0.0) continue;  // adds +0
       a1 = a1 + body[3] * w / S;
       a2 = a2 + body[4] * w / S;
     }
@@ -1478,7 +1498,7 @@ void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz
 }
 
 int body_chunks(const Grid& g, int nbody, int64_t cap) {
-  int64_t c = std::min<int64_t>({(int64_t)g.ny, 64, cap / (3 * (int64_t)std::max(nbody, 1))});
+  int64_t c = std::min<int64_t>({(int64_t)g.ny, 256, cap / (3 * (int64_t)std::max(nbody, 1))});
   return (int)std::max<int64_t>(c, 1);
 }
 
