@@ -968,29 +968,78 @@ __global__ __launch_bounds__(kBlock) void k_body_finish(int nbody, int chunks, d
   fin[3 * b + 2] = sv / s;
 }
 
-// particle_force1/2 per ghosted cell: the bodies' (F * weight) / normaliser
-// added in body order; ghosts 0.
+// True when every cell centre of the box [i0, i1] x [k0, k1] lies beyond the
+// cut of body b, for each of the centres fpd_dist_periodic may use (a lower
+// bound of the distance: the nearest x image and the nearest y image).
+__device__ __forceinline__ bool box_beyond(int i0, int i1, int k0, int k1, double dr, double dz, double lx,
+                                           double ly, const double* b, double I) {
+  const double bx = b[0], by = b[1], r = b[2];
+  double xs = 0.0, ys = 0.0;
+  if (bx < r) xs = -lx;
+  else if (lx - bx < r) xs = lx;
+  if (by < r) ys = -ly;
+  else if (ly - by < r) ys = ly;
+  const double x0 = (i0 + 0.5) * dr, x1 = (i1 + 0.5) * dr, y0 = (k0 + 0.5) * dz, y1 = (k1 + 0.5) * dz;
+  const double cx1 = bx - xs, cy1 = by - ys;
+  const double gx = fmin(fmax(0.0, fmax(x0 - bx, bx - x1)), fmax(0.0, fmax(x0 - cx1, cx1 - x1)));
+  const double gy = fmin(fmax(0.0, fmax(y0 - by, by - y1)), fmax(0.0, fmax(y0 - cy1, cy1 - y1)));
+  const double lim = r + kBodyCut * sqrt(2.0) * I;
+  return sqrt(gx * gx + gy * gy) > lim * (1.0 + 1e-12);
+}
+
+// particle_force1/2 per ghosted cell on kSpTX x kSpTY tiles: the bodies' (F *
+// weight) / normaliser added in body order; ghosts 0.  Per batch of kBlock
+// bodies the tile keeps, in order, those that can reach one of its cells
+// (wave ballots, then per-wave offsets); the others add exactly 0 at every
+// cell of the tile.
+constexpr int kSpTX = 64, kSpTY = kBlock / kSpTX;
+
 __global__ __launch_bounds__(kBlock) void k_body_spread(Grid g, double dr, double dz, double I, double lx, double ly,
                                                         const double* __restrict__ bodies, int nbody,
                                                         const double* __restrict__ fin, double* __restrict__ pf1,
                                                         double* __restrict__ pf2) {
-  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (q >= g.nc()) return;
-  const int ld = g.nx + 2;
-  const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int list[kBlock];
+  __shared__ int wcount[kWaves];
+  const int tx = (int)threadIdx.x % kSpTX, ty = (int)threadIdx.x / kSpTX;
+  const int ox = (int)blockIdx.x * kSpTX - 1, oy = (int)blockIdx.y * kSpTY - 1;  // ghosted tile origin
+  const int i = ox + tx, k = oy + ty;
+  const int i0 = max(ox, 0), i1 = min(ox + kSpTX - 1, g.nx - 1);
+  const int k0 = max(oy, 0), k1 = min(oy + kSpTY - 1, g.ny - 1);
+  const bool empty = i0 > i1 || k0 > k1;
+  const bool inside = i >= 0 && i < g.nx && k >= 0 && k < g.ny;
+  const double x = (i + 0.5) * dr, y = (k + 0.5) * dz;
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   double a1 = 0.0, a2 = 0.0;
-  if (i >= 0 && i < g.nx && k >= 0 && k < g.ny) {
-    const double x = (i + 0.5) * dr, y = (k + 0.5) * dz;
-    for (int b = 0; b < nbody; ++b) {
-      const double* body = bodies + 5 * (int64_t)b;
-      const double w = body_weight(x, y, lx, ly, body, I), S = fin[3 * b];
-      if (w == 0.0) continue;  // adds +0
-      a1 = a1 + body[3] * w / S;
-      a2 = a2 + body[4] * w / S;
+  for (int base = 0; base < nbody; base += kBlock) {
+    const int b = base + (int)threadIdx.x;
+    const bool keep = b < nbody && !empty && !box_beyond(i0, i1, k0, k1, dr, dz, lx, ly, bodies + 5 * (int64_t)b, I);
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wcount[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      off += w < wave ? wcount[w] : 0;
+      total += wcount[w];
     }
+    if (keep) list[off + __popcll(m & ((1ull << lane) - 1ull))] = b;
+    __syncthreads();
+    if (inside)
+      for (int j = 0; j < total; ++j) {
+        const int bj = list[j];
+        const double* body = bodies + 5 * (int64_t)bj;
+        const double w = body_weight(x, y, lx, ly, body, I), S = fin[3 * bj];
+        if (w == 0.0) continue;  // adds +0
+        a1 = a1 + body[3] * w / S;
+        a2 = a2 + body[4] * w / S;
+      }
+    __syncthreads();  // list and wcount are rewritten by the next batch
   }
-  pf1[q] = a1;
-  pf2[q] = a2;
+  if (i <= g.nx && k <= g.ny) {
+    const int64_t q = g.ci(i, k);
+    pf1[q] = a1;
+    pf2[q] = a2;
+  }
 }
 
 // FP32 copies of the CH block coefficients bu, M_inner, M_0 (ghosts included)
@@ -1518,8 +1567,8 @@ void launch_body_sums(hipStream_t st, const Grid& g, double dr, double dz, doubl
 
 void launch_body_spread(hipStream_t st, const Grid& g, double dr, double dz, double I, double lx, double ly,
                         const double* bodies, int nbody, const double* fin, double* pf1, double* pf2) {
-  hipLaunchKernelGGL(k_body_spread, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies,
-                     nbody, fin, pf1, pf2);
+  const dim3 grid((unsigned)((g.nx + 2 + kSpTX - 1) / kSpTX), (unsigned)((g.ny + 2 + kSpTY - 1) / kSpTY));
+  hipLaunchKernelGGL(k_body_spread, grid, dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies, nbody, fin, pf1, pf2);
 }
 
 void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32) {
