@@ -927,9 +927,18 @@ __global__ __launch_bounds__(kBlock) void k_body_sums(Grid g, double dr, double 
   const int k0 = (int)((int64_t)g.ny * c / chunks), k1 = (int)((int64_t)g.ny * (c + 1) / chunks);
   const double* body = bodies + 5 * (int64_t)b;
   double s = 0.0, su = 0.0, sv = 0.0;
-  const int64_t n = band_beyond(k0, k1, dz, ly, body, I) ? 0 : (int64_t)(k1 - k0) * g.nx;
+  // A body with no x image (fpd_dist_periodic takes one within a radius of
+  // x = 0 or lx) weighs 0 at every column whose centre is beyond the cut.
+  int ia = 0, ib = g.nx - 1;
+  if (!(body[0] < body[2]) && !(lx - body[0] < body[2])) {
+    const double lim = (body[2] + kBodyCut * sqrt(2.0) * I) * (1.0 + 1e-12);
+    ia = (int)fmax(0.0, floor((body[0] - lim) / dr - 0.5));
+    ib = (int)fmin((double)(g.nx - 1), ceil((body[0] + lim) / dr - 0.5));
+  }
+  const int wdt = ib - ia + 1;
+  const int64_t n = (band_beyond(k0, k1, dz, ly, body, I) || wdt <= 0) ? 0 : (int64_t)(k1 - k0) * wdt;
   for (int64_t q = threadIdx.x; q < n; q += kBlock) {
-    const int k = k0 + (int)(q / g.nx), i = (int)(q % g.nx);
+    const int k = k0 + (int)(q / wdt), i = ia + (int)(q % wdt);
     const double w = body_weight((i + 0.5) * dr, (k + 0.5) * dz, lx, ly, body, I);
     s += w;
     if (VEL) {
