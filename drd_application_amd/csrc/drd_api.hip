@@ -31,6 +31,8 @@ static int set_err(int code, const std::string& msg) {
   return code;
 }
 
+void drd::api_set_error(const std::string& msg) { g_err = msg; }
+
 #define HIPCHK(expr)                                                                  \
   do {                                                                                \
     hipError_t e_ = (expr);                                                           \
@@ -86,8 +88,12 @@ static void learn_destroy(Learn* L) {
 }
 
 // A captured chunk of BiCGSTAB iterations is valid for one operator,
-// preconditioner, chunk length and pair of vectors (the kernel arguments);
-// parameter changes and preconditioner rebuilds drop every graph.
+// preconditioner, chunk length and pair of vectors; everything else its
+// kernels bake in (workspace and coefficient pointers, the preconditioner
+// objects and their buffers, the scalars, the grid and the batch group) is
+// summed up in a fingerprint taken at capture and compared before each
+// replay: a graph whose fingerprint no longer matches is captured again.
+// Parameter changes and preconditioner rebuilds also drop every graph.
 struct GraphKey {
   int op, prec, chunk;
   const double* b;
@@ -95,6 +101,10 @@ struct GraphKey {
   bool operator<(const GraphKey& o) const {
     return std::tie(op, prec, chunk, b, x) < std::tie(o.op, o.prec, o.chunk, o.b, o.x);
   }
+};
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  uint64_t print = 0;
 };
 
 struct drd_ctx {
@@ -112,7 +122,8 @@ struct drd_ctx {
   // captured BiCGSTAB iteration chunks (drd_use_graphs)
   bool use_graphs = false;
   int guess = 0;  // drd_set_initial_guess
-  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::map<GraphKey, GraphEntry> graphs;
+  int64_t graph_recaptures = 0;  // replays refused for a changed fingerprint
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
   std::vector<hipEvent_t> tm_ev;
@@ -426,7 +437,7 @@ static int ctx_alloc(drd_ctx* c) {
 }
 
 static void drop_graphs(drd_ctx* c) {
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
   c->graphs.clear();
 }
 
@@ -531,6 +542,13 @@ extern "C" int drd_use_graphs(drd_ctx* c, int32_t on) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (!on) drop_graphs(c);
   c->use_graphs = on != 0;
+  return 0;
+}
+
+extern "C" int drd_graph_count(drd_ctx* c, int32_t* live, int64_t* recaptured) {
+  if (!c || !live || !recaptured) return set_err(DRD_E_ARG, "null argument");
+  *live = (int32_t)c->graphs.size();
+  *recaptured = c->graph_recaptures;
   return 0;
 }
 
@@ -691,7 +709,9 @@ static int prec_setup(drd_ctx* c, int prec) {
   }
   if (prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32) {
     const bool f32 = prec == DRD_PS_VELO_MG32;
-    if (c->vmg && velo_mg_f32(c->vmg) != f32) {
+    if (c->vmg && velo_mg_f32(c->vmg) != f32) {  // captured chunks point into the old hierarchy
+      HIPCHK(hipStreamSynchronize(c->stream));
+      drop_graphs(c);
       velo_mg_destroy(c->vmg);
       c->vmg = nullptr;
     }
@@ -834,6 +854,51 @@ static int true_resid(drd_ctx* c, int op, int64_t n, const double* b, const doub
   return 0;
 }
 
+// Everything a captured BiCGSTAB chunk bakes into its kernels besides the
+// GraphKey: FNV-1a over the workspace, coefficient and preconditioner
+// pointers, the scalars, the grid and the batch group.
+static uint64_t graph_print(const drd_ctx* c) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* q = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ q[i]) * 1099511628211ull;
+  };
+  auto ptr = [&](const void* p) { mix(&p, sizeof(p)); };
+  for (int f = 0; f < DRD_F_COUNT; ++f)
+    if (f != DRD_F_U && f != DRD_F_V && f != DRD_F_U_PRE && f != DRD_F_V_PRE && f != DRD_F_PH &&
+        f != DRD_F_PH_PRE && f != DRD_F_RMU && f != DRD_F_RMU_PRE && f != DRD_F_PSI1 && f != DRD_F_PSI2)
+      ptr(c->f[f]);  // the coefficient fields (the rotated state fields are not read by a chunk)
+  for (double* w : c->kw) ptr(w);
+  ptr(c->part);
+  ptr(c->ks);
+  ptr(c->diag);
+  ptr(c->rb_work);
+  ptr(c->rb_c32);
+  ptr(c->vprec);
+  ptr(c->vmg);
+  ptr(c->vp_work);
+  mix(&c->vprec_fp32, sizeof(c->vprec_fp32));
+  mix(&c->s, sizeof(c->s));
+  mix(&c->g, sizeof(c->g));
+  const BatchCfg bc = batch_cfg();
+  mix(&bc.n, sizeof(bc.n));
+  mix(&bc.bs, sizeof(bc.bs));
+  return h;
+}
+
+// Ends a stream capture left open by an error return, so the stream is
+// usable again and no half-captured graph survives.
+struct CaptureGuard {
+  hipStream_t st;
+  bool active = false;
+  ~CaptureGuard() {
+    if (!active) return;
+    hipGraph_t gr = nullptr;
+    if (hipStreamEndCapture(st, &gr) == hipSuccess && gr) (void)hipGraphDestroy(gr);
+    (void)hipGetLastError();
+  }
+};
+
 // BICGSTAB LIB/BiCGSTAB.f + LIB/BiCGSTABREVCOM.f on device vectors b, x.
 // Optional recording: x after iteration snap_at[j] is copied to snaps + j*n
 // (nsnap entries, increasing).
@@ -872,17 +937,24 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
                           prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32 || prec == DRD_PS_PHMU_RBGS ||
                           prec == DRD_PS_PHMU_RBGS32);
   int chunk = 1, total = 0, next_snap = 0;
+  const uint64_t print = graphable ? graph_print(c) : 0;
   for (;;) {
     hipGraphExec_t gx = nullptr;
-    bool capturing = false;
+    CaptureGuard cap{c->stream};
     if (graphable && chunk >= 2) {
       const GraphKey key{op, prec, chunk, b, x};
       auto hit = c->graphs.find(key);
+      if (hit != c->graphs.end() && hit->second.print != print) {  // stale: capture it again
+        (void)hipGraphExecDestroy(hit->second.exec);
+        c->graphs.erase(hit);
+        hit = c->graphs.end();
+        ++c->graph_recaptures;
+      }
       if (hit != c->graphs.end()) {
-        gx = hit->second;
+        gx = hit->second.exec;
       } else {
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-        capturing = true;
+        cap.active = true;
       }
     }
     for (int it = 0; it < (gx ? 0 : chunk); ++it) {
@@ -912,13 +984,14 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
         ++next_snap;
       }
     }
-    if (capturing) {
+    if (cap.active) {
       hipGraph_t gr = nullptr;
+      cap.active = false;
       HIPCHK(hipStreamEndCapture(c->stream, &gr));
       const hipError_t e = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
       (void)hipGraphDestroy(gr);
       HIPCHK(e);
-      c->graphs[GraphKey{op, prec, chunk, b, x}] = gx;
+      c->graphs[GraphKey{op, prec, chunk, b, x}] = GraphEntry{gx, print};
     }
     if (gx) HIPCHK(hipGraphLaunch(gx, c->stream));
     HIPCHK(hipGetLastError());

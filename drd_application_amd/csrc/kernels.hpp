@@ -1,9 +1,14 @@
 // kernels.hpp -- kernel launchers and the device-side BiCGSTAB state machine.
 #pragma once
 
+#include <string>
+
 #include "drd_common.hpp"
 
 namespace drd {
+
+// the message drd_last_error() returns (entry points outside drd_api.hip)
+void api_set_error(const std::string& msg);
 
 // matvec modes (see k_velo_matvec)
 enum { MV_PLAIN = 0, MV_DOT1 = 1, MV_DOT2 = 2, MV_INIT = 3 };

@@ -84,6 +84,7 @@ def lib():
         L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_use_graphs.argtypes = [V, C.c_int32]
+        L.drd_graph_count.argtypes = [V, I, C.POINTER(C.c_int64)]
         L.drd_write_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.c_double]
         L.drd_interpolate_coef.argtypes = [V, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
                                            C.c_double]
@@ -337,6 +338,12 @@ class GpuSolver:
     def use_graphs(self, on=True):
         """Capture BiCGSTAB iteration chunks as hipGraphs and replay them."""
         _check(lib().drd_use_graphs(self._h, 1 if on else 0))
+
+    def graph_count(self):
+        """(captured chunks held, replays refused for a changed fingerprint)."""
+        live, rec = C.c_int32(), C.c_int64()
+        _check(lib().drd_graph_count(self._h, C.byref(live), C.byref(rec)))
+        return live.value, rec.value
 
     def tune(self):
         """(chosen transpose tile shape, [ms per preconditioner application per shape])."""

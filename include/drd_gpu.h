@@ -294,6 +294,10 @@ int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, int32_t cap)
  * Results are bit for bit those without graphs.  Off by default for
  * contexts, on by default for batches.                                     */
 int drd_use_graphs(drd_ctx* ctx, int32_t on);
+/* Captured chunks held now (*live) and replays refused since the context was
+ * made because something a chunk bakes in had changed (*recaptured; such a
+ * chunk is captured again instead).                                          */
+int drd_graph_count(drd_ctx* ctx, int32_t* live, int64_t* recaptured);
 int drd_batch_use_graphs(drd_batch* b, int32_t on);
 
 /* ---- tile autotuner (no reference counterpart) ----------------------------

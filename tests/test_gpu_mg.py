@@ -122,3 +122,33 @@ def test_multigrid_steps_track_oracle_steps(prec):
         print(abi.FIELD_NAMES[fid], e)
         assert e < 1e-6
     gm.close()
+
+
+def test_precision_switch_with_graphs_matches_eager():
+    """MG, MG32, MG again on one context with captured iteration chunks: each
+    switch of the hierarchy's precision drops the captured chunks (they point
+    into the old levels), and every solve equals the same solve on a context
+    without graphs bit for bit."""
+    from drd_application_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    print("mg precision switch with graphs 256x128", flush=True)
+    cfg = _channel(256, 128)
+    gg = GpuSolver(cfg.params)
+    ge = GpuSolver(cfg.params)
+    for s in (gg, ge):
+        geometry.load(s, cfg)
+    gg.use_graphs(True)
+    o = Oracle(cfg.params)
+    geometry.load(o, cfg)
+    n = abi.n_uv(cfg.params)
+    b = o.matvec(abi.OP_VELO2, 1.0, np.random.default_rng(9).standard_normal(n), 0.0, np.zeros(n))
+    for k, prec in enumerate((abi.PS_VELO_MG, abi.PS_VELO_MG32, abi.PS_VELO_MG)):
+        xg, itg, rg, ig = gg.bicgstab(abi.OP_VELO2, prec, b, np.zeros(n), 2000, 1e-10)
+        live, rec = gg.graph_count()
+        print(f"solve {k} prec {prec}: {itg} its, {live} captured chunks, {rec} recaptured", flush=True)
+        assert live > 0 and rec == 0
+        xe, ite, re_, ie = ge.bicgstab(abi.OP_VELO2, prec, b, np.zeros(n), 2000, 1e-10)
+        assert ig == ie == 0 and itg == ite and rg == re_
+        np.testing.assert_array_equal(xg, xe)
+    gg.close()
+    ge.close()

@@ -26,15 +26,25 @@ def _pair(cfg):
 
 def _bodies(rng, n, lx, ly):
     # centres anywhere, the periodic edges included; radii of particles and swimmers
-    return np.stack([rng.uniform(0, lx, n), rng.uniform(0, ly, n), rng.choice([0.03, 0.05], n),
-                     rng.normal(size=n), rng.normal(size=n)], axis=1)
+    b = np.stack([rng.uniform(0, lx, n), rng.uniform(0, ly, n), rng.choice([0.03, 0.05], n),
+                  rng.normal(size=n), rng.normal(size=n)], axis=1)
+    if n >= 8:  # bodies at the four periodic corners and across each edge
+        b[:8, :2] = [[0.001, 0.001], [lx - 0.001, 0.001], [0.001, ly - 0.001], [lx - 0.001, ly - 0.001],
+                     [0.0, ly / 2], [lx / 2, 0.0], [lx - 1e-9, ly / 3], [lx / 3, ly - 1e-9]]
+    return b
 
 
-@pytest.mark.parametrize("nx,ny,nbody", [(64, 32, 0), (96, 48, 7), (257, 130, 40)])
-def test_particle_reductions_match_oracle(nx, ny, nbody):
+# 520 bodies: the spread builds its per-tile lists in batches of 256 bodies,
+# so three batches run; I_thickness 0.002 keeps each body's cutoff disc
+# (20 sqrt2 I beyond its radius) small enough for the pruning to act.
+@pytest.mark.parametrize("nx,ny,nbody,I", [(64, 32, 0, None), (96, 48, 7, None), (257, 130, 40, None),
+                                           (200, 100, 520, 0.002)])
+def test_particle_reductions_match_oracle(nx, ny, nbody, I):
     print(f"particle reductions {nx}x{ny} bodies {nbody}", flush=True)
     cfg = geometry.tp_case(nx, ny, lambda_eta=10.0, tol=1e-8)
     p = cfg.params
+    if I is not None:
+        p.I_thickness = I
     lx, ly = nx * p.dx, ny * p.dz
     g, o = _pair(cfg)
     rng = np.random.default_rng(nx + nbody)
