@@ -395,6 +395,24 @@ int drd_particle_force(drd_ctx* ctx, const double* bx, const double* by, const d
  * caller, as the particle ODE does.                                          */
 int drd_particle_velocity(drd_ctx* ctx, const double* bx, const double* by, const double* br, int32_t nbody,
                           double lx, double ly, double* u_aver, double* v_aver);
+/* No counterpart in the reference.  drd_particle_velocity for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): no context; u (u field layout),
+ * v (v field layout), bx, by, br and the outputs u_aver, v_aver are DEVICE
+ * arrays; the grid comes from prm (nx, ny, dx, dz, I_thickness).  The work
+ * buffers live in one fixed array of drd_noalloc_scratch_bytes() (16 MiB)
+ * per GPU, whose address is looked up on the first call on each device;
+ * nothing is allocated.  Rejected (DRD_E_ARG) beyond 8 nbody + 3 nbody *
+ * chunks doubles of that array (chunks = min(ny, 256) row chunks, as
+ * drd_particle_velocity takes them; e.g. nbody <= 2048 at ny >= 256).
+ * The work is queued on `stream` (a hipStream_t; null: the null stream) and
+ * the call returns without waiting.  Results equal drd_particle_velocity's
+ * bit for bit.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other streams, other host threads) race.            */
+int64_t drd_noalloc_scratch_bytes(void);
+int drd_particle_velocity_noalloc(const drd_params* prm, const double* u, const double* v, const double* bx,
+                                  const double* by, const double* br, int32_t nbody, double lx, double ly,
+                                  double* u_aver, double* v_aver, void* stream);
 int drd_kernel_timer_read(drd_ctx* ctx, int32_t* count, double* total_ms, double* bytes);
 /* The same groups one by one: ms[i] is the mean launch duration of timed
  * group i (i < min(*count, capacity)), in launch order; *count is the number
