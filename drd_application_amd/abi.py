@@ -59,6 +59,10 @@ PS_VELO_MG = 9
 PS_VELO_MG32 = 11
 PS_PHMU_RBGS = 10
 PS_PHMU_RBGS32 = 12
+PS_VELO1_DCT32 = 13
+PS_VELO1_DCT64 = 14
+PS_VELO1_MG = 15
+PS_VELO1_MG32 = 16
 
 LEARN_OFF, LEARN_JUDGE, LEARN_ON, LEARN_FALLBACK = 0, 1, 2, 3
 

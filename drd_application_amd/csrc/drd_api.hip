@@ -149,6 +149,11 @@ struct drd_ctx {
   VeloPrec* vprec = nullptr;  // fast-path velocity preconditioner (lazy)
   VeloMG* vmg = nullptr;      // multigrid velocity preconditioner (lazy)
   bool vprec_fp32 = true;
+  // the same two for the BDF1 operator (lazy); vprec1 shares vprec's
+  // per-instance buffers (one solve uses one of them at a time)
+  VeloPrec* vprec1 = nullptr;
+  bool vprec1_fp32 = true;
+  VeloMG* vmg1 = nullptr;
   // FGMRES workspace (lazy): V (m+1 vectors), Z (m vectors), small arrays
   int fg_m = 0;
   double *fg_V = nullptr, *fg_Z = nullptr, *fg_h = nullptr, *fg_hh = nullptr;
@@ -455,6 +460,8 @@ static void ctx_free(drd_ctx* c) {
   dct_plan_destroy(c->dct);
   velo_prec_destroy(c->vprec);
   velo_mg_destroy(c->vmg);
+  velo_prec_destroy(c->vprec1);
+  velo_mg_destroy(c->vmg1);
   learn_destroy(c->learn);
   c->learn = nullptr;
   hipFree(c->fg_V);
@@ -475,6 +482,17 @@ static void ctx_free(drd_ctx* c) {
 static VeloPrec* make_vprec(drd_ctx* c, bool fp32, int* err) {
   const VeloPrecBufs bufs{c->vp_inv64, c->vp_inv32, c->vp_work};
   return velo_prec_create(c->g, c->s, fp32, err, &bufs);
+}
+
+// The BDF1 operator (TP/VeloSolve_combined.f90:608-670) is the BDF2 stencil
+// with mass coefficient re rho instead of 1.5 re rho and no penalty: the
+// fast-path preconditioners built for BDF2 become BDF1 ones with these scalars.
+static Scal bdf1_scal(const Scal& s) {
+  Scal t = s;
+  t.re = s.re / 1.5;
+  t.penalty_k = 0.0;
+  scal_derive(t);
+  return t;
 }
 
 // The fast-Poisson velocity preconditioner is built with the context (or
@@ -571,6 +589,10 @@ extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
     return set_err(DRD_E_HIP, "preconditioner tables could not be rebuilt");
   velo_mg_destroy(c->vmg);  // its levels' scalars follow the parameters; rebuilt when next used
   c->vmg = nullptr;
+  velo_prec_destroy(c->vprec1);  // the BDF1 forms likewise
+  c->vprec1 = nullptr;
+  velo_mg_destroy(c->vmg1);
+  c->vmg1 = nullptr;
   RCHK(build_tables(c));
   RCHK(eager_vprec(c));
   return 0;
@@ -724,6 +746,42 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     velo_mg_setup(c->stream, c->vmg, coef_of(c));
   }
+  if (prec == DRD_PS_VELO1_DCT32 || prec == DRD_PS_VELO1_DCT64) {
+    const bool fp32 = prec == DRD_PS_VELO1_DCT32;
+    const Scal s1 = bdf1_scal(c->s);
+    if (c->vprec1 && c->vprec1_fp32 != fp32) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      drop_graphs(c);
+      velo_prec_destroy(c->vprec1);
+      c->vprec1 = nullptr;
+    }
+    if (!c->vprec1) {
+      int err = 0;
+      const VeloPrecBufs bufs{c->vp_inv64, c->vp_inv32, c->vp_work};
+      c->vprec1 = velo_prec_create(c->g, s1, fp32, &err, &bufs);
+      if (!c->vprec1)
+        return set_err(err ? err : DRD_E_HIP, "DCT velocity preconditioner needs power-of-two nx, ny <= 4096");
+      c->vprec1_fp32 = fp32;
+    }
+    velo_prec_setup(c->stream, c->vprec1, c->g, s1, coef_of(c));
+  }
+  if (prec == DRD_PS_VELO1_MG || prec == DRD_PS_VELO1_MG32) {
+    const bool f32 = prec == DRD_PS_VELO1_MG32;
+    if (c->vmg1 && velo_mg_f32(c->vmg1) != f32) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      drop_graphs(c);
+      velo_mg_destroy(c->vmg1);
+      c->vmg1 = nullptr;
+    }
+    if (!c->vmg1) {
+      int err = 0;
+      c->vmg1 = velo_mg_create(c->g, bdf1_scal(c->s), f32, &err);
+      if (!c->vmg1)
+        return set_err(err ? err : DRD_E_HIP,
+                       "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
+    }
+    velo_mg_setup(c->stream, c->vmg1, coef_of(c));
+  }
   return 0;
 }
 
@@ -802,6 +860,18 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       velo_mg_apply(c->stream, c->vmg, b, x, ks);
       *res = x;
       return 0;
+    case DRD_PS_VELO1_DCT32:
+    case DRD_PS_VELO1_DCT64:
+      if (op != DRD_OP_VELO1 || !c->vprec1 || c->vprec1_fp32 != (prec == DRD_PS_VELO1_DCT32)) break;
+      velo_prec_apply(c->stream, c->vprec1, c->g, b, x, ks);
+      *res = x;
+      return 0;
+    case DRD_PS_VELO1_MG:
+    case DRD_PS_VELO1_MG32:
+      if (op != DRD_OP_VELO1 || !c->vmg1 || velo_mg_f32(c->vmg1) != (prec == DRD_PS_VELO1_MG32)) break;
+      velo_mg_apply(c->stream, c->vmg1, b, x, ks);
+      *res = x;
+      return 0;
     case DRD_PS_PH_DCT:
       if (op != DRD_OP_PH) break;
       RCHK(aux_ready(c));
@@ -876,6 +946,8 @@ static uint64_t graph_print(const drd_ctx* c) {
   ptr(c->rb_c32);
   ptr(c->vprec);
   ptr(c->vmg);
+  ptr(c->vprec1);
+  ptr(c->vmg1);
   ptr(c->vp_work);
   mix(&c->vprec_fp32, sizeof(c->vprec_fp32));
   mix(&c->s, sizeof(c->s));
@@ -935,7 +1007,8 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
                          (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
                           prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
                           prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32 || prec == DRD_PS_PHMU_RBGS ||
-                          prec == DRD_PS_PHMU_RBGS32);
+                          prec == DRD_PS_PHMU_RBGS32 || prec == DRD_PS_VELO1_DCT32 ||
+                          prec == DRD_PS_VELO1_DCT64 || prec == DRD_PS_VELO1_MG || prec == DRD_PS_VELO1_MG32);
   int chunk = 1, total = 0, next_snap = 0;
   const uint64_t print = graphable ? graph_print(c) : 0;
   for (;;) {
@@ -1532,16 +1605,16 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return DRD_PS_IDENTITY;
     case DRD_PREC_JACOBI:
       return DRD_PS_VELO1_JACOBI;
-    case DRD_PREC_DCT32:  // built for the BDF2 operator; the first step keeps Jacobi
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
+    case DRD_PREC_DCT32:  // the first (BDF1) step takes the same preconditioner built for its operator
+      return bdf == 1 ? DRD_PS_VELO1_DCT32 : DRD_PS_VELO_DCT32;
     case DRD_PREC_DCT64:
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT64;
+      return bdf == 1 ? DRD_PS_VELO1_DCT64 : DRD_PS_VELO_DCT64;
     case DRD_PREC_LEARNED:  // plain FP32 DCT unless the correction is judged in (velo_solve_dev)
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_DCT32;
+      return bdf == 1 ? DRD_PS_VELO1_DCT32 : DRD_PS_VELO_DCT32;
     case DRD_PREC_MG:
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_MG;
+      return bdf == 1 ? DRD_PS_VELO1_MG : DRD_PS_VELO_MG;
     case DRD_PREC_MG32:
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_VELO_MG32;
+      return bdf == 1 ? DRD_PS_VELO1_MG32 : DRD_PS_VELO_MG32;
     default:
       return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
   }

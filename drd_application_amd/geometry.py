@@ -83,6 +83,24 @@ def _neumann_ghosts(a: np.ndarray) -> np.ndarray:
     return a
 
 
+def _window(p: abi.Params, cx: float, cz: float, reach: float):
+    """Ghosted index ranges (cols, rows) of the cells whose centres lie within
+    `reach` of (cx, cz) in each coordinate, and those centres."""
+    i0 = max(-1, int(math.floor((cx - reach) / p.dx - 0.5)) - 1)
+    i1 = min(p.nx, int(math.ceil((cx + reach) / p.dx - 0.5)) + 1)
+    k0 = max(-1, int(math.floor((cz - reach) / p.dz - 0.5)) - 1)
+    k1 = min(p.ny, int(math.ceil((cz + reach) / p.dz - 0.5)) + 1)
+    x = (np.arange(i0, i1 + 1) + 0.5) * p.dx
+    z = (np.arange(k0, k1 + 1) + 0.5) * p.dz
+    return slice(k0 + 1, k1 + 2), slice(i0 + 1, i1 + 2), x, z
+
+
+# tanh(-a) is exactly -1.0 in double precision for a >= 19.1, so a droplet or
+# disc changes no cell beyond TANH_SAT * sqrt2 * width from its rim and each
+# is evaluated on that window only (the fields equal the full-grid forms).
+TANH_SAT = 19.5
+
+
 def droplet_phase(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: float, eps: float):
     """phi = max_d tanh((r_d - |x - c_d|)/(sqrt2 eps)), centres uniform in the domain."""
     X, Z = _cell_centres(p)
@@ -91,8 +109,10 @@ def droplet_phase(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: flo
     for _ in range(n):
         cx, cz = rng.uniform(0, L), rng.uniform(0, H)
         r = rng.uniform(rmin, rmax)
-        d = r - np.hypot(X - cx, Z - cz)
-        phi = np.maximum(phi, np.tanh(d / (math.sqrt(2.0) * eps)))
+        ks, is_, x, z = _window(p, cx, cz, r + TANH_SAT * math.sqrt(2.0) * eps)
+        d = r - np.hypot(x[None, :] - cx, z[:, None] - cz)
+        w = phi[ks, is_]
+        np.maximum(w, np.tanh(d / (math.sqrt(2.0) * eps)), out=w)
     return phi
 
 
@@ -100,12 +120,15 @@ def disc_mask(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: float, 
     """bdr_ph = 1/2 (1 + tanh(-d_s/(sqrt2 w))) for the union of n seeded discs."""
     X, Z = _cell_centres(p)
     L, H = p.nx * p.dx, p.ny * p.dz
-    ds = np.full_like(X, np.inf)
+    out = np.zeros_like(X)
     for _ in range(n):
         cx, cz = rng.uniform(0, L), rng.uniform(0, H)
         r = rng.uniform(rmin, rmax)
-        ds = np.minimum(ds, np.hypot(X - cx, Z - cz) - r)
-    return 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width)))
+        ks, is_, x, z = _window(p, cx, cz, r + TANH_SAT * math.sqrt(2.0) * width)
+        ds = np.hypot(x[None, :] - cx, z[:, None] - cz) - r
+        w = out[ks, is_]
+        np.maximum(w, 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width))), out=w)
+    return out
 
 
 def serpentine_mask(p: abi.Params, amp: float, waves: float, half_width: float, width: float):
@@ -287,6 +310,63 @@ def batch_configs(count: int = 256, seed: int = 2000, sizes=(128, 256, 512, 1024
         cfg.notes = f"{nx}x{ny}, eta ratio {lam:.1f}, {n_discs} discs R={r_pen:.3g}, {n_drops} droplets"
         out.append(cfg)
     return out
+
+
+def add_body_mask(p: abi.Params, bdr: np.ndarray, table: np.ndarray, width: float) -> np.ndarray:
+    """max(bdr, 1/2 (1 + tanh(-d/(sqrt2 w)))) for the discs (x, y, r) of table,
+    each evaluated on its own window (the full-grid value beyond it is 0)."""
+    out = bdr.copy()
+    for x, y, r in table[:, :3]:
+        ks, is_, xc, zc = _window(p, x, y, r + TANH_SAT * math.sqrt(2.0) * width)
+        d = np.hypot(xc[None, :] - x, zc[:, None] - y) - r
+        w = out[ks, is_]
+        np.maximum(w, 0.5 * (1.0 + np.tanh(-d / (math.sqrt(2.0) * width))), out=w)
+    return out
+
+
+def c4_config(nx: int = 4096, ny: int = 2048, seed: int = 4000, n_particles: int = 400, n_fibers: int = 64,
+              beads: int = 20, steps: int = 200):
+    """The full combined workload: two-phase flow in the serpentine channel of
+    c1 (centreline 0.5 + 0.3 sin(8 pi x), width 0.25, wall smoothing 2
+    cells) on an nx x ny grid of height 1, 40 droplets of the physical sizes
+    of c1 (radii 20-80 cells at 4096 x 2048), viscosity ratio 100, dt = 2e-5;
+    n_particles rigid discs of radius 6-10 cells and n_fibers chains of
+    `beads` beads of radius 4 cells (touching: spacing 8 cells), seeded
+    inside the channel, all diffuse-resistance domains with R = 1e6
+    (bdr_ph and the penalty mask are the union of the walls and the bodies,
+    smoothed over 2 cells).  Returns (Config, bodies.Bodies)."""
+    from .bodies import Bodies
+    scale = nx / 2048.0
+    cfg = two_phase_config(nx, ny, seed, n_drops=40, r_drop=(10 * scale, 40 * scale), serpentine=True,
+                           lambda_eta=100.0, penalty=1e6, dt=2e-5, steps=steps, name="c4")
+    p = cfg.params
+    lx, dx = nx * p.dx, p.dx
+    rng = SplitMix64(seed + 17)
+
+    def in_channel(x):
+        return 0.5 + 0.3 * math.sin(8.0 * math.pi * x) + rng.uniform(-0.08, 0.08)
+
+    parts = np.zeros((n_particles, 3))
+    for j in range(n_particles):
+        x = rng.uniform(0.0, lx)
+        parts[j] = (x, in_channel(x), rng.uniform(6.0, 10.0) * dx)
+    rb, l0 = 4.0 * dx, 8.0 * dx
+    fibers = []
+    for _ in range(n_fibers):
+        x = rng.uniform(0.0, lx)
+        y = in_channel(x)
+        a = rng.uniform(0.0, 2.0 * math.pi)
+        s = (np.arange(beads) - (beads - 1) / 2.0) * l0
+        fibers.append(np.stack([x + s * math.cos(a), y + s * math.sin(a)], axis=1))
+    b = Bodies(parts, lx, fibers, bead_radius=rb, ks=1.0e3, kb=1.0, l0=l0, theta0=0.0)
+    # the bodies join the walls in bdr_ph; the phase, mobilities and penalty follow
+    table = b.table()
+    walls = cfg.fields[abi.F_BDR_PH]
+    bdr = add_body_mask(p, walls, np.concatenate([table[:, :1], table[:, 1:3]], axis=1), 2 * dx)
+    phi = np.where(walls < 1.0, cfg.fields[abi.F_PH] / np.maximum(1.0 - walls, 1e-300), -1.0)
+    cfg.fields = derived_fields(p, np.clip(phi, -1.0, 1.0), bdr, 100.0)
+    cfg.notes = (f"{nx}x{ny} serpentine, 40 droplets, {n_particles} particles, {n_fibers} fibers of {beads} beads")
+    return cfg, b
 
 
 def config(name: str) -> Config:

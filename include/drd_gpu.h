@@ -467,6 +467,16 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
 #define DRD_PS_VELO_MG 9
 #define DRD_PREC_MG32 7
 #define DRD_PS_VELO_MG32 11
+/* The same fast-path preconditioners built for the BDF1 operator
+ * MatrixVector_uv_CG (TP/VeloSolve_combined.f90:537-675: mass coefficient
+ * re rho instead of 1.5 re rho, no penalty term); op must be DRD_OP_VELO1.
+ * drd_step's first (BDF1) step takes them in the DCT, learned and multigrid
+ * modes, where the reference's Jacobi needs more than its 800 iterations on
+ * large grids.  No reference counterpart.                                    */
+#define DRD_PS_VELO1_DCT32 13
+#define DRD_PS_VELO1_DCT64 14
+#define DRD_PS_VELO1_MG 15
+#define DRD_PS_VELO1_MG32 16
 
 #define DRD_LEARN_OFF 0      /* not enabled: the plain method               */
 #define DRD_LEARN_JUDGE 1    /* the next velocity solve is judged           */

@@ -71,3 +71,31 @@ def test_probed_velocity_diagonal_is_the_diagonal():
     d = probe_diag(o, abi.OP_VELO2, p)
     want = np.array([o.matvec(abi.OP_VELO2, 1.0, np.eye(1, n, q).ravel(), 0.0, np.zeros(n))[q] for q in range(n)])
     np.testing.assert_array_equal(d, want)
+
+
+def test_bdf1_operator_is_the_bdf2_stencil_with_scaled_mass_and_no_penalty():
+    """MatrixVector_uv_CG (BDF1) equals MatrixVector_uv_2_CG (BDF2) with re / 1.5
+    and penalty_k = 0 (the fast-path BDF1 preconditioners rest on this), up
+    to the rounding of 1.5 (re / 1.5)."""
+    import numpy as np
+    from drd_application_amd import abi, geometry
+    from oracle.oracle import Oracle
+    cfg = geometry.two_phase_config(48, 24, 3, n_drops=3, r_drop=(3, 6), n_discs=2, r_disc=(3, 5),
+                                    lambda_eta=30.0, penalty=1e5, dt=5e-5)
+    p = cfg.params
+    q = p.copy()
+    q.re = p.re / 1.5
+    q.penalty_k = 0.0
+    o1, o2 = Oracle(p), Oracle(q)
+    geometry.load(o1, cfg)
+    geometry.load(o2, cfg)
+    n = abi.n_uv(p)
+    x = np.random.default_rng(1).standard_normal(n)
+    y1 = o1.matvec(abi.OP_VELO1, 1.0, x, 0.0, np.zeros(n))
+    y2 = o2.matvec(abi.OP_VELO2, 1.0, x, 0.0, np.zeros(n))
+    assert np.linalg.norm(y1 - y2) <= 1e-15 * np.linalg.norm(y1)
+    # and with the penalty kept, they differ (the BDF1 operator has none)
+    o3 = Oracle(p)
+    geometry.load(o3, cfg)
+    y3 = o3.matvec(abi.OP_VELO2, 1.0, x, 0.0, np.zeros(n))
+    assert np.linalg.norm(y1 - y3) > 1e-3 * np.linalg.norm(y1)

@@ -32,45 +32,7 @@ import pytest
 from drd_application_amd import abi, geometry
 from oracle.oracle import Oracle
 
-STATE = [abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_U_PRE, abi.F_V_PRE, abi.F_PH_PRE,
-         abi.F_RMU_PRE, abi.F_PSI1, abi.F_PSI2, abi.F_PH_M_0]
-
-
-def pack_uv(u, v):
-    ny = u.shape[0] - 2
-    nx = u.shape[1] - 3
-    return np.concatenate([u[1:ny + 1, 1:nx + 2].ravel(), v[1:ny, 1:nx + 1].ravel()])
-
-
-def pack_phmu(ph, mu):
-    return np.concatenate([ph[1:-1, 1:-1].ravel(), mu[1:-1, 1:-1].ravel()])
-
-
-def cell_div(p, u, v):
-    nx, ny = p.nx, p.ny
-    return (u[1:ny + 1, 2:nx + 2] - u[1:ny + 1, 1:nx + 1]) / p.dx + (v[1:ny + 1, 1:nx + 1] - v[0:ny, 1:nx + 1]) / p.dz
-
-
-def neumann_lap(p, psi):
-    """5-point Laplacian on the cell interior with the ghosts copying their
-    interior neighbour (psi(-1) = psi(0), ...), built here from the stencil."""
-    a = psi[1:-1, 1:-1]
-    g = np.pad(a, 1, mode="edge")
-    return (g[1:-1, 2:] - 2 * a + g[1:-1, :-2]) / p.dx ** 2 + (g[2:, 1:-1] - 2 * a + g[:-2, 1:-1]) / p.dz ** 2
-
-
-def snapshot(o):
-    return {f: o.get_field(f) for f in STATE}
-
-
-def restore(o, s):
-    for f, a in s.items():
-        o.set_field(f, a)
-
-
-def rel_resid(o, op, x, b):
-    r = b - o.matvec(op, 1.0, x, 0.0, np.zeros_like(b))
-    return np.linalg.norm(r) / np.linalg.norm(b)
+from step_check import cell_div, neumann_lap, projection_error, restore, snapshot, sub_solve_residuals
 
 
 @pytest.fixture(scope="module")
@@ -97,28 +59,11 @@ def stepped():
 
 def test_step_sub_solves_satisfy_the_sequence(stepped):
     p, o, rec = stepped
-    z = np.zeros(abi.cell_shape(p))
     for bdf, t, before, after in rec:
-        # Cahn-Hilliard from phi^n, u^n (ph_mu_rhs also forms M_0 from phi^n)
-        restore(o, before)
-        bch = o.ph_mu_rhs()
-        xch = pack_phmu(after[abi.F_PH], after[abi.F_RMU])
-        r_ch = rel_resid(o, abi.OP_PHMU, xch, bch)
-        # momentum from u^n, u^{n-1}, p^n, psi^n and the new phi, mu
-        op = abi.OP_VELO1 if bdf == 1 else abi.OP_VELO2
-        kw = dict(ph_after=after[abi.F_PH], rmu_after=after[abi.F_RMU], t=t + p.dt, pf1=z, pf2=z)
-        if bdf == 2:
-            kw.update(psi1=before[abi.F_PSI1], psi2=before[abi.F_PSI2], u_pre=before[abi.F_U_PRE],
-                      v_pre=before[abi.F_V_PRE])
-        bv = o.velo_rhs(bdf, **kw)
-        xv = pack_uv(after[abi.F_U], after[abi.F_V])
-        r_v = rel_resid(o, op, xv, bv)
-        # a shifted sequence: the momentum equation with the old phi and mu
-        kw_old = dict(kw, ph_after=before[abi.F_PH], rmu_after=before[abi.F_RMU])
-        r_v_old = rel_resid(o, op, xv, o.velo_rhs(bdf, **kw_old))
-        print(f"bdf{bdf}: CH {r_ch:.2e}, momentum {r_v:.2e} (old phi, mu: {r_v_old:.2e})")
-        assert r_ch < 1e-9 and r_v < 1e-9
-        assert r_v_old > 1e3 * r_v
+        r = sub_solve_residuals(o, p, bdf, t, before, after, shifted=True)
+        print(f"bdf{bdf}: CH {r['ch']:.2e}, momentum {r['velo']:.2e} (old phi, mu: {r['velo_old_phi']:.2e})")
+        assert r["ch"] < 1e-9 and r["velo"] < 1e-9
+        assert r["velo_old_phi"] > 1e3 * r["velo"]
     restore(o, rec[-1][3])
 
 
@@ -131,11 +76,10 @@ def test_step_projection_and_bookkeeping(stepped):
         div = cell_div(p, after[abi.F_U], after[abi.F_V])
         lap = neumann_lap(p, psi) / coef
         scale = np.abs(div).max()
-        e = np.abs(lap - (div - div.mean())).max() / scale
-        m = abs(psi[1:-1, 1:-1].mean() / coef + div.mean()) / scale
+        e, m = projection_error(p, bdf, after)
         # the same identity with the BDF order's other coefficient, and with u^n
         wrong = 3 * p.re * gamma1 / 2.0 / p.dt if bdf == 1 else p.re * gamma1 / p.dt
-        e_coef = np.abs(neumann_lap(p, psi) / wrong - (div - div.mean())).max() / scale
+        e_coef, _ = projection_error(p, bdf, after, coef=wrong)
         div_old = cell_div(p, before[abi.F_U], before[abi.F_V])
         e_old = np.abs(lap - (div_old - div_old.mean())).max() / max(np.abs(div_old).max(), scale)
         print(f"bdf{bdf}: lap/coef vs div {e:.2e}, mean {m:.2e}; other coef {e_coef:.2e}; u^n {e_old:.2e}")
