@@ -46,28 +46,55 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """(threads used, affinity count, os.cpu_count, CPU model) of this host.
+    The threads used are the affinity count, capped by OMP_NUM_THREADS when
+    the environment sets it (the share of the host the GPU's job owns)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    use = avail
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        use = min(use, int(env))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return use, avail, os.cpu_count(), model
+
+
 def cpu_baseline(cfg, velo_iters: float, phmu_iters: float):
     """Times the CPU oracle (the FP64 C restatement of the reference's path,
-    OpenMP loops whose results do not depend on the thread count) on a
-    bounded sample: k BiCGSTAB iterations of the momentum and of the CH solve
-    on the same fields with the reference's preconditioners, scaled by the
-    GPU drop-in run's iteration counts per step; at one thread (the
-    reference is serial) and at the host's cores (at most 16)."""
+    OpenMP loops whose results do not depend on the thread count) on bounded
+    samples, at one thread (the reference is serial) and at the host's cores:
+
+    * the workload itself: the step's right-hand-side assemblies and pressure
+      projection once, plus k BiCGSTAB iterations of the momentum and of the
+      CH solve with the reference's preconditioners, scaled by the GPU
+      drop-in run's iteration counts per step;
+    * whole steps, end to end, of c0 (256 x 128): a BDF1 and two BDF2 steps
+      with the reference's velocity preconditioners and the block CH
+      preconditioner (the reference's scalar Jacobi stops at its 5000
+      iterations there without converging)."""
     import numpy as np
 
     from drd_application_amd import abi, geometry
     from oracle import oracle as O
 
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
-    nthreads = max(1, min(16, avail))
+    nthreads, avail, ncpu, model = host_cores()
     p = cfg.params
     k = 4
     nv = abi.n_uv(p)
     bv = np.random.default_rng(1).standard_normal(nv)
     nc = 2 * p.nx * p.ny
+    cs = abi.cell_shape(p)
     res = {}
     for nt in sorted({1, nthreads}):
         O.set_threads(nt)
@@ -77,14 +104,37 @@ def cpu_baseline(cfg, velo_iters: float, phmu_iters: float):
         o.bicgstab(abi.OP_VELO2, abi.PS_IDENTITY, bv, np.zeros(nv), 1, 1e-300)  # first touch, thread start
         o.bicgstab(abi.OP_PHMU, abi.PS_PHMU_JACOBI, bc, np.zeros(nc), 1, 1e-300)
         t0 = time.perf_counter()
+        o.ph_mu_rhs()
+        z = np.zeros(cs)
+        o.velo_rhs(2, psi1=z, psi2=z, u_pre=o.get_field(abi.F_U), v_pre=o.get_field(abi.F_V),
+                   ph_after=o.get_field(abi.F_PH), rmu_after=o.get_field(abi.F_RMU), t=p.dt)
+        o.PreSolve_2(3 * p.re / 2.0 / p.dt)
+        tfix = time.perf_counter() - t0
+        t0 = time.perf_counter()
         _, itv, _, _ = o.bicgstab(abi.OP_VELO2, abi.PS_IDENTITY, bv, np.zeros(nv), k, 1e-300)
         tv = (time.perf_counter() - t0) / max(itv, 1)
         t0 = time.perf_counter()
         _, itc, _, _ = o.bicgstab(abi.OP_PHMU, abi.PS_PHMU_JACOBI, bc, np.zeros(nc), k, 1e-300)
         tc = (time.perf_counter() - t0) / max(itc, 1)
         o.close()
-        step_s = velo_iters * tv + phmu_iters * tc
-        res[nt] = (1.0 / step_s if step_s > 0 else None, tv, tc, itv, itc)
+        step_s = tfix + velo_iters * tv + phmu_iters * tc
+        res[nt] = (1.0 / step_s if step_s > 0 else None, tv, tc, itv, itc, tfix)
+    # c0 whole steps
+    c0 = geometry.config("c0")
+    c0.params.ch_prec = abi.CH_PREC_BLOCK
+    whole = {}
+    for nt in sorted({1, nthreads}):
+        O.set_threads(nt)
+        o = O.Oracle(c0.params)
+        geometry.load(o, c0)
+        t, its = 0.0, []
+        t0 = time.perf_counter()
+        for s in range(3):
+            info = o.step(1 if s == 0 else 2, t)
+            its.append((info["phmu"]["iter"], info["velo"]["iter"]))
+            t += c0.params.dt
+        whole[nt] = {"steps_per_s": 3 / (time.perf_counter() - t0), "iters": its}
+        o.close()
     O.set_threads(1)
     v1, vN = res[1], res[nthreads]
     return {
@@ -93,11 +143,18 @@ def cpu_baseline(cfg, velo_iters: float, phmu_iters: float):
         "cores": nthreads,
         "kind": "port",
         "value_1thread": v1[0],
-        "sample": (f"oracle (C, FP64, OpenMP) on the {cfg.name} fields: {vN[3]} momentum and {vN[4]} CH "
-                   f"BiCGSTAB iterations with the reference's preconditioners timed at {nthreads} threads "
-                   f"({vN[1]*1e3:.1f} and {vN[2]*1e3:.1f} ms each; at 1 thread {v1[1]*1e3:.1f} and "
-                   f"{v1[2]*1e3:.1f} ms), scaled by the GPU drop-in run's {velo_iters:.1f} + {phmu_iters:.1f} "
-                   f"iterations per step; RHS assembly and the pressure solve are not included"),
+        "host": {"threads_used": nthreads, "affinity_cpus": avail, "os_cpus": ncpu, "model": model},
+        "sample": (f"oracle (C, FP64, OpenMP) on the {cfg.name} fields at {nthreads} threads: the step's RHS "
+                   f"assemblies and pressure projection ({vN[5]*1e3:.0f} ms; {v1[5]*1e3:.0f} ms at 1 thread) "
+                   f"plus {vN[3]} momentum and {vN[4]} CH BiCGSTAB iterations with the reference's "
+                   f"preconditioners ({vN[1]*1e3:.1f} and {vN[2]*1e3:.1f} ms each; at 1 thread "
+                   f"{v1[1]*1e3:.1f} and {v1[2]*1e3:.1f} ms), scaled by the GPU drop-in run's "
+                   f"{velo_iters:.1f} + {phmu_iters:.1f} iterations per step"),
+        "c0_whole_steps": {
+            "workload": "c0 256x128, BDF1 + 2 BDF2 steps end to end (RHS, CH, momentum, pressure), reference "
+                        "velocity preconditioners, block CH preconditioner, tol 1e-10",
+            "steps_per_s": whole[nthreads]["steps_per_s"], "steps_per_s_1thread": whole[1]["steps_per_s"],
+            "iters": whole[nthreads]["iters"]},
     }
 
 
@@ -290,10 +347,11 @@ def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: i
 def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cpu: bool = True):
     """The particle coupling reductions of one step (force_caculation_vel and
     update_position_vel, TP/particles_move.f90:758-920): `nbody` bodies spread
-    over and averaged against every cell of the grid, O(nbody x cells) weight
-    evaluations each.  Each call includes its upload of the bodies and, for the
-    velocity averages, the read-back.  The CPU figure is the oracle on two
-    bodies, one thread, scaled to `nbody`."""
+    over and averaged against the grid (the reference evaluates every body at
+    every cell, O(nbody x cells)).  Each call includes its upload of the bodies and, for the
+    velocity averages, the read-back.  The device evaluates each body only
+    on the cells within its tanh cutoff; the CPU figure is the oracle (every
+    body at every cell) on two bodies, one thread, scaled to `nbody`."""
     import numpy as np
     from drd_application_amd import geometry
     from drd_application_amd.solver import GpuSolver
@@ -318,9 +376,6 @@ def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cp
             out[name] = (time.perf_counter() - t0) / reps * 1e3
     finally:
         g.close()
-    evals = float(nbody) * p.nx * p.ny
-    # the force spread evaluates every weight twice (sums, then spread)
-    out["weight_evals_per_s"] = 2 * evals / (out["force_ms"] * 1e-3)
     if cpu:
         from oracle.oracle import Oracle
         o = Oracle(p)
@@ -330,8 +385,58 @@ def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cp
         dt = time.perf_counter() - t0
         o.close()
         out["cpu_force_ms"] = dt / 2 * nbody * 1e3
-        out["cpu_sample"] = "oracle, 2 bodies, 1 thread, scaled to the body count"
+        out["cpu_sample"] = ("oracle, 2 bodies, 1 thread, scaled to the body count; the oracle evaluates every "
+                             "body at every cell (no cutoff), the device skips cells beyond a body's tanh cutoff, "
+                             "so the ratio mixes that pruning with the hardware")
     return out
+
+
+def combined_c4(steps: int = 10, warmup: int = 2):
+    """The full combined workload (geometry.c4_config): 4096 x 2048 serpentine
+    channel, 40 droplets, 400 particles and 64 fibers of 20 beads as
+    diffuse-resistance domains, headline preconditioners, dt 2e-5.  One
+    coupled step = the bodies' forces spread on the device
+    (drd_particle_force), the time step, and the bodies moved on the host
+    with their device-averaged velocities (drd_particle_velocity)."""
+    from drd_application_amd import abi, geometry
+    from drd_application_amd.solver import GpuSolver
+
+    t0 = time.perf_counter()
+    cfg, bodies = geometry.c4_config()
+    gen_s = time.perf_counter() - t0
+    q = cfg.params.copy()
+    vp, cp = MODES["mg32"]
+    q.velo_prec, q.ch_prec = getattr(abi, vp), getattr(abi, cp)
+    lx, ly = q.nx * q.dx, q.ny * q.dz
+    g = GpuSolver(q)
+    try:
+        geometry.load(g, cfg)
+        g.set_initial_guess(1)
+        t = 0.0
+        for s in range(warmup):
+            bodies.spread(g, lx, ly)
+            info = g.step(1 if s == 0 else 2, t)
+            bodies.advance(g, q.dt, lx, ly)
+            t += q.dt
+        g.synchronize()
+        its, ok = [], True
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bodies.spread(g, lx, ly)
+            info = g.step(2, t)
+            bodies.advance(g, q.dt, lx, ly)
+            t += q.dt
+            its.append((info["phmu"]["iter"], info["velo"]["iter"]))
+            ok = ok and info["phmu"]["info"] == 0 and info["velo"]["info"] == 0
+        g.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        g.close()
+    log(f"[c4] {cfg.notes}: {steps} coupled steps in {el:.3f} s ({steps/el:.2f} steps/s); CH/velocity its {its}")
+    return {"workload": cfg.notes + ", mg32 preconditioners, dt 2e-5, tol 1e-10",
+            "bodies": bodies.count, "generated_s": gen_s, "warmup": warmup, "steps": steps,
+            "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "converged": ok,
+            "mean_iters": {"phmu": sum(a for a, _ in its) / steps, "velo": sum(b for _, b in its) / steps}}
 
 
 def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4):
@@ -527,6 +632,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-c3", action="store_true", help="skip the learned correction's own workload")
     ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle coupling reductions")
+    ap.add_argument("--no-c4", action="store_true", help="skip the full combined 4096x2048 workload")
     ap.add_argument("--profile-network", action="store_true",
                     help="only launch the learned correction's 1024^2 network forward pass")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -600,6 +706,12 @@ def main(argv=None) -> int:
             pc = particle_coupling(cfg.name, cpu=not args.no_cpu)
         except Exception as e:  # reported beside the metric, never required for it
             log(f"particle coupling failed: {e}")
+    c4 = None
+    if not args.no_c4:
+        try:
+            c4 = combined_c4()
+        except Exception as e:  # reported beside the metric, never required for it
+            log(f"combined workload failed: {e}")
     c3 = None
     if not args.no_c3:
         try:
@@ -632,6 +744,9 @@ def main(argv=None) -> int:
                                           if "fp64dct" in modes and args.mode != "fp64dct" else None),
                    "speedup_vs_dropin": (head["steps_per_s"] / modes["dropin"]["steps_per_s"]
                                          if "dropin" in modes and args.mode != "dropin" else None),
+                   # the metric's own ratio: learned + FP32-preconditioned against the plain FP64 DCT
+                   "speedup_learned_vs_fp64dct": (modes["learned"]["steps_per_s"] / modes["fp64dct"]["steps_per_s"]
+                                                  if "learned" in modes and "fp64dct" in modes else None),
                    "field_rel_l2_vs_fp64dct": field_err,
                    "kernels": kern},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -644,6 +759,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "learned_c3": c3,
         "batched_c2": c2,
+        "combined_c4": c4,
         "particle_coupling": pc,
     }
     print(json.dumps(result))
