@@ -391,7 +391,7 @@ def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cp
     return out
 
 
-def combined_c4(steps: int = 10, warmup: int = 2):
+def combined_c4(steps: int = 20, warmup: int = 2):
     """The full combined workload (geometry.c4_config): 4096 x 2048 serpentine
     channel, 40 droplets, 400 particles and 64 fibers of 20 beads as
     diffuse-resistance domains, headline preconditioners, dt 2e-5.  One

@@ -333,8 +333,10 @@ def c4_config(nx: int = 4096, ny: int = 2048, seed: int = 4000, n_particles: int
     n_particles rigid discs of radius 6-10 cells and n_fibers chains of
     `beads` beads of radius 4 cells (touching: spacing 8 cells), seeded
     inside the channel, all diffuse-resistance domains with R = 1e6
-    (bdr_ph and the penalty mask are the union of the walls and the bodies,
-    smoothed over 2 cells).  Returns (Config, bodies.Bodies)."""
+    (bdr_ph and the penalty mask are the union of the walls and the bodies
+    at their starting positions, smoothed over 2 cells; the bodies move
+    about a cell in 200 steps, and the mask is not rebuilt as they do).
+    Returns (Config, bodies.Bodies)."""
     from .bodies import Bodies
     scale = nx / 2048.0
     cfg = two_phase_config(nx, ny, seed, n_drops=40, r_drop=(10 * scale, 40 * scale), serpentine=True,
@@ -358,7 +360,12 @@ def c4_config(nx: int = 4096, ny: int = 2048, seed: int = 4000, n_particles: int
         a = rng.uniform(0.0, 2.0 * math.pi)
         s = (np.arange(beads) - (beads - 1) / 2.0) * l0
         fibers.append(np.stack([x + s * math.cos(a), y + s * math.sin(a)], axis=1))
-    b = Bodies(parts, lx, fibers, bead_radius=rb, ks=1.0e3, kb=1.0, l0=l0, theta0=0.0)
+    # Spring and bending constants for a stable explicit coupling at this dt:
+    # a stretch d of one bond accelerates the fluid of a bead (area pi rb^2,
+    # density 1) by ks d / (pi rb^2), so dt^2 ks / (pi rb^2) stays below 1e-2;
+    # the bending force kb G / l0 / rd ~ kb d / l0^3 is held to a tenth of it.
+    ks = 1e-2 * math.pi * rb * rb / (p.dt * p.dt)
+    b = Bodies(parts, lx, fibers, bead_radius=rb, ks=ks, kb=0.1 * ks * l0 * l0, l0=l0, theta0=0.0)
     # the bodies join the walls in bdr_ph; the phase, mobilities and penalty follow
     table = b.table()
     walls = cfg.fields[abi.F_BDR_PH]
