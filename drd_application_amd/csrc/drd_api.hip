@@ -124,6 +124,7 @@ struct drd_ctx {
   int guess = 0;  // drd_set_initial_guess
   std::map<GraphKey, GraphEntry> graphs;
   int64_t graph_recaptures = 0;  // replays refused for a changed fingerprint
+  int pred_its[4] = {0, 0, 0, 0};  // per operator: iterations of its latest BiCGSTAB solve
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
   std::vector<hipEvent_t> tm_ev;
@@ -927,7 +928,7 @@ static int true_resid(drd_ctx* c, int op, int64_t n, const double* b, const doub
 // Everything a captured BiCGSTAB chunk bakes into its kernels besides the
 // GraphKey: FNV-1a over the workspace, coefficient and preconditioner
 // pointers, the scalars, the grid and the batch group.
-static uint64_t graph_print(const drd_ctx* c) {
+static uint64_t graph_print(const drd_ctx* c, int prec) {
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](const void* p, size_t n) {
     const unsigned char* q = static_cast<const unsigned char*>(p);
@@ -944,12 +945,14 @@ static uint64_t graph_print(const drd_ctx* c) {
   ptr(c->diag);
   ptr(c->rb_work);
   ptr(c->rb_c32);
-  ptr(c->vprec);
-  ptr(c->vmg);
-  ptr(c->vprec1);
-  ptr(c->vmg1);
+  // the preconditioner object the chunk's kernels read (others may come and go)
+  if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64) ptr(c->vprec);
+  if (prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32) ptr(c->vmg);
+  if (prec == DRD_PS_VELO1_DCT32 || prec == DRD_PS_VELO1_DCT64) ptr(c->vprec1);
+  if (prec == DRD_PS_VELO1_MG || prec == DRD_PS_VELO1_MG32) ptr(c->vmg1);
   ptr(c->vp_work);
   mix(&c->vprec_fp32, sizeof(c->vprec_fp32));
+  mix(&c->vprec1_fp32, sizeof(c->vprec1_fp32));
   mix(&c->s, sizeof(c->s));
   mix(&c->g, sizeof(c->g));
   const BatchCfg bc = batch_cfg();
@@ -1009,8 +1012,14 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
                           prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32 || prec == DRD_PS_PHMU_RBGS ||
                           prec == DRD_PS_PHMU_RBGS32 || prec == DRD_PS_VELO1_DCT32 ||
                           prec == DRD_PS_VELO1_DCT64 || prec == DRD_PS_VELO1_MG || prec == DRD_PS_VELO1_MG32);
-  int chunk = 1, total = 0, next_snap = 0;
-  const uint64_t print = graphable ? graph_print(c) : 0;
+  // The first chunk covers one iteration less than the previous solve of
+  // this operator took (solves of successive steps need similar counts);
+  // later chunks grow 1, 2, 4, ... 32.  A chunk queued past convergence
+  // costs one dispatch per kernel, so the tail a power-of-two schedule
+  // leaves (up to half the solve) is what this saves.
+  const int pred = (op >= 0 && op < 4) ? c->pred_its[op] : 0;
+  int chunk = pred > 2 ? std::min(pred - 1, maxit) : 1, after = 1, total = 0, next_snap = 0;
+  const uint64_t print = graphable ? graph_print(c, prec) : 0;
   for (;;) {
     hipGraphExec_t gx = nullptr;
     CaptureGuard cap{c->stream};
@@ -1071,11 +1080,16 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
     total += chunk;
     RCHK(read_state(c));
     if (all_done(c) || total > maxit + 2) break;
-    chunk = std::min(chunk * 2, 32);
+    chunk = after;
+    after = std::min(after * 2, 32);
   }
   if (!all_done(c)) return set_err(DRD_E_STATE, "BiCGSTAB state machine did not terminate");
-  for (int i = 0; i < c->nb; ++i)
+  int most = 0;
+  for (int i = 0; i < c->nb; ++i) {
     c->last[i] = drd_solve_info{c->ks_host[i].iter, c->ks_host[i].info, c->ks_host[i].resid, 0.0};
+    most = std::max(most, (int)c->ks_host[i].iter);
+  }
+  if (op >= 0 && op < 4) c->pred_its[op] = most;
   const KState& s = *c->ks_host;
   if (info) {
     info->iter = s.iter;
