@@ -439,7 +439,7 @@ def combined_c4(steps: int = 20, warmup: int = 2):
             "mean_iters": {"phmu": sum(a for a, _ in its) / steps, "velo": sum(b for _, b in its) / steps}}
 
 
-def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4):
+def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4, graphs: bool = True):
     """The batched workload: `count` independent instances of uneven sizes
     (geometry.batch_configs) stepped together by drd_batch_step, every size
     group's kernels launched once over all its instances, BiCGSTAB iteration
@@ -495,20 +495,23 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
             bad += sum(1 for i in infos if i["velo"]["info"] != 0 or i["phmu"]["info"] != 0)
         return time.perf_counter() - t1, its_v / n / count, its_c / n / count, bad
     el, iv, ic, bad = timed(steps)
-    # the same warm-up and timed steps again from the same fields, launched
-    # without graphs (later steps need fewer iterations, so only the same
-    # steps compare)
-    load()
-    b.use_graphs(False)
-    warm()
-    el_ng, iv_ng, ic_ng, _ = timed(steps)
-    el_ng /= steps
-    if (iv_ng, ic_ng) != (iv, ic):
-        log(f"[c2] the runs with and without graphs iterated differently: {iv}/{ic} and {iv_ng}/{ic_ng}")
+    # the same warm-up and timed steps again from the same fields, with the
+    # iteration chunks replayed from captured graphs (later steps need fewer
+    # iterations, so only the same steps compare); skipped under a profiler,
+    # whose kernel tracer has ended graph replays with a host SIGSEGV
+    el_g = None
+    if graphs:
+        load()
+        b.use_graphs(True)
+        warm()
+        el_g, iv_g, ic_g, _ = timed(steps)
+        el_g /= steps
+        if (iv_g, ic_g) != (iv, ic):
+            log(f"[c2] the runs with and without graphs iterated differently: {iv}/{ic} and {iv_g}/{ic_g}")
     b.close()
     log(f"[c2] batched: {steps} steps in {el:.3f} s ({el/steps*1e3:.0f} ms/step, {count*steps/el:.0f} "
-        f"instance-steps/s); no graphs {el_ng*1e3:.0f} ms/step; mean its velo {iv:.1f}, CH {ic:.1f}; "
-        f"non-converged solves {bad}")
+        f"instance-steps/s); with graphs {el_g * 1e3 if el_g else float('nan'):.0f} ms/step; mean its velo "
+        f"{iv:.1f}, CH {ic:.1f}; non-converged solves {bad}")
     # sequential estimate: one instance per size, its own context
     seq = 0.0
     per_size = {}
@@ -541,7 +544,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
                     f"FP64 BiCGSTAB, tol 1e-10",
         "cells": cells, "groups": [list(gr) for gr in tuned],
         "ms_per_batch_step": el / steps * 1e3, "instance_steps_per_s": count * steps / el,
-        "cell_steps_per_s": cells * steps / el, "ms_per_batch_step_no_graphs": el_ng * 1e3,
+        "cell_steps_per_s": cells * steps / el, "ms_per_batch_step_graphs": el_g * 1e3 if el_g else None,
         "sequential_ms_per_step_estimate": seq, "speedup_vs_sequential": seq / (el / steps * 1e3),
         "mean_iters": {"velo": iv, "phmu": ic}, "non_converged_solves": bad,
         "per_size_alone": per_size,
@@ -633,6 +636,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle coupling reductions")
     ap.add_argument("--no-c4", action="store_true", help="skip the full combined 4096x2048 workload")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="skip the batched workload's run with captured graphs (for profiled runs)")
     ap.add_argument("--profile-network", action="store_true",
                     help="only launch the learned correction's 1024^2 network forward pass")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -697,7 +702,7 @@ def main(argv=None) -> int:
     c2 = None
     if not args.no_c2:
         try:
-            c2 = batched_c2()
+            c2 = batched_c2(graphs=not args.no_graphs)
         except Exception as e:  # reported beside the metric, never required for it
             log(f"batched workload failed: {e}")
     pc = None

@@ -2357,8 +2357,7 @@ extern "C" int drd_batch_create(int32_t count, const drd_params* params, drd_bat
       drd_batch_destroy(b);
       return rc;
     }
-    c->use_graphs = true;
-    b->groups.push_back(c);
+    b->groups.push_back(c);  // captured chunks are opt-in (drd_batch_use_graphs)
     b->infos.emplace_back(counts[gI]);
   }
   *out = b;

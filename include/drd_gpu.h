@@ -287,12 +287,14 @@ int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info* infos);
 int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, int32_t cap);
 
 /* ---- captured iteration chunks (no reference counterpart) ----------------
- * With graphs on, BiCGSTAB queues its iterations in chunks (1, 2, 4, ... 32
- * between polls of the state machine) and each chunk of two or more is a
- * hipGraph captured on first use and replayed afterwards (one graph per
- * operator, preconditioner, chunk length and vector pair; kernels only).
- * Results are bit for bit those without graphs.  Off by default for
- * contexts, on by default for batches.                                     */
+ * BiCGSTAB queues its iterations in chunks between polls of the state
+ * machine (the first chunk one iteration shorter than the operator's
+ * previous solve, then 1, 2, 4, ... 32).  With graphs on, each chunk of two
+ * or more is a hipGraph captured on first use and replayed afterwards (one
+ * graph per operator, preconditioner, chunk length and vector pair; kernels
+ * only; replayed only while the pointers and scalars it baked in are
+ * unchanged).  Results are bit for bit those without graphs.  Off by
+ * default for contexts and batches.                                         */
 int drd_use_graphs(drd_ctx* ctx, int32_t on);
 /* Captured chunks held now (*live) and replays refused since the context was
  * made because something a chunk bakes in had changed (*recaptured; such a

@@ -4,7 +4,7 @@ Every instance of a batch runs the same kernels as a single-instance context,
 with its own state machine and its own canonical reductions, so each
 instance's fields and Krylov reports must be bit for bit those of the same
 instance stepped alone (drd_init/drd_step).  Batches replay captured hipGraph
-chunks by default; the single-instance runs they are compared with do not,
+chunks when asked (drd_batch_use_graphs); the single-instance runs they are compared with do not,
 and a single context with graphs on is compared step by step with one
 without (every replay after eager work and synchronisation).
 """
@@ -42,10 +42,12 @@ def test_batch_groups_and_field_round_trip():
                 np.testing.assert_array_equal(b.get_field(i, fid), cfg.fields[fid])
 
 
-@pytest.mark.parametrize("steps,ch", [(4, None), (3, "CH_PREC_RBGS32"), (3, "CH_PREC_RBGS")])
-def test_batch_steps_are_bitwise_the_single_instance_steps(steps, ch):
+@pytest.mark.parametrize("steps,ch,graphs", [(4, None, True), (3, "CH_PREC_RBGS32", False),
+                                            (3, "CH_PREC_RBGS", True)])
+def test_batch_steps_are_bitwise_the_single_instance_steps(steps, ch, graphs):
     """ch: the CH preconditioner for every instance (None: the configs' own);
-    the red-black forms keep their iterates in a per-instance scratch."""
+    the red-black forms keep their iterates in a per-instance scratch.
+    graphs: the batch replays captured iteration chunks (opt-in)."""
     from drd_application_amd.solver import GpuBatch, GpuSolver
     cfgs = _configs()
     if ch is not None:
@@ -54,6 +56,7 @@ def test_batch_steps_are_bitwise_the_single_instance_steps(steps, ch):
     print("batch of", [c.notes for c in cfgs], "ch", ch, flush=True)
     with GpuBatch([c.params for c in cfgs]) as b:
         _load_batch(b, cfgs)
+        b.use_graphs(graphs)
         tuned = b.tune()
         print("tuned", tuned, flush=True)
         assert all(0 <= t[2] < 3 and all(m > 0 for m in t[3]) for t in tuned)
