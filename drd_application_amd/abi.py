@@ -59,6 +59,7 @@ PS_VELO_MG = 9
 PS_VELO_MG32 = 11
 PS_PHMU_RBGS = 10
 PS_PHMU_RBGS32 = 12
+PS_VELO2_JACOBI = 17
 PS_VELO1_DCT32 = 13
 PS_VELO1_DCT64 = 14
 PS_VELO1_MG = 15

@@ -257,7 +257,8 @@ static Scal make_scal(const drd_params& p) {
   const double tmp = p.I_thickness / p.dz;
   s.coef_bot = (p.I_thickness / p.dz) / tmp;
   s.coef_top = (p.I_thickness / p.dz) / tmp;
-  s.lr_d = (p.lr_bd_type == 'd');
+  s.lr_d = (p.lr_bd_type == 'd') || p.variant == DRD_VARIANT_MF;
+  s.mf = p.variant == DRD_VARIANT_MF;
   s.phi_mode = p.phi_mode;
   scal_derive(s);
   return s;
@@ -521,7 +522,8 @@ static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
   if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
     return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
   if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
-  if (prm->variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "only the TP variant is built");
+  if (prm->variant != DRD_VARIANT_TP && prm->variant != DRD_VARIANT_MF)
+    return set_err(DRD_E_ARG, "variant must be DRD_VARIANT_TP or DRD_VARIANT_MF");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(DRD_E_HIP, "no HIP device visible");
@@ -577,7 +579,8 @@ extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
     return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
   if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
-  if (prm->variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "only the TP variant is built");
+  if (prm->variant != DRD_VARIANT_TP && prm->variant != DRD_VARIANT_MF)
+    return set_err(DRD_E_ARG, "variant must be DRD_VARIANT_TP or DRD_VARIANT_MF");
   if (std::memcmp(prm, &c->p, sizeof(drd_params)) == 0) return 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   drop_graphs(c);
@@ -636,7 +639,8 @@ static Coef coef_of(const drd_ctx* c) {
   Coef k{};
   k.rho = c->f[DRD_F_RHO];
   k.eta = c->f[DRD_F_ETA];
-  k.chi = c->f[DRD_F_BDR_PH_PENALTY];
+  // the BDF2 penalty mask: bdr_ph_penalty (TP), the smooth bdr_ph (MF/VeloSolve_2_combined.f90:580, :613)
+  k.chi = c->p.variant == DRD_VARIANT_MF ? c->f[DRD_F_BDR_PH] : c->f[DRD_F_BDR_PH_PENALTY];
   k.bdr = c->f[DRD_F_BDR_PH];
   k.bu = c->f[DRD_F_BDR_PH_USED];
   k.bg = c->f[DRD_F_BDR_PH_GRAD];
@@ -711,7 +715,9 @@ static PhmuCoef32 phmu_c32(const drd_ctx* c) {
 }
 
 static int prec_setup(drd_ctx* c, int prec) {
-  if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag);
+  if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, c->s.re);
+  if (prec == DRD_PS_VELO2_JACOBI)
+    launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, 1.5 * c->s.re);
   if (prec == DRD_PS_PHMU_RBGS32) launch_phmu_coef32(c->stream, c->g, coef_of(c), phmu_c32(c));
   if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 || prec == DRD_PS_VELO_LEARNED) {
     const bool fp32 = (prec != DRD_PS_VELO_DCT64);
@@ -823,6 +829,7 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       *res = b;
       return 0;
     case DRD_PS_VELO1_JACOBI:
+    case DRD_PS_VELO2_JACOBI:
       launch_div_diag(c->stream, op_size(c, op), b, c->diag, x, ks);
       *res = x;
       return 0;
@@ -1007,7 +1014,8 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   // chunks of iterations are queued without host round trips; with graphs
   // on, a chunk of length >= 2 is one replay of a captured hipGraph
   const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
-                         (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_PHMU_JACOBI ||
+                         (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_VELO2_JACOBI ||
+                          prec == DRD_PS_PHMU_JACOBI ||
                           prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
                           prec == DRD_PS_VELO_MG || prec == DRD_PS_VELO_MG32 || prec == DRD_PS_PHMU_RBGS ||
                           prec == DRD_PS_PHMU_RBGS32 || prec == DRD_PS_VELO1_DCT32 ||
@@ -1629,15 +1637,16 @@ static int velo_prec(const drd_ctx* c, int bdf) {
       return bdf == 1 ? DRD_PS_VELO1_MG : DRD_PS_VELO_MG;
     case DRD_PREC_MG32:
       return bdf == 1 ? DRD_PS_VELO1_MG32 : DRD_PS_VELO_MG32;
-    default:
-      return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
+    default:  // the reference's: Jacobi for BDF1; identity (TP) or Jacobi (MF) for BDF2
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI
+                      : (c->p.variant == DRD_VARIANT_MF ? DRD_PS_VELO2_JACOBI : DRD_PS_IDENTITY);
   }
 }
 
 // uL(k) TP/VeloSolve_2_combined.f90:272-307, z(0,k) = (k+0.5) dz
 static int inflow(drd_ctx* c, double t) {
   const drd_params& p = c->p;
-  if (p.lr_bd_type != 'd') return 0;
+  if (!c->s.lr_d) return 0;  // 'd', or the MF variant (inflow always on)
   const double cf = 1.0 - std::exp(-t / p.twall);
   const double th = p.bdr_upp - p.bdr_low;
   for (int k = 0; k < p.ny; ++k) {
@@ -1676,6 +1685,7 @@ static int velo_solve_dev(drd_ctx* c, int bdf, const double* psi1, const double*
   a.pf1 = pf1 ? pf1 : c->zero_c;
   a.pf2 = pf2 ? pf2 : c->zero_c;
   a.uL = c->uL;
+  a.bdr = c->f[DRD_F_BDR_PH];
   if (bdf == 1) {  // unused by the BDF1 expressions
     a.u_pre = c->f[DRD_F_U];
     a.v_pre = c->f[DRD_F_V];
@@ -2164,7 +2174,7 @@ extern "C" int drd_bench_kernel(drd_ctx* c, int32_t which, int32_t reps, double*
   return 0;
 }
 
-// ---- time step (TP/mcl6.f90:257-303 and :395-502) ---------------------------
+// ---- time step (TP/mcl6.f90:257-303 and :395-502; MF/mcl6.f90:239-300, :388-440) --
 // One step of every instance of the context's group; per-instance reports
 // go to out[0 .. nb).
 static int step_dev(drd_ctx* c, int bdf, double t, drd_step_info* out) {
@@ -2173,7 +2183,10 @@ static int step_dev(drd_ctx* c, int bdf, double t, drd_step_info* out) {
   drd_step_info loc{};
   const double gamma1 = std::min(1.0, p.lambda_rho);
   double** F = c->f;
-  RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false, bdf == 2 && c->guess == 1));
+  if (p.variant == DRD_VARIANT_MF)  // MF/mcl6.f90:239, :388
+    RCHK(ph_solve_dev(c, c->rmu_after, c->ph_after, &loc.phmu, false));
+  else
+    RCHK(ph_mu_solve_dev(c, c->ph_after, c->rmu_after, &loc.phmu, false, bdf == 2 && c->guess == 1));
   for (int i = 0; i < c->nb; ++i) out[i].phmu = c->nb == 1 ? loc.phmu : c->last[i];
   if (bdf == 1) {
     RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->ph_after, c->rmu_after, t + p.dt,

@@ -102,8 +102,9 @@ struct Scal {
   double dt, re, ca, xld, s, I, dr, dz, dr2, dz2;
   double penalty_k, ggx, ggz, uwT, uwB, adj;
   double coef_bot, coef_top;
-  int lr_d;  // lr_bd_type == 'd'
+  int lr_d;  // lr_bd_type == 'd', or the MF variant (whose inflow is always on)
   int phi_mode;
+  int mf;    // the MF variant's momentum right-hand side (MF/VeloSolve_2_combined.f90)
   // quotients of the above that the CH stencils use (scal_derive)
   double idr2, idz2, s_I, I_dz2, floor22;
   double idrdz;  // 1 / (dr dz): the FP32 velocity stencil multiplies by it

@@ -23,6 +23,7 @@ struct Coef {
 struct VeloRhsArgs {
   const double *u, *v, *u_pre, *v_pre, *p, *psi1, *psi2;
   const double *rho, *eta, *bu, *ph_after, *rmu_after, *pf1, *pf2, *uL;
+  const double* bdr;  // bdr_ph (the MF variant's surface-tension term)
 };
 
 struct PhmuRhsArgs {
@@ -306,7 +307,9 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
 void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const Coef& c,
                         const double* x, double* y, double alpha, double beta, double* a0,
                         const double* b, double* part, KState* ks, bool finish = true);
-void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag);
+// mcoef: the mass coefficient, re (PSolve_uv_CG) or 1.5 re (the MF PSolve_uv_2_CG)
+void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag,
+                             double mcoef);
 void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,
                      const KState* ks);
 void launch_phmu_jacobi(hipStream_t st, const Grid& g, const Scal& s, const double* b, double* x,

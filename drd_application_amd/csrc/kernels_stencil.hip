@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
 // with the reference's rho(i-1,j) average on the v rows (:720).
 __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, const double* __restrict__ rho,
                                                              const double* __restrict__ eta,
-                                                             double* __restrict__ diag, int64_t bs) {
+                                                             double* __restrict__ diag, double mcoef, int64_t bs) {
   rho = bofs(rho, bs);
   eta = bofs(eta, bs);
   diag = bofs(diag, bs);
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
   if (q < g.mn()) {
     const int j = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)j * (nx + 1));
-    tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
+    tmp = mcoef * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
     double cr = eta[g.ci(i, j)], cl = eta[g.ci(i - 1, j)];
     tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dr2;
     cr = (eta[g.ci(i, j + 1)] + eta[g.ci(i - 1, j + 1)] + eta[g.ci(i, j)] + eta[g.ci(i - 1, j)]) / 4.0;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
     const int64_t t = q - g.mn();
     const int j = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(j - 1) * nx);
-    tmp = s.re * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
+    tmp = mcoef * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
     double cr = eta[g.ci(i, j)], cl = eta[g.ci(i, j - 1)];
     tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dz2;
     cr = (eta[g.ci(i + 1, j)] + eta[g.ci(i, j)] + eta[g.ci(i + 1, j - 1)] + eta[g.ci(i, j - 1)]) / 4.0;
@@ -373,11 +373,12 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
   a.rmu_after = bofs(a.rmu_after, bs);
   a.pf1 = bofs(a.pf1, bs);
   a.pf2 = bofs(a.pf2, bs);
+  a.bdr = bofs(a.bdr, bs);
   rhs = bofs(rhs, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= g.nuv()) return;
   const int nx = g.nx, ny = g.ny;
-  const double *rho = a.rho, *eta = a.eta, *bu = a.bu, *ph = a.ph_after, *mu = a.rmu_after;
+  const double *rho = a.rho, *eta = a.eta, *bu = a.bu, *ph = a.ph_after, *mu = a.rmu_after, *bd = a.bdr;
   const double *u = a.u, *v = a.v, *up = a.u_pre, *vp = a.v_pre;
   // extrapolated velocities of the convection term (:109-110)
   auto UT = [&](int i, int k) -> double {
@@ -395,13 +396,18 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
     const int k = g.by_nx1.div(q);
     const int i = (int)(q - (int64_t)k * (nx + 1));
     const int64_t c0 = g.ci(i, k), c1 = g.ci(i - 1, k);
-    f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dr;
+    if (s.mf)  // MF/VeloSolve_2_combined.f90:64-65
+      f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] - ph[c1]) / s.dr * (1.0 - (bd[c0] + bd[c1]) / 2.0);
+    else
+      f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dr;
     f = f - s.re * (rho[c0] + rho[c1]) / 2.0 *
                 (UT(i, k) * (UT(i + 1, k) - UT(i - 1, k)) / 2.0 / s.dr +
                  (VT(i, k) + VT(i - 1, k) + VT(i, k + 1) + VT(i - 1, k + 1)) / 4.0 * (UT(i, k + 1) - UT(i, k - 1)) / 2.0 / s.dz);
     const double rb = (rho[c0] + rho[c1]) / 2.0;
     double in;
-    if (BDF == 2)
+    if (BDF == 2 && s.mf)  // MF/VeloSolve_2_combined.f90:201
+      in = 2.0 * u[g.ui(i, k)] * s.re / s.dt - up[g.ui(i, k)] * s.re / 2.0 / s.dt + s.ggx;
+    else if (BDF == 2)
       in = 2.0 * u[g.ui(i, k)] * s.re / s.dt - up[g.ui(i, k)] * s.re / 2.0 / s.dt + (ph[c0] + ph[c1]) / 2.0 * s.ggx;
     else
       in = u[g.ui(i, k)] * s.re / s.dt + s.ggx;
@@ -425,13 +431,18 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
     const int k = g.by_nx.div(t) + 1;
     const int i = (int)(t - (int64_t)(k - 1) * nx);
     const int64_t c0 = g.ci(i, k), c1 = g.ci(i, k - 1);
-    f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dz;
+    if (s.mf)  // MF/VeloSolve_2_combined.f90:78-79
+      f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] - ph[c1]) / s.dz * (1.0 - (bd[c0] + bd[c1]) / 2.0);
+    else
+      f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] * bu[c0] - ph[c1] * bu[c1]) / s.dz;
     f = f - s.re * (rho[c0] + rho[c1]) / 2.0 *
                 ((UT(i, k) + UT(i + 1, k) + UT(i, k - 1) + UT(i + 1, k - 1)) / 4.0 * (VT(i + 1, k) - VT(i - 1, k)) / 2.0 / s.dr +
                  VT(i, k) * (VT(i, k + 1) - VT(i, k - 1)) / 2.0 / s.dz);
     const double rb = (rho[c0] + rho[c1]) / 2.0;
     double in;
-    if (BDF == 2)
+    if (BDF == 2 && s.mf)  // MF/VeloSolve_2_combined.f90:208
+      in = 2.0 * v[g.vi(i, k)] * s.re / s.dt - vp[g.vi(i, k)] * s.re / 2.0 / s.dt + s.ggz;
+    else if (BDF == 2)
       in = 2.0 * v[g.vi(i, k)] * s.re / s.dt - vp[g.vi(i, k)] * s.re / 2.0 / s.dt +
            (ph[c0] + ph[g.ci(i - 1, k)]) / 2.0 * s.ggz;
     else
@@ -1344,9 +1355,10 @@ void launch_phmu_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, 
   if (finish) launch_mv_finalize(st, mode, part, grid.x, 2, ks);
 }
 
-void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag) {
+void launch_velo_jacobi_diag(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, double* diag,
+                             double mcoef) {
   hipLaunchKernelGGL(k_velo_jacobi_diag, dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s,
-                     c.rho, c.eta, diag, bat_bs());
+                     c.rho, c.eta, diag, mcoef, bat_bs());
 }
 
 void launch_div_diag(hipStream_t st, int64_t n, const double* b, const double* d, double* x,

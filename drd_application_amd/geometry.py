@@ -376,6 +376,53 @@ def c4_config(nx: int = 4096, ny: int = 2048, seed: int = 4000, n_particles: int
     return cfg, b
 
 
+def read_domain_bin(path: str, nx: int, ny: int) -> np.ndarray:
+    """A domain file of the reference (read_var TP/inout.f90:75-87): nx * ny
+    FP64 values in a stream file, the x index outer, ((var(i,k), k=1,ny),
+    i=1,nx).  Returns the (ny, nx) array a[k, i] = var(i+1, k+1), the
+    interior of a cell field in the abi layout."""
+    raw = np.fromfile(path, dtype="<f8")
+    if raw.size != nx * ny:
+        raise ValueError(f"{path}: {raw.size} values, expected {nx} x {ny}")
+    return raw.reshape(nx, ny).T.copy()
+
+
+def mf_case(nx: int = 128, ny: int = 64, seed: int = 5, steps: int = 10, tol: float = 1e-10) -> Config:
+    """The microfluidics variant (MF/, phi in [0, 1], PhSolve for the CH step,
+    inflow through the left side): a serpentine channel (centreline 0.5 + 0.2
+    sin(2 pi x), half width 0.2, wall smoothing 2 cells) on an nx x ny grid of
+    height 1, a parabolic inflow of peak 1 between the channel walls at x = 0,
+    3 seeded droplets (phi = (1 + tanh(d / (sqrt2 eps))) / 2), penalty 1e5
+    (MF/global.f90:91), eta ratio 10, re 1, ca 1, xld 0.05, contact angle 90,
+    dt 5e-5; the reference's MF iteration limits (800 for the BDF2 momentum
+    solve, MF/VeloSolve_2_combined.f90:398)."""
+    p = abi.default_params(nx, ny)
+    p.variant = abi.VARIANT_MF
+    p.phi_mode = 1
+    p.dx = p.dz = 1.0 / ny
+    p.dt, p.re, p.ca, p.xld = 5e-5, 1.0, 1.0, 0.05
+    p.s_implicit, p.I_thickness = 1.5, 1.5 * p.dx
+    p.tol, p.penalty_k, p.angle = tol, 1e5, 90.0
+    p.maxit_velo1, p.maxit_velo2 = 800, 800
+    p.UU, p.twall = 1.0, 0.01
+    p.bdr_low, p.bdr_upp = 0.3, 0.7
+    bdr = serpentine_mask(p, 0.2, 2.0, 0.2, 2 * p.dx)
+    rng = SplitMix64(seed)
+    X, Z = _cell_centres(p)
+    phi = np.zeros_like(X)
+    L = nx * p.dx
+    for _ in range(3):
+        cx = rng.uniform(0.2 * L, 0.8 * L)
+        cz = 0.5 + 0.2 * math.sin(2.0 * math.pi * cx) + rng.uniform(-0.05, 0.05)
+        r = rng.uniform(4, 8) * p.dx
+        d = r - np.hypot(X - cx, Z - cz)
+        phi = np.maximum(phi, 0.5 * (1.0 + np.tanh(d / (math.sqrt(2.0) * p.I_thickness))))
+    f = derived_fields(p, phi, bdr, 10.0)
+    # eta of the [0, 1] phase: 1 + (lambda - 1) phi
+    f[abi.F_ETA] = _neumann_ghosts(1.0 + 9.0 * f[abi.F_PH])
+    return Config(f"mf{nx}x{ny}", p, f, steps, "MF variant, serpentine channel with inflow")
+
+
 def config(name: str) -> Config:
     """Named workloads.
 

@@ -475,6 +475,9 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  * drd_step's first (BDF1) step takes them in the DCT, learned and multigrid
  * modes, where the reference's Jacobi needs more than its 800 iterations on
  * large grids.  No reference counterpart.                                    */
+#define DRD_PS_VELO2_JACOBI 17 /* PSolve_uv_2_CG of MF MF/VeloSolve_2_combined.f90:662-711:
+                                  Jacobi with mass coefficient 1.5 re, no penalty
+                                  (the MF variant's BDF2 preconditioner)       */
 #define DRD_PS_VELO1_DCT32 13
 #define DRD_PS_VELO1_DCT64 14
 #define DRD_PS_VELO1_MG 15

@@ -474,15 +474,23 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
 /* ---- velocity operator ------------------------------------------------
  * MatrixVector_uv_CG  TP/VeloSolve_combined.f90:537-675   (bdf = 1)
  * MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:536-676 (bdf = 2)   */
+/* The MF variant (MF/VeloSolve_combined.f90, MF/VeloSolve_2_combined.f90)
+ * has no lr_bd_type switch: its inflow and u side ghosts are always the
+ * Dirichlet ones; its BDF2 penalty mask is the smooth bdr_ph (:580, :613)
+ * instead of bdr_ph_penalty. */
+static int is_mf(const drd_params* p) { return p->variant == DRD_VARIANT_MF; }
+static int lr_dirichlet(const drd_params* p) { return p->lr_bd_type == 'd' || is_mf(p); }
+
 static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, double beta,
                         double* yy) {
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
   double *f1 = c->w1, *f2 = c->w2, *f11 = c->w3, *f22 = c->w4;
-  const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA], *chi = c->f[DRD_F_BDR_PH_PENALTY];
+  const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA];
+  const double* chi = is_mf(p) ? c->f[DRD_F_BDR_PH] : c->f[DRD_F_BDR_PH_PENALTY];
   double dt = p->dt, re = p->re, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
   btoa_uv(c, f1, f2, xx);
-  if (p->lr_bd_type == 'd') {
+  if (lr_dirichlet(p)) {
     for (int k = 0; k < ny; ++k) {
       f1[UI(-1, k)] = -f1[UI(1, k)];
       f1[UI(nx + 1, k)] = -f1[UI(nx - 1, k)];
@@ -549,18 +557,20 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
   for (int64_t q = 0; q < n; ++q) yy[q] = beta == 0.0 ? alpha * zz[q] : alpha * zz[q] + beta * yy[q];
 }
 
-/* PSolve_uv_CG (Jacobi) TP/VeloSolve_combined.f90:679-741.  The v rows
- * average rho over (i-1,j) as the reference does (:720).               */
-static void velo_jacobi(oc_ctx* c, double* yy, const double* xx) {
+/* PSolve_uv_CG (Jacobi) TP/VeloSolve_combined.f90:679-741 (mass
+ * coefficient re), and the MF BDF2 form PSolve_uv_2_CG
+ * MF/VeloSolve_2_combined.f90:662-711 (1.5 re; no penalty term, as there).
+ * The v rows average rho over (i-1,j) as the reference does (:720). */
+static void velo_jacobi_m(oc_ctx* c, double mcoef, double* yy, const double* xx) {
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
   double *f1 = c->w1, *f2 = c->w2, *f11 = c->w3, *f22 = c->w4;
   const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA];
-  double dt = p->dt, re = p->re, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  double dt = p->dt, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_uv(c, f1, f2, xx);
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i <= nx; ++i) {
-      double tmp = re * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
+      double tmp = mcoef * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
       double cr = eta[CI(i, j)], cl = eta[CI(i - 1, j)];
       tmp = tmp + 2.0 * dt * (cl + cr) / dr2;
       cr = (eta[CI(i, j + 1)] + eta[CI(i - 1, j + 1)] + eta[CI(i, j)] + eta[CI(i - 1, j)]) / 4.0;
@@ -575,7 +585,7 @@ static void velo_jacobi(oc_ctx* c, double* yy, const double* xx) {
     }
   for (int j = 1; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
-      double tmp = re * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
+      double tmp = mcoef * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
       double cr = eta[CI(i, j)], cl = eta[CI(i, j - 1)];
       tmp = tmp + 2.0 * dt * (cl + cr) / dz2;
       cr = (eta[CI(i + 1, j)] + eta[CI(i, j)] + eta[CI(i + 1, j - 1)] + eta[CI(i, j - 1)]) / 4.0;
@@ -590,6 +600,7 @@ static void velo_jacobi(oc_ctx* c, double* yy, const double* xx) {
     }
   atob_uv(c, f11, f22, yy);
 }
+static void velo_jacobi(oc_ctx* c, double* yy, const double* xx) { velo_jacobi_m(c, c->p.re, yy, xx); }
 
 /* ---- Cahn-Hilliard operators ------------------------------------------ */
 /* MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:203-299 */
@@ -1075,6 +1086,7 @@ int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* 
   switch (prec) {
     case DRD_PS_VELO_ESOLP: velo_esolp(c, x_out, b_in); return 0;
     case DRD_PS_VELO1_JACOBI: velo_jacobi(c, x_out, b_in); return 0;
+    case DRD_PS_VELO2_JACOBI: velo_jacobi_m(c, 1.5 * c->p.re, x_out, b_in); return 0;
     case DRD_PS_IDENTITY: memcpy(x_out, b_in, (size_t)n * sizeof(double)); return 0;
     case DRD_PS_PHMU_JACOBI: phmu_jacobi(c, x_out, b_in); return 0;
     case DRD_PS_PHMU_BLOCK: phmu_block(c, x_out, b_in); return 0;
@@ -1345,14 +1357,26 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
   for (int64_t q = 0; q < nufield(p); ++q) fu[q] = 0.0;
   for (int64_t q = 0; q < nvfield(p); ++q) fv[q] = 0.0;
   /* ca * mu * grad(phi * bdr_ph_used) :62-97 */
-  for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i)
-      fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
-                     (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i - 1, k)] * bu[CI(i - 1, k)]) / dr;
-  for (int k = 1; k < ny; ++k)
-    for (int i = 0; i < nx; ++i)
-      fv[VI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i, k - 1)]) / 2.0 *
-                     (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i, k - 1)] * bu[CI(i, k - 1)]) / dz;
+  if (is_mf(p)) { /* MF/VeloSolve_2_combined.f90:64-65, :78-79: the jump of phi, times 1 - mean bdr_ph */
+    const double* bd = c->f[DRD_F_BDR_PH];
+    for (int k = 0; k < ny; ++k)
+      for (int i = 0; i <= nx; ++i)
+        fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
+                       (ph_after[CI(i, k)] - ph_after[CI(i - 1, k)]) / dr * (1.0 - (bd[CI(i, k)] + bd[CI(i - 1, k)]) / 2.0);
+    for (int k = 1; k < ny; ++k)
+      for (int i = 0; i < nx; ++i)
+        fv[VI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i, k - 1)]) / 2.0 *
+                       (ph_after[CI(i, k)] - ph_after[CI(i, k - 1)]) / dz * (1.0 - (bd[CI(i, k)] + bd[CI(i, k - 1)]) / 2.0);
+  } else {
+    for (int k = 0; k < ny; ++k)
+      for (int i = 0; i <= nx; ++i)
+        fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
+                       (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i - 1, k)] * bu[CI(i - 1, k)]) / dr;
+    for (int k = 1; k < ny; ++k)
+      for (int i = 0; i < nx; ++i)
+        fv[VI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i, k - 1)]) / 2.0 *
+                       (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i, k - 1)] * bu[CI(i, k - 1)]) / dz;
+  }
   /* convection re*rho*(u.grad)u with u (bdf1) or 2u - u_pre (bdf2) :108-124 */
   if (bdf == 2) {
     for (int64_t q = 0; q < nufield(p); ++q) ut[q] = 2.0 * u[q] - u_pre[q];
@@ -1383,7 +1407,9 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     for (int i = 0; i <= nx; ++i) {
       double rb = (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0;
       double in;
-      if (bdf == 2)
+      if (bdf == 2 && is_mf(p)) /* MF/VeloSolve_2_combined.f90:201: plain ggx */
+        in = 2.0 * u[UI(i, k)] * re / dt - u_pre[UI(i, k)] * re / 2.0 / dt + p->ggx;
+      else if (bdf == 2)
         in = 2.0 * u[UI(i, k)] * re / dt - u_pre[UI(i, k)] * re / 2.0 / dt +
              (ph_after[CI(i, k)] + ph_after[CI(i - 1, k)]) / 2.0 * p->ggx;
       else
@@ -1394,7 +1420,9 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     for (int i = 0; i < nx; ++i) {
       double rb = (rho[CI(i, k)] + rho[CI(i, k - 1)]) / 2.0;
       double in;
-      if (bdf == 2)
+      if (bdf == 2 && is_mf(p)) /* MF/VeloSolve_2_combined.f90:208 */
+        in = 2.0 * v[VI(i, k)] * re / dt - v_pre[VI(i, k)] * re / 2.0 / dt + p->ggz;
+      else if (bdf == 2)
         in = 2.0 * v[VI(i, k)] * re / dt - v_pre[VI(i, k)] * re / 2.0 / dt +
              (ph_after[CI(i, k)] + ph_after[CI(i - 1, k)]) / 2.0 * p->ggz;
       else
@@ -1411,8 +1439,8 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     for (int i = 0; i <= nx; ++i) fu[UI(i, k)] = fu[UI(i, k)] * dt;
   for (int k = 1; k < ny; ++k)
     for (int i = 0; i < nx; ++i) fv[VI(i, k)] = fv[VI(i, k)] * dt;
-  /* Dirichlet inflow :272-310 */
-  if (p->lr_bd_type == 'd') {
+  /* Dirichlet inflow :272-310 (always in MF, MF/VeloSolve_2_combined.f90:261-295) */
+  if (lr_dirichlet(p)) {
     double cf = 1.0 - exp(-t / p->twall);
     double tmp_hig = p->bdr_upp - p->bdr_low;
     for (int k = 0; k < ny; ++k) {
@@ -1448,7 +1476,7 @@ static void velo_unpack(oc_ctx* c, const double* x, double* u_after, double* v_a
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
   btoa_uv(c, u_after, v_after, x);
-  if (p->lr_bd_type == 'd') {
+  if (lr_dirichlet(p)) {
     for (int k = 0; k < ny; ++k) u_after[UI(-1, k)] = 2.0 * c->uL[k] - u_after[UI(1, k)];
     for (int k = 0; k < ny; ++k) u_after[UI(nx + 1, k)] = 2.0 * c->uR[k] - u_after[UI(nx - 1, k)];
   } else {
@@ -1475,7 +1503,8 @@ static int velo_prec_id(const oc_ctx* c, int bdf) {
   switch (c->p.velo_prec) {
     case DRD_PREC_IDENTITY: return DRD_PS_IDENTITY;
     case DRD_PREC_JACOBI: return DRD_PS_VELO1_JACOBI;
-    default: return bdf == 1 ? DRD_PS_VELO1_JACOBI : DRD_PS_IDENTITY;
+    default: /* the reference's: Jacobi for BDF1; identity (TP) or Jacobi (MF) for BDF2 */
+      return bdf == 1 ? DRD_PS_VELO1_JACOBI : (is_mf(&c->p) ? DRD_PS_VELO2_JACOBI : DRD_PS_IDENTITY);
   }
 }
 
@@ -1876,7 +1905,8 @@ int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t, drd_sol
   return 0;
 }
 
-/* ---- one time step  TP/mcl6.f90:257-303 (bdf 1) and :395-502 (bdf 2) -- */
+/* ---- one time step  TP/mcl6.f90:257-303 (bdf 1) and :395-502 (bdf 2); the MF
+ * variant takes PhSolve for the CH solve (MF/mcl6.f90:239-300, :388-440) -- */
 int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
   const drd_params* p = &c->p;
   int64_t nc = ncell(p), nu = nufield(p), nv = nvfield(p);
@@ -1891,7 +1921,10 @@ int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
   double** F = c->f;
   int rc;
   if (bdf == 1) {
-    oc_ph_mu_solve(c, ph_after, rmu_after, t, &loc.phmu);
+    if (is_mf(p)) /* MF/mcl6.f90:239 */
+      oc_phsolve(c, rmu_after, ph_after, t + p->dt, &loc.phmu);
+    else
+      oc_ph_mu_solve(c, ph_after, rmu_after, t, &loc.phmu);
     rc = oc_velosolve_combined(c, F[DRD_F_PSI1], u_after, v_after, rmu_after, ph_after, t + p->dt,
                                c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2], &loc.velo);
     if (rc) goto out;
@@ -1906,7 +1939,10 @@ int oc_step(oc_ctx* c, int32_t bdf, double t, drd_step_info* info) {
     memcpy(F[DRD_F_RMU], rmu_after, (size_t)nc * sizeof(double));
     memcpy(F[DRD_F_PH], ph_after, (size_t)nc * sizeof(double));
   } else {
-    ph_mu_solve_x(c, ph_after, rmu_after, &loc.phmu, c->guess == 1);
+    if (is_mf(p)) /* MF/mcl6.f90:388 */
+      oc_phsolve(c, rmu_after, ph_after, t + p->dt, &loc.phmu);
+    else
+      ph_mu_solve_x(c, ph_after, rmu_after, &loc.phmu, c->guess == 1);
     rc = oc_velosolve_2_combined(c, F[DRD_F_PSI1], F[DRD_F_PSI2], F[DRD_F_U_PRE], u_after,
                                  F[DRD_F_V_PRE], v_after, F[DRD_F_PH_PRE], ph_after,
                                  F[DRD_F_RMU_PRE], rmu_after, t + p->dt, c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2], &loc.velo);

@@ -92,3 +92,24 @@ def test_splitmix_vectorised_matches_scalar():
     r = SplitMix64(12345)
     a = [r.uniform() for _ in range(100)]
     np.testing.assert_array_equal(mg.splitmix_uniform(12345, 100), a)
+
+
+def test_domain_bin_reader():
+    """read_domain_bin against the reference's own domain files (read_var,
+    TP/inout.f90:75-87), where the checkout is present: 420 x 180 values,
+    x index outer; the snake channel is a sharp phase, +1 in the solid and -1
+    in the fluid (finit smooths it by CH steps, MF/finit.f90:114-188), and
+    both walls (bottom and top rows) are solid."""
+    import os
+    from drd_application_amd import geometry
+    path = "/root/reference/microfluidics_complex_domain/domain_snake_420_180.bin"
+    if not os.path.exists(path):
+        pytest.skip("reference checkout not present")
+    a = geometry.read_domain_bin(path, 420, 180)
+    raw = np.fromfile(path, dtype="<f8")
+    assert a.shape == (180, 420)
+    assert a[7, 11] == raw[11 * 180 + 7]  # var(12, 8): k inner
+    assert set(np.unique(a)) == {-1.0, 1.0}
+    assert (a[0] == 1.0).all() and (a[-1] == 1.0).all()
+    with pytest.raises(ValueError):
+        geometry.read_domain_bin(path, 400, 180)

@@ -421,3 +421,47 @@ def test_oracle_threads_do_not_change_results():
     np.testing.assert_array_equal(a[0], b8[0])
     np.testing.assert_array_equal(a[2], b8[2])
     assert a[1] == b8[1] and a[3] == b8[3] and a[4] == b8[4]
+
+
+def test_velocity_rhs_mf_terms():
+    """The MF variant's two different RHS terms (MF/VeloSolve_2_combined.f90):
+    surface tension ca mu (phi(i) - phi(i-1))/dr (1 - mean bdr_ph) (:64-65)
+    and the BDF2 body force plain ggx (:201), and its always-on inflow."""
+    p, _ = _rhs_oracle()
+    p.variant = abi.VARIANT_MF
+    p.ggx = 2.0
+    o = oc.Oracle(p)
+    cs = abi.cell_shape(p)
+    for fid in (abi.F_RHO, abi.F_ETA):
+        o.set_field(fid, np.ones(cs))
+    X, Z = geometry._cell_centres(p)
+    zero = np.zeros(cs)
+    # plain ggx in BDF2 whatever phi is (TP would give phi_avg ggx)
+    fu, _ = _split(p, o.velo_rhs(2, ph_after=np.full(cs, 0.5), rmu_after=zero))
+    np.testing.assert_allclose(fu, 2.0 * p.dt, rtol=1e-12)
+    o.p.ggx = 0.0
+    o2 = oc.Oracle(o.p)
+    for fid in (abi.F_RHO, abi.F_ETA):
+        o2.set_field(fid, np.ones(cs))
+    # surface tension with bdr_ph = 0.25 everywhere: (1 - 0.25) ca mu b dt; bdr_ph_used plays no part
+    o2.set_field(abi.F_BDR_PH, np.full(cs, 0.25))
+    o2.set_field(abi.F_BDR_PH_USED, np.full(cs, 0.1))
+    fu, fv = _split(p, o2.velo_rhs(1, ph_after=0.3 * X, rmu_after=np.full(cs, 2.0)))
+    np.testing.assert_allclose(fu, 0.75 * p.ca * 2.0 * 0.3 * p.dt, rtol=1e-12)
+    np.testing.assert_allclose(fv, 0.0, atol=1e-15)
+    fu, fv = _split(p, o2.velo_rhs(1, ph_after=-0.2 * Z, rmu_after=np.full(cs, 1.5)))
+    np.testing.assert_allclose(fu, 0.0, atol=1e-15)
+    np.testing.assert_allclose(fv, 0.75 * p.ca * 1.5 * -0.2 * p.dt, rtol=1e-12)
+    # inflow on although lr_bd_type is 'n': u(0,k) rows get 2 uL eta 2 dt / dr^2
+    q = o.p.copy()
+    q.UU, q.bdr_low, q.bdr_upp, q.twall = 1.0, 0.0, q.ny * q.dz, 1e-9
+    o3 = oc.Oracle(q)
+    for fid in (abi.F_RHO, abi.F_ETA):
+        o3.set_field(fid, np.ones(cs))
+    fu, _ = _split(q, o3.velo_rhs(1, ph_after=zero, rmu_after=zero, t=1.0))
+    zc = (np.arange(q.ny) + 0.5) * q.dz
+    h = q.bdr_upp - q.bdr_low
+    uL = q.UU * (-((zc - q.bdr_low - h / 2) ** 2) / h / h * 4 + 1)
+    np.testing.assert_allclose(fu[:, 0], 2.0 * uL * 2.0 * q.dt / q.dx ** 2, rtol=1e-12)
+    np.testing.assert_allclose(fu[:, -1], 2.0 * uL * 2.0 * q.dt / q.dx ** 2, rtol=1e-12)
+    np.testing.assert_allclose(fu[:, 1:-1], 0.0, atol=1e-15)

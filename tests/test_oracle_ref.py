@@ -240,3 +240,27 @@ def test_mf_phsolve_operator_and_dct_preconditioner():
     a = o.psolve(abi.PS_PH_DCT, abi.OP_PH, x)
     b = mf.psolve("psolve_cg_", x)
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+@needs_mf
+def test_mf_velocity_matvec_and_jacobi():
+    """The MF momentum operators (MF/VeloSolve_combined.f90:537-675,
+    MF/VeloSolve_2_combined.f90:506-650: Dirichlet side ghosts always, the
+    smooth bdr_ph as the BDF2 penalty mask) and the MF BDF2 Jacobi
+    PSolve_uv_2_CG (:655-711) against the MF reference build, bit for bit."""
+    mf = refpy.Ref("mf")
+    p = abi.default_params(mf.nr, mf.nz)
+    p.penalty_k = mf.penalty_k
+    p.variant = abi.VARIANT_MF
+    p.dt, p.re = 3.7e-5, 2.5
+    mf.set_params(p)
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(111)
+    _random_state(mf, o, p, rng, [("rho", abi.F_RHO, 0.5, 2.0), ("eta", abi.F_ETA, 1.0, 100.0),
+                                  ("bdr_ph", abi.F_BDR_PH, 0.0, 1.0)])
+    o.set_field(abi.F_BDR_PH_PENALTY, rng.uniform(0.0, 1.0, mf.cell_shape))  # not the MF mask
+    x, y = rng.standard_normal(mf.nuv), rng.standard_normal(mf.nuv)
+    for op, name in ((abi.OP_VELO1, "matrixvector_uv_cg_"), (abi.OP_VELO2, "matrixvector_uv_2_cg_")):
+        np.testing.assert_array_equal(o.matvec(op, 0.7, x, -0.3, y), mf.matvec(name, 0.7, x, -0.3, y))
+    np.testing.assert_array_equal(o.psolve(abi.PS_VELO1_JACOBI, abi.OP_VELO1, x), mf.psolve("psolve_uv_cg_", x))
+    np.testing.assert_array_equal(o.psolve(abi.PS_VELO2_JACOBI, abi.OP_VELO2, x), mf.psolve("psolve_uv_2_cg_", x))
