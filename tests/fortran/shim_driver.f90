@@ -3,22 +3,29 @@
 ! Linked against the caller-side module `global` (TP/global.f90, grid fixed
 ! at its compile-time nr x nz) exactly as a reference driver would be.  It
 ! reads module scalars and fields from a stream file written by the test,
-! runs the reference's call sequence of one first time step
-! (TP/mcl6.f90:257-303: ph_mu_solve, VeloSolve_combined, PreSolve_2) through
-! the shim, and writes the results to a second stream file.
-!   shim_driver <in.bin> <out.bin>
+! runs the reference's call sequence of nsteps time steps through the shim
+! (TP/mcl6.f90:257-303 for the first, BDF1 step: ph_mu_solve,
+! VeloSolve_combined, PreSolve_2; :395-502 for the BDF2 steps: ph_mu_solve,
+! VeloSolve_2_combined, PreSolve_2, with the driver's own rotation of u_pre,
+! ph_pre, rmu_pre, psi1 and p = psi2 + p), and writes the results to a
+! second stream file.
+!   shim_driver <in.bin> <out.bin> <nsteps>
 program shim_driver
   use global
   implicit none
-  real ph_after(-1:nr,-1:nz), rmu_after(-1:nr,-1:nz), psi(-1:nr,-1:nz), pf(-1:nr,-1:nz)
-  real u_after(-1:nr+1,-1:nz), v_after(-1:nr,0:nz)
+  real ph_after(-1:nr,-1:nz), rmu_after(-1:nr,-1:nz), psi1(-1:nr,-1:nz), psi2(-1:nr,-1:nz)
+  real pf(-1:nr,-1:nz), ph_pre(-1:nr,-1:nz), rmu_pre(-1:nr,-1:nz)
+  real u_after(-1:nr+1,-1:nz), v_after(-1:nr,0:nz), u_pre(-1:nr+1,-1:nz), v_pre(-1:nr,0:nz)
   real(8) scal(24)
+  real t, gamma1
   integer*8 plan, plan_back
-  integer :: uin, uout, i, j
-  character(len=1024) :: fin, fout
+  integer :: uin, uout, i, j, s, nsteps
+  character(len=1024) :: fin, fout, arg
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  call get_command_argument(3, arg)
+  read(arg, *) nsteps
   call PreSolve_init(nr, nz, plan, plan_back)
   open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old')
   read(uin) scal
@@ -42,13 +49,45 @@ program shim_driver
      end do
   end do
   pf = 0.0
-  psi = 0.0
-  call ph_mu_solve(ph_after, rmu_after, 0.0)
-  call VeloSolve_combined(psi, u_after, v_after, rmu_after, ph_after, dt, pf, pf)
-  u = u_after
-  v = v_after
-  call PreSolve_2(plan, plan_back, psi, re/dt)
+  psi1 = 0.0
+  psi2 = 0.0
+  u_pre = u
+  v_pre = v
+  gamma1 = min(1.0, lambda_rho)
+  t = 0.0
+  do s = 1, nsteps
+     if (s == 1) then
+        call ph_mu_solve(ph_after, rmu_after, t)
+        call VeloSolve_combined(psi1, u_after, v_after, rmu_after, ph_after, t+dt, pf, pf)
+        u_pre = u
+        v_pre = v
+        u = u_after
+        v = v_after
+        call PreSolve_2(plan, plan_back, psi2, re*gamma1/dt)
+        p = psi2 + p
+        ph_pre = ph
+        rmu_pre = rmu
+        rmu = rmu_after
+        ph = ph_after
+     else
+        call ph_mu_solve(ph_after, rmu_after, t+dt)
+        call VeloSolve_2_combined(psi1, psi2, u_pre, u_after, v_pre, v_after, ph_pre, ph_after, &
+             rmu_pre, rmu_after, t+dt, pf, pf)
+        u_pre = u
+        v_pre = v
+        u = u_after
+        v = v_after
+        ph_pre = ph
+        rmu_pre = rmu
+        ph = ph_after
+        rmu = rmu_after
+        psi1 = psi2
+        call PreSolve_2(plan, plan_back, psi2, 3*re*gamma1/2./dt)
+        p = psi2 + p
+     end if
+     t = t + dt
+  end do
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(uout) ph_after, rmu_after, u_after, v_after, psi, ph_M_0
+  write(uout) u, v, p, ph, rmu, psi2, ph_M_0, uL(0:nz-1), uR(0:nz-1)
   close(uout)
 end program shim_driver

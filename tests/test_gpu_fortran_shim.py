@@ -1,11 +1,22 @@
-"""The Fortran drop-in: a Fortran driver calls the reference's subroutine names
-(PreSolve_init, ph_mu_solve, VeloSolve_combined, PreSolve_2) through
-drd_application_amd/fortran/drd_gpu_shim.f90 with module `global` set as the
-reference's driver would set it; the results must match the CPU oracle run
-through the same sequence (TP/mcl6.f90:257-303), bit for bit.
+"""The Fortran drop-ins: Fortran drivers call the reference's subroutine names
+through drd_application_amd/fortran/drd_gpu_shim.f90 (the TP program) and
+drd_gpu_shim_mf.f90 (the MF program), with module `global` set as the
+reference's drivers set it, for ten time steps of the reference's own call
+sequence (TP/mcl6.f90:257-303 and :395-502: ph_mu_solve, VeloSolve_combined
+then VeloSolve_2_combined, PreSolve_2; MF/mcl6.f90:239-300 and :388-440 with
+PhSolve), the drivers rotating u_pre, ph_pre, rmu_pre, psi1 and adding
+p = psi2 + p themselves.
 
-The driver binary (tests/fortran/build/shim_driver) is built where the
-reference checkout provides module `global` (tests/fortran/Makefile).
+* TP, Neumann sides, the reference's preconditioners: u, v, p, phi, mu,
+  psi2 and the hidden output ph_M_0 bit for bit the CPU oracle's ten steps.
+* TP with the Dirichlet inflow: the same fields bit for bit the C-ABI's own
+  device-resident steps (drd_step) on the same inputs, and the hidden
+  outputs uL, uR the inflow profile at the last solve's time.
+* MF (module `global` of MF, 420 x 180): likewise against drd_step of the MF
+  variant, uL, uR included.
+
+The drivers (tests/fortran/build/) are built where the reference checkout
+provides the modules `global` (tests/fortran/Makefile).
 """
 import os
 import subprocess
@@ -17,60 +28,124 @@ from drd_application_amd import abi, geometry
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "fortran", "build", "shim_driver")
+DRIVER_MF = os.path.join(HERE, "fortran", "build", "shim_driver_mf")
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran shim driver not built")]
+pytestmark = pytest.mark.gpu
 
-
-def rel_l2(a, b):
-    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+STEPS = 10
 
 
-def test_fortran_dropin_first_step(tmp_path):
-    from oracle.oracle import Oracle
-
-    cfg = geometry.tp_case(128, 128, lambda_eta=10.0, tol=1e-10)
-    p = cfg.params
-    f = cfg.fields
+def _scal(p):
     scal = np.zeros(24)
     scal[:22] = [p.dx, p.dz, p.dt, p.re, p.ca, p.xld, p.s_implicit, p.I_thickness, p.ggx, p.ggz,
                  p.uwT, p.uwB, p.tol, p.angle, p.lambda_rho, p.bdr_ph_used_adjust, p.twall, p.UU,
                  p.bdr_upp, p.bdr_low, p.phi_mode, 1.0 if p.lr_bd_type == ord("d") else 0.0]
-    order = [abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_RHO, abi.F_ETA, abi.F_BDR_PH,
-             abi.F_BDR_PH_USED, abi.F_BDR_PH_PENALTY, abi.F_BDR_PH_GRAD, abi.F_PH_M_INNER,
-             abi.F_ANGLEB, abi.F_ANGLET]
+    return scal
+
+
+def _run(driver, tmp_path, scal, fields, out_shapes):
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as fh:
         fh.write(scal.tobytes())
-        for fid in order:
-            fh.write(np.ascontiguousarray(f[fid], dtype=np.float64).tobytes())
-    print("running", DRIVER, flush=True)
-    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
-    print(r.stdout[-2000:], r.stderr[-2000:])
+        for a in fields:
+            fh.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    print("running", driver, STEPS, "steps", flush=True)
+    r = subprocess.run([driver, str(fin), str(fout), str(STEPS)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
     assert r.returncode == 0
     out = np.fromfile(fout, dtype=np.float64)
-    cs, us, vs = abi.cell_shape(p), abi.field_shape(p, abi.F_U), abi.field_shape(p, abi.F_V)
-    sizes = [cs, cs, us, vs, cs, cs]
     parts, off = [], 0
-    for s in sizes:
+    for s in out_shapes:
         n = int(np.prod(s))
         parts.append(out[off:off + n].reshape(s))
         off += n
     assert off == out.size
-    ph_a, mu_a, u_a, v_a, psi, m0 = parts
+    return parts, r.stdout
 
-    o = Oracle(p)
-    geometry.load(o, cfg)
-    oph, omu, _ = o.ph_mu_solve(0.0)
-    ou, ov, _ = o.VeloSolve_combined(np.zeros(cs), omu, oph, p.dt)
-    o.set_field(abi.F_U, ou)
-    o.set_field(abi.F_V, ov)
-    opsi, _ = o.PreSolve_2(p.re / p.dt)
-    for name, a, b in (("ph_after", ph_a, oph), ("rmu_after", mu_a, omu), ("u_after", u_a, ou),
-                       ("v_after", v_a, ov), ("psi", psi, opsi)):
-        e = rel_l2(a, b)
-        print(name, e)
-        assert e < 1e-9, (name, e)
-        np.testing.assert_array_equal(a, b, err_msg=name)
-    np.testing.assert_array_equal(m0, o.get_field(abi.F_PH_M_0))
-    assert "uv 1 BICGSTAB no slip" in r.stdout and "ph_mu_solve" in r.stdout
+
+def _inflow(p, t):
+    """uL(k) of TP/VeloSolve_combined.f90:266-306 at time t, z(0,k) = (k+0.5) dz."""
+    z = (np.arange(p.ny) + 0.5) * p.dz
+    h = p.bdr_upp - p.bdr_low
+    cf = 1.0 - np.exp(-t / p.twall)
+    return np.where((z > p.bdr_low) & (z < p.bdr_upp), p.UU * (-((z - p.bdr_low - h / 2) ** 2) / h / h * 4 + 1) * cf,
+                    0.0)
+
+
+TP_ORDER = [abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_RHO, abi.F_ETA, abi.F_BDR_PH,
+            abi.F_BDR_PH_USED, abi.F_BDR_PH_PENALTY, abi.F_BDR_PH_GRAD, abi.F_PH_M_INNER,
+            abi.F_ANGLEB, abi.F_ANGLET]
+OUT = (abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_PSI2)
+
+
+def _steps(solver, p):
+    t = 0.0
+    for s in range(STEPS):
+        info = solver.step(1 if s == 0 else 2, t)
+        t += p.dt
+    return info
+
+
+def _tp_case(lr):
+    cfg = geometry.tp_case(128, 128, lambda_eta=10.0, tol=1e-10)
+    p = cfg.params
+    if lr == "d":
+        p.lr_bd_type = ord("d")
+        p.UU, p.bdr_low, p.bdr_upp, p.twall = 1.0, 0.3, 0.9, 2e-4  # the ramp 1 - exp(-t/twall) nearly done
+    return cfg
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran shim driver not built")
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("lr", ["n", "d"])
+def test_tp_dropin_ten_steps(tmp_path, lr):
+    from drd_application_amd.solver import GpuSolver
+    from oracle.oracle import Oracle
+    cfg = _tp_case(lr)
+    p = cfg.params
+    cs, us, vs = abi.cell_shape(p), abi.field_shape(p, abi.F_U), abi.field_shape(p, abi.F_V)
+    parts, stdout = _run(DRIVER, tmp_path, _scal(p), [cfg.fields[f] for f in TP_ORDER],
+                         [us, vs, cs, cs, cs, cs, cs, (p.ny,), (p.ny,)])
+    got = dict(zip(OUT, parts[:6]))
+    m0, uL, uR = parts[6], parts[7], parts[8]
+    assert stdout.count("uv 2 BICGSTAB no slip") == STEPS - 1 and "uv 1 BICGSTAB no slip" in stdout
+    if lr == "n":  # against the oracle
+        ref = Oracle(p)
+    else:  # against the library's own device-resident steps
+        ref = GpuSolver(p)
+    geometry.load(ref, cfg)
+    _steps(ref, p)
+    for fid in OUT:
+        np.testing.assert_array_equal(got[fid], ref.get_field(fid), err_msg=abi.FIELD_NAMES[fid])
+    np.testing.assert_array_equal(m0, ref.get_field(abi.F_PH_M_0))
+    if lr == "d":
+        np.testing.assert_allclose(uL, _inflow(p, STEPS * p.dt), rtol=1e-14, atol=1e-15)
+        np.testing.assert_array_equal(uR, uL)
+        assert np.abs(uL).max() > 0.5
+    ref.close()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER_MF), reason="MF Fortran shim driver not built")
+@pytest.mark.timeout(600)
+def test_mf_dropin_ten_steps(tmp_path):
+    from drd_application_amd.solver import GpuSolver
+    cfg = geometry.mf_case(420, 180)  # the MF module's compile-time grid (MF/global.f90:18-19)
+    p = cfg.params
+    f = cfg.fields
+    cs, us, vs = abi.cell_shape(p), abi.field_shape(p, abi.F_U), abi.field_shape(p, abi.F_V)
+    order = [abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_RHO, abi.F_ETA, abi.F_BDR_PH,
+             abi.F_BDR_PH_GRAD, abi.F_PH_M_INNER, abi.F_PH_M_0]
+    parts, stdout = _run(DRIVER_MF, tmp_path, _scal(p), [f[fid] for fid in order],
+                         [us, vs, cs, cs, cs, cs, (p.ny,), (p.ny,)])
+    got = dict(zip(OUT, parts[:6]))
+    uL, uR = parts[6], parts[7]
+    assert stdout.count("\\mu BICGSTAB") == STEPS
+    g = GpuSolver(p)
+    geometry.load(g, cfg)
+    info = _steps(g, p)
+    print("last device step", info, flush=True)
+    for fid in OUT:
+        np.testing.assert_array_equal(got[fid], g.get_field(fid), err_msg=abi.FIELD_NAMES[fid])
+    np.testing.assert_allclose(uL, _inflow(p, STEPS * p.dt), rtol=1e-14, atol=1e-15)
+    np.testing.assert_array_equal(uR, uL)
+    g.close()
