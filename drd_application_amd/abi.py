@@ -147,6 +147,9 @@ class StepInfo(C.Structure):
                 ("div_before", C.c_double), ("div_after_max", C.c_double)]
 
 
+LR_NEUMANN, LR_DIRICHLET, LR_PERIODIC = ord("n"), ord("d"), ord("p")
+
+
 def default_params(nx: int, ny: int) -> Params:
     """TP defaults (TP/input, TP/mcl6.f90, TP/finit.f90), grid nx x ny on
     the unit-height domain used by BASELINE-style configs."""
@@ -188,8 +191,18 @@ def cell_shape(p: Params) -> tuple:
     return (p.ny + 2, p.nx + 2)
 
 
+def periodic_x(p: Params) -> bool:
+    """lr_bd_type 'p': periodic in x (u has nx unknowns per row, u(nx) = u(0))."""
+    return p.lr_bd_type == LR_PERIODIC
+
+
+def nx_u(p: Params) -> int:
+    """u unknowns per row of the packed velocity vector."""
+    return p.nx + (0 if periodic_x(p) else 1)
+
+
 def n_uv(p: Params) -> int:
-    return (p.nx + 1) * p.ny + p.nx * (p.ny - 1)
+    return nx_u(p) * p.ny + p.nx * (p.ny - 1)
 
 
 def op_size(p: Params, op: int) -> int:

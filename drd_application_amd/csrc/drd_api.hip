@@ -287,6 +287,16 @@ static int build_tables(drd_ctx* c) {
     const double sn = std::sin(kPiG * j / 2.0 / ny);
     fmy[j] = 4.0 * (sn * sn) / dz2;
   }
+  if (p.lr_bd_type == 'p') {
+    // periodic x: 4 sin^2(pi kx / nx) / dr2 of the Fourier mode kx at the
+    // halfcomplex indices m = kx (cosine part) and nx - kx (sine part)
+    for (int m = 0; m < nx; ++m) {
+      const int kx = m <= nx / 2 ? m : nx - m;
+      const double sn = std::sin(kPiG * kx / nx);
+      fmx[m] = 4.0 * (sn * sn) / dr2;
+    }
+    fmx[nx] = 0.0;
+  }
   for (int j = 0; j <= ny; ++j)
     for (int i = 0; i <= nx; ++i) pre[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
   pre[0] = -1.0;
@@ -515,15 +525,36 @@ static int eager_vprec(drd_ctx* c) {
   return 0;
 }
 
+// Periodic x (lr_bd_type 'p', no TP counterpart): the TP variant, nx a power
+// of two (the pressure's real FFT in x), one instance, and the
+// preconditioners built for it: the reference's (Jacobi / identity velocity,
+// Jacobi phi/mu), Jacobi, identity and the phi/mu block solve.
+static bool periodic_prec_ok(int prec) {
+  return prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_VELO2_JACOBI ||
+         prec == DRD_PS_PHMU_JACOBI || prec == DRD_PS_PHMU_BLOCK;
+}
+static int check_periodic(const drd_params& p, int nb) {
+  if (p.lr_bd_type != 'p') return 0;
+  if (p.variant != DRD_VARIANT_TP) return set_err(DRD_E_UNSUPPORTED, "periodic x takes the TP variant");
+  if ((p.nx & (p.nx - 1)) != 0) return set_err(DRD_E_UNSUPPORTED, "periodic x needs a power-of-two nx");
+  if (nb != 1) return set_err(DRD_E_UNSUPPORTED, "periodic x takes a single-instance context");
+  if (p.velo_prec != DRD_PREC_REFERENCE && p.velo_prec != DRD_PREC_IDENTITY && p.velo_prec != DRD_PREC_JACOBI)
+    return set_err(DRD_E_UNSUPPORTED, "periodic x takes the reference, identity or Jacobi velocity preconditioner");
+  if (p.ch_prec != DRD_CH_PREC_REFERENCE && p.ch_prec != DRD_CH_PREC_BLOCK)
+    return set_err(DRD_E_UNSUPPORTED, "periodic x takes the reference or block phi/mu preconditioner");
+  return 0;
+}
+
 static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
   if (!prm || !out) return set_err(DRD_E_ARG, "null argument");
   if (prm->nx < 4 || prm->ny < 4) return set_err(DRD_E_ARG, "nx, ny must be >= 4");
   if (prm->nx > 4096 || prm->ny > 4096) return set_err(DRD_E_ARG, "nx, ny must be <= 4096");
-  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
-    return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
+  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd' && prm->lr_bd_type != 'p')
+    return set_err(DRD_E_ARG, "lr_bd_type must be 'n', 'd' or 'p'");
   if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
   if (prm->variant != DRD_VARIANT_TP && prm->variant != DRD_VARIANT_MF)
     return set_err(DRD_E_ARG, "variant must be DRD_VARIANT_TP or DRD_VARIANT_MF");
+  RCHK(check_periodic(*prm, nb));
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(DRD_E_HIP, "no HIP device visible");
@@ -532,7 +563,7 @@ static int ctx_create(const drd_params* prm, int nb, drd_ctx** out) {
   drd_ctx* c = new drd_ctx();
   c->nb = nb;
   c->p = *prm;
-  c->g = Grid::make(prm->nx, prm->ny);
+  c->g = Grid::make(prm->nx, prm->ny, prm->lr_bd_type == 'p');
   c->s = make_scal(*prm);
   int rc = 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
@@ -576,11 +607,14 @@ extern "C" int drd_graph_count(drd_ctx* c, int32_t* live, int64_t* recaptured) {
 extern "C" int drd_set_params(drd_ctx* c, const drd_params* prm) {
   if (!c || !prm) return set_err(DRD_E_ARG, "null argument");
   if (prm->nx != c->p.nx || prm->ny != c->p.ny) return set_err(DRD_E_ARG, "grid size differs from the context");
-  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd')
-    return set_err(DRD_E_ARG, "lr_bd_type must be 'n' or 'd'");
+  if (prm->lr_bd_type != 'n' && prm->lr_bd_type != 'd' && prm->lr_bd_type != 'p')
+    return set_err(DRD_E_ARG, "lr_bd_type must be 'n', 'd' or 'p'");
+  if ((prm->lr_bd_type == 'p') != (c->g.px != 0))
+    return set_err(DRD_E_ARG, "periodic x is fixed when the context is created");
   if (prm->phi_mode != 1 && prm->phi_mode != 2) return set_err(DRD_E_ARG, "phi_mode must be 1 or 2");
   if (prm->variant != DRD_VARIANT_TP && prm->variant != DRD_VARIANT_MF)
     return set_err(DRD_E_ARG, "variant must be DRD_VARIANT_TP or DRD_VARIANT_MF");
+  RCHK(check_periodic(*prm, c->nb));
   if (std::memcmp(prm, &c->p, sizeof(drd_params)) == 0) return 0;
   HIPCHK(hipStreamSynchronize(c->stream));
   drop_graphs(c);
@@ -616,11 +650,21 @@ extern "C" int drd_synchronize(drd_ctx* c) {
   return 0;
 }
 
+// Periodic x: a field of n doubles written from outside takes the periodic
+// ghost columns (the kind follows from its size; 1-D fields are left alone).
+static void fill_ghosts(drd_ctx* c, double* f, int64_t n) {
+  if (!c->g.px) return;
+  const Grid& g = c->g;
+  const int kind = n == g.nuf() ? 1 : (n == g.nvf() ? 2 : (n == g.nc() ? 0 : -1));
+  if (kind >= 0) launch_xghost_fill(c->stream, g, f, kind);
+}
+
 extern "C" int drd_set_field(drd_ctx* c, int32_t field, const double* host) {
   if (!c || !host) return set_err(DRD_E_ARG, "null argument");
   const int64_t n = drd_field_size(&c->p, field);
   if (n <= 0) return set_err(DRD_E_ARG, "unknown field id");
   HIPCHK(hipMemcpyAsync(c->f[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  fill_ghosts(c, c->f[field], n);
   HIPCHK(hipStreamSynchronize(c->stream));
   if (field == DRD_F_ANGLEB || field == DRD_F_ANGLET) RCHK(upload_angles(c));
   return 0;
@@ -715,6 +759,8 @@ static PhmuCoef32 phmu_c32(const drd_ctx* c) {
 }
 
 static int prec_setup(drd_ctx* c, int prec) {
+  if (c->g.px && !periodic_prec_ok(prec))
+    return set_err(DRD_E_UNSUPPORTED, "periodic x takes the Jacobi, identity and phi/mu block preconditioners");
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, c->s.re);
   if (prec == DRD_PS_VELO2_JACOBI)
     launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, 1.5 * c->s.re);
@@ -1766,6 +1812,12 @@ static int up(drd_ctx* c, double* dst, const double* src, int64_t n) {
   HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   return 0;
 }
+// a field argument (n its field size): periodic x fills its ghost columns
+static int up_field(drd_ctx* c, double* dst, const double* src, int64_t n) {
+  RCHK(up(c, dst, src, n));
+  if (src) fill_ghosts(c, dst, n);
+  return 0;
+}
 static int down(drd_ctx* c, double* dst, const double* src, int64_t n) {
   if (!dst) return 0;
   HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1778,10 +1830,10 @@ extern "C" int drd_velosolve_combined(drd_ctx* c, const double* psi, double* u_a
   if (!c || !u_after || !v_after || !rmu_after || !ph_after) return set_err(DRD_E_ARG, "null argument");
   (void)psi;  // temp_p = p (TP/VeloSolve_combined.f90:192)
   const Grid& g = c->g;
-  RCHK(up(c, c->arg[0], ph_after, g.nc()));
-  RCHK(up(c, c->arg[1], rmu_after, g.nc()));
-  RCHK(up(c, c->arg[2], pf1, g.nc()));
-  RCHK(up(c, c->arg[3], pf2, g.nc()));
+  RCHK(up_field(c, c->arg[0], ph_after, g.nc()));
+  RCHK(up_field(c, c->arg[1], rmu_after, g.nc()));
+  RCHK(up_field(c, c->arg[2], pf1, g.nc()));
+  RCHK(up_field(c, c->arg[3], pf2, g.nc()));
   drd_solve_info loc{};
   RCHK(velo_solve_dev(c, 1, nullptr, nullptr, nullptr, nullptr, c->arg[0], c->arg[1], t, pf1 ? c->arg[2] : nullptr,
                       pf2 ? c->arg[3] : nullptr, c->u_after, c->v_after, &loc, true));
@@ -1802,14 +1854,14 @@ extern "C" int drd_velosolve_2_combined(drd_ctx* c, const double* psi1, const do
   (void)ph_pre;
   (void)rmu_pre;  // not read by the active code
   const Grid& g = c->g;
-  RCHK(up(c, c->arg[0], ph_after, g.nc()));
-  RCHK(up(c, c->arg[1], rmu_after, g.nc()));
-  RCHK(up(c, c->arg[2], pf1, g.nc()));
-  RCHK(up(c, c->arg[3], pf2, g.nc()));
-  RCHK(up(c, c->arg[4], psi1, g.nc()));
-  RCHK(up(c, c->arg[5], psi2, g.nc()));
-  RCHK(up(c, c->arg[6], u_pre, g.nuf()));
-  RCHK(up(c, c->arg[7], v_pre, g.nvf()));
+  RCHK(up_field(c, c->arg[0], ph_after, g.nc()));
+  RCHK(up_field(c, c->arg[1], rmu_after, g.nc()));
+  RCHK(up_field(c, c->arg[2], pf1, g.nc()));
+  RCHK(up_field(c, c->arg[3], pf2, g.nc()));
+  RCHK(up_field(c, c->arg[4], psi1, g.nc()));
+  RCHK(up_field(c, c->arg[5], psi2, g.nc()));
+  RCHK(up_field(c, c->arg[6], u_pre, g.nuf()));
+  RCHK(up_field(c, c->arg[7], v_pre, g.nvf()));
   drd_solve_info loc{};
   RCHK(velo_solve_dev(c, 2, c->arg[4], c->arg[5], c->arg[6], c->arg[7], c->arg[0], c->arg[1], t,
                       pf1 ? c->arg[2] : nullptr, pf2 ? c->arg[3] : nullptr, c->u_after, c->v_after, &loc, true));
@@ -1994,6 +2046,8 @@ extern "C" int drd_particle_force(drd_ctx* c, const double* bx, const double* by
                    c->arg[1], c->arg[2]);
   launch_body_spread(c->stream, c->g, p.dx, p.dz, p.I_thickness, lx, ly, c->arg[0], nbody, c->arg[2],
                      c->f[DRD_F_PFORCE1], c->f[DRD_F_PFORCE2]);
+  fill_ghosts(c, c->f[DRD_F_PFORCE1], c->g.nc());  // periodic x: the ghost columns the momentum rows read
+  fill_ghosts(c, c->f[DRD_F_PFORCE2], c->g.nc());
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -2343,6 +2397,7 @@ extern "C" int drd_batch_create(int32_t count, const drd_params* params, drd_bat
         q.velo_prec != DRD_PREC_IDENTITY && q.velo_prec != DRD_PREC_REFERENCE)
       return set_err(DRD_E_UNSUPPORTED, "batched steps take the DCT, Jacobi or identity velocity preconditioner");
     if (q.krylov != DRD_KRYLOV_BICGSTAB) return set_err(DRD_E_UNSUPPORTED, "batched steps use BiCGSTAB");
+    if (q.lr_bd_type == 'p') return set_err(DRD_E_UNSUPPORTED, "periodic x takes a single-instance context");
     for (size_t gI = 0; gI < rep_of.size(); ++gI) {
       const drd_params& r = params[rep_of[gI]];
       if (r.nx == q.nx && r.ny == q.ny) {

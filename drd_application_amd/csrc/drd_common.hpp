@@ -36,18 +36,27 @@ struct FastDiv {
   }
 };
 
+// px: periodic in x (lr_bd_type 'p'): a u row has nx unknowns (u(nx) =
+// u(0)) instead of nx + 1, and the ghost columns of every field hold the
+// periodic copies f(-1) = f(nx-1), f(nx) = f(0) (u: also f(nx+1) = f(1)).
 struct Grid {
   int nx, ny;
-  FastDiv by_nx1, by_nx;  // division by nx + 1 and by nx
-  __host__ static Grid make(int nx, int ny) {
-    return Grid{nx, ny, FastDiv::make((uint32_t)nx + 1), FastDiv::make((uint32_t)nx)};
+  FastDiv by_nxu, by_nx;  // division by nxu() (u unknowns per row) and by nx
+  int px;
+  __host__ static Grid make(int nx, int ny, int px = 0) {
+    return Grid{nx, ny, FastDiv::make((uint32_t)(nx + 1 - px)), FastDiv::make((uint32_t)nx), px};
   }
+  __host__ __device__ int nxu() const { return nx + 1 - px; }
   __host__ __device__ int ldc() const { return nx + 2; }
   __host__ __device__ int ldu() const { return nx + 3; }
   __host__ __device__ int64_t nc() const { return (int64_t)(nx + 2) * (ny + 2); }
   __host__ __device__ int64_t nuf() const { return (int64_t)(nx + 3) * (ny + 2); }
   __host__ __device__ int64_t nvf() const { return (int64_t)(nx + 2) * (ny + 1); }
-  __host__ __device__ int64_t mn() const { return (int64_t)(nx + 1) * ny; }
+  __host__ __device__ int64_t mn() const { return (int64_t)nxu() * ny; }
+  // the interior column of a ghost column: wrapped (px) or clamped
+  __host__ __device__ int xcol(int i) const {
+    return px ? (i < 0 ? i + nx : (i >= nx ? i - nx : i)) : (i < 0 ? 0 : (i >= nx ? nx - 1 : i));
+  }
   __host__ __device__ int64_t nuv() const { return mn() + (int64_t)nx * (ny - 1); }
   __host__ __device__ int64_t ncell() const { return (int64_t)nx * ny; }
   __host__ __device__ int64_t ci(int i, int k) const { return (int64_t)(i + 1) + (int64_t)(nx + 2) * (k + 1); }

@@ -115,7 +115,7 @@ __device__ inline double velo_diag2(const Grid& g, const Scal& s, const T* __res
   auto X = [&](int i, int k) { return (double)chi[g.ci(i, k)]; };
   double d;
   if (q < g.mn()) {
-    const int k = g.by_nx1.div(q), i = (int)(q - (int64_t)k * (nx + 1));
+    const int k = g.by_nxu.div(q), i = (int)(q - (int64_t)k * g.nxu());
     d = 1.5 * s.re * (R(i, k) + R(i - 1, k)) / 2.0 + s.penalty_k * (X(i, k) + X(i - 1, k)) / 2.0 * s.dt;
     d += 2.0 * s.dt * (E(i, k) + E(i - 1, k)) / s.dr2;
     const double cr = (E(i, k + 1) + E(i - 1, k + 1) + E(i, k) + E(i - 1, k)) / 4.0;
@@ -146,14 +146,15 @@ struct Nb5T {
   R c, xl, xr, yl, yr;
 };
 using Nb5 = Nb5T<double>;
+// (periodic x: the x neighbours wrap)
 __device__ __forceinline__ Nb5 nb5_cell(const Grid& g, const double* __restrict__ a, int i, int j) {
-  const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
+  const int il = i > 0 ? i - 1 : (g.px ? g.nx - 1 : i), ir = i < g.nx - 1 ? i + 1 : (g.px ? 0 : i);
   const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
   return Nb5{a[g.ci(i, j)], a[g.ci(il, j)], a[g.ci(ir, j)], a[g.ci(i, jl)], a[g.ci(i, jr)]};
 }
 // the same for a packed interior field x[i + nx j]
 __device__ __forceinline__ Nb5 nb5_packed(const Grid& g, const double* x, int i, int j) {
-  const int il = i > 0 ? i - 1 : i, ir = i < g.nx - 1 ? i + 1 : i;
+  const int il = i > 0 ? i - 1 : (g.px ? g.nx - 1 : i), ir = i < g.nx - 1 ? i + 1 : (g.px ? 0 : i);
   const int jl = j > 0 ? j - 1 : j, jr = j < g.ny - 1 ? j + 1 : j;
   const int64_t nx = g.nx;
   return Nb5{x[i + nx * j], x[il + nx * j], x[ir + nx * j], x[i + nx * jl], x[i + nx * jr]};
@@ -169,8 +170,8 @@ __device__ __forceinline__ R dgn_diag_v(const Grid& g, const Scal& s, const Nb5T
   const R tl = (mo.xl + mo.c) / R(2.0), tr = (mo.c + mo.xr) / R(2.0);
   const R bl = (mo.yl + mo.c) / R(2.0), br = (mo.c + mo.yr) / R(2.0);
   R dx = R(0.0), dy = R(0.0);
-  dx = i > 0 ? dx + tl : dx;
-  dx = i < g.nx - 1 ? dx + tr : dx;
+  dx = (g.px || i > 0) ? dx + tl : dx;
+  dx = (g.px || i < g.nx - 1) ? dx + tr : dx;
   dy = j > 0 ? dy + bl : dy;
   dy = j < g.ny - 1 ? dy + br : dy;
   if (sizeof(R) == sizeof(float)) return -(dx * R(s.idr2) + dy * R(s.idz2));
@@ -317,6 +318,8 @@ void launch_phmu_jacobi(hipStream_t st, const Grid& g, const Scal& s, const doub
 void launch_velo_pack(hipStream_t st, const Grid& g, const double* u, const double* v, double* x);
 void launch_velo_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* uL,
                         double* u, double* v);
+// periodic x: the ghost columns of a field (kind 0 cell, 1 u, 2 v) from its interior
+void launch_xghost_fill(hipStream_t st, const Grid& g, double* f, int kind);
 void launch_velo_rhs(hipStream_t st, int bdf, const Grid& g, const Scal& s, const VeloRhsArgs& a,
                      double* rhs);
 void launch_phmu_m0(hipStream_t st, const Grid& g, int mode, const double* ph, const double* bdr,

@@ -195,6 +195,60 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
   }
 }
 
+// Real DFT of a line held in registers, in FFTW's halfcomplex order (R2HC):
+// r_k = Re X_k for k <= N/2, r_{N-k} = Im X_k for 0 < k < N/2, X_k = sum_j
+// x_j exp(-2 pi i jk/N), as the complex FFT of (x, 0).  N a power of two.
+__device__ void r2hc_regs(double (&r)[kMaxPer], int N, double2* A, double2* B, const double2* __restrict__ tw) {
+#pragma unroll
+  for (int m = 0; m < kMaxPer; ++m) {
+    const int j = threadIdx.x + m * kBlock;
+    if (j < N) A[j] = make_double2(r[m], 0.0);
+  }
+  __syncthreads();
+  const double2* V = lds_fft(A, B, N, tw, -1);
+#pragma unroll
+  for (int m = 0; m < kMaxPer; ++m) {
+    const int k = threadIdx.x + m * kBlock;
+    if (k < N) r[m] = k <= N / 2 ? V[k].x : V[N - k].y;
+  }
+  __syncthreads();
+}
+
+// HC2R, the unnormalised inverse of r2hc_regs: x_j = sum_k X_k exp(+2 pi i
+// jk/N) of the Hermitian X the halfcomplex line holds.
+__device__ void hc2r_regs(double (&r)[kMaxPer], int N, double2* A, double2* B, const double2* __restrict__ tw) {
+  double* ys = reinterpret_cast<double*>(B);
+#pragma unroll
+  for (int m = 0; m < kMaxPer; ++m) {
+    const int k = threadIdx.x + m * kBlock;
+    if (k < N) ys[k] = r[m];
+  }
+  __syncthreads();
+  const int h = N / 2;
+#pragma unroll
+  for (int m = 0; m < kMaxPer; ++m) {
+    const int k = threadIdx.x + m * kBlock;
+    if (k < N) {
+      double2 v;
+      if (k == 0 || k == h)
+        v = make_double2(ys[k], 0.0);
+      else if (k < h)
+        v = make_double2(ys[k], ys[N - k]);
+      else
+        v = make_double2(ys[N - k], -ys[k]);
+      A[k] = v;
+    }
+  }
+  __syncthreads();
+  const double2* V = lds_fft(A, B, N, tw, +1);
+#pragma unroll
+  for (int m = 0; m < kMaxPer; ++m) {
+    const int j = threadIdx.x + m * kBlock;
+    if (j < N) r[m] = V[j].x;
+  }
+  __syncthreads();
+}
+
 enum { R_DCT2 = 0, R_DCT3, R_DIV_DCT2, R_SOLVE, R_DCT3_FINISH };
 
 struct RowArgs {
@@ -205,6 +259,7 @@ struct RowArgs {
   const double2 *tw, *rot;
   const double* cs;
   bool pow2;
+  int per;  // periodic x: the x passes of R_DIV_DCT2 / R_DCT3_FINISH are R2HC / HC2R
   // R_DIV_DCT2
   Grid g;
   const double *u, *v;
@@ -268,7 +323,10 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
     if (threadIdx.x == 0) rowsum = pairwise_fixed<kMaxPer * kWaves>(lc);
     __syncthreads();
   }
-  if (MODE == R_DCT2 || MODE == R_DIV_DCT2 || MODE == R_SOLVE) dct2_regs(r, N, a.pow2, A, B, a.tw, a.rot, a.cs);
+  if (MODE == R_DIV_DCT2 && a.per)
+    r2hc_regs(r, N, A, B, a.tw);
+  else if (MODE == R_DCT2 || MODE == R_DIV_DCT2 || MODE == R_SOLVE)
+    dct2_regs(r, N, a.pow2, A, B, a.tw, a.rot, a.cs);
   if (MODE == R_SOLVE) {
     // rhs_wk / pre_cof TP/dct_fftw.f90:22-26 (this line is column kx = row)
     const int ldp = a.g.nx + 1;
@@ -278,7 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
       if (ky < N) r[m] = r[m] / a.pre[row + (size_t)ldp * ky];
     }
   }
-  if (MODE == R_DCT3 || MODE == R_SOLVE || MODE == R_DCT3_FINISH) dct3_regs(r, N, a.pow2, A, B, a.tw, a.rot, a.cs);
+  if (MODE == R_DCT3_FINISH && a.per)
+    hc2r_regs(r, N, A, B, a.tw);
+  else if (MODE == R_DCT3 || MODE == R_SOLVE || MODE == R_DCT3_FINISH)
+    dct3_regs(r, N, a.pow2, A, B, a.tw, a.rot, a.cs);
   if (MODE == R_DCT3_FINISH) {
     // psi = rhs_wk/coef (dct_fftw.f90:32-36), Neumann ghosts (:43-51),
     // psi *= coef (PreSolve_2.f90:125), p = psi + p (TP/mcl6.f90:502)
@@ -299,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
     }
     __syncthreads();
     for (int i = (int)threadIdx.x - 1; i <= nx; i += blockDim.x) {
-      const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+      const int ic = g.xcol(i);  // Neumann or periodic side ghosts
       const double val = xs[ic] / a.c4;
       put(i, k, val);
       if (k == 0) put(i, -1, val);
@@ -415,6 +476,7 @@ void launch_pressure(hipStream_t st, DctPlan* pl, const Grid& g, const Scal& s, 
   a.out = w0;
   a.part = part;
   a.ks = ks;
+  a.per = g.px;
   launch_rows<R_DIV_DCT2>(st, a);
   if (ks) launch_finalize(st, part, a.rows, 1, 1, ks->red, nullptr, PH_NONE, FIN_ALWAYS);
   launch_transpose(st, g.ny, g.nx, w0, w1);
@@ -428,7 +490,8 @@ void launch_pressure(hipStream_t st, DctPlan* pl, const Grid& g, const Scal& s, 
   RowArgs c = rows_x(pl);
   c.g = g;
   c.in = w0;
-  c.c4 = 4.0 * g.nx * g.ny;
+  c.per = g.px;
+  c.c4 = g.px ? 2.0 * g.nx * g.ny : 4.0 * g.nx * g.ny;  // R2HC + HC2R scale by nx, DCT-II + III by 2 nx
   c.coef = coef;
   c.psi = psi;
   c.p = p;

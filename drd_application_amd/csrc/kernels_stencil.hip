@@ -25,13 +25,15 @@ struct LdVec {
   const T* __restrict__ x;
   __device__ __forceinline__ T operator()(uint32_t i) const { return ld32(x, i); }
 };
-template <class L>
+// PX: periodic in x, a u row of nx unknowns and every side ghost the
+// periodic copy (u(-1) = u(nx-1), u(nx) = u(0), u(nx+1) = u(1); v likewise).
+template <class L, bool PX = false>
 struct VeloAcc {
   using R = typename L::R;
   L ld;
   int nx, ny, lr_d;
   uint32_t mn;
-  __device__ __forceinline__ R uraw(int i, int k) const { return ld((uint32_t)(i + (nx + 1) * k)); }
+  __device__ __forceinline__ R uraw(int i, int k) const { return ld((uint32_t)(i + (PX ? nx : nx + 1) * k)); }
   // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
   // wall ghosts are odd mirrors of the first/last row, side ghosts even or
   // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
@@ -39,6 +41,11 @@ struct VeloAcc {
   __device__ __forceinline__ R U(int i, int k) const {
     const bool kw = (k < 0) | (k >= ny);
     const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
+    if (PX) {
+      const int ii = i < 0 ? i + nx : (i >= nx ? i - nx : i);
+      const R v = ld((uint32_t)(ii + nx * kk));
+      return kw ? -v : v;
+    }
     const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
     const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
     const R v = ld((uint32_t)(ii + (nx + 1) * kk));
@@ -47,14 +54,14 @@ struct VeloAcc {
   }
   // f2(i,k) for -1<=i<=nx, 0<=k<=ny (0 on the walls k = 0, ny)
   __device__ __forceinline__ R V(int i, int k) const {
-    const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+    const int ic = PX ? (i < 0 ? i + nx : (i >= nx ? i - nx : i)) : (i < 0 ? 0 : (i >= nx ? nx - 1 : i));
     const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
     const R v = ld(mn + (uint32_t)(ic + nx * (kc - 1)));
     return (k <= 0 || k >= ny) ? R(0.0) : v;
   }
 };
-template <class T>
-using VeloInT = VeloAcc<LdVec<T>>;
+template <class T, bool PX = false>
+using VeloInT = VeloAcc<LdVec<T>, PX>;
 using VeloIn = VeloInT<double>;
 
 // y = A x for one unknown q of the velocity operator.
@@ -74,8 +81,8 @@ __device__ __forceinline__ T velo_u_row(const Grid g, const Scal& s, const T* __
   const int nx = g.nx;
   T r;
   {
-    const int k = g.by_nx1.div(q);
-    const int i = (int)(q - (uint32_t)k * (uint32_t)(nx + 1));
+    const int k = g.by_nxu.div(q);
+    const int i = (int)(q - (uint32_t)k * (uint32_t)g.nxu());
     const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
     const T f = X.U(i, k);
     if (BDF == 2)
@@ -141,7 +148,7 @@ __device__ __forceinline__ bool kactive(const KState* st) {
 //   MV_DOT1  : y = A x, partial (a0, y)                 -> PH_MV1
 //   MV_DOT2  : y = A x, partials (y, a0), (y, y)        -> PH_MV2
 //   MV_INIT  : y = b - A x (if ||x||!=0, else b); a0 := y; partial (y,y) -> PH_INIT_R
-template <int BDF, int MODE>
+template <int BDF, int MODE, bool PX>
 __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const double* __restrict__ rho,
                                                         const double* __restrict__ eta,
                                                         const double* __restrict__ chi,
@@ -159,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_velo_matvec(Grid g, Scal s, const do
   a0 = bofs(a0, bs);
   b = bofs(b, bs);
   part = bofs(part, bs);
-  VeloIn X{{x}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
+  VeloInT<double, PX> X{{x}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
   const uint32_t n = (uint32_t)g.nuv();
   const bool xnz = (MODE == MV_INIT) ? (st->xnz != 0) : true;
   // A workgroup sweeps tile `pair` (u half of the vector) and tile
@@ -243,8 +250,8 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
   if (q >= g.nuv()) return;
   double tmp;
   if (q < g.mn()) {
-    const int j = g.by_nx1.div(q);
-    const int i = (int)(q - (int64_t)j * (nx + 1));
+    const int j = g.by_nxu.div(q);
+    const int i = (int)(q - (int64_t)j * g.nxu());
     tmp = mcoef * (rho[g.ci(i, j)] + rho[g.ci(i - 1, j)]) / 2.0;
     double cr = eta[g.ci(i, j)], cl = eta[g.ci(i - 1, j)];
     tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dr2;
@@ -265,9 +272,9 @@ __global__ __launch_bounds__(kBlock) void k_velo_jacobi_diag(Grid g, Scal s, con
     tmp = tmp + 2.0 * s.dt * (cl + cr) / s.dz2;
     cr = (eta[g.ci(i + 1, j)] + eta[g.ci(i, j)] + eta[g.ci(i + 1, j - 1)] + eta[g.ci(i, j - 1)]) / 4.0;
     cl = (eta[g.ci(i, j)] + eta[g.ci(i - 1, j)] + eta[g.ci(i, j - 1)] + eta[g.ci(i - 1, j - 1)]) / 4.0;
-    if (i == 0)
+    if (!g.px && i == 0)
       tmp = tmp + s.dt * cr / s.dr2;
-    else if (i == nx - 1)
+    else if (!g.px && i == nx - 1)
       tmp = tmp + s.dt * cl / s.dr2;
     else
       tmp = tmp + s.dt * (cl + cr) / s.dr2;
@@ -299,8 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __re
   if (q >= g.nuv()) return;
   const int nx = g.nx;
   if (q < g.mn()) {
-    const int k = g.by_nx1.div(q);
-    const int i = (int)(q - (int64_t)k * (nx + 1));
+    const int k = g.by_nxu.div(q);
+    const int i = (int)(q - (int64_t)k * g.nxu());
     x[q] = u[g.ui(i, k)];
   } else {
     const int64_t t = q - g.mn();
@@ -312,6 +319,8 @@ __global__ __launch_bounds__(kBlock) void k_velo_pack(Grid g, const double* __re
 
 // btoA_uv + the ghost rows of VeloSolve_2_combined TP/VeloSolve_2_combined.f90:426-457.
 // One thread per element of the u field and one per element of the v field.
+// PX: periodic side ghosts (the wall ghosts then follow from them).
+template <bool PX>
 __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const double* __restrict__ x,
                                                         const double* __restrict__ uL,
                                                         double* __restrict__ u, double* __restrict__ v,
@@ -322,12 +331,13 @@ __global__ __launch_bounds__(kBlock) void k_velo_unpack(Grid g, Scal s, const do
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int nx = g.nx, ny = g.ny;
   const int64_t nu = g.nuf();
-  VeloIn X{{x}, nx, ny, 0, (uint32_t)g.mn()};
+  VeloInT<double, PX> X{{x}, nx, ny, 0, (uint32_t)g.mn()};
   if (q < nu) {
     const int k = (int)(q / (nx + 3)) - 1;
     const int i = (int)(q - (int64_t)(k + 1) * (nx + 3)) - 1;
     // interior / left-right ghosts for rows 0..ny-1, then wall rows
     auto side = [&](int ii, int kk) -> double {
+      if (PX) return X.uraw(ii < 0 ? ii + nx : (ii >= nx ? ii - nx : ii), kk);
       if (ii >= 0 && ii <= nx) return X.uraw(ii, kk);
       if (s.lr_d) return ii < 0 ? 2.0 * uL[kk] - X.uraw(1, kk) : 2.0 * uL[kk] - X.uraw(nx - 1, kk);
       return ii < 0 ? X.uraw(1, kk) : X.uraw(nx - 1, kk);
@@ -393,8 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_velo_rhs(Grid g, Scal s, VeloRhsArgs
   };
   double f;
   if (q < g.mn()) {
-    const int k = g.by_nx1.div(q);
-    const int i = (int)(q - (int64_t)k * (nx + 1));
+    const int k = g.by_nxu.div(q);
+    const int i = (int)(q - (int64_t)k * g.nxu());
     const int64_t c0 = g.ci(i, k), c1 = g.ci(i - 1, k);
     if (s.mf)  // MF/VeloSolve_2_combined.f90:64-65
       f = s.ca * (mu[c0] + mu[c1]) / 2.0 * (ph[c0] - ph[c1]) / s.dr * (1.0 - (bd[c0] + bd[c1]) / 2.0);
@@ -591,8 +601,9 @@ __global__ __launch_bounds__(kBlock) void k_phmu_jacobi(Grid g, Scal s, const do
   if (q >= C) return;
   const int nx = g.nx, ny = g.ny;
   const int j = g.by_nx.div(q), i = (int)(q - (int64_t)j * nx);
-  const double tmpx = (i == 0 || j == nx - 1) ? 1.0 / s.dr2 : 2.0 / s.dr2;
-  const double tmpz = (i == 0 || j == ny - 1) ? 1.0 / s.dz2 : 2.0 / s.dz2;
+  // periodic x (no reference counterpart): the unit-mobility diagonal
+  const double tmpx = (!g.px && (i == 0 || j == nx - 1)) ? 1.0 / s.dr2 : 2.0 / s.dr2;
+  const double tmpz = ((g.px ? j == 0 : i == 0) || j == ny - 1) ? 1.0 / s.dz2 : 2.0 / s.dz2;
   x[q] = b[q] / ((tmpx + tmpz) * s.I + s.s / s.I);
   x[C + q] = b[C + q] / (s.dt * s.xld * (tmpx + tmpz));
 }
@@ -697,13 +708,13 @@ __global__ __launch_bounds__(kBlock) void k_phmu_unpack(Grid g, Scal s, const do
   const int k = (int)(q / (nx + 2)) - 1;
   const int i = (int)(q - (int64_t)(k + 1) * (nx + 2)) - 1;
   const int64_t C = g.ncell();
-  const int ic = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+  const int ic = g.xcol(i);
   const int kc = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
   const double p = x[ic + (int64_t)nx * kc], m = x[C + ic + (int64_t)nx * kc];
   double pv = p;
-  if (i >= 0 && i < nx) {
-    if (k < 0) pv = p * s.coef_bot + pn[i];
-    if (k >= ny) pv = p * s.coef_top + pn[nx + i];
+  if (g.px || (i >= 0 && i < nx)) {  // periodic x: the corners copy the wall ghosts too
+    if (k < 0) pv = p * s.coef_bot + pn[ic];
+    if (k >= ny) pv = p * s.coef_top + pn[nx + ic];
   }
   ph[q] = pv;
   mu[q] = m;
@@ -870,7 +881,7 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_coef(Grid g, double dr, 
   if (q >= g.nc()) return;
   const int ld = g.nx + 2;
   const int k = (int)(q / ld) - 1, i = (int)(q - (int64_t)(k + 1) * ld) - 1;
-  const int ic = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
+  const int ic = g.xcol(i);  // periodic x: the ghost column's periodic copy
   const int kc = k < 0 ? 0 : (k > g.ny - 1 ? g.ny - 1 : k);
   const double x = (ic + 0.5) * dr, y = (kc + 0.5) * dz;
   double d = 1.0E10;
@@ -1316,9 +1327,15 @@ void launch_velo_matvec(hipStream_t st, int bdf, int mode, const Grid& g, const 
                         hipEvent_t ev1) {
   const unsigned ntiles = nblk(g.nuv(), kTile);
   dim3 grid((ntiles + 1) / 2, 1, bat_n());  // two tiles per workgroup
-#define VM(B, M)                                                                                                \
-  hipExtLaunchKernelGGL((k_velo_matvec<B, M>), grid, dim3(kBlock), 0, st, ev0, ev1, 0u, g, s, c.rho, c.eta,     \
-                        c.chi, x, y, alpha, beta, a0, b, part, ks, bat_bs())
+#define VM(B, M)                                                                                               \
+  do {                                                                                                         \
+    if (g.px)                                                                                                  \
+      hipExtLaunchKernelGGL((k_velo_matvec<B, M, true>), grid, dim3(kBlock), 0, st, ev0, ev1, 0u, g, s, c.rho, \
+                            c.eta, c.chi, x, y, alpha, beta, a0, b, part, ks, bat_bs());                       \
+    else                                                                                                       \
+      hipExtLaunchKernelGGL((k_velo_matvec<B, M, false>), grid, dim3(kBlock), 0, st, ev0, ev1, 0u, g, s,       \
+                            c.rho, c.eta, c.chi, x, y, alpha, beta, a0, b, part, ks, bat_bs());                \
+  } while (0)
   if (bdf == 1) {
     switch (mode) {
       case MV_PLAIN: VM(1, MV_PLAIN); break;
@@ -1379,8 +1396,31 @@ void launch_velo_pack(hipStream_t st, const Grid& g, const double* u, const doub
 
 void launch_velo_unpack(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* uL,
                         double* u, double* v) {
-  hipLaunchKernelGGL(k_velo_unpack, dim3(nblk(g.nuf() + g.nvf(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s,
-                     x, uL, u, v, bat_bs());
+  if (g.px)
+    hipLaunchKernelGGL(k_velo_unpack<true>, dim3(nblk(g.nuf() + g.nvf(), kBlock), 1, bat_n()), dim3(kBlock), 0, st,
+                       g, s, x, uL, u, v, bat_bs());
+  else
+    hipLaunchKernelGGL(k_velo_unpack<false>, dim3(nblk(g.nuf() + g.nvf(), kBlock), 1, bat_n()), dim3(kBlock), 0, st,
+                       g, s, x, uL, u, v, bat_bs());
+}
+
+// Periodic x: the ghost columns of a field from its interior, over every row
+// (wall-ghost rows included): cell and v fields f(-1) = f(nx-1), f(nx) =
+// f(0); u fields also f(nx+1) = f(1).  One thread per row.
+__global__ __launch_bounds__(kBlock) void k_xghost_fill(Grid g, double* __restrict__ f, int kind, int64_t bs) {
+  f = bofs(f, bs);
+  const int r = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int rows = kind == 2 ? g.ny + 1 : g.ny + 2, ld = kind == 1 ? g.nx + 3 : g.nx + 2;
+  if (r >= rows) return;
+  double* row = f + (int64_t)r * ld;  // element i of the row at row[i + 1]
+  row[0] = row[g.nx];
+  row[g.nx + 1] = row[1];
+  if (kind == 1) row[g.nx + 2] = row[2];
+}
+
+void launch_xghost_fill(hipStream_t st, const Grid& g, double* f, int kind) {
+  hipLaunchKernelGGL(k_xghost_fill, dim3(nblk(g.ny + 2, kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, f, kind,
+                     bat_bs());
 }
 
 void launch_velo_rhs(hipStream_t st, int bdf, const Grid& g, const Scal& s, const VeloRhsArgs& a,

@@ -83,6 +83,28 @@ def _neumann_ghosts(a: np.ndarray) -> np.ndarray:
     return a
 
 
+def _ghosts(a: np.ndarray, periodic: bool) -> np.ndarray:
+    """Ghost rows copy their neighbour row (Neumann in z); ghost columns copy
+    their neighbour column, or with periodic the periodic image column
+    (f(-1) = f(nx-1), f(nx) = f(0)), wall-ghost rows included."""
+    if not periodic:
+        return _neumann_ghosts(a)
+    a[0, :] = a[1, :]
+    a[-1, :] = a[-2, :]
+    a[:, 0] = a[:, -2]
+    a[:, -1] = a[:, 1]
+    return a
+
+
+def _images(p: abi.Params, cx: float) -> list:
+    """x positions of a shape centred at cx: itself, and with periodic x its
+    images one domain length to either side."""
+    if not abi.periodic_x(p):
+        return [cx]
+    L = p.nx * p.dx
+    return [cx - L, cx, cx + L]
+
+
 def _window(p: abi.Params, cx: float, cz: float, reach: float):
     """Ghosted index ranges (cols, rows) of the cells whose centres lie within
     `reach` of (cx, cz) in each coordinate, and those centres."""
@@ -109,10 +131,13 @@ def droplet_phase(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: flo
     for _ in range(n):
         cx, cz = rng.uniform(0, L), rng.uniform(0, H)
         r = rng.uniform(rmin, rmax)
-        ks, is_, x, z = _window(p, cx, cz, r + TANH_SAT * math.sqrt(2.0) * eps)
-        d = r - np.hypot(x[None, :] - cx, z[:, None] - cz)
-        w = phi[ks, is_]
-        np.maximum(w, np.tanh(d / (math.sqrt(2.0) * eps)), out=w)
+        for xc in _images(p, cx):
+            ks, is_, x, z = _window(p, xc, cz, r + TANH_SAT * math.sqrt(2.0) * eps)
+            if x.size == 0:
+                continue
+            d = r - np.hypot(x[None, :] - xc, z[:, None] - cz)
+            w = phi[ks, is_]
+            np.maximum(w, np.tanh(d / (math.sqrt(2.0) * eps)), out=w)
     return phi
 
 
@@ -124,10 +149,13 @@ def disc_mask(p: abi.Params, rng: SplitMix64, n: int, rmin: float, rmax: float, 
     for _ in range(n):
         cx, cz = rng.uniform(0, L), rng.uniform(0, H)
         r = rng.uniform(rmin, rmax)
-        ks, is_, x, z = _window(p, cx, cz, r + TANH_SAT * math.sqrt(2.0) * width)
-        ds = np.hypot(x[None, :] - cx, z[:, None] - cz) - r
-        w = out[ks, is_]
-        np.maximum(w, 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width))), out=w)
+        for xc in _images(p, cx):
+            ks, is_, x, z = _window(p, xc, cz, r + TANH_SAT * math.sqrt(2.0) * width)
+            if x.size == 0:
+                continue
+            ds = np.hypot(x[None, :] - xc, z[:, None] - cz) - r
+            w = out[ks, is_]
+            np.maximum(w, 0.5 * (1.0 + np.tanh(-ds / (math.sqrt(2.0) * width))), out=w)
     return out
 
 
@@ -141,20 +169,24 @@ def serpentine_mask(p: abi.Params, amp: float, waves: float, half_width: float, 
 
 def derived_fields(p: abi.Params, phi: np.ndarray, bdr: np.ndarray, lambda_eta: float,
                    sharp_penalty: bool = False) -> dict:
-    """Secondary fields exactly as TP/finit.f90 derives them from bdr_ph and phi."""
-    bdr = _neumann_ghosts(bdr.copy())
-    ph = _neumann_ghosts((1.0 - bdr) * phi)
+    """Secondary fields exactly as TP/finit.f90 derives them from bdr_ph and phi
+    (ghost columns periodic with periodic x)."""
+    per = abi.periodic_x(p)
+    bdr = _ghosts(bdr.copy(), per)
+    ph = _ghosts((1.0 - bdr) * phi, per)
     eps = p.I_thickness
     dr2, dz2 = p.dx * p.dx, p.dz * p.dz
     rmu = np.zeros_like(ph)
     c = ph[1:-1, 1:-1]
     lap = (ph[1:-1, 2:] + ph[1:-1, :-2] - 2 * c) / dr2 + (ph[2:, 1:-1] + ph[:-2, 1:-1] - 2 * c) / dz2
     rmu[1:-1, 1:-1] = -eps * lap - c / eps + c ** 3 / eps
-    rmu = _neumann_ghosts(rmu)
+    rmu = _ghosts(rmu, per)
     grad = np.zeros_like(bdr)
     gx = (bdr[1:-1, 2:] - bdr[1:-1, :-2]) / 2.0 / p.dx
     gz = (bdr[2:, 1:-1] - bdr[:-2, 1:-1]) / 2.0 / p.dz
     grad[1:-1, 1:-1] = np.sqrt(gx ** 2 + gz ** 2)
+    if per:
+        grad = _ghosts(grad, True)
     # viscosity of the phase field (phi in [-1,1]); solids keep eta of the fluid
     eta = 1.0 + (lambda_eta - 1.0) * (1.0 + ph) / 2.0
     rho = np.ones_like(ph)
@@ -164,7 +196,7 @@ def derived_fields(p: abi.Params, phi: np.ndarray, bdr: np.ndarray, lambda_eta: 
     nv = abi.field_shape(p, abi.F_V)
     return {
         abi.F_U: np.zeros(nu), abi.F_V: np.zeros(nv), abi.F_P: np.zeros_like(ph),
-        abi.F_PH: ph, abi.F_RMU: rmu, abi.F_RHO: rho, abi.F_ETA: _neumann_ghosts(eta),
+        abi.F_PH: ph, abi.F_RMU: rmu, abi.F_RHO: rho, abi.F_ETA: _ghosts(eta, per),
         abi.F_BDR_PH: bdr, abi.F_BDR_PH_USED: 1.0 - bdr, abi.F_BDR_PH_PENALTY: chi,
         abi.F_BDR_PH_GRAD: grad, abi.F_PH_M_INNER: 1.0 - bdr,
         abi.F_PH_M_0: 1.0 - bdr, abi.F_ANGLEB: ang, abi.F_ANGLET: ang.copy(),
@@ -176,9 +208,13 @@ def derived_fields(p: abi.Params, phi: np.ndarray, bdr: np.ndarray, lambda_eta: 
 def two_phase_config(nx: int, ny: int, seed: int, *, n_drops: int, r_drop: tuple,
                      n_discs: int = 0, r_disc: tuple = (10, 16), serpentine: bool = False,
                      lambda_eta: float = 10.0, penalty: float = 1e4, dt: float = 1e-4,
-                     tol: float = 1e-10, steps: int = 10, name: str = "custom") -> Config:
-    """Two-phase CH + NS on an nx x ny grid of height 1 (dx = dz = 1/ny)."""
+                     tol: float = 1e-10, steps: int = 10, name: str = "custom",
+                     periodic: bool = False) -> Config:
+    """Two-phase CH + NS on an nx x ny grid of height 1 (dx = dz = 1/ny);
+    periodic: periodic in x (droplets and discs wrap across x = 0)."""
     p = abi.default_params(nx, ny)
+    if periodic:
+        p.lr_bd_type = abi.LR_PERIODIC
     p.dx = p.dz = 1.0 / ny
     p.dt = dt
     p.re, p.ca, p.xld = 1.0, 1.0, 1.0e-3
@@ -429,6 +465,7 @@ def config(name: str) -> Config:
     c0     256 x 128, 6 droplets r in [8,20] cells, 3 discs r in [10,16],
            eta ratio 10, R = 1e4, dt = 1e-4, tol 1e-10 (Neumann-x twin of the
            periodic case: the reference's TP path has no periodic x).
+    c0p    the same, periodic in x (the workload's own boundary).
     c1     2048 x 1024 serpentine channel (0.5 + 0.3 sin(8 pi x), width 0.25),
            40 droplets r in [10,40], eta ratio 100, R = 1e6, dt = 5e-5, 20 steps.
     small  64 x 32 version of c0 for parity tests.
@@ -437,6 +474,12 @@ def config(name: str) -> Config:
     if name == "c0":
         return two_phase_config(256, 128, 1000, n_drops=6, r_drop=(8, 20), n_discs=3,
                                 lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=50, name=name)
+    if name == "c0p":
+        return two_phase_config(256, 128, 1000, n_drops=6, r_drop=(8, 20), n_discs=3,
+                                lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=50, name=name, periodic=True)
+    if name == "smallp":
+        return two_phase_config(64, 32, 7, n_drops=3, r_drop=(3, 6), n_discs=2, r_disc=(3, 5),
+                                lambda_eta=10.0, penalty=1e4, dt=1e-4, steps=10, name=name, periodic=True)
     if name == "c1":
         return two_phase_config(2048, 1024, 1001, n_drops=40, r_drop=(10, 40), serpentine=True,
                                 lambda_eta=100.0, penalty=1e6, dt=5e-5, steps=20, name=name)

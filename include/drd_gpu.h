@@ -73,7 +73,15 @@ typedef struct drd_params {
   int32_t nx, ny;          /* nr, nz                        TP/global.f90:15-16 */
   int32_t variant;         /* DRD_VARIANT_*                                      */
   int32_t phi_mode;        /* 1: phi in [0,1]; 2: phi in [-1,1] TP/mcl6.f90     */
-  int32_t lr_bd_type;      /* 'n' (Neumann) or 'd' (Dirichlet inflow)            */
+  int32_t lr_bd_type;      /* 'n' (Neumann), 'd' (Dirichlet inflow) or 'p'
+                            * (periodic x: no TP counterpart; packing as
+                            * AM/matrix_tool.f90:81-139, pressure by a real
+                            * FFT in x and REDFT10/01 in y as TP/CFT.f90:4-57;
+                            * u has nx unknowns per row; every field keeps
+                            * periodic ghost columns, which drd_set_field
+                            * fills; TP variant, power-of-two nx, one
+                            * instance, the Jacobi / identity / phi-mu block
+                            * preconditioners)                                 */
   int32_t velo_prec;       /* DRD_PREC_*                                         */
   int32_t maxit_velo1;     /* 800   TP/VeloSolve_combined.f90:417                */
   int32_t maxit_velo2;     /* 2000  TP/VeloSolve_2_combined.f90:413              */

@@ -68,8 +68,36 @@ struct oc_ctx {
 static int64_t ncell(const drd_params* p) { return (int64_t)(p->nx + 2) * (p->ny + 2); }
 static int64_t nufield(const drd_params* p) { return (int64_t)(p->nx + 3) * (p->ny + 2); }
 static int64_t nvfield(const drd_params* p) { return (int64_t)(p->nx + 2) * (p->ny + 1); }
+/* Periodic x (lr_bd_type 'p'), an addition without a TP counterpart (TP's
+ * sides are Neumann or a Dirichlet inflow; the periodic packing follows
+ * AM/matrix_tool.f90:81-139): a u row has nx unknowns (u(nx) = u(0)), and
+ * every ghost column, wall-ghost rows included, holds the periodic copy of
+ * an interior column: f(-1) = f(nx-1), f(nx) = f(0), and for u f(nx+1) =
+ * f(1). */
+static int per_x(const drd_params* p) { return p->lr_bd_type == 'p'; }
+static int nxu(const drd_params* p) { return p->nx + (per_x(p) ? 0 : 1); }
 static int64_t nuv(const drd_params* p) {
-  return (int64_t)(p->nx + 1) * p->ny + (int64_t)p->nx * (p->ny - 1);
+  return (int64_t)nxu(p) * p->ny + (int64_t)p->nx * (p->ny - 1);
+}
+
+static void xghost_c(const oc_ctx* c, double* f) {
+  for (int j = -1; j <= NY; ++j) {
+    f[CI(-1, j)] = f[CI(NX - 1, j)];
+    f[CI(NX, j)] = f[CI(0, j)];
+  }
+}
+static void xghost_u(const oc_ctx* c, double* f) {
+  for (int j = -1; j <= NY; ++j) {
+    f[UI(-1, j)] = f[UI(NX - 1, j)];
+    f[UI(NX, j)] = f[UI(0, j)];
+    f[UI(NX + 1, j)] = f[UI(1, j)];
+  }
+}
+static void xghost_v(const oc_ctx* c, double* f) {
+  for (int j = 0; j <= NY; ++j) {
+    f[VI(-1, j)] = f[VI(NX - 1, j)];
+    f[VI(NX, j)] = f[VI(0, j)];
+  }
 }
 
 int64_t oc_field_size(const drd_params* p, int32_t f) {
@@ -128,6 +156,7 @@ void oc_params_default(drd_params* p, int32_t nx, int32_t ny) {
 
 int oc_init(const drd_params* prm, oc_ctx** out) {
   if (!prm || !out || prm->nx < 2 || prm->ny < 2) return DRD_E_ARG;
+  if (per_x(prm) && (prm->variant != DRD_VARIANT_TP || (prm->nx & (prm->nx - 1)) != 0)) return DRD_E_UNSUPPORTED;
   oc_ctx* c = (oc_ctx*)calloc(1, sizeof(oc_ctx));
   c->p = *prm;
   for (int f = 0; f < DRD_F_COUNT; ++f)
@@ -160,6 +189,18 @@ int oc_init(const drd_params* prm, oc_ctx** out) {
     double t1 = PI_G * j / 2.0 / ny;
     double s = sin(t1);
     fmy[j] = 4.0 * (s * s) / dz2;
+  }
+  if (per_x(p)) {
+    /* periodic x: the 5-point eigenvalue 4 sin^2(pi kx / nx) / dr2 of the
+     * Fourier mode kx, stored at the halfcomplex index m of its cosine
+     * (m = kx) and sine (m = nx - kx) parts; the mean mode as Pre_Lcc(0,0) */
+    for (int m = 0; m < nx; ++m) {
+      const int kx = m <= nx / 2 ? m : nx - m;
+      double t1 = PI_G * kx / nx;
+      double sn = sin(t1);
+      fmx[m] = 4.0 * (sn * sn) / dr2;
+    }
+    fmx[nx] = 0.0;
   }
   for (int j = 0; j <= ny; ++j)
     for (int i = 0; i <= nx; ++i) c->pre_lcc[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
@@ -247,6 +288,18 @@ int oc_init(const drd_params* prm, oc_ctx** out) {
 const double* oc_esolp_table(oc_ctx* c, int32_t which) {
   const double* t[6] = {c->ua, c->ub, c->uc, c->va, c->vb, c->vc};
   return (which >= 0 && which < 6) ? t[which] : NULL;
+}
+
+/* Periodic x: a field written from outside takes the periodic ghost columns
+ * (the device library applies the same rule to drd_set_field). */
+int oc_fill_ghosts(oc_ctx* c, int32_t f) {
+  if (!c || f < 0 || f >= DRD_F_COUNT) return DRD_E_ARG;
+  if (!per_x(&c->p)) return 0;
+  const int64_t n = oc_field_size(&c->p, f);
+  if (n == nufield(&c->p)) xghost_u(c, c->f[f]);
+  else if (n == nvfield(&c->p)) xghost_v(c, c->f[f]);
+  else if (n == ncell(&c->p)) xghost_c(c, c->f[f]);
+  return 0;
 }
 
 int oc_finalize(oc_ctx* c) {
@@ -341,42 +394,50 @@ static void daxpy(int64_t n, double a, const double* x, double* y) {
 
 /* ---- packers: Atob_uv / btoA_uv  TP/matrix_tool.f90:5-75 -------------- */
 static void atob_uv(const oc_ctx* c, const double* f1, const double* f2, double* b) {
-  int nx = NX, ny = NY;
+  int nx = NX, ny = NY, nu = nxu(&c->p);
   const int big = (int64_t)nx * ny > PAR_MIN;
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
-    for (int i = 0; i <= nx; ++i) b[i + (size_t)(nx + 1) * j] = f1[UI(i, j)];
-  size_t mn = (size_t)(nx + 1) * ny;
+    for (int i = 0; i < nu; ++i) b[i + (size_t)nu * j] = f1[UI(i, j)];
+  size_t mn = (size_t)nu * ny;
   OMP_FOR(big)
   for (int j = 1; j < ny; ++j)
     for (int i = 0; i < nx; ++i) b[i + (size_t)nx * (j - 1) + mn] = f2[VI(i, j)];
 }
 
 static void btoa_uv(const oc_ctx* c, double* f1, double* f2, const double* b) {
-  int nx = NX, ny = NY;
+  int nx = NX, ny = NY, nu = nxu(&c->p);
   const int big = (int64_t)nx * ny > PAR_MIN;
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
-    for (int i = 0; i <= nx; ++i) f1[UI(i, j)] = b[i + (size_t)(nx + 1) * j];
-  size_t mn = (size_t)(nx + 1) * ny;
+    for (int i = 0; i < nu; ++i) f1[UI(i, j)] = b[i + (size_t)nu * j];
+  size_t mn = (size_t)nu * ny;
   OMP_FOR(big)
   for (int j = 1; j < ny; ++j)
     for (int i = 0; i < nx; ++i) f2[VI(i, j)] = b[i + (size_t)nx * (j - 1) + mn];
-  for (int i = 0; i <= nx; ++i) {
+  for (int i = 0; i < nu; ++i) {
     f1[UI(i, -1)] = -f1[UI(i, 0)];
     f1[UI(i, ny)] = -f1[UI(i, ny - 1)];
   }
-  for (int j = -1; j <= ny; ++j) {
-    f1[UI(-1, j)] = f1[UI(1, j)];
-    f1[UI(nx + 1, j)] = f1[UI(nx - 1, j)];
+  if (per_x(&c->p)) {
+    xghost_u(c, f1);
+  } else {
+    for (int j = -1; j <= ny; ++j) {
+      f1[UI(-1, j)] = f1[UI(1, j)];
+      f1[UI(nx + 1, j)] = f1[UI(nx - 1, j)];
+    }
   }
   for (int i = 0; i < nx; ++i) {
     f2[VI(i, 0)] = 0;
     f2[VI(i, ny)] = 0;
   }
-  for (int j = 0; j <= ny; ++j) {
-    f2[VI(-1, j)] = f2[VI(0, j)];
-    f2[VI(nx, j)] = f2[VI(nx - 1, j)];
+  if (per_x(&c->p)) {
+    xghost_v(c, f2);
+  } else {
+    for (int j = 0; j <= ny; ++j) {
+      f2[VI(-1, j)] = f2[VI(0, j)];
+      f2[VI(nx, j)] = f2[VI(nx - 1, j)];
+    }
   }
 }
 
@@ -407,6 +468,11 @@ static void btoa_ph_mu(const oc_ctx* c, double* f1, double* f2, const double* b)
     f2[CI(i, -1)] = f2[CI(i, 0)];
     f2[CI(i, ny)] = f2[CI(i, ny - 1)];
   }
+  if (per_x(&c->p)) {
+    xghost_c(c, f1);
+    xghost_c(c, f2);
+    return;
+  }
   for (int j = -1; j <= ny; ++j) {
     f1[CI(-1, j)] = f1[CI(0, j)];
     f1[CI(nx, j)] = f1[CI(nx - 1, j)];
@@ -426,8 +492,10 @@ static void btoa_c(const oc_ctx* c, double* f, const double* b) {
 }
 
 /* op_double_gradient_Neum TP/matrix_tool.f90:178-219 */
-void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
-                                const double* rmu, const double* mo, double* rhs) {
+/* per: periodic in x (the interior expression with the neighbour index
+ * wrapped), Neumann in y as the reference */
+static void dgn_x(int32_t nx, int32_t ny, double dx2, double dy2, const double* rmu, const double* mo,
+                  double* rhs, int per) {
 #define GI(i, k) ((size_t)((i) + 1) + (size_t)(nx + 2) * (size_t)((k) + 1))
   size_t n = (size_t)(nx + 2) * (ny + 2);
   for (size_t q = 0; q < n; ++q) rhs[q] = 0.0;
@@ -435,7 +503,13 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       double moL, moR;
-      if (i == 0) {
+      if (per) {
+        const int il = i == 0 ? nx - 1 : i - 1, ir = i == nx - 1 ? 0 : i + 1;
+        moL = (mo[GI(il, j)] + mo[GI(i, j)]) / 2.0;
+        moR = (mo[GI(i, j)] + mo[GI(ir, j)]) / 2.0;
+        rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dx2 * (moL * rmu[GI(il, j)] + moR * rmu[GI(ir, j)] -
+                                                      (moL + moR) * rmu[GI(i, j)]);
+      } else if (i == 0) {
         moL = (mo[GI(i, j)] + mo[GI(i, j)]) / 2.0;
         moR = (mo[GI(i, j)] + mo[GI(i + 1, j)]) / 2.0;
         rhs[GI(i, j)] = rhs[GI(i, j)] + 1.0 / dx2 * (moL * rmu[GI(i, j)] + moR * rmu[GI(i + 1, j)] -
@@ -470,6 +544,10 @@ void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
     }
 #undef GI
 }
+void oc_op_double_gradient_neum(int32_t nx, int32_t ny, double dx2, double dy2,
+                                const double* rmu, const double* mo, double* rhs) {
+  dgn_x(nx, ny, dx2, dy2, rmu, mo, rhs, 0);
+}
 
 /* ---- velocity operator ------------------------------------------------
  * MatrixVector_uv_CG  TP/VeloSolve_combined.f90:537-675   (bdf = 1)
@@ -490,7 +568,9 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
   const double* chi = is_mf(p) ? c->f[DRD_F_BDR_PH] : c->f[DRD_F_BDR_PH_PENALTY];
   double dt = p->dt, re = p->re, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
   btoa_uv(c, f1, f2, xx);
-  if (lr_dirichlet(p)) {
+  if (per_x(p)) {
+    /* btoa_uv left the periodic side copies */
+  } else if (lr_dirichlet(p)) {
     for (int k = 0; k < ny; ++k) {
       f1[UI(-1, k)] = -f1[UI(1, k)];
       f1[UI(nx + 1, k)] = -f1[UI(nx - 1, k)];
@@ -501,10 +581,11 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
       f1[UI(nx + 1, k)] = f1[UI(nx - 1, k)];
     }
   }
-  for (int k = 0; k <= ny; ++k) {
-    f2[VI(-1, k)] = f2[VI(0, k)];
-    f2[VI(nx, k)] = f2[VI(nx - 1, k)];
-  }
+  if (!per_x(p))
+    for (int k = 0; k <= ny; ++k) {
+      f2[VI(-1, k)] = f2[VI(0, k)];
+      f2[VI(nx, k)] = f2[VI(nx - 1, k)];
+    }
   for (int i = -1; i <= nx + 1; ++i) {
     f1[UI(i, -1)] = -f1[UI(i, 0)];
     f1[UI(i, ny)] = -f1[UI(i, ny - 1)];
@@ -515,9 +596,10 @@ static void velo_matvec(oc_ctx* c, int bdf, double alpha, const double* xx, doub
   }
   const int big = (int64_t)nx * ny > PAR_MIN;
   /* u rows */
+  const int nu = nxu(p);
   OMP_FOR(big)
   for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i) {
+    for (int i = 0; i < nu; ++i) {
       double r;
       if (bdf == 2)
         r = f1[UI(i, k)] * 1.5 * re * (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0 +
@@ -568,8 +650,9 @@ static void velo_jacobi_m(oc_ctx* c, double mcoef, double* yy, const double* xx)
   const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA];
   double dt = p->dt, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_uv(c, f1, f2, xx);
+  const int nu = nxu(p), per = per_x(p);
   for (int j = 0; j < ny; ++j)
-    for (int i = 0; i <= nx; ++i) {
+    for (int i = 0; i < nu; ++i) {
       double tmp = mcoef * (rho[CI(i, j)] + rho[CI(i - 1, j)]) / 2.0;
       double cr = eta[CI(i, j)], cl = eta[CI(i - 1, j)];
       tmp = tmp + 2.0 * dt * (cl + cr) / dr2;
@@ -590,9 +673,9 @@ static void velo_jacobi_m(oc_ctx* c, double mcoef, double* yy, const double* xx)
       tmp = tmp + 2.0 * dt * (cl + cr) / dz2;
       cr = (eta[CI(i + 1, j)] + eta[CI(i, j)] + eta[CI(i + 1, j - 1)] + eta[CI(i, j - 1)]) / 4.0;
       cl = (eta[CI(i, j)] + eta[CI(i - 1, j)] + eta[CI(i, j - 1)] + eta[CI(i - 1, j - 1)]) / 4.0;
-      if (i == 0)
+      if (!per && i == 0)
         tmp = tmp + dt * cr / dr2;
-      else if (i == nx - 1)
+      else if (!per && i == nx - 1)
         tmp = tmp + dt * cl / dr2;
       else
         tmp = tmp + dt * (cl + cr) / dr2;
@@ -611,7 +694,7 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
   const double* bu = c->f[DRD_F_BDR_PH_USED];
   double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_ph_mu(c, f1, f2, xx);
-  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f1, c->f[DRD_F_PH_M_INNER], ff1);
+  dgn_x(nx, ny, dr2, dz2, f1, c->f[DRD_F_PH_M_INNER], ff1, per_x(p));
   const int big = (int64_t)nx * ny > PAR_MIN;
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
@@ -623,7 +706,7 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
     j = ny - 1;
     ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j + 1)] - c->coef_top * f1[CI(i, j)]);
   }
-  oc_op_double_gradient_neum(nx, ny, dr2, dz2, f2, c->f[DRD_F_PH_M_0], ff2);
+  dgn_x(nx, ny, dr2, dz2, f2, c->f[DRD_F_PH_M_0], ff2, per_x(p));
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i)
@@ -636,10 +719,13 @@ static void phmu_matvec(oc_ctx* c, double alpha, const double* xx, double beta, 
 }
 
 /* PSolve_PH_MU_CG (Jacobi) TP/ph_mu_solve.f90:304-356.  The x test reads
- * `j == nr-1` and the z test `i == 0` as in the reference (:323,:328).  */
+ * `j == nr-1` and the z test `i == 0` as in the reference (:323,:328).
+ * Periodic x (no reference counterpart): the unit-mobility diagonal, 2/dr2
+ * in x everywhere and 1/dz2 on the wall rows j = 0, ny-1.  */
 static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
+  const int per = per_x(p);
   double *f1 = c->w1, *f2 = c->w2, *ff1 = c->w3, *ff2 = c->w4;
   double I = p->I_thickness, s = p->s_implicit, dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
   btoa_ph_mu(c, f1, f2, xx);
@@ -647,15 +733,15 @@ static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
-      double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
-      double tmpz = (i == 0 || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
+      double tmpx = (!per && (i == 0 || j == nx - 1)) ? 1.0 / dr2 : 2.0 / dr2;
+      double tmpz = ((per ? j == 0 : i == 0) || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
       ff1[CI(i, j)] = f1[CI(i, j)] / ((tmpx + tmpz) * I + s / I);
     }
   OMP_FOR(big)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
-      double tmpx = (i == 0 || j == nx - 1) ? 1.0 / dr2 : 2.0 / dr2;
-      double tmpz = (i == 0 || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
+      double tmpx = (!per && (i == 0 || j == nx - 1)) ? 1.0 / dr2 : 2.0 / dr2;
+      double tmpz = ((per ? j == 0 : i == 0) || j == ny - 1) ? 1.0 / dz2 : 2.0 / dz2;
       ff2[CI(i, j)] = f2[CI(i, j)] / (p->dt * p->xld * (tmpx + tmpz));
     }
   atob_ph_mu(c, ff1, ff2, yy);
@@ -668,9 +754,10 @@ static void phmu_jacobi(oc_ctx* c, double* yy, const double* xx) {
 static double dgn_diag(const oc_ctx* c, double dx2, double dy2, const double* mo, int i, int j) {
   int nx = NX, ny = NY;
   const double m0 = mo[CI(i, j)];
+  const int per = per_x(&c->p), il = i == 0 ? nx - 1 : i - 1, ir = i == nx - 1 ? 0 : i + 1;
   double dx = 0.0, dy = 0.0;
-  if (i > 0) dx += (mo[CI(i - 1, j)] + m0) / 2.0;
-  if (i < nx - 1) dx += (m0 + mo[CI(i + 1, j)]) / 2.0;
+  if (per || i > 0) dx += (mo[CI(il, j)] + m0) / 2.0;
+  if (per || i < nx - 1) dx += (m0 + mo[CI(ir, j)]) / 2.0;
   if (j > 0) dy += (mo[CI(i, j - 1)] + m0) / 2.0;
   if (j < ny - 1) dy += (m0 + mo[CI(i, j + 1)]) / 2.0;
   return -(dx / dx2 + dy / dy2);
@@ -920,6 +1007,79 @@ static void dct_2d_fast(int nx, int ny, double* a, const double* pre, int mode) 
   line_plan_free(&px);
   line_plan_free(&py);
 }
+/* Real DFT of x[0..L) (L a power of two) in FFTW's halfcomplex order
+ * (R2HC): r_k = Re X_k for k <= L/2 and r_{L-k} = Im X_k for 0 < k < L/2,
+ * X_k = sum_j x_j exp(-2 pi i jk / L), as the complex Stockham FFT of
+ * (x, 0). */
+static void line_r2hc(const line_plan* p, double* x, cpx* A, cpx* B) {
+  const int N = p->L;
+  for (int j = 0; j < N; ++j) {
+    A[j].x = x[j];
+    A[j].y = 0.0;
+  }
+  const cpx* V = stockham(A, B, N, p->tw, -1);
+  for (int k = 0; k <= N / 2; ++k) x[k] = V[k].x;
+  for (int k = 1; k < N / 2; ++k) x[N - k] = V[k].y;
+}
+/* HC2R, the unnormalised inverse: x_j = sum_k X_k exp(+2 pi i jk / L) of
+ * the Hermitian X the halfcomplex r holds */
+static void line_hc2r(const line_plan* p, double* r, cpx* A, cpx* B) {
+  const int N = p->L, h = N / 2;
+  A[0].x = r[0];
+  A[0].y = 0.0;
+  for (int k = 1; k < h; ++k) {
+    A[k].x = r[k];
+    A[k].y = r[N - k];
+    A[N - k].x = r[k];
+    A[N - k].y = -r[N - k];
+  }
+  A[h].x = r[h];
+  A[h].y = 0.0;
+  const cpx* v = stockham(A, B, N, p->tw, +1);
+  for (int j = 0; j < N; ++j) r[j] = v[j].x;
+}
+/* The pressure solve of a periodic-x, Neumann-y box: R2HC along x, REDFT10
+ * along y, / pre[m + (nx+1) ky], REDFT01 along y, HC2R along x; the result
+ * is 2 nx ny times the solution (the periodic counterpart of
+ * TP/dct_fftw.f90:13-31; nx a power of two). */
+static void per_solve_2d(int nx, int ny, double* a, const double* pre) {
+  line_plan px, py;
+  line_plan_init(&px, nx);
+  line_plan_init(&py, ny);
+  int mx = nx > ny ? nx : ny;
+  cpx* A = (cpx*)malloc((size_t)mx * sizeof(cpx));
+  cpx* B = (cpx*)malloc((size_t)mx * sizeof(cpx));
+  double* col = (double*)malloc((size_t)ny * sizeof(double));
+  for (int k = 0; k < ny; ++k) line_r2hc(&px, a + (size_t)nx * k, A, B);
+  for (int i = 0; i < nx; ++i) {
+    for (int k = 0; k < ny; ++k) col[k] = a[i + (size_t)nx * k];
+    line_dct2(&py, col, A, B);
+    for (int k = 0; k < ny; ++k) col[k] = col[k] / pre[i + (size_t)(nx + 1) * k];
+    line_dct3(&py, col, A, B);
+    for (int k = 0; k < ny; ++k) a[i + (size_t)nx * k] = col[k];
+  }
+  for (int k = 0; k < ny; ++k) line_hc2r(&px, a + (size_t)nx * k, A, B);
+  free(A);
+  free(B);
+  free(col);
+  line_plan_free(&px);
+  line_plan_free(&py);
+}
+/* exported for tests: the 1-D halfcomplex transforms (dir 0 R2HC, 1 HC2R) */
+void oc_rfft_line(int32_t n, double* x, int32_t dir) {
+  line_plan pl;
+  line_plan_init(&pl, n);
+  cpx* A = (cpx*)malloc((size_t)n * sizeof(cpx));
+  cpx* B = (cpx*)malloc((size_t)n * sizeof(cpx));
+  if (dir == 0)
+    line_r2hc(&pl, x, A, B);
+  else
+    line_hc2r(&pl, x, A, B);
+  free(A);
+  free(B);
+  line_plan_free(&pl);
+}
+
 /* exported for tests: mode 0 = REDFT10 2-D, 1 = REDFT01 2-D (y then x) */
 void oc_dct_2d_fast(int32_t nx, int32_t ny, double* a, int32_t mode) { dct_2d_fast(nx, ny, a, NULL, mode); }
 
@@ -1083,6 +1243,7 @@ static void velo_esolp(oc_ctx* c, double* yy, const double* xx) {
 int oc_psolve(oc_ctx* c, int32_t prec, int32_t op, double* x_out, const double* b_in) {
   int64_t n = op_size(c, op);
   if (n < 0) return DRD_E_ARG;
+  if (per_x(&c->p) && (prec == DRD_PS_VELO_ESOLP || prec == DRD_PS_PH_DCT)) return DRD_E_UNSUPPORTED;
   switch (prec) {
     case DRD_PS_VELO_ESOLP: velo_esolp(c, x_out, b_in); return 0;
     case DRD_PS_VELO1_JACOBI: velo_jacobi(c, x_out, b_in); return 0;
@@ -1349,7 +1510,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
                 double* rhs) {
   const drd_params* p = &c->p;
   if (p->xlsB >= 1.0e-8) return DRD_E_UNSUPPORTED; /* GNBC branch is commented out */
-  int nx = NX, ny = NY;
+  int nx = NX, ny = NY, nu_ = nxu(p);
   const double *u = c->f[DRD_F_U], *v = c->f[DRD_F_V], *pp = c->f[DRD_F_P];
   const double *rho = c->f[DRD_F_RHO], *eta = c->f[DRD_F_ETA], *bu = c->f[DRD_F_BDR_PH_USED];
   double dt = p->dt, re = p->re, ca = p->ca, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
@@ -1360,7 +1521,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
   if (is_mf(p)) { /* MF/VeloSolve_2_combined.f90:64-65, :78-79: the jump of phi, times 1 - mean bdr_ph */
     const double* bd = c->f[DRD_F_BDR_PH];
     for (int k = 0; k < ny; ++k)
-      for (int i = 0; i <= nx; ++i)
+      for (int i = 0; i < nu_; ++i)
         fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
                        (ph_after[CI(i, k)] - ph_after[CI(i - 1, k)]) / dr * (1.0 - (bd[CI(i, k)] + bd[CI(i - 1, k)]) / 2.0);
     for (int k = 1; k < ny; ++k)
@@ -1369,7 +1530,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
                        (ph_after[CI(i, k)] - ph_after[CI(i, k - 1)]) / dz * (1.0 - (bd[CI(i, k)] + bd[CI(i, k - 1)]) / 2.0);
   } else {
     for (int k = 0; k < ny; ++k)
-      for (int i = 0; i <= nx; ++i)
+      for (int i = 0; i < nu_; ++i)
         fu[UI(i, k)] = ca * (rmu_after[CI(i, k)] + rmu_after[CI(i - 1, k)]) / 2.0 *
                        (ph_after[CI(i, k)] * bu[CI(i, k)] - ph_after[CI(i - 1, k)] * bu[CI(i - 1, k)]) / dr;
     for (int k = 1; k < ny; ++k)
@@ -1386,7 +1547,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     memcpy(vt, v, (size_t)nvfield(p) * sizeof(double));
   }
   for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i)
+    for (int i = 0; i < nu_; ++i)
       fu[UI(i, k)] = fu[UI(i, k)] - re * (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0 *
                                          (ut[UI(i, k)] * (ut[UI(i + 1, k)] - ut[UI(i - 1, k)]) / 2.0 / dr +
                                           (vt[VI(i, k)] + vt[VI(i - 1, k)] + vt[VI(i, k + 1)] + vt[VI(i - 1, k + 1)]) / 4.0 *
@@ -1404,7 +1565,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
   else
     memcpy(tp, pp, (size_t)nc * sizeof(double));
   for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i) {
+    for (int i = 0; i < nu_; ++i) {
       double rb = (rho[CI(i, k)] + rho[CI(i - 1, k)]) / 2.0;
       double in;
       if (bdf == 2 && is_mf(p)) /* MF/VeloSolve_2_combined.f90:201: plain ggx */
@@ -1431,12 +1592,12 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     }
   /* particle force :240-250 */
   for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i) fu[UI(i, k)] = fu[UI(i, k)] + (pf1[CI(i, k)] + pf1[CI(i - 1, k)]) / 2.0;
+    for (int i = 0; i < nu_; ++i) fu[UI(i, k)] = fu[UI(i, k)] + (pf1[CI(i, k)] + pf1[CI(i - 1, k)]) / 2.0;
   for (int k = 1; k < ny; ++k)
     for (int i = 0; i < nx; ++i) fv[VI(i, k)] = fv[VI(i, k)] + (pf2[CI(i, k)] + pf2[CI(i, k - 1)]) / 2.0;
   /* times dt :253-263 */
   for (int k = 0; k < ny; ++k)
-    for (int i = 0; i <= nx; ++i) fu[UI(i, k)] = fu[UI(i, k)] * dt;
+    for (int i = 0; i < nu_; ++i) fu[UI(i, k)] = fu[UI(i, k)] * dt;
   for (int k = 1; k < ny; ++k)
     for (int i = 0; i < nx; ++i) fv[VI(i, k)] = fv[VI(i, k)] * dt;
   /* Dirichlet inflow :272-310 (always in MF, MF/VeloSolve_2_combined.f90:261-295) */
@@ -1459,7 +1620,7 @@ int oc_velo_rhs(oc_ctx* c, int32_t bdf, const double* psi1, const double* psi2,
     }
   }
   /* wall speed terms :392-400 */
-  for (int i = 0; i <= nx; ++i) {
+  for (int i = 0; i < nu_; ++i) {
     int k = 0;
     double etay = (eta[CI(i, k)] + eta[CI(i - 1, k)] + eta[CI(i, k - 1)] + eta[CI(i - 1, k - 1)]) / 4.0;
     fu[UI(i, k)] = fu[UI(i, k)] + dt * (2.0 * p->uwB / dz2 * etay);
@@ -1476,7 +1637,9 @@ static void velo_unpack(oc_ctx* c, const double* x, double* u_after, double* v_a
   const drd_params* p = &c->p;
   int nx = NX, ny = NY;
   btoa_uv(c, u_after, v_after, x);
-  if (lr_dirichlet(p)) {
+  if (per_x(p)) {
+    /* btoa_uv left the periodic side copies */
+  } else if (lr_dirichlet(p)) {
     for (int k = 0; k < ny; ++k) u_after[UI(-1, k)] = 2.0 * c->uL[k] - u_after[UI(1, k)];
     for (int k = 0; k < ny; ++k) u_after[UI(nx + 1, k)] = 2.0 * c->uR[k] - u_after[UI(nx - 1, k)];
   } else {
@@ -1493,10 +1656,11 @@ static void velo_unpack(oc_ctx* c, const double* x, double* u_after, double* v_a
     v_after[VI(i, 0)] = 0.0;
     v_after[VI(i, ny)] = 0.0;
   }
-  for (int k = 1; k < ny; ++k) {
-    v_after[VI(-1, k)] = v_after[VI(0, k)];
-    v_after[VI(nx, k)] = v_after[VI(nx - 1, k)];
-  }
+  if (!per_x(p))
+    for (int k = 1; k < ny; ++k) {
+      v_after[VI(-1, k)] = v_after[VI(0, k)];
+      v_after[VI(nx, k)] = v_after[VI(nx - 1, k)];
+    }
 }
 
 static int velo_prec_id(const oc_ctx* c, int bdf) {
@@ -1583,13 +1747,17 @@ int oc_presolve_2(oc_ctx* c, double* psi, double coef, double* mean_div) {
   if (mean_div) *mean_div = sum / nx / ny;
   /* dct_fftw(nr,nz,plan,plan_back,uin,psi,Pre_Lcc,4.0*nr*nz): REDFT10,
    * rhs / Pre_Lcc, REDFT01 (dct_fftw.f90:13-31) */
-  dct_2d_fast(nx, ny, uin, c->pre_lcc, 2);
-  double sc = 4.0 * nx * ny;
+  const int per = per_x(&c->p);
+  if (per)
+    per_solve_2d(nx, ny, uin, c->pre_lcc);
+  else
+    dct_2d_fast(nx, ny, uin, c->pre_lcc, 2);
+  double sc = per ? 2.0 * nx * ny : 4.0 * nx * ny;
   for (int k = 0; k < ny; ++k)
     for (int i = 0; i < nx; ++i) psi[CI(i, k)] = uin[i + (size_t)nx * k] / sc;
   for (int k = 0; k < ny; ++k) {
-    psi[CI(-1, k)] = psi[CI(0, k)];
-    psi[CI(nx, k)] = psi[CI(nx - 1, k)];
+    psi[CI(-1, k)] = per ? psi[CI(nx - 1, k)] : psi[CI(0, k)];
+    psi[CI(nx, k)] = per ? psi[CI(0, k)] : psi[CI(nx - 1, k)];
   }
   for (int i = -1; i <= nx; ++i) {
     psi[CI(i, -1)] = psi[CI(i, 0)];
@@ -1646,9 +1814,10 @@ int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const do
     }
   for (int64_t q = 0; q < nc; ++q) fpd[q] = (fpd[q] + 1.0) / 2.0;
   for (int64_t q = 0; q < nc; ++q) fpd[q] = fpd[q] + bdr[q];
+  const int per = per_x(p);
   for (int k = 0; k < ny; ++k) {
-    fpd[CI(-1, k)] = fpd[CI(0, k)];
-    fpd[CI(nx, k)] = fpd[CI(nx - 1, k)];
+    fpd[CI(-1, k)] = per ? fpd[CI(nx - 1, k)] : fpd[CI(0, k)];
+    fpd[CI(nx, k)] = per ? fpd[CI(0, k)] : fpd[CI(nx - 1, k)];
   }
   for (int i = -1; i <= nx; ++i) {
     fpd[CI(i, -1)] = fpd[CI(i, 0)];
@@ -1706,6 +1875,10 @@ int oc_particle_force(oc_ctx* c, const double* bx, const double* by, const doubl
         pf1[CI(i, k)] = pf1[CI(i, k)] + (0.0 + fx[b] * w[CI(i, k)]) / sum_val;
         pf2[CI(i, k)] = pf2[CI(i, k)] + (0.0 + fy[b] * w[CI(i, k)]) / sum_val;
       }
+  }
+  if (per_x(p)) { /* periodic x: the ghost columns the momentum rows read */
+    xghost_c(c, pf1);
+    xghost_c(c, pf2);
   }
   free(w);
   return 0;
@@ -1831,6 +2004,7 @@ static int ph_mu_solve_x(oc_ctx* c, double* ph_after, double* rmu_after, drd_sol
     ph_after[CI(i, -1)] = ph_after[CI(i, 0)] * c->coef_bot + pn[i];
     ph_after[CI(i, ny)] = ph_after[CI(i, ny - 1)] * c->coef_top + pn[nx + i];
   }
+  if (per_x(&c->p)) xghost_c(c, ph_after); /* corners: copies of the wall ghosts */
   free(rhs);
   free(x);
   free(pn);

@@ -25,6 +25,9 @@ int oc_init(const drd_params* prm, oc_ctx** out);
 int oc_finalize(oc_ctx* c);
 /* direct pointer to a module-`global` field (host memory owned by c)     */
 double* oc_field(oc_ctx* c, int32_t field);
+/* periodic x (lr_bd_type 'p'): the periodic ghost columns of a field written
+ * through oc_field (no-op otherwise) */
+int oc_fill_ghosts(oc_ctx* c, int32_t field);
 int64_t oc_field_size(const drd_params* p, int32_t field);
 /* eigenvalue tables of setup (TP/setup.f90:108-155)                      */
 const double* oc_pre_lcc(oc_ctx* c); /* (0:nx,0:ny) */
@@ -109,6 +112,8 @@ double oc_pairwise(int64_t n, const double* x, const double* y);
 /* FFT-factored 2-D DCT of the pressure solve: mode 0 REDFT10 (x then y),
  * 1 REDFT01 (y then x) */
 void oc_dct_2d_fast(int32_t nx, int32_t ny, double* a, int32_t mode);
+/* FFTW R2HC (dir 0) / HC2R (dir 1, unnormalised) of one line, n a power of two */
+void oc_rfft_line(int32_t n, double* x, int32_t dir);
 double oc_ch_bulk_energy(double phi, double psi, int32_t mode);
 double oc_ch_bulk_energy_mf(double phi, double psi);
 double oc_ch_bdr_condition(double angle, double phi, int32_t mode);

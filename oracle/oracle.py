@@ -40,6 +40,8 @@ def lib():
         L.oc_get_threads.restype = C.c_int32
         L.oc_pairwise.restype = C.c_double
         L.oc_dct_2d_fast.argtypes = [C.c_int32, C.c_int32, P, C.c_int32]
+        L.oc_rfft_line.argtypes = [C.c_int32, P, C.c_int32]
+        L.oc_fill_ghosts.argtypes = [C.c_void_p, C.c_int32]
         L.oc_finalize.argtypes = [C.c_void_p]
         L.oc_field.argtypes = [C.c_void_p, C.c_int32]
         L.oc_field.restype = P
@@ -127,6 +129,7 @@ class Oracle:
 
     def set_field(self, fid: int, a: np.ndarray) -> None:
         self.field(fid)[...] = a
+        lib().oc_fill_ghosts(self._h, fid)  # periodic x: the periodic ghost columns
 
     def get_field(self, fid: int) -> np.ndarray:
         return self.field(fid).copy()
@@ -300,6 +303,13 @@ def dct_2d_fast(a, mode: int):
     a = np.array(a, dtype=np.float64, order="C")
     ny, nx = a.shape
     lib().oc_dct_2d_fast(nx, ny, abi.dptr(a), mode)
+    return a
+
+
+def rfft_line(x, direction):
+    """FFTW's R2HC (direction 0) or unnormalised HC2R (1) of one line."""
+    a = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib().oc_rfft_line(len(a), abi.dptr(a), direction)
     return a
 
 

@@ -13,10 +13,12 @@ STATE = [abi.F_U, abi.F_V, abi.F_P, abi.F_PH, abi.F_RMU, abi.F_U_PRE, abi.F_V_PR
          abi.F_RMU_PRE, abi.F_PSI1, abi.F_PSI2, abi.F_PFORCE1, abi.F_PFORCE2]
 
 
-def pack_uv(u, v):
+def pack_uv(u, v, periodic=False):
+    """Atob_uv: the u block (nx + 1 columns, or nx with periodic x), then v."""
     ny = u.shape[0] - 2
     nx = u.shape[1] - 3
-    return np.concatenate([u[1:ny + 1, 1:nx + 2].ravel(), v[1:ny, 1:nx + 1].ravel()])
+    nxu = nx if periodic else nx + 1
+    return np.concatenate([u[1:ny + 1, 1:nxu + 1].ravel(), v[1:ny, 1:nx + 1].ravel()])
 
 
 def pack_phmu(ph, mu):
@@ -30,9 +32,11 @@ def cell_div(p, u, v):
 
 def neumann_lap(p, psi):
     """5-point Laplacian on the cell interior with the ghosts copying their
-    interior neighbour (psi(-1) = psi(0), ...), built here from the stencil."""
+    interior neighbour (psi(-1) = psi(0), ...), built here from the stencil;
+    with periodic x the x neighbours wrap (psi(-1) = psi(nx-1))."""
     a = psi[1:-1, 1:-1]
-    g = np.pad(a, 1, mode="edge")
+    g = np.pad(a, ((1, 1), (0, 0)), mode="edge")
+    g = np.pad(g, ((0, 0), (1, 1)), mode="wrap" if abi.periodic_x(p) else "edge")
     return (g[1:-1, 2:] - 2 * a + g[1:-1, :-2]) / p.dx ** 2 + (g[2:, 1:-1] - 2 * a + g[:-2, 1:-1]) / p.dz ** 2
 
 
@@ -72,7 +76,7 @@ def sub_solve_residuals(o, p, bdf, t, before, after, shifted=False):
     if bdf == 2:
         kw.update(psi1=before[abi.F_PSI1], psi2=before[abi.F_PSI2], u_pre=before[abi.F_U_PRE],
                   v_pre=before[abi.F_V_PRE])
-    xv = pack_uv(after[abi.F_U], after[abi.F_V])
+    xv = pack_uv(after[abi.F_U], after[abi.F_V], abi.periodic_x(p))
     bv = o.velo_rhs(bdf, **kw)
     r_v = rel_resid(o, op, xv, bv)
     out = {"ch": r_ch, "velo": r_v, "bn_ch": bn_ch, "bn_velo": float(np.linalg.norm(bv))}
