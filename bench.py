@@ -119,8 +119,8 @@ def cpu_baseline(cfg, velo_iters: float, phmu_iters: float):
         o.close()
         step_s = tfix + velo_iters * tv + phmu_iters * tc
         res[nt] = (1.0 / step_s if step_s > 0 else None, tv, tc, itv, itc, tfix)
-    # c0 whole steps
-    c0 = geometry.config("c0")
+    # c0 whole steps (the periodic workload at its own boundary)
+    c0 = geometry.config("c0p")
     c0.params.ch_prec = abi.CH_PREC_BLOCK
     whole = {}
     for nt in sorted({1, nthreads}):
@@ -151,8 +151,8 @@ def cpu_baseline(cfg, velo_iters: float, phmu_iters: float):
                    f"{v1[1]*1e3:.1f} and {v1[2]*1e3:.1f} ms), scaled by the GPU drop-in run's "
                    f"{velo_iters:.1f} + {phmu_iters:.1f} iterations per step"),
         "c0_whole_steps": {
-            "workload": "c0 256x128, BDF1 + 2 BDF2 steps end to end (RHS, CH, momentum, pressure), reference "
-                        "velocity preconditioners, block CH preconditioner, tol 1e-10",
+            "workload": "c0 256x128 periodic in x, BDF1 + 2 BDF2 steps end to end (RHS, CH, momentum, "
+                        "pressure), reference velocity preconditioners, block CH preconditioner, tol 1e-10",
             "steps_per_s": whole[nthreads]["steps_per_s"], "steps_per_s_1thread": whole[1]["steps_per_s"],
             "iters": whole[nthreads]["iters"]},
     }
@@ -389,6 +389,41 @@ def particle_coupling(cfg_name: str = "c1", nbody: int = 400, reps: int = 10, cp
                              "body at every cell (no cutoff), the device skips cells beyond a body's tanh cutoff, "
                              "so the ratio mixes that pruning with the hardware")
     return out
+
+
+def periodic_c0(steps: int = 20, warmup: int = 2):
+    """The periodic workload (256 x 128, periodic in x, 6 droplets, 3 discs,
+    eta ratio 10, R 1e4, dt 1e-4, tol 1e-10): BDF2 steps on the GPU with the
+    reference's velocity preconditioners (Jacobi BDF1, identity BDF2) and the
+    phi/mu block preconditioner, the ones periodic x takes."""
+    from drd_application_amd import abi, geometry
+    from drd_application_amd.solver import GpuSolver
+
+    cfg = geometry.config("c0p")
+    p = cfg.params.copy()
+    p.ch_prec = abi.CH_PREC_BLOCK
+    g = GpuSolver(p)
+    geometry.load(g, cfg)
+    t = 0.0
+    for s in range(warmup):
+        g.step(1 if s == 0 else 2, t)
+        t += p.dt
+    g.synchronize()
+    its, infos = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = g.step(2, t)
+        t += p.dt
+        its.append((info["phmu"]["iter"], info["velo"]["iter"]))
+        infos.append((info["phmu"]["info"], info["velo"]["info"]))
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.close()
+    log(f"[c0p] {steps} steps in {el:.3f} s ({steps/el:.1f} steps/s); iters {its[:4]} ...")
+    return {"workload": "c0 256x128 periodic in x, reference velocity preconditioners, block CH, tol 1e-10",
+            "steps": steps, "warmup": warmup, "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps,
+            "phmu_iters": sum(a for a, _ in its) / steps, "velo_iters": sum(b for _, b in its) / steps,
+            "converged": all(a == 0 and b == 0 for a, b in infos)}
 
 
 def combined_c4(steps: int = 20, warmup: int = 2):
@@ -636,6 +671,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-c2", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle coupling reductions")
     ap.add_argument("--no-c4", action="store_true", help="skip the full combined 4096x2048 workload")
+    ap.add_argument("--no-c0", action="store_true", help="skip the periodic 256x128 workload")
     ap.add_argument("--no-graphs", action="store_true",
                     help="skip the batched workload's run with captured graphs (for profiled runs)")
     ap.add_argument("--profile-network", action="store_true",
@@ -717,6 +753,12 @@ def main(argv=None) -> int:
             c4 = combined_c4()
         except Exception as e:  # reported beside the metric, never required for it
             log(f"combined workload failed: {e}")
+    c0p = None
+    if not args.no_c0:
+        try:
+            c0p = periodic_c0()
+        except Exception as e:  # reported beside the metric, never required for it
+            log(f"periodic workload failed: {e}")
     c3 = None
     if not args.no_c3:
         try:
@@ -765,6 +807,7 @@ def main(argv=None) -> int:
         "learned_c3": c3,
         "batched_c2": c2,
         "combined_c4": c4,
+        "periodic_c0": c0p,
         "particle_coupling": pc,
     }
     print(json.dumps(result))

@@ -32,7 +32,7 @@
  * when the grid is large enough to pay for it; every element's arithmetic
  * and every sum's order are those of the serial code, so results do not
  * depend on the thread count. */
-#define PAR_MIN 262144
+#define PAR_MIN 16384
 #define OMP_STR(x) #x
 #define OMP_FOR(cond) _Pragma(OMP_STR(omp parallel for schedule(static) if(cond)))
 
