@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cstdio>
 #include <cstring>
@@ -170,7 +171,17 @@ struct drd_ctx {
   float* rb_c32 = nullptr;    // FP32 bu, M_inner, M_0 (nc each)
   double cosa = 0, sqrt2 = 0;
   Learn* learn = nullptr;  // drd_learn_init
+  // drd_bicgstab_cb: the caller's MATVEC / PSOLVE, its vector length, and
+  // pinned host copies of an operand and a result (nmax each, lazy)
+  drd_matvec_cb ext_mv = nullptr;
+  drd_psolve_cb ext_ps = nullptr;
+  int64_t ext_n = 0;
+  double *ext_h0 = nullptr, *ext_h1 = nullptr;
 };
+
+// operator / preconditioner ids of drd_bicgstab_cb (internal: a caller's own
+// MATVEC and PSOLVE, called on the host)
+constexpr int kOpExt = 100, kPsExt = 100;
 
 BatchCfg& drd::batch_cfg() {
   static thread_local BatchCfg cfg;
@@ -460,6 +471,9 @@ static void drop_graphs(drd_ctx* c) {
 
 static void ctx_free(drd_ctx* c) {
   drop_graphs(c);
+  if (c->ext_h0) hipHostFree(c->ext_h0);
+  if (c->ext_h1) hipHostFree(c->ext_h1);
+  c->ext_h0 = c->ext_h1 = nullptr;
   for (auto e : c->tm_ev) hipEventDestroy(e);
   c->tm_ev.clear();
   hipFree(c->slab);
@@ -702,6 +716,8 @@ static int64_t op_size(const drd_ctx* c, int op) {
       return 2 * c->g.ncell();
     case DRD_OP_PH:
       return c->g.ncell();
+    case kOpExt:
+      return c->ext_n;
   }
   return -1;
 }
@@ -721,6 +737,63 @@ static int aux_ready(drd_ctx* c) {
   RCHK(dalloc(&c->ph_f2, g.nc()));
   RCHK(dalloc(&c->ph_t, g.ncell()));
   RCHK(dalloc(&c->aux_work, std::max<int64_t>(2 * g.ncell(), 3 * g.nuv())));
+  return 0;
+}
+
+static int read_state(drd_ctx* c);
+
+// drd_bicgstab_cb's operator: the caller's MATVEC(ALPHA, X, BETA, Y) on host
+// copies, called as LIB/BiCGSTABREVCOM.f calls it (r0 = -1 A x0 + 1 b; the
+// iteration's products with beta 0 into a zeroed vector), then the fused
+// modes' reductions on the device.  A fused mode first reads the solve's
+// state (every queued kernel has run by then), and a finished solve or an
+// early exit calls no MATVEC, as the reference's loop would not.
+static int ext_host(drd_ctx* c) {
+  if (c->ext_h0) return 0;
+  HIPCHK(hipHostMalloc((void**)&c->ext_h0, (size_t)c->nmax * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&c->ext_h1, (size_t)c->nmax * sizeof(double), hipHostMallocDefault));
+  return 0;
+}
+static int ext_op(drd_ctx* c, int mode, const double* x, double* y, double alpha, double beta, double* a0,
+                  const double* b, KState* ks) {
+  if (!c->ext_mv) return set_err(DRD_E_ARG, "no MATVEC outside drd_bicgstab_cb");
+  const int64_t n = c->ext_n;
+  const size_t bytes = (size_t)n * sizeof(double);
+  RCHK(ext_host(c));
+  double *hx = c->ext_h0, *hy = c->ext_h1;
+  bool call = true;
+  if (mode != MV_PLAIN) {
+    RCHK(read_state(c));
+    const KState& st = *c->ks_host;
+    call = !st.done && !st.early;
+    if (call && mode == MV_INIT && !st.xnz) {  // x0 = 0: r0 = b (REVCOM:229-240)
+      launch_copy(c->stream, n, b, y);
+      call = false;
+    }
+  }
+  if (call) {
+    HIPCHK(hipMemcpyAsync(hx, x, bytes, hipMemcpyDeviceToHost, c->stream));
+    const double* yin = mode == MV_INIT ? b : (mode == MV_PLAIN && beta != 0.0 ? y : nullptr);
+    if (yin) HIPCHK(hipMemcpyAsync(hy, yin, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!yin) std::memset(hy, 0, bytes);
+    c->ext_mv(&alpha, hx, &beta, hy);
+    HIPCHK(hipMemcpyAsync(y, hy, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  if (mode != MV_PLAIN) launch_ext_mv(c->stream, mode, n, y, a0, c->part, ks);
+  return 0;
+}
+// drd_bicgstab_cb's preconditioner: the caller's PSOLVE(X, B) on host copies
+static int ext_prec(drd_ctx* c, const double* b, double* x) {
+  const size_t bytes = (size_t)c->ext_n * sizeof(double);
+  RCHK(ext_host(c));
+  RCHK(read_state(c));
+  const KState& st = *c->ks_host;
+  if (st.done || st.early) return 0;  // the queued kernels skip as well
+  HIPCHK(hipMemcpyAsync(c->ext_h1, b, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ext_ps(c->ext_h0, c->ext_h1);
+  HIPCHK(hipMemcpyAsync(x, c->ext_h0, bytes, hipMemcpyHostToDevice, c->stream));
   return 0;
 }
 
@@ -747,6 +820,9 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
     case DRD_OP_PHMU:
       launch_phmu_matvec(c->stream, mode, c->g, c->s, k, x, y, alpha, beta, a0, b, c->part, ks);
       break;
+    case kOpExt:
+      RCHK(ext_op(c, mode, x, y, alpha, beta, a0, b, ks));
+      break;
     default:
       return set_err(DRD_E_UNSUPPORTED, "operator not built");
   }
@@ -759,6 +835,7 @@ static PhmuCoef32 phmu_c32(const drd_ctx* c) {
 }
 
 static int prec_setup(drd_ctx* c, int prec) {
+  if (prec == kPsExt) return 0;
   if (c->g.px && !periodic_prec_ok(prec))
     return set_err(DRD_E_UNSUPPORTED, "periodic x takes the Jacobi, identity and phi/mu block preconditioners");
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, c->s.re);
@@ -863,6 +940,11 @@ static int learn_correct(drd_ctx* c, const double* b, double* z) {
 static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, const double** res,
                       const KState* ks) {
   switch (prec) {
+    case kPsExt:
+      if (op != kOpExt || !c->ext_ps) break;
+      RCHK(ext_prec(c, b, x));
+      *res = x;
+      return 0;
     case DRD_PS_VELO_LEARNED:
       if (op != DRD_OP_VELO2 || !c->vprec) break;
       if (!c->learn || c->learn->train_iters == 0)
@@ -1059,7 +1141,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   RCHK(apply_op(c, op, MV_INIT, x, R, -1.0, 1.0, RT, b, c->ks));
   // chunks of iterations are queued without host round trips; with graphs
   // on, a chunk of length >= 2 is one replay of a captured hipGraph
-  const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 &&
+  const bool graphable = c->use_graphs && nsnap == 0 && c->tm_which < 0 && op != kOpExt &&
                          (prec == DRD_PS_IDENTITY || prec == DRD_PS_VELO1_JACOBI || prec == DRD_PS_VELO2_JACOBI ||
                           prec == DRD_PS_PHMU_JACOBI ||
                           prec == DRD_PS_PHMU_BLOCK || prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 ||
@@ -1954,6 +2036,80 @@ extern "C" int drd_bicgstab(drd_ctx* c, int32_t op, int32_t prec, int32_t n, con
   RCHK(up(c, c->xv, x, n));
   drd_solve_info loc{};
   RCHK(run_bicgstab(c, op, prec, n, c->rhs, c->xv, *iter, *resid, &loc, false));
+  RCHK(down(c, x, c->xv, n));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *iter = loc.iter;
+  *resid = loc.resid;
+  *info = loc.info;
+  return 0;
+}
+
+// The context drd_bicgstab_cb uses when the caller gives none: Krylov
+// workspace for at least n unknowns (a square grid of side m, 2 m^2 >= n),
+// made on first use and remade larger when a longer system comes.
+static std::mutex g_vec_mu;
+static drd_ctx* g_vec_ctx = nullptr;
+static int vec_ctx(int64_t n, drd_ctx** out) {
+  if (g_vec_ctx && g_vec_ctx->nmax >= n) {
+    *out = g_vec_ctx;
+    return 0;
+  }
+  int m = 4;
+  while (2 * (int64_t)m * m < n) ++m;
+  if (m > 4096) return set_err(DRD_E_ARG, "system too long for one context (n > 2 * 4096^2)");
+  if (g_vec_ctx) {
+    drd_finalize(g_vec_ctx);
+    g_vec_ctx = nullptr;
+  }
+  drd_params p;
+  drd_params_default(&p, m, m);
+  RCHK(ctx_create(&p, 1, &g_vec_ctx));
+  *out = g_vec_ctx;
+  return 0;
+}
+
+extern "C" int drd_bicgstab_cb(drd_ctx* ctx, int32_t n, const double* b, double* x, int32_t ldw, int32_t* iter,
+                               double* resid, drd_matvec_cb matvec, drd_psolve_cb psolve, int32_t* info) {
+  if (!iter || !resid || !info) return set_err(DRD_E_ARG, "null argument");
+  *info = 0;
+  if (n < 0) {  // LIB/BiCGSTAB.f:155-163
+    *info = -1;
+    return 0;
+  }
+  if (ldw < std::max<int32_t>(1, n)) {
+    *info = -2;
+    return 0;
+  }
+  if (*iter <= 0) {
+    *info = -3;
+    return 0;
+  }
+  if (n == 0) {
+    *iter = 0;
+    *resid = 0.0;
+    return 0;
+  }
+  if (!b || !x || !matvec) return set_err(DRD_E_ARG, "null argument");
+  std::unique_lock<std::mutex> lock(g_vec_mu, std::defer_lock);
+  drd_ctx* c = ctx;
+  if (!c) {
+    lock.lock();
+    RCHK(vec_ctx(n, &c));
+  }
+  if (c->nb != 1) return set_err(DRD_E_ARG, "drd_bicgstab_cb takes a single-instance context");
+  if (n > c->nmax) return set_err(DRD_E_ARG, "n exceeds the context's Krylov workspace");
+  c->ext_n = n;
+  c->ext_mv = matvec;
+  c->ext_ps = psolve;
+  RCHK(up(c, c->rhs, b, n));
+  RCHK(up(c, c->xv, x, n));
+  drd_solve_info loc{};
+  const int rc = run_bicgstab(c, kOpExt, psolve ? kPsExt : DRD_PS_IDENTITY, n, c->rhs, c->xv, *iter, *resid, &loc,
+                              false);
+  c->ext_mv = nullptr;
+  c->ext_ps = nullptr;
+  c->ext_n = 0;
+  RCHK(rc);
   RCHK(down(c, x, c->xv, n));
   HIPCHK(hipStreamSynchronize(c->stream));
   *iter = loc.iter;

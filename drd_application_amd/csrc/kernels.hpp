@@ -337,6 +337,8 @@ void launch_init_norms(hipStream_t st, int64_t n, int nseg, const double* x, con
                        KState* ks);
 void launch_bicg_p(hipStream_t st, int64_t n, const double* r, const double* v, double* p, KState* ks);
 void launch_bicg_s(hipStream_t st, int64_t n, int nseg, double* r, const double* v, double* part, KState* ks);
+// the reductions of a fused matvec mode for a product y formed outside the library (one segment)
+void launch_ext_mv(hipStream_t st, int mode, int64_t n, const double* y, double* a0, double* part, KState* ks);
 void launch_bicg_xr(hipStream_t st, int64_t n, int nseg, double* x, double* r, const double* rt,
                     const double* ph, const double* sh, const double* t, double* part, KState* ks,
                     bool finish = true);

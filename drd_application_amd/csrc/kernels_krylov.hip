@@ -458,6 +458,45 @@ void launch_bicg_xr(hipStream_t s, int64_t n, int nseg, double* x, double* r, co
   DRD_SEG_LAUNCH(k_bicg_xr, n, nseg, x, r, rt, ph, sh, t, part, ks);
   if (finish) launch_finalize(s, part, tiles_of(n / nseg), nseg, 2, nullptr, ks, PH_XR, FIN_ACTIVE);
 }
+// The reductions a fused matvec sweep makes, for a product y = A x formed
+// elsewhere (a caller's MATVEC on the host, copied in): MV_INIT a0 := y and
+// (y, y); MV_DOT1 (a0, y); MV_DOT2 (y, a0), (y, y).  Same tile partials and
+// the same products as the fused sweeps, so the finish is theirs.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_ext_mv(int64_t n, const double* __restrict__ y, double* __restrict__ a0,
+                                                   double* part, KState* st) {
+  if (!kact(st)) return;
+  constexpr int K = MODE == MV_DOT2 ? 2 : 1;
+  double t[K][kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int64_t q = tile_elem(n, 0, j);
+    double z = 0.0, a = 0.0;
+    if (q >= 0) {
+      z = y[q];
+      if (MODE == MV_INIT)
+        a0[q] = z;
+      else
+        a = a0[q];
+    }
+    if (MODE == MV_INIT) t[0][j] = z * z;
+    if (MODE == MV_DOT1) t[0][j] = a * z;
+    if (MODE == MV_DOT2) {
+      t[0][j] = z * a;
+      t[1][j] = z * z;
+    }
+  }
+  tile_publish<1, K>(t, part);
+}
+
+void launch_ext_mv(hipStream_t s, int mode, int64_t n, const double* y, double* a0, double* part, KState* ks) {
+  const dim3 grid(tiles_of(n));
+  if (mode == MV_INIT) hipLaunchKernelGGL(k_ext_mv<MV_INIT>, grid, dim3(kBlock), 0, s, n, y, a0, part, ks);
+  if (mode == MV_DOT1) hipLaunchKernelGGL(k_ext_mv<MV_DOT1>, grid, dim3(kBlock), 0, s, n, y, a0, part, ks);
+  if (mode == MV_DOT2) hipLaunchKernelGGL(k_ext_mv<MV_DOT2>, grid, dim3(kBlock), 0, s, n, y, a0, part, ks);
+  launch_mv_finalize(s, mode, part, tiles_of(n), 1, ks);
+}
+
 void launch_copy(hipStream_t s, int64_t n, const double* a, double* b) {
   hipLaunchKernelGGL(k_copy, dim3(nb(n, kBlock), 1, bat_n()), dim3(kBlock), 0, s, n, a, b, bat_bs());
 }

@@ -1,8 +1,8 @@
 ! drd_gpu_c.f90 -- ISO_C_BINDING interfaces of the C-ABI (include/drd_gpu.h)
 ! that the Fortran drop-in shims call: drd_gpu_shim.f90 (the TP program,
-! two_phase_flow_complex_domain) and drd_gpu_shim_mf.f90 (the MF program,
-! microfluidics_complex_domain).  It uses no module `global`, so one build
-! serves both.
+! two_phase_flow_complex_domain), drd_gpu_shim_mf.f90 (the MF program,
+! microfluidics_complex_domain) and drd_gpu_bicgstab.f90 (BICGSTAB itself).
+! It uses no module `global`, so one build serves all of them.
 
 module drd_gpu_c
   use iso_c_binding
@@ -106,6 +106,18 @@ module drd_gpu_c
        real(c_double), value :: t
        type(drd_solve_info_t), intent(out) :: info
      end function drd_phsolve
+
+     integer(c_int) function drd_bicgstab_cb(ctx, n, b, x, ldw, iter, resid, matvec, psolve, info) &
+          bind(C, name='drd_bicgstab_cb')
+       import :: c_ptr, c_int, c_int32_t, c_double, c_funptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: n, ldw
+       real(c_double), intent(in) :: b(*)
+       real(c_double), intent(inout) :: x(*)
+       integer(c_int32_t), intent(inout) :: iter, info
+       real(c_double), intent(inout) :: resid
+       type(c_funptr), value :: matvec, psolve
+     end function drd_bicgstab_cb
 
      type(c_ptr) function drd_last_error() bind(C, name='drd_last_error')
        import :: c_ptr

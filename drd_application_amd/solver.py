@@ -32,6 +32,43 @@ class DrdError(RuntimeError):
     pass
 
 
+# MATVEC(ALPHA, X, BETA, Y) and PSOLVE(X, B) of LIB/BiCGSTAB.f, every argument by reference
+MATVEC_CB = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                        C.POINTER(C.c_double))
+PSOLVE_CB = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def bicgstab_cb(matvec, psolve, b, x0, maxit, tol, ldw=None):
+    """BICGSTAB with the caller's own operator (drd_bicgstab_cb, the
+    library's own context): matvec(alpha, x, beta, y) returns alpha A x +
+    beta y, psolve(b) returns M^-1 b (None: identity), on numpy vectors.
+    Returns (x, iter, resid, info, calls) with calls the (matvec, psolve)
+    call counts."""
+    b = _f64(b)
+    n = len(b)
+    x = _f64(x0).copy()
+    calls = [0, 0]
+
+    def mv(pa, px, pb, py):
+        calls[0] += 1
+        xa = np.ctypeslib.as_array(px, shape=(n,))
+        ya = np.ctypeslib.as_array(py, shape=(n,))
+        ya[...] = matvec(pa[0], xa.copy(), pb[0], ya.copy())
+
+    def ps(px, pb):
+        calls[1] += 1
+        np.ctypeslib.as_array(px, shape=(n,))[...] = psolve(np.ctypeslib.as_array(pb, shape=(n,)).copy())
+
+    fmv = MATVEC_CB(mv)
+    fps = PSOLVE_CB(ps) if psolve is not None else C.cast(None, PSOLVE_CB)
+    it = C.c_int32(maxit)
+    res = C.c_double(tol)
+    info = C.c_int32(0)
+    _check(lib().drd_bicgstab_cb(None, n, dp(b), dp(x), max(1, n) if ldw is None else ldw, C.byref(it), C.byref(res), fmv,
+                                 fps, C.byref(info)))
+    return x, it.value, res.value, info.value, tuple(calls)
+
+
 def lib():
     """Loads libdrd_gpu.so; raises if it has not been built."""
     global _LIB
@@ -62,6 +99,7 @@ def lib():
         L.drd_psolve.argtypes = [V, C.c_int32, C.c_int32, P, P]
         L.drd_bicgstab.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
         L.drd_fgmres.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
+        L.drd_bicgstab_cb.argtypes = [V, C.c_int32, P, P, C.c_int32, I, P, MATVEC_CB, PSOLVE_CB, I]
         L.drd_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_bench_kernel.argtypes = [V, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.drd_kernel_timer.argtypes = [V, C.c_int32, C.c_int32]

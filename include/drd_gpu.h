@@ -247,6 +247,26 @@ int drd_bicgstab(drd_ctx* ctx, int32_t op, int32_t prec, int32_t n,
                  const double* b, double* x, int32_t* iter, double* resid,
                  int32_t* info);
 
+/* BICGSTAB(N,B,X,WORK,LDW,ITER,RESID,MATVEC,PSOLVE,INFO) LIB/BiCGSTAB.f:2-22
+ * with the caller's own operator and preconditioner: MATVEC(ALPHA, X, BETA,
+ * Y) computes Y = ALPHA*A*X + BETA*Y and PSOLVE(X, B) X = M^-1 B, on host
+ * vectors, every argument by reference as a Fortran caller passes them.
+ * The Krylov vectors, updates, dot products and stopping rule run on the
+ * device (the state machine of drd_bicgstab, canonical reductions); each
+ * product and preconditioner application copies its operand to the host,
+ * calls the routine and copies the result back.  MATVEC is called as
+ * LIB/BiCGSTABREVCOM.f calls it (r0 with ALPHA = -1, BETA = 1 on b; the
+ * iteration's products with BETA = 0 into a zeroed Y), and neither routine
+ * is called after convergence or an early exit.  psolve NULL: identity.
+ * INFO -1, -2, -3 for N < 0, LDW < max(1, N), ITER <= 0, as the reference
+ * (nothing is called); N = 0 returns ITER = 0, RESID = 0.  ITER in: MAXIT,
+ * RESID in: the tolerance.  ctx NULL: the library's own context, sized for
+ * N (up to 2 * 4096^2); calls on it are serialised.                       */
+typedef void (*drd_matvec_cb)(const double* alpha, const double* x, const double* beta, double* y);
+typedef void (*drd_psolve_cb)(double* x, const double* b);
+int drd_bicgstab_cb(drd_ctx* ctx, int32_t n, const double* b, double* x, int32_t ldw, int32_t* iter,
+                    double* resid, drd_matvec_cb matvec, drd_psolve_cb psolve, int32_t* info);
+
 /* FGMRES(m) with right preconditioning (Saad 1993); no reference code
  * (LIB/fgmres.f wraps a closed library).  Same argument meaning as
  * drd_bicgstab; restart = m.                                               */

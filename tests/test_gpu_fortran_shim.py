@@ -14,6 +14,14 @@ p = psi2 + p themselves.
   outputs uL, uR the inflow profile at the last solve's time.
 * MF (module `global` of MF, 420 x 180): likewise against drd_step of the MF
   variant, uL, uR included.
+* The reference's own solve routines (TP/VeloSolve_combined.f90,
+  TP/VeloSolve_2_combined.f90, TP/ph_mu_solve.f90 and what they call,
+  compiled from their sources: right-hand sides, packers, operators and
+  preconditioners in Fortran) calling BICGSTAB of drd_gpu_bicgstab.f90 (the
+  Krylov loop on the GPU over their own MATVEC and PSOLVE), with the pressure
+  of drd_gpu_shim_base.f90: ten steps bit for bit the CPU oracle's, Neumann
+  sides and the Dirichlet inflow.  This pins the oracle's right-hand sides,
+  unpacking and inflow against the reference's code itself.
 
 The drivers (tests/fortran/build/) are built where the reference checkout
 provides the modules `global` (tests/fortran/Makefile).
@@ -29,6 +37,7 @@ from drd_application_amd import abi, geometry
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "fortran", "build", "shim_driver")
 DRIVER_MF = os.path.join(HERE, "fortran", "build", "shim_driver_mf")
+DRIVER_REF = os.path.join(HERE, "fortran", "build", "refsolve_driver")
 
 pytestmark = pytest.mark.gpu
 
@@ -123,6 +132,29 @@ def test_tp_dropin_ten_steps(tmp_path, lr):
         np.testing.assert_array_equal(uR, uL)
         assert np.abs(uL).max() > 0.5
     ref.close()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER_REF), reason="reference-solve driver not built")
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("lr", ["n", "d"])
+def test_reference_solves_on_the_gpu_bicgstab_ten_steps(tmp_path, lr):
+    from oracle.oracle import Oracle
+    cfg = _tp_case(lr)
+    p = cfg.params
+    cs, us, vs = abi.cell_shape(p), abi.field_shape(p, abi.F_U), abi.field_shape(p, abi.F_V)
+    parts, stdout = _run(DRIVER_REF, tmp_path, _scal(p), [cfg.fields[f] for f in TP_ORDER],
+                         [us, vs, cs, cs, cs, cs, cs, (p.ny,), (p.ny,)])
+    got = dict(zip(OUT, parts[:6]))
+    assert stdout.count("uv 2 BICGSTAB no slip") == STEPS - 1 and "uv 1 BICGSTAB no slip" in stdout
+    o = Oracle(p)
+    geometry.load(o, cfg)
+    _steps(o, p)
+    for fid in OUT:
+        np.testing.assert_array_equal(got[fid], o.get_field(fid), err_msg=abi.FIELD_NAMES[fid])
+    np.testing.assert_array_equal(parts[6], o.get_field(abi.F_PH_M_0))
+    if lr == "d":
+        np.testing.assert_allclose(parts[7], _inflow(p, STEPS * p.dt), rtol=1e-14, atol=1e-15)
+    o.close()
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER_MF), reason="MF Fortran shim driver not built")
