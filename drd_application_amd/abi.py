@@ -45,6 +45,7 @@ OP_VELO1 = 0
 OP_VELO2 = 1
 OP_PHMU = 2
 OP_PH = 3
+OP_PHORI = 4  # MATRIX_VECTOR_CG_ori TP/PhSolve_ori.f90:348-397
 
 PS_VELO1_JACOBI = 0
 PS_IDENTITY = 1
@@ -210,7 +211,7 @@ def op_size(p: Params, op: int) -> int:
         return n_uv(p)
     if op == OP_PHMU:
         return 2 * p.nx * p.ny
-    if op == OP_PH:
+    if op in (OP_PH, OP_PHORI):
         return p.nx * p.ny
     raise ValueError(op)
 

@@ -125,7 +125,8 @@ struct drd_ctx {
   int guess = 0;  // drd_set_initial_guess
   std::map<GraphKey, GraphEntry> graphs;
   int64_t graph_recaptures = 0;  // replays refused for a changed fingerprint
-  int pred_its[4] = {0, 0, 0, 0};  // per operator: iterations of its latest BiCGSTAB solve
+  int pred_its[5] = {0, 0, 0, 0, 0};  // per operator: iterations of its latest BiCGSTAB solve
+  double phori_cb = 1.0, phori_ct = 1.0;  // PhSolve_ori's wall coefficients (coef_bot, coef_top of its last step)
   // drd_kernel_timer: event pairs around the instrumented kernel's launches
   int tm_which = -1, tm_n = 0;
   std::vector<hipEvent_t> tm_ev;
@@ -284,6 +285,30 @@ static int dalloc(T** ptr, int64_t n) {
   return 0;
 }
 
+// Ph_Lcc TP/setup.f90:137-155 for the step dt, into the context's table
+static int build_ph_lcc(drd_ctx* c, double dt) {
+  const drd_params& p = c->p;
+  const int nx = p.nx, ny = p.ny;
+  const double dr2 = p.dx * p.dx, dz2 = p.dz * p.dz;
+  std::vector<double> fmx(nx + 1), fmy(ny + 1), ph((size_t)nx * ny);
+  for (int i = 0; i < nx; ++i) {
+    const double sn = std::sin(kPiG * i / 2.0 / (nx - 1));
+    fmx[i] = 4.0 * (sn * sn) / dr2;
+  }
+  for (int j = 0; j < ny; ++j) {
+    const double sn = std::sin(kPiG * j / 2.0 / (ny - 1));
+    fmy[j] = 4.0 * (sn * sn) / dz2;
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double temp = fmx[i] + fmy[j];
+      ph[i + (size_t)nx * j] =
+          1. + p.xld * dt * p.s_implicit * temp / p.I_thickness + p.xld * p.I_thickness * dt * (temp * temp);
+    }
+  HIPCHK(hipMemcpy(c->ph_lcc, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
 static int build_tables(drd_ctx* c) {
   const drd_params& p = c->p;
   const int nx = p.nx, ny = p.ny;
@@ -311,24 +336,10 @@ static int build_tables(drd_ctx* c) {
   for (int j = 0; j <= ny; ++j)
     for (int i = 0; i <= nx; ++i) pre[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
   pre[0] = -1.0;
-  for (int i = 0; i < nx; ++i) {
-    const double sn = std::sin(kPiG * i / 2.0 / (nx - 1));
-    fmx[i] = 4.0 * (sn * sn) / dr2;
-  }
-  for (int j = 0; j < ny; ++j) {
-    const double sn = std::sin(kPiG * j / 2.0 / (ny - 1));
-    fmy[j] = 4.0 * (sn * sn) / dz2;
-  }
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      const double temp = fmx[i] + fmy[j];
-      ph[i + (size_t)nx * j] =
-          1. + p.xld * p.dt * p.s_implicit * temp / p.I_thickness + p.xld * p.I_thickness * p.dt * (temp * temp);
-    }
   RCHK(dalloc(&c->pre_lcc, (int64_t)pre.size()));
   RCHK(dalloc(&c->ph_lcc, (int64_t)ph.size()));
   HIPCHK(hipMemcpy(c->pre_lcc, pre.data(), pre.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->ph_lcc, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice));
+  RCHK(build_ph_lcc(c, p.dt));
   // ESOLP tables TP/setup.f90:157-235 (BDF1 set).  t1 keeps the value of the
   // last loop iteration (:167, :203), (pi*nx/nx)/2 and (pi*(nx-1)/(nx-1))/2,
   // overwriting the mass shift of :159 as the reference does.
@@ -715,6 +726,7 @@ static int64_t op_size(const drd_ctx* c, int op) {
     case DRD_OP_PHMU:
       return 2 * c->g.ncell();
     case DRD_OP_PH:
+    case DRD_OP_PHORI:
       return c->g.ncell();
     case kOpExt:
       return c->ext_n;
@@ -822,6 +834,12 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
       break;
     case kOpExt:
       RCHK(ext_op(c, mode, x, y, alpha, beta, a0, b, ks));
+      break;
+    case DRD_OP_PHORI:
+      if (c->nb != 1) return set_err(DRD_E_UNSUPPORTED, "PhSolve_ori takes a single-instance context");
+      RCHK(aux_ready(c));
+      launch_phori_matvec(c->stream, mode, c->g, c->s, c->phori_cb, c->phori_ct, x, c->ph_t, y, alpha, beta, a0, b,
+                          c->part, ks);
       break;
     default:
       return set_err(DRD_E_UNSUPPORTED, "operator not built");
@@ -1008,8 +1026,8 @@ static int apply_prec(drd_ctx* c, int prec, int op, const double* b, double* x, 
       velo_mg_apply(c->stream, c->vmg1, b, x, ks);
       *res = x;
       return 0;
-    case DRD_PS_PH_DCT:
-      if (op != DRD_OP_PH) break;
+    case DRD_PS_PH_DCT:  // PSolve_CG MF/PhSolve.f90:296-313 = PSolve_CG_ori TP/PhSolve_ori.f90:402-416
+      if (op != DRD_OP_PH && op != DRD_OP_PHORI) break;
       RCHK(aux_ready(c));
       launch_ph_dct_prec(c->stream, c->g, c->d1_nx, c->d1_ny, c->ph_lcc, b, x, c->aux_work, ks);
       *res = x;
@@ -1153,7 +1171,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
   // later chunks grow 1, 2, 4, ... 32.  A chunk queued past convergence
   // costs one dispatch per kernel, so the tail a power-of-two schedule
   // leaves (up to half the solve) is what this saves.
-  const int pred = (op >= 0 && op < 4) ? c->pred_its[op] : 0;
+  const int pred = (op >= 0 && op < 5) ? c->pred_its[op] : 0;
   int chunk = pred > 2 ? std::min(pred - 1, maxit) : 1, after = 1, total = 0, next_snap = 0;
   const uint64_t print = graphable ? graph_print(c, prec) : 0;
   for (;;) {
@@ -1225,7 +1243,7 @@ static int run_bicgstab(drd_ctx* c, int op, int prec, int64_t n, const double* b
     c->last[i] = drd_solve_info{c->ks_host[i].iter, c->ks_host[i].info, c->ks_host[i].resid, 0.0};
     most = std::max(most, (int)c->ks_host[i].iter);
   }
-  if (op >= 0 && op < 4) c->pred_its[op] = most;
+  if (op >= 0 && op < 5) c->pred_its[op] = most;
   const KState& s = *c->ks_host;
   if (info) {
     info->iter = s.iter;
@@ -2041,6 +2059,67 @@ extern "C" int drd_bicgstab(drd_ctx* c, int32_t op, int32_t prec, int32_t n, con
   *iter = loc.iter;
   *resid = loc.resid;
   *info = loc.info;
+  return 0;
+}
+
+// PhSolve_ori TP/PhSolve_ori.f90:1-234 on the device fields, results into
+// ph_out, mu_out (device cell fields)
+static int phsolve_ori_dev(drd_ctx* c, double alpha_s, double* ph_out, double* mu_out, drd_solve_info* info) {
+  if (c->nb != 1) return set_err(DRD_E_UNSUPPORTED, "PhSolve_ori takes a single-instance context");
+  if (c->g.px) return set_err(DRD_E_UNSUPPORTED, "PhSolve_ori runs with Neumann sides");
+  RCHK(aux_ready(c));
+  const drd_params& p = c->p;
+  const double tmp1 = p.I_thickness / p.dz + 0.5 * alpha_s;  // :76-78, :114-116
+  c->phori_cb = c->phori_ct = (p.I_thickness / p.dz - 0.5 * alpha_s) / tmp1;
+  launch_phori_rhs(c->stream, c->g, c->s, c->f[DRD_F_PH], c->f[DRD_F_U], c->f[DRD_F_V], c->f[DRD_F_ANGLEB],
+                   c->f[DRD_F_ANGLET], alpha_s, c->rhs, c->xv, c->pn);
+  RCHK(run_bicgstab(c, DRD_OP_PHORI, DRD_PS_PH_DCT, c->g.ncell(), c->rhs, c->xv, 500, p.tol, info, true));  // :148
+  launch_phori_finish(c->stream, c->g, c->s, c->phori_cb, c->phori_ct, c->xv, c->pn, ph_out, mu_out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int drd_phsolve_ori(drd_ctx* c, double alpha_s, double* rmu_after, double* ph_after,
+                               drd_solve_info* info) {
+  if (!c || !rmu_after || !ph_after) return set_err(DRD_E_ARG, "null argument");
+  drd_solve_info loc{};
+  RCHK(phsolve_ori_dev(c, alpha_s, c->ph_after, c->rmu_after, &loc));
+  RCHK(down(c, ph_after, c->ph_after, c->g.nc()));
+  RCHK(down(c, rmu_after, c->rmu_after, c->g.nc()));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
+  return 0;
+}
+
+extern "C" int drd_reinit_solid(drd_ctx* c, int32_t steps, double dt, double alpha_s, drd_solve_info* info) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  if (steps < 0 || !(dt > 0.0)) return set_err(DRD_E_ARG, "steps >= 0 and dt > 0");
+  if (c->nb != 1) return set_err(DRD_E_UNSUPPORTED, "the re-initialisation takes a single-instance context");
+  if (c->g.px) return set_err(DRD_E_UNSUPPORTED, "the re-initialisation runs with Neumann sides");
+  const Grid& g = c->g;
+  launch_reinit_start(c->stream, g, c->f[DRD_F_BDR_PH], c->f[DRD_F_PH], c->f[DRD_F_PH_M_0],
+                      c->f[DRD_F_PH_M_INNER]);
+  const Scal s0 = c->s;
+  drd_params q = c->p;
+  q.dt = dt;
+  c->s = make_scal(q);
+  int rc = build_ph_lcc(c, dt);
+  drd_solve_info loc{};
+  for (int32_t k = 0; k < steps && !rc; ++k) {
+    rc = phsolve_ori_dev(c, alpha_s, c->ph_after, c->rmu_after, &loc);
+    if (!rc) {
+      launch_copy(c->stream, g.nc(), c->rmu_after, c->f[DRD_F_RMU]);
+      launch_copy(c->stream, g.nc(), c->ph_after, c->f[DRD_F_PH]);
+    }
+  }
+  c->s = s0;
+  const int rc2 = build_ph_lcc(c, c->p.dt);
+  RCHK(rc);
+  RCHK(rc2);
+  launch_reinit_end(c->stream, g, c->f[DRD_F_PH], c->f[DRD_F_BDR_PH], c->f[DRD_F_BDR_PH_PENALTY]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) *info = loc;
   return 0;
 }
 

@@ -395,6 +395,17 @@ void launch_ph_rhs(hipStream_t st, const Grid& g, const Scal& s, const double* p
 void launch_ph_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, const double* m0,
                       const double* minner, const double* x, double* t, double* y, double alpha, double beta,
                       double* a0, const double* b, double* part, KState* ks);
+// PhSolve_ori TP/PhSolve_ori.f90 and the solid re-initialisation (kernels_phori.hip)
+void launch_phori_rhs(hipStream_t st, const Grid& g, const Scal& s, const double* ph, const double* u,
+                      const double* v, const double* angB, const double* angT, double alpha_s, double* rhs,
+                      double* x0, double* pn);
+void launch_phori_matvec(hipStream_t st, int mode, const Grid& g, const Scal& s, double cbot, double ctop,
+                         const double* x, double* t, double* y, double alpha, double beta, double* a0,
+                         const double* b, double* part, KState* ks);
+void launch_phori_finish(hipStream_t st, const Grid& g, const Scal& s, double cbot, double ctop, const double* x,
+                         const double* pn, double* ph, double* mu);
+void launch_reinit_start(hipStream_t st, const Grid& g, const double* bdr, double* ph, double* m0, double* minner);
+void launch_reinit_end(hipStream_t st, const Grid& g, const double* ph, double* bdr, double* pen);
 void launch_ph_finish(hipStream_t st, const Grid& g, const Scal& s, const double* x, const double* f2,
                       const double* m0, double* mu, double* ph);
 // tabs: [ua|va], [ub|vb], [uc|vc], each table (1:n, 0:ld-1) column-major

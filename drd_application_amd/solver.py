@@ -100,6 +100,8 @@ def lib():
         L.drd_bicgstab.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
         L.drd_fgmres.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P, P, I, P, I]
         L.drd_bicgstab_cb.argtypes = [V, C.c_int32, P, P, C.c_int32, I, P, MATVEC_CB, PSOLVE_CB, I]
+        L.drd_phsolve_ori.argtypes = [V, C.c_double, P, P, C.POINTER(abi.SolveInfo)]
+        L.drd_reinit_solid.argtypes = [V, C.c_int32, C.c_double, C.c_double, C.POINTER(abi.SolveInfo)]
         L.drd_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_bench_kernel.argtypes = [V, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.drd_kernel_timer.argtypes = [V, C.c_int32, C.c_int32]
@@ -368,6 +370,20 @@ class GpuSolver:
         t = C.c_double(0.0)
         _check(lib().drd_read_snapshot(self._h, os.fsencode(path), f_field, f2_field, C.byref(t)))
         return t.value
+
+    def PhSolve_ori(self, alpha_s=0.0):
+        """PhSolve_ori TP/PhSolve_ori.f90:1-234 -> (rmu_after, ph_after, info)."""
+        cs = abi.cell_shape(self.p)
+        ph_a, mu_a = np.zeros(cs), np.zeros(cs)
+        info = abi.SolveInfo()
+        _check(lib().drd_phsolve_ori(self._h, alpha_s, dp(mu_a), dp(ph_a), C.byref(info)))
+        return mu_a, ph_a, info.as_dict()
+
+    def reinit_solid(self, steps=51, dt=1e-4, alpha_s=0.0):
+        """The CH re-initialisation of the solid domain MF/finit.f90:132-188."""
+        info = abi.SolveInfo()
+        _check(lib().drd_reinit_solid(self._h, steps, dt, alpha_s, C.byref(info)))
+        return info.as_dict()
 
     def set_initial_guess(self, mode):
         """0: BDF2 solves start from the previous step (the reference); 1: from 2x^n - x^{n-1}."""

@@ -216,6 +216,7 @@ int drd_phsolve(drd_ctx* ctx, double* rmu_after, double* ph_after, double t,
 #define DRD_OP_VELO2 1 /* MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:536-676 */
 #define DRD_OP_PHMU 2  /* MATRIX_VECTOR_PH_MU_CG TP/ph_mu_solve.f90:203-299        */
 #define DRD_OP_PH 3    /* MATRIX_VECTOR_CG     MF/PhSolve.f90:240-292              */
+#define DRD_OP_PHORI 4 /* MATRIX_VECTOR_CG_ori TP/PhSolve_ori.f90:348-397 (= MF)   */
 int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
                double beta, double* y);
 
@@ -239,6 +240,26 @@ int drd_matvec(drd_ctx* ctx, int32_t op, double alpha, const double* x,
                                  divisions); FP64 in and out (addition)      */
 int drd_psolve(drd_ctx* ctx, int32_t prec, int32_t op, double* x_out,
                const double* b_in);
+
+/* PhSolve_ori(rmu_after, ph_after, t) TP/PhSolve_ori.f90:1-234 (the same in
+ * MF/): one implicit CH step for phi with unit mobility (its operator is
+ * DRD_OP_PHORI, preconditioned by DRD_PS_PH_DCT) from the context's ph, u,
+ * v and contact angles; alpha_s is the wall term of module global
+ * (MF/finit.f90:374), wall types 'n', Neumann sides, at most 500
+ * iterations (:148).  Outputs are host cell fields.                        */
+int drd_phsolve_ori(drd_ctx* ctx, double alpha_s, double* rmu_after, double* ph_after,
+                    drd_solve_info* info);
+
+/* The CH re-initialisation of a solid domain, MF/finit.f90:132-188 (=
+ * TP/reinint_by_PhSolve.f90:1-91), on the context's fields: ph := bdr_ph
+ * (ghosts copied), ph_M_0 = ph_M_inner = 1, `steps` PhSolve_ori steps of
+ * size dt (the reference: 51 steps of 1e-4), then bdr_ph = (ph + 1) / 2
+ * (the interior clamped to [0, 1]) and bdr_ph_penalty = [bdr_ph > 0.5].
+ * ph and rmu keep the last step's state and the mobilities stay 1, as the
+ * reference leaves them; the context's dt (and its Ph_Lcc) is unchanged
+ * afterwards.  info: the last step's solve.  Single instance, Neumann
+ * sides (the reference forces lr_bd_type 'n' for it).                    */
+int drd_reinit_solid(drd_ctx* ctx, int32_t steps, double dt, double alpha_s, drd_solve_info* info);
 
 /* BICGSTAB(N,B,X,WORK,LDW,ITER,RESID,MATVEC,PSOLVE,INFO) LIB/BiCGSTAB.f:2-3
  * with the MATVEC/PSOLVE callbacks chosen by id.  *iter in: max

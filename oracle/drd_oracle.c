@@ -154,6 +154,33 @@ void oc_params_default(drd_params* p, int32_t nx, int32_t ny) {
   p->bdr_low = 0;
 }
 
+/* Ph_Lcc TP/setup.f90:137-155 for the step dt */
+static void build_ph_lcc(oc_ctx* c, double dt) {
+  const drd_params* p = &c->p;
+  const int nx = p->nx, ny = p->ny;
+  const double dr2 = p->dx * p->dx, dz2 = p->dz * p->dz;
+  double* fmx = (double*)calloc((size_t)nx + 1, sizeof(double));
+  double* fmy = (double*)calloc((size_t)ny + 1, sizeof(double));
+  for (int i = 0; i < nx; ++i) {
+    double t1 = PI_G * i / 2.0 / (nx - 1);
+    double s = sin(t1);
+    fmx[i] = 4.0 * (s * s) / dr2;
+  }
+  for (int j = 0; j < ny; ++j) {
+    double t1 = PI_G * j / 2.0 / (ny - 1);
+    double s = sin(t1);
+    fmy[j] = 4.0 * (s * s) / dz2;
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double temp = fmx[i] + fmy[j];
+      c->ph_lcc[i + (size_t)nx * j] = 1. + p->xld * dt * p->s_implicit * temp / p->I_thickness +
+                                       p->xld * p->I_thickness * dt * (temp * temp);
+    }
+  free(fmx);
+  free(fmy);
+}
+
 int oc_init(const drd_params* prm, oc_ctx** out) {
   if (!prm || !out || prm->nx < 2 || prm->ny < 2) return DRD_E_ARG;
   if (per_x(prm) && (prm->variant != DRD_VARIANT_TP || (prm->nx & (prm->nx - 1)) != 0)) return DRD_E_UNSUPPORTED;
@@ -205,22 +232,7 @@ int oc_init(const drd_params* prm, oc_ctx** out) {
   for (int j = 0; j <= ny; ++j)
     for (int i = 0; i <= nx; ++i) c->pre_lcc[i + (size_t)(nx + 1) * j] = -(fmx[i] + fmy[j]);
   c->pre_lcc[0] = -1.0;
-  for (int i = 0; i < nx; ++i) {
-    double t1 = PI_G * i / 2.0 / (nx - 1);
-    double s = sin(t1);
-    fmx[i] = 4.0 * (s * s) / dr2;
-  }
-  for (int j = 0; j < ny; ++j) {
-    double t1 = PI_G * j / 2.0 / (ny - 1);
-    double s = sin(t1);
-    fmy[j] = 4.0 * (s * s) / dz2;
-  }
-  for (int j = 0; j < ny; ++j)
-    for (int i = 0; i < nx; ++i) {
-      double temp = fmx[i] + fmy[j];
-      c->ph_lcc[i + (size_t)nx * j] = 1. + p->xld * p->dt * p->s_implicit * temp / p->I_thickness +
-                                       p->xld * p->I_thickness * p->dt * (temp * temp);
-    }
+  build_ph_lcc(c, p->dt);
   /* ESOLP tables TP/setup.f90:157-235 (BDF1 set).  The mass shift t1 set at
    * :159 is overwritten by the loops at :167 and :203, so the tables use the
    * last loop value, (pi*nx/nx)/2 and (pi*(nx-1)/(nx-1))/2. */
@@ -1206,12 +1218,14 @@ static void ph_psolve(oc_ctx* c, double* yy, const double* xx) {
   atob_c(c, f2, yy);
 }
 
+static void phori_matvec(oc_ctx* c, double alpha, const double* xx, double beta, double* yy);
 int oc_matvec(oc_ctx* c, int32_t op, double alpha, const double* x, double beta, double* y) {
   switch (op) {
     case DRD_OP_VELO1: velo_matvec(c, 1, alpha, x, beta, y); return 0;
     case DRD_OP_VELO2: velo_matvec(c, 2, alpha, x, beta, y); return 0;
     case DRD_OP_PHMU: phmu_matvec(c, alpha, x, beta, y); return 0;
     case DRD_OP_PH: ph_matvec(c, alpha, x, beta, y); return 0;
+    case DRD_OP_PHORI: phori_matvec(c, alpha, x, beta, y); return 0;
   }
   return DRD_E_ARG;
 }
@@ -1221,7 +1235,8 @@ static int64_t op_size(const oc_ctx* c, int32_t op) {
     case DRD_OP_VELO1:
     case DRD_OP_VELO2: return nuv(&c->p);
     case DRD_OP_PHMU: return 2 * (int64_t)c->p.nx * c->p.ny;
-    case DRD_OP_PH: return (int64_t)c->p.nx * c->p.ny;
+    case DRD_OP_PH:
+    case DRD_OP_PHORI: return (int64_t)c->p.nx * c->p.ny;
   }
   return -1;
 }
@@ -2077,6 +2092,206 @@ int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t, drd_sol
   }
   free(f1); free(f2); free(ff); free(rhs); free(x);
   return 0;
+}
+
+/* ---- PhSolve_ori and the CH re-initialisation of solid domains ---------- */
+/* laplace_nn TP/PhSolve_ori.f90:463-487: the 5-point Laplacian of the
+ * interior with the ghosts set to copies of their neighbour first (rows,
+ * then columns, corners included), as the reference sets them in a. */
+static void laplace_nn(const oc_ctx* c, double* a, double* b) {
+  const int nx = NX, ny = NY;
+  const double dr2 = c->p.dx * c->p.dx, dz2 = c->p.dz * c->p.dz;
+  for (int i = 0; i < nx; ++i) {
+    a[CI(i, -1)] = a[CI(i, 0)];
+    a[CI(i, ny)] = a[CI(i, ny - 1)];
+  }
+  for (int j = -1; j <= ny; ++j) {
+    a[CI(-1, j)] = a[CI(0, j)];
+    a[CI(nx, j)] = a[CI(nx - 1, j)];
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i)
+      b[CI(i, j)] = (a[CI(i + 1, j)] + a[CI(i - 1, j)] - 2 * a[CI(i, j)]) / dr2 +
+                    (a[CI(i, j + 1)] + a[CI(i, j - 1)] - 2 * a[CI(i, j)]) / dz2;
+}
+
+/* MATRIX_VECTOR_CG_ori TP/PhSolve_ori.f90:348-397 (MF/PhSolve_ori.f90 is
+ * the same): y = alpha (x - dt M lap(s/eps x - eps lap x + walls)) + beta y,
+ * the wall rows taking I/dz2 (x(i,-1) - coef_bot x(i,0)) with the ghost
+ * x(i,-1) = x(i,0) laplace_nn leaves.  Neumann sides (the re-initialisation
+ * runs with lr_bd_type 'n'). */
+static void phori_matvec(oc_ctx* c, double alpha, const double* xx, double beta, double* yy) {
+  const drd_params* p = &c->p;
+  const int nx = NX, ny = NY;
+  const double I = p->I_thickness, dz2 = p->dz * p->dz;
+  double *f1 = c->w1, *ff1 = c->w3, *ff2 = c->w4;
+  const int64_t nc = ncell(p);
+  for (int64_t q = 0; q < nc; ++q) f1[q] = ff1[q] = ff2[q] = 0.0;
+  btoa_c(c, f1, xx);
+  laplace_nn(c, f1, ff1);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) ff1[CI(i, j)] = -I * ff1[CI(i, j)];
+  for (int i = 0; i < nx; ++i) {
+    int j = 0;
+    ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j - 1)] - c->coef_bot * f1[CI(i, j)]);
+    j = ny - 1;
+    ff1[CI(i, j)] = ff1[CI(i, j)] + I / dz2 * (f1[CI(i, j + 1)] - c->coef_top * f1[CI(i, j)]);
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) ff1[CI(i, j)] = ff1[CI(i, j)] + f1[CI(i, j)] * p->s_implicit / I;
+  laplace_nn(c, ff1, ff2);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double z = f1[CI(i, j)] + -p->dt * p->xld * ff2[CI(i, j)];
+      const size_t q = i + (size_t)nx * j;
+      yy[q] = beta == 0.0 ? alpha * z : alpha * z + beta * yy[q];
+    }
+}
+
+/* PhSolve_ori TP/PhSolve_ori.f90:1-234: one implicit CH step for phi with
+ * unit mobility, wall ('n' type) boundary terms with alpha_s and the contact
+ * angles angleB/angleT, the DCT preconditioner of PhSolve (PSolve_CG_ori
+ * :402-416 = PSolve_CG), then mu from the new phi. */
+int oc_phsolve_ori(oc_ctx* c, double alpha_s, double* rmu_after, double* ph_after, drd_solve_info* info) {
+  const drd_params* p = &c->p;
+  if (per_x(p) || lr_dirichlet(p)) return DRD_E_UNSUPPORTED;
+  const int nx = NX, ny = NY;
+  const double *ph = c->f[DRD_F_PH], *u = c->f[DRD_F_U], *v = c->f[DRD_F_V];
+  const double *aB = c->f[DRD_F_ANGLEB], *aT = c->f[DRD_F_ANGLET];
+  const double dt = p->dt, dr = p->dx, dz = p->dz, dr2 = dr * dr, dz2 = dz * dz;
+  const double I = p->I_thickness, s = p->s_implicit;
+  const int64_t nc = ncell(p), n = (int64_t)nx * ny;
+  double* f1 = (double*)calloc((size_t)nc, sizeof(double));
+  double* f2 = (double*)calloc((size_t)nc, sizeof(double));
+  double* ff = (double*)calloc((size_t)nc, sizeof(double));
+  double* pnb = (double*)calloc((size_t)nx, sizeof(double));
+  double* pnt = (double*)calloc((size_t)nx, sizeof(double));
+  double* rhs = (double*)malloc((size_t)n * sizeof(double));
+  double* x = (double*)malloc((size_t)n * sizeof(double));
+  memcpy(f2, ph, (size_t)nc * sizeof(double));
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i)
+      f2[CI(i, k)] = f2[CI(i, k)] - dt * (u[UI(i + 1, k)] + u[UI(i, k)]) / 2.0 * (ph[CI(i + 1, k)] - ph[CI(i - 1, k)]) / 2.0 / dr -
+                     dt * (v[VI(i, k + 1)] + v[VI(i, k)]) / 2.0 * (ph[CI(i, k + 1)] - ph[CI(i, k - 1)]) / 2.0 / dz;
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      const double phv = ph[CI(i, k)];
+      f1[CI(i, k)] = ((1. + s) * phv - phv * phv * phv) / I;
+    }
+  for (int i = 0; i < nx; ++i) { /* bottom wall, 'n' type (:48-80) */
+    const double cf = sqrt(2.0) * cos(aB[i] * PI_G / 180.0) / 3.0 * PI_G / 2.;
+    const double tmp = (ph[CI(i, -1)] + ph[CI(i, 0)]) / 2.0;
+    pnb[i] = cf * cos(tmp * PI_G / 2) + alpha_s * tmp;
+    const double tmp1 = I / dz + 0.5 * alpha_s;
+    c->coef_bot = (I / dz - 0.5 * alpha_s) / tmp1;
+    pnb[i] = pnb[i] / tmp1;
+    f1[CI(i, 0)] = f1[CI(i, 0)] + pnb[i] * I / dz2;
+  }
+  for (int i = 0; i < nx; ++i) { /* top wall (:83-118) */
+    const double cf = sqrt(2.0) * cos(aT[i] * PI_G / 180.0) / 3.0 * PI_G / 2.;
+    const double tmp = (ph[CI(i, ny - 1)] + ph[CI(i, ny)]) / 2.0;
+    pnt[i] = cf * cos(tmp * PI_G / 2) + alpha_s * tmp;
+    const double tmp1 = I / dz + 0.5 * alpha_s;
+    c->coef_top = (I / dz - 0.5 * alpha_s) / tmp1;
+    pnt[i] = pnt[i] / tmp1;
+    f1[CI(i, ny - 1)] = f1[CI(i, ny - 1)] + pnt[i] * I / dz2;
+  }
+  laplace_nn(c, f1, ff);
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) f2[CI(i, k)] = f2[CI(i, k)] - dt * p->xld * ff[CI(i, k)];
+  atob_c(c, f2, rhs);
+  atob_c(c, ph, x);
+  int32_t it = 500, inf = 0; /* :148 */
+  double res = p->tol;
+  oc_bicgstab(c, DRD_OP_PHORI, DRD_PS_PH_DCT, (int32_t)n, rhs, x, &it, &res, &inf);
+  if (info) {
+    info->iter = it;
+    info->info = inf;
+    info->resid = res;
+    info->true_resid = true_resid(c, DRD_OP_PHORI, rhs, x);
+  }
+  for (int64_t q = 0; q < nc; ++q) ph_after[q] = 0.0;
+  btoa_c(c, ph_after, x);
+  for (int i = 0; i < nx; ++i) {
+    ph_after[CI(i, -1)] = ph_after[CI(i, 0)] * c->coef_bot + pnb[i];
+    ph_after[CI(i, ny)] = ph_after[CI(i, ny - 1)] * c->coef_top + pnt[i];
+  }
+  for (int k = -1; k <= ny; ++k) {
+    ph_after[CI(-1, k)] = ph_after[CI(0, k)];
+    ph_after[CI(nx, k)] = ph_after[CI(nx - 1, k)];
+  }
+  for (int k = 0; k < ny; ++k)
+    for (int i = 0; i < nx; ++i) {
+      const double a = ph_after[CI(i, k)];
+      rmu_after[CI(i, k)] = -I * ((ph_after[CI(i + 1, k)] + ph_after[CI(i - 1, k)] - 2 * a) / dr2 +
+                                  (ph_after[CI(i, k + 1)] + ph_after[CI(i, k - 1)] - 2 * a) / dz2) -
+                            a / I + a * a * a / I;
+    }
+  for (int i = 0; i < nx; ++i) {
+    rmu_after[CI(i, -1)] = rmu_after[CI(i, 0)];
+    rmu_after[CI(i, ny)] = rmu_after[CI(i, ny - 1)];
+  }
+  for (int k = -1; k <= ny; ++k) {
+    rmu_after[CI(-1, k)] = rmu_after[CI(0, k)];
+    rmu_after[CI(nx, k)] = rmu_after[CI(nx - 1, k)];
+  }
+  free(f1); free(f2); free(ff); free(pnb); free(pnt); free(rhs); free(x);
+  return 0;
+}
+
+/* The CH re-initialisation of a solid domain, MF/finit.f90:132-188 (=
+ * TP/reinint_by_PhSolve.f90:1-91): phi := bdr_ph with copied ghosts, unit
+ * mobilities, Ph_Lcc rebuilt for the step dt, `steps` PhSolve_ori steps
+ * (the reference takes 51 at dt = 1e-4), then bdr_ph = (phi + 1) / 2 (over
+ * the whole field; the interior clamped to [0, 1]) and bdr_ph_penalty =
+ * [bdr_ph > 0.5].  Left as the reference leaves them: phi and mu the last
+ * step's, ph_M_0 = ph_M_inner = 1; the context's own dt and Ph_Lcc are
+ * restored.  info: the last step's solve. */
+static void build_ph_lcc(oc_ctx* c, double dt);
+int oc_reinit_solid(oc_ctx* c, int32_t steps, double dt, double alpha_s, drd_solve_info* info) {
+  const drd_params* p = &c->p;
+  if (per_x(p) || lr_dirichlet(p)) return DRD_E_UNSUPPORTED;
+  const int nx = NX, ny = NY;
+  const int64_t nc = ncell(p);
+  double *ph = c->f[DRD_F_PH], *rmu = c->f[DRD_F_RMU], *bdr = c->f[DRD_F_BDR_PH];
+  memcpy(ph, bdr, (size_t)nc * sizeof(double));
+  for (int j = 0; j < ny; ++j) {
+    ph[CI(-1, j)] = ph[CI(0, j)];
+    ph[CI(nx, j)] = ph[CI(nx - 1, j)];
+  }
+  for (int i = -1; i <= nx; ++i) {
+    ph[CI(i, -1)] = ph[CI(i, 0)];
+    ph[CI(i, ny)] = ph[CI(i, ny - 1)];
+  }
+  for (int64_t q = 0; q < nc; ++q) c->f[DRD_F_PH_M_0][q] = c->f[DRD_F_PH_M_INNER][q] = 1.0;
+  const double dt0 = c->p.dt;
+  c->p.dt = dt;
+  build_ph_lcc(c, dt);
+  double* pa = (double*)calloc((size_t)nc, sizeof(double));
+  double* ma = (double*)calloc((size_t)nc, sizeof(double));
+  drd_solve_info loc;
+  memset(&loc, 0, sizeof(loc));
+  int rc = 0;
+  for (int32_t s = 0; s < steps && !rc; ++s) {
+    rc = oc_phsolve_ori(c, alpha_s, ma, pa, &loc);
+    memcpy(rmu, ma, (size_t)nc * sizeof(double));
+    memcpy(ph, pa, (size_t)nc * sizeof(double));
+  }
+  c->p.dt = dt0;
+  build_ph_lcc(c, dt0);
+  for (int64_t q = 0; q < nc; ++q) bdr[q] = (ph[q] + 1.0) / 2.0;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      if (bdr[CI(i, j)] >= 1.0)
+        bdr[CI(i, j)] = 1.0;
+      else if (bdr[CI(i, j)] <= 0.0)
+        bdr[CI(i, j)] = 0.0;
+    }
+  for (int64_t q = 0; q < nc; ++q) c->f[DRD_F_BDR_PH_PENALTY][q] = bdr[q] > 0.5 ? 1.0 : 0.0;
+  if (info) *info = loc;
+  free(pa);
+  free(ma);
+  return rc;
 }
 
 /* ---- one time step  TP/mcl6.f90:257-303 (bdf 1) and :395-502 (bdf 2); the MF

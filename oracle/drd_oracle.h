@@ -66,6 +66,11 @@ int oc_interpolate_coef(oc_ctx* c, double lambda_eta, const double* px, const do
                         double radius, const double* sx, const double* sy, int32_t nswm, double swm_radius);
 int oc_phsolve(oc_ctx* c, double* rmu_after, double* ph_after, double t,
                drd_solve_info* info);
+/* PhSolve_ori TP/PhSolve_ori.f90:1-234 (alpha_s: the wall stability term of
+ * module global) and the CH re-initialisation of a solid domain
+ * MF/finit.f90:132-188 = TP/reinint_by_PhSolve.f90 */
+int oc_phsolve_ori(oc_ctx* c, double alpha_s, double* rmu_after, double* ph_after, drd_solve_info* info);
+int oc_reinit_solid(oc_ctx* c, int32_t steps, double dt, double alpha_s, drd_solve_info* info);
 
 /* y = alpha A x + beta y; with beta == 0, y is not read (as in BLAS), so it
  * may hold anything, NaN included                                          */

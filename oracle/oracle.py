@@ -42,6 +42,8 @@ def lib():
         L.oc_dct_2d_fast.argtypes = [C.c_int32, C.c_int32, P, C.c_int32]
         L.oc_rfft_line.argtypes = [C.c_int32, P, C.c_int32]
         L.oc_fill_ghosts.argtypes = [C.c_void_p, C.c_int32]
+        L.oc_phsolve_ori.argtypes = [C.c_void_p, C.c_double, P, P, C.POINTER(abi.SolveInfo)]
+        L.oc_reinit_solid.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, C.POINTER(abi.SolveInfo)]
         L.oc_finalize.argtypes = [C.c_void_p]
         L.oc_field.argtypes = [C.c_void_p, C.c_int32]
         L.oc_field.restype = P
@@ -273,6 +275,23 @@ class Oracle:
         info = abi.SolveInfo()
         lib().oc_phsolve(self._h, dp(mu_a), dp(ph_a), t, C.byref(info))
         return mu_a, ph_a, info.as_dict()
+
+    def PhSolve_ori(self, alpha_s=0.0):
+        """PhSolve_ori TP/PhSolve_ori.f90:1-234 -> (rmu_after, ph_after, info)."""
+        cs = abi.cell_shape(self.p)
+        ph_a = np.zeros(cs)
+        mu_a = np.zeros(cs)
+        info = abi.SolveInfo()
+        rc = lib().oc_phsolve_ori(self._h, alpha_s, dp(mu_a), dp(ph_a), C.byref(info))
+        assert rc == 0, rc
+        return mu_a, ph_a, info.as_dict()
+
+    def reinit_solid(self, steps=51, dt=1e-4, alpha_s=0.0):
+        """The CH re-initialisation of the solid domain MF/finit.f90:132-188."""
+        info = abi.SolveInfo()
+        rc = lib().oc_reinit_solid(self._h, steps, dt, alpha_s, C.byref(info))
+        assert rc == 0, rc
+        return info.as_dict()
 
     def set_initial_guess(self, mode):
         assert lib().oc_set_initial_guess(self._h, mode) == 0

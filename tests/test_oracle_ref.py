@@ -264,3 +264,39 @@ def test_mf_velocity_matvec_and_jacobi():
         np.testing.assert_array_equal(o.matvec(op, 0.7, x, -0.3, y), mf.matvec(name, 0.7, x, -0.3, y))
     np.testing.assert_array_equal(o.psolve(abi.PS_VELO1_JACOBI, abi.OP_VELO1, x), mf.psolve("psolve_uv_cg_", x))
     np.testing.assert_array_equal(o.psolve(abi.PS_VELO2_JACOBI, abi.OP_VELO2, x), mf.psolve("psolve_uv_2_cg_", x))
+
+
+@needs_tp
+def test_phsolve_ori_operator_and_dct_preconditioner(tp):
+    """MATRIX_VECTOR_CG_ori and PSolve_CG_ori (TP/PhSolve_ori.f90:348-416, the
+    solve of the CH re-initialisation of solid domains) against the TP
+    reference build: the operator bit for bit (with the wall coefficients of
+    a nonzero alpha_s, set in the oracle by one PhSolve_ori step), the DCT
+    preconditioner to 1e-12 (the oracle's VCOST is its definition)."""
+    p = _params(tp)
+    p.xld, p.I_thickness, p.s_implicit, p.dt, p.tol = 0.05, 0.02, 1.5, 1e-4, 1e-10
+    tp.set_params(p)
+    tp.set_char("lr_bd_type", "n")
+    tp.set_scalar("lambda_eta", 1.0)
+    tp.set_scalar("xlsT", 0.0)
+    tp.set_scalar("slipratio", 1.0)
+    tp.setup()
+    o = oc.Oracle(p)
+    rng = np.random.default_rng(120)
+    o.set_field(abi.F_PH, rng.uniform(-1.0, 1.0, tp.cell_shape))
+    o.set_field(abi.F_ANGLEB, np.full(p.nx + 1, 75.0))
+    o.set_field(abi.F_ANGLET, np.full(p.nx + 1, 60.0))
+    alpha_s = 0.3
+    o.PhSolve_ori(alpha_s)  # sets the oracle's coef_bot, coef_top (:76-78, :114-116)
+    tmp1 = p.I_thickness / p.dz + 0.5 * alpha_s
+    coef = (p.I_thickness / p.dz - 0.5 * alpha_s) / tmp1
+    tp.set_scalar("coef_bot", coef)
+    tp.set_scalar("coef_top", coef)
+    n = p.nx * p.ny
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    np.testing.assert_array_equal(o.matvec(abi.OP_PHORI, 0.8, x, 0.2, y),
+                                  tp.matvec("matrix_vector_cg_ori_", 0.8, x, 0.2, y))
+    np.testing.assert_array_equal(o.ph_lcc(), tp.array("ph_lcc", (p.ny, p.nx)))
+    a = o.psolve(abi.PS_PH_DCT, abi.OP_PHORI, x)
+    b = tp.psolve("psolve_cg_ori_", x)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
