@@ -280,30 +280,39 @@ def run_mode(cfg, mode: str, steps: int, warmup: int, keep: bool = False, timer=
     return out, None
 
 
-def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: int = 600, batch: int = 8,
-               crop: int = 256, lr: float = 5e-4):
+def learned_c3(n: int = 1024, pairs: int = 4000, heldout: int = 12, iters: int = 2000, batch: int = 8,
+               crop: int = 256, crops_per_iterate: int = 16, lr: float = 5e-4):
     """The learned correction on its own workload: n x n variable-viscosity
-    velocity systems (geometry.velocity_system).  Pairs come from recorded
-    BiCGSTAB solves of the training systems; held-out systems are solved to
-    1e-10 by FGMRES with and without the correction (and by the plain
-    BiCGSTAB for reference).  Also times the n x n network's forward pass."""
+    velocity systems (geometry.velocity_system).  Pairs are crop x crop crops
+    of the iterates of recorded BiCGSTAB solves of training systems (16 per
+    iterate) until `pairs` are held; held-out systems are solved to 1e-10 by
+    FGMRES with and without the correction (which opens each restart cycle)
+    and by the plain BiCGSTAB for reference.  Also times the n x n network's
+    forward pass."""
     import numpy as np
 
     from drd_application_amd import abi, geometry
-    from drd_application_amd.solver import GpuSolver
+    from drd_application_amd.solver import DrdError, GpuSolver
 
     cfg0, _ = geometry.velocity_system(n, 3000)
     g = GpuSolver(cfg0.params)
-    g.learn_init(train_systems * 14, crop, crop, seed=11)
+    g.learn_init_crops(pairs, crop, crop, crops_per_iterate, seed=11)
     t0 = time.perf_counter()
-    for s in range(train_systems):
-        cfg, b = geometry.velocity_system(n, 3000 + s)
+    systems, dropped = 0, 0
+    while g.learn_info()["pairs"] < pairs:
+        cfg, b = geometry.velocity_system(n, 3000 + systems)
+        systems += 1
         geometry.load(g, cfg)
-        g.learn_collect(b, tol=1e-12)
+        try:
+            g.learn_collect(b, tol=1e-11)
+        except DrdError:  # not converged within maxit: its pairs are dropped
+            dropped += 1
+        if systems % 10 == 0:
+            log(f"[c3] {systems} training systems, {g.learn_info()['pairs']} pairs")
     collect_s = time.perf_counter() - t0
     info = g.learn_train(iters, batch, lr)
-    log(f"[c3] {info['pairs']} pairs from {train_systems} systems ({collect_s:.1f} s incl. generation), "
-        f"train {info['train_ms']:.0f} ms, loss {info['loss_first']:.3g} -> {info['loss_last']:.3g}")
+    log(f"[c3] {info['pairs']} pairs from {systems} systems ({dropped} not converged, dropped; {collect_s:.1f} s "
+        f"incl. generation), train {info['train_ms']:.0f} ms, loss {info['loss_first']:.3g} -> {info['loss_last']:.3g}")
     rows = []
     for s in range(heldout):
         cfg, b = geometry.velocity_system(n, 3900 + s)
@@ -311,32 +320,40 @@ def learned_c3(n: int = 1024, train_systems: int = 8, heldout: int = 6, iters: i
         x0 = np.zeros_like(b)
         g.synchronize()
         r = {"system": cfg.notes}
-        for key, fn in (("fgmres_plain", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 5000, 1e-10)),
-                        ("fgmres_learned", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_LEARNED, b, x0, 5000, 1e-10)),
-                        ("bicgstab_plain", lambda: g.bicgstab(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 5000, 1e-10))):
+        for key, fn in (("fgmres_plain", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 8000, 1e-10)),
+                        ("fgmres_learned", lambda: g.fgmres(abi.OP_VELO2, abi.PS_VELO_LEARNED, b, x0, 8000, 1e-10)),
+                        ("bicgstab_plain", lambda: g.bicgstab(abi.OP_VELO2, abi.PS_VELO_DCT32, b, x0, 8000, 1e-10))):
             t1 = time.perf_counter()
             _, its, _, inf = fn()
             r[key] = {"iters": its, "ms": 1e3 * (time.perf_counter() - t1), "info": inf}
         rows.append(r)
-        log(f"[c3] held-out {cfg.notes}: FGMRES plain {r['fgmres_plain']['iters']} its "
-            f"{r['fgmres_plain']['ms']:.0f} ms, learned {r['fgmres_learned']['iters']} its "
-            f"{r['fgmres_learned']['ms']:.0f} ms; BiCGSTAB {r['bicgstab_plain']['iters']} its "
-            f"{r['bicgstab_plain']['ms']:.0f} ms")
+        if s < 12 or s % 20 == 0:
+            log(f"[c3] held-out {s} {cfg.notes}: FGMRES plain {r['fgmres_plain']['iters']} its "
+                f"{r['fgmres_plain']['ms']:.0f} ms, learned {r['fgmres_learned']['iters']} its "
+                f"{r['fgmres_learned']['ms']:.0f} ms; BiCGSTAB {r['bicgstab_plain']['iters']} its "
+                f"{r['bicgstab_plain']['ms']:.0f} ms")
     ms, flops = g.bench_kernel(abi.K_UNET_FWD, 10)
     g.close()
-    tot = {k: sum(r[k]["iters"] for r in rows) for k in ("fgmres_plain", "fgmres_learned", "bicgstab_plain")}
-    tms = {k: sum(r[k]["ms"] for r in rows) for k in ("fgmres_plain", "fgmres_learned", "bicgstab_plain")}
+    keys = ("fgmres_plain", "fgmres_learned", "bicgstab_plain")
+    tot = {k: sum(r[k]["iters"] for r in rows) for k in keys}
+    tms = {k: sum(r[k]["ms"] for r in rows) for k in keys}
+    better = sum(1 for r in rows if r["fgmres_learned"]["iters"] < r["fgmres_plain"]["iters"])
     tflops = flops / (ms * 1e-3) / 1e12
+    log(f"[c3] held-out totals: FGMRES plain {tot['fgmres_plain']} its {tms['fgmres_plain']:.0f} ms, learned "
+        f"{tot['fgmres_learned']} its {tms['fgmres_learned']:.0f} ms (fewer iterations on {better} of {heldout}); "
+        f"BiCGSTAB {tot['bicgstab_plain']} its {tms['bicgstab_plain']:.0f} ms")
     log(f"[c3] network forward {n}x{n}: {ms*1e3:.0f} us, {flops/1e9:.1f} GFLOP -> {tflops:.1f} TFLOP/s")
     return {
         "workload": f"{n}x{n} velocity systems (I - dt div(eta grad) + dt chi R) u = f, eta contrast "
                     f"log-uniform to 1e3, R = 1e6 discs, FGMRES(30) to 1e-10",
-        "train_systems": train_systems, "pairs": info["pairs"], "adam_steps": info["train_iters"],
-        "crop": crop, "batch": batch, "train_ms": info["train_ms"],
+        "train_systems": systems, "train_systems_dropped": dropped, "pairs": info["pairs"],
+        "pair": f"{crop}x{crop} crop, {crops_per_iterate} per recorded iterate",
+        "adam_steps": info["train_iters"], "batch": batch, "train_ms": info["train_ms"],
         "loss_first": info["loss_first"], "loss_last": info["loss_last"],
-        "heldout_systems": heldout, "iters": tot, "ms": tms,
+        "heldout_systems": heldout, "iters": tot, "ms": tms, "heldout_fewer_iterations": better,
         "iteration_gain": (tot["fgmres_plain"] - tot["fgmres_learned"]) / tot["fgmres_plain"],
         "time_gain": (tms["fgmres_plain"] - tms["fgmres_learned"]) / tms["fgmres_plain"],
+        "time_gain_vs_bicgstab": (tms["bicgstab_plain"] - tms["fgmres_learned"]) / tms["bicgstab_plain"],
         "per_system": rows,
         "roofline_network": {"bound": "mfma", "achieved": tflops, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                              "frac": tflops / PEAK_BF16_TFLOPS, "flops_per_launch": flops,
@@ -674,6 +691,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-c0", action="store_true", help="skip the periodic 256x128 workload")
     ap.add_argument("--no-graphs", action="store_true",
                     help="skip the batched workload's run with captured graphs (for profiled runs)")
+    ap.add_argument("--c3-full", action="store_true",
+                    help="only the learned correction's workload at its full size: 20k training pairs, "
+                         "200 held-out systems (several minutes)")
     ap.add_argument("--profile-network", action="store_true",
                     help="only launch the learned correction's 1024^2 network forward pass")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -694,6 +714,9 @@ def main(argv=None) -> int:
         return profile_kernels(cfg, args.mode)
     if args.profile_network:
         return profile_network()
+    if args.c3_full:
+        print(json.dumps({"learned_c3": learned_c3(pairs=20000, heldout=200, iters=6000)}))
+        return 0
     rk_id, rk_name, rk_prefix = roof_kernel(args.mode)
     rk = getattr(abi, rk_id)
     head, g = run_mode(cfg, args.mode, args.steps, args.warmup, keep=True, timer=rk)

@@ -59,8 +59,15 @@ struct Learn {
   int ch = 0, cw = 0;
   int cap = 0, count = 0, next = 0;
   int64_t npx = 0;
-  uint16_t* feat = nullptr;  // cap x npx x 8 bf16
-  float* targ = nullptr;     // cap x npx x 2
+  // crops > 0: a pair is one crop_h x crop_w crop (crops of them taken at
+  // random from each recorded iterate); 0: a pair is the whole ny x nx image
+  int crops = 0;
+  int64_t spx = 0;           // pixels of a stored pair (npx, or crop_h * crop_w)
+  uint16_t* feat = nullptr;  // cap x spx x 8 bf16
+  float* targ = nullptr;     // cap x spx x 2
+  uint16_t* fimg = nullptr;  // crops > 0: the iterate's whole feature image
+  float* ftarg = nullptr;    //            and its target
+  int every = 0;             // FGMRES directions j with j % every == 0 take the correction (0: j == 0)
   uint16_t* img = nullptr;   // npx x 8 (preconditioner input)
   float* out = nullptr;      // npx x 2
   uint16_t* cfeat = nullptr; // crop input
@@ -1344,7 +1351,9 @@ static int run_fgmres(drd_ctx* c, int op, int prec, int64_t n, const double* b, 
       const double* zr = nullptr;
       // the learned correction opens each cycle; the cycle's other directions
       // come from the plain FP32 DCT solve
-      const int pj = (prec == DRD_PS_VELO_LEARNED && j > 0) ? DRD_PS_VELO_DCT32 : prec;
+      const int ev = c->learn ? c->learn->every : 0;
+      const bool corr = ev > 0 ? j % ev == 0 : j == 0;
+      const int pj = (prec == DRD_PS_VELO_LEARNED && !corr) ? DRD_PS_VELO_DCT32 : prec;
       RCHK(apply_prec(c, pj, op, vj, zj, &zr, nullptr));
       if (zr != zj) launch_copy(c->stream, n, zr, zj);
       RCHK(apply_op(c, op, MV_PLAIN, zj, w, 1.0, 0.0, nullptr, nullptr, nullptr));
@@ -1442,13 +1451,29 @@ static int learn_pairs(drd_ctx* c, const double* b, const double* x, const int* 
     launch_axpby(st, n, -1.0, xk, 1.0, L->e);
     launch_axpby(st, n, -1.0, L->z, 1.0, L->e);
     launch_norms3(st, n, L->r, L->d, L->z, L->ss_part, L->ss);
-    const size_t slot = (size_t)L->next;
-    launch_pack_features(st, c->g, L->r, L->d, L->z, c->f[DRD_F_ETA], c->f[DRD_F_BDR_PH_PENALTY], L->ss,
-                         L->feat + slot * L->npx * kUnetIn);
-    launch_make_target(st, c->g, L->e, L->ss, L->targ + slot * L->npx * kUnetOut);
-    L->next = (L->next + 1) % L->cap;
-    L->count = std::min(L->count + 1, L->cap);
-    ++*added;
+    if (L->crops == 0) {
+      const size_t slot = (size_t)L->next;
+      launch_pack_features(st, c->g, L->r, L->d, L->z, c->f[DRD_F_ETA], c->f[DRD_F_BDR_PH_PENALTY], L->ss,
+                           L->feat + slot * L->spx * kUnetIn);
+      launch_make_target(st, c->g, L->e, L->ss, L->targ + slot * L->spx * kUnetOut);
+      L->next = (L->next + 1) % L->cap;
+      L->count = std::min(L->count + 1, L->cap);
+      ++*added;
+      continue;
+    }
+    launch_pack_features(st, c->g, L->r, L->d, L->z, c->f[DRD_F_ETA], c->f[DRD_F_BDR_PH_PENALTY], L->ss, L->fimg);
+    launch_make_target(st, c->g, L->e, L->ss, L->ftarg);
+    const int nx = c->g.nx, ny = c->g.ny;
+    for (int q = 0; q < L->crops; ++q) {  // random crops of the iterate become pairs
+      const size_t slot = (size_t)L->next;
+      const int y0 = (int)(splitmix(L->rng) % (uint64_t)(ny - L->ch + 1));
+      const int x0 = (int)(splitmix(L->rng) % (uint64_t)(nx - L->cw + 1));
+      launch_crop(st, nx, kUnetIn, 2, L->fimg, y0, x0, L->ch, L->cw, L->feat + slot * L->spx * kUnetIn);
+      launch_crop(st, nx, kUnetOut, 4, L->ftarg, y0, x0, L->ch, L->cw, L->targ + slot * L->spx * kUnetOut);
+      L->next = (L->next + 1) % L->cap;
+      L->count = std::min(L->count + 1, L->cap);
+      ++*added;
+    }
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -1551,7 +1576,13 @@ static int learn_ready(drd_ctx* c) {
 }
 
 extern "C" int drd_learn_init(drd_ctx* c, int32_t capacity, int32_t crop_h, int32_t crop_w, uint64_t seed) {
+  return drd_learn_init_crops(c, capacity, crop_h, crop_w, 0, seed);
+}
+
+extern "C" int drd_learn_init_crops(drd_ctx* c, int32_t capacity, int32_t crop_h, int32_t crop_w,
+                                    int32_t crops_per_iterate, uint64_t seed) {
   if (!c) return set_err(DRD_E_ARG, "null ctx");
+  if (crops_per_iterate < 0) return set_err(DRD_E_ARG, "crops_per_iterate must be >= 0");
   const Grid& g = c->g;
   if (capacity < 1) return set_err(DRD_E_ARG, "capacity must be positive");
   if (g.nx % 4 || g.ny % 4) return set_err(DRD_E_UNSUPPORTED, "the network needs nx, ny multiples of 4");
@@ -1572,6 +1603,8 @@ extern "C" int drd_learn_init(drd_ctx* c, int32_t capacity, int32_t crop_h, int3
   L->cw = crop_w;
   L->cap = capacity;
   L->npx = (int64_t)g.nx * g.ny;
+  L->crops = crops_per_iterate;
+  L->spx = crops_per_iterate ? (int64_t)crop_h * crop_w : L->npx;
   L->rng = seed ^ 0x6A09E667F3BCC909ull;
   const int64_t nuv = g.nuv(), cpx = (int64_t)crop_h * crop_w;
   auto alloc = [&](void** p, size_t bytes) {
@@ -1583,8 +1616,12 @@ extern "C" int drd_learn_init(drd_ctx* c, int32_t capacity, int32_t crop_h, int3
     }
     L->allocs.push_back(*p);
   };
-  alloc((void**)&L->feat, (size_t)capacity * L->npx * kUnetIn * 2);
-  alloc((void**)&L->targ, (size_t)capacity * L->npx * kUnetOut * 4);
+  alloc((void**)&L->feat, (size_t)capacity * L->spx * kUnetIn * 2);
+  alloc((void**)&L->targ, (size_t)capacity * L->spx * kUnetOut * 4);
+  if (L->crops) {
+    alloc((void**)&L->fimg, (size_t)L->npx * kUnetIn * 2);
+    alloc((void**)&L->ftarg, (size_t)L->npx * kUnetOut * 4);
+  }
   alloc((void**)&L->img, (size_t)L->npx * kUnetIn * 2);
   alloc((void**)&L->out, (size_t)L->npx * kUnetOut * 4);
   alloc((void**)&L->cfeat, (size_t)cpx * kUnetIn * 2);
@@ -1621,12 +1658,26 @@ extern "C" int drd_learn_collect(drd_ctx* c, const double* b, double tol, double
   HIPCHK(hipMemsetAsync(c->xv, 0, n * sizeof(double), c->stream));
   int added = 0;
   drd_solve_info info{};
+  Learn* L = c->learn;
+  const int next0 = L->next, count0 = L->count;
   RCHK(learn_recorded_solve(c, c->rhs, c->xv, std::max(c->p.maxit_velo2, 1), tol, &info, &added));
   RCHK(down(c, x, c->xv, n));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (pairs_added) *pairs_added = added;
   if (iters) *iters = info.iter;
-  if (info.info != 0) return set_err(DRD_E_STATE, "recorded solve did not converge");
+  if (info.info != 0) {  // no converged solution, so no error to learn from: the pairs are dropped
+    L->next = next0;
+    L->count = count0;
+    if (pairs_added) *pairs_added = 0;
+    return set_err(DRD_E_STATE, "recorded solve did not converge (its pairs are dropped)");
+  }
+  if (pairs_added) *pairs_added = added;
+  return 0;
+}
+
+extern "C" int drd_learn_set_every(drd_ctx* c, int32_t every) {
+  RCHK(learn_ready(c));
+  if (every < 0) return set_err(DRD_E_ARG, "every must be >= 0");
+  c->learn->every = every;
   return 0;
 }
 
@@ -1650,10 +1701,11 @@ extern "C" int drd_learn_train(drd_ctx* c, int32_t iters, int32_t batch, double 
     RCHK(unet_zero_grad(st, L->tr));
     for (int bi = 0; bi < batch; ++bi) {
       const size_t slot = (size_t)(splitmix(L->rng) % (uint64_t)L->count);
-      const int y0 = (int)(splitmix(L->rng) % (uint64_t)(ny - L->ch + 1));
-      const int x0 = (int)(splitmix(L->rng) % (uint64_t)(nx - L->cw + 1));
-      launch_crop(st, nx, kUnetIn, 2, L->feat + slot * L->npx * kUnetIn, y0, x0, L->ch, L->cw, L->cfeat);
-      launch_crop(st, nx, kUnetOut, 4, L->targ + slot * L->npx * kUnetOut, y0, x0, L->ch, L->cw, L->ctarg);
+      const int wd = L->crops ? L->cw : nx, ht = L->crops ? L->ch : ny;  // a stored crop is taken whole
+      const int y0 = (int)(splitmix(L->rng) % (uint64_t)(ht - L->ch + 1));
+      const int x0 = (int)(splitmix(L->rng) % (uint64_t)(wd - L->cw + 1));
+      launch_crop(st, wd, kUnetIn, 2, L->feat + slot * L->spx * kUnetIn, y0, x0, L->ch, L->cw, L->cfeat);
+      launch_crop(st, wd, kUnetOut, 4, L->targ + slot * L->spx * kUnetOut, y0, x0, L->ch, L->cw, L->ctarg);
       RCHK(unet_forward(st, L->tr, L->cfeat, nullptr));
       RCHK(unet_backward(st, L->tr, L->cfeat, L->ctarg));
     }

@@ -109,6 +109,8 @@ def lib():
         L.drd_kernel_timer_launches.argtypes = [V, C.c_int32, C.POINTER(C.c_float), I]
         F = C.POINTER(C.c_float)
         L.drd_learn_init.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+        L.drd_learn_init_crops.argtypes = [V, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]
+        L.drd_learn_set_every.argtypes = [V, C.c_int32]
         L.drd_learn_collect.argtypes = [V, P, C.c_double, P, I, I]
         L.drd_learn_record.argtypes = [V, C.c_int32]
         L.drd_learn_train.argtypes = [V, C.c_int32, C.c_int32, C.c_double]
@@ -409,6 +411,12 @@ class GpuSolver:
     # ---- learned correction of the velocity preconditioner ------------------
     def learn_init(self, capacity, crop_h, crop_w, seed=1):
         _check(lib().drd_learn_init(self._h, capacity, crop_h, crop_w, seed))
+
+    def learn_init_crops(self, capacity, crop_h, crop_w, crops_per_iterate, seed=1):
+        _check(lib().drd_learn_init_crops(self._h, capacity, crop_h, crop_w, crops_per_iterate, seed))
+
+    def learn_set_every(self, every):
+        _check(lib().drd_learn_set_every(self._h, every))
 
     def learn_collect(self, b, tol=1e-12):
         """Recorded BiCGSTAB(FP32 DCT) solve of A x = b (BDF2 velocity

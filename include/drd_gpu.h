@@ -481,7 +481,8 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  * the step that minimises |r - A z'| (so |r - A z'| <= |d|).  The correction
  * is not linear in r, so only the flexible Krylov accepts it (drd_bicgstab
  * rejects it); drd_fgmres applies it at the first direction of each restart
- * cycle and the plain FP32 DCT solve for the cycle's other directions.
+ * cycle (or every drd_learn_set_every-th) and the plain FP32 DCT solve for
+ * the cycle's other directions.
  * velo_prec = DRD_PREC_LEARNED makes drd_step solve the BDF2 velocity system
  * with it, judged once: the first solve after drd_learn_enable runs FGMRES
  * with the correction, FGMRES with the plain preconditioner and the plain
@@ -558,6 +559,16 @@ typedef struct drd_learn_info {
  * for the preconditioner and a crop_h x crop_w one for training (multiples
  * of 4, at most ny x nx).  He-normal initial weights from seed.            */
 int drd_learn_init(drd_ctx* ctx, int32_t capacity, int32_t crop_h, int32_t crop_w, uint64_t seed);
+/* The same with a store of crops: each recorded iterate adds
+ * crops_per_iterate pairs, random crop_h x crop_w crops of its images
+ * (24 bytes per crop cell each), so the store holds many more pairs (20k
+ * crops of 256 x 256 take 31 GB).  crops_per_iterate 0 is drd_learn_init.   */
+int drd_learn_init_crops(drd_ctx* ctx, int32_t capacity, int32_t crop_h, int32_t crop_w,
+                         int32_t crops_per_iterate, uint64_t seed);
+/* FGMRES takes the correction at the directions j of a restart cycle with
+ * j % every == 0 (every 1: every direction); 0 (the default): j == 0 only,
+ * the cycle's other directions from the plain FP32 DCT solve.               */
+int drd_learn_set_every(drd_ctx* ctx, int32_t every);
 /* Solver-emitted pairs: solves A x = b (the BDF2 velocity operator of the
  * context's fields) from x = 0 by BiCGSTAB with the FP32 DCT preconditioner
  * to `tol`, keeping iterates x_k at k = 1, 2, 3, 5, 8, 12, ... (up to 0.8x

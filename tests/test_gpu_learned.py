@@ -6,9 +6,12 @@
   differences), the oracle mirroring where the kernels round to BF16;
 * training on pairs the solver emits lowers the loss, and FGMRES with the
   trained correction converges on held-out systems to the FP64 true
-  residual (checked with the oracle's operator); the iterations it needs
-  against FGMRES with the plain FP32 DCT preconditioner are printed (the
-  measured gain) and bounded;
+  residual (checked with the oracle's operator).  The correction does not
+  earn its place: on the full workload (20k pairs, 200 held-out 1024^2
+  systems) it needs 2.7% more FGMRES iterations and 7.7% more time than the
+  plain FP32 DCT preconditioner.  Here the iterations it needs are printed
+  and held within 5% of plain FGMRES (the step alpha keeps each corrected
+  residual within the plain one's, so it cannot stall the solve);
 * drd_step with velo_prec = LEARNED: the judged first solve, and the
   automatic fallback to the plain method when the network is bad.
 """
@@ -121,9 +124,9 @@ def test_learned_correction_trains_and_converges_on_held_out_systems():
         print(f"held-out {cfg.notes}: plain {itp} its, learned {itl} its", flush=True)
     plain, learned = sum(r[1] for r in results), sum(r[2] for r in results)
     print(f"learned correction: {learned} vs {plain} FGMRES iterations ({100.0 * (plain - learned) / plain:+.1f}%)")
-    # the step alpha keeps each corrected residual within the plain one's; a
-    # correction that opens each restart cycle cannot stall FGMRES
-    assert learned <= 1.3 * plain
+    # no gain to assert (see the module docstring): the correction stays
+    # within 5% of the plain FGMRES iterations
+    assert learned <= 1.05 * plain
     g.close()
 
 
