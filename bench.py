@@ -627,8 +627,9 @@ def measured_traffic(prefix: str, cfg_name: str):
 
 
 def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
-    """The velocity matvec, a plain copy of the same length, the mode's
-    velocity preconditioner and (FP32 red-black modes) one middle CH sweep,
+    """The velocity and CH matvecs, the BiCGSTAB x/r update, a plain copy of
+    the velocity vector's length, the mode's velocity preconditioner and
+    (FP32 red-black modes) one middle CH sweep,
     each launched `reps` times after one BDF1 step has filled the fields;
     under rocprofv3 --pmc the copy's FETCH_SIZE/WRITE_SIZE calibrate the
     counters."""
@@ -645,7 +646,8 @@ def profile_kernels(cfg, mode: str, reps: int = 10) -> int:
     out = {}
     prec_k = (abi.K_VELO_MG, "velo_mg_vcycle") if mode.startswith("mg") else (abi.K_VELO_PREC, "velo_prec")
     sweep = [(abi.K_PHMU_SWEEP32, "phmu_rb_sweep32")] if MODES[mode][1] == "CH_PREC_RBGS32" else []
-    for which, name in [(abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_COPY, "copy"), prec_k] + sweep:
+    for which, name in [(abi.K_VELO_MATVEC, "velo_matvec"), (abi.K_PHMU_MATVEC, "phmu_matvec"),
+                        (abi.K_BICG_XR, "bicg_xr"), (abi.K_COPY, "copy"), prec_k] + sweep:
         ms, by = g.bench_kernel(which, reps)
         out[name] = {"us": ms * 1e3, "bytes": by, "launches": reps + 1}
         log(f"{name}: {ms*1e3:.1f} us, {by/1e6:.1f} MB algorithmic")

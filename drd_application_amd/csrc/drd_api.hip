@@ -130,6 +130,7 @@ struct drd_ctx {
   // captured BiCGSTAB iteration chunks (drd_use_graphs)
   bool use_graphs = false;
   int guess = 0;  // drd_set_initial_guess
+  bool mg_fused = true;  // drd_mg_fused
   std::map<GraphKey, GraphEntry> graphs;
   int64_t graph_recaptures = 0;  // replays refused for a changed fingerprint
   int pred_its[5] = {0, 0, 0, 0, 0};  // per operator: iterations of its latest BiCGSTAB solve
@@ -621,6 +622,16 @@ extern "C" int drd_set_initial_guess(drd_ctx* c, int32_t mode) {
   return 0;
 }
 
+extern "C" int drd_mg_fused(drd_ctx* c, int32_t on) {
+  if (!c) return set_err(DRD_E_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // captured chunks hold the other form's launches
+  c->mg_fused = on != 0;
+  velo_mg_set_fused(c->vmg, c->mg_fused);
+  velo_mg_set_fused(c->vmg1, c->mg_fused);
+  return 0;
+}
+
 extern "C" int drd_use_graphs(drd_ctx* c, int32_t on) {
   if (!c) return set_err(DRD_E_ARG, "null ctx");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -899,6 +910,7 @@ static int prec_setup(drd_ctx* c, int prec) {
         return set_err(err ? err : DRD_E_HIP,
                        "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
     }
+    velo_mg_set_fused(c->vmg, c->mg_fused);
     velo_mg_setup(c->stream, c->vmg, coef_of(c));
   }
   if (prec == DRD_PS_VELO1_DCT32 || prec == DRD_PS_VELO1_DCT64) {
@@ -935,6 +947,7 @@ static int prec_setup(drd_ctx* c, int prec) {
         return set_err(err ? err : DRD_E_HIP,
                        "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
     }
+    velo_mg_set_fused(c->vmg1, c->mg_fused);
     velo_mg_setup(c->stream, c->vmg1, coef_of(c));
   }
   return 0;

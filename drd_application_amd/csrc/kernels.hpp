@@ -258,6 +258,9 @@ bool velo_mg_f32(const VeloMG* mg);
 void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0);
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks);
 int velo_mg_levels(const VeloMG* mg);
+// on (the default): V(2,2) cycles by the LDS-tiled legs of kernels_mgf.hip;
+// off: one launch per sweep and transfer (the same values bit for bit)
+void velo_mg_set_fused(VeloMG* mg, bool on);
 // its kernels (kernels_stencil.hip); a buffer's element type is FP32 where
 // its flag says so, else FP64
 struct MgCoef {
@@ -274,6 +277,16 @@ void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, cons
 void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
                       const MgCoef& c, const void* x, const void* b, const void* dw, void* out, float* bc,
                       const KState* ks);
+// LDS-tiled legs (kernels_mgf.hip).  down: b (FP64 when !b32), dw, the
+// level's coefficients -> x2 in xd, bcopy = FP32 b (optional), bc = R r
+// (FP32 when bc32).  up: out = two sweeps from xd + P xc (xc FP32 when c32,
+// out FP32 when o32).
+void launch_mg_down(hipStream_t st, bool f32, bool b32, bool bc32, const Grid& g, const Grid& gc, const Scal& s,
+                    const MgCoef& c, const void* b, const void* dw, void* xd, float* bcopy, void* bc,
+                    const KState* ks);
+void launch_mg_up(hipStream_t st, bool f32, bool c32, bool o32, const Grid& g, const Grid& gc, const Scal& s,
+                  const MgCoef& c, const void* b, const void* dw, const void* xd, const void* xc, void* out,
+                  const KState* ks);
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
                         void* bc, const KState* ks);
 void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,

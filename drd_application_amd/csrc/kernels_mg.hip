@@ -5,13 +5,15 @@
 // the result keeps nx >= 64 and ny >= 32.  The coarse operators are
 // rediscretised: the same MAC stencil (velo_row) with doubled spacings, on
 // coefficients rho, eta, chi averaged over 2x2 cells.  One V-cycle:
-//   down:  x = omega b / d, one more damped Jacobi sweep (the two in one
-//          pass), r = b - A x, b_{l+1} = R r
+//   down:  x = omega b / d, one more damped Jacobi sweep, r = b - A x,
+//          b_{l+1} = R r
 //   coarsest: x = the FP64 DCT/DST fast-Poisson solve of its b
 //   up:    x += P x_{l+1}, two damped Jacobi sweeps
 // with omega = 0.7 and d the exact diagonal of each level's operator.  The
 // cycle is a fixed map of b, so BiCGSTAB accepts it.  Every kernel skips
-// once the solve it serves is done or has taken its early exit.
+// once the solve it serves is done or has taken its early exit.  V(2,2)
+// runs each leg of a level as one LDS-tiled launch (kernels_mgf.hip); other
+// sweep counts, and V(2,2) when asked, one launch per sweep and transfer.
 //
 // FP32 storage: every level but the coarsest keeps its coefficients,
 // diagonal and vectors in FP32 (level 0 an FP32 copy of the context's
@@ -39,12 +41,16 @@ struct VeloMG {
   bool f32 = false;
   double omega = 0.7;
   int nu_pre = 2, nu_post = 2;  // damped Jacobi sweeps before / after the coarse correction (>= 1 each)
+  bool fused = true;            // V(2,2) by the LDS-tiled legs of kernels_mgf.hip
 };
 
 static bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 int velo_mg_levels(const VeloMG* mg) { return mg ? (int)mg->lv.size() : 0; }
 bool velo_mg_f32(const VeloMG* mg) { return mg && mg->f32; }
+void velo_mg_set_fused(VeloMG* mg, bool on) {
+  if (mg) mg->fused = on;
+}
 
 void velo_mg_destroy(VeloMG* mg) {
   if (!mg) return;
@@ -137,6 +143,35 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
   velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
 }
 
+// V(2,2) by one launch per level and leg (kernels_mgf.hip): the down leg of
+// level l leaves its x2 in lv[l].t and the coarse b in lv[l+1].b; the up leg
+// reads lv[l].t and the coarse level's x (lv[l+1].x) and writes lv[l].x, or
+// z on level 0.  The FP32 hierarchy's level 0 reads the FP64 b going down
+// and keeps an FP32 copy in lv[0].b for the way up.
+static void velo_mg_apply_fused(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks) {
+  const int nl = (int)mg->lv.size();
+  for (int l = 0; l + 1 < nl; ++l) {
+    MGLevel& L = mg->lv[l];
+    const MGLevel& N = mg->lv[l + 1];
+    const bool top = l == 0;
+    const void* bl = top ? (const void*)b : (const void*)L.b;
+    float* bcopy = top && L.f32 ? (float*)L.b : nullptr;
+    launch_mg_down(st, L.f32, L.f32 && !top, N.f32, L.g, N.g, L.s, L.c, bl, L.d, L.t, bcopy, N.b, ks);
+  }
+  {
+    const MGLevel& C = mg->lv[nl - 1];  // FP64
+    velo_prec_apply(st, mg->coarse, C.g, (const double*)C.b, (double*)C.x, ks);
+  }
+  for (int l = nl - 2; l >= 0; --l) {
+    MGLevel& L = mg->lv[l];
+    const MGLevel& N = mg->lv[l + 1];
+    const bool top = l == 0;
+    const void* bl = (top && !L.f32) ? (const void*)b : (const void*)L.b;
+    void* out = top ? (void*)z : L.x;
+    launch_mg_up(st, L.f32, N.f32, L.f32 && !top, L.g, N.g, L.s, L.c, bl, L.d, L.t, N.x, out, ks);
+  }
+}
+
 // z = V-cycle(b).  Level l smooths between two buffers, P (its temporary)
 // and Q (its x, which the level above prolongs from; FP64 level 0: z).  The
 // nu_post sweeps after the coarse correction alternate from D and must end
@@ -145,6 +180,10 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
 // correction added.  Level 0 of an FP32 hierarchy ends in z, a third
 // buffer, so its D is free (Q).  No sweep reads the buffer it writes.
 void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const KState* ks) {
+  if (mg->fused && mg->nu_pre == 2 && mg->nu_post == 2) {
+    velo_mg_apply_fused(st, mg, b, z, ks);
+    return;
+  }
   const int nl = (int)mg->lv.size();
   const int n1 = mg->nu_pre, n2 = mg->nu_post;
   auto P = [&](int l) { return mg->lv[l].t; };

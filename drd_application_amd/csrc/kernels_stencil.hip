@@ -11,133 +11,9 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "velo_rows.hpp"
 
 namespace drd {
-
-// Ghost rules applied on the fly to a packed velocity vector:
-// btoA_uv TP/matrix_tool.f90:32-75 plus the overrides of
-// MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:554-583.
-// The vector's values come from a loader L (value of packed element i, as
-// double): a stored FP64 or FP32 vector (LdVec), or values formed on the fly.
-template <class T>
-struct LdVec {
-  using R = T;  // the arithmetic type of values read through it
-  const T* __restrict__ x;
-  __device__ __forceinline__ T operator()(uint32_t i) const { return ld32(x, i); }
-};
-// PX: periodic in x, a u row of nx unknowns and every side ghost the
-// periodic copy (u(-1) = u(nx-1), u(nx) = u(0), u(nx+1) = u(1); v likewise).
-template <class L, bool PX = false>
-struct VeloAcc {
-  using R = typename L::R;
-  L ld;
-  int nx, ny, lr_d;
-  uint32_t mn;
-  __device__ __forceinline__ R uraw(int i, int k) const { return ld((uint32_t)(i + (PX ? nx : nx + 1) * k)); }
-  // f1(i,k) for -1<=i<=nx+1, -1<=k<=ny (corners unused by the stencils):
-  // wall ghosts are odd mirrors of the first/last row, side ghosts even or
-  // odd mirrors of column 1 / nx-1.  Negation is exact, so the mirrorings
-  // commute; both are applied branch-free.
-  __device__ __forceinline__ R U(int i, int k) const {
-    const bool kw = (k < 0) | (k >= ny);
-    const int kk = k < 0 ? 0 : (k >= ny ? ny - 1 : k);
-    if (PX) {
-      const int ii = i < 0 ? i + nx : (i >= nx ? i - nx : i);
-      const R v = ld((uint32_t)(ii + nx * kk));
-      return kw ? -v : v;
-    }
-    const bool iw = ((i < 0) | (i > nx)) & (lr_d != 0);
-    const int ii = i < 0 ? 1 : (i > nx ? nx - 1 : i);
-    const R v = ld((uint32_t)(ii + (nx + 1) * kk));
-    const R w = iw ? -v : v;
-    return kw ? -w : w;
-  }
-  // f2(i,k) for -1<=i<=nx, 0<=k<=ny (0 on the walls k = 0, ny)
-  __device__ __forceinline__ R V(int i, int k) const {
-    const int ic = PX ? (i < 0 ? i + nx : (i >= nx ? i - nx : i)) : (i < 0 ? 0 : (i >= nx ? nx - 1 : i));
-    const int kc = k < 1 ? 1 : (k > ny - 1 ? ny - 1 : k);
-    const R v = ld(mn + (uint32_t)(ic + nx * (kc - 1)));
-    return (k <= 0 || k >= ny) ? R(0.0) : v;
-  }
-};
-template <class T, bool PX = false>
-using VeloInT = VeloAcc<LdVec<T>, PX>;
-using VeloIn = VeloInT<double>;
-
-// y = A x for one unknown q of the velocity operator.
-// MatrixVector_uv_CG  TP/VeloSolve_combined.f90:606-670 (bdf 1)
-// MatrixVector_uv_2_CG TP/VeloSolve_2_combined.f90:607-670 (bdf 2)
-// Computed in the coefficients' type T: FP64 is the reference's arithmetic;
-// FP32 (the FP32 multigrid levels) multiplies by the reciprocals of the
-// spacings where the reference divides.  The vector's loader reads T.
-template <class T>
-__device__ __forceinline__ T vdiv(T a, const Scal& s, double d, double inv) {
-  return sizeof(T) == sizeof(float) ? a * T(inv) : a / T(d);
-}
-template <int BDF, class T, class XA>
-__device__ __forceinline__ T velo_u_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                        const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
-                                        uint32_t q) {
-  const int nx = g.nx;
-  T r;
-  {
-    const int k = g.by_nxu.div(q);
-    const int i = (int)(q - (uint32_t)k * (uint32_t)g.nxu());
-    const uint32_t c = g.cio(i, k), cw = c - 1, ld = (uint32_t)(nx + 2);
-    const T f = X.U(i, k);
-    if (BDF == 2)
-      r = f * T(1.5) * T(s.re) * (ld32(rho, c) + ld32(rho, cw)) / T(2.0) +
-          f * T(s.penalty_k) * (ld32(chi, c) + ld32(chi, cw)) / T(2.0) * T(s.dt);
-    else
-      r = f * T(s.re) * (ld32(rho, c) + ld32(rho, cw)) / T(2.0);
-    const T e00 = ld32(eta, c), e10 = ld32(eta, cw);
-    T cr = e00, cl = e10;
-    r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
-    cr = (ld32(eta, c + ld) + ld32(eta, cw + ld) + e00 + e10) / T(4.0);
-    cl = (e00 + e10 + ld32(eta, c - ld) + ld32(eta, cw - ld)) / T(4.0);
-    r = r - vdiv(T(s.dt) * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
-    const T cross = T(s.dt) * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k)));
-    r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
-  }
-  return r;
-}
-
-template <int BDF, class T, class XA>
-__device__ __forceinline__ T velo_v_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                        const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
-                                        uint32_t q) {
-  const int nx = g.nx;
-  T r;
-  {
-    const uint32_t t = q - X.mn;
-    const int k = g.by_nx.div(t) + 1;
-    const int i = (int)(t - (uint32_t)(k - 1) * (uint32_t)nx);
-    const uint32_t c = g.cio(i, k), ld = (uint32_t)(nx + 2), cs = c - ld;
-    const T f = X.V(i, k);
-    if (BDF == 2)
-      r = f * T(1.5) * T(s.re) * (ld32(rho, c) + ld32(rho, cs)) / T(2.0) +
-          f * T(s.penalty_k) * (ld32(chi, c) + ld32(chi, cs)) / T(2.0) * T(s.dt);
-    else
-      r = f * T(s.re) * (ld32(rho, c) + ld32(rho, cs)) / T(2.0);
-    const T e00 = ld32(eta, c), e01 = ld32(eta, cs);
-    T cr = e00, cl = e01;
-    r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
-    cr = (ld32(eta, c + 1) + e00 + ld32(eta, cs + 1) + e01) / T(4.0);
-    cl = (e00 + ld32(eta, c - 1) + e01 + ld32(eta, cs - 1)) / T(4.0);
-    r = r - vdiv(T(s.dt) * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
-    const T cross = T(s.dt) * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1)));
-    r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
-  }
-  return r;
-}
-
-template <int BDF, class T, class XA>
-__device__ __forceinline__ T velo_row(const Grid g, const Scal& s, const T* __restrict__ rho,
-                                      const T* __restrict__ eta, const T* __restrict__ chi, const XA& X,
-                                      uint32_t q) {
-  return q < X.mn ? velo_u_row<BDF, T, XA>(g, s, rho, eta, chi, X, q)
-                   : velo_v_row<BDF, T, XA>(g, s, rho, eta, chi, X, q);
-}
 
 __device__ __forceinline__ bool kactive(const KState* st) {
   return st == nullptr || (!st->done && !st->early);
@@ -1234,40 +1110,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nuv()) return;
-  // the weights of mg_w_face / mg_w_cell_odd / mg_w_cell_even in closed
-  // form; loads are clamped into the grid and their weights zeroed instead
+  const PackedUV<TR> R{r, gf.nx, gf.mn()};
   double acc;
-  if (q < gc.mn()) {  // u (I, K): fine faces 2I-1..2I+1 (1/2, 1, 1/2), rows 2K-1..2K+2
-    const int ldf = gf.nx + 1;
+  if (q < gc.mn()) {
     const int K = (int)(q / (gc.nx + 1)), I = (int)(q - (int64_t)K * (gc.nx + 1));
-    const bool lo = K >= 1, hi = K + 1 <= gc.ny - 1;  // the neighbouring coarse rows exist
-    const double wy[4] = {lo ? 0.25 : 0.0, lo ? 0.75 : 0.5, hi ? 0.75 : 0.5, hi ? 0.25 : 0.0};
-    const int il = I > 0 ? 2 * I - 1 : 0, ir = 2 * I + 1 <= gf.nx ? 2 * I + 1 : gf.nx;
-    const double wl = I > 0 ? 0.5 : 0.0, wr = 2 * I + 1 <= gf.nx ? 0.5 : 0.0;
-    acc = 0.0;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      int k = 2 * K - 1 + m;
-      k = k < 0 ? 0 : (k > gf.ny - 1 ? gf.ny - 1 : k);
-      const int64_t row = (int64_t)ldf * k;
-      const double sx = wl * (double)r[il + row] + (double)r[2 * I + row] + wr * (double)r[ir + row];
-      acc += wy[m] * sx;
-    }
-  } else {  // v (I, K), K = 1..ny-1: fine inner faces 2K-1..2K+1 (1/2, 1, 1/2), cells 2I-1..2I+2
+    acc = mg_restrict_u(gf, gc, R, I, K);
+  } else {
     const int64_t t = q - gc.mn();
     const int K = (int)(t / gc.nx) + 1, I = (int)(t - (int64_t)(K - 1) * gc.nx);
-    const bool left = I >= 1, right = I + 1 <= gc.nx - 1;  // the neighbouring coarse cells exist
-    const double wx[4] = {left ? 0.25 : 0.0, left ? 0.75 : 1.0, right ? 0.75 : 1.0, right ? 0.25 : 0.0};
-    const int i0 = left ? 2 * I - 1 : 0, i3 = right ? 2 * I + 2 : gf.nx - 1;
-    const double wy[3] = {0.5, 1.0, 0.5};
-    acc = 0.0;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const int64_t row = gf.mn() + (int64_t)gf.nx * (2 * K - 2 + m);  // fine face k = 2K-1+m
-      const double sx = wx[0] * (double)r[i0 + row] + wx[1] * (double)r[2 * I + row] +
-                        wx[2] * (double)r[2 * I + 1 + row] + wx[3] * (double)r[i3 + row];
-      acc += wy[m] * sx;
-    }
+    acc = mg_restrict_v(gf, gc, R, I, K);
   }
   if (late && st && (st->done || st->early)) return;
   bc[q] = (TB)(0.25 * acc);
@@ -1281,37 +1132,22 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, con
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gf.nuv()) return;
-  // closed-form weights as in k_mg_restrict
   double acc;
-  if (q < gf.mn()) {  // u (i, k): coarse faces i/2 (and i/2 + 1), rows k/2 and its neighbour
-    const int ldc = gc.nx + 1;
+  int I[4], K[4];
+  double xv[4];
+  if (q < gf.mn()) {
     const int k = (int)(q / (gf.nx + 1)), i = (int)(q - (int64_t)k * (gf.nx + 1));
-    const int I0 = i >> 1, I1 = (i & 1) ? I0 + 1 : I0;  // odd i: 1/2 each; even i: I0 alone
-    const double wx = (i & 1) ? 0.5 : 1.0;
-    const int K0 = k >> 1, K1 = (k & 1) ? K0 + 1 : K0 - 1;
-    const bool in1 = K1 >= 0 && K1 <= gc.ny - 1;
-    const int K1c = in1 ? K1 : K0;
-    const double w0 = in1 ? 0.75 : 0.5, w1 = in1 ? 0.25 : 0.0;
-    const double v0 = (double)xc[I0 + (int64_t)ldc * K0], u0 = (double)xc[I1 + (int64_t)ldc * K0];
-    const double v1 = (double)xc[I0 + (int64_t)ldc * K1c], u1 = (double)xc[I1 + (int64_t)ldc * K1c];
-    const double s0 = (i & 1) ? wx * v0 + wx * u0 : v0, s1 = (i & 1) ? wx * v1 + wx * u1 : v1;
-    acc = w0 * s0 + w1 * s1;
-  } else {  // v (i, k), k = 1..ny-1: coarse inner faces k/2 (and k/2 + 1), cells i/2 and its neighbour
+    mg_prolong_u_at(gc, i, k, I, K);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) xv[m] = (double)xc[mg_uidx(gc, I[m], K[m])];
+    acc = mg_prolong_u_sum(gc, i, k, xv);
+  } else {
     const int64_t t = q - gf.mn();
     const int k = (int)(t / gf.nx) + 1, i = (int)(t - (int64_t)(k - 1) * gf.nx);
-    // faces: even k -> K = k/2 (1); odd k -> k>>1 and k>>1 + 1 (1/2 each), wall faces (0, ny) dropped
-    const int Ka = k >> 1, Kb = (k & 1) ? Ka + 1 : Ka;
-    const bool ina = Ka >= 1 && Ka <= gc.ny - 1, inb = Kb >= 1 && Kb <= gc.ny - 1;
-    const double wa = ina ? ((k & 1) ? 0.5 : 1.0) : 0.0, wb = ((k & 1) && inb) ? 0.5 : 0.0;
-    const int Kac = ina ? Ka : 1, Kbc = inb ? Kb : 1;
-    // cells: c = i/2 (3/4), its neighbour (1/4), clamped (the weights add where they meet)
-    const int c = i >> 1;
-    int o = (i & 1) ? c + 1 : c - 1;
-    o = o < 0 ? 0 : (o > gc.nx - 1 ? gc.nx - 1 : o);
-    const int64_t ra = gc.mn() + (int64_t)gc.nx * (Kac - 1), rb = gc.mn() + (int64_t)gc.nx * (Kbc - 1);
-    const double sa = 0.75 * (double)xc[c + ra] + 0.25 * (double)xc[o + ra];
-    const double sb = 0.75 * (double)xc[c + rb] + 0.25 * (double)xc[o + rb];
-    acc = wa * sa + wb * sb;
+    mg_prolong_v_at(gc, i, k, I, K);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) xv[m] = (double)xc[mg_vidx(gc, I[m], K[m])];
+    acc = mg_prolong_v_sum(gc, i, k, xv);
   }
   const double xq = (double)x[q];
   if (late && st && (st->done || st->early)) return;

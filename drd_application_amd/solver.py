@@ -126,6 +126,7 @@ def lib():
         L.drd_batch_step.argtypes = [V, C.c_int32, C.c_double, C.POINTER(abi.StepInfo)]
         L.drd_batch_groups.argtypes = [V, I, I, C.c_int32]
         L.drd_use_graphs.argtypes = [V, C.c_int32]
+        L.drd_mg_fused.argtypes = [V, C.c_int32]
         L.drd_graph_count.argtypes = [V, I, C.POINTER(C.c_int64)]
         L.drd_write_snapshot.argtypes = [V, C.c_char_p, C.c_int32, C.c_int32, C.c_double]
         L.drd_interpolate_coef.argtypes = [V, C.c_double, P, P, C.c_int32, C.c_double, P, P, C.c_int32,
@@ -390,6 +391,12 @@ class GpuSolver:
     def set_initial_guess(self, mode):
         """0: BDF2 solves start from the previous step (the reference); 1: from 2x^n - x^{n-1}."""
         _check(lib().drd_set_initial_guess(self._h, mode))
+
+    def mg_fused(self, on=True):
+        """V(2,2) multigrid cycles by one LDS-tiled launch per level and leg
+        (on, the default) or one launch per sweep and transfer (off); the
+        same values either way."""
+        _check(lib().drd_mg_fused(self._h, 1 if on else 0))
 
     def use_graphs(self, on=True):
         """Capture BiCGSTAB iteration chunks as hipGraphs and replay them."""

@@ -351,6 +351,14 @@ int drd_use_graphs(drd_ctx* ctx, int32_t on);
 int drd_graph_count(drd_ctx* ctx, int32_t* live, int64_t* recaptured);
 int drd_batch_use_graphs(drd_batch* b, int32_t on);
 
+/* ---- multigrid V-cycle form (no reference counterpart) -------------------
+ * on (the default): each leg of each level of the V(2,2) cycle of
+ * DRD_PREC_MG / DRD_PREC_MG32 is one LDS-tiled launch (two sweeps, residual
+ * and restriction going down; prolongation and two sweeps going up).  off:
+ * one launch per sweep and transfer.  Both give the same values bit for bit;
+ * the switch exists to show that, and to time the two forms.                */
+int drd_mg_fused(drd_ctx* ctx, int32_t on);
+
 /* ---- tile autotuner (no reference counterpart) ----------------------------
  * Times the DCT/DST velocity preconditioner with each transpose tile shape
  * (0: 32x32, 1: 16x64, 2: 64x16 elements) on the current fields and keeps

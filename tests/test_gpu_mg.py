@@ -207,3 +207,37 @@ def test_bdf1_preconditioners(prec):
     assert info == 0 and np.linalg.norm(r) <= 1.01e-10 * max(np.linalg.norm(rhs), 1.0)
     assert (3 if "MG" in prec else 2) * it < itj
     g.close()
+
+
+@pytest.mark.parametrize("nx,ny,lr", [(128, 64, "n"), (256, 128, "d"), (512, 256, "n"), (2048, 1024, "n")])
+def test_tiled_cycle_equals_per_pass_cycle(nx, ny, lr):
+    """The V(2,2) cycle run as one LDS-tiled launch per level and leg equals
+    the cycle run as one launch per sweep and transfer bit for bit, FP64 and
+    FP32 hierarchies, BDF2 and BDF1 operators, side walls with and without
+    the Dirichlet inflow rule (the ghost rules the tiles apply themselves),
+    from two-level grids to the 2048 x 1024 workload grid.  Prints the time
+    of a cycle in each form."""
+    from drd_application_amd.solver import GpuSolver
+    print(f"tiled vs per-pass V-cycle {nx}x{ny} lr {lr}", flush=True)
+    cfg = _channel(nx, ny)
+    cfg.params.lr_bd_type = ord(lr)
+    g = GpuSolver(cfg.params)
+    geometry.load(g, cfg)
+    n = abi.n_uv(cfg.params)
+    b = np.random.default_rng(11).standard_normal(n)
+    for prec, op in ((abi.PS_VELO_MG, abi.OP_VELO2), (abi.PS_VELO_MG32, abi.OP_VELO2),
+                     (abi.PS_VELO1_MG32, abi.OP_VELO1)):
+        g.mg_fused(True)
+        tiled = g.psolve(prec, op, b)
+        g.mg_fused(False)
+        passes = g.psolve(prec, op, b)
+        assert np.all(np.isfinite(tiled))
+        np.testing.assert_array_equal(tiled, passes)
+    for prec in (abi.PS_VELO_MG32, abi.PS_VELO_MG):
+        g.psolve(prec, abi.OP_VELO2, b)  # the context's hierarchy is the one timed
+        g.mg_fused(True)
+        t_tiled, _ = g.bench_kernel(abi.K_VELO_MG, 20)
+        g.mg_fused(False)
+        t_pass, _ = g.bench_kernel(abi.K_VELO_MG, 20)
+        print(f"prec {prec}: V-cycle tiled {1e3 * t_tiled:.1f} us, per pass {1e3 * t_pass:.1f} us", flush=True)
+    g.close()
