@@ -177,7 +177,7 @@ struct drd_ctx {
   float* vp_inv32 = nullptr;
   void* vp_work = nullptr;
   void* rb_work = nullptr;    // red-black sweeps: the iterate between sweeps
-  float* rb_c32 = nullptr;    // FP32 bu, M_inner, M_0 (nc each)
+  float* rb_c32 = nullptr;    // FP32 bu, M_inner, M_0 and block factors d11, d22, 1/det (nc each)
   double cosa = 0, sqrt2 = 0;
   Learn* learn = nullptr;  // drd_learn_init
   // drd_bicgstab_cb: the caller's MATVEC / PSOLVE, its vector length, and
@@ -459,7 +459,7 @@ static int ctx_alloc(drd_ctx* c) {
   add(&c->vp_inv32, g.nuv(), 4);
   add(&c->vp_work, 2 * g.nuv(), 8);
   add(&c->rb_work, 3 * g.ncell(), 8);  // 2 ncell doubles, or 6 ncell floats
-  add(&c->rb_c32, 3 * g.nc(), 4);
+  add(&c->rb_c32, 6 * g.nc(), 4);
   add(&c->cfB, p.nx + 1, 8);
   add(&c->cfT, p.nx + 1, 8);
   size_t S = 0;
@@ -867,7 +867,9 @@ static int apply_op(drd_ctx* c, int op, int mode, const double* x, double* y, do
 
 // Prepares a preconditioner once per solve (Jacobi diagonal, DCT scaling).
 static PhmuCoef32 phmu_c32(const drd_ctx* c) {
-  return PhmuCoef32{c->rb_c32, c->rb_c32 + c->g.nc(), c->rb_c32 + 2 * c->g.nc()};
+  const int64_t n = c->g.nc();
+  const float* f = c->rb_c32;
+  return PhmuCoef32{f, f + n, f + 2 * n, f + 3 * n, f + 4 * n, f + 5 * n};
 }
 
 static int prec_setup(drd_ctx* c, int prec) {
@@ -877,7 +879,7 @@ static int prec_setup(drd_ctx* c, int prec) {
   if (prec == DRD_PS_VELO1_JACOBI) launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, c->s.re);
   if (prec == DRD_PS_VELO2_JACOBI)
     launch_velo_jacobi_diag(c->stream, c->g, c->s, coef_of(c), c->diag, 1.5 * c->s.re);
-  if (prec == DRD_PS_PHMU_RBGS32) launch_phmu_coef32(c->stream, c->g, coef_of(c), phmu_c32(c));
+  if (prec == DRD_PS_PHMU_RBGS32) launch_phmu_coef32(c->stream, c->g, c->s, coef_of(c), phmu_c32(c));
   if (prec == DRD_PS_VELO_DCT32 || prec == DRD_PS_VELO_DCT64 || prec == DRD_PS_VELO_LEARNED) {
     const bool fp32 = (prec != DRD_PS_VELO_DCT64);
     if (c->vprec && c->vprec_fp32 != fp32) {

@@ -178,11 +178,12 @@ __device__ __forceinline__ R dgn_diag_v(const Grid& g, const Scal& s, const Nb5T
   return -(dx / R(s.dr2) + dy / R(s.dz2));
 }
 
-// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell, from
-// bu at the cell and the two mobilities around it (R as for dgn_diag_v)
+// The 2x2 block of cell (i, j) from bu at the cell and the two mobilities
+// around it (R as for dgn_diag_v): its diagonal d11, d22 and q = 1 / det
+// (FP32) or det (FP64).  The off-diagonal entries are -bv and bv + adj.
 template <class R = double>
-__device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, R bv, const Nb5T<R>& mi,
-                                             const Nb5T<R>& m0, int i, int j, R r1, R r2, R& x1, R& x2) {
+__device__ __forceinline__ void phmu_block_f(const Grid& g, const Scal& s, R bv, const Nb5T<R>& mi,
+                                             const Nb5T<R>& m0, int i, int j, R& d11o, R& d22o, R& qo) {
   const R d11 = R(s.s_I) - R(s.I) * dgn_diag_v<R>(g, s, mi, i, j) +
                 ((j == 0) ? R(s.I_dz2) * (R(1.0) - R(s.coef_bot)) : R(0.0)) +
                 ((j == g.ny - 1) ? R(s.I_dz2) * (R(1.0) - R(s.coef_top)) : R(0.0));
@@ -192,14 +193,30 @@ __device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, R bv,
   R d22 = -R(s.dt) * R(s.xld) * dgn_diag_v<R>(g, s, m0, i, j);
   d22 = d22 > R(s.floor22) ? d22 : R(s.floor22);
   const R det = d11 * d22 - d12 * d21;
+  d11o = d11;
+  d22o = d22;
+  qo = sizeof(R) == sizeof(float) ? R(1.0) / det : det;
+}
+// x = B^-1 (r1, r2) from the block's factors (phmu_block_f)
+template <class R = double>
+__device__ __forceinline__ void phmu_block_x(const Scal& s, R bv, R d11, R d22, R q, R r1, R r2, R& x1, R& x2) {
+  const R d12 = -bv;
+  const R d21 = bv + R(s.adj);
   if (sizeof(R) == sizeof(float)) {
-    const R idet = R(1.0) / det;
-    x1 = (d22 * r1 - d12 * r2) * idet;
-    x2 = (-d21 * r1 + d11 * r2) * idet;
+    x1 = (d22 * r1 - d12 * r2) * q;
+    x2 = (-d21 * r1 + d11 * r2) * q;
   } else {
-    x1 = (d22 * r1 - d12 * r2) / det;
-    x2 = (-d21 * r1 + d11 * r2) / det;
+    x1 = (d22 * r1 - d12 * r2) / q;
+    x2 = (-d21 * r1 + d11 * r2) / q;
   }
+}
+// x = B_q^-1 (r1, r2) for cell q (i, j), B_q the 2x2 block of the cell
+template <class R = double>
+__device__ __forceinline__ void phmu_block_v(const Grid& g, const Scal& s, R bv, const Nb5T<R>& mi,
+                                             const Nb5T<R>& m0, int i, int j, R r1, R r2, R& x1, R& x2) {
+  R d11, d22, q;
+  phmu_block_f<R>(g, s, bv, mi, m0, i, j, d11, d22, q);
+  phmu_block_x<R>(s, bv, d11, d22, q, r1, r2, x1, x2);
 }
 __device__ __forceinline__ void phmu_block_solve(const Grid& g, const Scal& s, const double* __restrict__ bu,
                                                  const double* __restrict__ minner,
@@ -223,10 +240,12 @@ void launch_interpolate_coef(hipStream_t st, const Grid& g, double dr, double dz
                              double leta, const double* pts, int np, double radius, int nswm,
                              double swm_radius, const double* bdr, double* fpd, double* rho, double* eta);
 // FP32 copies of the CH block coefficients (cell fields, ghosts included)
+// FP32 copies of bu, M_inner, M_0 (ghosted cell layout) and each interior
+// cell's block factors d11, d22, 1 / det (phmu_block_f<float>, same layout)
 struct PhmuCoef32 {
-  const float *bu, *minner, *m0;
+  const float *bu, *minner, *m0, *d11, *d22, *q;
 };
-void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32);
+void launch_phmu_coef32(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32& c32);
 // Event pairs for timing: group k (k < cap) of consecutive launches of the
 // instrumented kernel is bracketed by ev[2k] and ev[2k+1]; *n counts the
 // groups.

@@ -367,7 +367,7 @@ __device__ __forceinline__ R op_dgn_v(R idx2, R idy2, const Nb5T<R>& mo, const N
 // rows) from the values around it: phi and mu (f1, f2) and the two mobilities
 // (R = float: FP32 arithmetic and s.s / s.I * f1 for s.s * f1 / s.I, not the
 // reference's rounding; for the FP32 preconditioner only)
-template <class R = double>
+template <class R = double, bool WALLS = true>
 __device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, R b, const Nb5T<R>& mi,
                                             const Nb5T<R>& m0, const Nb5T<R>& f1, const Nb5T<R>& f2, R& o1,
                                             R& o2) {
@@ -375,8 +375,8 @@ __device__ __forceinline__ void phmu_rows_v(const Grid g, const Scal& s, int j, 
   R ff1 = op_dgn_v<R>(R(s.idr2), R(s.idz2), mi, f1);
   ff1 = (kF32 ? R(s.s_I) * f1.c : R(s.s) * f1.c / R(s.I)) - R(s.I) * ff1 - f2.c * b;
   // wall rows: ghost f1(i,-1) = f1(i,0) from btoA_ph_mu (:229-232)
-  if (j == 0) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_bot) * f1.c);
-  if (j == g.ny - 1) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_top) * f1.c);
+  if (WALLS && j == 0) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_bot) * f1.c);
+  if (WALLS && j == g.ny - 1) ff1 = ff1 + R(s.I_dz2) * (f1.c - R(s.coef_top) * f1.c);
   R ff2 = op_dgn_v<R>(R(s.idr2), R(s.idz2), m0, f2);
   ff2 = f1.c * (b + R(s.adj)) - R(s.dt) * R(s.xld) * ff2;
   o1 = ff1;
@@ -631,14 +631,25 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 // tile is held in LDS and computed in TC: FP64 (the reference's arithmetic),
 // or FP32 (the FP32 preconditioner: FP32 arithmetic with reciprocals), where
 // bc, if given, receives the FP32 b of the tile for the later launches.
+// FP32 tiles also stage each cell's block factors (d11, d22, 1/det of
+// phmu_block_f, formed once per solve by k_phmu_coef32) instead of forming
+// them at every update.  A tile whose cells and neighbours all lie at least
+// one cell from the walls and side columns (interior) takes the update
+// without the clamped neighbour indices and wall rows, which are identities
+// there.
 constexpr int kRbTX = 64, kRbTY = 16, kRbThreads = 1024;  // a thread per cell of the widest phase
 template <bool FIRST, int NS, class TC, class TB, class TI, class TO>
 __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
                                                           const TC* __restrict__ minner,
                                                           const TC* __restrict__ m0, const TB* __restrict__ b,
                                                           float* __restrict__ bc, const TI* __restrict__ xi,
-                                                          TO* __restrict__ xo, const KState* st, int64_t bs) {
+                                                          TO* __restrict__ xo, const KState* st, int64_t bs,
+                                                          const float* __restrict__ fd11,
+                                                          const float* __restrict__ fd22,
+                                                          const float* __restrict__ fq) {
   constexpr int kH = 2 * NS, kW = kRbTX + 2 * kH, kHt = kRbTY + 2 * kH;
+  constexpr bool BF = sizeof(TC) == sizeof(float);  // staged block factors
+  constexpr int kF = BF ? kHt : 1, kFW = BF ? kW : 1;
   st = bofs(st, bs);
   if (st && (st->done || st->early)) return;
   bu = bofs(bu, bs);
@@ -648,8 +659,12 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
   bc = bofs(bc, bs);
   xi = bofs(xi, bs);
   xo = bofs(xo, bs);
+  fd11 = bofs(fd11, bs);
+  fd22 = bofs(fd22, bs);
+  fq = bofs(fq, bs);
   __shared__ TC sx1[kHt][kW], sx2[kHt][kW];
   __shared__ TC smi[kHt][kW], smz[kHt][kW], sbu[kHt][kW], sb1[kHt][kW], sb2[kHt][kW];
+  __shared__ float sd11[kF][kFW], sd22[kF][kFW], sq[kF][kFW];
   const int nx = g.nx, ny = g.ny;
   const int64_t C = g.ncell();
   // tiles in row-major order, a contiguous band of them per XCD (tile_id),
@@ -658,21 +673,28 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
   const int t = (int)tile_id();
   const int i0 = (t % ntx) * kRbTX, j0 = (t / ntx) * kRbTY;
   // staging: every load of the thread is issued before the first LDS store
-  // (indices clamped into the grid; the stores keep the cells inside it)
+  // (indices clamped into the grid; the stores keep the cells inside it),
+  // held in registers as TC (the stored values are (TC) of the loads)
   constexpr int kStage = kW * kHt, kStageIt = (kStage + kRbThreads - 1) / kRbThreads;
-  double v[kStageIt][7];
+  constexpr int kV = BF ? 10 : 7;
+  TC v[kStageIt][kV];
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
     const int e = min((int)threadIdx.x + it * kRbThreads, kStage - 1);
     const int i = min(max(i0 - kH + e % kW, 0), nx - 1), j = min(max(j0 - kH + e / kW, 0), ny - 1);
     const int64_t q = i + (int64_t)nx * j, c = g.ci(i, j);
-    v[it][0] = FIRST ? 0.0 : (double)xi[q];
-    v[it][1] = FIRST ? 0.0 : (double)xi[C + q];
-    v[it][2] = (double)minner[c];
-    v[it][3] = (double)m0[c];
-    v[it][4] = (double)bu[c];
-    v[it][5] = (double)b[q];
-    v[it][6] = (double)b[C + q];
+    v[it][0] = FIRST ? TC(0) : (TC)xi[q];
+    v[it][1] = FIRST ? TC(0) : (TC)xi[C + q];
+    v[it][2] = (TC)minner[c];
+    v[it][3] = (TC)m0[c];
+    v[it][4] = (TC)bu[c];
+    v[it][5] = (TC)b[q];
+    v[it][6] = (TC)b[C + q];
+    if (BF) {
+      v[it][kV - 3] = (TC)fd11[c];
+      v[it][kV - 2] = (TC)fd22[c];
+      v[it][kV - 1] = (TC)fq[c];
+    }
   }
 #pragma unroll
   for (int it = 0; it < kStageIt; ++it) {
@@ -680,18 +702,25 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
     const int li = e % kW, lj = e / kW;
     const int i = i0 - kH + li, j = j0 - kH + lj;
     if (e >= kStage || i < 0 || i >= nx || j < 0 || j >= ny) continue;
-    sx1[lj][li] = (TC)v[it][0];
-    sx2[lj][li] = (TC)v[it][1];
-    smi[lj][li] = (TC)v[it][2];
-    smz[lj][li] = (TC)v[it][3];
-    sbu[lj][li] = (TC)v[it][4];
-    sb1[lj][li] = (TC)v[it][5];
-    sb2[lj][li] = (TC)v[it][6];
+    sx1[lj][li] = v[it][0];
+    sx2[lj][li] = v[it][1];
+    smi[lj][li] = v[it][2];
+    smz[lj][li] = v[it][3];
+    sbu[lj][li] = v[it][4];
+    sb1[lj][li] = v[it][5];
+    sb2[lj][li] = v[it][6];
+    if (BF) {
+      sd11[BF ? lj : 0][BF ? li : 0] = (float)v[it][kV - 3];
+      sd22[BF ? lj : 0][BF ? li : 0] = (float)v[it][kV - 2];
+      sq[BF ? lj : 0][BF ? li : 0] = (float)v[it][kV - 1];
+    }
   }
   __syncthreads();
-  auto update = [&](int i, int j) {
-    const int il = i > 0 ? i - 1 : i, ir = i < nx - 1 ? i + 1 : i;
-    const int jl = j > 0 ? j - 1 : j, jr = j < ny - 1 ? j + 1 : j;
+  const bool interior = i0 - kH >= 0 && i0 + kRbTX + kH <= nx && j0 - kH >= 0 && j0 + kRbTY + kH <= ny;
+  auto update = [&](int i, int j, auto in) {
+    constexpr bool IN = decltype(in)::value;
+    const int il = (IN || i > 0) ? i - 1 : i, ir = (IN || i < nx - 1) ? i + 1 : i;
+    const int jl = (IN || j > 0) ? j - 1 : j, jr = (IN || j < ny - 1) ? j + 1 : j;
     const int a = i - i0 + kH, al = il - i0 + kH, ar = ir - i0 + kH;
     const int r = j - j0 + kH, rl = jl - j0 + kH, rr = jr - j0 + kH;
     const Nb5T<TC> mi{smi[r][a], smi[r][al], smi[r][ar], smi[rl][a], smi[rr][a]};
@@ -700,28 +729,38 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
     const Nb5T<TC> f2{sx2[r][a], sx2[r][al], sx2[r][ar], sx2[rl][a], sx2[rr][a]};
     const TC bv = sbu[r][a];
     TC z1, z2, y1, y2;
-    phmu_rows_v<TC>(g, s, j, bv, mi, mz, f1, f2, z1, z2);
-    phmu_block_v<TC>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
+    phmu_rows_v<TC, !IN>(g, s, j, bv, mi, mz, f1, f2, z1, z2);
+    if constexpr (BF)
+      phmu_block_x<TC>(s, bv, (TC)sd11[r][a], (TC)sd22[r][a], (TC)sq[r][a], sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
+    else
+      phmu_block_v<TC>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
     sx1[r][a] = f1.c + y1;
     sx2[r][a] = f2.c + y2;
   };
   // threads take the cells of one colour in row order: a row of a region
   // kRbTX + 2 w wide holds (kRbTX + 2 w) / 2 cells of each colour
+  auto sweeps = [&](auto in) {
+    constexpr bool IN = decltype(in)::value;
 #pragma unroll
-  for (int k = 0; k < NS; ++k) {
+    for (int k = 0; k < NS; ++k) {
 #pragma unroll
-    for (int colour = 0; colour < 2; ++colour) {
-      const int w = 2 * (NS - k) - 1 - colour;  // ring width of this phase
-      const int rowc = (kRbTX + 2 * w) / 2, rows = kRbTY + 2 * w;
-      for (int e = threadIdx.x; e < rowc * rows; e += kRbThreads) {
-        const int j = j0 - w + e / rowc;
-        const int i = i0 - w + 2 * (e % rowc) + ((i0 - w + j + colour) & 1);
-        if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
-        update(i, j);
+      for (int colour = 0; colour < 2; ++colour) {
+        const int w = 2 * (NS - k) - 1 - colour;  // ring width of this phase
+        const int rowc = (kRbTX + 2 * w) / 2, rows = kRbTY + 2 * w;
+        for (int e = threadIdx.x; e < rowc * rows; e += kRbThreads) {
+          const int j = j0 - w + e / rowc;
+          const int i = i0 - w + 2 * (e % rowc) + ((i0 - w + j + colour) & 1);
+          if (!IN && (i < 0 || i >= nx || j < 0 || j >= ny)) continue;
+          update(i, j, in);
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
+  };
+  if (interior)
+    sweeps(std::true_type{});
+  else
+    sweeps(std::false_type{});
   for (int e = threadIdx.x; e < kRbTX * kRbTY; e += kRbThreads) {
     const int i = i0 + e % kRbTX, j = j0 + e / kRbTX;
     if (i >= nx || j >= ny) continue;
@@ -950,22 +989,38 @@ __global__ __launch_bounds__(kBlock) void k_body_spread(Grid g, double dr, doubl
 }
 
 // FP32 copies of the CH block coefficients bu, M_inner, M_0 (ghosts included)
-__global__ __launch_bounds__(kBlock) void k_phmu_coef32(int64_t n, const double* __restrict__ bu,
+// and, at the interior cells, the FP32 block factors of phmu_block_f<float>
+// from those copies (the red-black sweeps read them instead of forming them
+// at every update)
+__global__ __launch_bounds__(kBlock) void k_phmu_coef32(Grid g, Scal s, const double* __restrict__ bu,
                                                         const double* __restrict__ mi,
-                                                        const double* __restrict__ m0, float* __restrict__ bu32,
-                                                        float* __restrict__ mi32, float* __restrict__ m032,
-                                                        int64_t bs) {
+                                                        const double* __restrict__ m0, PhmuCoef32 o, int64_t bs) {
   bu = bofs(bu, bs);
   mi = bofs(mi, bs);
   m0 = bofs(m0, bs);
-  bu32 = bofs(bu32, bs);
-  mi32 = bofs(mi32, bs);
-  m032 = bofs(m032, bs);
+  float* bu32 = bofs(const_cast<float*>(o.bu), bs);
+  float* mi32 = bofs(const_cast<float*>(o.minner), bs);
+  float* m032 = bofs(const_cast<float*>(o.m0), bs);
+  float* d11 = bofs(const_cast<float*>(o.d11), bs);
+  float* d22 = bofs(const_cast<float*>(o.d22), bs);
+  float* qq = bofs(const_cast<float*>(o.q), bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (q >= n) return;
+  if (q >= g.nc()) return;
   bu32[q] = (float)bu[q];
   mi32[q] = (float)mi[q];
   m032[q] = (float)m0[q];
+  const int ld = g.nx + 2;
+  const int j = (int)(q / ld) - 1, i = (int)(q - (int64_t)(j + 1) * ld) - 1;
+  float f11 = 0.f, f22 = 0.f, fq = 0.f;
+  if (i >= 0 && i < g.nx && j >= 0 && j < g.ny) {
+    const Nb5 a = nb5_cell(g, mi, i, j), z = nb5_cell(g, m0, i, j);
+    const Nb5T<float> af{(float)a.c, (float)a.xl, (float)a.xr, (float)a.yl, (float)a.yr};
+    const Nb5T<float> zf{(float)z.c, (float)z.xl, (float)z.xr, (float)z.yl, (float)z.yr};
+    phmu_block_f<float>(g, s, (float)bu[q], af, zf, i, j, f11, f22, fq);
+  }
+  d11[q] = f11;
+  d22[q] = f22;
+  qq[q] = fq;
 }
 
 // ---- multigrid pieces for the BDF2 velocity operator ----------------------
@@ -1385,7 +1440,7 @@ static dim3 rb_grid(const Grid& g) {  // one workgroup per tile (tile_id)
 void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const PhmuCoef32& c32, const float* b,
                             const float* xi, float* xo, const KState* ks) {
   hipLaunchKernelGGL((k_phmu_rb_sweep<false, kRbNS, float, float, float, float>), rb_grid(g), dim3(kRbThreads), 0, st,
-                     g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs());
+                     g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs(), c32.d11, c32.d22, c32.q);
 }
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
 // eight full sweeps (red cells, then black), from x = 0, kRbNS per launch,
@@ -1401,7 +1456,11 @@ void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& 
   const dim3 grid = rb_grid(g);
 #define RB(F, TC, TB, TI, TO, CO, B, BC, XI, XO)                                                                 \
   hipLaunchKernelGGL((k_phmu_rb_sweep<F, kRbNS, TC, TB, TI, TO>), grid, dim3(kRbThreads), 0, st, g, s, CO->bu,         \
-                     CO->minner, CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs())
+                     CO->minner, CO->m0, (const TB*)(B), (float*)(BC), (const TI*)(XI), (TO*)(XO), ks, bat_bs(), \
+                     fd11, fd22, fq)
+  const float* fd11 = c32 ? c32->d11 : nullptr;  // the FP32 sweeps' block factors
+  const float* fd22 = c32 ? c32->d22 : nullptr;
+  const float* fq = c32 ? c32->q : nullptr;
   // FP32 storage: two iterates and the FP32 b
   const int64_t n2 = 2 * g.ncell();
   float* wa = (float*)work;
@@ -1468,10 +1527,9 @@ void launch_body_spread(hipStream_t st, const Grid& g, double dr, double dz, dou
   hipLaunchKernelGGL(k_body_spread, grid, dim3(kBlock), 0, st, g, dr, dz, I, lx, ly, bodies, nbody, fin, pf1, pf2);
 }
 
-void launch_phmu_coef32(hipStream_t st, const Grid& g, const Coef& c, const PhmuCoef32& c32) {
-  hipLaunchKernelGGL(k_phmu_coef32, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g.nc(), c.bu,
-                     c.minner, c.m0, const_cast<float*>(c32.bu), const_cast<float*>(c32.minner),
-                     const_cast<float*>(c32.m0), bat_bs());
+void launch_phmu_coef32(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32& c32) {
+  hipLaunchKernelGGL(k_phmu_coef32, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g, s, c.bu,
+                     c.minner, c.m0, c32, bat_bs());
 }
 
 }  // namespace drd
