@@ -104,14 +104,32 @@ struct MgCoefStage {
   }
 };
 
-// Positions of the block: slot j of thread t is position j * kMgfThreads + t.
+// Positions of the block: slot j of thread t is position j * kMgfThreads + t,
+// each held as one packed word (i + 8 and k + 8 in 14 bits each, the ring
+// clamped to 7 in 3 bits, the u flag) to keep the slots' registers few.
 template <int TW, int TH, int H>
 struct MgSlots {
   using B = MgBlock<TW, TH, H>;
-  MgPos m[B::NS];
-  __device__ __forceinline__ MgSlots(int ci0, int ck0, int nx, int ny) {
+  uint32_t w[B::NS];
+  int ci0, nx;
+  __device__ __forceinline__ MgSlots(int ci0_, int ck0, int nx_, int ny) : ci0(ci0_), nx(nx_) {
 #pragma unroll
-    for (int j = 0; j < B::NS; ++j) m[j] = mg_pos<TW, TH, H>(j * kMgfThreads + (int)threadIdx.x, ci0, ck0, nx, ny);
+    for (int j = 0; j < B::NS; ++j) {
+      const MgPos m = mg_pos<TW, TH, H>(j * kMgfThreads + (int)threadIdx.x, ci0, ck0, nx, ny);
+      const uint32_t ring = (uint32_t)(m.ring < 7 ? m.ring : 7);
+      w[j] = (uint32_t)((m.i + 8) & 0x3fff) | ((uint32_t)((m.k + 8) & 0x3fff) << 14) | (ring << 28) |
+             (m.u ? 0x80000000u : 0u);
+    }
+  }
+  __device__ __forceinline__ MgPos operator[](int j) const {
+    MgPos m;
+    const uint32_t v = w[j];
+    m.i = (int)(v & 0x3fff) - 8;
+    m.k = (int)((v >> 14) & 0x3fff) - 8;
+    m.ring = (int)((v >> 28) & 7);
+    m.u = (v >> 31) != 0;
+    m.own = m.ring == 0 && (!m.u || m.i < ci0 + TW || m.i == nx);
+    return m;
   }
 };
 
@@ -119,7 +137,7 @@ struct MgSlots {
 // over sa, restriction into bc.  XV is the LDS vector view (ghost rules, or
 // none in an interior block).
 template <class XV, class T, class TBC, int TW, int TH>
-__device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, const Scal& s, const MgSlots<TW, TH, 3>& P,
+__device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, const ScalT<T>& s, const MgSlots<TW, TH, 3>& P,
                                                const CoefL<T>& C, const XV& XA, const XV& XB, T* sa, T* sb,
                                                const T (&bv)[MgBlock<TW, TH, 3>::NS],
                                                const T (&wv)[MgBlock<TW, TH, 3>::NS], T* __restrict__ xd,
@@ -128,7 +146,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
   // x2 = x1 + (b - A x1) omega / d at rings <= 2
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     if (m.ring > 2) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
@@ -140,7 +158,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
   // r = b - A x2 at rings <= 1, over x1 (no longer read)
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
@@ -178,7 +196,7 @@ __device__ __forceinline__ bool mg_interior(const Grid& g, int ci0, int ck0) {
 // Down leg of level g (coarse level gc): x2 to xd, bc = R (b - A x2); bcopy
 // (if given) receives b as FP32 at the owned unknowns.
 template <class T, class TB, class TBC, int TW, int TH>
-__global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
+__global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) void k_mg_down(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
                                                          const T* __restrict__ eta, const T* __restrict__ chi,
                                                          const TB* __restrict__ b, const T* __restrict__ dw,
                                                          T* __restrict__ xd, float* __restrict__ bcopy,
@@ -200,7 +218,7 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
   T wv[B::NS];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     const uint32_t q = m.ring <= 3 ? mg_packed(g, m.u, m.i, m.k) : 0u;
     braw[j] = ld32(b, q);
     wv[j] = ld32(dw, q);
@@ -211,7 +229,7 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
   T bv[B::NS];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     bv[j] = (T)braw[j];
     if (m.ring > 3) continue;
     sa[j * kMgfThreads + (int)threadIdx.x] = (T)((double)braw[j] * (double)wv[j]);
@@ -219,21 +237,22 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
   }
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
+  const ScalT<T> sT = ScalT<T>::make(s);
   if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
     const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
     const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
-    mg_down_stages(g, gc, s, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
+    mg_down_stages(g, gc, sT, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
   } else {
     const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
     const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    mg_down_stages(g, gc, s, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
+    mg_down_stages(g, gc, sT, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
   }
 }
 
 // Stages of the up leg after the first (x2 + P xc in sa): a sweep into sb,
 // a sweep into out.
 template <class XV, class T, class TO, int TW, int TH>
-__device__ __forceinline__ void mg_up_stages(const Grid& g, const Scal& s, const MgSlots<TW, TH, 2>& P,
+__device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, const MgSlots<TW, TH, 2>& P,
                                              const CoefL<T>& C, const XV& XA, const XV& XB, T* sa, T* sb,
                                              const T (&bv)[MgBlock<TW, TH, 2>::NS],
                                              const T (&wv)[MgBlock<TW, TH, 2>::NS], TO* __restrict__ out) {
@@ -241,7 +260,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const Scal& s, const
   // first sweep at rings <= 1
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
@@ -251,7 +270,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const Scal& s, const
   // second sweep at the owned unknowns
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     if (!m.own) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
@@ -262,7 +281,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const Scal& s, const
 // Up leg of level g (coarse level gc): x = xd + P xc, two damped Jacobi
 // sweeps, the second written to out.
 template <class T, class TB, class TC, class TO, int TW, int TH>
-__global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
+__global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) void k_mg_up(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
                                                        const T* __restrict__ eta, const T* __restrict__ chi,
                                                        const TB* __restrict__ b, const T* __restrict__ dw,
                                                        const T* __restrict__ xd, const TC* __restrict__ xc,
@@ -283,7 +302,7 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
   TC xcv[B::NS][4];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     const bool in = m.ring <= 2;
     const uint32_t q = in ? mg_packed(g, m.u, m.i, m.k) : 0u;
     xdv[j] = ld32(xd, q);
@@ -302,7 +321,7 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
   // x = x2 + P xc at rings <= 2
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos& m = P.m[j];
+    const MgPos m = P[j];
     if (m.ring > 2) continue;
     const double xv[4] = {(double)xcv[j][0], (double)xcv[j][1], (double)xcv[j][2], (double)xcv[j][3]};
     const double acc = m.u ? mg_prolong_u_sum(gc, m.i, m.k, xv) : mg_prolong_v_sum(gc, m.i, m.k, xv);
@@ -310,14 +329,15 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
   }
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
+  const ScalT<T> sT = ScalT<T>::make(s);
   if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
     const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
     const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
-    mg_up_stages(g, s, P, C, XA, XB, sa, sb, bv, wv, out);
+    mg_up_stages(g, sT, P, C, XA, XB, sa, sb, bv, wv, out);
   } else {
     const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
     const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    mg_up_stages(g, s, P, C, XA, XB, sa, sb, bv, wv, out);
+    mg_up_stages(g, sT, P, C, XA, XB, sa, sb, bv, wv, out);
   }
 }
 

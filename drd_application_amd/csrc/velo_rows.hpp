@@ -136,13 +136,23 @@ struct CoefL {
 // Computed in the coefficients' type T: FP64 is the reference's arithmetic;
 // FP32 (the FP32 multigrid levels) multiplies by the reciprocals of the
 // spacings where the reference divides.  The vector's loader reads T.
-template <class T>
-__device__ __forceinline__ T vdiv(T a, const Scal& s, double d, double inv) {
+template <class T, class D>
+__device__ __forceinline__ T vdiv(T a, D d, D inv) {
   return sizeof(T) == sizeof(float) ? a * T(inv) : a / T(d);
 }
+// The scalars a stencil reads, converted once to the stencil's type T (the
+// same values T(s.re), ... that the stencil would form from Scal; FP32
+// kernels convert them at their start instead of at every unknown).
+template <class T>
+struct ScalT {
+  T re, penalty_k, dt, dr, dz, dr2, dz2, idr2, idz2, idrdz;
+  __device__ __forceinline__ static ScalT make(const Scal& s) {
+    return ScalT{T(s.re), T(s.penalty_k), T(s.dt), T(s.dr), T(s.dz), T(s.dr2), T(s.dz2), T(s.idr2), T(s.idz2), T(s.idrdz)};
+  }
+};
 // u row at face (i, k)
-template <int BDF, class T, class XA, class CV>
-__device__ __forceinline__ T velo_u_at(const Scal& s, const CV& C, const XA& X, int i, int k) {
+template <int BDF, class T, class XA, class CV, class S>
+__device__ __forceinline__ T velo_u_at(const S& s, const CV& C, const XA& X, int i, int k) {
   T r;
   const uint32_t c = C.at(i, k), cw = c - 1, ld = C.ld;
   const T f = X.U(i, k);
@@ -152,17 +162,17 @@ __device__ __forceinline__ T velo_u_at(const Scal& s, const CV& C, const XA& X, 
     r = f * T(s.re) * (C.r(c) + C.r(cw)) / T(2.0);
   const T e00 = C.e(c), e10 = C.e(cw);
   T cr = e00, cl = e10;
-  r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
+  r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.U(i + 1, k) + cl * X.U(i - 1, k) - (cl + cr) * f), s.dr2, s.idr2);
   cr = (C.e(c + ld) + C.e(cw + ld) + e00 + e10) / T(4.0);
   cl = (e00 + e10 + C.e(c - ld) + C.e(cw - ld)) / T(4.0);
-  r = r - vdiv(T(s.dt) * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
+  r = r - vdiv(T(s.dt) * (cr * X.U(i, k + 1) + cl * X.U(i, k - 1) - (cl + cr) * f), s.dz2, s.idz2);
   const T cross = T(s.dt) * (cr * (X.V(i, k + 1) - X.V(i - 1, k + 1)) - cl * (X.V(i, k) - X.V(i - 1, k)));
   r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
   return r;
 }
 // v row at face (i, k)
-template <int BDF, class T, class XA, class CV>
-__device__ __forceinline__ T velo_v_at(const Scal& s, const CV& C, const XA& X, int i, int k) {
+template <int BDF, class T, class XA, class CV, class S>
+__device__ __forceinline__ T velo_v_at(const S& s, const CV& C, const XA& X, int i, int k) {
   T r;
   const uint32_t c = C.at(i, k), ld = C.ld, cs = c - ld;
   const T f = X.V(i, k);
@@ -172,10 +182,10 @@ __device__ __forceinline__ T velo_v_at(const Scal& s, const CV& C, const XA& X, 
     r = f * T(s.re) * (C.r(c) + C.r(cs)) / T(2.0);
   const T e00 = C.e(c), e01 = C.e(cs);
   T cr = e00, cl = e01;
-  r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f), s, s.dz2, s.idz2);
+  r = r - vdiv(T(2.0) * T(s.dt) * (cr * X.V(i, k + 1) + cl * X.V(i, k - 1) - (cl + cr) * f), s.dz2, s.idz2);
   cr = (C.e(c + 1) + e00 + C.e(cs + 1) + e01) / T(4.0);
   cl = (e00 + C.e(c - 1) + e01 + C.e(cs - 1)) / T(4.0);
-  r = r - vdiv(T(s.dt) * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f), s, s.dr2, s.idr2);
+  r = r - vdiv(T(s.dt) * (cr * X.V(i + 1, k) + cl * X.V(i - 1, k) - (cl + cr) * f), s.dr2, s.idr2);
   const T cross = T(s.dt) * (cr * (X.U(i + 1, k) - X.U(i + 1, k - 1)) - cl * (X.U(i, k) - X.U(i, k - 1)));
   r = r - (sizeof(T) == sizeof(float) ? cross * T(s.idrdz) : cross / T(s.dr) / T(s.dz));
   return r;
