@@ -104,32 +104,14 @@ struct MgCoefStage {
   }
 };
 
-// Positions of the block: slot j of thread t is position j * kMgfThreads + t,
-// each held as one packed word (i + 8 and k + 8 in 14 bits each, the ring
-// clamped to 7 in 3 bits, the u flag) to keep the slots' registers few.
+// Positions of the block: slot j of thread t is position j * kMgfThreads + t.
 template <int TW, int TH, int H>
 struct MgSlots {
   using B = MgBlock<TW, TH, H>;
-  uint32_t w[B::NS];
-  int ci0, nx;
-  __device__ __forceinline__ MgSlots(int ci0_, int ck0, int nx_, int ny) : ci0(ci0_), nx(nx_) {
+  MgPos m[B::NS];
+  __device__ __forceinline__ MgSlots(int ci0, int ck0, int nx, int ny) {
 #pragma unroll
-    for (int j = 0; j < B::NS; ++j) {
-      const MgPos m = mg_pos<TW, TH, H>(j * kMgfThreads + (int)threadIdx.x, ci0, ck0, nx, ny);
-      const uint32_t ring = (uint32_t)(m.ring < 7 ? m.ring : 7);
-      w[j] = (uint32_t)((m.i + 8) & 0x3fff) | ((uint32_t)((m.k + 8) & 0x3fff) << 14) | (ring << 28) |
-             (m.u ? 0x80000000u : 0u);
-    }
-  }
-  __device__ __forceinline__ MgPos operator[](int j) const {
-    MgPos m;
-    const uint32_t v = w[j];
-    m.i = (int)(v & 0x3fff) - 8;
-    m.k = (int)((v >> 14) & 0x3fff) - 8;
-    m.ring = (int)((v >> 28) & 7);
-    m.u = (v >> 31) != 0;
-    m.own = m.ring == 0 && (!m.u || m.i < ci0 + TW || m.i == nx);
-    return m;
+    for (int j = 0; j < B::NS; ++j) m[j] = mg_pos<TW, TH, H>(j * kMgfThreads + (int)threadIdx.x, ci0, ck0, nx, ny);
   }
 };
 
@@ -146,7 +128,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
   // x2 = x1 + (b - A x1) omega / d at rings <= 2
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     if (m.ring > 2) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
@@ -158,7 +140,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
   // r = b - A x2 at rings <= 1, over x1 (no longer read)
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
@@ -196,7 +178,7 @@ __device__ __forceinline__ bool mg_interior(const Grid& g, int ci0, int ck0) {
 // Down leg of level g (coarse level gc): x2 to xd, bc = R (b - A x2); bcopy
 // (if given) receives b as FP32 at the owned unknowns.
 template <class T, class TB, class TBC, int TW, int TH>
-__global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) void k_mg_down(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
+__global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
                                                          const T* __restrict__ eta, const T* __restrict__ chi,
                                                          const TB* __restrict__ b, const T* __restrict__ dw,
                                                          T* __restrict__ xd, float* __restrict__ bcopy,
@@ -218,7 +200,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   T wv[B::NS];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     const uint32_t q = m.ring <= 3 ? mg_packed(g, m.u, m.i, m.k) : 0u;
     braw[j] = ld32(b, q);
     wv[j] = ld32(dw, q);
@@ -229,7 +211,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   T bv[B::NS];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     bv[j] = (T)braw[j];
     if (m.ring > 3) continue;
     sa[j * kMgfThreads + (int)threadIdx.x] = (T)((double)braw[j] * (double)wv[j]);
@@ -237,7 +219,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   }
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
-  const ScalT<T> sT = ScalT<T>::make(s);
+  const ScalT<T> sT = ScalT<T>::make(s);  // converted once, not at every unknown
   if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
     const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
     const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
@@ -260,7 +242,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, c
   // first sweep at rings <= 1
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
@@ -270,7 +252,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, c
   // second sweep at the owned unknowns
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     if (!m.own) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
     const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
@@ -281,7 +263,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, c
 // Up leg of level g (coarse level gc): x = xd + P xc, two damped Jacobi
 // sweeps, the second written to out.
 template <class T, class TB, class TC, class TO, int TW, int TH>
-__global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) void k_mg_up(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
+__global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, const T* __restrict__ rho,
                                                        const T* __restrict__ eta, const T* __restrict__ chi,
                                                        const TB* __restrict__ b, const T* __restrict__ dw,
                                                        const T* __restrict__ xd, const TC* __restrict__ xc,
@@ -302,7 +284,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   TC xcv[B::NS][4];
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     const bool in = m.ring <= 2;
     const uint32_t q = in ? mg_packed(g, m.u, m.i, m.k) : 0u;
     xdv[j] = ld32(xd, q);
@@ -321,7 +303,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   // x = x2 + P xc at rings <= 2
 #pragma unroll
   for (int j = 0; j < B::NS; ++j) {
-    const MgPos m = P[j];
+    const MgPos& m = P.m[j];
     if (m.ring > 2) continue;
     const double xv[4] = {(double)xcv[j][0], (double)xcv[j][1], (double)xcv[j][2], (double)xcv[j][3]};
     const double acc = m.u ? mg_prolong_u_sum(gc, m.i, m.k, xv) : mg_prolong_v_sum(gc, m.i, m.k, xv);
@@ -329,7 +311,7 @@ __global__ __launch_bounds__(kMgfThreads, sizeof(T) == sizeof(float) ? 5 : 4) vo
   }
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
-  const ScalT<T> sT = ScalT<T>::make(s);
+  const ScalT<T> sT = ScalT<T>::make(s);  // converted once, not at every unknown
   if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
     const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
     const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
