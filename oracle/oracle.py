@@ -27,7 +27,8 @@ def build(quiet: bool = True) -> None:
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # DRD_ORACLE_LIB: another build of the same source (the sanitizer build)
+        path = os.environ.get("DRD_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
