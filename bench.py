@@ -491,7 +491,8 @@ def combined_c4(steps: int = 20, warmup: int = 2):
             "mean_iters": {"phmu": sum(a for a, _ in its) / steps, "velo": sum(b for _, b in its) / steps}}
 
 
-def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4, graphs: bool = True):
+def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4, graphs: bool = True,
+               velo: str = "PREC_MG32", sequential: bool = True):
     """The batched workload: `count` independent instances of uneven sizes
     (geometry.batch_configs) stepped together by drd_batch_step, every size
     group's kernels launched once over all its instances, BiCGSTAB iteration
@@ -499,7 +500,8 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     sequential figure steps up to `sample` instances of each size alone (each
     its own context, no graphs) and scales their mean by the number of
     instances of that size.  The CH preconditioner is the FP32 red-black one
-    (the configs' own choice is block Jacobi)."""
+    (the configs' own choice is block Jacobi); the velocity preconditioner
+    `velo` (default the FP32 multigrid; PREC_DCT32 the FP32 fast-Poisson)."""
     import numpy as np
 
     from drd_application_amd import abi, geometry
@@ -509,6 +511,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     cfgs = geometry.batch_configs(count)
     for c in cfgs:
         c.params.ch_prec = abi.CH_PREC_RBGS32
+        c.params.velo_prec = getattr(abi, velo)
     gen_s = time.perf_counter() - t0
     cells = sum(c.params.nx * c.params.ny for c in cfgs)
     dt = cfgs[0].params.dt
@@ -522,7 +525,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     load()
     b.set_initial_guess(1)
     tuned = b.tune()
-    log(f"[c2] {count} instances, {cells/1e6:.1f} M cells, groups {b.groups()} (generated in {gen_s:.1f} s); "
+    log(f"[c2] {velo}: {count} instances, {cells/1e6:.1f} M cells, groups {b.groups()} (generated in {gen_s:.1f} s); "
         f"tiles {[(t[0], t[2]) for t in tuned]}")
     t = 0.0
 
@@ -567,7 +570,7 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     # sequential estimate: one instance per size, its own context
     seq = 0.0
     per_size = {}
-    for nx in sorted({c.params.nx for c in cfgs}):
+    for nx in sorted({c.params.nx for c in cfgs}) if sequential else []:
         group = [c for c in cfgs if c.params.nx == nx]
         times = []
         for c in group[:sample]:
@@ -592,12 +595,14 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     log(f"[c2] sequential estimate {seq:.0f} ms/step ({per_size})")
     return {
         "workload": f"{count} instances, nx log-uniform over powers of two 128..1024, ny = nx/2, eta ratio "
-                    f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, FP32 fast-Poisson velocity and FP32 red-black CH preconditioners, "
-                    f"FP64 BiCGSTAB, tol 1e-10",
+                    f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, "
+                    f"{'FP32 multigrid' if velo == 'PREC_MG32' else velo} velocity and FP32 red-black CH "
+                    f"preconditioners, FP64 BiCGSTAB, tol 1e-10",
         "cells": cells, "groups": [list(gr) for gr in tuned],
         "ms_per_batch_step": el / steps * 1e3, "instance_steps_per_s": count * steps / el,
         "cell_steps_per_s": cells * steps / el, "ms_per_batch_step_graphs": el_g * 1e3 if el_g else None,
-        "sequential_ms_per_step_estimate": seq, "speedup_vs_sequential": seq / (el / steps * 1e3),
+        "sequential_ms_per_step_estimate": seq if sequential else None,
+        "speedup_vs_sequential": seq / (el / steps * 1e3) if sequential else None,
         "mean_iters": {"velo": iv, "phmu": ic}, "non_converged_solves": bad,
         "per_size_alone": per_size,
     }
@@ -764,6 +769,10 @@ def main(argv=None) -> int:
     if not args.no_c2:
         try:
             c2 = batched_c2(graphs=not args.no_graphs)
+            # the FP32 fast-Poisson velocity preconditioner on the same batch, for comparison
+            dct = batched_c2(graphs=False, velo="PREC_DCT32", sequential=False)
+            c2["fp32_fast_poisson"] = {k: dct[k] for k in ("ms_per_batch_step", "instance_steps_per_s",
+                                                           "mean_iters", "non_converged_solves")}
         except Exception as e:  # reported beside the metric, never required for it
             log(f"batched workload failed: {e}")
     pc = None

@@ -125,6 +125,9 @@ struct drd_ctx {
   int nb = 1;
   int64_t bs = 0;
   void* slab = nullptr;
+  // batch groups using multigrid: each instance's hierarchies (BDF2, BDF1)
+  // inside its block of the slab
+  void *mg_mem = nullptr, *mg1_mem = nullptr;
   std::vector<drd_solve_info> last;  // per instance: the latest BiCGSTAB solve
   std::vector<double> last_div;      // per instance: the latest mean divergence
   // captured BiCGSTAB iteration chunks (drd_use_graphs)
@@ -462,6 +465,11 @@ static int ctx_alloc(drd_ctx* c) {
   add(&c->rb_c32, 6 * g.nc(), 4);
   add(&c->cfB, p.nx + 1, 8);
   add(&c->cfT, p.nx + 1, 8);
+  if (c->nb > 1 && (p.velo_prec == DRD_PREC_MG || p.velo_prec == DRD_PREC_MG32)) {
+    const size_t mgb = std::max(velo_mg_bytes(g, true), velo_mg_bytes(g, false));
+    add(&c->mg_mem, (int64_t)mgb, 1);
+    add(&c->mg1_mem, (int64_t)mgb, 1);
+  }
   size_t S = 0;
   std::vector<size_t> off;
   for (const Item& it : items) {
@@ -907,7 +915,7 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     if (!c->vmg) {
       int err = 0;
-      c->vmg = velo_mg_create(c->g, c->s, f32, &err);
+      c->vmg = velo_mg_create(c->g, c->s, f32, &err, c->nb > 1 ? c->mg_mem : nullptr);
       if (!c->vmg)
         return set_err(err ? err : DRD_E_HIP,
                        "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
@@ -944,7 +952,7 @@ static int prec_setup(drd_ctx* c, int prec) {
     }
     if (!c->vmg1) {
       int err = 0;
-      c->vmg1 = velo_mg_create(c->g, bdf1_scal(c->s), f32, &err);
+      c->vmg1 = velo_mg_create(c->g, bdf1_scal(c->s), f32, &err, c->nb > 1 ? c->mg1_mem : nullptr);
       if (!c->vmg1)
         return set_err(err ? err : DRD_E_HIP,
                        "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
@@ -2696,8 +2704,10 @@ extern "C" int drd_batch_create(int32_t count, const drd_params* params, drd_bat
   for (int i = 0; i < count; ++i) {
     const drd_params& q = params[i];
     if (q.velo_prec != DRD_PREC_DCT32 && q.velo_prec != DRD_PREC_DCT64 && q.velo_prec != DRD_PREC_JACOBI &&
-        q.velo_prec != DRD_PREC_IDENTITY && q.velo_prec != DRD_PREC_REFERENCE)
-      return set_err(DRD_E_UNSUPPORTED, "batched steps take the DCT, Jacobi or identity velocity preconditioner");
+        q.velo_prec != DRD_PREC_IDENTITY && q.velo_prec != DRD_PREC_REFERENCE && q.velo_prec != DRD_PREC_MG &&
+        q.velo_prec != DRD_PREC_MG32)
+      return set_err(DRD_E_UNSUPPORTED,
+                     "batched steps take the DCT, multigrid, Jacobi or identity velocity preconditioner");
     if (q.krylov != DRD_KRYLOV_BICGSTAB) return set_err(DRD_E_UNSUPPORTED, "batched steps use BiCGSTAB");
     if (q.lr_bd_type == 'p') return set_err(DRD_E_UNSUPPORTED, "periodic x takes a single-instance context");
     for (size_t gI = 0; gI < rep_of.size(); ++gI) {

@@ -271,7 +271,11 @@ struct VeloMG;
 // f32: the levels' coefficients, diagonals and vectors are stored in FP32
 // and their stencils evaluated in FP32; the input b, the output z and the
 // coarsest level stay FP64
-VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err);
+// region: where the buffers go (velo_mg_bytes bytes per instance; instance
+// k's copy k * bat_bs() bytes further, as every per-instance buffer of a
+// batch group), or nullptr for allocations of their own
+size_t velo_mg_bytes(const Grid& g, bool f32);
+VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err, void* region = nullptr);
 void velo_mg_destroy(VeloMG* mg);
 bool velo_mg_f32(const VeloMG* mg);
 void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0);

@@ -38,6 +38,7 @@ struct MGLevel {
 struct VeloMG {
   std::vector<MGLevel> lv;
   VeloPrec* coarse = nullptr;
+  bool own = true;  // buffers from hipMalloc (else carved from a caller's region)
   bool f32 = false;
   double omega = 0.7;
   int nu_pre = 2, nu_post = 2;  // damped Jacobi sweeps before / after the coarse correction (>= 1 each)
@@ -55,6 +56,7 @@ void velo_mg_set_fused(VeloMG* mg, bool on) {
 void velo_mg_destroy(VeloMG* mg) {
   if (!mg) return;
   for (auto& L : mg->lv) {
+    if (!mg->own) break;
     (void)hipFree(L.fields);
     (void)hipFree(L.d);
     (void)hipFree(L.x);
@@ -66,7 +68,54 @@ void velo_mg_destroy(VeloMG* mg) {
   delete mg;
 }
 
-VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err) {
+// Levels of grid g: level l+1 halves level l while nx >= 64 and ny >= 32.
+static std::vector<Grid> mg_grids(const Grid& g) {
+  std::vector<Grid> v;
+  int nx = g.nx, ny = g.ny;
+  for (;;) {
+    v.push_back(Grid::make(nx, ny));
+    if (nx / 2 < 64 || ny / 2 < 32) break;
+    nx /= 2;
+    ny /= 2;
+  }
+  return v;
+}
+static size_t mg_round(size_t n) { return (n + 255) / 256 * 256; }
+// Bytes of every buffer of a hierarchy (levels, then the coarsest level's
+// fast-Poisson buffers), each rounded to 256 B, in the order velo_mg_create
+// lays them out in a caller's region.
+static size_t mg_layout(const Grid& g, bool f32, const std::vector<Grid>& gs, std::vector<size_t>* sizes) {
+  size_t S = 0;
+  auto add = [&](size_t n) {
+    if (sizes) sizes->push_back(n);
+    S += mg_round(n);
+  };
+  const size_t nl = gs.size();
+  for (size_t l = 0; l < nl; ++l) {
+    const Grid& G = gs[l];
+    const bool lf = f32 && l + 1 < nl;
+    const size_t es = lf ? sizeof(float) : sizeof(double);
+    const size_t n = (size_t)G.nuv() * es;
+    add(n);                                          // d
+    add(n);                                          // t
+    add(l + 1 < nl ? n : 0);                         // r
+    add((l > 0 || lf) ? n : 0);                      // x
+    add((l > 0 || lf) ? n : 0);                      // b
+    add((l > 0 || lf) ? 3 * (size_t)G.nc() * es : 0);  // fields
+  }
+  const Grid& C = gs.back();
+  add((size_t)C.nuv() * sizeof(double));      // fast-Poisson inv_s64
+  add((size_t)C.nuv() * sizeof(float));       // inv_s32
+  add(2 * (size_t)C.nuv() * sizeof(double));  // work
+  return S;
+}
+
+size_t velo_mg_bytes(const Grid& g, bool f32) {
+  if (!is_pow2(g.nx) || !is_pow2(g.ny) || g.nx < 128 || g.ny < 64) return 0;
+  return mg_layout(g, f32, mg_grids(g), nullptr);
+}
+
+VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err, void* region) {
   *err = 0;
   if (!is_pow2(g.nx) || !is_pow2(g.ny) || g.nx < 128 || g.ny < 64) {
     *err = DRD_E_UNSUPPORTED;
@@ -74,45 +123,66 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err) {
   }
   VeloMG* mg = new VeloMG();
   mg->f32 = f32;
-  int nx = g.nx, ny = g.ny;
+  mg->own = region == nullptr;
+  const std::vector<Grid> gs = mg_grids(g);
   Scal sl = s;
-  for (;;) {
+  for (size_t l = 0; l < gs.size(); ++l) {
+    if (l > 0) {
+      sl.dr *= 2.0;
+      sl.dz *= 2.0;
+      sl.dr2 = sl.dr * sl.dr;
+      sl.dz2 = sl.dz * sl.dz;
+      scal_derive(sl);
+    }
     MGLevel L;
-    L.g = Grid::make(nx, ny);
+    L.g = gs[l];
     L.s = sl;
     mg->lv.push_back(L);
-    if (nx / 2 < 64 || ny / 2 < 32) break;
-    nx /= 2;
-    ny /= 2;
-    sl.dr *= 2.0;
-    sl.dz *= 2.0;
-    sl.dr2 = sl.dr * sl.dr;
-    sl.dz2 = sl.dz * sl.dz;
-    scal_derive(sl);
   }
-  bool ok = true;
   const size_t nl = mg->lv.size();
+  std::vector<size_t> sizes;
+  mg_layout(g, f32, gs, &sizes);
+  size_t k = 0;
+  char* at = static_cast<char*>(region);
+  bool ok = true;
+  // the next buffer of the layout: from the region, or its own allocation
+  auto take = [&](void** p) {
+    const size_t n = sizes[k++];
+    if (!n || !ok) return;
+    if (region) {
+      *p = at;
+      at += mg_round(n);
+    } else {
+      ok = hipMalloc(p, n) == hipSuccess;
+    }
+  };
   for (size_t l = 0; l < nl && ok; ++l) {
     MGLevel& L = mg->lv[l];
     L.f32 = f32 && l + 1 < nl;
     const size_t es = L.f32 ? sizeof(float) : sizeof(double);
-    const size_t n = (size_t)L.g.nuv() * es;
-    ok = hipMalloc(&L.d, n) == hipSuccess && hipMalloc(&L.t, n) == hipSuccess;
-    if (ok && l + 1 < nl) ok = hipMalloc(&L.r, n) == hipSuccess;
-    if (ok && (l > 0 || L.f32)) ok = hipMalloc(&L.x, n) == hipSuccess;  // FP64 level 0: x is z
-    if (ok && (l > 0 || L.f32)) ok = hipMalloc(&L.b, n) == hipSuccess;  // FP64 level 0: b is the input
-    if (ok && (l > 0 || L.f32)) {
+    take(&L.d);
+    take(&L.t);
+    take(&L.r);  // FP64 level 0: x is z, b is the input
+    take(&L.x);
+    take(&L.b);
+    take(&L.fields);
+    if (ok && L.fields) {
       const size_t nc = (size_t)L.g.nc() * es;
-      ok = hipMalloc(&L.fields, 3 * nc) == hipSuccess;
-      if (ok) {
-        char* f = (char*)L.fields;
-        L.c = MgCoef{f, f + nc, f + 2 * nc};
-      }
+      char* f = (char*)L.fields;
+      L.c = MgCoef{f, f + nc, f + 2 * nc};
     }
   }
   if (ok) {
     const MGLevel& C = mg->lv.back();
-    mg->coarse = velo_prec_create(C.g, C.s, false, err, nullptr);
+    if (region) {
+      VeloPrecBufs bufs{};
+      take((void**)&bufs.inv_s64);
+      take((void**)&bufs.inv_s32);
+      take(&bufs.work);
+      mg->coarse = velo_prec_create(C.g, C.s, false, err, &bufs);
+    } else {
+      mg->coarse = velo_prec_create(C.g, C.s, false, err, nullptr);
+    }
     ok = mg->coarse != nullptr;
   }
   if (!ok || nl < 2) {

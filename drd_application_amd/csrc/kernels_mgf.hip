@@ -182,8 +182,17 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
                                                          const T* __restrict__ eta, const T* __restrict__ chi,
                                                          const TB* __restrict__ b, const T* __restrict__ dw,
                                                          T* __restrict__ xd, float* __restrict__ bcopy,
-                                                         TBC* __restrict__ bc, const KState* st) {
+                                                         TBC* __restrict__ bc, const KState* st, int64_t bs) {
   constexpr int H = 3;
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  chi = bofs(chi, bs);
+  b = bofs(b, bs);
+  dw = bofs(dw, bs);
+  xd = bofs(xd, bs);
+  bcopy = bofs(bcopy, bs);
+  bc = bofs(bc, bs);
+  st = bofs(st, bs);
   using B = MgBlock<TW, TH, H>;
   __shared__ T sc[3 * B::NC];
   __shared__ T sa[B::NP];
@@ -267,8 +276,17 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
                                                        const T* __restrict__ eta, const T* __restrict__ chi,
                                                        const TB* __restrict__ b, const T* __restrict__ dw,
                                                        const T* __restrict__ xd, const TC* __restrict__ xc,
-                                                       TO* __restrict__ out, const KState* st) {
+                                                       TO* __restrict__ out, const KState* st, int64_t bs) {
   constexpr int H = 2;
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  chi = bofs(chi, bs);
+  b = bofs(b, bs);
+  dw = bofs(dw, bs);
+  xd = bofs(xd, bs);
+  xc = bofs(xc, bs);
+  out = bofs(out, bs);
+  st = bofs(st, bs);
   using B = MgBlock<TW, TH, H>;
   __shared__ T sc[3 * B::NC];
   __shared__ T sa[B::NP];
@@ -351,9 +369,9 @@ template <class T, class TB, class TBC, bool SMALL>
 static void mg_down_s(hipStream_t st, const Grid& g, const Grid& gc, const Scal& s, const MgCoef& c, const void* b,
                       const void* dw, void* xd, float* bcopy, void* bc, const KState* ks) {
   using S = MgfShape<T, SMALL>;
-  hipLaunchKernelGGL((k_mg_down<T, TB, TBC, S::TW, S::TH>), dim3(mgf_blocks<S>(g)), dim3(kMgfThreads), 0, st, g,
-                     gc, s, (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const TB*)b, (const T*)dw, (T*)xd,
-                     bcopy, (TBC*)bc, ks);
+  hipLaunchKernelGGL((k_mg_down<T, TB, TBC, S::TW, S::TH>), dim3(mgf_blocks<S>(g), 1, bat_n()), dim3(kMgfThreads), 0,
+                     st, g, gc, s, (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const TB*)b, (const T*)dw,
+                     (T*)xd, bcopy, (TBC*)bc, ks, bat_bs());
 }
 template <class T, class TB, class TBC>
 static void mg_down_t(hipStream_t st, const Grid& g, const Grid& gc, const Scal& s, const MgCoef& c, const void* b,
@@ -381,9 +399,9 @@ template <class T, class TB, class TC, class TO, bool SMALL>
 static void mg_up_s(hipStream_t st, const Grid& g, const Grid& gc, const Scal& s, const MgCoef& c, const void* b,
                     const void* dw, const void* xd, const void* xc, void* out, const KState* ks) {
   using S = MgfShape<T, SMALL>;
-  hipLaunchKernelGGL((k_mg_up<T, TB, TC, TO, S::TW, S::TH>), dim3(mgf_blocks<S>(g)), dim3(kMgfThreads), 0, st, g,
-                     gc, s, (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const TB*)b, (const T*)dw,
-                     (const T*)xd, (const TC*)xc, (TO*)out, ks);
+  hipLaunchKernelGGL((k_mg_up<T, TB, TC, TO, S::TW, S::TH>), dim3(mgf_blocks<S>(g), 1, bat_n()), dim3(kMgfThreads), 0,
+                     st, g, gc, s, (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const TB*)b, (const T*)dw,
+                     (const T*)xd, (const TC*)xc, (TO*)out, ks, bat_bs());
 }
 template <class T, class TB, class TC, class TO>
 static void mg_up_t(hipStream_t st, const Grid& g, const Grid& gc, const Scal& s, const MgCoef& c, const void* b,

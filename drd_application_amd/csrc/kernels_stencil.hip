@@ -1055,7 +1055,13 @@ __device__ __forceinline__ double mg_w_cell_even(int f, int C, int nc) {
 __global__ __launch_bounds__(kBlock) void k_mg_convert(int64_t n, const double* __restrict__ r,
                                                        const double* __restrict__ e,
                                                        const double* __restrict__ c, float* __restrict__ rf,
-                                                       float* __restrict__ ef, float* __restrict__ cf) {
+                                                       float* __restrict__ ef, float* __restrict__ cf, int64_t bs) {
+  r = bofs(r, bs);
+  e = bofs(e, bs);
+  c = bofs(c, bs);
+  rf = bofs(rf, bs);
+  ef = bofs(ef, bs);
+  cf = bofs(cf, bs);
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= n) return;
   rf[q] = (float)r[q];
@@ -1070,7 +1076,13 @@ template <class T, class TO>
 __global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const T* __restrict__ rf,
                                                        const T* __restrict__ ef, const T* __restrict__ cf,
                                                        TO* __restrict__ rc, TO* __restrict__ ec,
-                                                       TO* __restrict__ cc) {
+                                                       TO* __restrict__ cc, int64_t bs) {
+  rf = bofs(rf, bs);
+  ef = bofs(ef, bs);
+  cf = bofs(cf, bs);
+  rc = bofs(rc, bs);
+  ec = bofs(ec, bs);
+  cc = bofs(cc, bs);
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q >= gc.nc()) return;
   const int ld = gc.nx + 2;
@@ -1094,7 +1106,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(Grid gf, Grid gc, const T
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_mg_diag(Grid g, Scal s, const T* __restrict__ rho,
                                                     const T* __restrict__ eta, const T* __restrict__ chi,
-                                                    T* __restrict__ dw, double omega) {
+                                                    T* __restrict__ dw, double omega, int64_t bs) {
+  rho = bofs(rho, bs);
+  eta = bofs(eta, bs);
+  chi = bofs(chi, bs);
+  dw = bofs(dw, bs);
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
   if (q < g.nuv()) dw[q] = (T)(omega / velo_diag2(g, s, rho, eta, chi, q));
 }
@@ -1124,7 +1140,16 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* _
                                                       const T* __restrict__ eta, const T* __restrict__ chi,
                                                       const T* __restrict__ x, const TB* __restrict__ b,
                                                       const T* __restrict__ dw, TO* __restrict__ out,
-                                                      float* __restrict__ bc, const KState* st) {
+                                                      float* __restrict__ bc, const KState* st, int64_t ibs) {
+  rho = bofs(rho, ibs);
+  eta = bofs(eta, ibs);
+  chi = bofs(chi, ibs);
+  x = bofs(x, ibs);
+  b = bofs(b, ibs);
+  dw = bofs(dw, ibs);
+  out = bofs(out, ibs);
+  bc = bofs(bc, ibs);
+  st = bofs(st, ibs);
   const uint32_t n = (uint32_t)g.nuv();
   const bool late = n < kMgLate;
   if (!late && st && (st->done || st->early)) return;
@@ -1160,7 +1185,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* _
 // bc = R r = P^T r / 4 (one thread per coarse unknown)
 template <class TR, class TB>
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const TR* __restrict__ r,
-                                                        TB* __restrict__ bc, const KState* st) {
+                                                        TB* __restrict__ bc, const KState* st, int64_t bs) {
+  r = bofs(r, bs);
+  bc = bofs(bc, bs);
+  st = bofs(st, bs);
   const bool late = gf.nuv() < kMgLate;
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
@@ -1182,7 +1210,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Grid gf, Grid gc, const 
 // x += P xc (one thread per fine unknown)
 template <class TC, class TF>
 __global__ __launch_bounds__(kBlock) void k_mg_prolong_add(Grid gc, Grid gf, const TC* __restrict__ xc,
-                                                           TF* __restrict__ x, const KState* st) {
+                                                           TF* __restrict__ x, const KState* st, int64_t bs) {
+  xc = bofs(xc, bs);
+  x = bofs(x, bs);
+  st = bofs(st, bs);
   const bool late = gf.nuv() < kMgLate;
   if (!late && st && (st->done || st->early)) return;
   const int64_t q = (int64_t)tile_id() * kBlock + threadIdx.x;  // XCD-contiguous blocks
@@ -1350,15 +1381,15 @@ void launch_phmu_unpack(hipStream_t st, const Grid& g, const Scal& s, const doub
 
 // ---- multigrid launchers ------------------------------------------------------
 void launch_mg_convert(hipStream_t st, const Grid& g, const Coef& c, float* rho, float* eta, float* chi) {
-  hipLaunchKernelGGL(k_mg_convert, dim3(nblk(g.nc(), kBlock)), dim3(kBlock), 0, st, g.nc(), c.rho, c.eta, c.chi,
-                     rho, eta, chi);
+  hipLaunchKernelGGL(k_mg_convert, dim3(nblk(g.nc(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g.nc(), c.rho, c.eta,
+                     c.chi, rho, eta, chi, bat_bs());
 }
 void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const Grid& gc, const MgCoef& f,
                        void* rc, void* ec, void* cc) {
-  const dim3 grid(nblk(gc.nc(), kBlock));
+  const dim3 grid(nblk(gc.nc(), kBlock), 1, bat_n());
 #define MGC(T, TO)                                                                                     \
   hipLaunchKernelGGL((k_mg_coarsen<T, TO>), grid, dim3(kBlock), 0, st, gf, gc, (const T*)f.rho,           \
-                     (const T*)f.eta, (const T*)f.chi, (TO*)rc, (TO*)ec, (TO*)cc)
+                     (const T*)f.eta, (const T*)f.chi, (TO*)rc, (TO*)ec, (TO*)cc, bat_bs())
   if (f32 && o32)
     MGC(float, float);
   else if (f32)  // into an FP64 coarsest level
@@ -1369,20 +1400,20 @@ void launch_mg_coarsen(hipStream_t st, bool f32, bool o32, const Grid& gf, const
 }
 void launch_mg_diag(hipStream_t st, bool f32, const Grid& g, const Scal& s, const MgCoef& c, void* dw,
                     double omega) {
-  const dim3 grid(nblk(g.nuv(), kBlock));
+  const dim3 grid(nblk(g.nuv(), kBlock), 1, bat_n());
   if (f32)
     hipLaunchKernelGGL(k_mg_diag<float>, grid, dim3(kBlock), 0, st, g, s, (const float*)c.rho, (const float*)c.eta,
-                       (const float*)c.chi, (float*)dw, omega);
+                       (const float*)c.chi, (float*)dw, omega, bat_bs());
   else
     hipLaunchKernelGGL(k_mg_diag<double>, grid, dim3(kBlock), 0, st, g, s, (const double*)c.rho,
-                       (const double*)c.eta, (const double*)c.chi, (double*)dw, omega);
+                       (const double*)c.eta, (const double*)c.chi, (double*)dw, omega, bat_bs());
 }
 template <int K, class T, class TB, class TO>
 static void mg_smooth_t(hipStream_t st, const Grid& g, const Scal& s, const MgCoef& c, const void* x,
                         const void* b, const void* dw, void* out, float* bc, const KState* ks) {
-  hipLaunchKernelGGL((k_mg_smooth<K, T, TB, TO>), dim3(nblk(g.nuv(), kBlock)), dim3(kBlock), 0, st, g, s,
-                     (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)dw,
-                     (TO*)out, bc, ks);
+  hipLaunchKernelGGL((k_mg_smooth<K, T, TB, TO>), dim3(nblk(g.nuv(), kBlock), 1, bat_n()), dim3(kBlock), 0, st, g,
+                     s, (const T*)c.rho, (const T*)c.eta, (const T*)c.chi, (const T*)x, (const TB*)b, (const T*)dw,
+                     (TO*)out, bc, ks, bat_bs());
 }
 template <int K>
 static void mg_smooth_k(hipStream_t st, bool f32, bool b32, bool o32, const Grid& g, const Scal& s,
@@ -1409,29 +1440,29 @@ void launch_mg_smooth(hipStream_t st, int kind, bool f32, bool b32, bool o32, co
 }
 void launch_mg_restrict(hipStream_t st, bool r32, bool b32, const Grid& gf, const Grid& gc, const void* r,
                         void* bc, const KState* ks) {
-  const dim3 grid(nblk(gc.nuv(), kBlock));
+  const dim3 grid(nblk(gc.nuv(), kBlock), 1, bat_n());
   if (r32 && b32)
     hipLaunchKernelGGL((k_mg_restrict<float, float>), grid, dim3(kBlock), 0, st, gf, gc, (const float*)r,
-                       (float*)bc, ks);
+                       (float*)bc, ks, bat_bs());
   else if (r32)  // into an FP64 coarsest level
     hipLaunchKernelGGL((k_mg_restrict<float, double>), grid, dim3(kBlock), 0, st, gf, gc, (const float*)r,
-                       (double*)bc, ks);
+                       (double*)bc, ks, bat_bs());
   else
     hipLaunchKernelGGL((k_mg_restrict<double, double>), grid, dim3(kBlock), 0, st, gf, gc, (const double*)r,
-                       (double*)bc, ks);
+                       (double*)bc, ks, bat_bs());
 }
 void launch_mg_prolong_add(hipStream_t st, bool c32, bool f32, const Grid& gc, const Grid& gf, const void* xc,
                            void* x, const KState* ks) {
-  const dim3 grid(nblk(gf.nuv(), kBlock));
+  const dim3 grid(nblk(gf.nuv(), kBlock), 1, bat_n());
   if (c32 && f32)
     hipLaunchKernelGGL((k_mg_prolong_add<float, float>), grid, dim3(kBlock), 0, st, gc, gf, (const float*)xc,
-                       (float*)x, ks);
+                       (float*)x, ks, bat_bs());
   else if (f32)  // from an FP64 coarsest level
     hipLaunchKernelGGL((k_mg_prolong_add<double, float>), grid, dim3(kBlock), 0, st, gc, gf, (const double*)xc,
-                       (float*)x, ks);
+                       (float*)x, ks, bat_bs());
   else
     hipLaunchKernelGGL((k_mg_prolong_add<double, double>), grid, dim3(kBlock), 0, st, gc, gf, (const double*)xc,
-                       (double*)x, ks);
+                       (double*)x, ks, bat_bs());
 }
 
 static dim3 rb_grid(const Grid& g) {  // one workgroup per tile (tile_id)

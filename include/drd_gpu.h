@@ -323,7 +323,9 @@ int drd_read_snapshot(drd_ctx* ctx, const char* path, int32_t f_field, int32_t f
  * the others iterate.  Instances of one size must share every drd_params
  * field; per-instance resistance goes into each instance's bdr_ph_penalty
  * field (the operator uses penalty_k * bdr_ph_penalty).  BiCGSTAB only;
- * velocity preconditioner DCT32, DCT64, Jacobi, identity or reference.     */
+ * velocity preconditioner DCT32, DCT64, MG, MG32 (power-of-two grids of at
+ * least 128 x 64: each instance's hierarchies sit in its own block of the
+ * group's memory), Jacobi, identity or reference.                          */
 typedef struct drd_batch drd_batch;
 int drd_batch_create(int32_t count, const drd_params* params, drd_batch** out);
 int drd_batch_destroy(drd_batch* b);

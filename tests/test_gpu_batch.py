@@ -42,6 +42,39 @@ def test_batch_groups_and_field_round_trip():
                 np.testing.assert_array_equal(b.get_field(i, fid), cfg.fields[fid])
 
 
+@pytest.mark.parametrize("velo,graphs", [("PREC_MG32", False), ("PREC_MG32", True), ("PREC_MG", False)])
+def test_batch_multigrid_steps_are_bitwise_the_single_instance_steps(velo, graphs):
+    """Batch groups with the multigrid velocity preconditioner (each
+    instance's BDF1 and BDF2 hierarchies inside its block of the group's
+    memory, every level's launches over all instances) and the FP32
+    red-black CH preconditioner: three steps of every instance equal the same
+    instance stepped alone, fields and Krylov reports bit for bit."""
+    from drd_application_amd.solver import GpuBatch, GpuSolver
+    cfgs = _configs(n=6, seed=5, sizes=(128, 256))  # three of each size
+    for c in cfgs:
+        c.params.velo_prec = getattr(abi, velo)
+        c.params.ch_prec = abi.CH_PREC_RBGS32
+    print("multigrid batch of", [c.notes for c in cfgs], velo, "graphs", graphs, flush=True)
+    steps = 3
+    with GpuBatch([c.params for c in cfgs]) as b:
+        _load_batch(b, cfgs)
+        b.use_graphs(graphs)
+        binfo = [b.step(1 if k == 0 else 2, k * cfgs[0].params.dt) for k in range(steps)]
+        for i, cfg in enumerate(cfgs):
+            g = GpuSolver(cfg.params)
+            geometry.load(g, cfg)
+            for k in range(steps):
+                s = g.step(1 if k == 0 else 2, k * cfg.params.dt)
+                for key in ("phmu", "velo"):
+                    assert s[key]["info"] == 0 and binfo[k][i][key]["info"] == 0, (i, k, key)
+                    assert s[key]["iter"] == binfo[k][i][key]["iter"], (i, k, key)
+                    assert s[key]["resid"] == binfo[k][i][key]["resid"]
+            for fid in FIELDS:
+                np.testing.assert_array_equal(b.get_field(i, fid), g.get_field(fid), err_msg=f"instance {i} field {fid}")
+            g.close()
+        print("velocity iterations per step", [[x["velo"]["iter"] for x in bi] for bi in binfo], flush=True)
+
+
 @pytest.mark.parametrize("steps,ch,graphs", [(4, None, True), (3, "CH_PREC_RBGS32", False),
                                             (3, "CH_PREC_RBGS", True)])
 def test_batch_steps_are_bitwise_the_single_instance_steps(steps, ch, graphs):
