@@ -492,7 +492,7 @@ def combined_c4(steps: int = 20, warmup: int = 2):
 
 
 def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 4, graphs: bool = True,
-               velo: str = "PREC_MG32", sequential: bool = True):
+               velo: str = "PREC_MG32", sequential: bool = True, pow2: bool = False):
     """The batched workload: `count` independent instances of uneven sizes
     (geometry.batch_configs) stepped together by drd_batch_step, every size
     group's kernels launched once over all its instances, BiCGSTAB iteration
@@ -501,14 +501,17 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
     its own context, no graphs) and scales their mean by the number of
     instances of that size.  The CH preconditioner is the FP32 red-black one
     (the configs' own choice is block Jacobi); the velocity preconditioner
-    `velo` (default the FP32 multigrid; PREC_DCT32 the FP32 fast-Poisson)."""
+    `velo` (default the FP32 multigrid; PREC_DCT32 the FP32 fast-Poisson).
+    nx is drawn from the multiples of 32 in [128, 1024] (geometry.C2_SIZES),
+    or with pow2 from the powers of two (the fast-Poisson preconditioner's
+    FFTs need them)."""
     import numpy as np
 
     from drd_application_amd import abi, geometry
     from drd_application_amd.solver import GpuBatch, GpuSolver
 
     t0 = time.perf_counter()
-    cfgs = geometry.batch_configs(count)
+    cfgs = geometry.batch_configs(count, sizes=(128, 256, 512, 1024) if pow2 else geometry.C2_SIZES)
     for c in cfgs:
         c.params.ch_prec = abi.CH_PREC_RBGS32
         c.params.velo_prec = getattr(abi, velo)
@@ -594,7 +597,8 @@ def batched_c2(count: int = 256, steps: int = 3, warmup: int = 2, sample: int = 
         seq += ms * len(group)
     log(f"[c2] sequential estimate {seq:.0f} ms/step ({per_size})")
     return {
-        "workload": f"{count} instances, nx log-uniform over powers of two 128..1024, ny = nx/2, eta ratio "
+        "workload": f"{count} instances, nx log-uniform over {'powers of two' if pow2 else 'multiples of 32'} "
+                    f"128..1024, ny = nx/2, eta ratio "
                     f"U[1,100], 0-8 discs R log-uniform 1e4..1e6, 2-12 droplets, "
                     f"{'FP32 multigrid' if velo == 'PREC_MG32' else velo} velocity and FP32 red-black CH "
                     f"preconditioners, FP64 BiCGSTAB, tol 1e-10",
@@ -769,10 +773,12 @@ def main(argv=None) -> int:
     if not args.no_c2:
         try:
             c2 = batched_c2(graphs=not args.no_graphs)
-            # the FP32 fast-Poisson velocity preconditioner on the same batch, for comparison
-            dct = batched_c2(graphs=False, velo="PREC_DCT32", sequential=False)
-            c2["fp32_fast_poisson"] = {k: dct[k] for k in ("ms_per_batch_step", "instance_steps_per_s",
-                                                           "mean_iters", "non_converged_solves")}
+            # the FP32 fast-Poisson and the multigrid velocity preconditioners on a
+            # batch of power-of-two sizes (the fast-Poisson FFTs need them), for comparison
+            for key, velo in (("pow2_fp32_fast_poisson", "PREC_DCT32"), ("pow2_fp32_multigrid", "PREC_MG32")):
+                r = batched_c2(graphs=False, velo=velo, sequential=False, pow2=True)
+                c2[key] = {k: r[k] for k in ("cells", "groups", "ms_per_batch_step", "instance_steps_per_s",
+                                             "mean_iters", "non_converged_solves")}
         except Exception as e:  # reported beside the metric, never required for it
             log(f"batched workload failed: {e}")
     pc = None

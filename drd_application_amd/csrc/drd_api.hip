@@ -18,6 +18,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -918,7 +919,7 @@ static int prec_setup(drd_ctx* c, int prec) {
       c->vmg = velo_mg_create(c->g, c->s, f32, &err, c->nb > 1 ? c->mg_mem : nullptr);
       if (!c->vmg)
         return set_err(err ? err : DRD_E_HIP,
-                       "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
+                       "multigrid velocity preconditioner needs even nx >= 128, ny >= 64 (and a coarsest grid of at most 256 x 256 cells or of powers of two)");
     }
     velo_mg_set_fused(c->vmg, c->mg_fused);
     velo_mg_setup(c->stream, c->vmg, coef_of(c));
@@ -955,7 +956,7 @@ static int prec_setup(drd_ctx* c, int prec) {
       c->vmg1 = velo_mg_create(c->g, bdf1_scal(c->s), f32, &err, c->nb > 1 ? c->mg1_mem : nullptr);
       if (!c->vmg1)
         return set_err(err ? err : DRD_E_HIP,
-                       "multigrid velocity preconditioner needs power-of-two nx >= 128, ny >= 64");
+                       "multigrid velocity preconditioner needs even nx >= 128, ny >= 64 (and a coarsest grid of at most 256 x 256 cells or of powers of two)");
     }
     velo_mg_set_fused(c->vmg1, c->mg_fused);
     velo_mg_setup(c->stream, c->vmg1, coef_of(c));
@@ -2830,13 +2831,34 @@ extern "C" int drd_batch_groups(drd_batch* b, int32_t* ngroups, int32_t* desc, i
   return 0;
 }
 
+// The size groups step at once, each from a host thread of its own on its
+// own stream (a group's host polls between iteration chunks then overlap the
+// other groups' kernels); the first group's error, if any, is reported.
+// Groups that capture iteration chunks as graphs step one after the other:
+// concurrent stream captures from several threads fail in the runtime.
 extern "C" int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info* infos) {
   if (!b) return set_err(DRD_E_ARG, "null batch");
-  for (size_t gI = 0; gI < b->groups.size(); ++gI) {
+  const size_t G = b->groups.size();
+  std::vector<int> rc(G, 0);
+  std::vector<std::string> msg(G);
+  auto run = [&](size_t gI) {
     drd_ctx* c = b->groups[gI];
     BatchScope scope(c);
-    RCHK(step_dev(c, bdf, t, b->infos[gI].data()));
+    rc[gI] = step_dev(c, bdf, t, b->infos[gI].data());
+    if (rc[gI]) msg[gI] = g_err;
+  };
+  bool graphs = false;
+  for (drd_ctx* c : b->groups) graphs = graphs || c->use_graphs;
+  if (G == 1 || graphs) {
+    for (size_t gI = 0; gI < G; ++gI) run(gI);
+  } else {
+    std::vector<std::thread> th;
+    th.reserve(G);
+    for (size_t gI = 0; gI < G; ++gI) th.emplace_back(run, gI);
+    for (auto& x : th) x.join();
   }
+  for (size_t gI = 0; gI < G; ++gI)
+    if (rc[gI]) return set_err(rc[gI], msg[gI]);
   if (infos)
     for (size_t i = 0; i < b->where.size(); ++i) infos[i] = b->infos[b->where[i].first][b->where[i].second];
   return 0;

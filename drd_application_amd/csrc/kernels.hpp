@@ -271,6 +271,17 @@ struct VeloMG;
 // f32: the levels' coefficients, diagonals and vectors are stored in FP32
 // and their stencils evaluated in FP32; the input b, the output z and the
 // coarsest level stay FP64
+// The multigrid's coarsest solve for any grid of at most 256 x 256 cells
+// (kernels_coarse.hip): the fast-Poisson preconditioner with dense line
+// transforms.  region: coarse_bytes(g) per instance, or nullptr.
+struct CoarseSolve;
+bool coarse_fits(const Grid& g);
+size_t coarse_bytes(const Grid& g);
+CoarseSolve* coarse_create(const Grid& g, const Scal& s, int* err, void* region);
+void coarse_destroy(CoarseSolve* cs);
+void coarse_setup(hipStream_t st, CoarseSolve* cs, const Grid& g, const Scal& s, const Coef& c);
+void coarse_apply(hipStream_t st, CoarseSolve* cs, const Grid& g, const Scal& s, const double* r, double* z,
+                  const KState* ks);
 // region: where the buffers go (velo_mg_bytes bytes per instance; instance
 // k's copy k * bat_bs() bytes further, as every per-instance buffer of a
 // batch group), or nullptr for allocations of their own

@@ -399,12 +399,14 @@ static size_t row_lds(int N) { return 2 * (size_t)N * sizeof(double2); }
 
 template <int MODE>
 static void launch_rows(hipStream_t st, const RowArgs& a) {
-  static bool raised = false;  // lines of 2048+ need more than 64 KiB of LDS
-  if (!raised) {
+  // lines of 2048+ need more than 64 KiB of LDS; raised once (a function-
+  // local static: concurrent first callers wait for it)
+  static const bool raised = [] {
     (void)hipFuncSetAttribute((const void*)k_rows<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)row_lds(4096));
-    raised = true;
-  }
+    return true;
+  }();
+  (void)raised;
   hipLaunchKernelGGL(k_rows<MODE>, dim3(a.rows, 1, bat_n()), dim3(kBlock), row_lds(a.N), st, a, bat_bs());
 }
 

@@ -2,12 +2,15 @@
 // velocity operator (fast path; no reference counterpart).
 //
 // Level 0 is the solve's own grid; level l+1 halves level l in x and y while
-// the result keeps nx >= 64 and ny >= 32.  The coarse operators are
+// level l's sizes are even and the result keeps nx >= 64 and ny >= 32.  The coarse operators are
 // rediscretised: the same MAC stencil (velo_row) with doubled spacings, on
 // coefficients rho, eta, chi averaged over 2x2 cells.  One V-cycle:
 //   down:  x = omega b / d, one more damped Jacobi sweep, r = b - A x,
 //          b_{l+1} = R r
-//   coarsest: x = the FP64 DCT/DST fast-Poisson solve of its b
+//   coarsest: x = the FP64 DCT/DST fast-Poisson solve of its b (the FFT form
+//             of kernels_prec.hip on a power-of-two coarsest level, the
+//             dense-transform form of kernels_coarse.hip on any other of at
+//             most 256 x 256 cells)
 //   up:    x += P x_{l+1}, two damped Jacobi sweeps
 // with omega = 0.7 and d the exact diagonal of each level's operator.  The
 // cycle is a fixed map of b, so BiCGSTAB accepts it.  Every kernel skips
@@ -37,7 +40,8 @@ struct MGLevel {
 
 struct VeloMG {
   std::vector<MGLevel> lv;
-  VeloPrec* coarse = nullptr;
+  CoarseSolve* csolve = nullptr;  // the coarsest solve, or
+  VeloPrec* coarse = nullptr;     // its FFT form
   bool own = true;  // buffers from hipMalloc (else carved from a caller's region)
   bool f32 = false;
   double omega = 0.7;
@@ -65,16 +69,21 @@ void velo_mg_destroy(VeloMG* mg) {
     (void)hipFree(L.r);
   }
   velo_prec_destroy(mg->coarse);
+  coarse_destroy(mg->csolve);
   delete mg;
 }
 
-// Levels of grid g: level l+1 halves level l while nx >= 64 and ny >= 32.
+// Levels of grid g: level l+1 halves level l while level l's sizes are even
+// and the halves keep nx >= 64 and ny >= 32.
+static bool is_pow2(int n);
+// the coarsest level takes the dense-transform solve (else the FFT form)
+static bool mg_dense(const Grid& C) { return !(is_pow2(C.nx) && is_pow2(C.ny)) && coarse_fits(C); }
 static std::vector<Grid> mg_grids(const Grid& g) {
   std::vector<Grid> v;
   int nx = g.nx, ny = g.ny;
   for (;;) {
     v.push_back(Grid::make(nx, ny));
-    if (nx / 2 < 64 || ny / 2 < 32) break;
+    if (nx / 2 < 64 || ny / 2 < 32 || (nx & 1) || (ny & 1)) break;
     nx /= 2;
     ny /= 2;
   }
@@ -104,20 +113,32 @@ static size_t mg_layout(const Grid& g, bool f32, const std::vector<Grid>& gs, st
     add((l > 0 || lf) ? 3 * (size_t)G.nc() * es : 0);  // fields
   }
   const Grid& C = gs.back();
-  add((size_t)C.nuv() * sizeof(double));      // fast-Poisson inv_s64
-  add((size_t)C.nuv() * sizeof(float));       // inv_s32
-  add(2 * (size_t)C.nuv() * sizeof(double));  // work
+  if (mg_dense(C)) {
+    add(coarse_bytes(C));  // dense-transform coarsest solve: inv_s, work
+  } else {
+    add((size_t)C.nuv() * sizeof(double));      // fast-Poisson inv_s64
+    add((size_t)C.nuv() * sizeof(float));       // inv_s32
+    add(2 * (size_t)C.nuv() * sizeof(double));  // work
+  }
   return S;
 }
 
+// Grids the hierarchy takes: at least two levels, and a coarsest level the
+// dense-transform solve fits or a power-of-two one for the FFT form.
+static bool mg_ok(const Grid& g) {
+  if (g.nx < 128 || g.ny < 64 || (g.nx & 1) || (g.ny & 1)) return false;
+  const Grid C = mg_grids(g).back();
+  return coarse_fits(C) || (is_pow2(C.nx) && is_pow2(C.ny));
+}
+
 size_t velo_mg_bytes(const Grid& g, bool f32) {
-  if (!is_pow2(g.nx) || !is_pow2(g.ny) || g.nx < 128 || g.ny < 64) return 0;
+  if (!mg_ok(g)) return 0;
   return mg_layout(g, f32, mg_grids(g), nullptr);
 }
 
 VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err, void* region) {
   *err = 0;
-  if (!is_pow2(g.nx) || !is_pow2(g.ny) || g.nx < 128 || g.ny < 64) {
+  if (!mg_ok(g)) {
     *err = DRD_E_UNSUPPORTED;
     return nullptr;
   }
@@ -174,16 +195,22 @@ VeloMG* velo_mg_create(const Grid& g, const Scal& s, bool f32, int* err, void* r
   }
   if (ok) {
     const MGLevel& C = mg->lv.back();
-    if (region) {
+    if (mg_dense(C.g)) {
+      void* cr = nullptr;
+      if (region) take(&cr);
+      mg->csolve = coarse_create(C.g, C.s, err, cr);
+      ok = mg->csolve != nullptr;
+    } else if (region) {
       VeloPrecBufs bufs{};
       take((void**)&bufs.inv_s64);
       take((void**)&bufs.inv_s32);
       take(&bufs.work);
       mg->coarse = velo_prec_create(C.g, C.s, false, err, &bufs);
+      ok = mg->coarse != nullptr;
     } else {
       mg->coarse = velo_prec_create(C.g, C.s, false, err, nullptr);
+      ok = mg->coarse != nullptr;
     }
-    ok = mg->coarse != nullptr;
   }
   if (!ok || nl < 2) {
     velo_mg_destroy(mg);
@@ -210,7 +237,18 @@ void velo_mg_setup(hipStream_t st, VeloMG* mg, const Coef& c0) {
   for (auto& L : mg->lv) launch_mg_diag(st, L.f32, L.g, L.s, L.c, L.d, mg->omega);
   const MGLevel& C = mg->lv.back();  // FP64
   const Coef cc{(const double*)C.c.rho, (const double*)C.c.eta, (const double*)C.c.chi};
-  velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
+  if (mg->csolve)
+    coarse_setup(st, mg->csolve, C.g, C.s, cc);
+  else
+    velo_prec_setup(st, mg->coarse, C.g, C.s, cc);
+}
+
+static void mg_coarsest(hipStream_t st, VeloMG* mg, const KState* ks) {
+  const MGLevel& C = mg->lv.back();  // FP64
+  if (mg->csolve)
+    coarse_apply(st, mg->csolve, C.g, C.s, (const double*)C.b, (double*)C.x, ks);
+  else
+    velo_prec_apply(st, mg->coarse, C.g, (const double*)C.b, (double*)C.x, ks);
 }
 
 // V(2,2) by one launch per level and leg (kernels_mgf.hip): the down leg of
@@ -228,10 +266,7 @@ static void velo_mg_apply_fused(hipStream_t st, VeloMG* mg, const double* b, dou
     float* bcopy = top && L.f32 ? (float*)L.b : nullptr;
     launch_mg_down(st, L.f32, L.f32 && !top, N.f32, L.g, N.g, L.s, L.c, bl, L.d, L.t, bcopy, N.b, ks);
   }
-  {
-    const MGLevel& C = mg->lv[nl - 1];  // FP64
-    velo_prec_apply(st, mg->coarse, C.g, (const double*)C.b, (double*)C.x, ks);
-  }
+  mg_coarsest(st, mg, ks);
   for (int l = nl - 2; l >= 0; --l) {
     MGLevel& L = mg->lv[l];
     const MGLevel& N = mg->lv[l + 1];
@@ -289,10 +324,7 @@ void velo_mg_apply(hipStream_t st, VeloMG* mg, const double* b, double* z, const
     launch_mg_smooth(st, MG_RES, f, b32(l), f, L.g, L.s, L.c, cur, rhs(l), L.d, L.r, nullptr, ks);
     launch_mg_restrict(st, f, mg->lv[l + 1].f32, L.g, mg->lv[l + 1].g, L.r, mg->lv[l + 1].b, ks);
   }
-  {
-    const MGLevel& C = mg->lv[nl - 1];  // FP64
-    velo_prec_apply(st, mg->coarse, C.g, (const double*)C.b, (double*)C.x, ks);
-  }
+  mg_coarsest(st, mg, ks);
   for (int l = nl - 2; l >= 0; --l) {
     MGLevel& L = mg->lv[l];
     const bool f = L.f32;

@@ -128,11 +128,11 @@ __global__ __launch_bounds__(kBlock) void k_transpose_d(int rows, int cols, cons
 
 static void vcost_launch(hipStream_t s, const Dct1Tab* t, int nlines, const double* in, int64_t ld_in, double* out,
                          int64_t ld_out, const double* cof, int64_t ld_cof, const KState* ks) {
-  static bool raised = false;
-  if (!raised) {
+  static const bool raised = [] {  // once; concurrent first callers wait for it
     (void)hipFuncSetAttribute((const void*)k_vcost_lines, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    raised = true;
-  }
+    return true;
+  }();
+  (void)raised;
   hipLaunchKernelGGL(k_vcost_lines, dim3(nlines), dim3(kBlock), vcost_lds(t->n), s, view(t), nlines, in, ld_in, out,
                      ld_out, cof, ld_cof, ks);
 }

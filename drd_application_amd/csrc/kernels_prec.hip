@@ -631,16 +631,18 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
                                const double* r, double* z, T* wu, T* wv, T* tu, T* tv, unsigned rows_blocks,
                                unsigned cols_blocks, size_t lds_rows, size_t lds_cols, const KState* ks, int shape) {
   const int nx = g.nx, ny = g.ny;
-  static bool raised = false;  // lines of 2nx or 2ny complex may exceed 64 KiB
-  if (!raised) {
+  // lines of 2nx or 2ny complex may exceed 64 KiB; raised once (a function-
+  // local static: concurrent first callers wait for it)
+  static const bool raised = [] {
     const int lim = 160 * 1024;
     (void)hipFuncSetAttribute((const void*)k_prec_rows<T, true, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
     (void)hipFuncSetAttribute((const void*)k_prec_rows<T, false, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
     (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAXC>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    raised = true;
-  }
+    return true;
+  }();
+  (void)raised;
   const unsigned B = bat_n();
   const int64_t bs = bat_bs();
   hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb, r,

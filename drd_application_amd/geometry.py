@@ -314,12 +314,18 @@ def velocity_system(n: int, seed: int, *, max_contrast: float = 1.0e3, n_discs: 
     return cfg, b
 
 
+# nx of the batched workload: the multiples of 32 from 128 to 1024
+C2_SIZES = tuple(range(128, 1025, 32))
+
+
 def batch_configs(count: int = 256, seed: int = 2000, sizes=(128, 256, 512, 1024), r_ref: float = 1.0e6,
                   dt: float = 5.0e-5, tol: float = 1.0e-10, steps: int = 20):
     """The batched workload: `count` independent two-phase instances.
 
-    nx log-uniform over `sizes` (powers of two, because the fast-Poisson
-    preconditioner's FFTs are radix-2^k), ny = nx/2, height 1; eta ratio
+    nx log-uniform over `sizes` (the draw taken to the nearest size in log;
+    C2_SIZES, the multiples of 32, for the workload as specified; powers of
+    two for the fast-Poisson preconditioner, whose FFTs are radix-2^k), ny =
+    nx/2, height 1; eta ratio
     uniform in [1, 100]; 0-8 seeded discs (radius 6-16 cells) whose
     resistance R is log-uniform in [1e4, 1e6]; 2-12 droplets (radius 4-16
     cells).  Every instance of one size shares its drd_params (penalty_k =

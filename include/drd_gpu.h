@@ -323,9 +323,9 @@ int drd_read_snapshot(drd_ctx* ctx, const char* path, int32_t f_field, int32_t f
  * the others iterate.  Instances of one size must share every drd_params
  * field; per-instance resistance goes into each instance's bdr_ph_penalty
  * field (the operator uses penalty_k * bdr_ph_penalty).  BiCGSTAB only;
- * velocity preconditioner DCT32, DCT64, MG, MG32 (power-of-two grids of at
- * least 128 x 64: each instance's hierarchies sit in its own block of the
- * group's memory), Jacobi, identity or reference.                          */
+ * velocity preconditioner DCT32, DCT64 (power-of-two grids), MG, MG32 (even
+ * grids of at least 128 x 64: each instance's hierarchies sit in its own
+ * block of the group's memory), Jacobi, identity or reference.             */
 typedef struct drd_batch drd_batch;
 int drd_batch_create(int32_t count, const drd_params* params, drd_batch** out);
 int drd_batch_destroy(drd_batch* b);
@@ -507,19 +507,22 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
 /* ---- multigrid velocity preconditioner -----------------------------------
  * No reference counterpart (an addition beside the drop-in).
  * DRD_PS_VELO_MG is one V-cycle of geometric multigrid on the BDF2 velocity
- * operator (power-of-two nx >= 128, ny >= 64):
- *   - the levels halve the grid while nx >= 64 and ny >= 32 remain;
+ * operator (even nx >= 128, ny >= 64):
+ *   - the levels halve the grid while the sizes are even and the halves keep
+ *     nx >= 64 and ny >= 32;
  *   - each coarse operator is the same MAC stencil with doubled spacings, on
  *     rho, eta, chi averaged over 2x2 cells;
  *   - the transfers are staggered bilinear prolongation P and R = P^T / 4;
  *   - damped Jacobi smoothing (omega = 0.7, exact diagonal): two sweeps
  *     before the coarse correction and two after;
- *   - the FP64 DCT/DST fast-Poisson solve on the coarsest level.
+ *   - the FP64 DCT/DST fast-Poisson solve on the coarsest level: the FFT
+ *     form when its sizes are powers of two, dense line transforms
+ *     otherwise (a coarsest level of at most 256 x 256 cells).
  * It is a fixed linear map, so BiCGSTAB accepts it.  Its BiCGSTAB iteration
  * counts stay nearly constant as the grid is refined, unlike the fast-Poisson
  * preconditioner's.  velo_prec = DRD_PREC_MG makes drd_step solve the BDF2
- * velocity system with it (the first, BDF1 step keeps Jacobi); batches do not
- * take it.
+ * velocity system with it (the first, BDF1 step takes its BDF1 form); batches
+ * take it too.
  * DRD_PS_VELO_MG32 / DRD_PREC_MG32: the same cycle with every level but the
  * coarsest stored in FP32 (coefficients, diagonals, vectors; level 0 holds an
  * FP32 copy of rho, eta, chi), which halves the bytes of each sweep, and its

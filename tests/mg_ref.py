@@ -99,9 +99,11 @@ def coarse_cell(a, nx, ny):
 
 
 def levels(o, p, Oracle):
-    """[(params, oracle, diagonal)] from the fine level down (nx >= 64, ny >= 32)."""
+    """[(params, oracle, diagonal)] from the fine level down: halve while the
+    sizes are even and the halves keep nx >= 64, ny >= 32."""
     out = [(p, o, prec_ref.probe_diag(o, abi.OP_VELO2, p))]
-    while out[-1][0].nx // 2 >= 64 and out[-1][0].ny // 2 >= 32:
+    while (out[-1][0].nx // 2 >= 64 and out[-1][0].ny // 2 >= 32 and out[-1][0].nx % 2 == 0
+           and out[-1][0].ny % 2 == 0):
         pf, of, _ = out[-1]
         q = pf.copy()
         q.nx //= 2

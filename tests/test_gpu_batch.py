@@ -42,15 +42,20 @@ def test_batch_groups_and_field_round_trip():
                 np.testing.assert_array_equal(b.get_field(i, fid), cfg.fields[fid])
 
 
-@pytest.mark.parametrize("velo,graphs", [("PREC_MG32", False), ("PREC_MG32", True), ("PREC_MG", False)])
-def test_batch_multigrid_steps_are_bitwise_the_single_instance_steps(velo, graphs):
+@pytest.mark.parametrize("velo,graphs,sizes", [("PREC_MG32", False, (128, 256)), ("PREC_MG32", True, (128, 256)),
+                                               ("PREC_MG", False, (128, 256)), ("PREC_MG32", False, (160, 224, 288))])
+def test_batch_multigrid_steps_are_bitwise_the_single_instance_steps(velo, graphs, sizes):
     """Batch groups with the multigrid velocity preconditioner (each
     instance's BDF1 and BDF2 hierarchies inside its block of the group's
     memory, every level's launches over all instances) and the FP32
     red-black CH preconditioner: three steps of every instance equal the same
-    instance stepped alone, fields and Krylov reports bit for bit."""
+    instance stepped alone, fields and Krylov reports bit for bit.  The size
+    groups step at once from host threads of their own; sizes that are
+    multiples of 32 but not powers of two take the dense-transform coarsest
+    solve."""
     from drd_application_amd.solver import GpuBatch, GpuSolver
-    cfgs = _configs(n=6, seed=5, sizes=(128, 256))  # three of each size
+    cfgs = _configs(n=6, seed=5, sizes=sizes)
+    assert len({c.params.nx for c in cfgs}) >= 2
     for c in cfgs:
         c.params.velo_prec = getattr(abi, velo)
         c.params.ch_prec = abi.CH_PREC_RBGS32

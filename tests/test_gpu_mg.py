@@ -34,7 +34,7 @@ def _channel(nx, ny):
                                      lambda_eta=100.0, penalty=1e6, dt=5e-5, steps=2)
 
 
-@pytest.mark.parametrize("nx,ny", [(128, 64), (256, 128)])
+@pytest.mark.parametrize("nx,ny", [(128, 64), (256, 128), (160, 80), (288, 144)])
 def test_vcycle_matches_reference(nx, ny):
     import mg_ref
     from drd_application_amd.solver import GpuSolver
@@ -46,7 +46,7 @@ def test_vcycle_matches_reference(nx, ny):
     o = Oracle(cfg.params)
     geometry.load(o, cfg)
     lv = mg_ref.levels(o, cfg.params, Oracle)
-    assert len(lv) == (2 if nx == 128 else 3)
+    assert len(lv) == {128: 2, 256: 3, 160: 2, 288: 3}[nx]  # coarsest 64x32, 64x32, 80x40, 72x36
     b = np.random.default_rng(5).standard_normal(abi.n_uv(cfg.params))
     want = mg_ref.vcycle(lv, b)
     got = g.psolve(abi.PS_VELO_MG, abi.OP_VELO2, b)
@@ -209,7 +209,8 @@ def test_bdf1_preconditioners(prec):
     g.close()
 
 
-@pytest.mark.parametrize("nx,ny,lr", [(128, 64, "n"), (256, 128, "d"), (512, 256, "n"), (2048, 1024, "n")])
+@pytest.mark.parametrize("nx,ny,lr", [(128, 64, "n"), (256, 128, "d"), (512, 256, "n"), (2048, 1024, "n"),
+                                     (160, 80, "n"), (352, 176, "d")])
 def test_tiled_cycle_equals_per_pass_cycle(nx, ny, lr):
     """The V(2,2) cycle run as one LDS-tiled launch per level and leg equals
     the cycle run as one launch per sweep and transfer bit for bit, FP64 and
