@@ -12,6 +12,8 @@
 * 1024x512 holds more tiles than the GPU runs at once, so some sweep
   workgroups start after their neighbours have finished: the sweeps must
   read one buffer and write another for the result to stay exact.
+  1100x600 (684 tiles of 64x16, a ragged last column and row of tiles)
+  adds a grid that is neither a power of two nor a whole number of tiles.
 * DRD_PS_PHMU_RBGS32 (FP32 mobilities, b, iterates and arithmetic, with
   reciprocals for the divisions; the 2x2 block determinant is a sum of
   positive terms, so FP32 loses nothing there) equals the FP64 form to a
@@ -54,7 +56,7 @@ def _case(nx, ny):
     return cfg
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 32), (130, 70), (67, 35), (1024, 512)])
+@pytest.mark.parametrize("nx,ny", [(64, 32), (130, 70), (67, 35), (1024, 512), (1100, 600)])
 def test_rbgs_matches_reference(nx, ny):
     from drd_application_amd.solver import GpuSolver
     from oracle.oracle import Oracle

@@ -15,9 +15,13 @@
   its early exit), and the projection inverts the 5-point Laplacian
   (tests/step_check.py).  The drop-in's non-converged solves report the
   residual the oracle measures.
-* c2 (256 instances, sizes 128x64 .. 1024x512, FP32 fast-Poisson velocity
-  and FP32 red-black CH preconditioners as the bench runs them): two batched
-  steps, every solve converged, and one instance of each size checked as c1.
+* c2 (256 instances, nx on the multiples of 32 in 128 .. 1024, ny = nx / 2,
+  FP32 multigrid velocity and FP32 red-black CH preconditioners as the bench
+  runs them): two batched steps, every solve converged, and one instance of
+  every sixth size and of the largest checked as c1.  (With the FP32
+  fast-Poisson velocity preconditioner, the highest-contrast instances take
+  nearly the reference's 800 BDF1 iterations, so whether one of them
+  converges within the cap turns on rounding-level changes elsewhere.)
 * c4 (4096 x 2048 serpentine, 400 particles, 64 fibers of 20 beads): two
   steps of the headline mode with the bodies' forces spread and the bodies
   moved on the host between the steps: converged, checked as c1, and every
@@ -148,11 +152,12 @@ class _Inst:
 def test_c2_256_instances():
     from drd_application_amd.solver import GpuBatch
     _, Oracle = _solvers()
-    cfgs = geometry.batch_configs(256)
-    for c in cfgs:  # the batched workload's CH preconditioner (block Jacobi breaks down on some instances)
+    cfgs = geometry.batch_configs(256, sizes=geometry.C2_SIZES)
+    for c in cfgs:  # the batched workload's preconditioners (CH block Jacobi breaks down on some instances)
+        c.params.velo_prec = abi.PREC_MG32
         c.params.ch_prec = abi.CH_PREC_RBGS32
     sizes = sorted({c.params.nx for c in cfgs})
-    sample = [next(i for i, c in enumerate(cfgs) if c.params.nx == n) for n in sizes]
+    sample = [next(i for i, c in enumerate(cfgs) if c.params.nx == n) for n in sorted(set(sizes[::6] + sizes[-1:]))]
     print(f"c2: 256 instances, sizes {sizes}, checking instances {sample}", flush=True)
     with GpuBatch([c.params for c in cfgs]) as b:
         for i, cfg in enumerate(cfgs):
