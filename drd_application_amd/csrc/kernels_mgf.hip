@@ -115,12 +115,62 @@ struct MgSlots {
   }
 };
 
+// One stencil evaluation at slot j of the block.  FP64 levels: velo_u_at /
+// velo_v_at.  FP32 levels: the regrouped form of velo_rows.hpp, with the
+// corner viscosities of the staged cells in LDS and each slot's centre
+// coefficient D in a register, both formed once per leg.
+template <class T>
+struct MgEvalRef {
+  const ScalT<T>& s;
+  const CoefL<T>& C;
+  template <class XV>
+  __device__ __forceinline__ double operator()(const XV& X, const MgPos& m, int) const {
+    return m.u ? (double)velo_u_at<2, T>(s, C, X, m.i, m.k) : (double)velo_v_at<2, T>(s, C, X, m.i, m.k);
+  }
+};
+template <class T, int NS>
+struct MgEvalPc {
+  PcK<T> K;
+  const CoefL<T>& C;
+  PcQArr<T> Q;
+  T d[NS];
+  template <class XV>
+  __device__ __forceinline__ double operator()(const XV& X, const MgPos& m, int j) const {
+    return m.u ? (double)velo_u_pc(K, C, Q, X, m.i, m.k, d[j]) : (double)velo_v_pc(K, C, Q, X, m.i, m.k, d[j]);
+  }
+};
+// Forms the corners (into sq, C's index space: the corner north-east of each
+// staged cell whose three neighbours there are staged; 0 elsewhere, where no
+// stencil of the leg reads it) and D at the slots of rings <= RMAX.  Ends
+// with a barrier; the cells must be staged and visible.
+template <class T, int TW, int TH, int H, int RMAX>
+__device__ __forceinline__ MgEvalPc<T, MgBlock<TW, TH, H>::NS> mg_eval_pc(const ScalT<T>& s, const CoefL<T>& C,
+                                                                          T* sq, const MgSlots<TW, TH, H>& P) {
+  using B = MgBlock<TW, TH, H>;
+  constexpr int CROWS = B::NC / B::CLD;
+#pragma unroll
+  for (int it = 0; it < B::NCS; ++it) {
+    const int p = it * kMgfThreads + (int)threadIdx.x;
+    if (p >= B::NC) continue;
+    const int lk = p / B::CLD, li = p - lk * B::CLD;
+    sq[p] = (li < B::CLD - 1 && lk < CROWS - 1) ? pc_corner<T>(C, (uint32_t)p) : T(0);
+  }
+  __syncthreads();
+  MgEvalPc<T, B::NS> A{pc_k<T>(s), C, PcQArr<T>{sq}, {}};
+#pragma unroll
+  for (int j = 0; j < B::NS; ++j) {
+    const MgPos& m = P.m[j];
+    A.d[j] = m.ring > RMAX ? T(0) : (m.u ? pc_du(A.K, C, A.Q, m.i, m.k) : pc_dv(A.K, C, A.Q, m.i, m.k));
+  }
+  return A;
+}
+
 // Stages of the down leg after the first (x1 in sa): x2 into sb and xd, r
 // over sa, restriction into bc.  XV is the LDS vector view (ghost rules, or
 // none in an interior block).
-template <class XV, class T, class TBC, int TW, int TH>
-__device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, const ScalT<T>& s, const MgSlots<TW, TH, 3>& P,
-                                               const CoefL<T>& C, const XV& XA, const XV& XB, T* sa, T* sb,
+template <class XV, class EV, class T, class TBC, int TW, int TH>
+__device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, const EV& A, const MgSlots<TW, TH, 3>& P,
+                                               const XV& XA, const XV& XB, T* sa, T* sb,
                                                const T (&bv)[MgBlock<TW, TH, 3>::NS],
                                                const T (&wv)[MgBlock<TW, TH, 3>::NS], T* __restrict__ xd,
                                                TBC* __restrict__ bc, int ci0, int ck0) {
@@ -131,7 +181,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
     const MgPos& m = P.m[j];
     if (m.ring > 2) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
-    const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
+    const double z = A(XA, m, j);
     const T x2 = (T)((double)sa[p] + ((double)bv[j] - z) * (double)wv[j]);
     sb[p] = x2;
     if (m.own) xd[mg_packed(g, m.u, m.i, m.k)] = x2;
@@ -143,7 +193,7 @@ __device__ __forceinline__ void mg_down_stages(const Grid& g, const Grid& gc, co
     const MgPos& m = P.m[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
-    const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
+    const double z = A(XB, m, j);
     sa[p] = (T)((double)bv[j] - z);
   }
   __syncthreads();
@@ -194,7 +244,9 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
   bc = bofs(bc, bs);
   st = bofs(st, bs);
   using B = MgBlock<TW, TH, H>;
+  constexpr bool PC = sizeof(T) == sizeof(float);
   __shared__ T sc[3 * B::NC];
+  __shared__ T sq[PC ? B::NC : 1];
   __shared__ T sa[B::NP];
   __shared__ T sb[B::NP];
   const int ntx = (g.nx + TW - 1) / TW;
@@ -229,22 +281,28 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_down(Grid g, Grid gc, Scal s
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
   const ScalT<T> sT = ScalT<T>::make(s);  // converted once, not at every unknown
-  if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
-    const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
-    const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
-    mg_down_stages(g, gc, sT, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
-  } else {
-    const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    mg_down_stages(g, gc, sT, P, C, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
-  }
+  auto legs = [&](const auto& A) {
+    if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
+      const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
+      const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
+      mg_down_stages(g, gc, A, P, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
+    } else {
+      const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
+      const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
+      mg_down_stages(g, gc, A, P, XA, XB, sa, sb, bv, wv, xd, bc, ci0, ck0);
+    }
+  };
+  if constexpr (PC)
+    legs(mg_eval_pc<T, TW, TH, H, 2>(sT, C, sq, P));  // stencils at rings <= 2
+  else
+    legs(MgEvalRef<T>{sT, C});
 }
 
 // Stages of the up leg after the first (x2 + P xc in sa): a sweep into sb,
 // a sweep into out.
-template <class XV, class T, class TO, int TW, int TH>
-__device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, const MgSlots<TW, TH, 2>& P,
-                                             const CoefL<T>& C, const XV& XA, const XV& XB, T* sa, T* sb,
+template <class XV, class EV, class T, class TO, int TW, int TH>
+__device__ __forceinline__ void mg_up_stages(const Grid& g, const EV& A, const MgSlots<TW, TH, 2>& P,
+                                             const XV& XA, const XV& XB, T* sa, T* sb,
                                              const T (&bv)[MgBlock<TW, TH, 2>::NS],
                                              const T (&wv)[MgBlock<TW, TH, 2>::NS], TO* __restrict__ out) {
   using B = MgBlock<TW, TH, 2>;
@@ -254,7 +312,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, c
     const MgPos& m = P.m[j];
     if (m.ring > 1) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
-    const double z = m.u ? (double)velo_u_at<2, T>(s, C, XA, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XA, m.i, m.k);
+    const double z = A(XA, m, j);
     sb[p] = (T)((double)sa[p] + ((double)bv[j] - z) * (double)wv[j]);
   }
   __syncthreads();
@@ -264,7 +322,7 @@ __device__ __forceinline__ void mg_up_stages(const Grid& g, const ScalT<T>& s, c
     const MgPos& m = P.m[j];
     if (!m.own) continue;
     const int p = j * kMgfThreads + (int)threadIdx.x;
-    const double z = m.u ? (double)velo_u_at<2, T>(s, C, XB, m.i, m.k) : (double)velo_v_at<2, T>(s, C, XB, m.i, m.k);
+    const double z = A(XB, m, j);
     out[mg_packed(g, m.u, m.i, m.k)] = (TO)((double)sb[p] + ((double)bv[j] - z) * (double)wv[j]);
   }
 }
@@ -288,7 +346,9 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
   out = bofs(out, bs);
   st = bofs(st, bs);
   using B = MgBlock<TW, TH, H>;
+  constexpr bool PC = sizeof(T) == sizeof(float);
   __shared__ T sc[3 * B::NC];
+  __shared__ T sq[PC ? B::NC : 1];
   __shared__ T sa[B::NP];
   __shared__ T sb[B::NP];
   const int ntx = (g.nx + TW - 1) / TW;
@@ -330,15 +390,21 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
   __syncthreads();
   const CoefL<T> C{sc, sc + B::NC, sc + 2 * B::NC, ci0 - H, ck0 - H, (uint32_t)B::CLD};
   const ScalT<T> sT = ScalT<T>::make(s);  // converted once, not at every unknown
-  if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
-    const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
-    const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
-    mg_up_stages(g, sT, P, C, XA, XB, sa, sb, bv, wv, out);
-  } else {
-    const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
-    mg_up_stages(g, sT, P, C, XA, XB, sa, sb, bv, wv, out);
-  }
+  auto legs = [&](const auto& A) {
+    if (mg_interior<TW, TH, H>(g, ci0, ck0)) {
+      const VeloLdsIn<T> XA{sa, sa + B::NV, ci0 - H, ck0 - H, B::LD};
+      const VeloLdsIn<T> XB{sb, sb + B::NV, ci0 - H, ck0 - H, B::LD};
+      mg_up_stages(g, A, P, XA, XB, sa, sb, bv, wv, out);
+    } else {
+      const VeloLds<T> XA{sa, sa + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
+      const VeloLds<T> XB{sb, sb + B::NV, g.nx, g.ny, s.lr_d, ci0 - H, ck0 - H, B::LD};
+      mg_up_stages(g, A, P, XA, XB, sa, sb, bv, wv, out);
+    }
+  };
+  if constexpr (PC)
+    legs(mg_eval_pc<T, TW, TH, H, 1>(sT, C, sq, P));  // stencils at rings <= 1
+  else
+    legs(MgEvalRef<T>{sT, C});
 }
 
 // Block shapes: 32 x 32 cells for FP32 levels of nx >= 1024 (41 KB of LDS

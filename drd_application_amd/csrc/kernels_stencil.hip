@@ -1164,13 +1164,21 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(Grid g, Scal s, const T* _
     return;
   }
   double z, xq;
+  // FP32 levels: the regrouped form of velo_row_pc
+  constexpr bool PC = sizeof(T) == sizeof(float);
   if (KIND == MG_SWEEP0) {
     const VeloAcc<LdFirst<T, TB>> X{{b, dw}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
-    z = velo_row<2>(g, s, rho, eta, chi, X, q);
+    if constexpr (PC)
+      z = velo_row_pc(g, s, rho, eta, chi, X, q);
+    else
+      z = velo_row<2>(g, s, rho, eta, chi, X, q);
     xq = X.ld(q);
   } else {
     const VeloInT<T> X{{x}, g.nx, g.ny, s.lr_d, (uint32_t)g.mn()};
-    z = velo_row<2>(g, s, rho, eta, chi, X, q);
+    if constexpr (PC)
+      z = velo_row_pc(g, s, rho, eta, chi, X, q);
+    else
+      z = velo_row<2>(g, s, rho, eta, chi, X, q);
     xq = (KIND == MG_RES) ? 0.0 : (double)x[q];
   }
   if (late && st && (st->done || st->early)) return;
