@@ -637,7 +637,11 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 // one cell from the walls and side columns (interior) takes the update
 // without the clamped neighbour indices and wall rows, which are identities
 // there.
-constexpr int kRbTX = 64, kRbTY = 16, kRbThreads = 1024;  // a thread per cell of the widest phase
+// 32 x 16 tiles of 512 threads (a thread per cell of the widest phase): the
+// FP32 middle launch at 2048 x 1024 took 29.6 us, against 31.3-52.9 us for
+// 64 x 16, 32 x 32, 48 x 24, 32 x 24, 48 x 16, 24 x 16, 16 x 16, 64 x 8,
+// 32 x 8 and 128 x 8 tiles (256 to 1024 threads)
+constexpr int kRbTX = 32, kRbTY = 16, kRbThreads = 512;
 template <bool FIRST, int NS, class TC, class TB, class TI, class TO>
 __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, const TC* __restrict__ bu,
                                                           const TC* __restrict__ minner,
