@@ -101,25 +101,30 @@ __device__ double2* lds_fft(double2* a, double2* b, int N, const double2* __rest
   return a;
 }
 
-constexpr int kMaxPer = 16;  // line values per thread: N <= 4096 with kBlock threads
+// A line per workgroup of kRowThreads threads, kMaxPer values each (N <=
+// 4096): 1024 threads give each radix-2 pass one butterfly per thread and
+// 16 waves per line to hide the passes' LDS latency (the same butterflies in
+// the same order as with fewer threads).
+constexpr int kRowThreads = 1024, kRowWaves = kRowThreads / 64;
+constexpr int kMaxPer = 4096 / kRowThreads;  // line values per thread
 
 __device__ __forceinline__ int makhoul(int j, int N) { return (j & 1) ? (N - 1 - (j >> 1)) : (j >> 1); }
 
-// REDFT10 of a line held in registers (r[m] = x[threadIdx.x + m*kBlock]).
+// REDFT10 of a line held in registers (r[m] = x[threadIdx.x + m*kRowThreads]).
 __device__ void dct2_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, double2* B,
                           const double2* __restrict__ tw, const double2* __restrict__ rot,
                           const double* __restrict__ cs) {
   if (pow2 && N >= 2) {
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int j = threadIdx.x + m * kBlock;
+      const int j = threadIdx.x + m * kRowThreads;
       if (j < N) A[makhoul(j, N)] = make_double2(r[m], 0.0);
     }
     __syncthreads();
     const double2* V = lds_fft(A, B, N, tw, -1);
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int k = threadIdx.x + m * kBlock;
+      const int k = threadIdx.x + m * kRowThreads;
       if (k < N) {
         const double2 w = rot[k], v = V[k];
         r[m] = 2.0 * (w.x * v.x - w.y * v.y);
@@ -131,13 +136,13 @@ __device__ void dct2_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
     double* xs = reinterpret_cast<double*>(A);
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int j = threadIdx.x + m * kBlock;
+      const int j = threadIdx.x + m * kRowThreads;
       if (j < N) xs[j] = r[m];
     }
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int k = threadIdx.x + m * kBlock;
+      const int k = threadIdx.x + m * kRowThreads;
       if (k < N) {
         double s = 0.0;
         for (int j = 0; j < N; ++j) s += xs[j] * cs[((long)(2 * j + 1) * k) % (4L * N)];
@@ -155,7 +160,7 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
   double* ys = reinterpret_cast<double*>(B);
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int k = threadIdx.x + m * kBlock;
+    const int k = threadIdx.x + m * kRowThreads;
     if (k < N) ys[k] = r[m];
   }
   __syncthreads();
@@ -164,7 +169,7 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
     // inverse FFT; x_{2n} = v_n, x_{2n+1} = v_{N-1-n}
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int k = threadIdx.x + m * kBlock;
+      const int k = threadIdx.x + m * kRowThreads;
       if (k < N) {
         const double a = ys[k];
         const double b = (k == 0) ? 0.0 : -ys[N - k];
@@ -176,7 +181,7 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
     const double2* v = lds_fft(A, B, N, tw, +1);
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int j = threadIdx.x + m * kBlock;
+      const int j = threadIdx.x + m * kRowThreads;
       if (j < N) r[m] = v[makhoul(j, N)].x;
     }
     __syncthreads();
@@ -184,7 +189,7 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
     // definition: X_k = Y_0 + 2 sum_{j>=1} Y_j cos(pi j (2k+1) / (2N))
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int k = threadIdx.x + m * kBlock;
+      const int k = threadIdx.x + m * kRowThreads;
       if (k < N) {
         double s = 0.0;
         for (int j = 1; j < N; ++j) s += ys[j] * cs[((long)j * (2 * k + 1)) % (4L * N)];
@@ -201,14 +206,14 @@ __device__ void dct3_regs(double (&r)[kMaxPer], int N, bool pow2, double2* A, do
 __device__ void r2hc_regs(double (&r)[kMaxPer], int N, double2* A, double2* B, const double2* __restrict__ tw) {
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int j = threadIdx.x + m * kBlock;
+    const int j = threadIdx.x + m * kRowThreads;
     if (j < N) A[j] = make_double2(r[m], 0.0);
   }
   __syncthreads();
   const double2* V = lds_fft(A, B, N, tw, -1);
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int k = threadIdx.x + m * kBlock;
+    const int k = threadIdx.x + m * kRowThreads;
     if (k < N) r[m] = k <= N / 2 ? V[k].x : V[N - k].y;
   }
   __syncthreads();
@@ -220,14 +225,14 @@ __device__ void hc2r_regs(double (&r)[kMaxPer], int N, double2* A, double2* B, c
   double* ys = reinterpret_cast<double*>(B);
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int k = threadIdx.x + m * kBlock;
+    const int k = threadIdx.x + m * kRowThreads;
     if (k < N) ys[k] = r[m];
   }
   __syncthreads();
   const int h = N / 2;
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int k = threadIdx.x + m * kBlock;
+    const int k = threadIdx.x + m * kRowThreads;
     if (k < N) {
       double2 v;
       if (k == 0 || k == h)
@@ -243,7 +248,7 @@ __device__ void hc2r_regs(double (&r)[kMaxPer], int N, double2* A, double2* B, c
   const double2* V = lds_fft(A, B, N, tw, +1);
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int j = threadIdx.x + m * kBlock;
+    const int j = threadIdx.x + m * kRowThreads;
     if (j < N) r[m] = V[j].x;
   }
   __syncthreads();
@@ -275,7 +280,7 @@ struct RowArgs {
 
 // One workgroup per line; dynamic LDS = 2N complex.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
+__global__ __launch_bounds__(kRowThreads) void k_rows(RowArgs a, int64_t bs) {
   a.in = bofs(a.in, bs);  // the tables (tw, rot, cs, pre) are shared by the group
   a.out = bofs(a.out, bs);
   a.u = bofs(a.u, bs);
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
   double r[kMaxPer];
 #pragma unroll
   for (int m = 0; m < kMaxPer; ++m) {
-    const int i = threadIdx.x + m * kBlock;
+    const int i = threadIdx.x + m * kRowThreads;
     r[m] = 0.0;
     if (i < N) {
       if (MODE == R_DIV_DCT2) {
@@ -308,19 +313,19 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
     }
   }
   // canonical (pairwise) sum of the row's divergence, taken before the
-  // transform overwrites it; element i = m*kBlock + t lies in 64-chunk
-  // m*kWaves + wave.
+  // transform overwrites it; element i = m*kRowThreads + t lies in 64-chunk
+  // m*kRowWaves + wave.
   double rowsum = 0.0;
   if (MODE == R_DIV_DCT2 && a.ks) {
-    __shared__ double lc[kMaxPer * kWaves];
+    __shared__ double lc[kMaxPer * kRowWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
       const double c = wave_pairwise(r[m]);
-      if (lane == 0) lc[m * kWaves + w] = c;
+      if (lane == 0) lc[m * kRowWaves + w] = c;
     }
     __syncthreads();
-    if (threadIdx.x == 0) rowsum = pairwise_fixed<kMaxPer * kWaves>(lc);
+    if (threadIdx.x == 0) rowsum = pairwise_fixed<kMaxPer * kRowWaves>(lc);
     __syncthreads();
   }
   if (MODE == R_DIV_DCT2 && a.per)
@@ -332,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
     const int ldp = a.g.nx + 1;
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int ky = threadIdx.x + m * kBlock;
+      const int ky = threadIdx.x + m * kRowThreads;
       if (ky < N) r[m] = r[m] / a.pre[row + (size_t)ldp * ky];
     }
   }
@@ -355,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
     double* xs = reinterpret_cast<double*>(A);
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int i = threadIdx.x + m * kBlock;
+      const int i = threadIdx.x + m * kRowThreads;
       if (i < N) xs[i] = r[m];
     }
     __syncthreads();
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(RowArgs a, int64_t bs) {
   } else {
 #pragma unroll
     for (int m = 0; m < kMaxPer; ++m) {
-      const int i = threadIdx.x + m * kBlock;
+      const int i = threadIdx.x + m * kRowThreads;
       if (i < N) a.out[(size_t)row * N + i] = r[m];
     }
   }
@@ -407,7 +412,7 @@ static void launch_rows(hipStream_t st, const RowArgs& a) {
     return true;
   }();
   (void)raised;
-  hipLaunchKernelGGL(k_rows<MODE>, dim3(a.rows, 1, bat_n()), dim3(kBlock), row_lds(a.N), st, a, bat_bs());
+  hipLaunchKernelGGL(k_rows<MODE>, dim3(a.rows, 1, bat_n()), dim3(kRowThreads), row_lds(a.N), st, a, bat_bs());
 }
 
 static void launch_transpose(hipStream_t st, int rows, int cols, const double* in, double* out) {
