@@ -408,9 +408,11 @@ __global__ __launch_bounds__(kMgfThreads) void k_mg_up(Grid g, Grid gc, Scal s, 
 }
 
 // Block shapes: 32 x 16 cells for levels of nx >= 1024 (FP64: 48 KB of LDS
-// going down), 16 x 16 below (more blocks where the level is small).  For
+// going down), 16 x 8 below (more blocks where the level is small).  For
 // the FP32 levels 32 x 16 beat 32 x 32, 64 x 16, 32 x 24, 48 x 16 and 32 x 8
-// (V-cycle at 2048 x 1024: 199 us against 209-216 us).
+// (V-cycle at 2048 x 1024: 199 us against 209-216 us); on the small levels
+// 16 x 8 beat 16 x 16, 8 x 8 and 32 x 8 (V-cycle at 256 x 128: 47.9 us
+// against 52.5 us with 16 x 16).
 template <class T, bool SMALL>
 struct MgfShape;
 template <>
@@ -423,7 +425,7 @@ struct MgfShape<double, false> {
 };
 template <class T>
 struct MgfShape<T, true> {
-  static constexpr int TW = 16, TH = 16;
+  static constexpr int TW = 16, TH = 8;
 };
 static bool mgf_small(const Grid& g) { return g.nx < 1024; }
 
