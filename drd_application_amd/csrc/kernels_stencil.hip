@@ -1486,14 +1486,14 @@ void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const 
                      g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs(), c32.d11, c32.d22, c32.q);
 }
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
-// eight full sweeps (red cells, then black), from x = 0, kRbNS per launch,
+// twelve full sweeps (red cells, then black), from x = 0, kRbNS per launch,
 // alternating between work and x so that the last launch writes x.
 // FP64: work holds 2 ncell doubles.  FP32 storage (c32 = FP32 bu, M_inner,
 // M_0): work holds 6 ncell floats, two FP32 iterates and the FP32 b the
 // first launch writes.
 constexpr int kRbgsLaunches = kRbgsMiddle + 2;
 static_assert(kRbgsLaunches % 2 == 0, "the last launch must write x");
-static_assert(kRbgsLaunches * kRbNS == 8, "eight sweeps per application");
+static_assert(kRbgsLaunches * kRbNS == 12, "twelve sweeps per application");
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm) {
   const dim3 grid = rb_grid(g);

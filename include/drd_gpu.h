@@ -56,7 +56,7 @@ extern "C" {
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
 #define DRD_CH_PREC_BLOCK 1     /* exact 2x2 per-cell block Jacobi (no
                                    reference counterpart)                   */
-#define DRD_CH_PREC_RBGS 2      /* eight red-black Gauss-Seidel sweeps of the
+#define DRD_CH_PREC_RBGS 2      /* twelve red-black Gauss-Seidel sweeps of the
                                    same 2x2 blocks (no reference
                                    counterpart)                             */
 #define DRD_CH_PREC_RBGS32 3    /* the same sweeps in FP32: mobilities, b,

@@ -27,7 +27,7 @@ from drd_application_amd import abi, geometry
 
 pytestmark = pytest.mark.gpu
 
-SWEEPS = 8
+SWEEPS = 12
 
 
 def rel_l2(a, b):
