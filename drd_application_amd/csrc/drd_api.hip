@@ -2853,8 +2853,13 @@ extern "C" int drd_batch_step(drd_batch* b, int32_t bdf, double t, drd_step_info
     for (size_t gI = 0; gI < G; ++gI) run(gI);
   } else {
     std::vector<std::thread> th;
-    th.reserve(G);
-    for (size_t gI = 0; gI < G; ++gI) th.emplace_back(run, gI);
+    size_t started = 0;
+    try {
+      th.reserve(G);
+      for (; started < G; ++started) th.emplace_back(run, started);
+    } catch (...) {  // no thread for the rest (system limits): they step on this one
+    }
+    for (size_t gI = started; gI < G; ++gI) run(gI);
     for (auto& x : th) x.join();
   }
   for (size_t gI = 0; gI < G; ++gI)
