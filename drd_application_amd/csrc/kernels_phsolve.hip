@@ -87,8 +87,12 @@ static size_t vcost_lds(int n) {
   return 2 * (size_t)(2 * n) * sizeof(Cx<double>) + (size_t)(n + 1) * sizeof(double);
 }
 
-// lines l < nlines of length t.n: in[l*ld_in + i] -> VCOST -> out[l*ld_out + i]
-__global__ __launch_bounds__(kBlock) void k_vcost_lines(D1 t, int nlines, const double* in, int64_t ld_in,
+// lines l < nlines of length t.n: in[l*ld_in + i] -> VCOST -> out[l*ld_out + i],
+// one line per workgroup of kVcostThreads threads (each radix-2 pass of a
+// line up to 2048 points is one butterfly per thread; the same butterflies in
+// the same order as with fewer threads)
+constexpr int kVcostThreads = 1024;
+__global__ __launch_bounds__(kVcostThreads) void k_vcost_lines(D1 t, int nlines, const double* in, int64_t ld_in,
                                                         double* out, int64_t ld_out, const double* cof,
                                                         int64_t ld_cof, const KState* st) {
   if (st && (st->done || st->early)) return;
@@ -133,7 +137,7 @@ static void vcost_launch(hipStream_t s, const Dct1Tab* t, int nlines, const doub
     return true;
   }();
   (void)raised;
-  hipLaunchKernelGGL(k_vcost_lines, dim3(nlines), dim3(kBlock), vcost_lds(t->n), s, view(t), nlines, in, ld_in, out,
+  hipLaunchKernelGGL(k_vcost_lines, dim3(nlines), dim3(kVcostThreads), vcost_lds(t->n), s, view(t), nlines, in, ld_in, out,
                      ld_out, cof, ld_cof, ks);
 }
 
