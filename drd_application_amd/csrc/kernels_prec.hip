@@ -351,7 +351,12 @@ template <class T, int LMAX>
 constexpr int cols_threads() {
   return kBlock;
 }
-template <class T, int LMAX>
+// STRIDED: the columns are read and written in place in the row-major
+// blocks of pass 1 (element i of u column c at tu[i (nx + 1) + c], of v
+// column c at tv[i nx + c]) instead of in transposed blocks: for small grids,
+// whose blocks stay in cache, this saves the two transpose launches.  The
+// same transforms in the same order either way.
+template <class T, int LMAX, bool STRIDED = false>
 __global__ __launch_bounds__((cols_threads<T, LMAX>())) void k_prec_cols(Grid g, PrecTables<T> tb, T* __restrict__ tu,
                                                                        T* __restrict__ tv, const KState* st,
                                                                        int64_t bs) {
@@ -369,11 +374,13 @@ __global__ __launch_bounds__((cols_threads<T, LMAX>())) void k_prec_cols(Grid g,
   if ((int)blockIdx.x < nu) {  // u columns: DST-II, / lambda, DST-III (n = ny)
     const int c0 = 2 * blockIdx.x, n = ny;
     const bool has1 = c0 + 1 < nx + 1;
-    T* col0 = tu + (int64_t)c0 * n;
-    T* col1 = col0 + n;
+    // element i of the pair's columns: col0[i * cs], col1[i * cs]
+    const int64_t cs = STRIDED ? nx + 1 : 1;
+    T* col0 = STRIDED ? tu + c0 : tu + (int64_t)c0 * n;
+    T* col1 = STRIDED ? col0 + 1 : col0 + n;
     const T e0 = tb.a0 + tb.ex_u[c0], e1 = has1 ? tb.a0 + tb.ex_u[c0 + 1] : T(1);
     dct2_pair<T, LMAX, NT>(A, n, tb.twny, tb.rotny, true,
-                 [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
+                 [&](int i) { return Cx<T>{col0[i * cs], has1 ? col1[i * cs] : T(0)}; },
                  [&](int k, T y0, T y1) {
                    const T ey = tb.ey_u[k];
                    l0[k] = y0 / (tb.sc * (e0 + ey));
@@ -381,16 +388,17 @@ __global__ __launch_bounds__((cols_threads<T, LMAX>())) void k_prec_cols(Grid g,
                  });
     dct3_pair<T, LMAX, NT>(A, n, tb.twny, tb.rotny, true, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int j, T x0, T x1) {
-                   col0[j] = x0;
-                   if (has1) col1[j] = x1;
+                   col0[j * cs] = x0;
+                   if (has1) col1[j * cs] = x1;
                  });
   } else {  // v columns: DST-I, / lambda, DST-I (n = ny-1)
     const int c0 = 2 * (blockIdx.x - nu), n = ny - 1;
     const bool has1 = c0 + 1 < nx;
-    T* col0 = tv + (int64_t)c0 * n;
-    T* col1 = col0 + n;
+    const int64_t cs = STRIDED ? nx : 1;
+    T* col0 = STRIDED ? tv + c0 : tv + (int64_t)c0 * n;
+    T* col1 = STRIDED ? col0 + 1 : col0 + n;
     const T e0 = tb.a0 + tb.ex_v[c0], e1 = has1 ? tb.a0 + tb.ex_v[c0 + 1] : T(1);
-    dst1_pair<T, LMAX, NT>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i], has1 ? col1[i] : T(0)}; },
+    dst1_pair<T, LMAX, NT>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{col0[i * cs], has1 ? col1[i * cs] : T(0)}; },
                  [&](int k, T y0, T y1) {
                    const T ey = tb.ey_v[k];
                    l0[k] = y0 / (tb.sc * (e0 + ey));
@@ -398,8 +406,8 @@ __global__ __launch_bounds__((cols_threads<T, LMAX>())) void k_prec_cols(Grid g,
                  });
     dst1_pair<T, LMAX, NT>(A, n, tb.tw2ny, [&](int i) { return Cx<T>{l0[i], l1[i]}; },
                  [&](int k, T y0, T y1) {
-                   col0[k] = y0;
-                   if (has1) col1[k] = y1;
+                   col0[k * cs] = y0;
+                   if (has1) col1[k * cs] = y1;
                  });
   }
 }
@@ -625,6 +633,9 @@ void velo_prec_setup(hipStream_t st, VeloPrec* pc, const Grid& g, const Scal& s,
                      st, g, s, c.rho, c.eta, c.chi, pc->a0, pc->inv_s64, pc->inv_s32, bat_bs());
 }
 
+// Grids up to this nx take the column pass in place (STRIDED); the coarsest
+// multigrid level, 64 x 32 at c1, is one of them.
+constexpr int kPrecStridedMaxNx = 256;
 // LMAX: longest row line (2nx); LMAXC: longest column line (2ny)
 template <class T, int LMAX, int LMAXC>
 static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T>& tb, const T* inv_s,
@@ -640,6 +651,8 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
     (void)hipFuncSetAttribute((const void*)k_prec_rows<T, false, LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               lim);
     (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAXC>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_prec_cols<T, LMAXC, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lim);
     return true;
   }();
   (void)raised;
@@ -647,6 +660,13 @@ static void launch_prec_passes(hipStream_t st, const Grid& g, const PrecTables<T
   const int64_t bs = bat_bs();
   hipLaunchKernelGGL((k_prec_rows<T, true, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb, r,
                      inv_s, wu, wv, (double*)nullptr, ks, bs);
+  if (nx <= kPrecStridedMaxNx) {  // small grids: columns in place, no transposes
+    hipLaunchKernelGGL((k_prec_cols<T, LMAXC, true>), dim3(cols_blocks, 1, B), dim3(cols_threads<T, LMAXC>()),
+                       lds_cols, st, g, tb, wu, wv, ks, bs);
+    hipLaunchKernelGGL((k_prec_rows<T, false, LMAX>), dim3(rows_blocks, 1, B), dim3(kBlock), lds_rows, st, g, tb,
+                       (const double*)nullptr, inv_s, wu, wv, z, ks, bs);
+    return;
+  }
   launch_transpose2T<T>(st, shape, TBlock<T>{ny, nx + 1, wu, tu}, TBlock<T>{ny - 1, nx, wv, tv}, ks, B, bs);
   hipLaunchKernelGGL((k_prec_cols<T, LMAXC>), dim3(cols_blocks, 1, B), dim3(cols_threads<T, LMAXC>()), lds_cols, st,
                      g, tb, tu, tv, ks, bs);
