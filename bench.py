@@ -16,9 +16,11 @@ Modes
             record solver-emitted pairs, the network trains on them, one more
             warmup step judges it (learned FGMRES against plain FGMRES and
             the fp32 method) and the timed steps use what the judge kept.
-  mg      : (the headline) the BDF2 velocity solve preconditioned by one
-            geometric multigrid V-cycle, the CH solve by four red-black
-            block Gauss-Seidel sweeps, extrapolated start.
+  mg      : the BDF2 velocity solve preconditioned by one geometric
+            multigrid V-cycle, the CH solve by sixteen red-black block SOR
+            sweeps (weight 1.25), extrapolated start, FP64.
+  mg32    : (the headline) the same with FP32 multigrid levels and FP32
+            red-black sweeps.
 
 The learned correction is also evaluated on its own workload (1024^2
 variable-viscosity velocity systems, FGMRES to 1e-10 with and without it on

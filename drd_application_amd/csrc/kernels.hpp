@@ -255,7 +255,9 @@ struct KTimer {
   int* n;
 };
 constexpr int kRbNS = 2;        // red-black sweeps per launch
-constexpr int kRbgsMiddle = 4;  // launches per red-black application besides the first and the last
+constexpr int kRbgsMiddle = 6;  // launches per red-black application besides the first and the last
+// over-relaxation weight of each red-black cell update: x_c += w B_c^-1 (b - A x)_c
+constexpr double kRbOmega = 1.25;
 // c32 null: FP64 throughout, work = 2 ncell doubles; else FP32 storage,
 // work = 6 ncell floats, and tm (optional) times the middle launches of each
 // application as one group of kRbgsMiddle launches

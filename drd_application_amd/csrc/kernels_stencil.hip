@@ -613,8 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_phmu_pack(Grid g, const double* __re
 
 // ---- red-black block Gauss-Seidel for the phi/mu system -------------------
 // NS full sweeps per launch: each the red cells ((i + j) even), then the
-// black ones, updated as x_c += B_c^-1 (b - A x)_c with the rows of the
-// matvec (phmu_rows_v) and the exact 2x2 block solve (phmu_block_v).
+// black ones, updated as x_c += w B_c^-1 (b - A x)_c (over-relaxation,
+// w = kRbOmega) with the rows of the matvec (phmu_rows_v) and the exact
+// 2x2 block solve (phmu_block_v).
 // A workgroup stages its tile of kRbTX x kRbTY cells and a halo of 2 NS
 // cells (phi, mu, b, both mobilities, bu) in LDS.  Sweep k of NS updates the
 // red cells of the tile and a ring of 2 (NS - k) - 1 cells around it, then
@@ -738,8 +739,8 @@ __global__ __launch_bounds__(kRbThreads) void k_phmu_rb_sweep(Grid g, Scal s, co
       phmu_block_x<TC>(s, bv, (TC)sd11[r][a], (TC)sd22[r][a], (TC)sq[r][a], sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
     else
       phmu_block_v<TC>(g, s, bv, mi, mz, i, j, sb1[r][a] - z1, sb2[r][a] - z2, y1, y2);
-    sx1[r][a] = f1.c + y1;
-    sx2[r][a] = f2.c + y2;
+    sx1[r][a] = f1.c + (TC)kRbOmega * y1;  // two roundings: no contraction in this build
+    sx2[r][a] = f2.c + (TC)kRbOmega * y2;
   };
   // threads take the cells of one colour in row order: a row of a region
   // kRbTX + 2 w wide holds (kRbTX + 2 w) / 2 cells of each colour
@@ -1486,14 +1487,14 @@ void launch_phmu_rb_sweep32(hipStream_t st, const Grid& g, const Scal& s, const 
                      g, s, c32.bu, c32.minner, c32.m0, b, nullptr, xi, xo, ks, bat_bs(), c32.d11, c32.d22, c32.q);
 }
 // x = the red-black block Gauss-Seidel preconditioner applied to b:
-// twelve full sweeps (red cells, then black), from x = 0, kRbNS per launch,
+// sixteen full sweeps (red cells, then black), from x = 0, kRbNS per launch,
 // alternating between work and x so that the last launch writes x.
 // FP64: work holds 2 ncell doubles.  FP32 storage (c32 = FP32 bu, M_inner,
 // M_0): work holds 6 ncell floats, two FP32 iterates and the FP32 b the
 // first launch writes.
 constexpr int kRbgsLaunches = kRbgsMiddle + 2;
 static_assert(kRbgsLaunches % 2 == 0, "the last launch must write x");
-static_assert(kRbgsLaunches * kRbNS == 12, "twelve sweeps per application");
+static_assert(kRbgsLaunches * kRbNS == 16, "sixteen sweeps per application");
 void launch_phmu_rbgs(hipStream_t st, const Grid& g, const Scal& s, const Coef& c, const PhmuCoef32* c32,
                       const double* b, double* x, void* work, const KState* ks, const KTimer* tm) {
   const dim3 grid = rb_grid(g);

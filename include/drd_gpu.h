@@ -56,9 +56,9 @@ extern "C" {
 #define DRD_CH_PREC_REFERENCE 0 /* scalar Jacobi TP/ph_mu_solve.f90:304-356 */
 #define DRD_CH_PREC_BLOCK 1     /* exact 2x2 per-cell block Jacobi (no
                                    reference counterpart)                   */
-#define DRD_CH_PREC_RBGS 2      /* twelve red-black Gauss-Seidel sweeps of the
-                                   same 2x2 blocks (no reference
-                                   counterpart)                             */
+#define DRD_CH_PREC_RBGS 2      /* sixteen red-black block SOR sweeps (weight
+                                   1.25) of the same 2x2 blocks (no
+                                   reference counterpart)                   */
 #define DRD_CH_PREC_RBGS32 3    /* the same sweeps in FP32: mobilities, b,
                                    iterates and arithmetic (reciprocals
                                    where the FP64 form divides); FP64 b in,
@@ -409,7 +409,7 @@ int drd_bench_kernel(drd_ctx* ctx, int32_t which, int32_t reps, double* avg_ms,
  *     BiCGSTAB launches it; a group is one launch, timed by its own
  *     dispatch timestamps (hipExtLaunchKernelGGL);
  *   DRD_K_PHMU_SWEEP32: the middle sweeps of DRD_PS_PHMU_RBGS32; a group is
- *     the 2 of one application, between two event markers on the stream.
+ *     the six of one application, between two event markers on the stream.
  * drd_kernel_timer_read waits for the stream and returns the number of timed
  * launches, their summed duration and the kernel's algorithmic bytes per
  * launch.                                                                 */

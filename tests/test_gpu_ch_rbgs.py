@@ -1,7 +1,7 @@
 """Red-black block Gauss-Seidel for the phi/mu system (DRD_PS_PHMU_RBGS) on the GPU.
 
-* The preconditioner against its numpy form on the oracle: x = red * B^-1 b,
-  then for each half-sweep x = x + mask * B^-1 (b - A x), with A the
+* The preconditioner against its numpy form on the oracle: x = w red B^-1 b,
+  then for each half-sweep x = x + w mask B^-1 (b - A x) (w = 1.25), with A the
   oracle's phi/mu operator and B^-1 the oracle's 2x2 block solve (both bit
   for bit the GPU's, so the whole preconditioner is too, the sign of zero
   aside).
@@ -27,7 +27,8 @@ from drd_application_amd import abi, geometry
 
 pytestmark = pytest.mark.gpu
 
-SWEEPS = 12
+SWEEPS = 16
+OMEGA = 1.25  # over-relaxation weight of the sweeps
 
 
 def rel_l2(a, b):
@@ -43,9 +44,9 @@ def rbgs_reference(o, p, b):
     black = 1.0 - red
     A = lambda x: o.matvec(abi.OP_PHMU, 1.0, x, 0.0, np.zeros(n))
     B = lambda r: o.psolve(abi.PS_PHMU_BLOCK, abi.OP_PHMU, r)
-    x = red * B(b)
+    x = OMEGA * (red * B(b))
     for m in ([black, red] * SWEEPS)[:2 * SWEEPS - 1]:
-        x = x + m * B(b - A(x))
+        x = x + OMEGA * (m * B(b - A(x)))
     return x
 
 
