@@ -513,7 +513,8 @@ int drd_kernel_timer_launches(drd_ctx* ctx, int32_t capacity, float* ms, int32_t
  *   - each coarse operator is the same MAC stencil with doubled spacings, on
  *     rho, eta, chi averaged over 2x2 cells;
  *   - the transfers are staggered bilinear prolongation P and R = P^T / 4;
- *   - damped Jacobi smoothing (omega = 0.7, exact diagonal): two sweeps
+ *   - Jacobi smoothing (exact diagonal; weight 0.7 on the finest level,
+ *     1.2 on the coarser ones): two sweeps
  *     before the coarse correction and two after;
  *   - the FP64 DCT/DST fast-Poisson solve on the coarsest level: the FFT
  *     form when its sizes are powers of two, dense line transforms

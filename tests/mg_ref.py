@@ -15,7 +15,8 @@ import scipy.sparse as sp
 from drd_application_amd import abi
 import prec_ref
 
-OMEGA = 0.7
+OMEGA = 0.7          # damped Jacobi weight on the finest level
+OMEGA_COARSE = 1.2   # and on every coarser one
 
 
 # ---- 1-D prolongation weights P[f, C] -------------------------------------------
@@ -129,12 +130,13 @@ def vcycle(lv, b):
         q, o, d = lv[l]
         if l == len(lv) - 1:
             return prec_ref.scipy_dct_prec(o, q, bl)
-        xa = OMEGA * bl / d
-        xb = xa + OMEGA * (bl - A(l, xa)) / d
+        w = OMEGA if l == 0 else OMEGA_COARSE
+        xa = w * bl / d
+        xb = xa + w * (bl - A(l, xa)) / d
         r = bl - A(l, xb)
         xc = down(l + 1, 0.25 * (P[l].T @ r))
         xb = xb + P[l] @ xc
-        xa = xb + OMEGA * (bl - A(l, xb)) / d
-        return xa + OMEGA * (bl - A(l, xa)) / d
+        xa = xb + w * (bl - A(l, xb)) / d
+        return xa + w * (bl - A(l, xa)) / d
 
     return down(0, b)
